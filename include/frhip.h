@@ -1,0 +1,158 @@
+/*
+ * frhip.h -- C ABI of the MI355X (gfx950) flux-reconstruction kernel library.
+ *
+ * Conventions (mirroring the ctypes LibWrapper contract of the reference,
+ * pyfr/ctypesutil.py:8-40 and pyfr/backends/hip/driver.py:74-153):
+ *   - every function returns int: 0 on success, nonzero on failure;
+ *     frhip_last_error() returns a thread-local message for the last failure;
+ *   - no C++ exceptions cross the ABI;
+ *   - device buffers are plain pointers owned by the caller; the library
+ *     never frees caller memory; operator plans own their constant data;
+ *   - streams, events and graphs are opaque void* handles;
+ *   - dtype selects the working precision: FRHIP_F64 or FRHIP_F32.
+ *
+ * Storage layout (the reference's HIP layout, pyfr/backends/base/types.py:40-100
+ * with soasz = csubsz = 64 = the gfx950 wavefront): a (nrow, nvar, neles)
+ * array is row-major with leading dimension ld = nvar*neles_pad and element
+ * e / variable v of row r at  r*ld + ((e/64)*nvar + v)*64 + e%64.
+ * A 2-D (nrow, ncol) array is row-major with leading dimension ld.
+ */
+#ifndef FRHIP_H
+#define FRHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { FRHIP_F64 = 0, FRHIP_F32 = 1 };
+enum { FRHIP_RS_RUSANOV = 0, FRHIP_RS_HLLC = 1 };
+
+/* ---- runtime: replaces the ctypes HIP driver wrapper
+ *      pyfr/backends/hip/driver.py:402-482 (hipSetDevice, hipMalloc, hipFree,
+ *      hipMemcpy*, hipMemset*, streams, events) ---- */
+const char *frhip_last_error(void);
+int frhip_init(int device);
+int frhip_device_info(char *name, size_t len, size_t *free_b, size_t *total_b);
+int frhip_malloc(void **ptr, size_t nbytes);
+int frhip_free(void *ptr);
+int frhip_host_malloc(void **ptr, size_t nbytes);
+int frhip_host_free(void *ptr);
+int frhip_memcpy_htod(void *dst, const void *src, size_t nbytes, void *stream);
+int frhip_memcpy_dtoh(void *dst, const void *src, size_t nbytes, void *stream);
+int frhip_memcpy_dtod(void *dst, const void *src, size_t nbytes, void *stream);
+int frhip_memset(void *ptr, int value, size_t nbytes, void *stream);
+int frhip_stream_create(void **stream);
+int frhip_stream_destroy(void *stream);
+int frhip_stream_sync(void *stream);
+int frhip_device_sync(void);
+int frhip_event_create(void **event);
+int frhip_event_destroy(void *event);
+int frhip_event_record(void *event, void *stream);
+int frhip_event_sync(void *event);
+int frhip_event_elapsed(float *ms, void *start, void *end);
+
+/* ---- graphs: replaces HIPGraph (pyfr/backends/hip/types.py:79-116) and the
+ *      graph API of pyfr/backends/hip/driver.py:303-399.  Kernels launched on
+ *      a capturing stream are recorded, then replayed by frhip_graph_launch. */
+int frhip_graph_begin(void *stream);
+int frhip_graph_end(void *stream, void **graph_exec);
+int frhip_graph_launch(void *graph_exec, void *stream);
+int frhip_graph_destroy(void *graph_exec);
+
+/* ---- operator application: replaces the `mul` providers
+ *      HIPGiMMiKKernels.mul (pyfr/backends/hip/gimmik.py:24-117) and
+ *      HIPRocBLASKernels.mul (pyfr/backends/hip/rocblas.py:93-192):
+ *      C(m, ncol) = alpha * A(m, k) @ B(k, ncol) + beta * C.
+ * frhip_mul_plan_create takes the dense row-major host operator A.  If its
+ * sparsity pattern matches a compiled structured kernel (hex tensor-product
+ * operators), that kernel is used with A's nonzeros uploaded; otherwise a
+ * dense kernel is used.  *kind receives 1 (structured) or 0 (dense). */
+int frhip_mul_plan_create(void **plan, int m, int k, const double *a_host,
+                          int dtype, int *kind);
+int frhip_mul_plan_destroy(void *plan);
+int frhip_mul(void *plan, int ncol, const void *b, long ldb, void *c,
+              long ldc, double alpha, double beta, void *stream);
+
+/* ---- pointwise kernels (element-major 2-D over (npts, neles)) ---- */
+
+/* Euler tflux: pyfr/solvers/euler/kernels/tflux.mako:8-30 with flux.mako:3-26.
+ * linear = 1: S from the 8 vertices (verts: (8, 3, neles), upts: (npts, 3)
+ * row-major, smats.mako:13-25); linear = 0: stored smats (3, npts, 3, neles). */
+int frhip_tflux_euler(int dtype, int linear, int npts, int neles,
+                      const void *u, long ldu, void *f, long ldf,
+                      const void *geo, long ldgeo, const void *upts,
+                      double gamma, void *stream);
+
+/* NS fused tflux: pyfr/solvers/navstokes/kernels/tflux.mako:13-50 with
+ * transform_grad.mako:3-10 and navstokes/kernels/flux.mako:48-105.  gradu is
+ * transformed in place to physical space; f = S (Fi + Fv).
+ * For linear = 0, rcpdjac is (npts, neles) with leading dimension ldr. */
+int frhip_tflux_ns(int dtype, int linear, int npts, int neles,
+                   const void *u, long ldu, void *f, long ldf, void *gradu,
+                   long ldg, const void *geo, long ldgeo, const void *upts,
+                   const void *rcpdjac, long ldr, const double *consts,
+                   void *stream);
+
+/* negdivconf: pyfr/solvers/baseadvec/kernels/negdivconf.mako:7-22 (no source
+ * terms): t[v] = -rcpdjac * t[v]. */
+int frhip_negdivconf(int dtype, int npts, int neles, void *t, long ldt,
+                     const void *rcpdjac, long ldr, void *stream);
+
+/* Interface kernels, 1-D over n flux-point pairs.  Views are (base,
+ * mapping) with variable stride 64; vector views add a row stride.
+ * Euler intcflux: pyfr/solvers/euler/kernels/intcflux.mako:6-22 with
+ * rsolvers/rusanov.mako:4-25 or rsolvers/hllc.mako:4-77. nl: (3, n), ld ldn. */
+int frhip_intcflux_euler(int dtype, int rsolver, int n, void *base,
+                         const int32_t *lmap, const int32_t *rmap,
+                         const void *nl, long ldn, double gamma,
+                         void *stream);
+
+/* NS intconu: pyfr/solvers/navstokes/kernels/intconu.mako:4-19. */
+int frhip_intconu(int dtype, int n, const void *ubase, const int32_t *ulin,
+                  const int32_t *urin, void *cbase, const int32_t *ulout,
+                  const int32_t *urout, double beta, void *stream);
+
+/* NS intcflux: pyfr/solvers/navstokes/kernels/intcflux.mako:10-54.
+ * gbase/glmap/grmap address the physical gradient at the flux points with row
+ * stride gstride between dimensions.  consts = {gamma, mu, Pr, beta, tau,
+ * sutherland(0/1), cpTref, cpTs}. */
+int frhip_intcflux_ns(int dtype, int rsolver, int n, void *ubase,
+                      const int32_t *lmap, const int32_t *rmap,
+                      const void *gbase, const int32_t *glmap,
+                      const int32_t *grmap, long gstride, const void *nl,
+                      long ldn, const double *consts, void *stream);
+
+/* ---- BLAS extensions: pyfr/backends/hip/blasext.py:8-136 ---- */
+
+/* axnpby: x0 = a0*x0 + sum_i a_i*x_i over (nrow, nvar*neles_pad) arrays
+ * with the a0 == 0 / a0 == 1 special cases of hip/kernels/axnpby.mako:4-44.
+ * xs[0] is x0; nv <= 8. */
+int frhip_axnpby(int dtype, int nrow, long ncol, long ld, int nv,
+                 void *const *xs, const double *a, void *stream);
+
+/* rkvdh2 stage update: pyfr/integrators/explicit/kernels/rkvdh2.mako:4-31
+ * (no error estimate): if last: r1 += dt*b*r2 else r1' = r1 + dt*a*r2,
+ * r2' = r1 + dt*b*r2. */
+int frhip_rkvdh2(int dtype, int nrow, long ncol, long ld, void *r1, void *r2,
+                 double dta, double dtb, int last, void *stream);
+
+/* Reductions over an (nrow, nvar, neles) array: per variable sum of w*x
+ * (w: (nrow, neles) weights, may be NULL) or max |x|; FP64 accumulation.
+ * out: device buffer of nvar doubles.  Replaces HIPBlasExtKernels.reduction
+ * (pyfr/backends/hip/blasext.py:76-136). op: 0 = sum, 1 = max-abs. */
+int frhip_reduce(int dtype, int op, int nrow, int nvar, int neles, long ld,
+                 const void *x, const void *w, long ldw, double *out,
+                 void *stream);
+
+/* Copy/convert between precisions (same layout); n elements. */
+int frhip_convert(int dst_dtype, void *dst, int src_dtype, const void *src,
+                  long n, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FRHIP_H */

@@ -1,0 +1,273 @@
+"""Oracle: polynomial bases and FR operator matrices (test infrastructure).
+
+Restates ``pyfr/polys.py``, ``pyfr/nputil.py:10-57`` and the hex/quad parts of
+``pyfr/shapes.py`` in NumPy.  Quadrature points come from
+``numpy.polynomial.legendre.leggauss`` instead of the reference's tabulated
+rule files; tests pin them against values read from those files.
+"""
+from functools import cached_property
+
+import numpy as np
+
+
+def block_diag(arrs):
+    """``pyfr/nputil.py:10-20``."""
+    shapes = [a.shape for a in arrs]
+    out = np.zeros(np.sum(shapes, axis=0), dtype=arrs[0].dtype)
+    r = c = 0
+    for a, (rr, cc) in zip(arrs, shapes):
+        out[r:r + rr, c:c + cc] = a
+        r, c = r + rr, c + cc
+    return out
+
+
+def clean(arr, tol=1e-10):
+    """Flush tiny entries and coalesce near-equal magnitudes to their median.
+
+    Follows ``pyfr/nputil.py:23-57``: |x| < tol -> 0; runs of sorted
+    magnitudes that are np.isclose(rtol=tol, atol=tol/10) to the run's first
+    element are replaced by the run's median; signs are restored.
+    """
+    arr = np.array(arr, dtype=float, copy=True)
+    arr[np.abs(arr) < tol] = 0
+    if arr.size > 1:
+        amfl = np.abs(arr.ravel()).copy()
+        amix = np.argsort(amfl)
+        i, ix = 0, amix[0]
+        j = 0
+        for j, jx in enumerate(amix[1:], start=1):
+            if not np.isclose(amfl[jx], amfl[ix], rtol=tol, atol=0.1*tol):
+                if j - i > 1:
+                    amfl[amix[i:j]] = np.median(amfl[amix[i:j]])
+                i, ix = j, jx
+        if i != j:
+            amfl[amix[i:]] = np.median(amfl[amix[i:]])
+        arr = np.copysign(amfl, arr.ravel()).reshape(arr.shape)
+    return arr
+
+
+def jacobi(n, a, b, z):
+    """Jacobi polynomials P_0..P_n^(a,b)(z); ``pyfr/polys.py:9-28``."""
+    j = [np.ones_like(z)]
+    if n >= 1:
+        j.append(((a + b + 2)*z + a - b) / 2)
+    if n >= 2:
+        apb, bbmaa = a + b, b*b - a*a
+        for q in range(2, n + 1):
+            qapbpq, apbp2q = q*(apb + q), apb + 2*q
+            apbp2qm1, apbp2qm2 = apbp2q - 1, apbp2q - 2
+            aq = apbp2q*apbp2qm1/(2*qapbpq)
+            bq = apbp2qm1*bbmaa/(2*qapbpq*apbp2qm2)
+            cq = apbp2q*(a + q - 1)*(b + q - 1)/(qapbpq*apbp2qm2)
+            j.append((aq*z - bq)*j[-1] - cq*j[-2])
+    return j
+
+
+def jacobi_diff(n, a, b, z):
+    """Derivatives of the Jacobi polynomials; ``pyfr/polys.py:31-38``."""
+    dj = [np.zeros_like(z)]
+    if n >= 1:
+        dj.extend(jp*(i + a + b + 2)/2
+                  for i, jp in enumerate(jacobi(n - 1, a + 1, b + 1, z)))
+    return dj
+
+
+class _Basis:
+    """Orthonormal + nodal basis; ``pyfr/polys.py:45-96``."""
+
+    def __init__(self, order, pts):
+        self.order = order
+        self.pts = np.asarray(pts, dtype=float)
+
+    def ortho_basis_at(self, pts, do_clean=True):
+        pts = np.atleast_2d(np.atleast_1d(np.asarray(pts, dtype=float)).T)
+        m = np.array(self._ortho(*pts))
+        return clean(m) if do_clean else m
+
+    def jac_ortho_basis_at(self, pts, do_clean=True):
+        pts = np.atleast_2d(np.atleast_1d(np.asarray(pts, dtype=float)).T)
+        m = np.array(self._jac_ortho(*pts)).swapaxes(0, 1)
+        return clean(m) if do_clean else m
+
+    @cached_property
+    def vdm(self):
+        return self.ortho_basis_at(self.pts)
+
+    @cached_property
+    def invvdm(self):
+        return clean(np.linalg.inv(self.vdm))
+
+    def nodal_basis_at(self, epts):
+        return clean((self.invvdm @ self.ortho_basis_at(epts, False)).T)
+
+    def jac_nodal_basis_at(self, epts):
+        return clean(self.invvdm @ self.jac_ortho_basis_at(epts, False))
+
+
+class LineBasis(_Basis):
+    """``pyfr/polys.py:99-114``."""
+
+    def _ortho(self, p):
+        return [(i + 0.5)**0.5*jp for i, jp in enumerate(jacobi(self.order, 0, 0, p))]
+
+    def _jac_ortho(self, p):
+        return [((i + 0.5)**0.5*jp,)
+                for i, jp in enumerate(jacobi_diff(self.order, 0, 0, p))]
+
+
+class QuadBasis(_Basis):
+    """``pyfr/polys.py:170-195``."""
+
+    def _ortho(self, p, q):
+        sk = [(k + 0.5)**0.5 for k in range(self.order + 1)]
+        pa = [c*jp for c, jp in zip(sk, jacobi(self.order, 0, 0, p))]
+        pb = [c*jp for c, jp in zip(sk, jacobi(self.order, 0, 0, q))]
+        return [pi*pj for pi in pa for pj in pb]
+
+
+class HexBasis(_Basis):
+    """``pyfr/polys.py:397-433``."""
+
+    def _ortho(self, p, q, r):
+        sk = [(k + 0.5)**0.5 for k in range(self.order + 1)]
+        pa = [c*jp for c, jp in zip(sk, jacobi(self.order, 0, 0, p))]
+        pb = [c*jp for c, jp in zip(sk, jacobi(self.order, 0, 0, q))]
+        pc = [c*jp for c, jp in zip(sk, jacobi(self.order, 0, 0, r))]
+        return [pi*pj*pk for pi in pa for pj in pb for pk in pc]
+
+    def _jac_ortho(self, p, q, r):
+        sk = [(k + 0.5)**0.5 for k in range(self.order + 1)]
+        pa = [c*jp for c, jp in zip(sk, jacobi(self.order, 0, 0, p))]
+        pb = [c*jp for c, jp in zip(sk, jacobi(self.order, 0, 0, q))]
+        pc = [c*jp for c, jp in zip(sk, jacobi(self.order, 0, 0, r))]
+        dpa = [c*jp for c, jp in zip(sk, jacobi_diff(self.order, 0, 0, p))]
+        dpb = [c*jp for c, jp in zip(sk, jacobi_diff(self.order, 0, 0, q))]
+        dpc = [c*jp for c, jp in zip(sk, jacobi_diff(self.order, 0, 0, r))]
+        return [[dpi*pj*pk, pi*dpj*pk, pi*pj*dpk]
+                for pi, dpi in zip(pa, dpa)
+                for pj, dpj in zip(pb, dpb)
+                for pk, dpk in zip(pc, dpc)]
+
+
+def gauss_legendre_line(n):
+    """n-point Gauss-Legendre rule on [-1, 1] (points ascending)."""
+    x, w = np.polynomial.legendre.leggauss(n)
+    return x, w
+
+
+def gauss_legendre_quad(n):
+    """Tensor GL rule with the first coordinate fastest (the ordering of
+    ``pyfr/quadrules/quad/gauss-legendre-n*.txt``)."""
+    x, w = gauss_legendre_line(n)
+    pts = np.array([(x[i], x[j]) for j in range(n) for i in range(n)])
+    wts = np.array([w[i]*w[j] for j in range(n) for i in range(n)])
+    return pts, wts
+
+
+def gauss_legendre_hex(n):
+    """Tensor GL rule, x fastest then y then z
+    (``pyfr/quadrules/hex/gauss-legendre-n*.txt``)."""
+    x, w = gauss_legendre_line(n)
+    pts = np.array([(x[i], x[j], x[k])
+                    for k in range(n) for j in range(n) for i in range(n)])
+    wts = np.array([w[i]*w[j]*w[k]
+                    for k in range(n) for j in range(n) for i in range(n)])
+    return pts, wts
+
+
+# Hex faces: (reference-to-face projection, outward normal); shapes.py:378-385
+HEX_FACES = [
+    (lambda s, t: (s, t, -1*np.ones_like(s)), (0, 0, -1)),
+    (lambda s, t: (s, -1*np.ones_like(s), t), (0, -1, 0)),
+    (lambda s, t: (np.ones_like(s), s, t), (1, 0, 0)),
+    (lambda s, t: (s, np.ones_like(s), t), (0, 1, 0)),
+    (lambda s, t: (-1*np.ones_like(s), s, t), (-1, 0, 0)),
+    (lambda s, t: (s, t, np.ones_like(s)), (0, 0, 1)),
+]
+
+
+def _proj(fn, pts2):
+    """``pyfr/shapes.py:12-14``."""
+    return np.vstack(np.broadcast_arrays(*fn(*pts2.T))).T
+
+
+class HexShapeOracle:
+    """FR operators for a hex with GL solution and flux points.
+
+    Follows ``pyfr/shapes.py:22-135`` (m0..m6), ``:231-259`` (gbasis) and the
+    ``HexShape`` face table ``:375-385``.
+    """
+    ndims = 3
+    nfaces = 6
+
+    def __init__(self, order):
+        self.order = order
+        n = order + 1
+        self.upts, self.upts_wts = gauss_legendre_hex(n)
+        self.nupts = n**3
+        self.ubasis = HexBasis(order, self.upts)
+        fq, self.fq_wts = gauss_legendre_quad(n)
+        self.face_pts2 = fq
+        self.nfacefpts = [n*n]*6
+        self.fpts = np.vstack([_proj(p, fq) for p, _ in HEX_FACES])
+        self.nfpts = 6*n*n
+        self.norm_fpts = np.vstack([[nrm]*(n*n) for _, nrm in HEX_FACES]).astype(float)
+        nf = np.cumsum([0] + self.nfacefpts)
+        self.facefpts = [list(range(nf[i], nf[i + 1])) for i in range(6)]
+
+    @cached_property
+    def m0(self):
+        return self.ubasis.nodal_basis_at(self.fpts)
+
+    @cached_property
+    def m1(self):
+        m = np.rollaxis(self.ubasis.jac_nodal_basis_at(self.upts), 2)
+        return m.reshape(self.nupts, -1)
+
+    @cached_property
+    def m2(self):
+        m = self.norm_fpts[..., None]*self.m0[:, None, :]
+        return m.reshape(self.nfpts, -1)
+
+    @cached_property
+    def gbasis_coeffs(self):
+        # shapes.py:231-255: quad faces use a (p+1)^2 GL rule
+        n = self.order + 1
+        qp, qw = gauss_legendre_quad(n)
+        fbasis = QuadBasis(self.order, self.face_pts2)
+        coeffs = []
+        for proj, _ in HEX_FACES:
+            L = fbasis.nodal_basis_at(qp)
+            M = self.ubasis.ortho_basis_at(_proj(proj, qp))
+            coeffs.append(np.einsum('i,ik,ji->kj', qw, L, M))
+        return np.vstack(coeffs)
+
+    def gbasis_at(self, pts):
+        return clean((self.gbasis_coeffs @ self.ubasis.ortho_basis_at(pts)).T)
+
+    @cached_property
+    def m3(self):
+        return self.gbasis_at(self.upts)
+
+    @cached_property
+    def m4(self):
+        m = self.m1.reshape(self.nupts, -1, self.nupts).swapaxes(0, 1)
+        return m.reshape(-1, self.nupts)
+
+    @cached_property
+    def m6(self):
+        m = self.norm_fpts.T[:, None, :]*self.m3
+        return m.reshape(-1, self.nfpts)
+
+    def opmat(self, expr):
+        """Evaluate an operator expression such as 'M1 - M3*M2' and clean it
+        (``pyfr/shapes.py:79-87``)."""
+        env = {f'M{i}': getattr(self, f'm{i}') for i in (0, 1, 2, 3, 4, 6)}
+        return clean(eval(expr.replace('*', '@'), {'__builtins__': {}}, env))
+
+    # Shape points of a linear hex (std_ele(1)); shapes.py:278-286
+    @staticmethod
+    def linspts():
+        pts1d = np.linspace(-1, 1, 2)
+        import itertools as it
+        return np.array([p[::-1] for p in it.product(pts1d, repeat=3)])

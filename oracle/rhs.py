@@ -1,0 +1,263 @@
+"""Oracle: FR right-hand side and explicit RK steppers on periodic hex boxes
+(test infrastructure).
+
+The RHS follows the reference's own kernel decomposition:
+
+Euler (``pyfr/solvers/baseadvec/system.py:57-136``):
+  disu -> intcflux -> tdisf(tflux) -> tdivtpcorf -> tdivtconf -> negdivconf
+Navier-Stokes with gradient fusion (``pyfr/solvers/baseadvecdiff/system.py:53-231``):
+  disu -> intconu -> tgradpcoru -> tgradcoru -> tdisf_fused -> gradcoru_fpts
+  -> intcflux -> tdivtpcorf -> tdivtconf -> negdivconf
+"""
+import numpy as np
+
+from oracle.basis import HexShapeOracle
+from oracle.geom import (linear_smats_djac, pnorm_at, rcpdjac_at, smats_at,
+                         trilinear_ploc)
+from oracle.mesh import box_hex_verts, pair_faces_bruteforce
+from oracle.physics import RSOLVERS, inviscid_flux, pri_to_con, viscous_flux_add
+
+
+TGV_ICS = {
+    'rho': lambda x, y, z: np.ones_like(x),
+    'u': lambda x, y, z: np.sin(x)*np.cos(y)*np.cos(z),
+    'v': lambda x, y, z: -np.cos(x)*np.sin(y)*np.cos(z),
+    'w': lambda x, y, z: np.zeros_like(x),
+    'p': lambda x, y, z: 100.0/1.4 + (np.cos(2.0*x) + np.cos(2.0*y)) *
+    (np.cos(2.0*z) + 2.0)/16.0,
+}
+
+
+class OracleHexSystem:
+    """Periodic box of n^3 linear hexes at FR order ``order``.
+
+    constants: dict with 'gamma' (Euler) and additionally 'mu', 'Pr',
+    'ldg-beta', 'ldg-tau' (NS).
+    """
+
+    def __init__(self, n, order, system='euler', rsolver='rusanov',
+                 constants=None, length=2*np.pi, verts=None):
+        self.n, self.order, self.system = n, order, system
+        self.rsolver = RSOLVERS[rsolver]
+        self.c = dict(constants or {'gamma': 1.4})
+        self.length = length
+
+        self.shape = sh = HexShapeOracle(order)
+        self.nupts, self.nfpts = sh.nupts, sh.nfpts
+        self.verts = box_hex_verts(n, length) if verts is None else verts
+        self.neles = self.verts.shape[-1]
+
+        # Operators (pyfr/solvers/baseadvec/elements.py:73-100,
+        # pyfr/solvers/baseadvecdiff/elements.py:47-55)
+        self.M0 = sh.opmat('M0')
+        self.M132 = sh.opmat('M1 - M3*M2')
+        self.M3 = sh.opmat('M3')
+        self.M460 = sh.opmat('M4 - M6*M0')
+        self.M6 = sh.opmat('M6')
+
+        # Geometry; p < 2 => every element uses the stored ("curved")
+        # metric (pyfr/solvers/base/elements.py:224-227)
+        self.linear = order >= 2
+        if self.linear:
+            s, d = linear_smats_djac(self.verts, sh.upts)
+            self.smats_u = s                        # (Nu, 3, 3, E)
+            self.rcpdjac_l = 1/d                    # (Nu, E)
+        else:
+            sm = smats_at(self.verts, order, sh.upts)   # (3, Nu, 3, E)
+            self.smats_u = sm.transpose(1, 0, 2, 3)
+            self.rcpdjac_l = rcpdjac_at(self.verts, order, sh.upts)
+        self.rcpdjac = rcpdjac_at(self.verts, order, sh.upts)
+        self.pnorm = pnorm_at(self.verts, order, sh.fpts, sh.norm_fpts)
+
+        # Physical flux-point locations and the interface pairing
+        self.plocf = trilinear_ploc(self.verts, sh.fpts)
+        pairs = pair_faces_bruteforce(self.plocf, 6, sh.nfacefpts, length)
+        self.lf = np.concatenate([p[2] for p in pairs])
+        self.le = np.concatenate([np.full(len(p[2]), p[0]) for p in pairs])
+        self.rf = np.concatenate([p[5] for p in pairs])
+        self.re = np.concatenate([np.full(len(p[5]), p[3]) for p in pairs])
+        self.nl = self.pnorm[self.lf, self.le].T    # (3, npairs)
+
+    # -- initial conditions (pyfr/solvers/base/elements.py:86-129)
+    def ics(self, ics=TGV_ICS, vscale=1.0):
+        x, y, z = trilinear_ploc(self.verts, self.shape.upts).transpose(1, 0, 2)
+        pri = [ics['rho'](x, y, z), vscale*ics['u'](x, y, z),
+               vscale*ics['v'](x, y, z), vscale*ics['w'](x, y, z),
+               ics['p'](x, y, z)]
+        con = pri_to_con(pri, self.c['gamma'])
+        return np.stack(con, axis=1)               # (Nu, 5, E)
+
+    # -- helpers
+    def _mul(self, A, B):
+        return (A @ B.reshape(A.shape[1], -1)).reshape(A.shape[0], *B.shape[1:])
+
+    def _tflux_inv(self, u):
+        f, _, _ = inviscid_flux(u.transpose(1, 0, 2), self.c['gamma'])
+        return f                                    # (3, 5, Nu, E)
+
+    def _transform(self, f):
+        # F[i] = sum_k s[i][k] f[k]; tflux.mako:24-28
+        s = self.smats_u                            # (Nu, 3, 3, E)
+        F = np.zeros((3, self.nupts, 5, self.neles))
+        for i in range(3):
+            acc = 0
+            for k in range(3):
+                acc = acc + s[:, i, k][:, None, :]*f[k].transpose(1, 0, 2)
+            F[i] = acc
+        return F
+
+    def _intcflux_euler(self, uf):
+        ul = uf[self.lf, :, self.le].T
+        ur = uf[self.rf, :, self.re].T
+        nl = self.nl
+        mag = np.sqrt(nl[0]*nl[0] + nl[1]*nl[1] + nl[2]*nl[2])
+        nn = (1/mag)*nl
+        fn = self.rsolver(ul, ur, nn, self.c['gamma'])
+        uf[self.lf, :, self.le] = (mag*fn).T
+        uf[self.rf, :, self.re] = (-mag*fn).T
+
+    # -- Euler RHS
+    def rhs_euler(self, u):
+        uf = self._mul(self.M0, u)                          # disu
+        self._intcflux_euler(uf)                            # intcflux
+        F = self._transform(self._tflux_inv(u))             # tdisf
+        div = self._mul(self.M132, F.reshape(3*self.nupts, 5, -1))
+        div = self._mul(self.M3, uf) + div                  # tdivtconf
+        return -self.rcpdjac[:, None, :]*div                # negdivconf
+
+    # -- NS RHS
+    def rhs_ns(self, u):
+        c = self.c
+        beta, tau = c['ldg-beta'], c['ldg-tau']
+        Nu, E = self.nupts, self.neles
+
+        uf = self._mul(self.M0, u)                          # disu
+        ul = uf[self.lf, :, self.le].T
+        ur = uf[self.rf, :, self.re].T
+        comm = np.zeros_like(uf)                            # intconu
+        cu = ur*(0.5 + beta) + ul*(0.5 - beta)
+        comm[self.lf, :, self.le] = cu.T
+        comm[self.rf, :, self.re] = cu.T
+
+        grad = self._mul(self.M460, u)                      # tgradpcoru
+        grad = self._mul(self.M6, comm) + grad              # tgradcoru
+        grad = grad.reshape(3, Nu, 5, E)
+
+        # tdisf_fused: transform_grad.mako:3-10 then F = Fi + Fv
+        s = self.smats_u
+        gp = np.zeros_like(grad)
+        for i in range(3):
+            acc = 0
+            for k in range(3):
+                acc = acc + s[:, k, i][:, None, :]*grad[k]
+            gp[i] = self.rcpdjac_l[:, None, :]*acc
+        f = self._tflux_inv(u)                              # (3, 5, Nu, E)
+        viscous_flux_add(u.transpose(1, 0, 2), gp.transpose(0, 2, 1, 3), f, c)
+        F = self._transform(f)
+
+        gf = np.stack([self._mul(self.M0, gp[d]) for d in range(3)])
+
+        # intcflux (navstokes/kernels/intcflux.mako:10-54)
+        gl = gf[:, self.lf, :, self.le].transpose(1, 2, 0)  # (3, 5, npairs)
+        gr = gf[:, self.rf, :, self.re].transpose(1, 2, 0)
+        nl = self.nl
+        mag = np.sqrt(nl[0]*nl[0] + nl[1]*nl[1] + nl[2]*nl[2])
+        nn = (1/mag)*nl
+        fi = self.rsolver(ul, ur, nn, c['gamma'])
+        fvl = viscous_flux_add(ul, gl, np.zeros((3, 5, ul.shape[1])), c)
+        fvr = viscous_flux_add(ur, gr, np.zeros((3, 5, ur.shape[1])), c)
+        out = np.empty_like(fi)
+        for i in range(5):
+            nfl = nn[0]*fvl[0, i] + nn[1]*fvl[1, i] + nn[2]*fvl[2, i]
+            nfr = nn[0]*fvr[0, i] + nn[1]*fvr[1, i] + nn[2]*fvr[2, i]
+            if beta == -0.5:
+                fv = nfr
+            elif beta == 0.5:
+                fv = nfl
+            else:
+                fv = (0.5 + beta)*nfl + (0.5 - beta)*nfr
+            if tau != 0.0:
+                fv = fv + tau*(ul[i] - ur[i])
+            out[i] = fi[i] + fv
+        uf[self.lf, :, self.le] = (mag*out).T
+        uf[self.rf, :, self.re] = (-mag*out).T
+
+        div = self._mul(self.M132, F.reshape(3*Nu, 5, -1))  # tdivtpcorf
+        div = self._mul(self.M3, uf) + div                  # tdivtconf
+        return -self.rcpdjac[:, None, :]*div                # negdivconf
+
+    def rhs(self, u):
+        return self.rhs_euler(u) if self.system == 'euler' else self.rhs_ns(u)
+
+    @property
+    def ndofs(self):
+        return self.nupts*5*self.neles
+
+
+# -- time integration (pyfr/integrators/...)
+
+def clamp_dt(dt_want, t, tcurr, lookahead=10, dtmin=1e-12):
+    """``pyfr/integrators/base.py:289-296``."""
+    remaining = t - tcurr
+    nsteps = -(-remaining // dt_want)
+    if nsteps > lookahead:
+        return dt_want
+    return max(remaining/nsteps, dtmin)
+
+
+def rk4_step(rhs, u, dt):
+    """One RK4 step with the reference's register choreography and axnpby
+    special cases (``steppers.py:67-108``, ``hip/kernels/axnpby.mako:21-43``)."""
+    r0 = u
+    r1 = rhs(r0)
+    r2 = (1.0*r0 + (dt/2.0)*r1)
+    r2 = rhs(r2)
+    r1 = ((dt/6.0)*r1 + 1.0*r0 + (dt/3.0)*r2)
+    r2 = ((dt/2.0)*r2 + 1.0*r0)
+    r2 = rhs(r2)
+    r1 = r1 + ((dt/3.0)*r2)
+    r2 = (dt*r2 + 1.0*r0)
+    r2 = rhs(r2)
+    r1 = r1 + ((dt/6.0)*r2)
+    return r1
+
+
+RK45_A = [970286171893 / 4311952581923, 6584761158862 / 12103376702013,
+          2251764453980 / 15575788980749, 26877169314380 / 34165994151039]
+RK45_B = [1153189308089 / 22510343858157, 1772645290293 / 4653164025191,
+          -1672844663538 / 4480602732383, 2114624349019 / 3568978502595,
+          5198255086312 / 14908931495163]
+
+
+def rk45_step(rhs, u, dt):
+    """Low-storage 2-register RK45 (``steppers.py:111-190``, ``:218-243``;
+    ``kernels/rkvdh2.mako:4-31``)."""
+    a, b = RK45_A, RK45_B
+    r1, r2 = u.copy(), None
+    for i in range(5):
+        r2 = rhs(r1 if i == 0 else r2)
+        if i < 4:
+            n1 = r1 + (dt*a[i])*r2
+            n2 = r1 + (dt*b[i])*r2
+            r1, r2 = n1, n2
+        else:
+            r1 = r1 + (dt*b[i])*r2
+        r1, r2 = r2, r1
+    return r2
+
+
+def advance(rhs, u, dt, tend, scheme='rk4', tstart=0.0):
+    """Fixed-dt ``none`` controller with Kahan-summed time
+    (``explicit/controllers.py:48-60``, ``integrators/base.py:282-296``).
+    Returns (u, nsteps)."""
+    step = rk4_step if scheme == 'rk4' else rk45_step
+    tcurr, comp, nsteps = tstart, 0.0, 0
+    while tcurr < tend:
+        h = clamp_dt(dt, tend, tcurr)
+        u = step(rhs, u, h)
+        y = h - comp
+        t = tcurr + y
+        comp = (t - tcurr) - y
+        tcurr = t
+        nsteps += 1
+    return u, nsteps

@@ -1,0 +1,790 @@
+"""MI355X backend with the reference's backend / kernel-provider interface.
+
+Mirrors ``pyfr/backends/base/backend.py:32-225`` and the types of
+``pyfr/backends/base/types.py`` (extents, matrices, slices, views, graphs,
+kernels) so that solver-side code written against PyFR's backend API drives
+it unchanged, while all device work goes through ``libfrhip.so``.
+
+Kernel names and arguments are the reference's:
+
+* ``mul(a, b, out, alpha=1.0, beta=0.0)``  (``hip/gimmik.py:24``)
+* pointwise ``tflux``, ``intcflux``, ``intconu``, ``negdivconf``, ``rkvdh2``
+  as ``kernel(name, tplargs=..., dims=[...], **args)``
+  (``backends/base/provider.py:188-200``)
+* ``axnpby(*arr, in_scale=(), in_scale_idxs=(), out_scale=())`` + ``bind``,
+  ``copy(dst, src)``, ``zero(m)`` (``hip/blasext.py:8-74``)
+
+Device layout is the reference HIP layout (AoSoA with 64-wide SoA blocks,
+``types.py:40-100``); ``get()``/``set()`` round-trip ``ioshape`` arrays.
+"""
+from collections import defaultdict
+from contextlib import contextmanager
+import ctypes as ct
+import itertools as it
+import math
+from weakref import WeakValueDictionary
+
+import numpy as np
+
+from pyfr_amd._lib import get_lib
+
+F64, F32 = 0, 1
+
+
+class NotSuitableError(Exception):
+    """Raised by a provider that declines a kernel (``provider.py:209``)."""
+
+
+def _dtcode(dtype):
+    dt = np.dtype(dtype)
+    if dt == np.float64:
+        return F64
+    if dt == np.float32:
+        return F32
+    raise ValueError(f'Unsupported dtype {dt}')
+
+
+# ---------------------------------------------------------------------------
+# Storage
+# ---------------------------------------------------------------------------
+
+class Extent:
+    """A named group of allocations committed as one device buffer
+    (``types.py:19-37``)."""
+
+    def __init__(self, name=None):
+        self.name = name
+        self.nbytes = 0
+        self.basedata = None
+        self._pending = []
+        self._storage_root = self
+
+    def reserve(self, obj, nbytes):
+        self._pending.append((obj, self.nbytes))
+        self.nbytes += nbytes
+
+    def commit(self, alloc_fn):
+        self.basedata = alloc_fn(self.nbytes)
+        for obj, off in self._pending:
+            obj._storage_root = self
+            obj.onalloc(self.basedata, off)
+        self._pending.clear()
+
+
+class MatrixBase:
+    """Matrix of ``ioshape`` in the AoSoA layout (``types.py:40-150``)."""
+    _base_tags = set()
+
+    def __init__(self, backend, dtype, ioshape, initval, extent, tags):
+        self.backend = backend
+        self.tags = self._base_tags | set(tags)
+        self.dtype = np.dtype(dtype).type
+        self.itemsize = np.dtype(dtype).itemsize
+
+        shape, ndim = list(ioshape), len(ioshape)
+        soasz, csubsz = backend.soasz, backend.csubsz
+        if ndim == 2:
+            nrow, ncol = shape
+            ldmod = csubsz if 'align' in self.tags else 1
+            leaddim = ncol - (ncol % -ldmod)
+            datashape = [1, nrow, leaddim]
+        else:
+            nvar, narr = shape[-2], shape[-1]
+            nparr = narr - narr % -csubsz
+            nrow = shape[0] if ndim == 3 else shape[0]*shape[1]
+            ncol = nvar*nparr
+            leaddim = ncol
+            datashape = [1, *shape[:-2], nparr // soasz, nvar, soasz]
+
+        self.nrow, self.ncol, self.leaddim = nrow, ncol, leaddim
+        self.datashape = datashape
+        self.ioshape = tuple(ioshape)
+        self.nbytes = nrow*leaddim*self.itemsize
+        self.traits = (1, nrow, ncol, leaddim, self.dtype)
+
+        if initval is not None:
+            initval = np.asarray(initval)
+            if initval.shape != self.ioshape:
+                raise ValueError('Invalid initial value')
+            self._initval = np.asanyarray(initval, dtype=self.dtype)
+        else:
+            self._initval = None
+
+        backend.malloc(self, extent)
+
+    # -- allocation hooks
+    def onalloc(self, basedata, offset):
+        self.basedata = int(basedata)
+        self.offset = int(offset)
+        self.data = self.basedata + self.offset
+        if self._initval is not None:
+            self._set(self._initval)
+        else:
+            self.backend.lib.memset(self.data, 0, self.nbytes,
+                                    self.backend.stream)
+        del self._initval
+
+    @property
+    def storage_root(self):
+        return self._storage_root
+
+    def same_storage(self, other):
+        return self.storage_root is other.storage_root
+
+    # -- AoSoA pack / unpack (host side)
+    def _pack(self, ary):
+        ary = np.asarray(ary, dtype=self.dtype)
+        if ary.ndim == 2:
+            out = np.zeros((self.nrow, self.leaddim), dtype=self.dtype)
+            out[:, :ary.shape[1]] = ary
+            return out
+        k = self.backend.soasz
+        n = ary.shape[-1]
+        ary = np.pad(ary, [(0, 0)]*(ary.ndim - 1) + [(0, -n % k)])
+        ary = ary.reshape(ary.shape[:-1] + (-1, k)).swapaxes(-2, -3)
+        return np.ascontiguousarray(ary.reshape(self.nrow, self.leaddim))
+
+    def _unpack(self, buf):
+        if len(self.ioshape) == 2:
+            return buf.reshape(self.nrow, self.leaddim)[:, :self.ioshape[1]]
+        ary = buf.reshape(self.datashape[1:]).swapaxes(-2, -3)
+        ary = ary.reshape(self.ioshape[:-1] + (-1,))
+        return ary[..., :self.ioshape[-1]]
+
+    def _get(self):
+        buf = np.empty(self.nrow*self.leaddim, dtype=self.dtype)
+        self.backend.lib.memcpy_dtoh(buf.ctypes.data, self.data, self.nbytes,
+                                     self.backend.stream)
+        return self._unpack(buf)
+
+    def _set(self, ary):
+        buf = self._pack(ary)
+        self.backend.lib.memcpy_htod(self.data, buf.ctypes.data, self.nbytes,
+                                     self.backend.stream)
+
+    def get(self):
+        if hasattr(self, '_initval'):
+            return (self._initval if self._initval is not None
+                    else np.zeros(self.ioshape, dtype=self.dtype))
+        self.backend.wait()
+        return np.array(self._get())
+
+    def slice(self, ra=None, rb=None, ca=None, cb=None):
+        ra, rb = ra or 0, rb or self.nrow
+        ca, cb = ca or 0, cb or self.ncol
+        return self.backend.matrix_slice(self, ra, rb, ca, cb)
+
+
+class Matrix(MatrixBase):
+    def set(self, ary):
+        if ary.shape != self.ioshape:
+            raise ValueError('Invalid matrix shape')
+        if hasattr(self, '_initval'):
+            self._initval = np.asanyarray(ary, dtype=self.dtype)
+        else:
+            self.backend.wait()
+            self._set(ary)
+
+
+class ConstMatrix(MatrixBase):
+    _base_tags = {'const'}
+
+    def __init__(self, backend, dtype, initval, tags):
+        super().__init__(backend, dtype, np.asarray(initval).shape, initval,
+                         None, tags)
+        self._host = np.array(initval, dtype=np.float64)
+
+    def get(self):
+        return self._host.astype(self.dtype)
+
+
+class MatrixSlice:
+    """Row/column window onto a matrix (``types.py:169-209``)."""
+
+    def __init__(self, backend, mat, ra, rb, ca, cb):
+        self.backend = backend
+        self.parent = mat
+        if not (0 <= ra < rb <= mat.nrow and 0 <= ca < cb <= mat.ncol):
+            raise ValueError('Invalid slice region')
+        self.ra, self.rb, self.ca, self.cb = ra, rb, ca, cb
+        self.nrow, self.ncol = rb - ra, cb - ca
+        self.leaddim = mat.leaddim
+        self.dtype, self.itemsize = mat.dtype, mat.itemsize
+        self.tags = mat.tags | {'slice'}
+        self.basedata = mat.basedata
+        self.offset = mat.offset + (ra*mat.leaddim + ca)*mat.itemsize
+        self.data = self.basedata + self.offset
+        self.traits = (1, self.nrow, self.ncol, self.leaddim, self.dtype)
+        self._storage_root = mat.storage_root
+        # An (nrow, nvar, neles) style slice keeps its parent's ioshape tail
+        if len(mat.ioshape) >= 3:
+            nv = mat.ioshape[-2]
+            self.ioshape = (self.nrow, nv, (self.ncol // nv))
+        else:
+            self.ioshape = (self.nrow, self.ncol)
+
+    @property
+    def storage_root(self):
+        return self._storage_root
+
+    def same_storage(self, other):
+        return self.storage_root is other.storage_root
+
+
+class View:
+    """Gather descriptor for interface kernels (``types.py:260-320``).
+
+    ``mapping`` holds, per point, the element offset (from ``basedata``) of
+    variable 0 in the AoSoA layout; variable v is at +64*v and, for vector
+    views, dimension d at +d*rstride."""
+
+    def __init__(self, backend, matmap, rmap, cmap, rstridemap, vshape, tags):
+        self.n = len(matmap)
+        self.nvrow = vshape[-2] if len(vshape) == 2 else 1
+        self.nvcol = vshape[-1] if len(vshape) >= 1 else 1
+
+        mats = [backend.mats[i] for i in np.unique(matmap)]
+        self._mats = mats
+        self.basedata = mats[0].basedata
+        self.refdtype = mats[0].dtype
+        if any(not m.same_storage(mats[0]) for m in mats):
+            raise TypeError('All viewed matrices must belong to the same '
+                            'storage object')
+        if any(m.dtype != self.refdtype for m in mats):
+            raise TypeError('Mixed data types are not supported')
+
+        k = backend.soasz
+        offset = np.empty(self.n, dtype=np.int64)
+        leaddim = np.empty(self.n, dtype=np.int64)
+        for m in mats:
+            ix = np.where(matmap == m.mid)
+            offset[ix] = m.offset // m.itemsize
+            leaddim[ix] = m.leaddim
+        cmap = np.asarray(cmap, dtype=np.int64)
+        rowdisp = np.asarray(rmap, dtype=np.int64)*leaddim
+        coldisp = (cmap // k)*k*self.nvcol + cmap % k
+        mapping = offset + rowdisp + coldisp
+        if mapping.size and mapping.max() >= 2**31:
+            raise ValueError('View offsets exceed the int32 index range')
+        self.mapping_np = mapping.astype(np.int32)
+        self.mapping = backend.const_matrix(self.mapping_np[None, :],
+                                            dtype=np.int32, tags=tags)
+        if self.nvrow > 1:
+            rstrides = (np.asarray(rstridemap)*leaddim)
+            self.rstrides_val = int(np.atleast_1d(rstrides).flat[0])
+            if np.any(rstrides != self.rstrides_val):
+                raise ValueError('Non-uniform row strides are unsupported')
+        else:
+            self.rstrides_val = 0
+        self.rstrides = None
+
+
+# ---------------------------------------------------------------------------
+# Kernels and graphs
+# ---------------------------------------------------------------------------
+
+class Kernel:
+    """A launchable unit (``provider.py:10-66``): ``run(stream)`` launches,
+    ``bind(**kw)`` updates runtime arguments."""
+    dt = float('nan')
+
+    def __init__(self, fn, rtargs=None, mats=(), views=(), name=''):
+        self._fn = fn
+        self.rtargs = dict(rtargs or {})
+        self.mats, self.views, self.name = mats, views, name
+        self.retval = None
+        self.version = 0
+
+    @property
+    def rtnames(self):
+        return set(self.rtargs)
+
+    def bind(self, *args, **kwargs):
+        if args:
+            raise TypeError('positional bind not supported for this kernel')
+        for k, v in kwargs.items():
+            if k not in self.rtargs:
+                raise KeyError(k)
+            if self.rtargs[k] != v:
+                self.rtargs[k] = v
+                self.version += 1
+
+    def run(self, stream):
+        self._fn(stream, **self.rtargs)
+
+
+class NullKernel(Kernel):
+    def __init__(self):
+        super().__init__(lambda stream: None, name='null')
+
+
+class MetaKernel(Kernel):
+    def __init__(self, kernels, splits=None):
+        self.kernels = list(kernels)
+        self.splits = splits
+        super().__init__(self._run_all, name='meta')
+
+    def _run_all(self, stream):
+        for k in self.kernels:
+            k.run(stream)
+
+    @property
+    def version(self):
+        return sum(getattr(k, 'version', 0) for k in self.kernels)
+
+    @version.setter
+    def version(self, v):
+        pass
+
+
+class Graph:
+    """Kernel DAG replayed as a hipGraph (``types.py:343-533``,
+    ``hip/types.py:79-116``).
+
+    Kernels are recorded in insertion order, which respects the declared
+    dependencies on a single stream.  On first run (and whenever a member
+    kernel's bound arguments change) the sequence is captured into a HIP
+    graph; later runs launch that graph."""
+
+    def __init__(self, backend):
+        self.backend = backend
+        self.klist = []
+        self.knodes = {}
+        self.committed = False
+        self._exec = None
+        self._versions = None
+
+    def add(self, kern, deps=[], pdeps=[]):
+        if self.committed:
+            raise RuntimeError('Can not add nodes to a committed graph')
+        for d in deps:
+            if d not in self.knodes:
+                raise ValueError('Dependency not in graph')
+        self.knodes[kern] = len(self.klist)
+        self.klist.append(kern)
+
+    def add_all(self, kerns, deps=[]):
+        for k in kerns:
+            self.add(k, deps)
+
+    def add_mpi_req(self, req, deps=[]):
+        raise NotImplementedError('MPI is not supported on this backend')
+
+    def add_mpi_reqs(self, reqs):
+        if reqs:
+            raise NotImplementedError('MPI is not supported on this backend')
+
+    def commit(self):
+        self.committed = True
+
+    def _cur_versions(self):
+        return tuple(k.version for k in self.klist)
+
+    def run(self, stream):
+        be = self.backend
+        if not be.use_graphs:
+            for k in self.klist:
+                k.run(stream)
+            return
+        vers = self._cur_versions()
+        if self._exec is None or vers != self._versions:
+            if self._exec is not None:
+                be.lib.graph_destroy(self._exec)
+                self._exec = None
+            be.lib.graph_begin(stream)
+            try:
+                for k in self.klist:
+                    k.run(stream)
+            finally:
+                ge = ct.c_void_p()
+                be.lib.graph_end(stream, ct.byref(ge))
+            self._exec, self._versions = ge.value, vers
+        be.lib.graph_launch(self._exec, stream)
+
+    def __del__(self):
+        try:
+            if self._exec is not None:
+                self.backend.lib.graph_destroy(self._exec)
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# Providers
+# ---------------------------------------------------------------------------
+
+def _ptr(m):
+    return None if m is None else m.data
+
+
+class MulProvider:
+    """``mul``: structured (sparsity-pattern matched) or dense operator
+    kernels; replaces the GiMMiK/rocBLAS providers."""
+
+    def __init__(self, backend):
+        self.backend = backend
+        self._plans = {}
+
+    def _plan(self, a, dtype):
+        key = (id(a), dtype)
+        if key not in self._plans:
+            host = np.ascontiguousarray(a.get(), dtype=np.float64)
+            plan, kind = ct.c_void_p(), ct.c_int()
+            self.backend.lib.mul_plan_create(
+                ct.byref(plan), host.shape[0], host.shape[1],
+                host.ctypes.data, _dtcode(dtype), ct.byref(kind))
+            self._plans[key] = (plan.value, kind.value, a)
+        return self._plans[key][:2]
+
+    def mul(self, a, b, out, alpha=1.0, beta=0.0):
+        if 'const' not in a.tags:
+            raise NotSuitableError('mul requires a constant operator')
+        if a.ncol != b.nrow or a.nrow != out.nrow:
+            raise ValueError('Incompatible matrices for out = a*b')
+        if b.ncol != out.ncol or b.dtype != out.dtype:
+            raise ValueError('Incompatible matrices for out = a*b')
+        lib = self.backend.lib
+        plan, kind = self._plan(a, b.dtype)
+        ncol, ldb, ldc = b.ncol, b.leaddim, out.leaddim
+        bp, cp = b.data, out.data
+        k = Kernel(lambda stream, alpha=alpha, beta=beta: lib.mul(
+            plan, ncol, bp, ldb, cp, ldc, alpha, beta, stream),
+            rtargs={'alpha': alpha, 'beta': beta}, mats=(a, b, out),
+            name='mul')
+        k.kind = 'structured' if kind == 1 else 'dense'
+        return k
+
+    def __del__(self):
+        try:
+            for plan, _, _ in self._plans.values():
+                self.backend.lib.mul_plan_destroy(plan)
+        except Exception:
+            pass
+
+
+class BlasExtProvider:
+    """``axnpby``, ``copy``, ``zero`` (``hip/blasext.py:8-74``)."""
+
+    def __init__(self, backend):
+        self.backend = backend
+
+    def axnpby(self, *arr, in_scale=(), in_scale_idxs=(), out_scale=()):
+        if in_scale or out_scale:
+            raise NotSuitableError('scaled axnpby is not supported')
+        if any(arr[0].traits != x.traits for x in arr[1:]):
+            raise ValueError('Incompatible matrix types')
+        lib, be = self.backend.lib, self.backend
+        nv = len(arr)
+        nrow, ncol, ld = arr[0].nrow, arr[0].ncol, arr[0].leaddim
+        dt = _dtcode(arr[0].dtype)
+        ptrs = (ct.c_void_p*nv)(*[x.data for x in arr])
+        coefs = (ct.c_double*nv)()
+
+        class AxnpbyKernel(Kernel):
+            def bind(self, *consts):
+                if len(consts) != nv:
+                    raise ValueError('axnpby expects one constant per array')
+                if tuple(coefs) != consts:
+                    coefs[:] = consts
+                    self.version += 1
+
+            def run(self, stream):
+                lib.axnpby(dt, nrow, ncol, ld, nv, ptrs, coefs, stream)
+
+        return AxnpbyKernel(None, mats=arr, name='axnpby')
+
+    def copy(self, dst, src):
+        if dst.traits != src.traits:
+            raise ValueError('Incompatible matrix types')
+        lib, n = self.backend.lib, dst.nrow*dst.leaddim*dst.itemsize
+        d, s = dst.data, src.data
+        return Kernel(lambda stream: lib.memcpy_dtod(d, s, n, stream),
+                      mats=(dst, src), name='copy')
+
+    def zero(self, m):
+        lib, n, d = self.backend.lib, m.nrow*m.leaddim*m.itemsize, m.data
+        return Kernel(lambda stream: lib.memset(d, 0, n, stream), mats=(m,),
+                      name='zero')
+
+
+class PointwiseProvider:
+    """Hand-written pointwise kernels addressed by the reference's kernel
+    names.  ``register(mod)`` accepts the reference's template module paths
+    and checks that a native kernel exists for them."""
+
+    _known = {
+        'pyfr.solvers.euler.kernels.tflux': 'tflux',
+        'pyfr.solvers.navstokes.kernels.tflux': 'tflux',
+        'pyfr.solvers.euler.kernels.intcflux': 'intcflux',
+        'pyfr.solvers.navstokes.kernels.intcflux': 'intcflux',
+        'pyfr.solvers.navstokes.kernels.intconu': 'intconu',
+        'pyfr.solvers.baseadvec.kernels.negdivconf': 'negdivconf',
+        'pyfr.integrators.explicit.kernels.rkvdh2': 'rkvdh2',
+    }
+
+    def __init__(self, backend):
+        self.backend = backend
+        self.registered = set()
+
+    def register(self, mod):
+        if mod not in self._known:
+            raise NotSuitableError(f'No native kernel for {mod}')
+        self.registered.add(self._known[mod])
+
+    # -- tflux (Euler and NS) ------------------------------------------------
+    def tflux(self, tplargs, dims, u, f, smats=None, verts=None, upts=None,
+              gradu=None, rcpdjac=None, artvisc_vtx=None, **kw):
+        lib, be = self.backend.lib, self.backend
+        npts, neles = dims
+        ktype = tplargs['ktype']
+        linear = 'linear' in ktype
+        dt = _dtcode(u.dtype)
+        geo = verts if linear else smats
+        uptsp = _ptr(upts) if linear else None
+        c = tplargs['c']
+        if 'fused' in ktype:
+            consts = (ct.c_double*8)(
+                c['gamma'], c.get('mu', 0.0), c.get('Pr', 1.0),
+                c.get('ldg-beta', 0.0), c.get('ldg-tau', 0.0),
+                1.0 if tplargs.get('visc_corr') == 'sutherland' else 0.0,
+                c.get('cpTref', 1.0), c.get('cpTs', 0.0))
+            rp = None if linear else _ptr(rcpdjac)
+            ldr = 0 if linear else rcpdjac.leaddim
+            args = (dt, int(linear), npts, neles, u.data, u.leaddim, f.data,
+                    f.leaddim, gradu.data, gradu.leaddim, geo.data,
+                    geo.leaddim, uptsp, rp, ldr, consts)
+            return Kernel(lambda stream: lib.tflux_ns(*args, stream),
+                          mats=(u, f, gradu, geo), name='tflux_ns')
+        args = (dt, int(linear), npts, neles, u.data, u.leaddim, f.data,
+                f.leaddim, geo.data, geo.leaddim, uptsp, c['gamma'])
+        return Kernel(lambda stream: lib.tflux_euler(*args, stream),
+                      mats=(u, f, geo), name='tflux')
+
+    # -- negdivconf ----------------------------------------------------------
+    def negdivconf(self, tplargs, dims, tdivtconf, rcpdjac, ploc=None, u=None,
+                   **kw):
+        if tplargs.get('src_macros'):
+            raise NotSuitableError('source terms are not supported')
+        lib = self.backend.lib
+        npts, neles = dims
+        args = (_dtcode(tdivtconf.dtype), npts, neles, tdivtconf.data,
+                tdivtconf.leaddim, rcpdjac.data, rcpdjac.leaddim)
+        return Kernel(lambda stream: lib.negdivconf(*args, stream),
+                      mats=(tdivtconf, rcpdjac), name='negdivconf')
+
+    # -- interfaces ----------------------------------------------------------
+    @staticmethod
+    def _rs(tplargs):
+        rs = tplargs['rsolver']
+        if rs == 'rusanov':
+            return 0
+        if rs == 'hllc':
+            return 1
+        raise NotSuitableError(f'Riemann solver {rs} not available')
+
+    def intcflux(self, tplargs, dims, ul, ur, nl, gradul=None, gradur=None,
+                 artvisc=None, **kw):
+        lib = self.backend.lib
+        (n,) = dims
+        if ul.basedata != ur.basedata:
+            raise ValueError('Interface views must share storage')
+        dt = _dtcode(ul.refdtype)
+        rs = self._rs(tplargs)
+        c = tplargs['c']
+        if gradul is None:
+            args = (dt, rs, n, ul.basedata, ul.mapping.data, ur.mapping.data,
+                    nl.data, nl.leaddim, c['gamma'])
+            return Kernel(lambda stream: lib.intcflux_euler(*args, stream),
+                          views=(ul, ur), mats=(nl,), name='intcflux')
+        if artvisc is not None:
+            raise NotSuitableError('artificial viscosity is not supported')
+        if gradul.rstrides_val != gradur.rstrides_val:
+            raise ValueError('Gradient views must share a row stride')
+        consts = (ct.c_double*8)(
+            c['gamma'], c['mu'], c['Pr'], c['ldg-beta'], c['ldg-tau'],
+            1.0 if tplargs.get('visc_corr') == 'sutherland' else 0.0,
+            c.get('cpTref', 1.0), c.get('cpTs', 0.0))
+        args = (dt, rs, n, ul.basedata, ul.mapping.data, ur.mapping.data,
+                gradul.basedata, gradul.mapping.data, gradur.mapping.data,
+                gradul.rstrides_val, nl.data, nl.leaddim, consts)
+        return Kernel(lambda stream: lib.intcflux_ns(*args, stream),
+                      views=(ul, ur, gradul, gradur), mats=(nl,),
+                      name='intcflux_ns')
+
+    def intconu(self, tplargs, dims, ulin, urin, ulout, urout, **kw):
+        lib = self.backend.lib
+        (n,) = dims
+        args = (_dtcode(ulin.refdtype), n, ulin.basedata, ulin.mapping.data,
+                urin.mapping.data, ulout.basedata, ulout.mapping.data,
+                urout.mapping.data, tplargs['c']['ldg-beta'])
+        return Kernel(lambda stream: lib.intconu(*args, stream),
+                      views=(ulin, urin, ulout, urout), name='intconu')
+
+    # -- rkvdh2 --------------------------------------------------------------
+    def rkvdh2(self, tplargs, dims, r1, r2, rold=None, rerr=None, dt=None,
+               **kw):
+        if tplargs.get('errest'):
+            raise NotSuitableError('error-estimating rkvdh2 not supported')
+        lib = self.backend.lib
+        stage, nst = tplargs['stage'], tplargs['nstages']
+        a, b = tplargs['a'], tplargs['b']
+        last = stage == nst - 1
+        aa = 0.0 if last else a[stage]
+        bb = b[stage]
+        d = _dtcode(r1.dtype)
+        nrow, ncol, ld = r1.nrow, r1.ncol, r1.leaddim
+
+        def fn(stream, dt=0.0):
+            lib.rkvdh2(d, nrow, ncol, ld, r1.data, r2.data, dt*aa, dt*bb,
+                       int(last), stream)
+        return Kernel(fn, rtargs={'dt': 0.0}, mats=(r1, r2), name='rkvdh2')
+
+
+# ---------------------------------------------------------------------------
+# Backend
+# ---------------------------------------------------------------------------
+
+class AMDBackend:
+    """The MI355X backend (``backends/base/backend.py:32-225``,
+    ``hip/base.py:9-140``)."""
+    name = 'hip-amd'
+    blocks = False
+    alignb = 256
+    soasz = 64
+    csubsz = 64
+
+    def __init__(self, cfg):
+        self.cfg = cfg
+        prec = cfg.get('backend', 'precision', 'double')
+        if prec not in {'single', 'double'}:
+            raise ValueError('Backend precision must be either single or '
+                             'double')
+        self.fpdtype = np.dtype(prec).type
+        self.fpdtype_eps = float(np.finfo(self.fpdtype).eps)
+        self.fpdtype_max = float(np.finfo(self.fpdtype).max)
+        if cfg.get('backend', 'memory-model', 'normal') != 'normal':
+            raise ValueError('Only the normal memory model is supported')
+        self.ixdtype = np.int32
+        self.use_graphs = cfg.getbool('backend-hip', 'graphs', True)
+
+        self.lib = get_lib()
+        self.lib.init(cfg.getint('backend-hip', 'device-id', 0))
+        s = ct.c_void_p()
+        self.lib.stream_create(ct.byref(s))
+        self.stream = s.value
+
+        self.mats = WeakValueDictionary()
+        self._mat_counter = it.count()
+        self._extents = {}
+        self._all_allocs = []
+        self._mem_current = 0
+        self._mem_peak = 0
+
+        self.pointwise = PointwiseProvider(self)
+        self._providers = [self.pointwise, BlasExtProvider(self),
+                           MulProvider(self)]
+
+    # -- memory
+    def _extent_alloc_bytes(self, nbytes):
+        return nbytes - (nbytes % -self.alignb)
+
+    def _malloc_impl(self, nbytes):
+        p = ct.c_void_p()
+        self.lib.malloc(ct.byref(p), nbytes)
+        self._all_allocs.append(p.value)
+        self._mem_current += nbytes
+        self._mem_peak = max(self._mem_peak, self._mem_current)
+        return p.value
+
+    def malloc(self, obj, extent):
+        nb = self._extent_alloc_bytes(obj.nbytes)
+        if extent is None:
+            ext = Extent()
+            ext.reserve(obj, nb)
+            ext.commit(self._malloc_impl)
+        else:
+            ext = self._extents.setdefault(extent, Extent(extent))
+            ext.reserve(obj, nb)
+
+    def commit(self):
+        for ext in self._extents.values():
+            ext.commit(self._malloc_impl)
+        self._extents.clear()
+
+    def memory_info(self):
+        return (self._mem_current, self._mem_peak)
+
+    def _record(self, m):
+        m.mid = next(self._mat_counter)
+        self.mats[m.mid] = m
+        return m
+
+    def const_matrix(self, initval, dtype=None, tags=set()):
+        dtype = dtype or self.fpdtype
+        initval = np.asarray(initval)
+        return self._record(ConstMatrix(self, dtype, initval, tags))
+
+    def matrix(self, ioshape, initval=None, extent=None, tags=set(),
+               dtype=None):
+        dtype = dtype or self.fpdtype
+        return self._record(Matrix(self, dtype, ioshape, initval, extent,
+                                   tags))
+
+    def matrix_slice(self, mat, ra, rb, ca, cb):
+        return self._record(MatrixSlice(self, mat, ra, rb, ca, cb))
+
+    def view(self, matmap, rmap, cmap, rstridemap=1, vshape=(), tags=set()):
+        return View(self, np.asarray(matmap), np.asarray(rmap),
+                    np.asarray(cmap), rstridemap, vshape, tags)
+
+    def xchg_matrix(self, *a, **k):
+        raise NotImplementedError('MPI exchange is not supported (one GPU)')
+
+    xchg_view = xchg_matrix_for_view = xchg_matrix
+
+    # -- kernels
+    def kernel(self, name, *args, **kwargs):
+        for prov in self._providers:
+            meth = getattr(prov, name, None)
+            if meth:
+                try:
+                    return meth(*args, **kwargs)
+                except NotSuitableError:
+                    continue
+        raise KeyError(f'Kernel {name!r} has no providers')
+
+    def ordered_meta_kernel(self, kerns):
+        return MetaKernel(kerns)
+
+    def unordered_meta_kernel(self, kerns, splits=None):
+        return MetaKernel(kerns, splits)
+
+    def graph(self):
+        return Graph(self)
+
+    def run_kernels(self, kernels, wait=False):
+        for k in kernels:
+            k.run(self.stream)
+        if wait:
+            self.wait()
+
+    def run_graph(self, graph, wait=False):
+        graph.run(self.stream)
+        if wait:
+            self.wait()
+
+    def wait(self):
+        self.lib.stream_sync(self.stream)
+
+    @contextmanager
+    def region(self, name):
+        yield
+
+    def __del__(self):
+        try:
+            self.lib.stream_sync(self.stream)
+            for p in self._all_allocs:
+                self.lib.free(p)
+            self.lib.stream_destroy(self.stream)
+        except Exception:
+            pass

@@ -1,0 +1,47 @@
+// Internal header for the structured operator kernels (see gen_ops.py).
+#pragma once
+
+#include "frhip_common.h"
+
+namespace frhip {
+
+// Streaming load of an operand that is read exactly once.
+template <typename T>
+__device__ __forceinline__ T ldg(const T *p)
+{
+    return __builtin_nontemporal_load(p);
+}
+
+// Output modes: 0: c = acc; 1: c = acc + c; 2: c = alpha*acc + beta*c;
+// 3: c = alpha*acc.
+template <typename T, int BM>
+__device__ __forceinline__ void opstore(T *c, T acc, T alpha, T beta)
+{
+    if constexpr (BM == 0)
+        __builtin_nontemporal_store(acc, c);
+    else if constexpr (BM == 1)
+        *c = acc + *c;
+    else if constexpr (BM == 2)
+        *c = alpha*acc + beta*(*c);
+    else
+        __builtin_nontemporal_store(alpha*acc, c);
+}
+
+using OpKD = void (*)(int, const double *, long, double *, long,
+                      const double *, double, double);
+using OpKF = void (*)(int, const float *, long, float *, long,
+                      const float *, float, float);
+
+struct OpKernelEntry {
+    int m, k, nnz;
+    unsigned long long hash;
+    const int32_t *lin;      // row-major linear indices of the nonzeros
+    const char *desc;
+    OpKD fd[4];
+    OpKF ff[4];
+};
+
+extern const OpKernelEntry g_opk_table[];
+extern const int g_opk_count;
+
+}  // namespace frhip

@@ -1,0 +1,871 @@
+// Pointwise, interface, RK and reduction kernels of the FR right-hand side.
+//
+// Element kernels run on a 2-D grid: x covers 64 consecutive elements (one
+// wavefront, so each (point, variable) access is one coalesced 512 B FP64
+// transaction in the AoSoA layout) and y covers solution points.  Interface
+// kernels are 1-D over flux-point pairs and gather through int32 view maps.
+#include "frhip_common.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace frhip {
+
+// ---------------------------------------------------------------------------
+// Point physics (device)
+// ---------------------------------------------------------------------------
+
+// Inviscid Euler flux; pyfr/solvers/euler/kernels/flux.mako:3-26.
+template <typename T>
+__device__ __forceinline__ void inviscid_flux(const T s[5], T f[3][5], T &p,
+                                              T v[3], T gm1)
+{
+    const T invrho = T(1)/s[0], E = s[4];
+    const T rhov[3] = {s[1], s[2], s[3]};
+    for (int i = 0; i < 3; ++i) v[i] = invrho*rhov[i];
+    p = gm1*(E - T(0.5)*invrho*((rhov[0])*(rhov[0]) + (rhov[1])*(rhov[1]) +
+                               (rhov[2])*(rhov[2])));
+    for (int i = 0; i < 3; ++i) {
+        f[i][0] = rhov[i];
+        f[i][4] = (E + p)*v[i];
+    }
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            f[i][j + 1] = (i == j) ? rhov[i]*v[j] + p : rhov[i]*v[j];
+}
+
+// Rusanov; pyfr/solvers/euler/kernels/rsolvers/rusanov.mako:4-25.
+template <typename T>
+__device__ __forceinline__ void rsolve_rusanov(const T ul[5], const T ur[5],
+                                               const T n[3], T nf[5], T gm1,
+                                               T qgamma)
+{
+    T fl[3][5], fr[3][5], vl[3], vr[3], pl, pr;
+    inviscid_flux(ul, fl, pl, vl, gm1);
+    inviscid_flux(ur, fr, pr, vr, gm1);
+    const T nv = ((n[0])*(vl[0] + vr[0]) + (n[1])*(vl[1] + vr[1]) +
+                  (n[2])*(vl[2] + vr[2]));
+    const T a = sqrt(qgamma*(pl + pr)/(ul[0] + ur[0])) + T(0.25)*fabs(nv);
+    for (int i = 0; i < 5; ++i)
+        nf[i] = T(0.5)*(n[0]*(fl[0][i] + fr[0][i]) + n[1]*(fl[1][i] + fr[1][i])
+                        + n[2]*(fl[2][i] + fr[2][i])) + a*(ul[i] - ur[i]);
+}
+
+// HLLC; pyfr/solvers/euler/kernels/rsolvers/hllc.mako:4-77.
+template <typename T>
+__device__ __forceinline__ void rsolve_hllc(const T ul[5], const T ur[5],
+                                            const T n[3], T nf[5], T gm1,
+                                            T gamma)
+{
+    T fl[3][5], fr[3][5], vl[3], vr[3], pl, pr;
+    inviscid_flux(ul, fl, pl, vl, gm1);
+    inviscid_flux(ur, fr, pr, vr, gm1);
+    const T nvl = n[0]*vl[0] + n[1]*vl[1] + n[2]*vl[2];
+    const T nvr = n[0]*vr[0] + n[1]*vr[1] + n[2]*vr[2];
+    const T al = sqrt(gamma*pl/ul[0]);
+    const T ar = sqrt(gamma*pr/ur[0]);
+    const T srl = sqrt(ul[0]), srr = sqrt(ur[0]);
+    const T nv = (srl*nvl + srr*nvr)/(srl + srr);
+    const T H = (srl*(pr + ur[4]) + srr*(pl + ul[4]))/(srl*ur[0] + srr*ul[0]);
+    const T inv_rar = T(1)/(srl + srr);
+    T va[3];
+    for (int i = 0; i < 3; ++i) va[i] = (vl[i]*srl + vr[i]*srr)*inv_rar;
+    const T qq = (va[0])*(va[0]) + (va[1])*(va[1]) + (va[2])*(va[2]);
+    const T a = sqrt(gm1*(H - T(0.5)*qq));
+    const T sl = fmin(nv - a, nvl - al);
+    const T sr = fmax(nv + a, nvr + ar);
+    const T sstar = (pr - pl + ul[0]*nvl*(sl - nvl) - ur[0]*nvr*(sr - nvr)) /
+                    (ul[0]*(sl - nvl) - ur[0]*(sr - nvr));
+    const T ul_com = (sl - nvl)/(sl - sstar);
+    const T ur_com = (sr - nvr)/(sr - sstar);
+    T usl[5], usr[5];
+    usl[0] = ul_com*ul[0];
+    usr[0] = ur_com*ur[0];
+    for (int i = 0; i < 3; ++i) {
+        usl[i + 1] = usl[0]*(vl[i] + (sstar - nvl)*n[i]);
+        usr[i + 1] = usr[0]*(vr[i] + (sstar - nvr)*n[i]);
+    }
+    usl[4] = ul_com*(ul[4] + (sstar - nvl)*(ul[0]*sstar + pl/(sl - nvl)));
+    usr[4] = ur_com*(ur[4] + (sstar - nvr)*(ur[0]*sstar + pr/(sr - nvr)));
+    for (int i = 0; i < 5; ++i) {
+        const T nf_fl = n[0]*fl[0][i] + n[1]*fl[1][i] + n[2]*fl[2][i];
+        const T nf_fr = n[0]*fr[0][i] + n[1]*fr[1][i] + n[2]*fr[2][i];
+        const T nf_fsl = nf_fl + sl*(usl[i] - ul[i]);
+        const T nf_fsr = nf_fr + sr*(usr[i] - ur[i]);
+        nf[i] = (T(0) <= sl) ? nf_fl
+              : (sl <= T(0) && T(0) <= sstar) ? nf_fsl
+              : (sstar <= T(0) && T(0) <= sr) ? nf_fsr : nf_fr;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void rsolve(int rs, const T ul[5], const T ur[5],
+                                       const T n[3], T nf[5], T gm1, T gamma,
+                                       T qgamma)
+{
+    if (rs == FRHIP_RS_HLLC) rsolve_hllc(ul, ur, n, nf, gm1, gamma);
+    else rsolve_rusanov(ul, ur, n, nf, gm1, qgamma);
+}
+
+// NS constants prepared on the host in double, then cast.
+template <typename T>
+struct NSConsts {
+    T gamma, gm1, qgamma, mu, gpr, beta, tau, rcpTref, musuth, cpTs;
+    int suth;
+};
+
+// Viscous flux (added to f); pyfr/solvers/navstokes/kernels/flux.mako:48-105.
+template <typename T>
+__device__ __forceinline__ void viscous_flux_add(const T u[5], const T g[3][5],
+                                                 T f[3][5],
+                                                 const NSConsts<T> &c)
+{
+    const T rho = u[0], rhou = u[1], rhov = u[2], rhow = u[3], E = u[4];
+    const T rcprho = T(1)/rho;
+    const T uu = rcprho*rhou, vv = rcprho*rhov, ww = rcprho*rhow;
+    const T rho_x = g[0][0], rho_y = g[1][0], rho_z = g[2][0];
+    const T u_x = g[0][1] - uu*rho_x, u_y = g[1][1] - uu*rho_y,
+            u_z = g[2][1] - uu*rho_z;
+    const T v_x = g[0][2] - vv*rho_x, v_y = g[1][2] - vv*rho_y,
+            v_z = g[2][2] - vv*rho_z;
+    const T w_x = g[0][3] - ww*rho_x, w_y = g[1][3] - ww*rho_y,
+            w_z = g[2][3] - ww*rho_z;
+    const T E_x = g[0][4], E_y = g[1][4], E_z = g[2][4];
+
+    T mu_c;
+    if (c.suth) {
+        const T cpT = c.gamma*(rcprho*E - T(0.5)*(uu*uu + vv*vv + ww*ww));
+        const T Trat = c.rcpTref*cpT;
+        mu_c = c.musuth*Trat*sqrt(Trat)/(cpT + c.cpTs);
+    }
+    else
+        mu_c = c.mu;
+
+    const T T_x = rcprho*(E_x - (rcprho*rho_x*E + uu*u_x + vv*v_x + ww*w_x));
+    const T T_y = rcprho*(E_y - (rcprho*rho_y*E + uu*u_y + vv*v_y + ww*w_y));
+    const T T_z = rcprho*(E_z - (rcprho*rho_z*E + uu*u_z + vv*v_z + ww*w_z));
+
+    const T third = T(1.0/3.0);
+    const T t_xx = -2*mu_c*rcprho*(u_x - third*(u_x + v_y + w_z));
+    const T t_yy = -2*mu_c*rcprho*(v_y - third*(u_x + v_y + w_z));
+    const T t_zz = -2*mu_c*rcprho*(w_z - third*(u_x + v_y + w_z));
+    const T t_xy = -mu_c*rcprho*(v_x + u_y);
+    const T t_xz = -mu_c*rcprho*(u_z + w_x);
+    const T t_yz = -mu_c*rcprho*(w_y + v_z);
+
+    f[0][1] += t_xx; f[1][1] += t_xy; f[2][1] += t_xz;
+    f[0][2] += t_xy; f[1][2] += t_yy; f[2][2] += t_yz;
+    f[0][3] += t_xz; f[1][3] += t_yz; f[2][3] += t_zz;
+    f[0][4] += uu*t_xx + vv*t_xy + ww*t_xz + -mu_c*c.gpr*T_x;
+    f[1][4] += uu*t_xy + vv*t_yy + ww*t_yz + -mu_c*c.gpr*T_y;
+    f[2][4] += uu*t_xz + vv*t_yz + ww*t_zz + -mu_c*c.gpr*T_z;
+}
+
+// S = |J| J^-T rows and |J| of a linear hex at reference point x from its 8
+// vertices; pyfr/solvers/baseadvec/kernels/smats.mako:13-25 with the
+// derivative of HexShape.interp_expr (pyfr/shapes.py:388-432).
+template <typename T>
+__device__ __forceinline__ void hex_smats(const T V[8][3], const T x[3],
+                                          T s[3][3], T &d)
+{
+    T j[3][3];
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) j[a][b] = T(0);
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+        const T s0 = (v & 1) ? T(1) : T(-1);
+        const T s1 = (v & 2) ? T(1) : T(-1);
+        const T s2 = (v & 4) ? T(1) : T(-1);
+        const T f0 = T(1) + s0*x[0], f1 = T(1) + s1*x[1], f2 = T(1) + s2*x[2];
+        const T d0 = s0*f1*f2*T(0.125), d1 = s1*f0*f2*T(0.125),
+                d2 = s2*f0*f1*T(0.125);
+        for (int b = 0; b < 3; ++b) {
+            j[0][b] += d0*V[v][b];
+            j[1][b] += d1*V[v][b];
+            j[2][b] += d2*V[v][b];
+        }
+    }
+    const int ja[3] = {1, 2, 0}, jb[3] = {2, 0, 1};
+    for (int i = 0; i < 3; ++i) {
+        const int a = ja[i], b = jb[i];
+        s[i][0] = j[a][1]*j[b][2] - j[a][2]*j[b][1];
+        s[i][1] = j[a][2]*j[b][0] - j[a][0]*j[b][2];
+        s[i][2] = j[a][0]*j[b][1] - j[a][1]*j[b][0];
+    }
+    d = j[0][0]*s[0][0] + j[0][1]*s[0][1] + j[0][2]*s[0][2];
+}
+
+// ---------------------------------------------------------------------------
+// Element kernels
+// ---------------------------------------------------------------------------
+
+static dim3 elem_grid(int npts, int neles)
+{
+    const unsigned gx = cdiv(neles, 64);
+    unsigned gy = std::max(1u, std::min(cdiv(npts, 4), 2048u/std::max(gx, 1u)));
+    return dim3(gx, gy);
+}
+
+// tflux (Euler); pyfr/solvers/euler/kernels/tflux.mako:8-30.
+template <typename T, bool LIN>
+__global__ __launch_bounds__(256) void k_tflux_euler(
+    int npts, int neles, const T *__restrict__ u, long ldu, T *__restrict__ f,
+    long ldf, const T *__restrict__ geo, long ldg, const T *__restrict__ upts,
+    T gm1)
+{
+    const int e = blockIdx.x*64 + threadIdx.x;
+    if (e >= neles) return;
+
+    T V[8][3];
+    if (LIN)
+        for (int v = 0; v < 8; ++v)
+            for (int d = 0; d < 3; ++d)
+                V[v][d] = geo[v*ldg + soa_ix(e, d, 3)];
+
+    for (int pt = blockIdx.y*4 + threadIdx.y; pt < npts; pt += 4*gridDim.y) {
+        T s[3][3], djac;
+        if (LIN) {
+            const T x[3] = {upts[3*pt], upts[3*pt + 1], upts[3*pt + 2]};
+            hex_smats(V, x, s, djac);
+        }
+        else
+            for (int i = 0; i < 3; ++i)
+                for (int k = 0; k < 3; ++k)
+                    s[i][k] = geo[((long)i*npts + pt)*ldg + soa_ix(e, k, 3)];
+
+        T uu[5];
+        for (int v = 0; v < 5; ++v) uu[v] = u[pt*ldu + soa_ix(e, v, 5)];
+        T ft[3][5], p, vel[3];
+        inviscid_flux(uu, ft, p, vel, gm1);
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 5; ++j)
+                f[((long)i*npts + pt)*ldf + soa_ix(e, j, 5)] =
+                    s[i][0]*ft[0][j] + s[i][1]*ft[1][j] + s[i][2]*ft[2][j];
+    }
+}
+
+// tflux (NS, gradient-fused); pyfr/solvers/navstokes/kernels/tflux.mako:13-50.
+template <typename T, bool LIN>
+__global__ __launch_bounds__(256) void k_tflux_ns(
+    int npts, int neles, const T *__restrict__ u, long ldu, T *__restrict__ f,
+    long ldf, T *__restrict__ gradu, long ldgr, const T *__restrict__ geo,
+    long ldg, const T *__restrict__ upts, const T *__restrict__ rcpdjac,
+    long ldr, NSConsts<T> c)
+{
+    const int e = blockIdx.x*64 + threadIdx.x;
+    if (e >= neles) return;
+
+    T V[8][3];
+    if (LIN)
+        for (int v = 0; v < 8; ++v)
+            for (int d = 0; d < 3; ++d)
+                V[v][d] = geo[v*ldg + soa_ix(e, d, 3)];
+
+    for (int pt = blockIdx.y*4 + threadIdx.y; pt < npts; pt += 4*gridDim.y) {
+        T s[3][3], rcpd;
+        if (LIN) {
+            const T x[3] = {upts[3*pt], upts[3*pt + 1], upts[3*pt + 2]};
+            T djac;
+            hex_smats(V, x, s, djac);
+            rcpd = T(1)/djac;
+        }
+        else {
+            for (int i = 0; i < 3; ++i)
+                for (int k = 0; k < 3; ++k)
+                    s[i][k] = geo[((long)i*npts + pt)*ldg + soa_ix(e, k, 3)];
+            rcpd = rcpdjac[pt*ldr + e];
+        }
+
+        T uu[5], g[3][5], tg[3][5];
+        for (int v = 0; v < 5; ++v) uu[v] = u[pt*ldu + soa_ix(e, v, 5)];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 5; ++j)
+                tg[i][j] = gradu[((long)i*npts + pt)*ldgr + soa_ix(e, j, 5)];
+
+        // transform_grad.mako:3-10
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 5; ++j) {
+                g[i][j] = rcpd*(s[0][i]*tg[0][j] + s[1][i]*tg[1][j] +
+                                s[2][i]*tg[2][j]);
+                gradu[((long)i*npts + pt)*ldgr + soa_ix(e, j, 5)] = g[i][j];
+            }
+
+        T ft[3][5], p, vel[3];
+        inviscid_flux(uu, ft, p, vel, c.gm1);
+        viscous_flux_add(uu, g, ft, c);
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 5; ++j)
+                f[((long)i*npts + pt)*ldf + soa_ix(e, j, 5)] =
+                    s[i][0]*ft[0][j] + s[i][1]*ft[1][j] + s[i][2]*ft[2][j];
+    }
+}
+
+// negdivconf; pyfr/solvers/baseadvec/kernels/negdivconf.mako:7-22.
+template <typename T>
+__global__ __launch_bounds__(256) void k_negdivconf(
+    int npts, int neles, T *__restrict__ t, long ldt,
+    const T *__restrict__ rcpdjac, long ldr)
+{
+    const int e = blockIdx.x*64 + threadIdx.x;
+    if (e >= neles) return;
+    for (int pt = blockIdx.y*4 + threadIdx.y; pt < npts; pt += 4*gridDim.y) {
+        const T r = rcpdjac[pt*ldr + e];
+        for (int v = 0; v < 5; ++v) {
+            T *tp = t + pt*ldt + soa_ix(e, v, 5);
+            *tp = -r*(*tp) + T(0);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Interface kernels
+// ---------------------------------------------------------------------------
+
+// intcflux (Euler); pyfr/solvers/euler/kernels/intcflux.mako:6-22.
+template <typename T>
+__global__ __launch_bounds__(256) void k_intcflux_euler(
+    int n, int rs, T *__restrict__ base, const int32_t *__restrict__ lmap,
+    const int32_t *__restrict__ rmap, const T *__restrict__ nl, long ldn,
+    T gamma, T gm1, T qgamma)
+{
+    const int i = blockIdx.x*blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long li = lmap[i], ri = rmap[i];
+    T ul[5], ur[5];
+    for (int v = 0; v < 5; ++v) {
+        ul[v] = base[li + v*SOA];
+        ur[v] = base[ri + v*SOA];
+    }
+    const T n0 = nl[i], n1 = nl[ldn + i], n2 = nl[2*ldn + i];
+    const T mag = sqrt((n0)*(n0) + (n1)*(n1) + (n2)*(n2));
+    const T nn[3] = {(T(1)/mag)*n0, (T(1)/mag)*n1, (T(1)/mag)*n2};
+    T fn[5];
+    rsolve(rs, ul, ur, nn, fn, gm1, gamma, qgamma);
+    for (int v = 0; v < 5; ++v) {
+        base[li + v*SOA] = mag*fn[v];
+        base[ri + v*SOA] = -mag*fn[v];
+    }
+}
+
+// intconu; pyfr/solvers/navstokes/kernels/intconu.mako:4-19.
+template <typename T>
+__global__ __launch_bounds__(256) void k_intconu(
+    int n, const T *__restrict__ ub, const int32_t *__restrict__ ulin,
+    const int32_t *__restrict__ urin, T *__restrict__ cb,
+    const int32_t *__restrict__ ulout, const int32_t *__restrict__ urout,
+    T beta, T wr, T wl)
+{
+    const int i = blockIdx.x*blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long li = ulin[i], ri = urin[i], lo = ulout[i], ro = urout[i];
+    for (int v = 0; v < 5; ++v) {
+        if (beta == T(-0.5))
+            cb[ro + v*SOA] = ub[li + v*SOA];
+        else if (beta == T(0.5))
+            cb[lo + v*SOA] = ub[ri + v*SOA];
+        else {
+            const T cu = ub[ri + v*SOA]*wr + ub[li + v*SOA]*wl;
+            cb[lo + v*SOA] = cu;
+            cb[ro + v*SOA] = cu;
+        }
+    }
+}
+
+// intcflux (NS); pyfr/solvers/navstokes/kernels/intcflux.mako:10-54.
+template <typename T>
+__global__ __launch_bounds__(256) void k_intcflux_ns(
+    int n, int rs, T *__restrict__ base, const int32_t *__restrict__ lmap,
+    const int32_t *__restrict__ rmap, const T *__restrict__ gb,
+    const int32_t *__restrict__ glmap, const int32_t *__restrict__ grmap,
+    long gstride, const T *__restrict__ nl, long ldn, NSConsts<T> c)
+{
+    const int i = blockIdx.x*blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long li = lmap[i], ri = rmap[i], gli = glmap[i], gri = grmap[i];
+    T ul[5], ur[5];
+    for (int v = 0; v < 5; ++v) {
+        ul[v] = base[li + v*SOA];
+        ur[v] = base[ri + v*SOA];
+    }
+    const T n0 = nl[i], n1 = nl[ldn + i], n2 = nl[2*ldn + i];
+    const T mag = sqrt((n0)*(n0) + (n1)*(n1) + (n2)*(n2));
+    const T nn[3] = {(T(1)/mag)*n0, (T(1)/mag)*n1, (T(1)/mag)*n2};
+
+    T fi[5];
+    rsolve(rs, ul, ur, nn, fi, c.gm1, c.gamma, c.qgamma);
+
+    T fvl[3][5] = {}, fvr[3][5] = {};
+    if (c.beta != T(-0.5)) {
+        T gl[3][5];
+        for (int d = 0; d < 3; ++d)
+            for (int v = 0; v < 5; ++v)
+                gl[d][v] = gb[gli + d*gstride + v*SOA];
+        viscous_flux_add(ul, gl, fvl, c);
+    }
+    if (c.beta != T(0.5)) {
+        T gr[3][5];
+        for (int d = 0; d < 3; ++d)
+            for (int v = 0; v < 5; ++v)
+                gr[d][v] = gb[gri + d*gstride + v*SOA];
+        viscous_flux_add(ur, gr, fvr, c);
+    }
+
+    for (int v = 0; v < 5; ++v) {
+        const T nfl = nn[0]*fvl[0][v] + nn[1]*fvl[1][v] + nn[2]*fvl[2][v];
+        const T nfr = nn[0]*fvr[0][v] + nn[1]*fvr[1][v] + nn[2]*fvr[2][v];
+        T fvcomm;
+        if (c.beta == T(-0.5)) fvcomm = nfr;
+        else if (c.beta == T(0.5)) fvcomm = nfl;
+        else fvcomm = (T(0.5) + c.beta)*nfl + (T(0.5) - c.beta)*nfr;
+        if (c.tau != T(0)) fvcomm += c.tau*(ul[v] - ur[v]);
+        base[li + v*SOA] = mag*(fi[v] + fvcomm);
+        base[ri + v*SOA] = -mag*(fi[v] + fvcomm);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// BLAS extensions
+// ---------------------------------------------------------------------------
+
+template <int NV>
+struct Ptrs {
+    void *p[NV];
+};
+
+template <typename T, int NV>
+struct Coefs {
+    T a[NV];
+};
+
+// axnpby; pyfr/backends/hip/kernels/axnpby.mako:4-44.
+template <typename T, int NV>
+__global__ __launch_bounds__(256) void k_axnpby(long ncol, long ld,
+                                                Ptrs<NV> xs, Coefs<T, NV> ca)
+{
+    const long j = (long)blockIdx.x*blockDim.x + threadIdx.x;
+    if (j >= ncol) return;
+    const long idx = (long)blockIdx.y*ld + j;
+    const T *a = ca.a;
+    T *x0 = static_cast<T *>(xs.p[0]);
+    T acc;
+    if (a[0] == T(0)) {
+        acc = (a[1])*(static_cast<const T *>(xs.p[1])[idx]);
+        for (int i = 2; i < NV; ++i)
+            acc = acc + (a[i])*(static_cast<const T *>(xs.p[i])[idx]);
+        x0[idx] = acc;
+    }
+    else if (a[0] == T(1)) {
+        acc = (a[1])*(static_cast<const T *>(xs.p[1])[idx]);
+        for (int i = 2; i < NV; ++i)
+            acc = acc + (a[i])*(static_cast<const T *>(xs.p[i])[idx]);
+        x0[idx] += acc;
+    }
+    else {
+        acc = (a[0])*(x0[idx]);
+        for (int i = 1; i < NV; ++i)
+            acc = acc + (a[i])*(static_cast<const T *>(xs.p[i])[idx]);
+        x0[idx] = acc;
+    }
+}
+
+// rkvdh2 without error estimate; rkvdh2.mako:4-31.
+template <typename T>
+__global__ __launch_bounds__(256) void k_rkvdh2(long ncol, long ld,
+                                                T *__restrict__ r1,
+                                                T *__restrict__ r2, T dta,
+                                                T dtb, int last)
+{
+    const long j = (long)blockIdx.x*blockDim.x + threadIdx.x;
+    if (j >= ncol) return;
+    const long idx = (long)blockIdx.y*ld + j;
+    const T t1 = r1[idx], t2 = r2[idx];
+    if (last)
+        r1[idx] = t1 + dtb*t2;
+    else {
+        r1[idx] = t1 + dta*t2;
+        r2[idx] = t1 + dtb*t2;
+    }
+}
+
+// Per-variable partial reductions, FP64 accumulation: block (256), grid
+// (nblk, nvar).  op 0: sum of w*x, op 1: max |x|.
+template <typename T>
+__global__ __launch_bounds__(256) void k_reduce_part(
+    int op, int nrow, int nvar, int neles, long ld, const T *__restrict__ x,
+    const T *__restrict__ w, long ldw, double *__restrict__ part)
+{
+    const int v = blockIdx.y;
+    double acc = 0.0;
+    const long total = (long)nrow*neles;
+    for (long q = (long)blockIdx.x*blockDim.x + threadIdx.x; q < total;
+         q += (long)gridDim.x*blockDim.x) {
+        const int r = (int)(q/neles);
+        const long e = q - (long)r*neles;
+        const double xv = (double)x[r*ld + soa_ix(e, v, nvar)];
+        if (op == 0) acc += w ? (double)w[r*ldw + e]*xv : xv;
+        else acc = fmax(acc, fabs(xv));
+    }
+    __shared__ double sm[256];
+    sm[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            sm[threadIdx.x] = op == 0 ? sm[threadIdx.x] + sm[threadIdx.x + s]
+                                      : fmax(sm[threadIdx.x], sm[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(long)v*gridDim.x + blockIdx.x] = sm[0];
+}
+
+__global__ __launch_bounds__(256) void k_reduce_final(int op, int nblk,
+                                                      const double *part,
+                                                      double *out)
+{
+    const int v = blockIdx.x;
+    double acc = 0.0;
+    for (int q = threadIdx.x; q < nblk; q += blockDim.x)
+        acc = op == 0 ? acc + part[(long)v*nblk + q]
+                      : fmax(acc, part[(long)v*nblk + q]);
+    __shared__ double sm[256];
+    sm[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            sm[threadIdx.x] = op == 0 ? sm[threadIdx.x] + sm[threadIdx.x + s]
+                                      : fmax(sm[threadIdx.x], sm[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[v] = sm[0];
+}
+
+template <typename D, typename S>
+__global__ __launch_bounds__(256) void k_convert(long n, D *__restrict__ d,
+                                                 const S *__restrict__ s)
+{
+    const long i = (long)blockIdx.x*blockDim.x + threadIdx.x;
+    if (i < n) d[i] = (D)s[i];
+}
+
+template <typename T>
+static NSConsts<T> ns_consts(const double *c)
+{
+    NSConsts<T> k;
+    const double gamma = c[0], mu = c[1], Pr = c[2];
+    k.gamma = (T)gamma;
+    k.gm1 = (T)(gamma - 1);
+    k.qgamma = (T)(0.25*gamma);
+    k.mu = (T)mu;
+    k.gpr = (T)(gamma/Pr);
+    k.beta = (T)c[3];
+    k.tau = (T)c[4];
+    k.suth = c[5] != 0.0;
+    k.rcpTref = (T)(k.suth ? 1/c[6] : 0.0);
+    k.musuth = (T)(k.suth ? mu*(c[6] + c[7]) : 0.0);
+    k.cpTs = (T)c[7];
+    return k;
+}
+
+template <typename T, int NV>
+static void launch_axnpby(int nrow, long ncol, long ld, void *const *xs,
+                          const double *a, hipStream_t s)
+{
+    Ptrs<NV> px;
+    Coefs<T, NV> pa;
+    for (int i = 0; i < NV; ++i) {
+        px.p[i] = xs[i];
+        pa.a[i] = (T)a[i];
+    }
+    dim3 grid(cdiv(ncol, 256), nrow), block(256);
+    hipLaunchKernelGGL((k_axnpby<T, NV>), grid, block, 0, s, ncol, ld, px, pa);
+}
+
+}  // namespace frhip
+
+using namespace frhip;
+
+#define FR_DTYPE_OK(dt) \
+    FR_CHECK((dt) == FRHIP_F64 || (dt) == FRHIP_F32, "invalid dtype")
+
+extern "C" {
+
+int frhip_tflux_euler(int dtype, int linear, int npts, int neles,
+                      const void *u, long ldu, void *f, long ldf,
+                      const void *geo, long ldgeo, const void *upts,
+                      double gamma, void *stream)
+{
+    FR_API_BEGIN
+    FR_DTYPE_OK(dtype);
+    FR_CHECK(npts > 0 && neles >= 0, "invalid extents");
+    if (neles == 0) return 0;
+    FR_CHECK(!linear || upts, "linear tflux needs upts");
+    dim3 grid = elem_grid(npts, neles), block(64, 4);
+    hipStream_t s = as_stream(stream);
+    if (dtype == FRHIP_F64) {
+        if (linear)
+            hipLaunchKernelGGL((k_tflux_euler<double, true>), grid, block, 0, s,
+                               npts, neles, (const double *)u, ldu, (double *)f,
+                               ldf, (const double *)geo, ldgeo,
+                               (const double *)upts, gamma - 1);
+        else
+            hipLaunchKernelGGL((k_tflux_euler<double, false>), grid, block, 0,
+                               s, npts, neles, (const double *)u, ldu,
+                               (double *)f, ldf, (const double *)geo, ldgeo,
+                               (const double *)upts, gamma - 1);
+    }
+    else {
+        if (linear)
+            hipLaunchKernelGGL((k_tflux_euler<float, true>), grid, block, 0, s,
+                               npts, neles, (const float *)u, ldu, (float *)f,
+                               ldf, (const float *)geo, ldgeo,
+                               (const float *)upts, (float)(gamma - 1));
+        else
+            hipLaunchKernelGGL((k_tflux_euler<float, false>), grid, block, 0,
+                               s, npts, neles, (const float *)u, ldu,
+                               (float *)f, ldf, (const float *)geo, ldgeo,
+                               (const float *)upts, (float)(gamma - 1));
+    }
+    FR_LAUNCHED();
+    return 0;
+    FR_API_END
+}
+
+int frhip_tflux_ns(int dtype, int linear, int npts, int neles,
+                   const void *u, long ldu, void *f, long ldf, void *gradu,
+                   long ldg, const void *geo, long ldgeo, const void *upts,
+                   const void *rcpdjac, long ldr, const double *consts,
+                   void *stream)
+{
+    FR_API_BEGIN
+    FR_DTYPE_OK(dtype);
+    FR_CHECK(npts > 0 && neles >= 0 && consts, "invalid arguments");
+    if (neles == 0) return 0;
+    FR_CHECK(linear ? upts != nullptr : rcpdjac != nullptr,
+             "missing geometry argument");
+    dim3 grid = elem_grid(npts, neles), block(64, 4);
+    hipStream_t s = as_stream(stream);
+    if (dtype == FRHIP_F64) {
+        auto c = ns_consts<double>(consts);
+        auto k = linear ? k_tflux_ns<double, true> : k_tflux_ns<double, false>;
+        hipLaunchKernelGGL(k, grid, block, 0, s, npts, neles,
+                           (const double *)u, ldu, (double *)f, ldf,
+                           (double *)gradu, ldg, (const double *)geo, ldgeo,
+                           (const double *)upts, (const double *)rcpdjac, ldr,
+                           c);
+    }
+    else {
+        auto c = ns_consts<float>(consts);
+        auto k = linear ? k_tflux_ns<float, true> : k_tflux_ns<float, false>;
+        hipLaunchKernelGGL(k, grid, block, 0, s, npts, neles,
+                           (const float *)u, ldu, (float *)f, ldf,
+                           (float *)gradu, ldg, (const float *)geo, ldgeo,
+                           (const float *)upts, (const float *)rcpdjac, ldr,
+                           c);
+    }
+    FR_LAUNCHED();
+    return 0;
+    FR_API_END
+}
+
+int frhip_negdivconf(int dtype, int npts, int neles, void *t, long ldt,
+                     const void *rcpdjac, long ldr, void *stream)
+{
+    FR_API_BEGIN
+    FR_DTYPE_OK(dtype);
+    if (neles == 0) return 0;
+    dim3 grid = elem_grid(npts, neles), block(64, 4);
+    if (dtype == FRHIP_F64)
+        hipLaunchKernelGGL(k_negdivconf<double>, grid, block, 0,
+                           as_stream(stream), npts, neles, (double *)t, ldt,
+                           (const double *)rcpdjac, ldr);
+    else
+        hipLaunchKernelGGL(k_negdivconf<float>, grid, block, 0,
+                           as_stream(stream), npts, neles, (float *)t, ldt,
+                           (const float *)rcpdjac, ldr);
+    FR_LAUNCHED();
+    return 0;
+    FR_API_END
+}
+
+int frhip_intcflux_euler(int dtype, int rsolver, int n, void *base,
+                         const int32_t *lmap, const int32_t *rmap,
+                         const void *nl, long ldn, double gamma, void *stream)
+{
+    FR_API_BEGIN
+    FR_DTYPE_OK(dtype);
+    FR_CHECK(rsolver == FRHIP_RS_RUSANOV || rsolver == FRHIP_RS_HLLC,
+             "invalid Riemann solver");
+    if (n == 0) return 0;
+    dim3 grid(cdiv(n, 256)), block(256);
+    if (dtype == FRHIP_F64)
+        hipLaunchKernelGGL(k_intcflux_euler<double>, grid, block, 0,
+                           as_stream(stream), n, rsolver, (double *)base, lmap,
+                           rmap, (const double *)nl, ldn, gamma, gamma - 1,
+                           0.25*gamma);
+    else
+        hipLaunchKernelGGL(k_intcflux_euler<float>, grid, block, 0,
+                           as_stream(stream), n, rsolver, (float *)base, lmap,
+                           rmap, (const float *)nl, ldn, (float)gamma,
+                           (float)(gamma - 1), (float)(0.25*gamma));
+    FR_LAUNCHED();
+    return 0;
+    FR_API_END
+}
+
+int frhip_intconu(int dtype, int n, const void *ubase, const int32_t *ulin,
+                  const int32_t *urin, void *cbase, const int32_t *ulout,
+                  const int32_t *urout, double beta, void *stream)
+{
+    FR_API_BEGIN
+    FR_DTYPE_OK(dtype);
+    if (n == 0) return 0;
+    dim3 grid(cdiv(n, 256)), block(256);
+    if (dtype == FRHIP_F64)
+        hipLaunchKernelGGL(k_intconu<double>, grid, block, 0, as_stream(stream),
+                           n, (const double *)ubase, ulin, urin,
+                           (double *)cbase, ulout, urout, beta, 0.5 + beta,
+                           0.5 - beta);
+    else
+        hipLaunchKernelGGL(k_intconu<float>, grid, block, 0, as_stream(stream),
+                           n, (const float *)ubase, ulin, urin, (float *)cbase,
+                           ulout, urout, (float)beta, (float)(0.5 + beta),
+                           (float)(0.5 - beta));
+    FR_LAUNCHED();
+    return 0;
+    FR_API_END
+}
+
+int frhip_intcflux_ns(int dtype, int rsolver, int n, void *ubase,
+                      const int32_t *lmap, const int32_t *rmap,
+                      const void *gbase, const int32_t *glmap,
+                      const int32_t *grmap, long gstride, const void *nl,
+                      long ldn, const double *consts, void *stream)
+{
+    FR_API_BEGIN
+    FR_DTYPE_OK(dtype);
+    FR_CHECK(consts, "null constants");
+    if (n == 0) return 0;
+    dim3 grid(cdiv(n, 256)), block(256);
+    if (dtype == FRHIP_F64)
+        hipLaunchKernelGGL(k_intcflux_ns<double>, grid, block, 0,
+                           as_stream(stream), n, rsolver, (double *)ubase,
+                           lmap, rmap, (const double *)gbase, glmap, grmap,
+                           gstride, (const double *)nl, ldn,
+                           ns_consts<double>(consts));
+    else
+        hipLaunchKernelGGL(k_intcflux_ns<float>, grid, block, 0,
+                           as_stream(stream), n, rsolver, (float *)ubase, lmap,
+                           rmap, (const float *)gbase, glmap, grmap, gstride,
+                           (const float *)nl, ldn, ns_consts<float>(consts));
+    FR_LAUNCHED();
+    return 0;
+    FR_API_END
+}
+
+int frhip_axnpby(int dtype, int nrow, long ncol, long ld, int nv,
+                 void *const *xs, const double *a, void *stream)
+{
+    FR_API_BEGIN
+    FR_DTYPE_OK(dtype);
+    FR_CHECK(nv >= 2 && nv <= 8, "axnpby supports 2..8 operands");
+    FR_CHECK(nrow > 0 && nrow < 65536, "invalid row count");
+    if (ncol == 0) return 0;
+    hipStream_t s = as_stream(stream);
+#define FR_AX(NVV)                                                        \
+    case NVV:                                                             \
+        if (dtype == FRHIP_F64)                                           \
+            launch_axnpby<double, NVV>(nrow, ncol, ld, xs, a, s);         \
+        else                                                              \
+            launch_axnpby<float, NVV>(nrow, ncol, ld, xs, a, s);          \
+        break;
+    switch (nv) {
+        FR_AX(2) FR_AX(3) FR_AX(4) FR_AX(5) FR_AX(6) FR_AX(7) FR_AX(8)
+    }
+#undef FR_AX
+    FR_LAUNCHED();
+    return 0;
+    FR_API_END
+}
+
+int frhip_rkvdh2(int dtype, int nrow, long ncol, long ld, void *r1, void *r2,
+                 double dta, double dtb, int last, void *stream)
+{
+    FR_API_BEGIN
+    FR_DTYPE_OK(dtype);
+    FR_CHECK(nrow > 0 && nrow < 65536, "invalid row count");
+    if (ncol == 0) return 0;
+    dim3 grid(cdiv(ncol, 256), nrow), block(256);
+    if (dtype == FRHIP_F64)
+        hipLaunchKernelGGL(k_rkvdh2<double>, grid, block, 0, as_stream(stream),
+                           ncol, ld, (double *)r1, (double *)r2, dta, dtb,
+                           last);
+    else
+        hipLaunchKernelGGL(k_rkvdh2<float>, grid, block, 0, as_stream(stream),
+                           ncol, ld, (float *)r1, (float *)r2, (float)dta,
+                           (float)dtb, last);
+    FR_LAUNCHED();
+    return 0;
+    FR_API_END
+}
+
+// Scratch for the two-pass reduction, allocated on first use (outside any
+// graph capture) and kept for the life of the process.
+static double *g_red_part = nullptr;
+static const int g_red_nblk = 512;
+static int g_red_nvar = 0;
+
+int frhip_reduce(int dtype, int op, int nrow, int nvar, int neles, long ld,
+                 const void *x, const void *w, long ldw, double *out,
+                 void *stream)
+{
+    FR_API_BEGIN
+    FR_DTYPE_OK(dtype);
+    FR_CHECK(op == 0 || op == 1, "invalid reduction op");
+    FR_CHECK(nvar > 0 && nvar <= 64 && nrow > 0, "invalid extents");
+    if (nvar > g_red_nvar) {
+        if (g_red_part) FR_HIP(hipFree(g_red_part));
+        g_red_part = nullptr;
+        FR_HIP(hipMalloc(&g_red_part, sizeof(double)*g_red_nblk*64));
+        g_red_nvar = 64;
+    }
+    hipStream_t s = as_stream(stream);
+    dim3 grid(g_red_nblk, nvar), block(256);
+    if (dtype == FRHIP_F64)
+        hipLaunchKernelGGL(k_reduce_part<double>, grid, block, 0, s, op, nrow,
+                           nvar, neles, ld, (const double *)x,
+                           (const double *)w, ldw, g_red_part);
+    else
+        hipLaunchKernelGGL(k_reduce_part<float>, grid, block, 0, s, op, nrow,
+                           nvar, neles, ld, (const float *)x,
+                           (const float *)w, ldw, g_red_part);
+    FR_LAUNCHED();
+    hipLaunchKernelGGL(k_reduce_final, dim3(nvar), dim3(256), 0, s, op,
+                       g_red_nblk, g_red_part, out);
+    FR_LAUNCHED();
+    return 0;
+    FR_API_END
+}
+
+int frhip_convert(int dst_dtype, void *dst, int src_dtype, const void *src,
+                  long n, void *stream)
+{
+    FR_API_BEGIN
+    FR_DTYPE_OK(dst_dtype);
+    FR_DTYPE_OK(src_dtype);
+    if (n == 0) return 0;
+    dim3 grid(cdiv(n, 256)), block(256);
+    hipStream_t s = as_stream(stream);
+    if (dst_dtype == FRHIP_F64 && src_dtype == FRHIP_F32)
+        hipLaunchKernelGGL((k_convert<double, float>), grid, block, 0, s, n,
+                           (double *)dst, (const float *)src);
+    else if (dst_dtype == FRHIP_F32 && src_dtype == FRHIP_F64)
+        hipLaunchKernelGGL((k_convert<float, double>), grid, block, 0, s, n,
+                           (float *)dst, (const double *)src);
+    else
+        FR_HIP(hipMemcpyAsync(dst, src, n*(dst_dtype == FRHIP_F64 ? 8 : 4),
+                              hipMemcpyDeviceToDevice, s));
+    FR_LAUNCHED();
+    return 0;
+    FR_API_END
+}
+
+}  // extern "C"

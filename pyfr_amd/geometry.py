@@ -1,0 +1,140 @@
+"""Metric terms of hex elements (setup, host side).
+
+Restates the reference's metric pipeline for linear (trilinear-mapped) hexes:
+
+* physical locations by the shape-point basis (``elements.py:331-350``);
+* S = |J| J^-1 at pseudo-grid points in Kopriva's invariant curl form and |J|
+  (``pyfr/solvers/base/elements.py:382-440``), interpolated to any point set
+  with the order-max(p, 1) basis on equispaced pseudo-grid points
+  (``elements.py:276-307``);
+* physical normals S^T n (``elements.py:364-380``).
+
+The bases are tensor-product Lagrange polynomials on equispaced points,
+evaluated directly rather than through an orthonormal Vandermonde inverse.
+"""
+import numpy as np
+
+
+def _lagrange_1d(x, pts):
+    """Values and derivatives of the Lagrange basis through x at pts."""
+    n = len(x)
+    L = np.ones((len(pts), n))
+    dL = np.zeros((len(pts), n))
+    for j in range(n):
+        for m in range(n):
+            if m == j:
+                continue
+            L[:, j] *= (pts - x[m])/(x[j] - x[m])
+        for m in range(n):
+            if m == j:
+                continue
+            t = np.full(len(pts), 1/(x[j] - x[m]))
+            for q in range(n):
+                if q != j and q != m:
+                    t = t*(pts - x[q])/(x[j] - x[q])
+            dL[:, j] += t
+    return L, dL
+
+
+class TensorLagrange:
+    """Nodal basis on the (n1)^3 tensor grid of 1-D points x1, x fastest."""
+
+    def __init__(self, x1):
+        self.x1 = np.asarray(x1, dtype=float)
+        self.n1 = len(x1)
+        n = self.n1
+        self.pts = np.array([(x1[i], x1[j], x1[k]) for k in range(n)
+                             for j in range(n) for i in range(n)])
+
+    def nodal_basis_at(self, pts):
+        """(npts, nnodes)"""
+        pts = np.atleast_2d(pts)
+        Lx, _ = _lagrange_1d(self.x1, pts[:, 0])
+        Ly, _ = _lagrange_1d(self.x1, pts[:, 1])
+        Lz, _ = _lagrange_1d(self.x1, pts[:, 2])
+        return np.einsum('pk,pj,pi->pkji', Lz, Ly, Lx).reshape(len(pts), -1)
+
+    def jac_nodal_basis_at(self, pts):
+        """(3, npts, nnodes): derivative along each reference axis."""
+        pts = np.atleast_2d(pts)
+        Lx, dLx = _lagrange_1d(self.x1, pts[:, 0])
+        Ly, dLy = _lagrange_1d(self.x1, pts[:, 1])
+        Lz, dLz = _lagrange_1d(self.x1, pts[:, 2])
+        np_ = len(pts)
+        d0 = np.einsum('pk,pj,pi->pkji', Lz, Ly, dLx).reshape(np_, -1)
+        d1 = np.einsum('pk,pj,pi->pkji', Lz, dLy, Lx).reshape(np_, -1)
+        d2 = np.einsum('pk,pj,pi->pkji', dLz, Ly, Lx).reshape(np_, -1)
+        return np.stack([d0, d1, d2])
+
+
+def shape_basis():
+    """Trilinear shape-point basis of a linear hex: nodes are the vertices
+    in ``HexShape`` order (x fastest)."""
+    return TensorLagrange(np.array([-1.0, 1.0]))
+
+
+class HexGeometry:
+    """Metric terms for hex elements with vertex array eles (8, neles, 3)."""
+
+    def __init__(self, eles, order):
+        self.eles = np.asarray(eles, dtype=float)
+        self.nspts, self.neles, self.ndims = self.eles.shape
+        if self.nspts != 8:
+            raise ValueError('Only linear (8-node) hexes are supported')
+        self.order = order
+        self.sbasis = shape_basis()
+        mord = max(order, 1)
+        self.mbasis = TensorLagrange(np.linspace(-1, 1, mord + 1))
+        self._smats_djacs = None
+
+    def ploc_at(self, pts):
+        """Physical locations (npts, 3, neles)."""
+        op = self.sbasis.nodal_basis_at(pts)
+        ploc = op @ self.eles.reshape(self.nspts, -1)
+        return ploc.reshape(len(pts), -1, self.ndims).swapaxes(1, 2)
+
+    def smats_djacs_mpts(self):
+        if self._smats_djacs is not None:
+            return self._smats_djacs
+        mb = self.mbasis
+        mpts, nm, ne = mb.pts, len(mb.pts), self.neles
+        x = self.ploc_at(mpts)                               # (nm, 3, E)
+        jacop = mb.jac_nodal_basis_at(mpts)                  # (3, nm, nm)
+        # jac[a][b] = dX_b/dxi_a at the pseudo grid points, (3, 3, nm, E)
+        jac = np.einsum('apq,qbe->abpe', jacop, x)
+        dtt = []
+        for a in range(3):
+            tt = np.cross(x, jac[a].transpose(1, 0, 2), axisa=1, axisb=1,
+                          axisc=1)                           # (nm, 3, E)
+            # derivative along each axis c of (X x X_a): (c, nm, comp, E)
+            dtt.append(np.einsum('cpq,qke->cpke', jacop, tt))
+        smats = np.empty((3, nm, 3, ne))
+        smats[0] = 0.5*(dtt[2][1] - dtt[1][2])
+        smats[1] = 0.5*(dtt[0][2] - dtt[2][0])
+        smats[2] = 0.5*(dtt[1][0] - dtt[0][1])
+        x0, x1, x2 = jac
+        x1cx2 = np.cross(x1, x2, axisa=0, axisb=0, axisc=0)
+        djacs = np.einsum('bpe,bpe->pe', x0, x1cx2)
+        self._smats_djacs = (smats, djacs)
+        return smats, djacs
+
+    def smat_at(self, pts):
+        """(3, npts, 3, neles): S[i][k] at each point."""
+        smats, _ = self.smats_djacs_mpts()
+        m0 = self.mbasis.nodal_basis_at(pts)
+        return np.einsum('pm,imke->ipke', m0, smats)
+
+    def rcpdjac_at(self, pts):
+        _, djacs = self.smats_djacs_mpts()
+        djac = self.mbasis.nodal_basis_at(pts) @ djacs
+        if np.any(djac < -1e-5):
+            raise RuntimeError('Negative mesh Jacobians detected')
+        return 1.0/djac
+
+    def pnorm_at(self, pts, norms):
+        """Physical normals (npts, neles, 3)."""
+        s = self.smat_at(pts)
+        pn = np.einsum('ipke,pi->pek', s, norms)
+        if np.any(np.sqrt(np.einsum('...i,...i', pn, pn)) < 1e-10):
+            raise RuntimeError('Zero face normals detected')
+        return pn

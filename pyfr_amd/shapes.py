@@ -1,0 +1,240 @@
+"""FR operator matrices for tensor-product hexahedra.
+
+The reference builds its operators from a Vandermonde matrix of an
+orthonormal basis (``pyfr/shapes.py:79-135``, ``pyfr/polys.py:45-96``).  Here
+the hex operators are assembled in closed form from one-dimensional
+Gauss-Legendre quantities, which exposes the sparsity the device kernels use:
+
+* ``l_j(+-1)``  Lagrange basis of the n GL points evaluated at the ends,
+* ``D[i, j] = l_j'(x_i)``  the 1-D differentiation matrix,
+* ``h^{+-}(x_i) = sum_k (k + 1/2) (+-1)^k P_k(x_i)``  the 1-D reproducing
+  kernel at the ends, which is the value of the DG/FR correction function of
+  ``BaseShape.gbasis_at`` (``pyfr/shapes.py:231-259``) on the line normal to a
+  face when the face flux points are the GL points.
+
+Orderings follow the reference exactly: solution points are the GL tensor
+rule with x fastest (``pyfr/quadrules/hex/gauss-legendre-n*.txt``), flux points
+run face by face through ``HexShape.faces`` (``pyfr/shapes.py:378-385``) with
+the first face coordinate fastest, and the operator layouts are those of
+``m0``..``m6`` (``pyfr/shapes.py:89-123``).
+"""
+from functools import cached_property
+import itertools as it
+
+import numpy as np
+
+
+# Face table of HexShape (pyfr/shapes.py:378-385): for each face the axis of
+# the face normal, its side (-1/+1) and the axes that the face coordinates
+# (s, t) map onto.
+HEX_FACES = [
+    # (normal axis, side, s axis, t axis)
+    (2, -1, 0, 1),
+    (1, -1, 0, 2),
+    (0, +1, 1, 2),
+    (1, +1, 0, 2),
+    (0, -1, 1, 2),
+    (2, +1, 0, 1),
+]
+
+
+def gauss_legendre(n):
+    x, w = np.polynomial.legendre.leggauss(n)
+    return x, w
+
+
+def lagrange_at(x, pts):
+    """Values l_j(pts[q]) of the Lagrange basis through x: (len(pts), n)."""
+    n = len(x)
+    out = np.ones((len(pts), n))
+    for j in range(n):
+        for m in range(n):
+            if m != j:
+                out[:, j] *= (pts - x[m])/(x[j] - x[m])
+    return out
+
+
+def diff_matrix(x):
+    """D[i, j] = l_j'(x_i) via barycentric weights."""
+    n = len(x)
+    wb = np.array([1/np.prod([x[j] - x[m] for m in range(n) if m != j])
+                   for j in range(n)])
+    D = np.zeros((n, n))
+    for i in range(n):
+        for j in range(n):
+            if i != j:
+                D[i, j] = wb[j]/wb[i]/(x[i] - x[j])
+        D[i, i] = -D[i].sum()
+    return D
+
+
+def legendre_kernel_end(x, side):
+    """h(x_i) = sum_{k<=p} (k + 1/2) side^k P_k(x_i) for n = len(x) points."""
+    n = len(x)
+    P = np.polynomial.legendre.legvander(x, n - 1)      # (n, n): P_k(x_i)
+    coef = np.array([(k + 0.5)*side**k for k in range(n)])
+    return P @ coef
+
+
+def flush(a, tol=1e-10):
+    """Zero entries below tol (the first step of ``pyfr/nputil.py:23-57``)."""
+    a = np.array(a, dtype=float, copy=True)
+    a[np.abs(a) < tol] = 0
+    return a
+
+
+class HexShape:
+    """Hex FR element of order p with GL solution and GL flux points."""
+    name = 'hex'
+    ndims = 3
+    nfaces = 6
+
+    def __init__(self, order):
+        if order < 1:
+            raise ValueError('Order must be >= 1')
+        self.order = order
+        self.n = n = order + 1
+        self.x1d, self.w1d = gauss_legendre(n)
+        self.nupts = n**3
+        self.nfacefpts = [n*n]*6
+        self.nfpts = 6*n*n
+
+    # -- point sets
+    def uidx(self, i, j, k):
+        return i + self.n*(j + self.n*k)
+
+    @cached_property
+    def upts(self):
+        x, n = self.x1d, self.n
+        return np.array([(x[i], x[j], x[k])
+                         for k in range(n) for j in range(n) for i in range(n)])
+
+    @cached_property
+    def upts_wts(self):
+        w, n = self.w1d, self.n
+        return np.array([w[i]*w[j]*w[k]
+                         for k in range(n) for j in range(n) for i in range(n)])
+
+    @cached_property
+    def fpts(self):
+        x, n = self.x1d, self.n
+        pts = []
+        for nax, side, sax, tax in HEX_FACES:
+            for b in range(n):
+                for a in range(n):
+                    p = [0.0, 0.0, 0.0]
+                    p[nax], p[sax], p[tax] = float(side), x[a], x[b]
+                    pts.append(p)
+        return np.array(pts)
+
+    @cached_property
+    def norm_fpts(self):
+        out = np.zeros((self.nfpts, 3))
+        for f, (nax, side, _, _) in enumerate(HEX_FACES):
+            out[f*self.n**2:(f + 1)*self.n**2, nax] = side
+        return out
+
+    @cached_property
+    def facefpts(self):
+        m = self.n**2
+        return [list(range(f*m, (f + 1)*m)) for f in range(6)]
+
+    def fpt_line(self, f, a, b):
+        """Solution points on the line normal to face f through its flux point
+        (a, b), ordered along the normal axis, and that axis."""
+        nax, side, sax, tax = HEX_FACES[f]
+        line = []
+        for l in range(self.n):
+            c = [0, 0, 0]
+            c[nax], c[sax], c[tax] = l, a, b
+            line.append(self.uidx(*c))
+        return line, nax, side
+
+    # -- 1-D building blocks
+    @cached_property
+    def lend(self):
+        """{-1: l_j(-1), +1: l_j(+1)}"""
+        v = lagrange_at(self.x1d, np.array([-1.0, 1.0]))
+        return {-1: v[0], +1: v[1]}
+
+    @cached_property
+    def dmat(self):
+        return diff_matrix(self.x1d)
+
+    @cached_property
+    def hend(self):
+        return {s: legendre_kernel_end(self.x1d, s) for s in (-1, +1)}
+
+    # -- operators (layouts of pyfr/shapes.py:89-123)
+    @cached_property
+    def m0(self):
+        n = self.n
+        m = np.zeros((self.nfpts, self.nupts))
+        for f in range(6):
+            for b in range(n):
+                for a in range(n):
+                    line, _, side = self.fpt_line(f, a, b)
+                    m[f*n*n + a + n*b, line] = self.lend[side]
+        return m
+
+    @cached_property
+    def m1(self):
+        n, Nu = self.n, self.nupts
+        m = np.zeros((Nu, 3*Nu))
+        D = self.dmat
+        for k, j, i in it.product(range(n), repeat=3):
+            u = self.uidx(i, j, k)
+            c = (i, j, k)
+            for d in range(3):
+                for l in range(n):
+                    cc = list(c)
+                    cc[d] = l
+                    m[u, d*Nu + self.uidx(*cc)] = D[c[d], l]
+        return m
+
+    @cached_property
+    def m2(self):
+        m = self.norm_fpts[..., None]*self.m0[:, None, :]
+        return m.reshape(self.nfpts, -1)
+
+    @cached_property
+    def m3(self):
+        n = self.n
+        m = np.zeros((self.nupts, self.nfpts))
+        for f in range(6):
+            for b in range(n):
+                for a in range(n):
+                    line, _, side = self.fpt_line(f, a, b)
+                    m[line, f*n*n + a + n*b] = self.hend[side]
+        return m
+
+    @cached_property
+    def m4(self):
+        m = self.m1.reshape(self.nupts, -1, self.nupts).swapaxes(0, 1)
+        return m.reshape(-1, self.nupts)
+
+    @cached_property
+    def m6(self):
+        m = self.norm_fpts.T[:, None, :]*self.m3
+        return m.reshape(-1, self.nfpts)
+
+    def opmat(self, expr):
+        """Evaluate an operator expression ('M0', 'M1 - M3*M2', ...) the way
+        ``BaseShape.opmat`` (``pyfr/shapes.py:79-87``) does, flushing
+        |x| < 1e-10 to zero."""
+        import re
+        e = expr.lower().replace('*', '@')
+        if not re.match(r'[m0-9\-+@() ]+$', e):
+            raise ValueError('Invalid operator matrix expression')
+        mats = {m: getattr(self, m) for m in re.findall(r'm\d+', e)}
+        return flush(eval(e, {'__builtins__': None}, mats))
+
+    # -- linear element geometry helpers
+    @staticmethod
+    def linspts():
+        """Vertices of the reference hex (``TensorProdShape.std_ele(1)``)."""
+        pts1d = np.linspace(-1, 1, 2)
+        return np.array([p[::-1] for p in it.product(pts1d, repeat=3)])
+
+
+SHAPES = {'hex': HexShape}

@@ -1,0 +1,793 @@
+"""Solver-side host code: elements, interfaces, systems and explicit
+integrators for the Euler and Navier-Stokes FR right-hand side.
+
+This is the build's counterpart of the reference's solver layer, written
+against the backend API of ``pyfr_amd.backend`` the way the reference's is
+written against ``pyfr/backends/base``:
+
+* elements     -- ``pyfr/solvers/base/elements.py``,
+  ``pyfr/solvers/baseadvec/elements.py:55-127``,
+  ``pyfr/solvers/baseadvecdiff/elements.py:22-131``,
+  ``pyfr/solvers/euler/elements.py:149-206``,
+  ``pyfr/solvers/navstokes/elements.py:31-128``;
+* interfaces   -- ``pyfr/solvers/base/inters.py``,
+  ``pyfr/solvers/baseadvec/inters.py:8-29``,
+  ``pyfr/solvers/baseadvecdiff/inters.py:7-35``,
+  ``pyfr/solvers/euler/inters.py:28-37``, ``pyfr/solvers/navstokes/inters.py:29-47``;
+* systems      -- ``pyfr/solvers/base/system.py``,
+  ``pyfr/solvers/baseadvec/system.py:57-136``,
+  ``pyfr/solvers/baseadvecdiff/system.py:53-231``;
+* integrators  -- ``pyfr/integrators/explicit/steppers.py:60-243``,
+  ``pyfr/integrators/base.py:282-296``, ``:401-426``,
+  ``pyfr/integrators/explicit/controllers.py:42-60``.
+
+Scope: periodic meshes of hexahedra on one GPU (no boundary conditions, no
+MPI interfaces, no anti-aliasing, no source terms).
+"""
+from functools import cached_property
+import math
+import time
+
+import numpy as np
+
+from pyfr_amd.geometry import HexGeometry
+from pyfr_amd.mesh import sorted_face_fpts_structured
+from pyfr_amd.shapes import SHAPES
+
+
+def _npeval(expr, env):
+    """Evaluate an initial-condition expression (``pyfr/nputil.py:103-117``)."""
+    syms = {'__builtins__': {}, 'exp': np.exp, 'log': np.log, 'sin': np.sin,
+            'cos': np.cos, 'tan': np.tan, 'tanh': np.tanh, 'sqrt': np.sqrt,
+            'abs': np.abs, 'pow': np.power, 'pi': np.pi, 'atan': np.arctan,
+            'atan2': np.arctan2, 'asin': np.arcsin, 'acos': np.arccos,
+            'max': np.maximum, 'min': np.minimum}
+    if '**' in expr or '^' in expr:
+        raise ValueError('Direct exponentiation is not supported; use pow')
+    return eval(expr, syms, env)
+
+
+class Interside:
+    """One side of a set of interfaces: (etype, eidx, fidx) per interface,
+    grouped like ``Connectivity.foreach`` (``pyfr/readers/native.py:76-80``)."""
+
+    def __init__(self, side):
+        self.side = side
+
+    def __len__(self):
+        return len(self.side)
+
+    def foreach(self):
+        groups = {}
+        for idx, (et, e, f) in enumerate(self.side):
+            groups.setdefault((et, f), ([], []))
+            groups[(et, f)][0].append(e)
+            groups[(et, f)][1].append(idx)
+        for (et, f), (eidxs, idxs) in sorted(groups.items()):
+            yield et, f, np.array(eidxs), np.array(idxs)
+
+    def items(self):
+        for et, f, eidxs, _ in self.foreach():
+            yield et, f, eidxs
+
+
+# ---------------------------------------------------------------------------
+# Elements
+# ---------------------------------------------------------------------------
+
+class FluidElements:
+    """Base elements with the fluid variable sets
+    (``pyfr/solvers/euler/elements.py:8-114``)."""
+    convars = ['rho', 'rhou', 'rhov', 'rhow', 'E']
+    privars = ['rho', 'u', 'v', 'w', 'p']
+
+    def __init__(self, basiscls, eles, cfg):
+        self._be = None
+        self.eles = eles
+        self.cfg = cfg
+        self.nspts, self.neles, self.ndims = eles.shape
+        self.nvars = len(self.convars)
+        self.kernels = {}
+        order = cfg.getint('solver', 'order')
+        self.basis = basis = basiscls(order)
+        self.name = basis.name
+        self.nupts, self.nfpts = basis.nupts, basis.nfpts
+        self.nfacefpts = basis.nfacefpts
+        self.geom = HexGeometry(eles, order)
+
+    # -- initial conditions (pyfr/solvers/base/elements.py:86-129)
+    @staticmethod
+    def pri_to_con(pris, gamma):
+        rho, p = pris[0], pris[-1]
+        rhovs = [rho*c for c in pris[1:-1]]
+        E = p/(gamma - 1) + 0.5*rho*sum(c*c for c in pris[1:-1])
+        return [rho, *rhovs, E]
+
+    def set_ics_from_cfg(self):
+        env = self.cfg.items_as('constants', float)
+        coords = self.geom.ploc_at(self.basis.upts)
+        env |= dict(zip('xyz', coords.swapaxes(0, 1)))
+        ics = [_npeval(self.cfg.getexpr('soln-ics', v), env)
+               for v in self.privars]
+        ics = [np.broadcast_to(v, (self.nupts, self.neles)) for v in ics]
+        con = self.pri_to_con(ics, self.cfg.getfloat('constants', 'gamma'))
+        out = np.empty((self.nupts, self.nvars, self.neles))
+        for i, v in enumerate(con):
+            out[:, i] = v
+        return out
+
+    # -- geometry
+    @cached_property
+    def plocfpts(self):
+        p = self.geom.ploc_at(self.basis.fpts)           # (nfpts, 3, E)
+        return p.transpose(0, 2, 1)                        # (nfpts, E, 3)
+
+    @cached_property
+    def srtd_face_fpts(self):
+        return sorted_face_fpts_structured(self.plocfpts, self.basis.facefpts)
+
+    @cached_property
+    def _pnorm_fpts(self):
+        return self.geom.pnorm_at(self.basis.fpts, self.basis.norm_fpts)
+
+    def rcpdjac_at_np(self, name):
+        return self.geom.rcpdjac_at(getattr(self.basis, name))
+
+    # -- backend set-up (pyfr/solvers/base/elements.py:218-266)
+    @property
+    def _scratch_bufs(self):
+        return {'scal_fpts', 'vect_upts'}
+
+    def set_backend(self, backend, nonce, linoff):
+        self._be = backend
+        self.grad_fusion = not backend.blocks
+        if self.basis.order >= 2:
+            self.linoff = linoff - linoff % -backend.csubsz
+        else:
+            self.linoff = self.neles
+
+        nvars, neles, ndims = self.nvars, self.neles, self.ndims
+        sbufs = self._scratch_bufs
+
+        def alloc(ex, shape):
+            return backend.matrix(shape, extent=nonce + ex, tags={'align'})
+
+        if 'scal_fpts' in sbufs:
+            self._scal_fpts = alloc('scal_fpts', (self.nfpts, nvars, neles))
+        if 'vect_upts' in sbufs:
+            self._vect_upts = alloc('vect_upts',
+                                    (ndims, self.nupts, nvars, neles))
+        if 'vect_fpts' in sbufs:
+            self._vect_fpts = alloc('vect_fpts',
+                                    (ndims, self.nfpts, nvars, neles))
+            self._comm_fpts = None   # a slice once allocated (see below)
+        if 'grad_upts' in sbufs:
+            self._grad_upts = alloc('grad_upts',
+                                    (ndims, self.nupts, nvars, neles))
+        elif 'vect_upts' in sbufs:
+            self._grad_upts = self._vect_upts
+        self.scal_upts = []
+
+    def post_commit(self):
+        if hasattr(self, '_vect_fpts'):
+            self._comm_fpts = self._vect_fpts.slice(0, self.nfpts)
+
+    def alloc_bank(self, extent, ic=None):
+        m = self._be.matrix((self.nupts, self.nvars, self.neles), ic,
+                            extent=extent, tags={'align'})
+        self.scal_upts.append(m)
+        return m
+
+    _opmats = None
+
+    def opmat(self, expr):
+        if self._opmats is None:
+            self._opmats = {}
+        if expr not in self._opmats:
+            self._opmats[expr] = self._be.const_matrix(
+                self.basis.opmat(expr), tags={expr, 'align'})
+        return self._opmats[expr]
+
+    @property
+    def mesh_regions(self):
+        off = self.linoff
+        if off == 0:
+            return {'linear': self.neles}
+        elif off >= self.neles:
+            return {'curved': self.neles}
+        return {'curved': off, 'linear': self.neles - off}
+
+    def _slice_mat(self, mat, region):
+        off = self.linoff
+        if len(mat.ioshape) >= 3:
+            off *= mat.ioshape[-2]
+        else:
+            off = min(off, mat.ncol)
+        if region == 'curved':
+            return mat.slice(0, mat.nrow, 0, off)
+        elif region == 'linear':
+            if off == 0:
+                return mat
+            return mat.slice(0, mat.nrow, off, mat.ncol)
+        raise ValueError('Invalid slice region')
+
+    # -- constant geometric matrices
+    @cached_property
+    def _verts_linear(self):
+        # ploc_at('linspts') = the vertices (nspts, ndims, neles)
+        v = self.eles[:, self.linoff:].transpose(0, 2, 1)
+        return self._be.const_matrix(np.ascontiguousarray(v), tags={'align'})
+
+    @cached_property
+    def _upts_const(self):
+        return self._be.const_matrix(self.basis.upts)
+
+    @cached_property
+    def _smats_curved(self):
+        s = self.geom.smat_at(self.basis.upts)[..., :self.linoff]
+        return self._be.const_matrix(s, tags={'align'})
+
+    @cached_property
+    def _rcpdjac_upts(self):
+        return self._be.const_matrix(self.rcpdjac_at_np('upts'),
+                                     tags={'align'})
+
+    def _rcpdjac_curved(self):
+        r = self.rcpdjac_at_np('upts')[:, :self.linoff]
+        return self._be.const_matrix(r, tags={'align'})
+
+    # -- interface helpers (pyfr/solvers/base/elements.py:446-469)
+    def get_pnorms_for_inters(self, eidxs, fidx):
+        fpts_idx = self.srtd_face_fpts[fidx][eidxs]
+        pn = self._pnorm_fpts[fpts_idx, eidxs[:, None]]
+        return pn.reshape(-1, self.ndims),
+
+    def _inters_map(self, mid, rmap, eidxs, fidx, lda=None):
+        nfp = self.nfacefpts[fidx]
+        n = len(eidxs)
+        cmap = np.repeat(eidxs, nfp)
+        out = [np.full(n*nfp, mid), rmap.ravel(), cmap]
+        if lda is not None:
+            out.append(np.full(n*nfp, lda))
+        return out
+
+    def get_vect_fpts_for_inters(self, eidxs, fidx):
+        rmap = self.srtd_face_fpts[fidx][eidxs]
+        return self._inters_map(self._vect_fpts.mid, rmap, eidxs, fidx,
+                                self.nfpts)
+
+    def get_comm_fpts_for_inters(self, eidxs, fidx):
+        # fpts not in upts: the common solution lives in the first nfpts rows
+        # of vect_fpts (pyfr/solvers/base/elements.py:70-75, :254-259)
+        rmap = self.srtd_face_fpts[fidx][eidxs]
+        return self._inters_map(self._vect_fpts.mid, rmap, eidxs, fidx,
+                                self.nfpts)
+
+
+class EulerElements(FluidElements):
+    def set_backend(self, backend, nonce, linoff):
+        super().set_backend(backend, nonce, linoff)
+        be, kernels = backend, self.kernels
+        be.pointwise.register('pyfr.solvers.baseadvec.kernels.negdivconf')
+
+        # pyfr/solvers/baseadvec/elements.py:73-119
+        kernels['disu'] = lambda uin: be.kernel(
+            'mul', self.opmat('M0'), self.scal_upts[uin], out=self._scal_fpts)
+        kernels['tdivtpcorf'] = lambda fout: be.kernel(
+            'mul', self.opmat('M1 - M3*M2'), self._vect_upts,
+            out=self.scal_upts[fout])
+        kernels['tdivtconf'] = lambda fout: be.kernel(
+            'mul', self.opmat('M3'), self._scal_fpts,
+            out=self.scal_upts[fout], beta=float(self.basis.order > 0))
+        kernels['negdivconf'] = lambda fout: be.kernel(
+            'negdivconf', tplargs={'ndims': self.ndims, 'nvars': self.nvars,
+                                   'src_macros': []},
+            dims=[self.nupts, self.neles], tdivtconf=self.scal_upts[fout],
+            rcpdjac=self._rcpdjac_upts)
+
+        # pyfr/solvers/euler/elements.py:149-206
+        be.pointwise.register('pyfr.solvers.euler.kernels.tflux')
+        tplargs = {'ndims': self.ndims, 'nvars': self.nvars,
+                   'nverts': self.nspts,
+                   'c': self.cfg.items_as('constants', float)}
+        r, s = self.mesh_regions, self._slice_mat
+
+        def tdisf(uin):
+            ks = []
+            if 'curved' in r:
+                ks.append(be.kernel(
+                    'tflux', tplargs=tplargs | {'ktype': 'curved'},
+                    dims=[self.nupts, r['curved']],
+                    u=s(self.scal_upts[uin], 'curved'),
+                    f=s(self._vect_upts, 'curved'), smats=self._smats_curved))
+            if 'linear' in r:
+                ks.append(be.kernel(
+                    'tflux', tplargs=tplargs | {'ktype': 'linear'},
+                    dims=[self.nupts, r['linear']],
+                    u=s(self.scal_upts[uin], 'linear'),
+                    f=s(self._vect_upts, 'linear'), verts=self._verts_linear,
+                    upts=self._upts_const))
+            return ks[0] if len(ks) == 1 else be.unordered_meta_kernel(ks)
+
+        kernels['tdisf'] = tdisf
+
+
+class NavierStokesElements(EulerElements):
+    @property
+    def _scratch_bufs(self):
+        # pyfr/solvers/baseadvecdiff/elements.py:7-20 (gradient fusion, GL)
+        return {'scal_fpts', 'vect_fpts', 'vect_upts', 'grad_upts'}
+
+    def set_backend(self, backend, nonce, linoff):
+        FluidElements.set_backend(self, backend, nonce, linoff)
+        be, kernels = backend, self.kernels
+        be.pointwise.register('pyfr.solvers.baseadvec.kernels.negdivconf')
+        be.pointwise.register('pyfr.solvers.navstokes.kernels.tflux')
+
+        kernels['disu'] = lambda uin: be.kernel(
+            'mul', self.opmat('M0'), self.scal_upts[uin], out=self._scal_fpts)
+        kernels['tdivtpcorf'] = lambda fout: be.kernel(
+            'mul', self.opmat('M1 - M3*M2'), self._vect_upts,
+            out=self.scal_upts[fout])
+        kernels['tdivtconf'] = lambda fout: be.kernel(
+            'mul', self.opmat('M3'), self._scal_fpts,
+            out=self.scal_upts[fout], beta=float(self.basis.order > 0))
+        kernels['negdivconf'] = lambda fout: be.kernel(
+            'negdivconf', tplargs={'ndims': self.ndims, 'nvars': self.nvars,
+                                   'src_macros': []},
+            dims=[self.nupts, self.neles], tdivtconf=self.scal_upts[fout],
+            rcpdjac=self._rcpdjac_upts)
+
+        # pyfr/solvers/baseadvecdiff/elements.py:47-55
+        kernels['tgradpcoru_upts'] = lambda uin: be.kernel(
+            'mul', self.opmat('M4 - M6*M0'), self.scal_upts[uin],
+            out=self._grad_upts)
+        kernels['tgradcoru_upts'] = lambda: be.kernel(
+            'mul', self.opmat('M6'), self._comm_fpts, out=self._grad_upts,
+            beta=float(self.basis.order > 0))
+
+        # pyfr/solvers/baseadvecdiff/elements.py:87-101
+        def gradcoru_fpts():
+            nupts, nfpts = self.nupts, self.nfpts
+            vu, vf = self._grad_upts, self._vect_fpts
+            muls = [be.kernel('mul', self.opmat('M0'),
+                              vu.slice(i*nupts, (i + 1)*nupts),
+                              vf.slice(i*nfpts, (i + 1)*nfpts))
+                    for i in range(self.ndims)]
+            return be.unordered_meta_kernel(muls)
+        kernels['gradcoru_fpts'] = gradcoru_fpts
+
+        # pyfr/solvers/navstokes/elements.py:31-128 (tdisf_fused)
+        visc_corr = self.cfg.get('solver', 'viscosity-correction', 'none')
+        if visc_corr not in {'sutherland', 'none'}:
+            raise ValueError('Invalid viscosity-correction option')
+        tplargs = {'ndims': self.ndims, 'nvars': self.nvars,
+                   'nverts': self.nspts,
+                   'c': self.cfg.items_as('constants', float),
+                   'visc_corr': visc_corr, 'shock_capturing': 'none'}
+        r, s = self.mesh_regions, self._slice_mat
+
+        def tdisf_fused(uin):
+            ks = []
+            if 'curved' in r:
+                ks.append(be.kernel(
+                    'tflux', tplargs=tplargs | {'ktype': 'curved-fused'},
+                    dims=[self.nupts, r['curved']],
+                    u=s(self.scal_upts[uin], 'curved'),
+                    f=s(self._vect_upts, 'curved'),
+                    gradu=s(self._grad_upts, 'curved'),
+                    smats=self._smats_curved,
+                    rcpdjac=self._rcpdjac_curved(), upts=self._upts_const))
+            if 'linear' in r:
+                ks.append(be.kernel(
+                    'tflux', tplargs=tplargs | {'ktype': 'linear-fused'},
+                    dims=[self.nupts, r['linear']],
+                    u=s(self.scal_upts[uin], 'linear'),
+                    f=s(self._vect_upts, 'linear'),
+                    gradu=s(self._grad_upts, 'linear'),
+                    verts=self._verts_linear, upts=self._upts_const))
+            return ks[0] if len(ks) == 1 else be.unordered_meta_kernel(ks)
+
+        kernels['tdisf_fused'] = tdisf_fused
+
+
+# ---------------------------------------------------------------------------
+# Interfaces
+# ---------------------------------------------------------------------------
+
+def _get_inter_arrays(interside, meth, elemap, perm=Ellipsis):
+    """``pyfr/solvers/base/inters.py:6-17``."""
+    parts, reorder = [], []
+    for etype, fidx, eidxs, idx in interside.foreach():
+        parts.append(getattr(elemap[etype], meth)(eidxs, fidx))
+        reorder.append(np.repeat(idx, elemap[etype].nfacefpts[fidx]))
+    if not parts:
+        return []
+    ro = np.argsort(np.concatenate(reorder), kind='stable')[perm]
+    return [np.concatenate(a)[ro] for a in zip(*parts)]
+
+
+class EulerIntInters:
+    """Internal interfaces (``pyfr/solvers/base/inters.py:20-119``,
+    ``baseadvec/inters.py:8-29``, ``euler/inters.py:28-37``)."""
+
+    def __init__(self, be, lhs, rhs, elemap, cfg):
+        self._be = be
+        self.elemap = elemap
+        self.cfg = cfg
+        first = next(iter(elemap.values()))
+        self.ndims, self.nvars = first.ndims, first.nvars
+        self.lhs, self.rhs = Interside(lhs), Interside(rhs)
+        self.ninters = len(lhs)
+        self.ninterfpts = sum(elemap[et].nfacefpts[f]*len(e)
+                              for et, f, e in self.lhs.items())
+        self.c = cfg.items_as('constants', float)
+        self.kernels = {}
+        self._perm = Ellipsis
+        scal = {t: e._scal_fpts for t, e in elemap.items()}
+        self._gen_perm(scal)
+        self._pnorm_lhs = self._const_mat(self.lhs, 'get_pnorms_for_inters')
+
+        rsolver = cfg.get('solver-interfaces', 'riemann-solver')
+        self._tplargs = dict(ndims=self.ndims, nvars=self.nvars,
+                             rsolver=rsolver, c=self.c)
+        be.pointwise.register('pyfr.solvers.euler.kernels.intcflux')
+        self.kernels['comm_flux'] = lambda: be.kernel(
+            'intcflux', tplargs=self._tplargs, dims=[self.ninterfpts],
+            ul=self.scal_lhs, ur=self.scal_rhs, nl=self._pnorm_lhs)
+
+    def _gen_perm(self, scal):
+        self._perm = self._get_perm_for_field(self.lhs, scal)
+
+    def _const_mat(self, inter, meth):
+        m = _get_inter_arrays(inter, meth, self.elemap, self._perm)
+        m = m[0] if m else np.empty((0, self.ndims))
+        return self._be.const_matrix(np.atleast_2d(m.T))
+
+    def _field_rows(self, inter):
+        matmap, rowmap, colmap, reorder = [], [], [], []
+        for etype, fidx, eidxs, idx in inter.foreach():
+            eles = self.elemap[etype]
+            fpts = eles.srtd_face_fpts[fidx][eidxs]
+            nfp = fpts.shape[1]
+            rowmap.append(fpts.ravel())
+            colmap.append(np.repeat(eidxs, nfp))
+            matmap.append(np.full(len(eidxs)*nfp, etype))
+            reorder.append(np.repeat(idx, nfp))
+        ro = np.argsort(np.concatenate(reorder), kind='stable')
+        return (np.concatenate(matmap)[ro], np.concatenate(rowmap)[ro],
+                np.concatenate(colmap)[ro])
+
+    def _get_perm_for_field(self, inter, field):
+        """Order the points by their LHS storage address
+        (``pyfr/solvers/base/inters.py:73-90``)."""
+        et, r, c = self._field_rows(inter)
+        mids = np.array([field[t].mid for t in et])
+        v = self._be.view(mids, r, c, vshape=())
+        return np.argsort(v.mapping_np)
+
+    def _view(self, inter, meth, vshape=()):
+        vm = _get_inter_arrays(inter, meth, self.elemap, self._perm)
+        return self._be.view(*vm, vshape=vshape)
+
+
+class NavierStokesIntInters(EulerIntInters):
+    """``baseadvecdiff/inters.py:7-35``, ``navstokes/inters.py:29-47``."""
+
+    def __init__(self, be, lhs, rhs, elemap, cfg):
+        self._beta = cfg.getfloat('solver-interfaces', 'ldg-beta')
+        super().__init__(be, lhs, rhs, elemap, cfg)
+        self.c |= cfg.items_as('solver-interfaces', float)
+        visc_corr = cfg.get('solver', 'viscosity-correction', 'none')
+        self._tplargs = dict(ndims=self.ndims, nvars=self.nvars,
+                             rsolver=self._tplargs['rsolver'], c=self.c,
+                             visc_corr=visc_corr)
+        self._vect_lhs = self._view(self.lhs, 'get_vect_fpts_for_inters',
+                                    (self.ndims, self.nvars))
+        self._vect_rhs = self._view(self.rhs, 'get_vect_fpts_for_inters',
+                                    (self.ndims, self.nvars))
+        self._comm_lhs = self._view(self.lhs, 'get_comm_fpts_for_inters',
+                                    (self.nvars,))
+        self._comm_rhs = self._view(self.rhs, 'get_comm_fpts_for_inters',
+                                    (self.nvars,))
+        be.pointwise.register('pyfr.solvers.navstokes.kernels.intconu')
+        be.pointwise.register('pyfr.solvers.navstokes.kernels.intcflux')
+        self.kernels['con_u'] = lambda: be.kernel(
+            'intconu', tplargs=self._tplargs, dims=[self.ninterfpts],
+            ulin=self.scal_lhs, urin=self.scal_rhs, ulout=self._comm_lhs,
+            urout=self._comm_rhs)
+        self.kernels['comm_flux'] = lambda: be.kernel(
+            'intcflux', tplargs=self._tplargs, dims=[self.ninterfpts],
+            ul=self.scal_lhs, ur=self.scal_rhs, gradul=self._vect_lhs,
+            gradur=self._vect_rhs, nl=self._pnorm_lhs)
+
+    def _gen_perm(self, scal):
+        side = self.lhs if self._beta != -0.5 else self.rhs
+        self._perm = self._get_perm_for_field(side, scal)
+
+
+# ---------------------------------------------------------------------------
+# Systems
+# ---------------------------------------------------------------------------
+
+class BaseSystem:
+    elementscls = None
+    intinterscls = None
+    name = None
+
+    def __init__(self, backend, mesh, cfg, nregs):
+        self.backend = backend
+        self.mesh = mesh
+        self.cfg = cfg
+        self.ndims = mesh.ndims
+        self.nvars = 5
+        nonce = 'sys0'
+
+        # Elements, ICs and register banks (system.py:28-105)
+        elemap = {}
+        for etype, spts in mesh.spts.items():
+            elemap[etype] = self.elementscls(SHAPES[etype], spts, cfg)
+        ics = {et: e.set_ics_from_cfg() for et, e in elemap.items()}
+        for et, e in elemap.items():
+            e.set_backend(backend, nonce, linoff=0)
+        self.ele_map = elemap
+        self.ele_types = list(elemap)
+        self.ele_ndofs = [e.neles*e.nupts*e.nvars for e in elemap.values()]
+        self.ele_shapes = {et: (e.nupts, e.nvars, e.neles)
+                           for et, e in elemap.items()}
+        self.ele_banks = []
+        for et, e in elemap.items():
+            banks = [e.alloc_bank(f'bank_{et}', ic=ics[et] if i == 0 else None)
+                     for i in range(nregs)]
+            self.ele_banks.append(banks)
+        backend.commit()
+        for e in elemap.values():
+            e.post_commit()
+
+        # Interfaces
+        lhs, rhs = mesh.con
+        self._int_inters = [self.intinterscls(backend, lhs, rhs, elemap, cfg)]
+        self._commit()
+        self._kcache = {}
+
+    def _commit(self):
+        # make_field_views for scal_fpts (baseadvec/system.py:20-55)
+        for i in self._int_inters:
+            i.scal_lhs = i._view(i.lhs, '_scal_fpts_for_inters', (self.nvars,))
+            i.scal_rhs = i._view(i.rhs, '_scal_fpts_for_inters', (self.nvars,))
+
+    @property
+    def gndofs(self):
+        return sum(self.ele_ndofs)
+
+    def rhs(self, t, uinbank, foutbank):
+        """``pyfr/solvers/base/system.py:420-430``."""
+        for g in self._rhs_graphs(uinbank, foutbank):
+            self.backend.run_graph(g)
+
+    def _rhs_graphs(self, uin, fout):
+        key = (uin, fout)
+        if key not in self._kcache:
+            self._kcache[key] = self._build_graphs(uin, fout)
+        return self._kcache[key]
+
+    def ele_scal_upts(self, idx):
+        return [b[idx].get() for b in self.ele_banks]
+
+    def set_ele_scal_upts(self, idx, arrs):
+        for b, a in zip(self.ele_banks, arrs):
+            b[idx].set(a)
+
+
+def _scal_fpts_for_inters(self, eidxs, fidx):
+    rmap = self.srtd_face_fpts[fidx][eidxs]
+    return self._inters_map(self._scal_fpts.mid, rmap, eidxs, fidx)
+
+
+FluidElements._scal_fpts_for_inters = _scal_fpts_for_inters
+
+
+class EulerSystem(BaseSystem):
+    name = 'euler'
+    elementscls = EulerElements
+    intinterscls = EulerIntInters
+
+    def _build_graphs(self, uin, fout):
+        """One graph holding ``g_intf`` then ``g_flux_div`` of
+        ``pyfr/solvers/baseadvec/system.py:57-136`` (no MPI to overlap)."""
+        be = self.backend
+        g = be.graph()
+        eles = list(self.ele_map.values())
+        ks = [e.kernels['disu'](uin) for e in eles]
+        ks += [i.kernels['comm_flux']() for i in self._int_inters]
+        for e in eles:
+            ks += [e.kernels['tdisf'](uin), e.kernels['tdivtpcorf'](fout),
+                   e.kernels['tdivtconf'](fout), e.kernels['negdivconf'](fout)]
+        g.add_all(ks)
+        g.commit()
+        return [g]
+
+
+class NavierStokesSystem(BaseSystem):
+    name = 'navier-stokes'
+    elementscls = NavierStokesElements
+    intinterscls = NavierStokesIntInters
+
+    def _build_graphs(self, uin, fout):
+        """``pyfr/solvers/baseadvecdiff/system.py:53-231`` with gradient
+        fusion, as one graph."""
+        be = self.backend
+        g = be.graph()
+        eles = list(self.ele_map.values())
+        ks = [e.kernels['disu'](uin) for e in eles]
+        ks += [i.kernels['con_u']() for i in self._int_inters]
+        for e in eles:
+            ks += [e.kernels['tgradpcoru_upts'](uin),
+                   e.kernels['tgradcoru_upts'](),
+                   e.kernels['tdisf_fused'](uin),
+                   e.kernels['gradcoru_fpts']()]
+        ks += [i.kernels['comm_flux']() for i in self._int_inters]
+        for e in eles:
+            ks += [e.kernels['tdivtpcorf'](fout), e.kernels['tdivtconf'](fout),
+                   e.kernels['negdivconf'](fout)]
+        g.add_all(ks)
+        g.commit()
+        return [g]
+
+
+SYSTEMS = {'euler': EulerSystem, 'navier-stokes': NavierStokesSystem}
+
+
+# ---------------------------------------------------------------------------
+# Explicit integrators with the fixed-dt ("none") controller
+# ---------------------------------------------------------------------------
+
+class BaseIntegrator:
+    nregs = None
+    nrhs_per_step = None
+
+    def __init__(self, backend, mesh, cfg):
+        self.backend = backend
+        self.cfg = cfg
+        sysname = cfg.get('solver', 'system')
+        self.system = SYSTEMS[sysname](backend, mesh, cfg, self.nregs)
+        self.tcurr = cfg.getfloat('solver-time-integrator', 'tstart', 0.0)
+        self._tcurr_comp = 0.0
+        self.dt = cfg.getfloat('solver-time-integrator', 'dt')
+        self.dtmin = cfg.getfloat('solver-time-integrator', 'dt-min', 1e-12)
+        self._dt_lookhead = cfg.getint('solver-time-integrator',
+                                       'dt-lookahead', 10)
+        self.idxcurr = 0
+        self.nsteps = 0
+        self.nrhsevals = 0
+        self._axnpby = {}
+        self._wtime = 0.0
+
+    def _rhs(self, t, uin, fout):
+        self.system.rhs(t, uin, fout)
+        self.nrhsevals += 1
+
+    def _add(self, *args):
+        """``pyfr/integrators/base.py:401-426``."""
+        consts, regidxs = args[::2], args[1::2]
+        if len(regidxs) != len(set(regidxs)):
+            raise ValueError('Duplicate register indices')
+        if regidxs not in self._axnpby:
+            self._axnpby[regidxs] = [
+                self.backend.kernel('axnpby', *[em[r] for r in regidxs])
+                for em in self.system.ele_banks]
+        ks = self._axnpby[regidxs]
+        for k in ks:
+            k.bind(*consts)
+        self.backend.run_kernels(ks)
+
+    def _advance_time(self, dt):
+        y = dt - self._tcurr_comp
+        t = self.tcurr + y
+        self._tcurr_comp = (t - self.tcurr) - y
+        self.tcurr = t
+
+    def _clamp_dt(self, dt_want, t):
+        remaining = t - self.tcurr
+        nsteps = -(-remaining // dt_want)
+        if nsteps > self._dt_lookhead:
+            return dt_want
+        return max(remaining/nsteps, self.dtmin)
+
+    def advance_to(self, t):
+        """``pyfr/integrators/explicit/controllers.py:48-60``."""
+        if t < self.tcurr:
+            raise ValueError('Advance time is in the past')
+        while self.tcurr < t:
+            dt = self._clamp_dt(self.dt, t)
+            self.idxcurr = self.step(self.tcurr, dt)
+            self._advance_time(dt)
+            self.nsteps += 1
+
+    def run_steps(self, nsteps, dt=None):
+        """Take exactly nsteps fixed steps (no clamping); used for timing."""
+        dt = self.dt if dt is None else dt
+        for _ in range(nsteps):
+            self.idxcurr = self.step(self.tcurr, dt)
+            self._advance_time(dt)
+            self.nsteps += 1
+
+    @property
+    def soln(self):
+        return self.system.ele_scal_upts(self.idxcurr)
+
+
+class RK4Integrator(BaseIntegrator):
+    """``pyfr/integrators/explicit/steppers.py:60-108``."""
+    nregs = 3
+    nrhs_per_step = 4
+
+    def step(self, t, dt):
+        add, rhs = self._add, self._rhs
+        r0, r1, r2 = 0, 1, 2
+        if r0 != self.idxcurr:
+            r0, r1 = r1, r0
+        rhs(t, r0, r1)
+        add(0.0, r2, 1.0, r0, dt/2.0, r1)
+        rhs(t + dt/2.0, r2, r2)
+        add(dt/6.0, r1, 1.0, r0, dt/3.0, r2)
+        add(dt/2.0, r2, 1.0, r0)
+        rhs(t + dt/2.0, r2, r2)
+        add(1.0, r1, dt/3.0, r2)
+        add(dt, r2, 1.0, r0)
+        rhs(t + dt, r2, r2)
+        add(1.0, r1, dt/6.0, r2)
+        return r1
+
+
+class RK45Integrator(BaseIntegrator):
+    """Low-storage RK45 (``steppers.py:111-190``, ``:218-243``)."""
+    nregs = 2
+    nrhs_per_step = 5
+    a = [970286171893 / 4311952581923, 6584761158862 / 12103376702013,
+         2251764453980 / 15575788980749, 26877169314380 / 34165994151039]
+    b = [1153189308089 / 22510343858157, 1772645290293 / 4653164025191,
+         -1672844663538 / 4480602732383, 2114624349019 / 3568978502595,
+         5198255086312 / 14908931495163]
+
+    def __init__(self, *args, **kwargs):
+        super().__init__(*args, **kwargs)
+        self.backend.pointwise.register(
+            'pyfr.integrators.explicit.kernels.rkvdh2')
+        self.c = [0.0] + [sum(self.b[:i]) + ai for i, ai in enumerate(self.a)]
+        self._nstages = len(self.c)
+        self._rk = {}
+
+    def _rkvdh2_kerns(self, stage, r1, r2):
+        key = (stage, r1, r2)
+        if key not in self._rk:
+            tplargs = {'a': self.a, 'b': self.b, 'stage': stage,
+                       'nstages': self._nstages, 'nvars': 5, 'errest': False}
+            self._rk[key] = [
+                self.backend.kernel('rkvdh2', tplargs=tplargs,
+                                    dims=[d[0], d[2]], r1=em[r1], r2=em[r2])
+                for d, em in zip(self.system.ele_shapes.values(),
+                                 self.system.ele_banks)]
+        return self._rk[key]
+
+    def step(self, t, dt):
+        r1 = self.idxcurr
+        r2 = 1 - r1
+        for i, ci in enumerate(self.c):
+            self._rhs(t + ci*dt, r2 if i > 0 else r1, r2)
+            kerns = self._rkvdh2_kerns(i, r1, r2)
+            for k in kerns:
+                k.bind(dt=dt)
+            self.backend.run_kernels(kerns)
+            r1, r2 = r2, r1
+        return r2
+
+
+INTEGRATORS = {'rk4': RK4Integrator, 'rk45': RK45Integrator}
+
+
+def get_integrator(backend, mesh, cfg):
+    scheme = cfg.get('solver-time-integrator', 'scheme')
+    if cfg.get('solver-time-integrator', 'controller', 'none') != 'none':
+        raise ValueError('Only the fixed-dt "none" controller is supported')
+    return INTEGRATORS[scheme](backend, mesh, cfg)
