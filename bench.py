@@ -1,0 +1,241 @@
+"""Benchmark: GDOF.RHS/s of the FR right-hand side on one MI355X.
+
+Workload (the headline configuration that fits one GPU): compressible
+Navier-Stokes on the periodic cube [0, 2pi]^3 meshed as 64^3 linear hexes,
+FR order p=3 (64 Gauss-Legendre solution points per element, 83.9M DOF),
+FP64, Rusanov + LDG (beta=0, tau=0) interfaces, fixed-dt RK45 (dt=2e-4),
+Taylor-Green initial field at Mach 0.1, Re 1600.  A step is one RK45 step
+(5 RHS evaluations + 5 stage updates) with all data resident in HBM.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--config cfg1|cfg0]
+
+Prints one JSON line with the metric, a roofline object for the dominant
+kernel (algorithmic bytes per launch / mean launch time from HIP events on
+the solver's stream) and a CPU baseline (the oracle on a bounded sample).
+"""
+import argparse
+import ctypes as ct
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+CONFIGS = {
+    'cfg1': dict(system='navier-stokes', n=64, order=3, scheme='rk45',
+                 dt=2e-4, workload='ns-hex64-p3-rk45-fp64'),
+    'cfg0': dict(system='euler', n=8, order=2, scheme='rk4', dt=1e-3,
+                 workload='euler-hex8-p2-rk4-fp64'),
+}
+
+
+def make_cfg(c, precision='double'):
+    from tests.fr_cases import make_cfg as mk
+    return mk(c['system'], c['order'], 'rusanov', precision=precision,
+              scheme=c['scheme'], dt=c['dt'])
+
+
+# -- algorithmic bytes per launch (every input read once, every output
+#    written once, read-modify-write counted twice)
+def kernel_bytes(k, isz, nvars=5):
+    name = k.name
+    if name == 'mul':
+        a, b, c = k.mats
+        beta = k.rtargs.get('beta', 0.0)
+        return (b.nrow + c.nrow*(1 + (beta != 0)))*b.ncol*isz + a.nrow*a.ncol*0
+    if name == 'tflux':
+        u, f, geo = k.mats
+        return (u.nrow*u.ncol + f.nrow*f.ncol + geo.nrow*geo.ncol)*isz
+    if name == 'tflux_ns':
+        u, f, g, geo = k.mats
+        return (u.nrow*u.ncol + f.nrow*f.ncol + 2*g.nrow*g.ncol +
+                geo.nrow*geo.ncol)*isz
+    if name == 'negdivconf':
+        t, r = k.mats
+        return (2*t.nrow*t.ncol + r.nrow*r.ncol)*isz
+    if name == 'intcflux':
+        ul = k.views[0]
+        return ul.n*(4*nvars*isz + 3*isz + 2*4)
+    if name == 'intcflux_ns':
+        ul = k.views[0]
+        return ul.n*(4*nvars*isz + 2*3*nvars*isz + 3*isz + 4*4)
+    if name == 'intconu':
+        ul = k.views[0]
+        return ul.n*(4*nvars*isz + 4*4)
+    if name == 'rkvdh2':
+        r1, r2 = k.mats
+        return 4*r1.nrow*r1.ncol*isz
+    if name == 'axnpby':
+        x = k.mats
+        return (len(x) + 1)*x[0].nrow*x[0].ncol*isz
+    return 0
+
+
+def flatten(kerns):
+    out = []
+    for k in kerns:
+        if hasattr(k, 'kernels'):
+            out += flatten(k.kernels)
+        else:
+            out.append(k)
+    return out
+
+
+def time_kernels(be, kerns, reps):
+    """Mean duration (ms) of each kernel over reps launches, with HIP events
+    recorded on the stream the kernels run on."""
+    lib = be.lib
+    e0, e1 = ct.c_void_p(), ct.c_void_p()
+    lib.event_create(ct.byref(e0))
+    lib.event_create(ct.byref(e1))
+    out = []
+    for k in kerns:
+        k.run(be.stream)
+        lib.event_record(e0, be.stream)
+        for _ in range(reps):
+            k.run(be.stream)
+        lib.event_record(e1, be.stream)
+        lib.event_sync(e1)
+        ms = ct.c_float()
+        lib.event_elapsed(ct.byref(ms), e0, e1)
+        out.append(ms.value/reps)
+    lib.event_destroy(e0)
+    lib.event_destroy(e1)
+    return out
+
+
+def cpu_baseline(c, budget_s=12.0):
+    """The oracle (NumPy, one process) on a bounded sample of the same
+    workload: same system, order and field on a 12^3 box."""
+    os.environ.setdefault('OMP_NUM_THREADS', '1')
+    from oracle.rhs import OracleHexSystem
+    from tests.fr_cases import oracle_consts
+    n = 12 if c['order'] >= 3 else 8
+    orc = OracleHexSystem(n, c['order'], c['system'],
+                          constants=oracle_consts(c['system']))
+    u = orc.ics()
+    orc.rhs(u)
+    t0, nrhs = time.perf_counter(), 0
+    while time.perf_counter() - t0 < budget_s:
+        orc.rhs(u)
+        nrhs += 1
+    dt = time.perf_counter() - t0
+    return {'value': orc.ndofs*nrhs/dt/1e9, 'unit': 'GDOF*RHS/s',
+            'cores': 1, 'kind': 'port',
+            'sample': f'oracle NumPy FP64 {c["system"]} RHS, {n}^3 hex '
+                      f'p={c["order"]} ({orc.ndofs} DOF), {nrhs} RHS in '
+                      f'{dt:.1f} s'}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=20)
+    ap.add_argument('--warmup', type=int, default=3)
+    ap.add_argument('--config', default='cfg1', choices=sorted(CONFIGS))
+    ap.add_argument('--precision', default='double')
+    ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--kernel-reps', type=int, default=5)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) > 1:
+        print('bench.py measures one GPU only: this build has no multi-GPU '
+              'path (the domain-decomposed design is described in '
+              'DESIGN.md).', file=sys.stderr)
+        sys.exit(2)
+
+    from pyfr_amd.backend import AMDBackend
+    from pyfr_amd.mesh import BoxMesh
+    from pyfr_amd.solvers import get_integrator
+
+    c = CONFIGS[args.config]
+    cfg = make_cfg(c, args.precision)
+    t0 = time.perf_counter()
+    be = AMDBackend(cfg)
+    integ = get_integrator(be, BoxMesh(c['n']), cfg)
+    sysm = integ.system
+    tsetup = time.perf_counter() - t0
+    print(f'# setup {tsetup:.1f} s, {sysm.gndofs} DOF', file=sys.stderr,
+          flush=True)
+
+    dt = c['dt']
+    integ.run_steps(args.warmup, dt)
+    be.wait()
+    t0 = time.perf_counter()
+    integ.run_steps(args.steps, dt)
+    be.wait()
+    t1 = time.perf_counter()
+    nrhs = integ.nrhs_per_step*args.steps
+    gdofs = sysm.gndofs*nrhs/(t1 - t0)/1e9
+    ms_per_step = (t1 - t0)*1e3/args.steps
+    print(f'# {args.steps} steps in {t1 - t0:.3f} s: {gdofs:.3f} GDOF*RHS/s',
+          file=sys.stderr, flush=True)
+
+    # Per-kernel times inside one RHS (+ stage update)
+    g = sysm._rhs_graphs(0, 1)[0]
+    kerns = flatten(g.klist)
+    if c['scheme'] == 'rk45':
+        rk = integ._rkvdh2_kerns(0, 0, 1)
+        for k in rk:
+            k.bind(dt=dt)
+        kerns += rk
+    isz = np.dtype(be.fpdtype).itemsize
+    times = time_kernels(be, kerns, args.kernel_reps)
+    per = {}
+    for k, ms in zip(kerns, times):
+        nm = k.name
+        if nm == 'mul':
+            nm = f'mul{k.mats[0].nrow}x{k.mats[0].ncol}'
+        d = per.setdefault(nm, {'ms': 0.0, 'bytes': 0, 'launches': 0})
+        d['ms'] += ms
+        d['bytes'] += kernel_bytes(k, isz)
+        d['launches'] += 1
+    for nm, d in per.items():
+        d['GBs'] = d['bytes']/(d['ms']*1e-3)/1e9 if d['ms'] > 0 else 0.0
+        d['frac'] = d['GBs']/HBM_PEAK_GBS
+        print(f'#   {nm:14s} {d["ms"]:8.3f} ms/RHS  {d["GBs"]:8.1f} GB/s '
+              f'({100*d["frac"]:5.1f}% of peak)', file=sys.stderr)
+    dom = max(per, key=lambda n: per[n]['ms'])
+    dd = per[dom]
+    avg_ms = dd['ms']/dd['launches']
+    roof = {'bound': 'hbm', 'kernel': dom,
+            'achieved': round(dd['GBs'], 1), 'peak': HBM_PEAK_GBS,
+            'unit': 'GB/s', 'frac': round(dd['frac'], 4),
+            'traffic': None, 'bytes_per_launch': dd['bytes']//dd['launches'],
+            'avg_launch_ms': round(avg_ms, 4)}
+    tot_ms = sum(d['ms'] for d in per.values())
+    tot_b = sum(d['bytes'] for d in per.values())
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(c)
+
+    dtype = 'f64' if be.fpdtype == np.float64 else 'f32'
+    line = {
+        'metric': 'GDOF*RHS/s', 'value': round(gdofs, 4),
+        'unit': 'GDOF*RHS/s', 'n_gpus': 1, 'steps': args.steps,
+        'warmup': args.warmup, 'ms_per_step': round(ms_per_step, 3),
+        'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': dtype, 'data': 'synthetic (analytic Taylor-Green field)',
+        'config': {'workload': c['workload'], 'mesh': f'{c["n"]}^3 hex',
+                   'order': c['order'], 'dof': sysm.gndofs,
+                   'scheme': c['scheme'], 'dt': dt,
+                   'rhs_per_step': integ.nrhs_per_step},
+        'roofline': roof,
+        'rhs_kernels': {n: {k: (round(v, 4) if isinstance(v, float) else v)
+                            for k, v in d.items()} for n, d in per.items()},
+        'rhs_alg_bytes_per_dof': round(tot_b/sysm.gndofs, 1),
+        'rhs_kernel_ms_sum': round(tot_ms, 3),
+        'cpu_baseline': cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == '__main__':
+    main()

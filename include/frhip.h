@@ -67,7 +67,7 @@ int frhip_graph_destroy(void *graph_exec);
  *      HIPRocBLASKernels.mul (pyfr/backends/hip/rocblas.py:93-192):
  *      C(m, ncol) = alpha * A(m, k) @ B(k, ncol) + beta * C.
  * frhip_mul_plan_create takes the dense row-major host operator A.  If its
- * sparsity pattern matches a compiled structured kernel (hex tensor-product
+ * sparsity pattern and values match a compiled structured kernel (hex tensor-product
  * operators), that kernel is used with A's nonzeros uploaded; otherwise a
  * dense kernel is used.  *kind receives 1 (structured) or 0 (dense). */
 int frhip_mul_plan_create(void **plan, int m, int k, const double *a_host,

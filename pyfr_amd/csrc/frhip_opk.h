@@ -36,6 +36,7 @@ struct OpKernelEntry {
     int m, k, nnz;
     unsigned long long hash;
     const int32_t *lin;      // row-major linear indices of the nonzeros
+    const double *vals;      // the baked nonzero values, same order
     const char *desc;
     OpKD fd[4];
     OpKF ff[4];

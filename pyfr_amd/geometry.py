@@ -93,28 +93,32 @@ class HexGeometry:
         ploc = op @ self.eles.reshape(self.nspts, -1)
         return ploc.reshape(len(pts), -1, self.ndims).swapaxes(1, 2)
 
-    def smats_djacs_mpts(self):
+    def smats_djacs_mpts(self, chunk=16384):
         if self._smats_djacs is not None:
             return self._smats_djacs
         mb = self.mbasis
         mpts, nm, ne = mb.pts, len(mb.pts), self.neles
         x = self.ploc_at(mpts)                               # (nm, 3, E)
-        jacop = mb.jac_nodal_basis_at(mpts)                  # (3, nm, nm)
-        # jac[a][b] = dX_b/dxi_a at the pseudo grid points, (3, 3, nm, E)
-        jac = np.einsum('apq,qbe->abpe', jacop, x)
-        dtt = []
-        for a in range(3):
-            tt = np.cross(x, jac[a].transpose(1, 0, 2), axisa=1, axisb=1,
-                          axisc=1)                           # (nm, 3, E)
-            # derivative along each axis c of (X x X_a): (c, nm, comp, E)
-            dtt.append(np.einsum('cpq,qke->cpke', jacop, tt))
+        # jacop[(a, p), q] = d l_q / d xi_a at pseudo grid point p
+        jacop = mb.jac_nodal_basis_at(mpts).reshape(3*nm, nm)
         smats = np.empty((3, nm, 3, ne))
-        smats[0] = 0.5*(dtt[2][1] - dtt[1][2])
-        smats[1] = 0.5*(dtt[0][2] - dtt[2][0])
-        smats[2] = 0.5*(dtt[1][0] - dtt[0][1])
-        x0, x1, x2 = jac
-        x1cx2 = np.cross(x1, x2, axisa=0, axisb=0, axisc=0)
-        djacs = np.einsum('bpe,bpe->pe', x0, x1cx2)
+        djacs = np.empty((nm, ne))
+        for s0 in range(0, ne, chunk):
+            s1 = min(ne, s0 + chunk)
+            xc = np.ascontiguousarray(x[:, :, s0:s1])
+            n = s1 - s0
+            # jac[a, p, b, e] = dX_b/dxi_a
+            jac = (jacop @ xc.reshape(nm, -1)).reshape(3, nm, 3, n)
+            dtt = []
+            for a in range(3):
+                tt = np.cross(xc, jac[a], axisa=1, axisb=1, axisc=1)
+                # dtt[a][c, p, k, e]: derivative along c of (X x X_a)_k
+                dtt.append((jacop @ tt.reshape(nm, -1)).reshape(3, nm, 3, n))
+            smats[0, :, :, s0:s1] = 0.5*(dtt[2][1] - dtt[1][2])
+            smats[1, :, :, s0:s1] = 0.5*(dtt[0][2] - dtt[2][0])
+            smats[2, :, :, s0:s1] = 0.5*(dtt[1][0] - dtt[0][1])
+            x1cx2 = np.cross(jac[1], jac[2], axisa=1, axisb=1, axisc=1)
+            djacs[:, s0:s1] = np.einsum('pbe,pbe->pe', jac[0], x1cx2)
         self._smats_djacs = (smats, djacs)
         return smats, djacs
 
@@ -122,7 +126,9 @@ class HexGeometry:
         """(3, npts, 3, neles): S[i][k] at each point."""
         smats, _ = self.smats_djacs_mpts()
         m0 = self.mbasis.nodal_basis_at(pts)
-        return np.einsum('pm,imke->ipke', m0, smats)
+        nm, ne = smats.shape[1], smats.shape[3]
+        return np.stack([(m0 @ smats[i].reshape(nm, -1)).reshape(-1, 3, ne)
+                         for i in range(3)])
 
     def rcpdjac_at(self, pts):
         _, djacs = self.smats_djacs_mpts()
@@ -133,8 +139,10 @@ class HexGeometry:
 
     def pnorm_at(self, pts, norms):
         """Physical normals (npts, neles, 3)."""
-        s = self.smat_at(pts)
-        pn = np.einsum('ipke,pi->pek', s, norms)
+        s = self.smat_at(pts)                             # (3, np, 3, E)
+        pn = np.zeros((len(pts), self.neles, 3))
+        for i in range(3):
+            pn += (norms[:, i, None, None]*s[i]).transpose(0, 2, 1)
         if np.any(np.sqrt(np.einsum('...i,...i', pn, pn)) < 1e-10):
             raise RuntimeError('Zero face normals detected')
         return pn

@@ -20,6 +20,8 @@ the first face coordinate fastest, and the operator layouts are those of
 """
 from functools import cached_property
 import itertools as it
+import math
+import re
 
 import numpy as np
 
@@ -38,42 +40,108 @@ HEX_FACES = [
 ]
 
 
+def _legendre(n, x):
+    """P_n(x) and P_n'(x) by the three-term recurrence (pure Python)."""
+    p0, p1 = 1.0, x
+    if n == 0:
+        return 1.0, 0.0
+    for k in range(2, n + 1):
+        p0, p1 = p1, ((2*k - 1)*x*p1 - (k - 1)*p0)/k
+    dp = n*(x*p1 - p0)/(x*x - 1.0)
+    return p1, dp
+
+
 def gauss_legendre(n):
-    x, w = np.polynomial.legendre.leggauss(n)
-    return x, w
+    """n-point Gauss-Legendre nodes (ascending) and weights.
+
+    Newton iteration on P_n in plain Python floats from cosine initial
+    guesses, symmetrised, so that the result is bit-identical on every host
+    (the operator kernels bake these values in at build time and match them
+    bit for bit at run time)."""
+    x = [0.0]*n
+    w = [0.0]*n
+    for i in range((n + 1)//2):
+        z = math.cos(math.pi*(i + 0.75)/(n + 0.5))
+        for _ in range(100):
+            p, dp = _legendre(n, z)
+            dz = p/dp
+            z -= dz
+            if abs(dz) < 1e-16:
+                break
+        p, dp = _legendre(n, z)
+        wi = 2.0/((1.0 - z*z)*dp*dp)
+        x[i], x[n - 1 - i] = -z, z
+        w[i] = w[n - 1 - i] = wi
+    if n % 2:
+        x[n//2] = 0.0
+    return np.array(x), np.array(w)
 
 
 def lagrange_at(x, pts):
     """Values l_j(pts[q]) of the Lagrange basis through x: (len(pts), n)."""
     n = len(x)
+    x = [float(v) for v in x]
     out = np.ones((len(pts), n))
-    for j in range(n):
-        for m in range(n):
-            if m != j:
-                out[:, j] *= (pts - x[m])/(x[j] - x[m])
+    for q, t in enumerate(pts):
+        for j in range(n):
+            v = 1.0
+            for m in range(n):
+                if m != j:
+                    v *= (float(t) - x[m])/(x[j] - x[m])
+            out[q, j] = v
     return out
 
 
 def diff_matrix(x):
     """D[i, j] = l_j'(x_i) via barycentric weights."""
     n = len(x)
-    wb = np.array([1/np.prod([x[j] - x[m] for m in range(n) if m != j])
-                   for j in range(n)])
+    x = [float(v) for v in x]
+    wb = []
+    for j in range(n):
+        p = 1.0
+        for m in range(n):
+            if m != j:
+                p *= x[j] - x[m]
+        wb.append(1.0/p)
     D = np.zeros((n, n))
     for i in range(n):
         for j in range(n):
             if i != j:
                 D[i, j] = wb[j]/wb[i]/(x[i] - x[j])
-        D[i, i] = -D[i].sum()
+        D[i, i] = -math.fsum(D[i, j] for j in range(n) if j != i)
     return D
 
 
 def legendre_kernel_end(x, side):
     """h(x_i) = sum_{k<=p} (k + 1/2) side^k P_k(x_i) for n = len(x) points."""
     n = len(x)
-    P = np.polynomial.legendre.legvander(x, n - 1)      # (n, n): P_k(x_i)
-    coef = np.array([(k + 0.5)*side**k for k in range(n)])
-    return P @ coef
+    out = np.empty(n)
+    for i, xi in enumerate(x):
+        terms, p0, p1 = [], 1.0, float(xi)
+        for k in range(n):
+            pk = 1.0 if k == 0 else (float(xi) if k == 1 else None)
+            if k >= 2:
+                p0, p1 = p1, ((2*k - 1)*float(xi)*p1 - (k - 1)*p0)/k
+                pk = p1
+            terms.append((k + 0.5)*(side**k)*pk)
+        out[i] = math.fsum(terms)
+    return out
+
+
+def sparse_matmul(a, b):
+    """a @ b with each entry the correctly rounded (math.fsum) sum of its
+    nonzero products: bit-reproducible on any host, unlike BLAS."""
+    out = np.zeros((a.shape[0], b.shape[1]))
+    bnz = [np.flatnonzero(b[k]) for k in range(b.shape[0])]
+    for i in range(a.shape[0]):
+        acc = {}
+        for k in np.flatnonzero(a[i]):
+            aik = float(a[i, k])
+            for j in bnz[k]:
+                acc.setdefault(j, []).append(aik*float(b[k, j]))
+        for j, terms in acc.items():
+            out[i, j] = math.fsum(terms)
+    return out
 
 
 def flush(a, tol=1e-10):
@@ -221,13 +289,12 @@ class HexShape:
     def opmat(self, expr):
         """Evaluate an operator expression ('M0', 'M1 - M3*M2', ...) the way
         ``BaseShape.opmat`` (``pyfr/shapes.py:79-87``) does, flushing
-        |x| < 1e-10 to zero."""
-        import re
-        e = expr.lower().replace('*', '@')
-        if not re.match(r'[m0-9\-+@() ]+$', e):
+        |x| < 1e-10 to zero.  Products are evaluated with correctly rounded
+        sums so the result is the same on every host."""
+        e = expr.lower().replace(' ', '')
+        if not re.match(r'[m0-9\-+*()]+$', e):
             raise ValueError('Invalid operator matrix expression')
-        mats = {m: getattr(self, m) for m in re.findall(r'm\d+', e)}
-        return flush(eval(e, {'__builtins__': None}, mats))
+        return flush(_OpExpr(self, e).parse())
 
     # -- linear element geometry helpers
     @staticmethod
@@ -235,6 +302,53 @@ class HexShape:
         """Vertices of the reference hex (``TensorProdShape.std_ele(1)``)."""
         pts1d = np.linspace(-1, 1, 2)
         return np.array([p[::-1] for p in it.product(pts1d, repeat=3)])
+
+
+class _OpExpr:
+    """Recursive-descent evaluator for operator expressions over m0..m9 with
+    +, -, * (matrix product) and parentheses."""
+
+    def __init__(self, shape, s):
+        self.shape, self.s, self.i = shape, s, 0
+
+    def parse(self):
+        v = self._sum()
+        if self.i != len(self.s):
+            raise ValueError('Invalid operator matrix expression')
+        return v
+
+    def _peek(self):
+        return self.s[self.i] if self.i < len(self.s) else ''
+
+    def _sum(self):
+        v = self._prod()
+        while self._peek() in ('+', '-'):
+            op = self.s[self.i]
+            self.i += 1
+            r = self._prod()
+            v = v + r if op == '+' else v - r
+        return v
+
+    def _prod(self):
+        v = self._atom()
+        while self._peek() == '*':
+            self.i += 1
+            v = sparse_matmul(v, self._atom())
+        return v
+
+    def _atom(self):
+        if self._peek() == '(':
+            self.i += 1
+            v = self._sum()
+            if self._peek() != ')':
+                raise ValueError('Invalid operator matrix expression')
+            self.i += 1
+            return v
+        m = re.match(r'm(\d+)', self.s[self.i:])
+        if not m:
+            raise ValueError('Invalid operator matrix expression')
+        self.i += len(m.group(0))
+        return np.array(getattr(self.shape, f'm{m.group(1)}'), dtype=float)
 
 
 SHAPES = {'hex': HexShape}
