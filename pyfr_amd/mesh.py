@@ -18,11 +18,8 @@ lexicographic sort of flux points (``pyfr/nputil.py:120-181``) used by
 """
 import numpy as np
 
-# Vertices of each hex face, in the order of HexShape.faces
-HEX_FACE_VERTS = [
-    (0, 1, 2, 3), (0, 1, 4, 5), (1, 3, 5, 7),
-    (2, 3, 6, 7), (0, 2, 4, 6), (4, 5, 6, 7),
-]
+# (normal axis, side) of each hex face, in the order of HexShape.faces
+HEX_FACE_AXES = [(2, -1), (1, -1), (0, +1), (1, +1), (0, -1), (2, +1)]
 
 
 class BoxMesh:
@@ -38,25 +35,29 @@ class BoxMesh:
         e = np.arange(E)
         ii, jj, kk = e % nx, (e // nx) % ny, e // (nx*ny)
         verts = np.empty((8, E, 3))
-        vids = np.empty((8, E), dtype=np.int64)
         for v in range(8):
             bx, by, bz = v & 1, (v >> 1) & 1, (v >> 2) & 1
             verts[v, :, 0] = origin + (ii + bx)*hx
             verts[v, :, 1] = origin + (jj + by)*hy
             verts[v, :, 2] = origin + (kk + bz)*hz
-            vids[v] = (((ii + bx) % nx) + nx*(((jj + by) % ny) +
-                                              ny*((kk + bz) % nz)))
         self.spts = {'hex': verts}
         self.etypes = ['hex']
         self.neles = E
-        self.con = self._connect(vids)
+        self.con = self._connect(np.stack([ii, jj, kk]))
 
-    def _connect(self, vids):
-        E = vids.shape[1]
-        nf = len(HEX_FACE_VERTS)
-        # Face keys: sorted periodic vertex ids; face-major flattening
-        keys = np.stack([np.sort(vids[list(fv)], axis=0).T
-                         for fv in HEX_FACE_VERTS])          # (6, E, 4)
+    def _connect(self, ijk):
+        """Pair faces through a structured key: (normal axis, periodic plane
+        index, the element's indices along the two other axes)."""
+        E = ijk.shape[1]
+        nf = len(HEX_FACE_AXES)
+        dims = np.array(self.shape)
+        keys = np.empty((nf, E, 4), dtype=np.int64)
+        for f, (nax, side) in enumerate(HEX_FACE_AXES):
+            oth = [a for a in range(3) if a != nax]
+            keys[f, :, 0] = nax
+            keys[f, :, 1] = (ijk[nax] + (side > 0)) % dims[nax]
+            keys[f, :, 2] = ijk[oth[0]]
+            keys[f, :, 3] = ijk[oth[1]]
         keys = keys.reshape(nf*E, 4)
         fidx = np.repeat(np.arange(nf), E)
         eidx = np.tile(np.arange(E), nf)

@@ -1,0 +1,241 @@
+"""Pinning the oracle (CPU).
+
+What pins each oracle function:
+* operators M0..M3 (oracle.basis)   -- the reference's golden fixture
+  hex-gleg-ord3.npz (its only test data for the path), plus polynomial
+  exactness and the quadrature identity of the correction matrix at p=1..5;
+* GL points                         -- values parsed from the reference's rule
+  tables (tests/golden/gl_rules.npz);
+* clean()                           -- a crafted array with known output;
+* point physics (oracle.physics)    -- closed forms (flux of a state, simple
+  shear stress), Riemann consistency and conservation, HLLC upwinding;
+* the RHS (oracle.rhs)              -- free-stream preservation, discrete
+  conservation and the convergence order of an exact travelling density
+  wave (catches sign, index, metric and correction-scaling mistakes);
+* RK steppers                       -- order of accuracy on y' = lambda*y;
+* controller                        -- step count / final time for cfg0;
+* pairing (oracle.mesh)             -- every face used exactly once and
+  paired flux points coincide, on 1^3..4^3 boxes.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle.basis import (HexShapeOracle, clean, gauss_legendre_hex,
+                          gauss_legendre_line, gauss_legendre_quad)
+from oracle.geom import trilinear_ploc
+from oracle.mesh import box_hex_verts, pair_faces_bruteforce
+from oracle.physics import hllc, inviscid_flux, rusanov, viscous_flux_add
+from oracle.rhs import (OracleHexSystem, advance, clamp_dt, rk4_step,
+                        rk45_step)
+
+GOLD = os.path.join(os.path.dirname(__file__), 'golden')
+
+
+def test_fixture_hex_gleg_ord3():
+    ref = np.load(os.path.join(GOLD, 'hex_gleg_ord3.npz'))
+    h = HexShapeOracle(3)
+    for k in ('m0', 'm1', 'm2', 'm3'):
+        a, b = getattr(h, k), ref[k]
+        assert a.shape == b.shape
+        assert np.abs(a - b).max() < 1e-13*max(1, np.abs(b).max())
+
+
+def test_gl_rules_against_reference_tables():
+    g = np.load(os.path.join(GOLD, 'gl_rules.npz'))
+    for n in range(1, 7):
+        x, w = gauss_legendre_line(n)
+        assert np.abs(x - g[f'line_n{n}_pts'][:, 0]).max() <= 4e-16
+        assert np.abs(w - g[f'line_n{n}_wts']).max() <= 1e-15
+        assert np.abs(gauss_legendre_quad(n)[0] -
+                      g[f'quad_n{n}_pts']).max() <= 4e-16
+        assert np.abs(gauss_legendre_hex(n)[0] -
+                      g[f'hex_n{n}_pts']).max() <= 4e-16
+
+
+def test_clean():
+    a = np.array([1e-11, -0.5, 0.5 + 1e-12, 0.5 - 2e-12, 0.7, -1e-12])
+    c = clean(a)
+    assert c[0] == 0 and c[5] == 0
+    assert c[1] == -c[2] == -c[3] == -np.median([0.5, 0.5 + 1e-12,
+                                                  0.5 - 2e-12])
+    assert c[4] == 0.7
+
+
+def _poly(pts, ex):
+    x, y, z = pts.T
+    return x**ex[0]*y**ex[1]*z**ex[2]
+
+
+@pytest.mark.parametrize('order', [1, 2, 3, 4, 5])
+def test_operator_exactness(order):
+    h = HexShapeOracle(order)
+    rng = np.random.default_rng(order)
+    for _ in range(6):
+        ex = rng.integers(0, order + 1, 3)
+        u = _poly(h.upts, ex)
+        # M0 interpolates degree-p polynomials exactly
+        assert np.allclose(h.m0 @ u, _poly(h.fpts, ex), atol=1e-12)
+        # M1 (through its re-layout M4) differentiates them exactly
+        g = h.m4 @ u
+        for d in range(3):
+            exd = ex.copy()
+            c = exd[d]
+            exd[d] = max(c - 1, 0)
+            ref = c*_poly(h.upts, exd) if c > 0 else 0*u
+            assert np.allclose(g[d*h.nupts:(d + 1)*h.nupts], ref, atol=1e-11)
+    # Integral of each correction function equals its face weight:
+    # sum_u w_u M3[u, f] = w_f  (g_f lifts the face delta; shapes.py:231-259)
+    _, fw = gauss_legendre_quad(order + 1)
+    fwts = np.tile(fw, 6)
+    assert np.allclose(h.upts_wts @ h.m3, fwts, atol=1e-13)
+
+
+def test_inviscid_flux_closed_form():
+    gamma = 1.4
+    rho, u, v, w, p = 1.3, 0.4, -0.2, 0.7, 2.1
+    E = p/(gamma - 1) + 0.5*rho*(u*u + v*v + w*w)
+    s = np.array([rho, rho*u, rho*v, rho*w, E])
+    f, pp, vv = inviscid_flux(s, gamma)
+    vel = np.array([u, v, w])
+    assert np.isclose(pp, p)
+    for i in range(3):
+        assert np.isclose(f[i, 0], rho*vel[i])
+        for j in range(3):
+            assert np.isclose(f[i, j + 1], rho*vel[i]*vel[j] + p*(i == j))
+        assert np.isclose(f[i, 4], (E + p)*vel[i])
+
+
+def _rand_states(rng, n):
+    rho = rng.uniform(0.5, 2, n)
+    vel = rng.uniform(-1, 1, (3, n))
+    p = rng.uniform(0.5, 3, n)
+    E = p/0.4 + 0.5*rho*(vel**2).sum(0)
+    return np.vstack([rho, rho*vel, E])
+
+
+@pytest.mark.parametrize('rs', [rusanov, hllc])
+def test_riemann_consistency_and_conservation(rs):
+    rng = np.random.default_rng(3)
+    ul, ur = _rand_states(rng, 50), _rand_states(rng, 50)
+    n = rng.standard_normal((3, 50))
+    n /= np.linalg.norm(n, axis=0)
+    f, _, _ = inviscid_flux(ul, 1.4)
+    fn = np.einsum('dvn,dn->vn', f, n)
+    assert np.allclose(rs(ul, ul, n, 1.4), fn, rtol=1e-13, atol=1e-13)
+    assert np.allclose(rs(ul, ur, n, 1.4), -rs(ur, ul, -n, 1.4),
+                       rtol=1e-12, atol=1e-12)
+
+
+def test_hllc_supersonic_upwind():
+    # Both states moving at Mach ~3 along +n: the flux is the left flux
+    ul = np.array([[1.0], [3.5], [0.0], [0.0], [1/0.4 + 0.5*3.5**2]])
+    ur = np.array([[1.1], [3.8], [0.1], [0.0], [1.1/0.4 + 0.5*(3.8**2 + 0.01)/1.1]])
+    n = np.array([[1.0], [0.0], [0.0]])
+    f, _, _ = inviscid_flux(ul, 1.4)
+    assert np.allclose(hllc(ul, ur, n, 1.4), f[0], rtol=1e-14)
+
+
+def test_viscous_flux_simple_shear():
+    mu, s = 0.01, 0.3
+    c = {'gamma': 1.4, 'mu': mu, 'Pr': 0.71}
+    u = np.array([1.0, 0.2, 0.0, 0.0, 2.5])
+    g = np.zeros((3, 5))
+    g[1, 1] = s                     # d(rho u)/dy with rho = 1: du/dy = s
+    f = viscous_flux_add(u, g, np.zeros((3, 5)), c)
+    # negated stress: f_y(x-mom) = f_x(y-mom) = -mu du/dy.  Energy: work
+    # u*t_xy plus heat flux -mu*gamma/Pr*dT/dy where, with E and rho fixed,
+    # c_v dT/dy = -u du/dy.
+    assert np.isclose(f[1, 1], -mu*s) and np.isclose(f[0, 2], -mu*s)
+    assert np.isclose(f[1, 4], 0.2*(-mu*s) + mu*(1.4/0.71)*(0.2*s))
+    assert np.allclose(f[:, 0], 0)
+
+
+UNIF = (1.0, 0.3, -0.2, 0.1, 2.5/0.4 + 0.5*(0.09 + 0.04 + 0.01))
+
+
+@pytest.mark.parametrize('system,order', [('euler', 1), ('euler', 2),
+                                          ('euler', 3),
+                                          ('navier-stokes', 2)])
+def test_freestream_and_conservation(system, order):
+    c = {'gamma': 1.4} if system == 'euler' else \
+        {'gamma': 1.4, 'mu': 0.01, 'Pr': 0.71, 'ldg-beta': 0.0,
+         'ldg-tau': 0.0}
+    warp = lambda x, y, z: (x + 0.1*np.sin(y)*np.sin(z),
+                            y + 0.1*np.sin(x)*np.sin(z), z)
+    verts = box_hex_verts(3, warp=warp)
+    o = OracleHexSystem(3, order, system, constants=c, verts=verts)
+    u = np.empty((o.nupts, 5, o.neles))
+    u[:] = np.array(UNIF)[None, :, None]
+    assert np.abs(o.rhs(u)).max() < 1e-11
+    r = o.rhs(o.ics())
+    w = o.shape.upts_wts
+    tot = [(w[:, None]*r[:, v]/o.rcpdjac).sum() for v in range(5)]
+    assert np.abs(tot).max() < 1e-10
+
+
+@pytest.mark.parametrize('order', [1, 2, 3])
+def test_rhs_convergence_density_wave(order):
+    """Euler exact solution rho = 1 + 0.2 sin(x + y + z), velocity (U, V, W)
+    and pressure uniform: d(rho)/dt = -(U + V + W) rho'(.)"""
+    U = np.array([0.5, 0.3, -0.2])
+    errs = []
+    for n in (4, 8):
+        o = OracleHexSystem(n, order)
+        x, y, z = trilinear_ploc(o.verts, o.shape.upts).transpose(1, 0, 2)
+        rho = 1 + 0.2*np.sin(x + y + z)
+        p = 1.0
+        u = np.stack([rho, rho*U[0], rho*U[1], rho*U[2],
+                      p/0.4 + 0.5*rho*(U**2).sum()], axis=1)
+        r = o.rhs(u)
+        exact = -U.sum()*0.2*np.cos(x + y + z)
+        errs.append(np.sqrt(np.mean((r[:, 0] - exact)**2)))
+    # The RHS is a derivative of a degree-p interpolant: error O(h^p)
+    # (measured 1.4 / 2.0 / 3.6 for p = 1 / 2 / 3).
+    rate = np.log2(errs[0]/errs[1])
+    assert rate > order - 0.7, (errs, rate)
+
+
+def test_rk_orders():
+    lam = -1.0 + 0.5j
+
+    def f(y):
+        return lam*y
+    for step, order in ((rk4_step, 4), (rk45_step, 4)):
+        errs = []
+        for dt in (0.1, 0.05):
+            y = np.array([1.0 + 0j])
+            for _ in range(int(round(1/dt))):
+                y = step(f, y, dt)
+            errs.append(abs(y[0] - np.exp(lam)))
+        assert np.log2(errs[0]/errs[1]) > order - 0.3
+
+
+def test_controller_cfg0_steps():
+    # The reference's clamp (-(-r // dt) and a 10-step look-ahead) takes
+    # 101 steps to reach t=0.1 with dt=1e-3; the last ten are 9e-4 each.
+    t, comp, n, dts = 0.0, 0.0, 0, []
+    while t < 0.1:
+        h = clamp_dt(1e-3, 0.1, t)
+        y = h - comp
+        tt = t + y
+        comp = (tt - t) - y
+        t = tt
+        n += 1
+        dts.append(h)
+    assert n == 101 and t == 0.1
+    assert np.allclose(dts[-10:], 9e-4, rtol=1e-12)
+    _, ns = advance(lambda u: 0*u, np.zeros(1), 1e-3, 0.1)
+    assert ns == 101
+
+
+@pytest.mark.parametrize('n', [1, 2, 3, 4])
+def test_pairing_bruteforce(n):
+    h = HexShapeOracle(2)
+    v = box_hex_verts(n)
+    pl = trilinear_ploc(v, h.fpts)
+    pairs = pair_faces_bruteforce(pl, 6, h.nfacefpts, 2*np.pi)
+    assert len(pairs) == 3*n**3
+    used = [(p[0], p[1]) for p in pairs] + [(p[3], p[4]) for p in pairs]
+    assert len(set(used)) == 6*n**3
