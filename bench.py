@@ -71,10 +71,23 @@ def kernel_bytes(k, isz, nvars=5):
     if name == 'rkvdh2':
         r1, r2 = k.mats
         return 4*r1.nrow*r1.ncol*isz
+    if name == 'rkvdh2_mixed':
+        acc, w = k.mats
+        return acc.nrow*acc.ncol*(2*8 + 2*4)
     if name == 'axnpby':
         x = k.mats
         return (len(x) + 1)*x[0].nrow*x[0].ncol*isz
     return 0
+
+
+def mixed_stage_kernel(be, integ, dt):
+    """The FP64/FP32 stage update of the precision form as a timed kernel."""
+    from pyfr_amd.backend import Kernel
+    acc, w = integ.system.ele_banks[0][0], integ.system.ele_banks[0][1]
+    a, b = dt*integ.a[0], dt*integ.b[0]
+    return Kernel(lambda stream: be.lib.rkvdh2_mixed(
+        acc.nrow, acc.ncol, acc.leaddim, acc.data, w.data, a, b, 0, stream),
+        mats=(acc, w), name='rkvdh2_mixed')
 
 
 def flatten(kerns):
@@ -133,15 +146,63 @@ def cpu_baseline(c, budget_s=12.0):
                       f'{dt:.1f} s'}
 
 
+def accuracy_report(c, checkpoints=(100, 500)):
+    """Per-field relative L2 and max errors of the mixed precision form
+    against the FP64 run from the same initial field, plus FP64
+    conservation sums of both."""
+    from pyfr_amd.backend import AMDBackend
+    from pyfr_amd.mesh import BoxMesh
+    from pyfr_amd.solvers import get_integrator
+    res = {}
+    for prec in ('double', 'mixed'):
+        cfg = make_cfg(c, prec)
+        be = AMDBackend(cfg)
+        integ = get_integrator(be, BoxMesh(c['n']), cfg)
+        res[prec] = {'sums0': integ.system.conservation_sums(0).tolist()}
+        done = 0
+        t0 = time.perf_counter()
+        for cp in checkpoints:
+            integ.run_steps(cp - done, c['dt'])
+            done = cp
+            res[prec][cp] = integ.soln[0]
+            res[prec][f'sums{cp}'] = integ.system.conservation_sums(
+                integ.idxcurr).tolist()
+            print(f'# {prec}: step {cp} at {time.perf_counter() - t0:.1f} s',
+                  file=sys.stderr, flush=True)
+        del integ, be
+    names = ['rho', 'rhou', 'rhov', 'rhow', 'E']
+    out = {'workload': c['workload'].replace('-fp64', '-mixed-vs-fp64'),
+           'checkpoints': {}}
+    for cp in checkpoints:
+        a, b = res['mixed'][cp], res['double'][cp]
+        d = {}
+        for v, nm in enumerate(names):
+            da, db = a[:, v], b[:, v]
+            d[nm] = {'rel_l2': float(np.linalg.norm(da - db) /
+                                     np.linalg.norm(db)),
+                     'max_abs': float(np.abs(da - db).max()),
+                     'ref_max_abs': float(np.abs(db).max())}
+        d['state_rel_l2'] = float(np.linalg.norm(a - b)/np.linalg.norm(b))
+        d['sums_fp64'] = res['double'][f'sums{cp}']
+        d['sums_mixed'] = res['mixed'][f'sums{cp}']
+        out['checkpoints'][cp] = d
+    out['sums_initial'] = res['double']['sums0']
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=20)
     ap.add_argument('--warmup', type=int, default=3)
     ap.add_argument('--config', default='cfg1', choices=sorted(CONFIGS))
-    ap.add_argument('--precision', default='double')
+    ap.add_argument('--precision', default='double',
+                    choices=['double', 'mixed', 'single'])
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--kernel-reps', type=int, default=5)
+    ap.add_argument('--accuracy', action='store_true',
+                    help='precision form: compare mixed against FP64 at '
+                    'steps 100 and 500 instead of timing')
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) > 1:
@@ -155,6 +216,8 @@ def main():
     from pyfr_amd.solvers import get_integrator
 
     c = CONFIGS[args.config]
+    if args.accuracy:
+        return accuracy_report(c)
     cfg = make_cfg(c, args.precision)
     t0 = time.perf_counter()
     be = AMDBackend(cfg)
@@ -178,13 +241,16 @@ def main():
           file=sys.stderr, flush=True)
 
     # Per-kernel times inside one RHS (+ stage update)
-    g = sysm._rhs_graphs(0, 1)[0]
+    g = sysm._rhs_graphs(*((1, 1) if args.precision == 'mixed'
+                           else (0, 1)))[0]
     kerns = flatten(g.klist)
-    if c['scheme'] == 'rk45':
+    if c['scheme'] == 'rk45' and args.precision != 'mixed':
         rk = integ._rkvdh2_kerns(0, 0, 1)
         for k in rk:
             k.bind(dt=dt)
         kerns += rk
+    elif args.precision == 'mixed':
+        kerns += [mixed_stage_kernel(be, integ, dt)]
     isz = np.dtype(be.fpdtype).itemsize
     times = time_kernels(be, kerns, args.kernel_reps)
     per = {}
@@ -216,14 +282,18 @@ def main():
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(c)
 
-    dtype = 'f64' if be.fpdtype == np.float64 else 'f32'
+    dtype = {'double': 'f64', 'single': 'f32',
+             'mixed': 'f32 RHS / f64 state'}[args.precision]
     line = {
         'metric': 'GDOF*RHS/s', 'value': round(gdofs, 4),
         'unit': 'GDOF*RHS/s', 'n_gpus': 1, 'steps': args.steps,
         'warmup': args.warmup, 'ms_per_step': round(ms_per_step, 3),
         'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
         'dtype': dtype, 'data': 'synthetic (analytic Taylor-Green field)',
-        'config': {'workload': c['workload'], 'mesh': f'{c["n"]}^3 hex',
+        'config': {'workload': c['workload'].replace(
+                       '-fp64', '-' + {'double': 'fp64', 'single': 'fp32',
+                                       'mixed': 'mixed'}[args.precision]),
+                   'mesh': f'{c["n"]}^3 hex',
                    'order': c['order'], 'dof': sysm.gndofs,
                    'scheme': c['scheme'], 'dt': dt,
                    'rhs_per_step': integ.nrhs_per_step},

@@ -139,10 +139,20 @@ int frhip_axnpby(int dtype, int nrow, long ncol, long ld, int nv,
 int frhip_rkvdh2(int dtype, int nrow, long ncol, long ld, void *r1, void *r2,
                  double dta, double dtb, int last, void *stream);
 
+/* Mixed-precision rkvdh2 stage (no counterpart kernel in the reference, which
+ * runs rkvdh2.mako in one precision): FP64 accumulated solution acc, FP32
+ * array kin holding the stage RHS on entry and the next stage input on exit.
+ * acc' = acc + dtb*k; kin' = (float)(acc + dta*k), or (float)acc' if last.
+ * Same (nrow, ld) for both arrays (element counts, not bytes). */
+int frhip_rkvdh2_mixed(int nrow, long ncol, long ld, void *acc, void *kin,
+                       double dta, double dtb, int last, void *stream);
+
 /* Reductions over an (nrow, nvar, neles) array: per variable sum of w*x
- * (w: (nrow, neles) weights, may be NULL) or max |x|; FP64 accumulation.
- * out: device buffer of nvar doubles.  Replaces HIPBlasExtKernels.reduction
- * (pyfr/backends/hip/blasext.py:76-136). op: 0 = sum, 1 = max-abs. */
+ * (w: (nrow, neles) weights, may be NULL), max |x|, or sum of w*x*x; FP64
+ * accumulation, deterministic two-pass order.  out: device buffer of nvar
+ * doubles.  Replaces HIPBlasExtKernels.reduction
+ * (pyfr/backends/hip/blasext.py:76-136). op: 0 = sum, 1 = max-abs,
+ * 2 = weighted sum of squares.  Allocates its scratch on first use. */
 int frhip_reduce(int dtype, int op, int nrow, int nvar, int neles, long ld,
                  const void *x, const void *w, long ldw, double *out,
                  void *stream);

@@ -57,6 +57,7 @@ _FUNCS = {
                           _L, _P, _L, _P, _P],
     'frhip_axnpby': [_I, _I, _L, _L, _I, _P, _P, _P],
     'frhip_rkvdh2': [_I, _I, _L, _L, _P, _P, _D, _D, _I, _P],
+    'frhip_rkvdh2_mixed': [_I, _L, _L, _P, _P, _D, _D, _I, _P],
     'frhip_reduce': [_I, _I, _I, _I, _I, _L, _P, _P, _L, _P, _P],
     'frhip_convert': [_I, _P, _I, _P, _L, _P],
 }

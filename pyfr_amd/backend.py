@@ -428,7 +428,12 @@ class MulProvider:
     def _plan(self, a, dtype):
         key = (id(a), dtype)
         if key not in self._plans:
-            host = np.ascontiguousarray(a.get(), dtype=np.float64)
+            # the exact FP64 operator, also for FP32 plans (the library
+            # rounds it to the plan's precision)
+            host = getattr(a, '_host', None)
+            if host is None:
+                host = a.get()
+            host = np.ascontiguousarray(host, dtype=np.float64)
             plan, kind = ct.c_void_p(), ct.c_int()
             self.backend.lib.mul_plan_create(
                 ct.byref(plan), host.shape[0], host.shape[1],
@@ -656,10 +661,14 @@ class AMDBackend:
     def __init__(self, cfg):
         self.cfg = cfg
         prec = cfg.get('backend', 'precision', 'double')
-        if prec not in {'single', 'double'}:
-            raise ValueError('Backend precision must be either single or '
-                             'double')
-        self.fpdtype = np.dtype(prec).type
+        if prec not in {'single', 'double', 'mixed'}:
+            raise ValueError('Backend precision must be single, double or '
+                             'mixed')
+        # 'mixed': FP32 RHS arithmetic (fpdtype) with FP64 time-level state
+        # (state_dtype), RK stage updates and reductions
+        self.precision = prec
+        self.fpdtype = np.float64 if prec == 'double' else np.float32
+        self.state_dtype = np.float32 if prec == 'single' else np.float64
         self.fpdtype_eps = float(np.finfo(self.fpdtype).eps)
         self.fpdtype_max = float(np.finfo(self.fpdtype).max)
         if cfg.get('backend', 'memory-model', 'normal') != 'normal':

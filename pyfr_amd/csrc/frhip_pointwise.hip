@@ -487,8 +487,27 @@ __global__ __launch_bounds__(256) void k_rkvdh2(long ncol, long ld,
     }
 }
 
+// Mixed-precision rkvdh2 stage: the accumulated solution stays FP64, the
+// RHS k and the next stage input live in one FP32 array (read k, then write
+// the next input in place).  acc' = acc + dtb*k; in = (float)(acc + dta*k),
+// or in = (float)acc' after the last stage.
+__global__ __launch_bounds__(256) void k_rkvdh2_mixed(long ncol, long ld,
+                                                      double *__restrict__ acc,
+                                                      float *__restrict__ kin,
+                                                      double dta, double dtb,
+                                                      int last)
+{
+    const long j = (long)blockIdx.x*blockDim.x + threadIdx.x;
+    if (j >= ncol) return;
+    const long idx = (long)blockIdx.y*ld + j;
+    const double t1 = acc[idx], t2 = (double)kin[idx];
+    const double na = t1 + dtb*t2;
+    acc[idx] = na;
+    kin[idx] = last ? (float)na : (float)(t1 + dta*t2);
+}
+
 // Per-variable partial reductions, FP64 accumulation: block (256), grid
-// (nblk, nvar).  op 0: sum of w*x, op 1: max |x|.
+// (nblk, nvar).  op 0: sum of w*x, op 1: max |x|, op 2: sum of w*x*x.
 template <typename T>
 __global__ __launch_bounds__(256) void k_reduce_part(
     int op, int nrow, int nvar, int neles, long ld, const T *__restrict__ x,
@@ -502,7 +521,9 @@ __global__ __launch_bounds__(256) void k_reduce_part(
         const int r = (int)(q/neles);
         const long e = q - (long)r*neles;
         const double xv = (double)x[r*ld + soa_ix(e, v, nvar)];
-        if (op == 0) acc += w ? (double)w[r*ldw + e]*xv : xv;
+        const double wv = w ? (double)w[r*ldw + e] : 1.0;
+        if (op == 0) acc += wv*xv;
+        else if (op == 2) acc += wv*xv*xv;
         else acc = fmax(acc, fabs(xv));
     }
     __shared__ double sm[256];
@@ -510,7 +531,7 @@ __global__ __launch_bounds__(256) void k_reduce_part(
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s)
-            sm[threadIdx.x] = op == 0 ? sm[threadIdx.x] + sm[threadIdx.x + s]
+            sm[threadIdx.x] = op != 1 ? sm[threadIdx.x] + sm[threadIdx.x + s]
                                       : fmax(sm[threadIdx.x], sm[threadIdx.x + s]);
         __syncthreads();
     }
@@ -524,14 +545,14 @@ __global__ __launch_bounds__(256) void k_reduce_final(int op, int nblk,
     const int v = blockIdx.x;
     double acc = 0.0;
     for (int q = threadIdx.x; q < nblk; q += blockDim.x)
-        acc = op == 0 ? acc + part[(long)v*nblk + q]
+        acc = op != 1 ? acc + part[(long)v*nblk + q]
                       : fmax(acc, part[(long)v*nblk + q]);
     __shared__ double sm[256];
     sm[threadIdx.x] = acc;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s)
-            sm[threadIdx.x] = op == 0 ? sm[threadIdx.x] + sm[threadIdx.x + s]
+            sm[threadIdx.x] = op != 1 ? sm[threadIdx.x] + sm[threadIdx.x + s]
                                       : fmax(sm[threadIdx.x], sm[threadIdx.x + s]);
         __syncthreads();
     }
@@ -807,6 +828,20 @@ int frhip_rkvdh2(int dtype, int nrow, long ncol, long ld, void *r1, void *r2,
     FR_API_END
 }
 
+int frhip_rkvdh2_mixed(int nrow, long ncol, long ld, void *acc, void *kin,
+                       double dta, double dtb, int last, void *stream)
+{
+    FR_API_BEGIN
+    FR_CHECK(nrow > 0 && nrow < 65536, "invalid row count");
+    if (ncol == 0) return 0;
+    dim3 grid(cdiv(ncol, 256), nrow), block(256);
+    hipLaunchKernelGGL(k_rkvdh2_mixed, grid, block, 0, as_stream(stream), ncol,
+                       ld, (double *)acc, (float *)kin, dta, dtb, last);
+    FR_LAUNCHED();
+    return 0;
+    FR_API_END
+}
+
 // Scratch for the two-pass reduction, allocated on first use (outside any
 // graph capture) and kept for the life of the process.
 static double *g_red_part = nullptr;
@@ -819,7 +854,7 @@ int frhip_reduce(int dtype, int op, int nrow, int nvar, int neles, long ld,
 {
     FR_API_BEGIN
     FR_DTYPE_OK(dtype);
-    FR_CHECK(op == 0 || op == 1, "invalid reduction op");
+    FR_CHECK(op == 0 || op == 1 || op == 2, "invalid reduction op");
     FR_CHECK(nvar > 0 && nvar <= 64 && nrow > 0, "invalid extents");
     if (nvar > g_red_nvar) {
         if (g_red_part) FR_HIP(hipFree(g_red_part));

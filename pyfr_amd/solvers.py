@@ -172,9 +172,9 @@ class FluidElements:
         if hasattr(self, '_vect_fpts'):
             self._comm_fpts = self._vect_fpts.slice(0, self.nfpts)
 
-    def alloc_bank(self, extent, ic=None):
+    def alloc_bank(self, extent, ic=None, dtype=None):
         m = self._be.matrix((self.nupts, self.nvars, self.neles), ic,
-                            extent=extent, tags={'align'})
+                            extent=extent, tags={'align'}, dtype=dtype)
         self.scal_upts.append(m)
         return m
 
@@ -515,7 +515,9 @@ class BaseSystem:
     intinterscls = None
     name = None
 
-    def __init__(self, backend, mesh, cfg, nregs):
+    def __init__(self, backend, mesh, cfg, nregs, nwork=0):
+        """nregs register banks in the state precision (bank 0 holds the
+        initial condition), then nwork banks in the working precision."""
         self.backend = backend
         self.mesh = mesh
         self.cfg = cfg
@@ -536,9 +538,13 @@ class BaseSystem:
         self.ele_shapes = {et: (e.nupts, e.nvars, e.neles)
                            for et, e in elemap.items()}
         self.ele_banks = []
+        self.nregs, self.nwork = nregs, nwork
         for et, e in elemap.items():
-            banks = [e.alloc_bank(f'bank_{et}', ic=ics[et] if i == 0 else None)
+            banks = [e.alloc_bank(f'bank_{et}', ic=ics[et] if i == 0 else None,
+                                  dtype=backend.state_dtype)
                      for i in range(nregs)]
+            banks += [e.alloc_bank(f'work_{et}', dtype=backend.fpdtype)
+                      for i in range(nwork)]
             self.ele_banks.append(banks)
         backend.commit()
         for e in elemap.values():
@@ -573,6 +579,51 @@ class BaseSystem:
 
     def ele_scal_upts(self, idx):
         return [b[idx].get() for b in self.ele_banks]
+
+    # -- FP64 reductions over register banks (replaces the reference's
+    #    reduction kernels as used by its residual/integrate plugins)
+    def _reduce(self, idx, op, weighted):
+        import ctypes as ct
+        from pyfr_amd.backend import _dtcode
+        be = self.backend
+        if not hasattr(self, '_red_out'):
+            p = ct.c_void_p()
+            be.lib.malloc(ct.byref(p), 8*64)
+            self._red_out = p.value
+            self._red_w = {}
+        out = []
+        for et, banks in zip(self.ele_map, self.ele_banks):
+            m = banks[idx]
+            e = self.ele_map[et]
+            w = None
+            if weighted:
+                if (et, m.dtype) not in self._red_w:
+                    wts = (e.basis.upts_wts[:, None] /
+                           e.rcpdjac_at_np('upts'))
+                    self._red_w[(et, m.dtype)] = be.const_matrix(
+                        wts, dtype=m.dtype, tags={'align'})
+                w = self._red_w[(et, m.dtype)]
+            be.lib.reduce(_dtcode(m.dtype), op, m.nrow, e.nvars, e.neles,
+                          m.leaddim, m.data, None if w is None else w.data,
+                          0 if w is None else w.leaddim, self._red_out,
+                          be.stream)
+            res = np.empty(e.nvars)
+            be.lib.memcpy_dtoh(res.ctypes.data, self._red_out, 8*e.nvars,
+                               be.stream)
+            out.append(res)
+        return out
+
+    def conservation_sums(self, idx):
+        """Per-variable integral of the bank over the domain (sum of
+        w_u |J| u), FP64 accumulation."""
+        return np.sum(self._reduce(idx, 0, True), axis=0)
+
+    def l2_norms(self, idx):
+        """Per-variable quadrature L2 norm of the bank."""
+        return np.sqrt(np.sum(self._reduce(idx, 2, True), axis=0))
+
+    def max_abs(self, idx):
+        return np.max(self._reduce(idx, 1, False), axis=0)
 
     def set_ele_scal_upts(self, idx, arrs):
         for b, a in zip(self.ele_banks, arrs):
@@ -646,11 +697,14 @@ class BaseIntegrator:
     nregs = None
     nrhs_per_step = None
 
+    nwork = 0
+
     def __init__(self, backend, mesh, cfg):
         self.backend = backend
         self.cfg = cfg
         sysname = cfg.get('solver', 'system')
-        self.system = SYSTEMS[sysname](backend, mesh, cfg, self.nregs)
+        self.system = SYSTEMS[sysname](backend, mesh, cfg, self.nregs,
+                                       self.nwork)
         self.tcurr = cfg.getfloat('solver-time-integrator', 'tstart', 0.0)
         self._tcurr_comp = 0.0
         self.dt = cfg.getfloat('solver-time-integrator', 'dt')
@@ -783,6 +837,43 @@ class RK45Integrator(BaseIntegrator):
         return r2
 
 
+class MixedRK45Integrator(RK45Integrator):
+    """RK45 for the precision form: the RHS runs in FP32 on one working
+    array, the accumulated solution stays FP64.  Each stage update
+    (``frhip_rkvdh2_mixed``) is done in FP64 and writes the next FP32 stage
+    input in place of the stage RHS, so one FP64 and one FP32 array replace
+    the two registers of ``steppers.py:111-190``."""
+    nregs = 1
+    nwork = 1
+
+    def __init__(self, *args, **kwargs):
+        super().__init__(*args, **kwargs)
+        self._work_valid = False
+
+    def _sync_work(self):
+        lib, be = self.backend.lib, self.backend
+        for banks in self.system.ele_banks:
+            acc, w = banks[0], banks[1]
+            lib.convert(1, w.data, 0, acc.data, acc.nrow*acc.leaddim,
+                        be.stream)
+        self._work_valid = True
+
+    def step(self, t, dt):
+        if not self._work_valid:
+            self._sync_work()
+        lib, be = self.backend.lib, self.backend
+        last = self._nstages - 1
+        for i, ci in enumerate(self.c):
+            self._rhs(t + ci*dt, 1, 1)
+            dta = dt*self.a[i] if i < last else 0.0
+            dtb = dt*self.b[i]
+            for banks in self.system.ele_banks:
+                acc, w = banks[0], banks[1]
+                lib.rkvdh2_mixed(acc.nrow, acc.ncol, acc.leaddim, acc.data,
+                                 w.data, dta, dtb, int(i == last), be.stream)
+        return 0
+
+
 INTEGRATORS = {'rk4': RK4Integrator, 'rk45': RK45Integrator}
 
 
@@ -790,4 +881,9 @@ def get_integrator(backend, mesh, cfg):
     scheme = cfg.get('solver-time-integrator', 'scheme')
     if cfg.get('solver-time-integrator', 'controller', 'none') != 'none':
         raise ValueError('Only the fixed-dt "none" controller is supported')
+    if getattr(backend, 'precision', 'double') == 'mixed':
+        if scheme != 'rk45':
+            raise NotImplementedError('The mixed precision form is '
+                                      'implemented for rk45')
+        return MixedRK45Integrator(backend, mesh, cfg)
     return INTEGRATORS[scheme](backend, mesh, cfg)

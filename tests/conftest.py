@@ -21,6 +21,7 @@ def backend_factory():
     made = []
 
     def make(precision='double', graphs=True):
+        """precision: 'double', 'single' or 'mixed'."""
         cfg = Inifile.from_dict({'backend': {'precision': precision},
                                  'backend-hip': {'graphs': str(graphs)}})
         be = AMDBackend(cfg)
