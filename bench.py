@@ -146,6 +146,42 @@ def cpu_baseline(c, budget_s=12.0):
                       f'{dt:.1f} s'}
 
 
+def serving_report(nreq, seed=0, rate=50.0):
+    """Serving form: a seeded Poisson stream (mean 50 req/s) of independent
+    periodic-box solves (N ~ U{4..24}, p ~ U{1..4}, Mach ~ U[0.05, 0.3],
+    Re ~ U[100, 1600] or inviscid w.p. 0.3; 200 RK4 steps each), served with
+    continuous batching and, as the baseline, one solve at a time."""
+    from pyfr_amd.backend import AMDBackend
+    from pyfr_amd.inifile import Inifile
+    from pyfr_amd.serving import ServingEngine, poisson_requests, summarize
+    cfg = Inifile.from_dict({'backend': {'precision': 'double'}})
+    be = AMDBackend(cfg)
+    t0 = time.perf_counter()
+    eng = ServingEngine(be)
+    reqs = poisson_requests(seed, rate, nreq)
+    for r in reqs:
+        eng.box(r.N, r.order)
+    tsetup = time.perf_counter() - t0
+    print(f'# serving: {len(reqs)} requests, pools {eng.hbm_bytes/2**30:.1f}'
+          f' GiB, host set-up {tsetup:.1f} s', file=sys.stderr, flush=True)
+    # warm-up: capture/compile paths for every class once
+    warm = poisson_requests(seed + 1, rate, 8, nsteps=2)
+    eng.run([type(r)(r.rid, 4, p, 0.1, re, 2, 0.0) for p in (1, 2, 3, 4)
+             for re in (None, 500.0) for r in warm[:1]])
+    out = {}
+    for mode in ('batched', 'sequential'):
+        for r in reqs:
+            r.admit_time = r.finish_time = r.result = None
+        done = eng.run(reqs, mode=mode)
+        out[mode] = summarize(done)
+        print(f'# {mode}: {out[mode]}', file=sys.stderr, flush=True)
+    out['throughput_ratio'] = (out['batched']['gdof_rhs_per_s'] /
+                               out['sequential']['gdof_rhs_per_s'])
+    out['workload'] = (f'poisson-{rate:g}req/s-{nreq}solves-hex-N4..24-'
+                       'p1..4-rk4x200-fp64')
+    return out
+
+
 def accuracy_report(c, checkpoints=(100, 500)):
     """Per-field relative L2 and max errors of the mixed precision form
     against the FP64 run from the same initial field, plus FP64
@@ -200,6 +236,10 @@ def main():
                     choices=['double', 'mixed', 'single'])
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--kernel-reps', type=int, default=5)
+    ap.add_argument('--serving', action='store_true',
+                    help='serving form: continuous batching of a Poisson '
+                    'stream of independent solves vs one at a time')
+    ap.add_argument('--serving-requests', type=int, default=48)
     ap.add_argument('--accuracy', action='store_true',
                     help='precision form: compare mixed against FP64 at '
                     'steps 100 and 500 instead of timing')
@@ -218,6 +258,9 @@ def main():
     c = CONFIGS[args.config]
     if args.accuracy:
         return accuracy_report(c)
+    if args.serving:
+        print(json.dumps(serving_report(args.serving_requests)))
+        return
     cfg = make_cfg(c, args.precision)
     t0 = time.perf_counter()
     be = AMDBackend(cfg)

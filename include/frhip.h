@@ -44,6 +44,11 @@ int frhip_memcpy_htod(void *dst, const void *src, size_t nbytes, void *stream);
 int frhip_memcpy_dtoh(void *dst, const void *src, size_t nbytes, void *stream);
 int frhip_memcpy_dtod(void *dst, const void *src, size_t nbytes, void *stream);
 int frhip_memset(void *ptr, int value, size_t nbytes, void *stream);
+/* 2-D copies (hipMemcpy2DAsync): height rows of width bytes between pitched
+ * buffers; kind 0 = host to device, 1 = device to host, 2 = device to
+ * device.  Host-side copies are synchronised before returning. */
+int frhip_memcpy2d(void *dst, size_t dpitch, const void *src, size_t spitch,
+                   size_t width, size_t height, int kind, void *stream);
 int frhip_stream_create(void **stream);
 int frhip_stream_destroy(void *stream);
 int frhip_stream_sync(void *stream);
@@ -52,6 +57,8 @@ int frhip_event_create(void **event);
 int frhip_event_destroy(void *event);
 int frhip_event_record(void *event, void *stream);
 int frhip_event_sync(void *event);
+/* *done = 1 if all work before the event has completed, else 0. */
+int frhip_event_query(void *event, int *done);
 int frhip_event_elapsed(float *ms, void *start, void *end);
 
 /* ---- graphs: replaces HIPGraph (pyfr/backends/hip/types.py:79-116) and the
@@ -89,12 +96,15 @@ int frhip_tflux_euler(int dtype, int linear, int npts, int neles,
 /* NS fused tflux: pyfr/solvers/navstokes/kernels/tflux.mako:13-50 with
  * transform_grad.mako:3-10 and navstokes/kernels/flux.mako:48-105.  gradu is
  * transformed in place to physical space; f = S (Fi + Fv).
- * For linear = 0, rcpdjac is (npts, neles) with leading dimension ldr. */
+ * For linear = 0, rcpdjac is (npts, neles) with leading dimension ldr.
+ * consts = {gamma, mu, Pr, beta, tau, sutherland(0/1), cpTref, cpTs}.
+ * mu_e: optional per-element viscosity (neles values) replacing consts[1]
+ * (used to batch independent solves); NULL for the reference behaviour. */
 int frhip_tflux_ns(int dtype, int linear, int npts, int neles,
                    const void *u, long ldu, void *f, long ldf, void *gradu,
                    long ldg, const void *geo, long ldgeo, const void *upts,
                    const void *rcpdjac, long ldr, const double *consts,
-                   void *stream);
+                   const void *mu_e, void *stream);
 
 /* negdivconf: pyfr/solvers/baseadvec/kernels/negdivconf.mako:7-22 (no source
  * terms): t[v] = -rcpdjac * t[v]. */
@@ -118,12 +128,14 @@ int frhip_intconu(int dtype, int n, const void *ubase, const int32_t *ulin,
 /* NS intcflux: pyfr/solvers/navstokes/kernels/intcflux.mako:10-54.
  * gbase/glmap/grmap address the physical gradient at the flux points with row
  * stride gstride between dimensions.  consts = {gamma, mu, Pr, beta, tau,
- * sutherland(0/1), cpTref, cpTs}. */
+ * sutherland(0/1), cpTref, cpTs}.  mu_p: optional per-pair viscosity (n
+ * values), NULL for the reference behaviour. */
 int frhip_intcflux_ns(int dtype, int rsolver, int n, void *ubase,
                       const int32_t *lmap, const int32_t *rmap,
                       const void *gbase, const int32_t *glmap,
                       const int32_t *grmap, long gstride, const void *nl,
-                      long ldn, const double *consts, void *stream);
+                      long ldn, const double *consts, const void *mu_p,
+                      void *stream);
 
 /* ---- BLAS extensions: pyfr/backends/hip/blasext.py:8-136 ---- */
 
@@ -132,6 +144,16 @@ int frhip_intcflux_ns(int dtype, int rsolver, int n, void *ubase,
  * xs[0] is x0; nv <= 8. */
 int frhip_axnpby(int dtype, int nrow, long ncol, long ld, int nv,
                  void *const *xs, const double *a, void *stream);
+
+/* axnpby over a pool of independent solves stored side by side (no
+ * counterpart in the reference; it batches the RK4 axpys of
+ * pyfr/integrators/base.py:401-426 across solves with different dt): the
+ * coefficient of operand i at element e is dt[e]/a[i] if bit i of dtmask is
+ * set, else a[i].  dt: per-element array (neles_pad values) in dtype.
+ * ncol must be a multiple of 64*nvar; nv <= 4. */
+int frhip_axnpby_pool(int dtype, int nrow, long ncol, long ld, int nv,
+                      void *const *xs, const double *a, unsigned dtmask,
+                      const void *dt, int nvar, void *stream);
 
 /* rkvdh2 stage update: pyfr/integrators/explicit/kernels/rkvdh2.mako:4-31
  * (no error estimate): if last: r1 += dt*b*r2 else r1' = r1 + dt*a*r2,

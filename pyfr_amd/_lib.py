@@ -29,6 +29,7 @@ _FUNCS = {
     'frhip_memcpy_dtoh': [_P, _P, _S, _P],
     'frhip_memcpy_dtod': [_P, _P, _S, _P],
     'frhip_memset': [_P, _I, _S, _P],
+    'frhip_memcpy2d': [_P, _S, _P, _S, _S, _S, _I, _P],
     'frhip_stream_create': [ct.POINTER(_P)],
     'frhip_stream_destroy': [_P],
     'frhip_stream_sync': [_P],
@@ -37,6 +38,7 @@ _FUNCS = {
     'frhip_event_destroy': [_P],
     'frhip_event_record': [_P, _P],
     'frhip_event_sync': [_P],
+    'frhip_event_query': [_P, ct.POINTER(_I)],
     'frhip_event_elapsed': [ct.POINTER(ct.c_float), _P, _P],
     'frhip_graph_begin': [_P],
     'frhip_graph_end': [_P, ct.POINTER(_P)],
@@ -49,13 +51,14 @@ _FUNCS = {
     'frhip_tflux_euler': [_I, _I, _I, _I, _P, _L, _P, _L, _P, _L, _P, _D,
                           _P],
     'frhip_tflux_ns': [_I, _I, _I, _I, _P, _L, _P, _L, _P, _L, _P, _L, _P,
-                       _P, _L, _P, _P],
+                       _P, _L, _P, _P, _P],
     'frhip_negdivconf': [_I, _I, _I, _P, _L, _P, _L, _P],
     'frhip_intcflux_euler': [_I, _I, _I, _P, _PI32, _PI32, _P, _L, _D, _P],
     'frhip_intconu': [_I, _I, _P, _PI32, _PI32, _P, _PI32, _PI32, _D, _P],
     'frhip_intcflux_ns': [_I, _I, _I, _P, _PI32, _PI32, _P, _PI32, _PI32,
-                          _L, _P, _L, _P, _P],
+                          _L, _P, _L, _P, _P, _P],
     'frhip_axnpby': [_I, _I, _L, _L, _I, _P, _P, _P],
+    'frhip_axnpby_pool': [_I, _I, _L, _L, _I, _P, _P, ct.c_uint, _P, _I, _P],
     'frhip_rkvdh2': [_I, _I, _L, _L, _P, _P, _D, _D, _I, _P],
     'frhip_rkvdh2_mixed': [_I, _L, _L, _P, _P, _D, _D, _I, _P],
     'frhip_reduce': [_I, _I, _I, _I, _I, _L, _P, _P, _L, _P, _P],

@@ -557,7 +557,7 @@ class PointwiseProvider:
             ldr = 0 if linear else rcpdjac.leaddim
             args = (dt, int(linear), npts, neles, u.data, u.leaddim, f.data,
                     f.leaddim, gradu.data, gradu.leaddim, geo.data,
-                    geo.leaddim, uptsp, rp, ldr, consts)
+                    geo.leaddim, uptsp, rp, ldr, consts, None)
             return Kernel(lambda stream: lib.tflux_ns(*args, stream),
                           mats=(u, f, gradu, geo), name='tflux_ns')
         args = (dt, int(linear), npts, neles, u.data, u.leaddim, f.data,
@@ -611,7 +611,7 @@ class PointwiseProvider:
             c.get('cpTref', 1.0), c.get('cpTs', 0.0))
         args = (dt, rs, n, ul.basedata, ul.mapping.data, ur.mapping.data,
                 gradul.basedata, gradul.mapping.data, gradur.mapping.data,
-                gradul.rstrides_val, nl.data, nl.leaddim, consts)
+                gradul.rstrides_val, nl.data, nl.leaddim, consts, None)
         return Kernel(lambda stream: lib.intcflux_ns(*args, stream),
                       views=(ul, ur, gradul, gradur), mats=(nl,),
                       name='intcflux_ns')

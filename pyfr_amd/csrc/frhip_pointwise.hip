@@ -110,8 +110,17 @@ __device__ __forceinline__ void rsolve(int rs, const T ul[5], const T ur[5],
 // NS constants prepared on the host in double, then cast.
 template <typename T>
 struct NSConsts {
-    T gamma, gm1, qgamma, mu, gpr, beta, tau, rcpTref, musuth, cpTs;
+    T gamma, gm1, qgamma, mu, gpr, beta, tau, rcpTref, musuth, cpTs, suthfac;
     int suth;
+
+    // Copy with the viscosity replaced (per-element / per-pair viscosity)
+    __device__ NSConsts with_mu(T m) const
+    {
+        NSConsts c = *this;
+        c.mu = m;
+        c.musuth = m*suthfac;
+        return c;
+    }
 };
 
 // Viscous flux (added to f); pyfr/solvers/navstokes/kernels/flux.mako:48-105.
@@ -250,10 +259,11 @@ __global__ __launch_bounds__(256) void k_tflux_ns(
     int npts, int neles, const T *__restrict__ u, long ldu, T *__restrict__ f,
     long ldf, T *__restrict__ gradu, long ldgr, const T *__restrict__ geo,
     long ldg, const T *__restrict__ upts, const T *__restrict__ rcpdjac,
-    long ldr, NSConsts<T> c)
+    long ldr, NSConsts<T> c0, const T *__restrict__ mu_e)
 {
     const int e = blockIdx.x*64 + threadIdx.x;
     if (e >= neles) return;
+    const NSConsts<T> c = mu_e ? c0.with_mu(mu_e[e]) : c0;
 
     T V[8][3];
     if (LIN)
@@ -377,10 +387,12 @@ __global__ __launch_bounds__(256) void k_intcflux_ns(
     int n, int rs, T *__restrict__ base, const int32_t *__restrict__ lmap,
     const int32_t *__restrict__ rmap, const T *__restrict__ gb,
     const int32_t *__restrict__ glmap, const int32_t *__restrict__ grmap,
-    long gstride, const T *__restrict__ nl, long ldn, NSConsts<T> c)
+    long gstride, const T *__restrict__ nl, long ldn, NSConsts<T> c0,
+    const T *__restrict__ mu_p)
 {
     const int i = blockIdx.x*blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const NSConsts<T> c = mu_p ? c0.with_mu(mu_p[i]) : c0;
     const long li = lmap[i], ri = rmap[i], gli = glmap[i], gri = grmap[i];
     T ul[5], ur[5];
     for (int v = 0; v < 5; ++v) {
@@ -466,6 +478,39 @@ __global__ __launch_bounds__(256) void k_axnpby(long ncol, long ld,
             acc = acc + (a[i])*(static_cast<const T *>(xs.p[i])[idx]);
         x0[idx] = acc;
     }
+}
+
+// axnpby over a pool of independent solves: coefficient i of element e is
+// dt[e]/a[i] when bit i of dtmask is set (dt/6.0 rounds like the host's
+// dt/6.0), else a[i].  Same a0 special cases as the uniform kernel for an
+// unscaled a0.
+template <typename T, int NV>
+__global__ __launch_bounds__(256) void k_axnpby_pool(long ncol, long ld,
+                                                     Ptrs<NV> xs,
+                                                     Coefs<T, NV> ca,
+                                                     unsigned dtmask,
+                                                     const T *__restrict__ dt,
+                                                     int nvar)
+{
+    const long j = (long)blockIdx.x*blockDim.x + threadIdx.x;
+    if (j >= ncol) return;
+    const long idx = (long)blockIdx.y*ld + j;
+    const long e = (j/(SOA*nvar))*SOA + (j & (SOA - 1));
+    const T dte = dt[e];
+    T a[NV];
+    for (int i = 0; i < NV; ++i)
+        a[i] = ((dtmask >> i) & 1) ? dte/ca.a[i] : ca.a[i];
+    T *x0 = static_cast<T *>(xs.p[0]);
+    const bool a0dt = dtmask & 1;
+    T acc = (a[1])*(static_cast<const T *>(xs.p[1])[idx]);
+    for (int i = 2; i < NV; ++i)
+        acc = acc + (a[i])*(static_cast<const T *>(xs.p[i])[idx]);
+    if (!a0dt && ca.a[0] == T(0))
+        x0[idx] = acc;
+    else if (!a0dt && ca.a[0] == T(1))
+        x0[idx] += acc;
+    else
+        x0[idx] = (a[0])*(x0[idx]) + acc;
 }
 
 // rkvdh2 without error estimate; rkvdh2.mako:4-31.
@@ -583,6 +628,7 @@ static NSConsts<T> ns_consts(const double *c)
     k.rcpTref = (T)(k.suth ? 1/c[6] : 0.0);
     k.musuth = (T)(k.suth ? mu*(c[6] + c[7]) : 0.0);
     k.cpTs = (T)c[7];
+    k.suthfac = (T)(k.suth ? c[6] + c[7] : 0.0);
     return k;
 }
 
@@ -598,6 +644,22 @@ static void launch_axnpby(int nrow, long ncol, long ld, void *const *xs,
     }
     dim3 grid(cdiv(ncol, 256), nrow), block(256);
     hipLaunchKernelGGL((k_axnpby<T, NV>), grid, block, 0, s, ncol, ld, px, pa);
+}
+
+template <typename T, int NV>
+static void launch_axnpby_pool(int nrow, long ncol, long ld, void *const *xs,
+                               const double *a, unsigned dtmask,
+                               const void *dt, int nvar, hipStream_t s)
+{
+    Ptrs<NV> px;
+    Coefs<T, NV> pa;
+    for (int i = 0; i < NV; ++i) {
+        px.p[i] = xs[i];
+        pa.a[i] = (T)a[i];
+    }
+    dim3 grid(cdiv(ncol, 256), nrow), block(256);
+    hipLaunchKernelGGL((k_axnpby_pool<T, NV>), grid, block, 0, s, ncol, ld, px,
+                       pa, dtmask, (const T *)dt, nvar);
 }
 
 }  // namespace frhip
@@ -654,7 +716,7 @@ int frhip_tflux_ns(int dtype, int linear, int npts, int neles,
                    const void *u, long ldu, void *f, long ldf, void *gradu,
                    long ldg, const void *geo, long ldgeo, const void *upts,
                    const void *rcpdjac, long ldr, const double *consts,
-                   void *stream)
+                   const void *mu_e, void *stream)
 {
     FR_API_BEGIN
     FR_DTYPE_OK(dtype);
@@ -671,7 +733,7 @@ int frhip_tflux_ns(int dtype, int linear, int npts, int neles,
                            (const double *)u, ldu, (double *)f, ldf,
                            (double *)gradu, ldg, (const double *)geo, ldgeo,
                            (const double *)upts, (const double *)rcpdjac, ldr,
-                           c);
+                           c, (const double *)mu_e);
     }
     else {
         auto c = ns_consts<float>(consts);
@@ -680,7 +742,7 @@ int frhip_tflux_ns(int dtype, int linear, int npts, int neles,
                            (const float *)u, ldu, (float *)f, ldf,
                            (float *)gradu, ldg, (const float *)geo, ldgeo,
                            (const float *)upts, (const float *)rcpdjac, ldr,
-                           c);
+                           c, (const float *)mu_e);
     }
     FR_LAUNCHED();
     return 0;
@@ -759,7 +821,8 @@ int frhip_intcflux_ns(int dtype, int rsolver, int n, void *ubase,
                       const int32_t *lmap, const int32_t *rmap,
                       const void *gbase, const int32_t *glmap,
                       const int32_t *grmap, long gstride, const void *nl,
-                      long ldn, const double *consts, void *stream)
+                      long ldn, const double *consts, const void *mu_p,
+                      void *stream)
 {
     FR_API_BEGIN
     FR_DTYPE_OK(dtype);
@@ -771,12 +834,13 @@ int frhip_intcflux_ns(int dtype, int rsolver, int n, void *ubase,
                            as_stream(stream), n, rsolver, (double *)ubase,
                            lmap, rmap, (const double *)gbase, glmap, grmap,
                            gstride, (const double *)nl, ldn,
-                           ns_consts<double>(consts));
+                           ns_consts<double>(consts), (const double *)mu_p);
     else
         hipLaunchKernelGGL(k_intcflux_ns<float>, grid, block, 0,
                            as_stream(stream), n, rsolver, (float *)ubase, lmap,
                            rmap, (const float *)gbase, glmap, grmap, gstride,
-                           (const float *)nl, ldn, ns_consts<float>(consts));
+                           (const float *)nl, ldn, ns_consts<float>(consts),
+                           (const float *)mu_p);
     FR_LAUNCHED();
     return 0;
     FR_API_END
@@ -802,6 +866,35 @@ int frhip_axnpby(int dtype, int nrow, long ncol, long ld, int nv,
         FR_AX(2) FR_AX(3) FR_AX(4) FR_AX(5) FR_AX(6) FR_AX(7) FR_AX(8)
     }
 #undef FR_AX
+    FR_LAUNCHED();
+    return 0;
+    FR_API_END
+}
+
+int frhip_axnpby_pool(int dtype, int nrow, long ncol, long ld, int nv,
+                      void *const *xs, const double *a, unsigned dtmask,
+                      const void *dt, int nvar, void *stream)
+{
+    FR_API_BEGIN
+    FR_DTYPE_OK(dtype);
+    FR_CHECK(nv >= 2 && nv <= 4, "pool axnpby supports 2..4 operands");
+    FR_CHECK(nrow > 0 && nrow < 65536 && nvar > 0 && dt, "invalid arguments");
+    FR_CHECK(ncol % (SOA*nvar) == 0, "ncol must cover whole SoA blocks");
+    if (ncol == 0) return 0;
+    hipStream_t s = as_stream(stream);
+#define FR_AXP(NVV)                                                        \
+    case NVV:                                                              \
+        if (dtype == FRHIP_F64)                                            \
+            launch_axnpby_pool<double, NVV>(nrow, ncol, ld, xs, a, dtmask, \
+                                            dt, nvar, s);                 \
+        else                                                               \
+            launch_axnpby_pool<float, NVV>(nrow, ncol, ld, xs, a, dtmask,  \
+                                           dt, nvar, s);                  \
+        break;
+    switch (nv) {
+        FR_AXP(2) FR_AXP(3) FR_AXP(4)
+    }
+#undef FR_AXP
     FR_LAUNCHED();
     return 0;
     FR_API_END

@@ -122,6 +122,21 @@ int frhip_memset(void *ptr, int value, size_t nbytes, void *stream)
     FR_API_END
 }
 
+int frhip_memcpy2d(void *dst, size_t dpitch, const void *src, size_t spitch,
+                   size_t width, size_t height, int kind, void *stream)
+{
+    FR_API_BEGIN
+    FR_CHECK(kind >= 0 && kind <= 2, "invalid copy kind");
+    const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice
+                          : kind == 1 ? hipMemcpyDeviceToHost
+                                      : hipMemcpyDeviceToDevice;
+    FR_HIP(hipMemcpy2DAsync(dst, dpitch, src, spitch, width, height, k,
+                            as_stream(stream)));
+    if (kind != 2) FR_HIP(hipStreamSynchronize(as_stream(stream)));
+    return 0;
+    FR_API_END
+}
+
 int frhip_stream_create(void **stream)
 {
     FR_API_BEGIN
@@ -186,6 +201,20 @@ int frhip_event_sync(void *event)
 {
     FR_API_BEGIN
     FR_HIP(hipEventSynchronize(static_cast<hipEvent_t>(event)));
+    return 0;
+    FR_API_END
+}
+
+int frhip_event_query(void *event, int *done)
+{
+    FR_API_BEGIN
+    hipError_t e = hipEventQuery(static_cast<hipEvent_t>(event));
+    if (e == hipErrorNotReady) {
+        *done = 0;
+        return 0;
+    }
+    FR_HIP(e);
+    *done = 1;
     return 0;
     FR_API_END
 }
