@@ -9,9 +9,13 @@ Taylor-Green initial field at Mach 0.1, Re 1600.  A step is one RK45 step
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--config cfg1|cfg0]
 
-Prints one JSON line with the metric, a roofline object for the dominant
-kernel (algorithmic bytes per launch / mean launch time from HIP events on
-the solver's stream) and a CPU baseline (the oracle on a bounded sample).
+Prints one JSON line: `value` is the FP64 plain GDOF*RHS/s; a roofline
+object for the dominant kernel (algorithmic bytes per launch / mean launch
+time from HIP events on the solver's stream); `precision_form` (FP32 RHS,
+FP64 RK state) with its speed-up; `serving` (continuous batching of a seeded
+Poisson stream of independent solves: p50/p99 latency, aggregate
+GDOF*RHS/s and the ratio to one-solve-at-a-time); and a CPU baseline (the
+oracle on a bounded sample).
 """
 import argparse
 import ctypes as ct
@@ -35,10 +39,15 @@ CONFIGS = {
 }
 
 
+FUSED = True
+
+
 def make_cfg(c, precision='double'):
     from tests.fr_cases import make_cfg as mk
-    return mk(c['system'], c['order'], 'rusanov', precision=precision,
-              scheme=c['scheme'], dt=c['dt'])
+    cfg = mk(c['system'], c['order'], 'rusanov', precision=precision,
+             scheme=c['scheme'], dt=c['dt'])
+    cfg.set('backend-hip', 'fused', str(FUSED))
+    return cfg
 
 
 # -- algorithmic bytes per launch (every input read once, every output
@@ -74,6 +83,18 @@ def kernel_bytes(k, isz, nvars=5):
     if name == 'rkvdh2_mixed':
         acc, w = k.mats
         return acc.nrow*acc.ncol*(2*8 + 2*4)
+    if name.startswith('mul2_ep'):
+        ep = int(name[-1])
+        a0, a1, b0, b1, out = k.mats
+        n = out.ncol
+        by = (b0.nrow + b1.nrow + out.nrow)*n*isz
+        if ep >= 1:
+            by += out.nrow*(n // nvars)*isz          # 1/|J|
+        if ep == 2:
+            by += 2*out.nrow*n*isz                   # r1 read + write
+        if ep == 3:
+            by += 2*out.nrow*n*8                     # FP64 solution r/w
+        return by
     if name == 'axnpby':
         x = k.mats
         return (len(x) + 1)*x[0].nrow*x[0].ncol*isz
@@ -235,6 +256,12 @@ def main():
     ap.add_argument('--precision', default='double',
                     choices=['double', 'mixed', 'single'])
     ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--unfused', action='store_true',
+                    help="run the reference's unfused kernel decomposition")
+    ap.add_argument('--no-mixed', action='store_true',
+                    help='skip the precision-form measurement')
+    ap.add_argument('--no-serving', action='store_true',
+                    help='skip the serving-form measurement')
     ap.add_argument('--kernel-reps', type=int, default=5)
     ap.add_argument('--serving', action='store_true',
                     help='serving form: continuous batching of a Poisson '
@@ -251,51 +278,101 @@ def main():
               'DESIGN.md).', file=sys.stderr)
         sys.exit(2)
 
-    from pyfr_amd.backend import AMDBackend
-    from pyfr_amd.mesh import BoxMesh
-    from pyfr_amd.solvers import get_integrator
-
+    global FUSED
+    FUSED = not args.unfused
     c = CONFIGS[args.config]
     if args.accuracy:
         return accuracy_report(c)
     if args.serving:
         print(json.dumps(serving_report(args.serving_requests)))
         return
-    cfg = make_cfg(c, args.precision)
+
+    res = measure(c, args.precision, args.steps, args.warmup,
+                  args.kernel_reps)
+    line = {
+        'metric': 'GDOF*RHS/s', 'value': round(res['gdofs'], 4),
+        'unit': 'GDOF*RHS/s', 'n_gpus': 1, 'steps': args.steps,
+        'warmup': args.warmup, 'ms_per_step': round(res['ms_per_step'], 3),
+        'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': res['dtype'],
+        'data': 'synthetic (analytic Taylor-Green field)',
+        'config': res['config'], 'roofline': res['roofline'],
+        'rhs_kernels': res['rhs_kernels'],
+        'rhs_alg_bytes_per_dof': res['rhs_alg_bytes_per_dof'],
+        'rhs_kernel_ms_sum': res['rhs_kernel_ms_sum'],
+    }
+    if args.precision == 'double' and not args.no_mixed:
+        mx = measure(c, 'mixed', args.steps, args.warmup, args.kernel_reps)
+        line['precision_form'] = {
+            'value': round(mx['gdofs'], 4), 'unit': 'GDOF*RHS/s',
+            'dtype': mx['dtype'], 'ms_per_step': round(mx['ms_per_step'], 3),
+            'speedup_vs_fp64': round(mx['gdofs']/res['gdofs'], 3),
+            'roofline': mx['roofline'],
+            'rhs_alg_bytes_per_dof': mx['rhs_alg_bytes_per_dof'],
+            'accuracy': 'profiles/accuracy_ns-hex64-p3-rk45_mixed-vs-fp64.md'}
+    if not args.no_serving and args.config == 'cfg1':
+        sv = serving_report(args.serving_requests)
+        line['serving'] = {
+            'workload': sv['workload'],
+            'p50_latency_s': round(sv['batched']['p50_s'], 3),
+            'p99_latency_s': round(sv['batched']['p99_s'], 3),
+            'value': round(sv['batched']['gdof_rhs_per_s'], 4),
+            'unit': 'GDOF*RHS/s',
+            'one_at_a_time': {
+                'value': round(sv['sequential']['gdof_rhs_per_s'], 4),
+                'p50_latency_s': round(sv['sequential']['p50_s'], 3),
+                'p99_latency_s': round(sv['sequential']['p99_s'], 3)},
+            'throughput_ratio': round(sv['throughput_ratio'], 3)}
+    line['cpu_baseline'] = None if args.no_cpu_baseline else cpu_baseline(c)
+    print(json.dumps(line))
+
+
+def measure(c, precision, steps, warmup, kernel_reps):
+    """Time `steps` steps of the workload after `warmup` steps, then time
+    each kernel of one RHS (+ stage update) with HIP events."""
+    from pyfr_amd.backend import AMDBackend
+    from pyfr_amd.mesh import BoxMesh
+    from pyfr_amd.solvers import get_integrator
+
+    cfg = make_cfg(c, precision)
     t0 = time.perf_counter()
     be = AMDBackend(cfg)
     integ = get_integrator(be, BoxMesh(c['n']), cfg)
     sysm = integ.system
     tsetup = time.perf_counter() - t0
-    print(f'# setup {tsetup:.1f} s, {sysm.gndofs} DOF', file=sys.stderr,
-          flush=True)
-
-    dt = c['dt']
-    integ.run_steps(args.warmup, dt)
-    be.wait()
-    t0 = time.perf_counter()
-    integ.run_steps(args.steps, dt)
-    be.wait()
-    t1 = time.perf_counter()
-    nrhs = integ.nrhs_per_step*args.steps
-    gdofs = sysm.gndofs*nrhs/(t1 - t0)/1e9
-    ms_per_step = (t1 - t0)*1e3/args.steps
-    print(f'# {args.steps} steps in {t1 - t0:.3f} s: {gdofs:.3f} GDOF*RHS/s',
+    print(f'# {precision}: setup {tsetup:.1f} s, {sysm.gndofs} DOF',
           file=sys.stderr, flush=True)
 
-    # Per-kernel times inside one RHS (+ stage update)
-    g = sysm._rhs_graphs(*((1, 1) if args.precision == 'mixed'
-                           else (0, 1)))[0]
-    kerns = flatten(g.klist)
-    if c['scheme'] == 'rk45' and args.precision != 'mixed':
-        rk = integ._rkvdh2_kerns(0, 0, 1)
-        for k in rk:
-            k.bind(dt=dt)
-        kerns += rk
-    elif args.precision == 'mixed':
-        kerns += [mixed_stage_kernel(be, integ, dt)]
+    dt = c['dt']
+    integ.run_steps(warmup, dt)
+    be.wait()
+    t0 = time.perf_counter()
+    integ.run_steps(steps, dt)
+    be.wait()
+    t1 = time.perf_counter()
+    nrhs = integ.nrhs_per_step*steps
+    gdofs = sysm.gndofs*nrhs/(t1 - t0)/1e9
+    ms_per_step = (t1 - t0)*1e3/steps
+    print(f'# {precision}: {steps} steps in {t1 - t0:.3f} s: '
+          f'{gdofs:.3f} GDOF*RHS/s', file=sys.stderr, flush=True)
+
+    mixed = precision == 'mixed'
+    if c['scheme'] == 'rk45' and sysm.fused_stages:
+        # the graph of a middle RK45 stage (RHS fused with its update)
+        key = [k for k in sysm._kcache if len(k) == 5 and k[3] == 1][0]
+        kerns = flatten(sysm._kcache[key][0].klist)
+    else:
+        g = sysm._rhs_graphs(*((1, 1) if mixed else (0, 1)))[0]
+        kerns = flatten(g.klist)
+        if c['scheme'] == 'rk45' and not mixed:
+            rk = integ._rkvdh2_kerns(0, 0, 1)
+            for k in rk:
+                k.bind(dt=dt)
+            kerns += rk
+        elif mixed:
+            kerns += [mixed_stage_kernel(be, integ, dt)]
     isz = np.dtype(be.fpdtype).itemsize
-    times = time_kernels(be, kerns, args.kernel_reps)
+    times = time_kernels(be, kerns, kernel_reps)
     per = {}
     for k, ms in zip(kerns, times):
         nm = k.name
@@ -312,42 +389,32 @@ def main():
               f'({100*d["frac"]:5.1f}% of peak)', file=sys.stderr)
     dom = max(per, key=lambda n: per[n]['ms'])
     dd = per[dom]
-    avg_ms = dd['ms']/dd['launches']
     roof = {'bound': 'hbm', 'kernel': dom,
             'achieved': round(dd['GBs'], 1), 'peak': HBM_PEAK_GBS,
             'unit': 'GB/s', 'frac': round(dd['frac'], 4),
             'traffic': None, 'bytes_per_launch': dd['bytes']//dd['launches'],
-            'avg_launch_ms': round(avg_ms, 4)}
-    tot_ms = sum(d['ms'] for d in per.values())
-    tot_b = sum(d['bytes'] for d in per.values())
-
-    cpu = None
-    if not args.no_cpu_baseline:
-        cpu = cpu_baseline(c)
-
+            'avg_launch_ms': round(dd['ms']/dd['launches'], 4)}
     dtype = {'double': 'f64', 'single': 'f32',
-             'mixed': 'f32 RHS / f64 state'}[args.precision]
-    line = {
-        'metric': 'GDOF*RHS/s', 'value': round(gdofs, 4),
-        'unit': 'GDOF*RHS/s', 'n_gpus': 1, 'steps': args.steps,
-        'warmup': args.warmup, 'ms_per_step': round(ms_per_step, 3),
-        'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
-        'dtype': dtype, 'data': 'synthetic (analytic Taylor-Green field)',
+             'mixed': 'f32 RHS / f64 state'}[precision]
+    out = {
+        'gdofs': gdofs, 'ms_per_step': ms_per_step, 'dtype': dtype,
         'config': {'workload': c['workload'].replace(
                        '-fp64', '-' + {'double': 'fp64', 'single': 'fp32',
-                                       'mixed': 'mixed'}[args.precision]),
-                   'mesh': f'{c["n"]}^3 hex',
-                   'order': c['order'], 'dof': sysm.gndofs,
-                   'scheme': c['scheme'], 'dt': dt,
+                                       'mixed': 'mixed'}[precision]),
+                   'kernels': 'fused' if FUSED else 'reference-decomposition',
+                   'mesh': f'{c["n"]}^3 hex', 'order': c['order'],
+                   'dof': sysm.gndofs, 'scheme': c['scheme'], 'dt': dt,
                    'rhs_per_step': integ.nrhs_per_step},
         'roofline': roof,
         'rhs_kernels': {n: {k: (round(v, 4) if isinstance(v, float) else v)
                             for k, v in d.items()} for n, d in per.items()},
-        'rhs_alg_bytes_per_dof': round(tot_b/sysm.gndofs, 1),
-        'rhs_kernel_ms_sum': round(tot_ms, 3),
-        'cpu_baseline': cpu,
+        'rhs_alg_bytes_per_dof': round(sum(d['bytes'] for d in per.values())
+                                       / sysm.gndofs, 1),
+        'rhs_kernel_ms_sum': round(sum(d['ms'] for d in per.values()), 3),
     }
-    print(json.dumps(line))
+    del integ, sysm, kerns
+    be.wait()
+    return out
 
 
 if __name__ == '__main__':

@@ -75,19 +75,68 @@ int frhip_graph_destroy(void *graph_exec);
  *      C(m, ncol) = alpha * A(m, k) @ B(k, ncol) + beta * C.
  * frhip_mul_plan_create takes the dense row-major host operator A.  If its
  * sparsity pattern and values match a compiled structured kernel (hex tensor-product
- * operators), that kernel is used with A's nonzeros uploaded; otherwise a
- * dense kernel is used.  *kind receives 1 (structured) or 0 (dense). */
+ * operators), that kernel is used with A's nonzeros uploaded; otherwise the
+ * plan is dense (tet/prism operators): matrix-core (MFMA) kernels with the
+ * operator in fragment order, or a scalar kernel.  *kind receives 1
+ * (structured) or 2 (dense). */
 int frhip_mul_plan_create(void **plan, int m, int k, const double *a_host,
                           int dtype, int *kind);
 int frhip_mul_plan_destroy(void *plan);
 int frhip_mul(void *plan, int ncol, const void *b, long ldb, void *c,
               long ldc, double alpha, double beta, void *stream);
 
+/* Autotune a dense plan for (ncol, ldb): times the scalar kernel and the
+ * MFMA kernel at 1, 2 and 4 wavefronts per block on the given input, writing
+ * only to a scratch output, and keeps the fastest (*variant: 0 scalar, else
+ * wavefronts per block; -1 for structured plans, which have no variants).
+ * Synchronous (uses the null stream); the input must be ready.  Replaces
+ * the reference's rocBLAS solution-index sweep (pyfr/backends/hip/
+ * rocblas.py:139-177). */
+int frhip_mul_plan_tune(void *plan, int ncol, const void *b, long ldb,
+                        int reps, int *variant);
+
+/* ---- fused operator applications: C = [A0 | A1] @ [B0; B1] followed by an
+ *      epilogue ep: 0 = store; 1 = negdivconf (c = -rcp*acc, rcp (m, neles)
+ *      with leading dimension ldr); 2 = negdivconf then the rkvdh2 stage on
+ *      r1 = xa and r2 = c (r1' = r1 + dta*t, r2' = r1 + dtb*t; if last only
+ *      r1' = r1 + dtb*t); 3 = as 2 with FP32 operands, an FP64 accumulated
+ *      solution xa and the FP32 next-stage input written to c.
+ * They fuse kernels the reference runs back to back on the same element
+ * data: tgradpcoru_upts + tgradcoru_upts (pyfr/solvers/baseadvecdiff/
+ * elements.py:47-55) and tdivtpcorf + tdivtconf + negdivconf
+ * (pyfr/solvers/baseadvec/elements.py:84-119) + rkvdh2
+ * (pyfr/integrators/explicit/kernels/rkvdh2.mako:4-31).  A plan matches a
+ * generated kernel by pattern and values (*kind = 1); otherwise it is a
+ * dense plan (*kind = 2, as for frhip_mul).  ncol must be a multiple of 320
+ * (whole 64-element blocks of 5 variables).  ldr = 0 gives every point of
+ * an element the same rcp value (affine elements: one 1/|J| per element). */
+int frhip_mul2_plan_create(void **plan, int m, int k0, int k1,
+                           const double *a0, const double *a1, int dtype,
+                           int *kind);
+int frhip_mul2_plan_destroy(void *plan);
+int frhip_mul2(void *plan, int ncol, const void *b0, long ld0, const void *b1,
+               long ld1, void *c, long ldc, int ep, const void *rcp, long ldr,
+               void *xa, double dta, double dtb, int last, void *stream);
+/* As frhip_mul_plan_tune for a two-input plan. */
+int frhip_mul2_plan_tune(void *plan, int ncol, const void *b0, long ld0,
+                         const void *b1, long ld1, int reps, int *variant);
+/* Pin a dense plan's variant (0, 1, 2 or 4 as reported by the tuners), e.g.
+ * from a stored tuning result.  Operands the MFMA kernel cannot take (ncol
+ * not a multiple of 64, misaligned pointers) always use the scalar kernel. */
+int frhip_mul_plan_set_variant(void *plan, int variant);
+int frhip_mul2_plan_set_variant(void *plan, int variant);
+
 /* ---- pointwise kernels (element-major 2-D over (npts, neles)) ---- */
 
+/* Geometry modes of the tflux kernels (the `linear` argument). */
+#define FRHIP_GEO_CURVED 0      /* stored smats (3, npts, 3, neles) */
+#define FRHIP_GEO_HEX_LINEAR 1  /* S from the 8 hex vertices (8, 3, neles) */
+#define FRHIP_GEO_AFFINE 2      /* one S per element: (9, neles), row 3i+k */
+
 /* Euler tflux: pyfr/solvers/euler/kernels/tflux.mako:8-30 with flux.mako:3-26.
- * linear = 1: S from the 8 vertices (verts: (8, 3, neles), upts: (npts, 3)
- * row-major, smats.mako:13-25); linear = 0: stored smats (3, npts, 3, neles). */
+ * linear = FRHIP_GEO_HEX_LINEAR: S from the 8 vertices (upts: (npts, 3)
+ * row-major, smats.mako:13-25); FRHIP_GEO_CURVED: stored smats;
+ * FRHIP_GEO_AFFINE: per-element S of straight-sided tets/prisms. */
 int frhip_tflux_euler(int dtype, int linear, int npts, int neles,
                       const void *u, long ldu, void *f, long ldf,
                       const void *geo, long ldgeo, const void *upts,
@@ -96,7 +145,8 @@ int frhip_tflux_euler(int dtype, int linear, int npts, int neles,
 /* NS fused tflux: pyfr/solvers/navstokes/kernels/tflux.mako:13-50 with
  * transform_grad.mako:3-10 and navstokes/kernels/flux.mako:48-105.  gradu is
  * transformed in place to physical space; f = S (Fi + Fv).
- * For linear = 0, rcpdjac is (npts, neles) with leading dimension ldr.
+ * For linear = 0 or 2, rcpdjac is (npts, neles) with leading dimension ldr
+ * (ldr = 0: one value per element).
  * consts = {gamma, mu, Pr, beta, tau, sutherland(0/1), cpTref, cpTs}.
  * mu_e: optional per-element viscosity (neles values) replacing consts[1]
  * (used to batch independent solves); NULL for the reference behaviour. */
@@ -107,7 +157,8 @@ int frhip_tflux_ns(int dtype, int linear, int npts, int neles,
                    const void *mu_e, void *stream);
 
 /* negdivconf: pyfr/solvers/baseadvec/kernels/negdivconf.mako:7-22 (no source
- * terms): t[v] = -rcpdjac * t[v]. */
+ * terms): t[v] = -rcpdjac * t[v]; rcpdjac (npts, neles) with leading
+ * dimension ldr, or ldr = 0 for one value per element. */
 int frhip_negdivconf(int dtype, int npts, int neles, void *t, long ldt,
                      const void *rcpdjac, long ldr, void *stream);
 

@@ -271,3 +271,212 @@ class HexShapeOracle:
         pts1d = np.linspace(-1, 1, 2)
         import itertools as it
         return np.array([p[::-1] for p in it.product(pts1d, repeat=3)])
+
+
+# ---------------------------------------------------------------------------
+# Simplex and prism bases (pyfr/polys.py:117-167, 198-276, 279-338) and the
+# TetShape / PriShape operators (pyfr/shapes.py:476-570).  Point tables are
+# passed in by the caller (the tests take them from the product's embedded
+# tables, which are themselves checked against the reference's rule files).
+# ---------------------------------------------------------------------------
+
+class TriBasis(_Basis):
+    """``pyfr/polys.py:117-167``."""
+
+    def _ortho(self, p, q):
+        with np.errstate(divide='ignore', invalid='ignore'):
+            a = np.where(q != 1, 2*(1 + p)/(1 - q) - 1, -1)
+        b = q
+        ob = []
+        for i, pi in enumerate(jacobi(self.order, 0, 0, a)):
+            pa = pi*(1 - b)**i
+            for j, pj in enumerate(jacobi(self.order - i, 2*i + 1, 0, b)):
+                cij = ((2*i + 1)*(2*i + 2*j + 2))**0.5 / 2**(i + 1)
+                ob.append(cij*pa*pj)
+        return ob
+
+
+class TetBasis(_Basis):
+    """``pyfr/polys.py:198-276``."""
+
+    def _ortho(self, p, q, r):
+        with np.errstate(divide='ignore', invalid='ignore'):
+            a = np.where(r != -q, -2*(1 + p)/(q + r) - 1, -1)
+            b = np.where(r != 1, 2*(1 + q)/(1 - r) - 1, -1)
+        c = r
+        ob = []
+        for i, pi in enumerate(jacobi(self.order, 0, 0, a)):
+            ci = 2**(-2*i - 1)*(2*i + 1)**0.5*(1 - b)**i
+            for j, pj in enumerate(jacobi(self.order - i, 2*i + 1, 0, b)):
+                cj = (i + j + 1)**0.5*2**-j*(1 - c)**(i + j)
+                cij, pij = ci*cj, pi*pj
+                jp = jacobi(self.order - i - j, 2*(i + j + 1), 0, c)
+                for k, pk in enumerate(jp):
+                    ck = (2*(k + j + i) + 3)**0.5
+                    ob.append(cij*ck*pij*pk)
+        return ob
+
+    def _jac_ortho(self, p, q, r):
+        with np.errstate(divide='ignore', invalid='ignore'):
+            a = np.where(r != -q, -2*(1 + p)/(q + r) - 1, -1)
+            b = np.where(r != 1, 2*(1 + q)/(1 - r) - 1, -1)
+        c = r
+        f = jacobi(self.order, 0, 0, a)
+        df = jacobi_diff(self.order, 0, 0, a)
+        ob = []
+        for i, (fi, dfi) in enumerate(zip(f, df)):
+            ci = 2**(-2*i - 1)*(2*i + 1)**0.5
+            g = jacobi(self.order - i, 2*i + 1, 0, b)
+            dg = jacobi_diff(self.order - i, 2*i + 1, 0, b)
+            for j, (gj, dgj) in enumerate(zip(g, dg)):
+                cj = (i + j + 1)**0.5*2**-j
+                cij = ci*cj
+                h = jacobi(self.order - i - j, 2*(i + j + 1), 0, c)
+                dh = jacobi_diff(self.order - i - j, 2*(i + j + 1), 0, c)
+                for k, (hk, dhk) in enumerate(zip(h, dh)):
+                    ck = (2*(k + j + i) + 3)**0.5
+                    cijk = cij*ck
+                    tmp1 = (1 - c)**(i + j - 1) if i + j > 0 else 1
+                    tmp2 = tmp1*(1 - b)**(i - 1) if i > 0 else 1
+                    pijk = 4*tmp2*dfi*gj*hk
+                    qijk = 2*(tmp2*(-i*fi + (1 + a)*dfi)*gj
+                              + tmp1*(1 - b)**i*fi*dgj)*hk
+                    rijk = (2*(1 + a)*tmp2*dfi*gj*hk
+                            + (1 + b)*tmp1*(1 - b)**i*fi*dgj*hk
+                            + (1 - c)**(i + j)*(1 - b)**i*fi*gj*dhk
+                            - (i*(1 + b)*tmp2 + (i + j)*tmp1*(1 - b)**i)
+                            * fi*gj*hk)
+                    ob.append([cijk*pijk, cijk*qijk, cijk*rijk])
+        return ob
+
+
+class PriBasis(_Basis):
+    """``pyfr/polys.py:279-338``."""
+
+    def _ortho(self, p, q, r):
+        with np.errstate(divide='ignore', invalid='ignore'):
+            a = np.where(q != 1, 2*(1 + p)/(1 - q) - 1, -1)
+        b, c = q, r
+        pab = []
+        for i, pi in enumerate(jacobi(self.order, 0, 0, a)):
+            ci = (1 - b)**i / 2**(i + 1)
+            for j, pj in enumerate(jacobi(self.order - i, 2*i + 1, 0, b)):
+                cij = ((2*i + 1)*(2*i + 2*j + 2))**0.5*ci
+                pab.append(cij*pi*pj)
+        sk = [(k + 0.5)**0.5 for k in range(self.order + 1)]
+        pc = [s*jp for s, jp in zip(sk, jacobi(self.order, 0, 0, c))]
+        return [pij*pk for pij in pab for pk in pc]
+
+    def _jac_ortho(self, p, q, r):
+        with np.errstate(divide='ignore', invalid='ignore'):
+            a = np.where(q != 1, 2*(1 + p)/(1 - q) - 1, -1)
+        b, c = q, r
+        f = jacobi(self.order, 0, 0, a)
+        df = jacobi_diff(self.order, 0, 0, a)
+        pab = []
+        for i, (fi, dfi) in enumerate(zip(f, df)):
+            g = jacobi(self.order - i, 2*i + 1, 0, b)
+            dg = jacobi_diff(self.order - i, 2*i + 1, 0, b)
+            for j, (gj, dgj) in enumerate(zip(g, dg)):
+                cij = ((2*i + 1)*(2*i + 2*j + 2))**0.5 / 2**(i + 1)
+                tmp = (1 - b)**(i - 1) if i > 0 else 1
+                pij = 2*tmp*dfi*gj
+                qij = tmp*(-i*fi + (1 + a)*dfi)*gj + (1 - b)**i*fi*dgj
+                rij = (1 - b)**i*fi*gj
+                pab.append([cij*pij, cij*qij, cij*rij])
+        sk = [(k + 0.5)**0.5 for k in range(self.order + 1)]
+        hc = [s*jp for s, jp in zip(sk, jacobi(self.order, 0, 0, c))]
+        dhc = [s*jp for s, jp in zip(sk, jacobi_diff(self.order, 0, 0, c))]
+        return [[pij*hk, qij*hk, rij*dhk]
+                for pij, qij, rij in pab for hk, dhk in zip(hc, dhc)]
+
+
+TET_FACES = [
+    ('tri', lambda s, t: (s, t, -1*np.ones_like(s)), (0, 0, -1)),
+    ('tri', lambda s, t: (s, -1*np.ones_like(s), t), (0, -1, 0)),
+    ('tri', lambda s, t: (-1*np.ones_like(s), t, s), (-1, 0, 0)),
+    ('tri', lambda s, t: (s, t, -s - t - 1), (1, 1, 1)),
+]
+PRI_FACES = [
+    ('tri', lambda s, t: (s, t, -1*np.ones_like(s)), (0, 0, -1)),
+    ('tri', lambda s, t: (s, t, np.ones_like(s)), (0, 0, 1)),
+    ('quad', lambda s, t: (s, -1*np.ones_like(s), t), (0, -1, 0)),
+    ('quad', lambda s, t: (-s, s, t), (1, 1, 0)),
+    ('quad', lambda s, t: (-1*np.ones_like(s), s, t), (-1, 0, 0)),
+]
+
+
+class SimplexShapeOracle:
+    """TetShape / PriShape operators by the reference's algorithm.
+
+    tables: {'upts': (pts, wts), 'tri_fpts': pts, 'tri_q36': (pts, wts)}."""
+
+    def __init__(self, name, order, tables):
+        self.name, self.order = name, order
+        self.faces = TET_FACES if name == 'tet' else PRI_FACES
+        basis = TetBasis if name == 'tet' else PriBasis
+        self.upts = np.asarray(tables['upts'][0], dtype=float)
+        self.nupts = len(self.upts)
+        self.ubasis = basis(order, self.upts)
+        n = order + 1
+        gq, _ = gauss_legendre_quad(n)
+        self._frule = {'tri': np.asarray(tables['tri_fpts'], dtype=float),
+                       'quad': gq}
+        self._q36 = tables['tri_q36']
+        self.nfacefpts = [len(self._frule[k]) for k, _, _ in self.faces]
+        self.nfpts = sum(self.nfacefpts)
+        self.fpts = np.vstack([_proj(p, self._frule[k])
+                               for k, p, _ in self.faces])
+        self.norm_fpts = np.vstack([[nm]*c for (_, _, nm), c in
+                                    zip(self.faces, self.nfacefpts)]) * 1.0
+
+    def _facebasis(self, kind):
+        cls = TriBasis if kind == 'tri' else QuadBasis
+        return cls(self.order, self._frule[kind])
+
+    @cached_property
+    def gbasis_coeffs(self):
+        coeffs = []
+        n = self.order + 1
+        for kind, proj, _ in self.faces:
+            if kind == 'tri':
+                qp, qw = (np.asarray(x, dtype=float) for x in self._q36)
+            else:
+                qp, qw = gauss_legendre_quad(n)
+            L = self._facebasis(kind).nodal_basis_at(qp)
+            M = self.ubasis.ortho_basis_at(_proj(proj, qp))
+            coeffs.append(np.einsum('i,ik,ji->kj', qw, L, M))
+        return np.vstack(coeffs)
+
+    @cached_property
+    def m0(self):
+        return self.ubasis.nodal_basis_at(self.fpts)
+
+    @cached_property
+    def m1(self):
+        m = np.rollaxis(self.ubasis.jac_nodal_basis_at(self.upts), 2)
+        return m.reshape(self.nupts, -1)
+
+    @cached_property
+    def m2(self):
+        m = self.norm_fpts[..., None]*self.m0[:, None, :]
+        return m.reshape(self.nfpts, -1)
+
+    @cached_property
+    def m3(self):
+        return clean((self.gbasis_coeffs @
+                      self.ubasis.ortho_basis_at(self.upts)).T)
+
+    @cached_property
+    def m4(self):
+        m = self.m1.reshape(self.nupts, -1, self.nupts).swapaxes(0, 1)
+        return m.reshape(-1, self.nupts)
+
+    @cached_property
+    def m6(self):
+        m = self.norm_fpts.T[:, None, :]*self.m3
+        return m.reshape(-1, self.nfpts)
+
+    def opmat(self, expr):
+        env = {f'M{i}': getattr(self, f'm{i}') for i in (0, 1, 2, 3, 4, 6)}
+        return clean(eval(expr.replace('*', '@'), {'__builtins__': {}}, env))

@@ -35,6 +35,12 @@ class NotSuitableError(Exception):
     """Raised by a provider that declines a kernel (``provider.py:209``)."""
 
 
+def _rstride(m):
+    """Row stride of a per-point (npts, neles) array; 0 for a one-row
+    array of per-element values (affine elements)."""
+    return 0 if m.nrow == 1 else m.leaddim
+
+
 def _dtcode(dtype):
     dt = np.dtype(dtype)
     if dt == np.float64:
@@ -424,6 +430,23 @@ class MulProvider:
     def __init__(self, backend):
         self.backend = backend
         self._plans = {}
+        self._plans2 = {}
+        self._tuned = {}
+
+    def _tune(self, fn, plan, ncol, *bargs):
+        """Pick a dense plan's kernel variant for this column count by
+        timing (once per plan and shape; replaces the reference's rocBLAS
+        solution sweep, pyfr/backends/hip/rocblas.py:139-177).  Runs at
+        kernel construction, outside any graph capture."""
+        be = self.backend
+        if not be.autotune:
+            return
+        key = (plan, ncol, bargs[1::2])
+        if key not in self._tuned:
+            be.wait()
+            v = ct.c_int()
+            fn(plan, ncol, *bargs, 5, ct.byref(v))
+            self._tuned[key] = v.value
 
     def _plan(self, a, dtype):
         key = (id(a), dtype)
@@ -452,6 +475,8 @@ class MulProvider:
         plan, kind = self._plan(a, b.dtype)
         ncol, ldb, ldc = b.ncol, b.leaddim, out.leaddim
         bp, cp = b.data, out.data
+        if kind == 2:
+            self._tune(lib.mul_plan_tune, plan, ncol, bp, ldb)
         k = Kernel(lambda stream, alpha=alpha, beta=beta: lib.mul(
             plan, ncol, bp, ldb, cp, ldc, alpha, beta, stream),
             rtargs={'alpha': alpha, 'beta': beta}, mats=(a, b, out),
@@ -459,10 +484,71 @@ class MulProvider:
         k.kind = 'structured' if kind == 1 else 'dense'
         return k
 
+    # -- fused two-operator applications (no reference counterpart: they fuse
+    #    kernels the reference runs back to back, see include/frhip.h)
+    def _plan2(self, a0, a1, dtype):
+        key = (id(a0), id(a1), dtype)
+        if key not in self._plans2:
+            h0 = np.ascontiguousarray(a0._host, dtype=np.float64)
+            h1 = np.ascontiguousarray(a1._host, dtype=np.float64)
+            if h0.shape[0] != h1.shape[0]:
+                raise ValueError('Operators must have the same row count')
+            plan, kind = ct.c_void_p(), ct.c_int()
+            self.backend.lib.mul2_plan_create(
+                ct.byref(plan), h0.shape[0], h0.shape[1], h1.shape[1],
+                h0.ctypes.data, h1.ctypes.data, _dtcode(dtype),
+                ct.byref(kind))
+            self._plans2[key] = (plan.value, kind.value, a0, a1)
+        return self._plans2[key][:2]
+
+    def _fused(self, a0, a1, b0, b1, out, ep, rcpdjac=None, xa=None):
+        for a in (a0, a1):
+            if 'const' not in a.tags:
+                raise NotSuitableError('fused mul requires constant operators')
+        if (a0.ncol != b0.nrow or a1.ncol != b1.nrow or
+                a0.nrow != out.nrow or b0.ncol != out.ncol or
+                b1.ncol != out.ncol or b0.leaddim != b1.leaddim or
+                b0.dtype != out.dtype or b1.dtype != out.dtype):
+            raise ValueError('Incompatible matrices for the fused mul')
+        plan, kind = self._plan2(a0, a1, out.dtype)
+        if out.ncol % (64*5):
+            raise NotSuitableError('fused kernels need 5-variable AoSoA data')
+        lib = self.backend.lib
+        ncol = out.ncol
+        if kind == 2:
+            self._tune(lib.mul2_plan_tune, plan, ncol, b0.data, b0.leaddim,
+                       b1.data, b1.leaddim)
+        args = (plan, ncol, b0.data, b0.leaddim, b1.data, b1.leaddim,
+                out.data, out.leaddim, ep,
+                None if rcpdjac is None else rcpdjac.data,
+                0 if rcpdjac is None else _rstride(rcpdjac),
+                None if xa is None else xa.data)
+
+        def fn(stream, dta=0.0, dtb=0.0, last=0):
+            lib.mul2(*args, dta, dtb, int(last), stream)
+        k = Kernel(fn, rtargs={'dta': 0.0, 'dtb': 0.0, 'last': 0},
+                   mats=(a0, a1, b0, b1, out), name=f'mul2_ep{ep}')
+        k.kind = 'structured' if kind == 1 else 'dense'
+        return k
+
+    def mul2(self, a0, a1, b0, b1, out):
+        """out = a0 @ b0 + a1 @ b1 (tgradpcoru + tgradcoru)."""
+        return self._fused(a0, a1, b0, b1, out, 0)
+
+    def tdivf(self, a0, a1, b0, b1, out, rcpdjac, xa=None, stage=None):
+        """tdivtpcorf + tdivtconf + negdivconf, optionally followed by the
+        rkvdh2 stage update (stage='rk45': xa = r1, out = r2; 'rk45-mixed':
+        xa = FP64 accumulated solution, out = FP32 stage input).  The stage
+        coefficients are bound at run time: bind(dta=, dtb=, last=)."""
+        ep = {None: 1, 'rk45': 2, 'rk45-mixed': 3}[stage]
+        return self._fused(a0, a1, b0, b1, out, ep, rcpdjac, xa)
+
     def __del__(self):
         try:
             for plan, _, _ in self._plans.values():
                 self.backend.lib.mul_plan_destroy(plan)
+            for plan, _, _, _ in self._plans2.values():
+                self.backend.lib.mul2_plan_destroy(plan)
         except Exception:
             pass
 
@@ -542,7 +628,9 @@ class PointwiseProvider:
         lib, be = self.backend.lib, self.backend
         npts, neles = dims
         ktype = tplargs['ktype']
-        linear = 'linear' in ktype
+        # geometry mode: 1 hex vertices, 2 one S per element, 0 stored S
+        mode = 1 if 'linear' in ktype else 2 if 'affine' in ktype else 0
+        linear = mode == 1
         dt = _dtcode(u.dtype)
         geo = verts if linear else smats
         uptsp = _ptr(upts) if linear else None
@@ -554,13 +642,13 @@ class PointwiseProvider:
                 1.0 if tplargs.get('visc_corr') == 'sutherland' else 0.0,
                 c.get('cpTref', 1.0), c.get('cpTs', 0.0))
             rp = None if linear else _ptr(rcpdjac)
-            ldr = 0 if linear else rcpdjac.leaddim
-            args = (dt, int(linear), npts, neles, u.data, u.leaddim, f.data,
+            ldr = 0 if linear else _rstride(rcpdjac)
+            args = (dt, mode, npts, neles, u.data, u.leaddim, f.data,
                     f.leaddim, gradu.data, gradu.leaddim, geo.data,
                     geo.leaddim, uptsp, rp, ldr, consts, None)
             return Kernel(lambda stream: lib.tflux_ns(*args, stream),
                           mats=(u, f, gradu, geo), name='tflux_ns')
-        args = (dt, int(linear), npts, neles, u.data, u.leaddim, f.data,
+        args = (dt, mode, npts, neles, u.data, u.leaddim, f.data,
                 f.leaddim, geo.data, geo.leaddim, uptsp, c['gamma'])
         return Kernel(lambda stream: lib.tflux_euler(*args, stream),
                       mats=(u, f, geo), name='tflux')
@@ -573,7 +661,7 @@ class PointwiseProvider:
         lib = self.backend.lib
         npts, neles = dims
         args = (_dtcode(tdivtconf.dtype), npts, neles, tdivtconf.data,
-                tdivtconf.leaddim, rcpdjac.data, rcpdjac.leaddim)
+                tdivtconf.leaddim, rcpdjac.data, _rstride(rcpdjac))
         return Kernel(lambda stream: lib.negdivconf(*args, stream),
                       mats=(tdivtconf, rcpdjac), name='negdivconf')
 
@@ -675,6 +763,10 @@ class AMDBackend:
             raise ValueError('Only the normal memory model is supported')
         self.ixdtype = np.int32
         self.use_graphs = cfg.getbool('backend-hip', 'graphs', True)
+        # fuse element-local operator chains where a fused kernel exists
+        self.fused = cfg.getbool('backend-hip', 'fused', True)
+        # time the variants of dense operator plans at kernel construction
+        self.autotune = cfg.getbool('backend-hip', 'autotune', True)
 
         self.lib = get_lib()
         self.lib.init(cfg.getint('backend-hip', 'device-id', 0))

@@ -13,7 +13,8 @@ CSRC = os.path.join(HERE, 'csrc')
 BUILD = os.path.join(CSRC, '_build')
 LIB = os.path.join(HERE, 'libfrhip.so')
 
-SOURCES = ['frhip_runtime.hip', 'frhip_ops.hip', 'frhip_pointwise.hip']
+SOURCES = ['frhip_runtime.hip', 'frhip_ops.hip', 'frhip_dense.hip',
+           'frhip_pointwise.hip']
 
 HIPFLAGS = [
     '--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC', '-shared',
@@ -31,12 +32,16 @@ def hipcc():
 
 def build(verbose=False, jobs=8):
     os.makedirs(BUILD, exist_ok=True)
-    gen = os.path.join(BUILD, 'gen_ops.hip')
-    subprocess.run([sys.executable, os.path.join(CSRC, 'gen_ops.py'), gen],
+    for f in os.listdir(BUILD):
+        if f.startswith('gen_ops'):
+            os.remove(os.path.join(BUILD, f))
+    subprocess.run([sys.executable, os.path.join(CSRC, 'gen_ops.py'), BUILD],
                    check=True)
+    gens = sorted(os.path.join(BUILD, f) for f in os.listdir(BUILD)
+                  if f.startswith('gen_ops') and f.endswith('.hip'))
 
     # Compile each translation unit to an object in parallel, then link.
-    srcs = [os.path.join(CSRC, s) for s in SOURCES] + [gen]
+    srcs = [os.path.join(CSRC, s) for s in SOURCES] + gens
     objs, procs = [], []
     for s in srcs:
         o = os.path.join(BUILD, os.path.basename(s) + '.o')

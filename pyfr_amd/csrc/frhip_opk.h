@@ -45,4 +45,25 @@ struct OpKernelEntry {
 extern const OpKernelEntry g_opk_table[];
 extern const int g_opk_count;
 
+using FKD = void (*)(int, const double *, long, const double *, long,
+                     double *, long, const double *, long, void *, double,
+                     double, int);
+using FKF = void (*)(int, const float *, long, const float *, long, float *,
+                     long, const float *, long, void *, double, double, int);
+
+// Fused two-input kernels: C = [A0 | A1] @ [B0; B1] with an epilogue
+// (0: store, 1: negdivconf, 2: + RK45 stage, 3: + mixed RK45 stage,
+// 4 / 5: the last stage of 2 / 3).  fd[3], fd[5] are null (FP32 only).
+struct FusedKernelEntry {
+    int m, k0, k1, nnz;
+    const int32_t *lin;      // row-major indices into the m x (k0 + k1) A
+    const double *vals;
+    const char *desc;
+    FKD fd[6];
+    FKF ff[6];
+};
+
+extern const FusedKernelEntry g_fk_table[];
+extern const int g_fk_count;
+
 }  // namespace frhip

@@ -215,8 +215,47 @@ static dim3 elem_grid(int npts, int neles)
     return dim3(gx, gy);
 }
 
+// Metric terms S at point pt of element e in geometry mode GEO (the
+// FRHIP_GEO_* modes): V holds a linear hex's vertices (GEO 1) or the
+// element's constant S (GEO 2).
+template <typename T, int GEO>
+__device__ __forceinline__ void load_geo(const T *__restrict__ geo, long ldg,
+                                         int e, T V[8][3])
+{
+    if constexpr (GEO == 1) {
+        for (int v = 0; v < 8; ++v)
+            for (int d = 0; d < 3; ++d)
+                V[v][d] = geo[v*ldg + soa_ix(e, d, 3)];
+    }
+    else if constexpr (GEO == 2) {
+        for (int i = 0; i < 3; ++i)
+            for (int k = 0; k < 3; ++k)
+                V[i][k] = geo[(3*i + k)*ldg + e];
+    }
+}
+
+template <typename T, int GEO>
+__device__ __forceinline__ void smats_at(const T *__restrict__ geo, long ldg,
+                                         int e, int pt, int npts,
+                                         const T *__restrict__ upts,
+                                         const T V[8][3], T s[3][3], T &djac)
+{
+    if constexpr (GEO == 1) {
+        const T x[3] = {upts[3*pt], upts[3*pt + 1], upts[3*pt + 2]};
+        hex_smats(V, x, s, djac);
+    }
+    else if constexpr (GEO == 2) {
+        for (int i = 0; i < 3; ++i)
+            for (int k = 0; k < 3; ++k) s[i][k] = V[i][k];
+    }
+    else
+        for (int i = 0; i < 3; ++i)
+            for (int k = 0; k < 3; ++k)
+                s[i][k] = geo[((long)i*npts + pt)*ldg + soa_ix(e, k, 3)];
+}
+
 // tflux (Euler); pyfr/solvers/euler/kernels/tflux.mako:8-30.
-template <typename T, bool LIN>
+template <typename T, int GEO>
 __global__ __launch_bounds__(256) void k_tflux_euler(
     int npts, int neles, const T *__restrict__ u, long ldu, T *__restrict__ f,
     long ldf, const T *__restrict__ geo, long ldg, const T *__restrict__ upts,
@@ -226,21 +265,11 @@ __global__ __launch_bounds__(256) void k_tflux_euler(
     if (e >= neles) return;
 
     T V[8][3];
-    if (LIN)
-        for (int v = 0; v < 8; ++v)
-            for (int d = 0; d < 3; ++d)
-                V[v][d] = geo[v*ldg + soa_ix(e, d, 3)];
+    load_geo<T, GEO>(geo, ldg, e, V);
 
     for (int pt = blockIdx.y*4 + threadIdx.y; pt < npts; pt += 4*gridDim.y) {
         T s[3][3], djac;
-        if (LIN) {
-            const T x[3] = {upts[3*pt], upts[3*pt + 1], upts[3*pt + 2]};
-            hex_smats(V, x, s, djac);
-        }
-        else
-            for (int i = 0; i < 3; ++i)
-                for (int k = 0; k < 3; ++k)
-                    s[i][k] = geo[((long)i*npts + pt)*ldg + soa_ix(e, k, 3)];
+        smats_at<T, GEO>(geo, ldg, e, pt, npts, upts, V, s, djac);
 
         T uu[5];
         for (int v = 0; v < 5; ++v) uu[v] = u[pt*ldu + soa_ix(e, v, 5)];
@@ -254,7 +283,7 @@ __global__ __launch_bounds__(256) void k_tflux_euler(
 }
 
 // tflux (NS, gradient-fused); pyfr/solvers/navstokes/kernels/tflux.mako:13-50.
-template <typename T, bool LIN>
+template <typename T, int GEO>
 __global__ __launch_bounds__(256) void k_tflux_ns(
     int npts, int neles, const T *__restrict__ u, long ldu, T *__restrict__ f,
     long ldf, T *__restrict__ gradu, long ldgr, const T *__restrict__ geo,
@@ -266,25 +295,15 @@ __global__ __launch_bounds__(256) void k_tflux_ns(
     const NSConsts<T> c = mu_e ? c0.with_mu(mu_e[e]) : c0;
 
     T V[8][3];
-    if (LIN)
-        for (int v = 0; v < 8; ++v)
-            for (int d = 0; d < 3; ++d)
-                V[v][d] = geo[v*ldg + soa_ix(e, d, 3)];
+    load_geo<T, GEO>(geo, ldg, e, V);
 
     for (int pt = blockIdx.y*4 + threadIdx.y; pt < npts; pt += 4*gridDim.y) {
-        T s[3][3], rcpd;
-        if (LIN) {
-            const T x[3] = {upts[3*pt], upts[3*pt + 1], upts[3*pt + 2]};
-            T djac;
-            hex_smats(V, x, s, djac);
+        T s[3][3], rcpd, djac;
+        smats_at<T, GEO>(geo, ldg, e, pt, npts, upts, V, s, djac);
+        if constexpr (GEO == 1)
             rcpd = T(1)/djac;
-        }
-        else {
-            for (int i = 0; i < 3; ++i)
-                for (int k = 0; k < 3; ++k)
-                    s[i][k] = geo[((long)i*npts + pt)*ldg + soa_ix(e, k, 3)];
+        else
             rcpd = rcpdjac[pt*ldr + e];
-        }
 
         T uu[5], g[3][5], tg[3][5];
         for (int v = 0; v < 5; ++v) uu[v] = u[pt*ldu + soa_ix(e, v, 5)];
@@ -679,34 +698,28 @@ int frhip_tflux_euler(int dtype, int linear, int npts, int neles,
     FR_API_BEGIN
     FR_DTYPE_OK(dtype);
     FR_CHECK(npts > 0 && neles >= 0, "invalid extents");
+    FR_CHECK(linear >= 0 && linear <= 2, "invalid geometry mode");
     if (neles == 0) return 0;
-    FR_CHECK(!linear || upts, "linear tflux needs upts");
+    FR_CHECK(linear != FRHIP_GEO_HEX_LINEAR || upts,
+             "linear tflux needs upts");
     dim3 grid = elem_grid(npts, neles), block(64, 4);
     hipStream_t s = as_stream(stream);
+#define FR_LAUNCH_TF(T, G)                                                  \
+    hipLaunchKernelGGL((k_tflux_euler<T, G>), grid, block, 0, s, npts,      \
+                       neles, (const T *)u, ldu, (T *)f, ldf,               \
+                       (const T *)geo, ldgeo, (const T *)upts,              \
+                       (T)(gamma - 1))
     if (dtype == FRHIP_F64) {
-        if (linear)
-            hipLaunchKernelGGL((k_tflux_euler<double, true>), grid, block, 0, s,
-                               npts, neles, (const double *)u, ldu, (double *)f,
-                               ldf, (const double *)geo, ldgeo,
-                               (const double *)upts, gamma - 1);
-        else
-            hipLaunchKernelGGL((k_tflux_euler<double, false>), grid, block, 0,
-                               s, npts, neles, (const double *)u, ldu,
-                               (double *)f, ldf, (const double *)geo, ldgeo,
-                               (const double *)upts, gamma - 1);
+        if (linear == 0) FR_LAUNCH_TF(double, 0);
+        else if (linear == 1) FR_LAUNCH_TF(double, 1);
+        else FR_LAUNCH_TF(double, 2);
     }
     else {
-        if (linear)
-            hipLaunchKernelGGL((k_tflux_euler<float, true>), grid, block, 0, s,
-                               npts, neles, (const float *)u, ldu, (float *)f,
-                               ldf, (const float *)geo, ldgeo,
-                               (const float *)upts, (float)(gamma - 1));
-        else
-            hipLaunchKernelGGL((k_tflux_euler<float, false>), grid, block, 0,
-                               s, npts, neles, (const float *)u, ldu,
-                               (float *)f, ldf, (const float *)geo, ldgeo,
-                               (const float *)upts, (float)(gamma - 1));
+        if (linear == 0) FR_LAUNCH_TF(float, 0);
+        else if (linear == 1) FR_LAUNCH_TF(float, 1);
+        else FR_LAUNCH_TF(float, 2);
     }
+#undef FR_LAUNCH_TF
     FR_LAUNCHED();
     return 0;
     FR_API_END
@@ -721,29 +734,31 @@ int frhip_tflux_ns(int dtype, int linear, int npts, int neles,
     FR_API_BEGIN
     FR_DTYPE_OK(dtype);
     FR_CHECK(npts > 0 && neles >= 0 && consts, "invalid arguments");
+    FR_CHECK(linear >= 0 && linear <= 2, "invalid geometry mode");
     if (neles == 0) return 0;
-    FR_CHECK(linear ? upts != nullptr : rcpdjac != nullptr,
+    FR_CHECK(linear == FRHIP_GEO_HEX_LINEAR ? upts != nullptr
+                                            : rcpdjac != nullptr,
              "missing geometry argument");
     dim3 grid = elem_grid(npts, neles), block(64, 4);
     hipStream_t s = as_stream(stream);
+#define FR_LAUNCH_NS(T, G)                                                  \
+    hipLaunchKernelGGL((k_tflux_ns<T, G>), grid, block, 0, s, npts, neles,  \
+                       (const T *)u, ldu, (T *)f, ldf, (T *)gradu, ldg,     \
+                       (const T *)geo, ldgeo, (const T *)upts,              \
+                       (const T *)rcpdjac, ldr, c, (const T *)mu_e)
     if (dtype == FRHIP_F64) {
         auto c = ns_consts<double>(consts);
-        auto k = linear ? k_tflux_ns<double, true> : k_tflux_ns<double, false>;
-        hipLaunchKernelGGL(k, grid, block, 0, s, npts, neles,
-                           (const double *)u, ldu, (double *)f, ldf,
-                           (double *)gradu, ldg, (const double *)geo, ldgeo,
-                           (const double *)upts, (const double *)rcpdjac, ldr,
-                           c, (const double *)mu_e);
+        if (linear == 0) FR_LAUNCH_NS(double, 0);
+        else if (linear == 1) FR_LAUNCH_NS(double, 1);
+        else FR_LAUNCH_NS(double, 2);
     }
     else {
         auto c = ns_consts<float>(consts);
-        auto k = linear ? k_tflux_ns<float, true> : k_tflux_ns<float, false>;
-        hipLaunchKernelGGL(k, grid, block, 0, s, npts, neles,
-                           (const float *)u, ldu, (float *)f, ldf,
-                           (float *)gradu, ldg, (const float *)geo, ldgeo,
-                           (const float *)upts, (const float *)rcpdjac, ldr,
-                           c, (const float *)mu_e);
+        if (linear == 0) FR_LAUNCH_NS(float, 0);
+        else if (linear == 1) FR_LAUNCH_NS(float, 1);
+        else FR_LAUNCH_NS(float, 2);
     }
+#undef FR_LAUNCH_NS
     FR_LAUNCHED();
     return 0;
     FR_API_END

@@ -21,7 +21,7 @@ element-major AoSoA layout), so every B and C access is a fully coalesced
     (column-oriented).
 Both accumulate each output over the columns of A in ascending order.
 
-Usage: python gen_ops.py OUTFILE
+Usage: python gen_ops.py OUTDIR
 """
 import hashlib
 import os
@@ -49,6 +49,21 @@ def lit(x):
     return f'T({float(x).hex()})'
 
 
+def pipelined(batches, loads, fmas, L):
+    """Software-pipelined k loop: the loads of batch q + 1 are issued before
+    the FMAs of batch q.  Scheduling barriers keep the compiler from hoisting
+    every load of the loop to the top, which spills the accumulators."""
+    if not batches:
+        return
+    loads(batches[0])
+    for q, ks in enumerate(batches):
+        if q + 1 < len(batches):
+            loads(batches[q + 1])
+        L.append('    __builtin_amdgcn_sched_barrier(0);')
+        fmas(ks)
+        L.append('    __builtin_amdgcn_sched_barrier(0);')
+
+
 def emit_kernel(name, A):
     m, k = A.shape
     rows, cols = np.nonzero(A)          # row-major order of nonzeros
@@ -68,7 +83,7 @@ def emit_kernel(name, A):
     # Loads are issued in batches so that each wavefront keeps many memory
     # requests in flight: all B inputs up front (row-oriented) or GB at a time
     # (column-oriented), and C (read-modify-write modes) GC rows at a time.
-    GB, GC = 16, 8
+    GB, GC = 8, 8
 
     def store_rows(rs, accs):
         # rs: rows, accs: expressions holding their accumulated values
@@ -104,10 +119,13 @@ def emit_kernel(name, A):
         seen = set()
         for r in range(m):
             L.append(f'    T a{r} = T(0);')
-        for q0 in range(0, len(usedk), GB):
-            ks = usedk[q0:q0 + GB]
+        batches = [usedk[q0:q0 + GB] for q0 in range(0, len(usedk), GB)]
+
+        def loads(ks):
             for kk in ks:
                 L.append(f'    const T b{kk} = ldg(b + {kk}*ldb);')
+
+        def fmas(ks):
             for kk in ks:
                 for r in rows[cols == kk]:
                     if r in seen:
@@ -115,16 +133,137 @@ def emit_kernel(name, A):
                     else:
                         L.append(f'    a{r} = {vid[(r, kk)]}*b{kk};')
                         seen.add(r)
+        pipelined(batches, loads, fmas, L)
         store_rows(list(range(m)), [f'a{r}' for r in range(m)])
     L.append('}')
     return '\n'.join(L), rows*k + cols, nnz, A[rows, cols]
 
 
-def main(outfile):
-    parts = ['// Generated by gen_ops.py -- structured FR operator kernels.',
-             '#include "frhip_opk.h"', '', 'namespace frhip {', '']
-    table = []
+# Fused two-input kernels.  C = A0 @ B0 + A1 @ B1, then an epilogue:
+#   EP 0: c = acc
+#   EP 1: c = -rcp*acc                          (negdivconf)
+#   EP 2: t = -rcp*acc; RK45 stage on r1 (xa) and r2 (c) in T
+#   EP 3: t = -rcp*acc in float; RK45 stage with FP64 accumulated solution
+#         (xa) and FP32 next-stage input (c)
+#   EP 4, 5: the last stage of EP 2, 3 (r1 += dtb*t; EP 5 also stores the
+#         new solution rounded to float into c)
+# The last stage is a template value, not a run-time flag: a uniform branch
+# per output row makes the compiler keep every row's RK operands live and
+# spill.  The trailing int argument is unused.
+# Used for tgradpcoru + tgradcoru (EP 0: [M4 - M6*M0 | M6] on [u; comm]) and
+# tdivtpcorf + tdivtconf + negdivconf (+ rkvdh2) (EP 1-3: [M1 - M3*M2 | M3]
+# on [F; fn]).  The fused ops follow the element-local grouping the
+# reference's OpenMP backend applies (pyfr/solvers/baseadvec/system.py:126-132,
+# pyfr/solvers/baseadvecdiff/system.py:174-203).
+FUSED = [('grad', ['M4 - M6*M0', 'M6']), ('div', ['M1 - M3*M2', 'M3'])]
+FUSED_EPS = {'double': [0, 1, 2, 4], 'float': [0, 1, 2, 3, 4, 5]}
+
+
+def emit_fused(name, A0, A1):
+    m, k0 = A0.shape
+    k1 = A1.shape[1]
+    A = np.hstack([A0, A1])
+    rows, cols = np.nonzero(A)
+    vid = {(r, c): lit(A[r, c]) for r, c in zip(rows, cols)}
+    L = []
+    L.append('template <typename T, int EP>')
+    L.append(f'__global__ __launch_bounds__(256) void {name}('
+             'int ncol, const T *__restrict__ b0, long ld0, '
+             'const T *__restrict__ b1, long ld1, T *__restrict__ c, '
+             'long ldc, const T *__restrict__ rcp, long ldr, '
+             'void *__restrict__ xa, double dta, double dtb, int last)')
+    L.append('{')
+    L.append('    const long j = (long)blockIdx.x*blockDim.x + threadIdx.x;')
+    L.append('    if (j >= ncol) return;')
+    L.append('    b0 += j; b1 += j; c += j;')
+    L.append('    const long e = (j/(64*5))*64 + (j & 63);')
+
+    def bexpr(kk):
+        return (f'ldg(b0 + {kk}*ld0)' if kk < k0 else
+                f'ldg(b1 + {kk - k0}*ld1)')
+
+    def epilogue(r, a):
+        L.append('    {')
+        L.append(f'      const T acc = {a};')
+        L.append('      if constexpr (EP == 0) {')
+        L.append(f'        __builtin_nontemporal_store(acc, c + {r}*ldc);')
+        L.append('      } else {')
+        L.append(f'        const T t = -rcp[{r}*ldr + e]*acc;')
+        L.append('        if constexpr (EP == 1) {')
+        L.append(f'          c[{r}*ldc] = t;')
+        L.append('        } else if constexpr (EP == 2) {')
+        L.append(f'          T *r1 = static_cast<T *>(xa) + j + {r}*ldc;')
+        L.append('          const T t1 = *r1;')
+        L.append('          *r1 = t1 + T(dta)*t;')
+        L.append(f'          c[{r}*ldc] = t1 + T(dtb)*t;')
+        L.append('        } else if constexpr (EP == 4) {')
+        L.append(f'          T *r1 = static_cast<T *>(xa) + j + {r}*ldc;')
+        L.append('          *r1 = *r1 + T(dtb)*t;')
+        L.append('        } else {')
+        L.append(f'          double *r1 = static_cast<double *>(xa) + j + '
+                 f'{r}*ldc;')
+        L.append('          const double t1 = *r1, td = (double)t;')
+        L.append('          const double na = t1 + dtb*td;')
+        L.append('          *r1 = na;')
+        L.append(f'          c[{r}*ldc] = EP == 5 ? (T)na : (T)(t1 + dta*td);')
+        L.append('        }')
+        L.append('      }')
+        L.append('    }')
+
+    usedk = sorted(set(cols.tolist()))
+    GB = 8
+    if len(usedk) <= m:
+        for kk in usedk:
+            L.append(f'    const T v{kk} = {bexpr(kk)};')
+        for r in range(m):
+            cs = cols[rows == r]
+            if len(cs) == 0:
+                epilogue(r, 'T(0)')
+                continue
+            terms = f'{vid[(r, cs[0])]}*v{cs[0]}'
+            for cc in cs[1:]:
+                terms = f'fma({vid[(r, cc)]}, v{cc}, {terms})'
+            epilogue(r, terms)
+            if r % 8 == 7:
+                L.append('    __builtin_amdgcn_sched_barrier(0);')
+    else:
+        seen = set()
+        for r in range(m):
+            L.append(f'    T a{r} = T(0);')
+        batches = [usedk[q0:q0 + GB] for q0 in range(0, len(usedk), GB)]
+
+        def loads(ks):
+            for kk in ks:
+                L.append(f'    const T v{kk} = {bexpr(kk)};')
+
+        def fmas(ks):
+            for kk in ks:
+                for r in rows[cols == kk]:
+                    if r in seen:
+                        L.append(f'    a{r} = fma({vid[(r, kk)]}, v{kk}, '
+                                 f'a{r});')
+                    else:
+                        L.append(f'    a{r} = {vid[(r, kk)]}*v{kk};')
+                        seen.add(r)
+        pipelined(batches, loads, fmas, L)
+        # the epilogue's own loads (1/|J|, RK registers) go 8 rows at a time
+        for r in range(m):
+            epilogue(r, f'a{r}')
+            if r % 8 == 7:
+                L.append('    __builtin_amdgcn_sched_barrier(0);')
+    L.append('}')
+    return '\n'.join(L), rows*(k0 + k1) + cols, len(rows), A[rows, cols]
+
+
+def main(outdir):
+    """Write one translation unit per order (gen_ops_p<k>.hip, compiled in
+    parallel) and the kernel tables (gen_ops_table.hip)."""
+    head = ['// Generated by gen_ops.py -- structured FR operator kernels.',
+            '#include "frhip_opk.h"', '', 'namespace frhip {', '']
+    table, ftable, decls = [], [], []
+    files = []
     for p in ORDERS:
+        parts = list(head)
         sh = HexShape(p)
         for oi, op in enumerate(OPS):
             A = sh.opmat(op)
@@ -133,27 +272,82 @@ def main(outfile):
             parts.append(f'// hex p={p} {op}: {A.shape[0]}x{A.shape[1]}, '
                          f'nnz={nnz}')
             parts.append(src)
-            linname = f'lin_{name}'
-            parts.append(f'static const int32_t {linname}[{nnz}] = {{' +
+            parts.append(f'extern const int32_t lin_{name}[{nnz}] = {{' +
                          ','.join(str(int(x)) for x in lin) + '};')
-            parts.append(f'static const double val_{name}[{nnz}] = {{' +
+            parts.append(f'extern const double val_{name}[{nnz}] = {{' +
                          ','.join(float(x).hex() for x in vals) + '};')
+            for bm in range(4):
+                for t in ('double', 'float'):
+                    parts.append(f'template __global__ void {name}<{t}, {bm}>'
+                                 f'(int, const {t} *, long, {t} *, long, '
+                                 f'const {t} *, {t}, {t});')
             parts.append('')
             h = pattern_hash(A.shape[0], A.shape[1], lin)
-            table.append((name, A.shape[0], A.shape[1], nnz, h, linname,
+            table.append((name, A.shape[0], A.shape[1], nnz, h,
                           f'hex p={p} {op}'))
-    parts.append('const OpKernelEntry g_opk_table[] = {')
-    for name, m, k, nnz, h, linname, desc in table:
+            decls.append(f'template <typename T, int BM> __global__ void '
+                         f'{name}(int, const T *, long, T *, long, '
+                         f'const T *, T, T);')
+            decls.append(f'extern const int32_t lin_{name}[];')
+            decls.append(f'extern const double val_{name}[];')
+        for fname, (op0, op1) in FUSED:
+            A0, A1 = sh.opmat(op0), sh.opmat(op1)
+            name = f'fk_hex{p}_{fname}'
+            src, lin, nnz, vals = emit_fused(name, A0, A1)
+            parts.append(f'// hex p={p} fused {fname}: [{op0} | {op1}] '
+                         f'{A0.shape[0]}x({A0.shape[1]}+{A1.shape[1]}), '
+                         f'nnz={nnz}')
+            parts.append(src)
+            parts.append(f'extern const int32_t lin_{name}[{nnz}] = {{' +
+                         ','.join(str(int(x)) for x in lin) + '};')
+            parts.append(f'extern const double val_{name}[{nnz}] = {{' +
+                         ','.join(float(x).hex() for x in vals) + '};')
+            for t in ('double', 'float'):
+                for ep in FUSED_EPS[t]:
+                    parts.append(f'template __global__ void {name}<{t}, {ep}>'
+                                 f'(int, const {t} *, long, const {t} *, '
+                                 f'long, {t} *, long, const {t} *, long, '
+                                 f'void *, double, double, int);')
+            parts.append('')
+            ftable.append((name, A0.shape[0], A0.shape[1], A1.shape[1], nnz,
+                           f'hex p={p} fused {fname}'))
+            decls.append(f'template <typename T, int EP> __global__ void '
+                         f'{name}(int, const T *, long, const T *, long, T *, '
+                         f'long, const T *, long, void *, double, double, '
+                         f'int);')
+            decls.append(f'extern const int32_t lin_{name}[];')
+            decls.append(f'extern const double val_{name}[];')
+        parts.append('}  // namespace frhip')
+        fn = os.path.join(outdir, f'gen_ops_p{p}.hip')
+        with open(fn, 'w') as f:
+            f.write('\n'.join(parts) + '\n')
+        files.append(fn)
+
+    t = list(head) + decls + ['']
+    t.append('const FusedKernelEntry g_fk_table[] = {')
+    for name, m, k0, k1, nnz, desc in ftable:
+        fnd = ', '.join(f'&{name}<double, {ep}>'
+                        if ep in FUSED_EPS['double'] else 'nullptr'
+                        for ep in range(6))
+        fnf = ', '.join(f'&{name}<float, {ep}>' for ep in range(6))
+        t.append(f'    {{{m}, {k0}, {k1}, {nnz}, lin_{name}, val_{name}, '
+                 f'"{desc}", {{{fnd}}}, {{{fnf}}}}},')
+    t.append('};')
+    t.append(f'const int g_fk_count = {len(ftable)};')
+    t.append('const OpKernelEntry g_opk_table[] = {')
+    for name, m, k, nnz, h, desc in table:
         fnd = ', '.join(f'&{name}<double, {bm}>' for bm in range(4))
         fnf = ', '.join(f'&{name}<float, {bm}>' for bm in range(4))
-        parts.append(f'    {{{m}, {k}, {nnz}, {h}ULL, {linname}, '
-                     f'val_{name}, "{desc}", {{{fnd}}}, {{{fnf}}}}},')
-    parts.append('};')
-    parts.append('const int g_opk_count = '
-                 f'{len(table)};')
-    parts.append('}  // namespace frhip')
-    with open(outfile, 'w') as f:
-        f.write('\n'.join(parts) + '\n')
+        t.append(f'    {{{m}, {k}, {nnz}, {h}ULL, lin_{name}, val_{name}, '
+                 f'"{desc}", {{{fnd}}}, {{{fnf}}}}},')
+    t.append('};')
+    t.append(f'const int g_opk_count = {len(table)};')
+    t.append('}  // namespace frhip')
+    fn = os.path.join(outdir, 'gen_ops_table.hip')
+    with open(fn, 'w') as f:
+        f.write('\n'.join(t) + '\n')
+    files.append(fn)
+    return files
 
 
 if __name__ == '__main__':

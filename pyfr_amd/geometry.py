@@ -146,3 +146,58 @@ class HexGeometry:
         if np.any(np.sqrt(np.einsum('...i,...i', pn, pn)) < 1e-10):
             raise RuntimeError('Zero face normals detected')
         return pn
+
+
+class AffineGeometry:
+    """Metric terms of straight-sided tetrahedra and right prisms, whose
+    reference-to-physical maps are affine: the reference's metric pipeline
+    (``elements.py:331-440``) then yields one S = |J| J^-1 and |J| per
+    element at every point.
+
+    eles: (nverts, neles, 3) in ``std_ele(1)`` vertex order; vertices 1, 2,
+    3 are the neighbours of vertex 0 along the three reference axes for both
+    shapes, so dX/dxi_a = (v_{a+1} - v_0)/2.
+    """
+
+    def __init__(self, eles, name):
+        v = np.asarray(eles, dtype=float)
+        self.name = name
+        self.v0 = v[0]                                   # (E, 3)
+        j = np.stack([(v[a + 1] - v[0])/2 for a in range(3)], axis=1)
+        if name == 'pri':
+            # a right prism: the top triangle is the bottom one translated
+            if (np.abs(v[4] - v[3] - (v[1] - v[0])).max() > 1e-12 or
+                    np.abs(v[5] - v[3] - (v[2] - v[0])).max() > 1e-12):
+                raise ValueError('prisms must be affine (right prisms)')
+        elif name != 'tet':
+            raise ValueError(f'no affine map for {name}')
+        self.jac = j                                     # (E, a, b)
+        s = np.empty_like(j)
+        for i, (a, b) in enumerate([(1, 2), (2, 0), (0, 1)]):
+            s[:, i] = np.cross(j[:, a], j[:, b])
+        self.smats = s                                   # (E, i, k)
+        self.djac = np.einsum('ek,ek->e', j[:, 0], s[:, 0])
+        if np.any(self.djac <= 0):
+            raise ValueError('Negative mesh Jacobians detected')
+
+    def ploc_at(self, pts):
+        """(npts, 3, E)"""
+        pts = np.atleast_2d(pts)
+        x = self.v0.T[None] + np.einsum('pa,eab->pbe', pts + 1, self.jac)
+        return x
+
+    def smat_at(self, pts):
+        """(3, npts, 3, E): S[i][k] at every point."""
+        n = len(np.atleast_2d(pts))
+        return np.broadcast_to(self.smats.transpose(1, 2, 0)[:, None],
+                               (3, n, 3, len(self.djac))).copy()
+
+    def rcpdjac_at(self, pts):
+        """(npts, E)"""
+        n = len(np.atleast_2d(pts))
+        return np.broadcast_to(1/self.djac, (n, len(self.djac))).copy()
+
+    def pnorm_at(self, pts, norm):
+        """Physical normals S^T n (``elements.py:364-380``): (npts, E, 3)."""
+        return np.einsum('pi,eik->pek', np.asarray(norm, dtype=float),
+                         self.smats)

@@ -122,3 +122,166 @@ def sorted_face_fpts_structured(plocfpts, facefpts, tol=1e-6):
             perm[e] = _fuzzysort(c[e], list(range(len(ffpts))), 0, tol)
         out.append(ffpts[perm])
     return out
+
+
+# Vertices of each face in TetShape / PriShape face order
+# (pyfr/shapes.py:476-570 with the vertex order of their std_ele(1))
+TET_FACE_VERTS = [(0, 1, 2), (0, 1, 3), (0, 2, 3), (1, 2, 3)]
+PRI_FACE_VERTS = [(0, 1, 2), (3, 4, 5), (0, 1, 4, 3), (1, 2, 5, 4),
+                  (0, 2, 5, 3)]
+FACE_VERTS = {'tet': TET_FACE_VERTS, 'pri': PRI_FACE_VERTS}
+
+
+def _kuhn_offsets():
+    """Lattice offsets of the 6 tets of a unit cube (Kuhn split along the
+    (0,0,0)-(1,1,1) diagonal), each ordered so that its Jacobian is
+    positive: v1 - v0, v2 - v0, v3 - v0 form a right-handed frame."""
+    out = []
+    for perm in [(0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1),
+                 (2, 1, 0)]:
+        p = [np.zeros(3, dtype=int)]
+        for ax in perm:
+            q = p[-1].copy()
+            q[ax] += 1
+            p.append(q)
+        if np.linalg.det(np.array([p[1] - p[0], p[2] - p[0],
+                                   p[3] - p[0]])) < 0:
+            p[1], p[2] = p[2], p[1]
+        out.append(np.array(p))
+    return np.array(out)                                # (6, 4, 3)
+
+
+def _prism_offsets():
+    """Two right prisms per cube, split along the (0,0)-(1,1) diagonal of
+    the xy faces (conforming with the Kuhn tets' z faces)."""
+    tris = [[(0, 0), (1, 0), (1, 1)], [(0, 0), (1, 1), (0, 1)]]
+    return np.array([[(x, y, z) for z in (0, 1) for (x, y) in t]
+                     for t in tris])                    # (2, 6, 3)
+
+
+CELL_SPLITS = {'tet': _kuhn_offsets(), 'pri': _prism_offsets()}
+
+
+class SimplexBoxMesh:
+    """Periodic boxes of straight-sided tetrahedra and prisms.
+
+    blocks: list of (shape (nx, ny, nz), spacing h, layers) where layers is
+    a list of (etype, nlayers) filling the block's z extent bottom to top;
+    every block is periodic on its own.  Cells split into 6 tets or 2
+    prisms (``CELL_SPLITS``); all splits share the cube diagonal, so faces
+    conform within and across element types.
+
+    ``spts``: {etype: (nverts, neles, 3)}; elements are ordered by block,
+    then cell (x fastest), then split index.  ``con``: (lhs, rhs) lists of
+    (etype, eidx, fidx), each face pair once, left side the smaller
+    (etype index, face index, element index) key, sorted by left side.
+    """
+    ndims = 3
+    etype_order = ('tet', 'pri')
+
+    def __init__(self, blocks):
+        verts = {et: [] for et in self.etype_order}
+        nodes = {et: [] for et in self.etype_order}
+        noff = 0
+        for (nx, ny, nz), h, layers in blocks:
+            if sum(n for _, n in layers) != nz:
+                raise ValueError('layers must fill the block')
+            k0 = 0
+            for et, nl in layers:
+                kk, jj, ii = np.meshgrid(np.arange(k0, k0 + nl),
+                                         np.arange(ny), np.arange(nx),
+                                         indexing='ij')
+                cell = np.stack([ii.ravel(), jj.ravel(), kk.ravel()], 1)
+                off = CELL_SPLITS[et]                   # (ns, nv, 3)
+                lat = cell[:, None, None, :] + off[None]   # (C, ns, nv, 3)
+                lat = lat.reshape(-1, off.shape[1], 3)     # (E, nv, 3)
+                verts[et].append(lat*h)
+                nodes[et].append((lat, np.full(len(lat), noff),
+                                  np.array([nx, ny, nz])))
+                k0 += nl
+            noff += 1
+        self.spts, self._nodes = {}, {}
+        for et in self.etype_order:
+            if verts[et]:
+                v = np.concatenate(verts[et])           # (E, nv, 3)
+                self.spts[et] = np.ascontiguousarray(v.transpose(1, 0, 2))
+                self._nodes[et] = nodes[et]
+        self.etypes = list(self.spts)
+        self.neles = sum(v.shape[1] for v in self.spts.values())
+        self.con = self._connect()
+
+    @classmethod
+    def stacked(cls, n, ntet, npri, length=2*np.pi):
+        """n x n x (ntet + npri) cells of size length/n: tets below, prisms
+        above, periodic in all three directions (tet-prism interfaces at
+        both ends of the prism slab)."""
+        layers = [(et, k) for et, k in (('tet', ntet), ('pri', npri)) if k]
+        return cls([((n, n, ntet + npri), length/n, layers)])
+
+    @classmethod
+    def disjoint(cls, ntet, npri, length=2*np.pi):
+        """A periodic box of ntet^3 tet cells and a separate periodic box of
+        npri^3 prism cells, both of edge ``length``."""
+        blocks = []
+        if ntet:
+            blocks.append(((ntet,)*3, length/ntet, [('tet', ntet)]))
+        if npri:
+            blocks.append(((npri,)*3, length/npri, [('pri', npri)]))
+        return cls(blocks)
+
+    def _connect(self):
+        """Pair faces through a structured key: (block, face kind, face
+        centroid in lattice units, times 12 so that triangle and quad
+        centroids are integers, modulo the block's periods)."""
+        keys, owners = [], []
+        for ti, et in enumerate(self.etypes):
+            E0 = 0
+            for lat, blk, dims in self._nodes[et]:
+                E = len(lat)
+                for f, fv in enumerate(FACE_VERTS[et]):
+                    c = lat[:, list(fv)].sum(axis=1)*(12//len(fv))
+                    k = np.empty((E, 5), dtype=np.int64)
+                    k[:, 0] = blk
+                    k[:, 1] = len(fv)
+                    k[:, 2:] = c % (12*dims)
+                    keys.append(k)
+                    owners.append(np.stack([np.full(E, ti), np.full(E, f),
+                                            E0 + np.arange(E)], 1))
+                E0 += E
+        keys, owners = np.concatenate(keys), np.concatenate(owners)
+        order = np.lexsort(keys.T[::-1])
+        sk = keys[order]
+        same = np.all(sk[1:] == sk[:-1], axis=1)
+        starts = np.flatnonzero(same)
+        if 2*len(starts) != len(keys) or np.any(same[1:] & same[:-1]):
+            raise ValueError('Faces are not all paired exactly once')
+        a, b = owners[order[starts]], owners[order[starts + 1]]
+        # left side = smaller (etype index, face, element) key
+        less = ((a[:, 0] < b[:, 0]) |
+                ((a[:, 0] == b[:, 0]) & ((a[:, 1] < b[:, 1]) |
+                 ((a[:, 1] == b[:, 1]) & (a[:, 2] < b[:, 2])))))
+        lft = np.where(less[:, None], a, b)
+        rgt = np.where(less[:, None], b, a)
+        srt = np.lexsort(lft.T[::-1])
+        lft, rgt = lft[srt], rgt[srt]
+        ets = self.etypes
+        lhs = [(ets[t], int(e), int(f)) for t, f, e in lft]
+        rhs = [(ets[t], int(e), int(f)) for t, f, e in rgt]
+        return lhs, rhs
+
+
+def sorted_face_fpts(plocfpts, facefpts, tol=1e-6):
+    """srtd_face_fpts for elements of any layout: elements whose face points
+    have the same relative layout (to 1e-9) share one fuzzy sort."""
+    out = []
+    for ffpts in facefpts:
+        ffpts = np.asarray(ffpts)
+        c = plocfpts[ffpts].transpose(1, 2, 0)           # (E, 3, nfp)
+        rel = c - c[:, :, :1]
+        q = np.round(rel/1e-9).astype(np.int64).reshape(len(c), -1)
+        uniq, first, inv = np.unique(q, axis=0, return_index=True,
+                                     return_inverse=True)
+        perms = np.array([_fuzzysort(rel[i], list(range(len(ffpts))), 0, tol)
+                          for i in first])
+        out.append(ffpts[perms[inv.ravel()]])
+    return out

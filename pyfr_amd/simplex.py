@@ -1,0 +1,324 @@
+"""FR operators for tetrahedra and prisms (dense operators).
+
+Shapes follow the reference's ``TetShape`` and ``PriShape``
+(``pyfr/shapes.py:476-570``): reference elements, face tables (projections and
+unnormalised reference normals), solution points (Shunn-Ham tets,
+Williams-Shunn x Gauss-Legendre prisms), flux points (Williams-Shunn
+triangles, Gauss-Legendre quads) and the operator layouts m0..m6
+(``pyfr/shapes.py:89-135``, ``:231-259``).
+
+The orthonormal bases are built here independently of the reference's Jacobi
+formulas: products of Legendre polynomials span the polynomial space
+(P_p for tets and triangles, P_p(tri) x P_p(line) for prisms) and are
+orthonormalised by Cholesky against an exact collapsed (Duffy) Gauss-Legendre
+quadrature.  The nodal operators and the correction functions (which use the
+space's reproducing kernel) do not depend on which orthonormal basis spans
+the space, so the result matches the reference's operators to rounding.
+"""
+from functools import cached_property
+import itertools as it
+
+import numpy as np
+
+from pyfr_amd.quadtables import TABLES
+from pyfr_amd.shapes import gauss_legendre
+
+
+def _leg(n, x):
+    """P_0..P_n and derivatives at x (arrays (n+1, len(x)))."""
+    x = np.asarray(x, dtype=float)
+    P = np.zeros((n + 1,) + x.shape)
+    D = np.zeros_like(P)
+    P[0] = 1.0
+    if n >= 1:
+        P[1] = x
+        D[1] = 1.0
+    for k in range(2, n + 1):
+        P[k] = ((2*k - 1)*x*P[k - 1] - (k - 1)*P[k - 2])/k
+        D[k] = D[k - 2] + (2*k - 1)*P[k - 1]
+    return P, D
+
+
+def _degrees(kind, p):
+    if kind == 'tri':
+        return [(i, j) for i in range(p + 1) for j in range(p + 1 - i)]
+    if kind == 'tet':
+        return [(i, j, k) for i in range(p + 1) for j in range(p + 1 - i)
+                for k in range(p + 1 - i - j)]
+    if kind == 'pri':
+        return [(i, j, k) for i in range(p + 1) for j in range(p + 1 - i)
+                for k in range(p + 1)]
+    if kind == 'quad':
+        return [(i, j) for i in range(p + 1) for j in range(p + 1)]
+    if kind == 'line':
+        return [(i,) for i in range(p + 1)]
+    raise ValueError(kind)
+
+
+def _collapsed_rule(kind, n):
+    """Exact (for degree <= 2n - 3 after the Duffy Jacobian) quadrature on
+    the reference element from tensor Gauss-Legendre points."""
+    g, w = gauss_legendre(n)
+    if kind == 'line':
+        return g[:, None], w
+    if kind == 'quad':
+        a, b = np.meshgrid(g, g, indexing='ij')
+        wa, wb = np.meshgrid(w, w, indexing='ij')
+        return np.stack([a.ravel(), b.ravel()], 1), (wa*wb).ravel()
+    if kind == 'tri':
+        a, b = np.meshgrid(g, g, indexing='ij')
+        wa, wb = np.meshgrid(w, w, indexing='ij')
+        a, b, wt = a.ravel(), b.ravel(), (wa*wb).ravel()
+        x = (1 + a)*(1 - b)/2 - 1
+        return np.stack([x, b], 1), wt*(1 - b)/2
+    if kind == 'tet':
+        a, b, c = np.meshgrid(g, g, g, indexing='ij')
+        wa, wb, wc = np.meshgrid(w, w, w, indexing='ij')
+        a, b, c = a.ravel(), b.ravel(), c.ravel()
+        wt = (wa*wb*wc).ravel()
+        y = (1 + b)*(1 - c)/2 - 1
+        x = (1 + a)*(1 - b)*(1 - c)/4 - 1
+        return np.stack([x, y, c], 1), wt*(1 - b)*(1 - c)**2/8
+    if kind == 'pri':
+        t, tw = _collapsed_rule('tri', n)
+        pts = np.array([(*tp, z) for z in g for tp in t])
+        wts = np.array([tw_*wz for wz in w for tw_ in tw])
+        return pts, wts
+    raise ValueError(kind)
+
+
+class OrthoBasis:
+    """Orthonormal basis of the order-p space on a reference element."""
+
+    def __init__(self, kind, order):
+        self.kind, self.order = kind, order
+        self.degs = _degrees(kind, order)
+        qp, qw = _collapsed_rule(kind, order + 3)
+        V = self._span(qp)                               # (nb, nq)
+
+        # phi = C @ span; the Legendre-product Gram matrix is ill
+        # conditioned on simplices, so orthonormalise twice (the second
+        # pass removes the first pass's rounding, as in re-orthogonalised
+        # Gram-Schmidt)
+        C = np.eye(len(V))
+        for _ in range(2):
+            W = C @ V
+            C = np.linalg.inv(np.linalg.cholesky((W*qw) @ W.T)) @ C
+        self._C = C
+
+    def _span(self, pts, deriv=None):
+        pts = np.atleast_2d(pts)
+        p = self.order
+        nd = pts.shape[1]
+        PD = [_leg(p, pts[:, d]) for d in range(nd)]
+        out = []
+        for dg in self.degs:
+            v = np.ones(len(pts))
+            for d in range(nd):
+                P, D = PD[d]
+                v = v*(D[dg[d]] if d == deriv else P[dg[d]])
+            out.append(v)
+        return np.array(out)
+
+    def at(self, pts):
+        """(nbasis, npts)"""
+        return self._C @ self._span(pts)
+
+    def jac_at(self, pts):
+        """(ndims, nbasis, npts)"""
+        nd = np.atleast_2d(pts).shape[1]
+        return np.array([self._C @ self._span(pts, d) for d in range(nd)])
+
+
+class NodalBasis:
+    """Lagrange basis through points ``pts`` of the order-p space."""
+
+    def __init__(self, kind, order, pts):
+        self.ortho = OrthoBasis(kind, order)
+        self.pts = np.atleast_2d(np.asarray(pts, dtype=float))
+        # l_n(x) = sum_b (V^-1)_nb phi_b(x) with V_bn = phi_b(x_n)
+        self.invvdm = np.linalg.inv(self.ortho.at(self.pts))
+
+    def nodal_basis_at(self, epts):
+        """(npts, nnodes)"""
+        return (self.invvdm @ self.ortho.at(epts)).T
+
+    def jac_nodal_basis_at(self, epts):
+        """(ndims, nnodes, npts): the layout of ``polys.py:63-66``."""
+        return self.invvdm @ self.ortho.jac_at(epts)
+
+
+def _proj(fn, pts2):
+    return np.array([fn(s, t) for s, t in pts2], dtype=float)
+
+
+def _flush(a, tol=1e-10):
+    a = np.array(a, dtype=float, copy=True)
+    a[np.abs(a) < tol] = 0
+    return a
+
+
+class _SimplexShape:
+    ndims = 3
+    name = None
+    faces = None
+
+    def __init__(self, order):
+        self.order = order
+        p = order
+        self.nupts = len(self.upts)
+        self.ubasis = NodalBasis(self.name, p, self.upts)
+        self.nfacefpts = [len(self._face_rule(k)[0]) for k, _, _ in self.faces]
+        self.nfpts = sum(self.nfacefpts)
+        nf = np.cumsum([0] + self.nfacefpts)
+        self.facefpts = [list(range(nf[i], nf[i + 1]))
+                         for i in range(len(self.faces))]
+
+    # face flux points: Williams-Shunn (tri) / Gauss-Legendre (quad)
+    def _face_rule(self, kind):
+        p = self.order
+        if kind == 'tri':
+            pts, _ = TABLES[('tri', 'williams-shunn', (p + 1)*(p + 2)//2)]
+            return np.array(pts), None
+        g, _ = gauss_legendre(p + 1)
+        return np.array([(g[i], g[j]) for j in range(p + 1)
+                         for i in range(p + 1)]), None
+
+    @cached_property
+    def fpts(self):
+        return np.vstack([_proj(proj, self._face_rule(kind)[0])
+                          for kind, proj, _ in self.faces])
+
+    @cached_property
+    def norm_fpts(self):
+        return np.vstack([[nrm]*n for (_, _, nrm), n in
+                          zip(self.faces, self.nfacefpts)]).astype(float)
+
+    @cached_property
+    def facebases(self):
+        out = {}
+        for kind in {k for k, _, _ in self.faces}:
+            out[kind] = NodalBasis(kind, self.order, self._face_rule(kind)[0])
+        return out
+
+    @cached_property
+    def gbasis_coeffs(self):
+        """Correction functions from face quadrature
+        (``pyfr/shapes.py:231-255``): tri faces use the 36-point
+        Williams-Shunn rule, quad faces (p+1)^2 Gauss-Legendre."""
+        p = self.order
+        rows = []
+        for kind, proj, _ in self.faces:
+            if kind == 'tri':
+                qp, qw = TABLES[('tri', 'williams-shunn', 36)]
+                qp, qw = np.array(qp), np.array(qw)
+            else:
+                g, w = gauss_legendre(p + 1)
+                qp = np.array([(g[i], g[j]) for j in range(p + 1)
+                               for i in range(p + 1)])
+                qw = np.array([w[i]*w[j] for j in range(p + 1)
+                               for i in range(p + 1)])
+            L = self.facebases[kind].nodal_basis_at(qp)      # (nq, nfp)
+            M = self.ubasis.ortho.at(_proj(proj, qp))        # (nb, nq)
+            rows.append(np.einsum('i,ik,ji->kj', qw, L, M))
+        return np.vstack(rows)
+
+    @cached_property
+    def m0(self):
+        return self.ubasis.nodal_basis_at(self.fpts)
+
+    @cached_property
+    def m1(self):
+        m = np.rollaxis(self.ubasis.jac_nodal_basis_at(self.upts), 2)
+        return m.reshape(self.nupts, -1)
+
+    @cached_property
+    def m2(self):
+        m = self.norm_fpts[..., None]*self.m0[:, None, :]
+        return m.reshape(self.nfpts, -1)
+
+    @cached_property
+    def m3(self):
+        return (self.gbasis_coeffs @ self.ubasis.ortho.at(self.upts)).T
+
+    @cached_property
+    def m4(self):
+        m = self.m1.reshape(self.nupts, -1, self.nupts).swapaxes(0, 1)
+        return m.reshape(-1, self.nupts)
+
+    @cached_property
+    def m6(self):
+        m = self.norm_fpts.T[:, None, :]*self.m3
+        return m.reshape(-1, self.nfpts)
+
+    def opmat(self, expr):
+        env = {f'M{i}': getattr(self, f'm{i}') for i in (0, 1, 2, 3, 4, 6)}
+        return _flush(eval(expr.replace('*', '@'), {'__builtins__': {}}, env))
+
+    @cached_property
+    def upts_wts(self):
+        return np.array(self._upts_rule[1])
+
+
+class TetShape(_SimplexShape):
+    """``pyfr/shapes.py:476-518``."""
+    name = 'tet'
+    nverts = 4
+    faces = [
+        ('tri', lambda s, t: (s, t, -1.0), (0, 0, -1)),
+        ('tri', lambda s, t: (s, -1.0, t), (0, -1, 0)),
+        ('tri', lambda s, t: (-1.0, t, s), (-1, 0, 0)),
+        ('tri', lambda s, t: (s, t, -s - t - 1), (1, 1, 1)),
+    ]
+
+    @cached_property
+    def _upts_rule(self):
+        n = (self.order + 1)*(self.order + 2)*(self.order + 3)//6
+        return TABLES[('tet', 'shunn-ham', n)]
+
+    @cached_property
+    def upts(self):
+        return np.array(self._upts_rule[0])
+
+    @staticmethod
+    def std_ele(sptord):
+        """Equispaced points (``TetShape.std_ele``, ``shapes.py:502-510``)."""
+        pts1d = np.linspace(-1, 1, sptord + 1)
+        return np.array([(p, q, r)
+                         for i, r in enumerate(pts1d)
+                         for j, q in enumerate(pts1d[:(sptord + 1 - i)])
+                         for p in pts1d[:(sptord + 1 - i - j)]])
+
+
+class PriShape(_SimplexShape):
+    """``pyfr/shapes.py:521-570``."""
+    name = 'pri'
+    nverts = 6
+    faces = [
+        ('tri', lambda s, t: (s, t, -1.0), (0, 0, -1)),
+        ('tri', lambda s, t: (s, t, 1.0), (0, 0, 1)),
+        ('quad', lambda s, t: (s, -1.0, t), (0, -1, 0)),
+        ('quad', lambda s, t: (-s, s, t), (1, 1, 0)),
+        ('quad', lambda s, t: (-1.0, s, t), (-1, 0, 0)),
+    ]
+
+    @cached_property
+    def _upts_rule(self):
+        n = (self.order + 1)**2*(self.order + 2)//2
+        return TABLES[('pri', 'williams-shunn~gauss-legendre', n)]
+
+    @cached_property
+    def upts(self):
+        return np.array(self._upts_rule[0])
+
+    @staticmethod
+    def std_ele(sptord):
+        """``PriShape.std_ele`` (``shapes.py:555-562``)."""
+        pts1d = np.linspace(-1, 1, sptord + 1)
+        return np.array([(p, q, r)
+                         for r in pts1d
+                         for i, q in enumerate(pts1d)
+                         for p in pts1d[:(sptord + 1 - i)]])
+
+
+SIMPLEX_SHAPES = {'tet': TetShape, 'pri': PriShape}

@@ -21,8 +21,9 @@ written against ``pyfr/backends/base``:
   ``pyfr/integrators/base.py:282-296``, ``:401-426``,
   ``pyfr/integrators/explicit/controllers.py:42-60``.
 
-Scope: periodic meshes of hexahedra on one GPU (no boundary conditions, no
-MPI interfaces, no anti-aliasing, no source terms).
+Scope: periodic meshes of hexahedra, straight-sided tetrahedra and right
+prisms on one GPU (no boundary conditions, no MPI interfaces, no
+anti-aliasing, no source terms).
 """
 from functools import cached_property
 import math
@@ -30,9 +31,12 @@ import time
 
 import numpy as np
 
-from pyfr_amd.geometry import HexGeometry
-from pyfr_amd.mesh import sorted_face_fpts_structured
-from pyfr_amd.shapes import SHAPES
+from pyfr_amd.geometry import AffineGeometry, HexGeometry
+from pyfr_amd.mesh import sorted_face_fpts, sorted_face_fpts_structured
+from pyfr_amd.shapes import SHAPES as HEX_SHAPES
+from pyfr_amd.simplex import SIMPLEX_SHAPES
+
+SHAPES = {**HEX_SHAPES, **SIMPLEX_SHAPES}
 
 
 def _npeval(expr, env):
@@ -93,7 +97,12 @@ class FluidElements:
         self.name = basis.name
         self.nupts, self.nfpts = basis.nupts, basis.nfpts
         self.nfacefpts = basis.nfacefpts
-        self.geom = HexGeometry(eles, order)
+        # tets and prisms here are straight-sided: one S and |J| per element
+        self.affine = self.name in SIMPLEX_SHAPES
+        if self.affine:
+            self.geom = AffineGeometry(eles, self.name)
+        else:
+            self.geom = HexGeometry(eles, order)
 
     # -- initial conditions (pyfr/solvers/base/elements.py:86-129)
     @staticmethod
@@ -124,6 +133,8 @@ class FluidElements:
 
     @cached_property
     def srtd_face_fpts(self):
+        if self.affine:
+            return sorted_face_fpts(self.plocfpts, self.basis.facefpts)
         return sorted_face_fpts_structured(self.plocfpts, self.basis.facefpts)
 
     @cached_property
@@ -190,6 +201,8 @@ class FluidElements:
 
     @property
     def mesh_regions(self):
+        if self.affine:
+            return {'affine': self.neles}
         off = self.linoff
         if off == 0:
             return {'linear': self.neles}
@@ -229,8 +242,18 @@ class FluidElements:
 
     @cached_property
     def _rcpdjac_upts(self):
+        if self.affine:
+            # one value per element (kernels read it with row stride 0)
+            return self._be.const_matrix(1/self.geom.djac[None, :],
+                                         tags={'align'})
         return self._be.const_matrix(self.rcpdjac_at_np('upts'),
                                      tags={'align'})
+
+    @cached_property
+    def _smats_affine(self):
+        # (9, neles): row 3i + k holds S[i][k]
+        s = self.geom.smats.reshape(self.neles, 9).T
+        return self._be.const_matrix(np.ascontiguousarray(s), tags={'align'})
 
     def _rcpdjac_curved(self):
         r = self.rcpdjac_at_np('upts')[:, :self.linoff]
@@ -294,6 +317,11 @@ class EulerElements(FluidElements):
 
         def tdisf(uin):
             ks = []
+            if 'affine' in r:
+                ks.append(be.kernel(
+                    'tflux', tplargs=tplargs | {'ktype': 'affine'},
+                    dims=[self.nupts, self.neles], u=self.scal_upts[uin],
+                    f=self._vect_upts, smats=self._smats_affine))
             if 'curved' in r:
                 ks.append(be.kernel(
                     'tflux', tplargs=tplargs | {'ktype': 'curved'},
@@ -310,6 +338,20 @@ class EulerElements(FluidElements):
             return ks[0] if len(ks) == 1 else be.unordered_meta_kernel(ks)
 
         kernels['tdisf'] = tdisf
+        self._add_fused_kernels()
+
+    def _add_fused_kernels(self):
+        """Fused tdivtpcorf + tdivtconf + negdivconf (+ RK45 stage): one
+        pass over the element's flux and common normal flux."""
+        be = self._be
+
+        def tdivf(fout, xa=None, stage=None):
+            return be.kernel(
+                'tdivf', self.opmat('M1 - M3*M2'), self.opmat('M3'),
+                self._vect_upts, self._scal_fpts, self.scal_upts[fout],
+                self._rcpdjac_upts,
+                xa=None if xa is None else self.scal_upts[xa], stage=stage)
+        self.kernels['tdivf'] = tdivf
 
 
 class NavierStokesElements(EulerElements):
@@ -369,6 +411,12 @@ class NavierStokesElements(EulerElements):
 
         def tdisf_fused(uin):
             ks = []
+            if 'affine' in r:
+                ks.append(be.kernel(
+                    'tflux', tplargs=tplargs | {'ktype': 'affine-fused'},
+                    dims=[self.nupts, self.neles], u=self.scal_upts[uin],
+                    f=self._vect_upts, gradu=self._grad_upts,
+                    smats=self._smats_affine, rcpdjac=self._rcpdjac_upts))
             if 'curved' in r:
                 ks.append(be.kernel(
                     'tflux', tplargs=tplargs | {'ktype': 'curved-fused'},
@@ -389,6 +437,12 @@ class NavierStokesElements(EulerElements):
             return ks[0] if len(ks) == 1 else be.unordered_meta_kernel(ks)
 
         kernels['tdisf_fused'] = tdisf_fused
+
+        # Fused tgradpcoru + tgradcoru: grad = (M4 - M6 M0) u + M6 comm
+        kernels['tgradf'] = lambda uin: be.kernel(
+            'mul2', self.opmat('M4 - M6*M0'), self.opmat('M6'),
+            self.scal_upts[uin], self._comm_fpts, self._grad_upts)
+        self._add_fused_kernels()
 
 
 # ---------------------------------------------------------------------------
@@ -574,8 +628,51 @@ class BaseSystem:
     def _rhs_graphs(self, uin, fout):
         key = (uin, fout)
         if key not in self._kcache:
-            self._kcache[key] = self._build_graphs(uin, fout)
+            g = None
+            if self.backend.fused:
+                g = self._try_fused(uin, fout)
+            self._kcache[key] = g or self._build_graphs(uin, fout)
         return self._kcache[key]
+
+    def _try_fused(self, uin, fout, xa=None, stage=None):
+        from pyfr_amd.backend import NotSuitableError
+        try:
+            ks = self._fused_kernels(uin, fout, xa, stage)
+        except NotSuitableError:
+            return None
+        g = self.backend.graph()
+        g.add_all(ks)
+        g.commit()
+        g.final = ks[-1]
+        return [g]
+
+    @property
+    def fused_stages(self):
+        """True if RK45 stage updates can be fused into the RHS."""
+        if not self.backend.fused:
+            return False
+        if not hasattr(self, '_fused_ok'):
+            from pyfr_amd.backend import NotSuitableError
+            w = len(self.ele_banks[0]) - 1     # a working-precision bank
+            try:
+                self._fused_kernels(w, w, None, None)
+                self._fused_ok = True
+            except NotSuitableError:
+                self._fused_ok = False
+        return self._fused_ok
+
+    def rhs_stage(self, t, uin, fout, xa, stage, dta, dtb, last, mixed=False):
+        """RHS into fout fused with the rkvdh2 stage update
+        (``rkvdh2.mako:4-31``): xa' = xa + dta*k, fout' = xa + dtb*k (if last,
+        only xa' = xa + dtb*k).  mixed: xa is the FP64 accumulated solution
+        and fout the FP32 stage input."""
+        key = (uin, fout, xa, stage, mixed)
+        if key not in self._kcache:
+            self._kcache[key] = self._try_fused(
+                uin, fout, xa, 'rk45-mixed' if mixed else 'rk45')
+        for g in self._kcache[key]:
+            g.final.bind(dta=dta, dtb=dtb, last=int(last))
+            self.backend.run_graph(g)
 
     def ele_scal_upts(self, idx):
         return [b[idx].get() for b in self.ele_banks]
@@ -643,6 +740,15 @@ class EulerSystem(BaseSystem):
     elementscls = EulerElements
     intinterscls = EulerIntInters
 
+    def _fused_kernels(self, uin, fout, xa, stage):
+        eles = list(self.ele_map.values())
+        ks = [e.kernels['disu'](uin) for e in eles]
+        ks += [i.kernels['comm_flux']() for i in self._int_inters]
+        for e in eles:
+            ks += [e.kernels['tdisf'](uin),
+                   e.kernels['tdivf'](fout, xa, stage)]
+        return ks
+
     def _build_graphs(self, uin, fout):
         """One graph holding ``g_intf`` then ``g_flux_div`` of
         ``pyfr/solvers/baseadvec/system.py:57-136`` (no MPI to overlap)."""
@@ -663,6 +769,18 @@ class NavierStokesSystem(BaseSystem):
     name = 'navier-stokes'
     elementscls = NavierStokesElements
     intinterscls = NavierStokesIntInters
+
+    def _fused_kernels(self, uin, fout, xa, stage):
+        eles = list(self.ele_map.values())
+        ks = [e.kernels['disu'](uin) for e in eles]
+        ks += [i.kernels['con_u']() for i in self._int_inters]
+        for e in eles:
+            ks += [e.kernels['tgradf'](uin), e.kernels['tdisf_fused'](uin),
+                   e.kernels['gradcoru_fpts']()]
+        ks += [i.kernels['comm_flux']() for i in self._int_inters]
+        for e in eles:
+            ks += [e.kernels['tdivf'](fout, xa, stage)]
+        return ks
 
     def _build_graphs(self, uin, fout):
         """``pyfr/solvers/baseadvecdiff/system.py:53-231`` with gradient
@@ -827,12 +945,22 @@ class RK45Integrator(BaseIntegrator):
     def step(self, t, dt):
         r1 = self.idxcurr
         r2 = 1 - r1
+        last = self._nstages - 1
+        fused = self.system.fused_stages
         for i, ci in enumerate(self.c):
-            self._rhs(t + ci*dt, r2 if i > 0 else r1, r2)
-            kerns = self._rkvdh2_kerns(i, r1, r2)
-            for k in kerns:
-                k.bind(dt=dt)
-            self.backend.run_kernels(kerns)
+            uin = r2 if i > 0 else r1
+            if fused:
+                # RHS with the stage update in its final kernel
+                self.system.rhs_stage(t + ci*dt, uin, r2, r1, i,
+                                      dt*self.a[i] if i < last else 0.0,
+                                      dt*self.b[i], i == last)
+                self.nrhsevals += 1
+            else:
+                self._rhs(t + ci*dt, uin, r2)
+                kerns = self._rkvdh2_kerns(i, r1, r2)
+                for k in kerns:
+                    k.bind(dt=dt)
+                self.backend.run_kernels(kerns)
             r1, r2 = r2, r1
         return r2
 
@@ -863,10 +991,16 @@ class MixedRK45Integrator(RK45Integrator):
             self._sync_work()
         lib, be = self.backend.lib, self.backend
         last = self._nstages - 1
+        fused = self.system.fused_stages
         for i, ci in enumerate(self.c):
-            self._rhs(t + ci*dt, 1, 1)
             dta = dt*self.a[i] if i < last else 0.0
             dtb = dt*self.b[i]
+            if fused:
+                self.system.rhs_stage(t + ci*dt, 1, 1, 0, i, dta, dtb,
+                                      i == last, mixed=True)
+                self.nrhsevals += 1
+                continue
+            self._rhs(t + ci*dt, 1, 1)
             for banks in self.system.ele_banks:
                 acc, w = banks[0], banks[1]
                 lib.rkvdh2_mixed(acc.nrow, acc.ncol, acc.leaddim, acc.data,

@@ -20,10 +20,12 @@ def backend_factory():
 
     made = []
 
-    def make(precision='double', graphs=True):
-        """precision: 'double', 'single' or 'mixed'."""
+    def make(precision='double', graphs=True, fused=True):
+        """precision: 'double', 'single' or 'mixed'; fused: use the fused
+        operator kernels where available."""
         cfg = Inifile.from_dict({'backend': {'precision': precision},
-                                 'backend-hip': {'graphs': str(graphs)}})
+                                 'backend-hip': {'graphs': str(graphs),
+                                                 'fused': str(fused)}})
         be = AMDBackend(cfg)
         made.append(be)
         return be

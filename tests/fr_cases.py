@@ -44,3 +44,44 @@ def oracle_consts(system, beta=0.0, tau=0.0):
 
 def rel_l2(a, b):
     return float(np.linalg.norm((a - b).ravel())/np.linalg.norm(b.ravel()))
+
+
+def mode_ics(seed=0, nmodes=8, amp=0.05, kmax=2):
+    """Superposition of ``nmodes`` random-phase sinusoidal modes (integer
+    wavevectors in [-kmax, kmax]^3, periodic on [0, 2 pi]^3) of amplitude
+    ``amp`` in density and each velocity component, on rho = 1, p = 1/1.4
+    (sound speed 1).  Returns initial-condition expressions."""
+    rng = np.random.default_rng(seed)
+    out = {}
+    for var, base in (('rho', '1.0'), ('u', '0.0'), ('v', '0.0'),
+                      ('w', '0.0')):
+        terms = [base]
+        for _ in range(nmodes):
+            k = np.zeros(3, dtype=int)
+            while not k.any():
+                k = rng.integers(-kmax, kmax + 1, 3)
+            ph = rng.uniform(0, 2*np.pi)
+            terms.append(f'{amp!r}*sin({k[0]}*x + {k[1]}*y + {k[2]}*z + '
+                         f'{ph!r})')
+        out[var] = ' + '.join(terms)
+    out['p'] = repr(1/1.4)
+    return out
+
+
+def eval_ics(ics):
+    """Callables of (x, y, z) for expressions of ``mode_ics``/``TGV_ICS``."""
+    env = {'sin': np.sin, 'cos': np.cos, '__builtins__': {}}
+
+    def mk(expr):
+        return lambda x, y, z: eval(expr, env, {'x': x, 'y': y, 'z': z}) \
+            + 0*x
+    return {k: mk(v) for k, v in ics.items()}
+
+
+def make_simplex_cfg(order=2, precision='double', scheme='rk4', dt=1e-3,
+                     ics=None, rsolver='rusanov'):
+    cfg = make_cfg('euler', order, rsolver, precision=precision,
+                   scheme=scheme, dt=dt)
+    for k, v in (ics or mode_ics()).items():
+        cfg.set('soln-ics', k, v)
+    return cfg

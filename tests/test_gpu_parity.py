@@ -70,12 +70,14 @@ def _systems(be, system, n, order, rsolver='rusanov', scheme='rk4'):
     from pyfr_amd.mesh import BoxMesh
     from pyfr_amd.solvers import get_integrator
     cfg = make_cfg(system, order, rsolver, scheme=scheme)
+    cfg.set('backend-hip', 'fused', str(be.fused))
     integ = get_integrator(be, BoxMesh(n), cfg)
     orc = OracleHexSystem(n, order, system, rsolver,
                           constants=oracle_consts(system))
     return integ, orc
 
 
+@pytest.mark.parametrize('fused', [True, False])
 @pytest.mark.parametrize('system,n,order,rsolver', [
     ('euler', 8, 2, 'rusanov'),
     ('euler', 5, 3, 'hllc'),
@@ -85,9 +87,10 @@ def _systems(be, system, n, order, rsolver='rusanov', scheme='rk4'):
     ('navier-stokes', 4, 2, 'hllc'),
     ('navier-stokes', 3, 1, 'rusanov'),
 ])
-def test_rhs_parity(backend_factory, system, n, order, rsolver):
-    be = backend_factory()
-    print(f'\n[rhs] {system} n={n} p={order} {rsolver}', flush=True)
+def test_rhs_parity(backend_factory, system, n, order, rsolver, fused):
+    be = backend_factory(fused=fused)
+    print(f'\n[rhs] {system} n={n} p={order} {rsolver} fused={fused}',
+          flush=True)
     integ, orc = _systems(be, system, n, order, rsolver)
     u0 = orc.ics()
     # the product's ICs must equal the oracle's
@@ -101,11 +104,12 @@ def test_rhs_parity(backend_factory, system, n, order, rsolver):
     assert err < tol
 
 
-def test_cfg0_rk4_100_steps(backend_factory):
+@pytest.mark.parametrize('fused', [True, False])
+def test_cfg0_rk4_100_steps(backend_factory, fused):
     """Plain method, CPU-runnable config: 8^3 hex p=2 Euler, Rusanov, RK4,
     dt=1e-3 to t=0.1.  The reference's fixed-dt controller takes 101 steps
     here (the last ten at dt=9e-4; see oracle.rhs.clamp_dt)."""
-    be = backend_factory()
+    be = backend_factory(fused=fused)
     integ, orc = _systems(be, 'euler', 8, 2)
     integ.advance_to(0.1)
     u_ref, nsteps = advance(orc.rhs, orc.ics(), 1e-3, 0.1, 'rk4')
@@ -115,9 +119,11 @@ def test_cfg0_rk4_100_steps(backend_factory):
     assert err < 1e-12
 
 
-def test_ns_rk45_steps(backend_factory):
-    be = backend_factory()
+@pytest.mark.parametrize('fused', [True, False])
+def test_ns_rk45_steps(backend_factory, fused):
+    be = backend_factory(fused=fused)
     integ, orc = _systems(be, 'navier-stokes', 4, 3, scheme='rk45')
+    assert integ.system.fused_stages == fused
     integ.dt = 2e-4
     integ.advance_to(20*2e-4)
     u_ref, nsteps = advance(orc.rhs, orc.ics(), 2e-4, 20*2e-4, 'rk45')
@@ -154,22 +160,27 @@ def test_mixed_precision_rk45(backend_factory):
     from pyfr_amd.mesh import BoxMesh
     from pyfr_amd.solvers import MixedRK45Integrator, get_integrator
     sols = {}
-    for prec in ('double', 'mixed'):
+    for prec, fused in (('double', False), ('mixed', True),
+                        ('mixed', False)):
         cfg = make_cfg('navier-stokes', 3, scheme='rk45', dt=2e-4,
                        precision=prec)
-        be = backend_factory(precision=prec)
+        be = backend_factory(precision=prec, fused=fused)
+        cfg.set('backend-hip', 'fused', str(fused))
         integ = get_integrator(be, BoxMesh(4), cfg)
         assert isinstance(integ, MixedRK45Integrator) == (prec == 'mixed')
         integ.run_steps(20)
-        sols[prec] = integ.soln[0]
+        sols[prec, fused] = integ.soln[0]
         if prec == 'mixed':
             # FP64 conservation sums of the FP64 state
             cs = integ.system.conservation_sums(0)
             w = integ.system.ele_map['hex'].basis.upts_wts
             rj = integ.system.ele_map['hex'].rcpdjac_at_np('upts')
-            ref = np.einsum('u,uve->v', w, sols[prec]/rj[:, None, :])
+            ref = np.einsum('u,uve->v', w, sols[prec, fused]/rj[:, None, :])
             assert np.allclose(cs, ref, rtol=1e-12, atol=1e-9)
-    a, b = sols['mixed'], sols['double']
+    b = sols['double', False]
+    # fused and unfused mixed forms agree to rounding
+    assert rel_l2(sols['mixed', True], sols['mixed', False]) < 1e-6
+    a = sols['mixed', True]
     for v in range(5):
         print(f'[mixed] field {v} rel L2 {rel_l2(a[:, v], b[:, v]):.2e}',
               flush=True)

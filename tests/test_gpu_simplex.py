@@ -1,0 +1,250 @@
+"""GPU parity of the dense (tet/prism) path: matrix-core operator kernels
+against NumPy, and the Euler RHS / RK4 trajectories on periodic meshes of
+tets and prisms (with tet-prism interfaces) against the oracle.
+
+Tolerances: FP64 as for the hex path (operators 1e-13 relative L2, one RHS
+max(1e-12, 10 x the oracle's own 1e-16-input-noise floor), trajectories
+1e-12); FP32 operators 2e-6 and FP32 RHS / trajectories 1e-4 relative L2
+(FP32 rounding amplified by the flux cancellation the FP64 floor measures).
+"""
+import numpy as np
+import pytest
+
+from tests.fr_cases import eval_ics, make_simplex_cfg, mode_ics, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+def _tables(name, p):
+    from pyfr_amd.quadtables import TABLES
+    nu = ((p + 1)*(p + 2)*(p + 3)//6 if name == 'tet'
+          else (p + 1)**2*(p + 2)//2)
+    key = (('tet', 'shunn-ham', nu) if name == 'tet' else
+           ('pri', 'williams-shunn~gauss-legendre', nu))
+    return {'upts': TABLES[key],
+            'tri_fpts': TABLES[('tri', 'williams-shunn',
+                                (p + 1)*(p + 2)//2)][0],
+            'tri_q36': TABLES[('tri', 'williams-shunn', 36)]}
+
+
+@pytest.mark.parametrize('precision', ['double', 'single'])
+@pytest.mark.parametrize('m,k', [(60, 35), (35, 105), (105, 75), (75, 225),
+                                 (13, 9), (112, 3)])
+@pytest.mark.parametrize('alpha,beta', [(1.0, 0.0), (1.0, 1.0), (0.5, 2.0),
+                                        (-2.0, 0.0)])
+def test_mul_dense(backend_factory, precision, m, k, alpha, beta):
+    be = backend_factory(precision=precision)
+    rng = np.random.default_rng(m*1000 + k)
+    A = rng.standard_normal((m, k))
+    neles = 200
+    bnp = rng.standard_normal((k, 5, neles))
+    cnp = rng.standard_normal((m, 5, neles))
+    a = be.const_matrix(A)
+    b = be.matrix((k, 5, neles), bnp, tags={'align'})
+    c = be.matrix((m, 5, neles), cnp, tags={'align'})
+    print(f'\n[dense mul] {precision} {m}x{k} alpha={alpha} beta={beta}',
+          flush=True)
+    kern = be.kernel('mul', a, b, out=c, alpha=alpha, beta=beta)
+    assert kern.kind == 'dense'
+    be.run_kernels([kern], wait=True)
+    ref = alpha*(A @ bnp.reshape(k, -1)).reshape(m, 5, neles) + beta*cnp
+    tol = 1e-13 if precision == 'double' else 2e-6
+    assert rel_l2(c.get(), ref) < tol
+
+
+class _Plan2:
+    """A raw two-input plan through the C ABI (for epilogue tests)."""
+
+    def __init__(self, be, A0, A1):
+        import ctypes as ct
+        self.lib = be.lib
+        self.h0, self.h1 = np.ascontiguousarray(A0), np.ascontiguousarray(A1)
+        plan, kind = ct.c_void_p(), ct.c_int()
+        dt = 0 if be.fpdtype == np.float64 else 1
+        self.lib.mul2_plan_create(ct.byref(plan), A0.shape[0], A0.shape[1],
+                                  A1.shape[1], self.h0.ctypes.data,
+                                  self.h1.ctypes.data, dt, ct.byref(kind))
+        self.plan, self.kind = plan.value, kind.value
+
+    def __del__(self):
+        self.lib.mul2_plan_destroy(self.plan)
+
+
+def _operands(be, m, k0, k1, neles, seed):
+    rng = np.random.default_rng(seed)
+    A0, A1 = rng.standard_normal((m, k0)), rng.standard_normal((m, k1))
+    b0 = be.matrix((k0, 5, neles), rng.standard_normal((k0, 5, neles)),
+                   tags={'align'})
+    b1 = be.matrix((k1, 5, neles), rng.standard_normal((k1, 5, neles)),
+                   tags={'align'})
+    c = be.matrix((m, 5, neles), rng.standard_normal((m, 5, neles)),
+                  tags={'align'})
+    return A0, A1, b0, b1, c, rng
+
+
+@pytest.mark.parametrize('precision', ['double', 'single'])
+@pytest.mark.parametrize('variant', [0, 1, 2, 4])
+@pytest.mark.parametrize('shape', [(35, 105, 60), (75, 225, 105),
+                                   (10, 30, 24)])
+def test_dense_variants_agree(backend_factory, precision, variant, shape):
+    """Every kernel variant the tuner may pick gives the same product."""
+    be = backend_factory(precision=precision)
+    m, k0, k1 = shape
+    neles = 256
+    A0, A1, b0, b1, c, _ = _operands(be, m, k0, k1, neles, sum(shape))
+    p = _Plan2(be, A0, A1)
+    assert p.kind == 2
+    print(f'\n[dense variant] {precision} {shape} v={variant}', flush=True)
+    be.lib.mul2_plan_set_variant(p.plan, variant)
+    be.lib.mul2(p.plan, c.ncol, b0.data, b0.leaddim, b1.data, b1.leaddim,
+                c.data, c.leaddim, 0, None, 0, None, 0.0, 0.0, 0, be.stream)
+    be.wait()
+    ref = ((A0 @ b0.get().reshape(k0, -1) + A1 @ b1.get().reshape(k1, -1))
+           .reshape(m, 5, neles))
+    tol = 1e-13 if precision == 'double' else 2e-6
+    assert rel_l2(c.get(), ref) < tol
+
+
+@pytest.mark.parametrize('precision', ['double', 'single'])
+@pytest.mark.parametrize('per_element', [True, False])
+@pytest.mark.parametrize('ep,last', [(1, 0), (2, 0), (2, 1)])
+def test_dense_epilogues(backend_factory, precision, per_element, ep, last):
+    """negdivconf and the RK45 stage epilogues of the dense kernel, with
+    1/|J| per point or one per element (row stride 0)."""
+    be = backend_factory(precision=precision)
+    m, k0, k1, neles = 35, 105, 60, 200
+    A0, A1, b0, b1, c, rng = _operands(be, m, k0, k1, neles, 11)
+    rc = (rng.uniform(0.5, 2.0, (1, neles)) if per_element else
+          rng.uniform(0.5, 2.0, (m, neles)))
+    rcp = be.const_matrix(rc, tags={'align'})
+    xa = be.matrix((m, 5, neles), rng.standard_normal((m, 5, neles)),
+                   tags={'align'})
+    c0, x0 = c.get(), xa.get()
+    p = _Plan2(be, A0, A1)
+    print(f'\n[dense epilogue] {precision} ep={ep} last={last} '
+          f'per_element={per_element}', flush=True)
+    dta, dtb = 0.3, -0.7
+    be.lib.mul2(p.plan, c.ncol, b0.data, b0.leaddim, b1.data, b1.leaddim,
+                c.data, c.leaddim, ep, rcp.data,
+                0 if per_element else rcp.leaddim, xa.data, dta, dtb, last,
+                be.stream)
+    be.wait()
+    acc = ((A0 @ b0.get().reshape(k0, -1) + A1 @ b1.get().reshape(k1, -1))
+           .reshape(m, 5, neles))
+    r = np.broadcast_to(rc, (m, neles))[:, None, :]
+    t = -r*acc
+    tol = 1e-13 if precision == 'double' else 2e-6
+    if ep == 1:
+        assert rel_l2(c.get(), t) < tol
+    elif not last:
+        assert rel_l2(xa.get(), x0 + dta*t) < tol
+        assert rel_l2(c.get(), x0 + dtb*t) < tol
+    else:
+        assert rel_l2(xa.get(), x0 + dtb*t) < tol
+        assert np.array_equal(c.get(), c0)
+
+
+# ---------------------------------------------------------------------------
+# RHS and trajectories on tet/prism meshes
+# ---------------------------------------------------------------------------
+
+def _mixed(be, n, nt, npr, order, scheme='rk4', dt=1e-3, precision=None):
+    from oracle.simplex import OracleMixedSystem
+    from pyfr_amd.mesh import SimplexBoxMesh
+    from pyfr_amd.solvers import get_integrator
+    ics = mode_ics(seed=3)
+    cfg = make_simplex_cfg(order, precision or be.precision, scheme, dt, ics)
+    cfg.set('backend-hip', 'fused', str(be.fused))
+    mesh = SimplexBoxMesh.stacked(n, nt, npr)
+    integ = get_integrator(be, mesh, cfg)
+    L = 2*np.pi
+    Lz = (nt + npr)*L/n
+    verts = {et: v.transpose(0, 2, 1) for et, v in mesh.spts.items()}
+    idx = {et: np.arange(v.shape[1]) for et, v in mesh.spts.items()}
+    orc = OracleMixedSystem(verts, order, {et: _tables(et, order)
+                                           for et in verts},
+                            [((L, L, Lz), idx)])
+    return integ, orc, orc.ics(eval_ics(ics))
+
+
+def _cat(d, names):
+    return np.concatenate([d[n].ravel() for n in names])
+
+
+def _noise_floor(orc, u, seed=0):
+    rng = np.random.default_rng(seed)
+    up = {k: v*(1 + 1e-16*rng.standard_normal(v.shape)) for k, v in u.items()}
+    names = list(u)
+    return rel_l2(_cat(orc.rhs(up), names), _cat(orc.rhs(u), names))
+
+
+@pytest.mark.parametrize('fused', [True, False])
+@pytest.mark.parametrize('n,nt,npr,order', [(3, 2, 1, 1), (3, 2, 1, 2),
+                                            (3, 2, 1, 3), (3, 2, 1, 4),
+                                            (2, 2, 0, 4), (2, 0, 2, 3)])
+def test_simplex_rhs_parity(backend_factory, n, nt, npr, order, fused):
+    be = backend_factory(fused=fused)
+    print(f'\n[simplex rhs] n={n} tets={nt} prisms={npr} p={order} '
+          f'fused={fused}', flush=True)
+    integ, orc, u0 = _mixed(be, n, nt, npr, order)
+    sysm = integ.system
+    names = sysm.ele_types
+    got0 = dict(zip(names, sysm.ele_scal_upts(0)))
+    assert rel_l2(_cat(got0, names), _cat(u0, names)) < 1e-14
+    sysm.rhs(0.0, 0, 1)
+    got = dict(zip(names, sysm.ele_scal_upts(1)))
+    ref = orc.rhs(u0)
+    tol = max(1e-12, 10*_noise_floor(orc, u0))
+    err = rel_l2(_cat(got, names), _cat(ref, names))
+    print(f'[simplex rhs] rel L2 {err:.3e} (tol {tol:.3e})', flush=True)
+    assert err < tol
+
+
+def test_simplex_rk4_trajectory(backend_factory):
+    be = backend_factory()
+    integ, orc, u0 = _mixed(be, 3, 2, 1, 3, dt=2e-3)
+    names = integ.system.ele_types
+    integ.run_steps(10, 2e-3)
+    u = u0
+    for _ in range(10):
+        u = _rk4_dict(orc.rhs, u, 2e-3)
+    err = rel_l2(_cat(dict(zip(names, integ.soln)), names), _cat(u, names))
+    print(f'\n[simplex rk4] 10 steps rel L2 {err:.3e}', flush=True)
+    assert err < 1e-12
+
+
+def _rk4_dict(rhs, u, dt):
+    """oracle.rhs.rk4_step over dicts of per-type arrays."""
+    from oracle.rhs import rk4_step
+    names = list(u)
+    shapes = [u[n].shape for n in names]
+    sizes = [u[n].size for n in names]
+
+    def flat(d):
+        return np.concatenate([d[n].ravel() for n in names])
+
+    def unflat(v):
+        out, o = {}, 0
+        for n, sh, sz in zip(names, shapes, sizes):
+            out[n] = v[o:o + sz].reshape(sh)
+            o += sz
+        return out
+    return unflat(rk4_step(lambda v: flat(rhs(unflat(v))), flat(u), dt))
+
+
+def test_simplex_fp32_rhs_and_trajectory(backend_factory):
+    be = backend_factory(precision='single')
+    integ, orc, u0 = _mixed(be, 3, 2, 1, 4, dt=2e-3)
+    sysm = integ.system
+    names = sysm.ele_types
+    sysm.rhs(0.0, 0, 1)
+    got = dict(zip(names, sysm.ele_scal_upts(1)))
+    err = rel_l2(_cat(got, names), _cat(orc.rhs(u0), names))
+    print(f'\n[simplex fp32 rhs] rel L2 {err:.3e}', flush=True)
+    assert err < 1e-4
+    integ.run_steps(10, 2e-3)
+    u = u0
+    for _ in range(10):
+        u = _rk4_dict(orc.rhs, u, 2e-3)
+    err = rel_l2(_cat(dict(zip(names, integ.soln)), names), _cat(u, names))
+    print(f'[simplex fp32 rk4] 10 steps rel L2 {err:.3e}', flush=True)
+    assert err < 1e-5

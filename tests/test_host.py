@@ -50,10 +50,11 @@ def test_gauss_legendre_tables():
 def test_generated_kernels_match_operators():
     """The values baked into the generated kernels are bit-identical to the
     operators the host builds at run time (plans match bit for bit)."""
-    gen = os.path.join(ROOT, 'pyfr_amd', 'csrc', '_build', 'gen_ops.hip')
-    if not os.path.exists(gen):
+    bdir = os.path.join(ROOT, 'pyfr_amd', 'csrc', '_build')
+    gens = [os.path.join(bdir, f'gen_ops_p{p}.hip') for p in (1, 2, 3, 4)]
+    if not all(os.path.exists(g) for g in gens):
         pytest.skip('library not built')
-    src = open(gen).read()
+    src = ''.join(open(g).read() for g in gens)
     for p in (1, 2, 3, 4):
         h = HexShape(p)
         for i, op in enumerate(OPS):

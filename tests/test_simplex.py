@@ -1,0 +1,343 @@
+"""Tetrahedron / prism operators (CPU).
+
+The oracle restates the reference's Jacobi-polynomial bases
+(``pyfr/polys.py:117-338``) and ``TetShape`` / ``PriShape``
+(``pyfr/shapes.py:476-570``).  The reference holds no tet/prism fixture, so
+the oracle is pinned here by what the mathematics fixes, each check built
+from numpy's Gauss-Legendre rules collapsed onto the simplex (independent of
+both the oracle and the product):
+
+* the tabulated point sets integrate monomials to their stated degree;
+* the bases are orthonormal and their Jacobians match finite differences;
+* M0 / M1 / M4 reproduce interpolants and gradients of degree-p polynomials;
+* M3 and M6 satisfy the lifting identity  int_E g_j v = int_face L_j v  for
+  every v of degree p (this is what the correction functions are);
+* a planted mistake (a wrong Jacobi weight) fails the orthonormality check.
+
+The product builds its bases independently (Legendre products orthonormalised
+on the same collapsed rules) and must agree with the oracle to rounding.
+"""
+import itertools as it
+
+import numpy as np
+import pytest
+
+import oracle.basis as ob
+from pyfr_amd.quadtables import TABLES
+from pyfr_amd.simplex import PriShape, TetShape
+
+OPS = ['M0', 'M1 - M3*M2', 'M3', 'M4 - M6*M0', 'M6']
+
+
+def _gl(n):
+    return np.polynomial.legendre.leggauss(n)
+
+
+def _rule(kind, n):
+    """Collapsed tensor Gauss-Legendre rule on the reference element."""
+    g, w = _gl(n)
+    if kind == 'tri':
+        a, b = (x.ravel() for x in np.meshgrid(g, g, indexing='ij'))
+        wt = np.outer(w, w).ravel()*(1 - b)/2
+        return np.stack([(1 + a)*(1 - b)/2 - 1, b], 1), wt
+    if kind == 'quad':
+        a, b = (x.ravel() for x in np.meshgrid(g, g, indexing='ij'))
+        return np.stack([a, b], 1), np.outer(w, w).ravel()
+    if kind == 'tet':
+        a, b, c = (x.ravel() for x in np.meshgrid(g, g, g, indexing='ij'))
+        wt = np.einsum('i,j,k->ijk', w, w, w).ravel()
+        wt = wt*(1 - b)*(1 - c)**2/8
+        y = (1 + b)*(1 - c)/2 - 1
+        x = (1 + a)*(1 - b)*(1 - c)/4 - 1
+        return np.stack([x, y, c], 1), wt
+    if kind == 'pri':
+        tp, tw = _rule('tri', n)
+        pts = np.array([(*p, z) for z in g for p in tp])
+        return pts, np.outer(w, tw).ravel()
+    raise ValueError(kind)
+
+
+def _tables(name, p):
+    nu = (p + 1)*(p + 2)*(p + 3)//6 if name == 'tet' else (p + 1)**2*(p + 2)//2
+    key = (('tet', 'shunn-ham', nu) if name == 'tet' else
+           ('pri', 'williams-shunn~gauss-legendre', nu))
+    return {'upts': TABLES[key],
+            'tri_fpts': TABLES[('tri', 'williams-shunn',
+                                (p + 1)*(p + 2)//2)][0],
+            'tri_q36': TABLES[('tri', 'williams-shunn', 36)]}
+
+
+def _oracle(name, p):
+    return ob.SimplexShapeOracle(name, p, _tables(name, p))
+
+
+def _randpoly(rng, ndim, deg, kind):
+    """A random polynomial of the order-`deg` space of `kind` and its
+    gradient, as callables of an (n, ndim) array."""
+    if kind == 'pri':
+        exps = [(i, j, k) for i in range(deg + 1) for j in range(deg + 1 - i)
+                for k in range(deg + 1)]
+    else:
+        exps = [e for e in it.product(range(deg + 1), repeat=ndim)
+                if sum(e) <= deg]
+    c = rng.standard_normal(len(exps))
+
+    def f(x):
+        return sum(ci*np.prod([x[:, d]**e[d] for d in range(ndim)], axis=0)
+                   for ci, e in zip(c, exps))
+
+    def grad(x):
+        out = np.zeros_like(x)
+        for ci, e in zip(c, exps):
+            for d in range(ndim):
+                if e[d]:
+                    t = ci*e[d]*x[:, d]**(e[d] - 1)
+                    for dd in range(ndim):
+                        if dd != d:
+                            t = t*x[:, dd]**e[dd]
+                    out[:, d] += t
+        return out
+    return f, grad
+
+
+# Degrees stated in the reference's rule-file names (d2, d3, ...)
+@pytest.mark.parametrize('key,deg', [
+    (('tet', 'shunn-ham', 4), 2), (('tet', 'shunn-ham', 10), 3),
+    (('tet', 'shunn-ham', 20), 5), (('tet', 'shunn-ham', 35), 6),
+    (('tri', 'williams-shunn', 3), 2), (('tri', 'williams-shunn', 6), 4),
+    (('tri', 'williams-shunn', 10), 5), (('tri', 'williams-shunn', 15), 7),
+    (('tri', 'williams-shunn', 36), 12),
+    (('pri', 'williams-shunn~gauss-legendre', 6), 2),
+    (('pri', 'williams-shunn~gauss-legendre', 18), 4),
+    (('pri', 'williams-shunn~gauss-legendre', 40), 5),
+    (('pri', 'williams-shunn~gauss-legendre', 75), 7)])
+def test_quadrature_tables_exact_to_stated_degree(key, deg):
+    pts, wts = (np.array(x, dtype=float) for x in TABLES[key])
+    qp, qw = _rule(key[0], deg + 2)
+    nd = pts.shape[1]
+    for e in it.product(range(deg + 1), repeat=nd):
+        if key[0] == 'pri':
+            if e[0] + e[1] > deg or e[2] > deg:
+                continue
+        elif sum(e) > deg:
+            continue
+        f = lambda x: np.prod([x[:, d]**e[d] for d in range(nd)], axis=0)
+        assert abs(wts @ f(pts) - qw @ f(qp)) < 1e-13, e
+
+
+@pytest.mark.parametrize('cls,kind,nd', [
+    (ob.TriBasis, 'tri', 2), (ob.TetBasis, 'tet', 3), (ob.PriBasis, 'pri', 3)])
+@pytest.mark.parametrize('p', [1, 2, 3, 4])
+def test_oracle_basis_orthonormal(cls, kind, nd, p):
+    qp, qw = _rule(kind, p + 3)
+    phi = cls(p, qp).ortho_basis_at(qp, do_clean=False)
+    g = (phi*qw) @ phi.T
+    assert np.abs(g - np.eye(len(g))).max() < 1e-13
+
+
+@pytest.mark.parametrize('cls,kind', [(ob.TetBasis, 'tet'),
+                                      (ob.PriBasis, 'pri')])
+def test_oracle_basis_jacobian_finite_difference(cls, kind):
+    rng = np.random.default_rng(3)
+    qp, _ = _rule(kind, 3)
+    x = qp[rng.choice(len(qp), 6, replace=False)]
+    b = cls(4, x)
+    j = b.jac_ortho_basis_at(x, do_clean=False)     # (ndim, nb, npts)
+    h = 1e-6
+    for d in range(3):
+        dx = np.zeros(3)
+        dx[d] = h
+        fd = (b.ortho_basis_at(x + dx, False) -
+              b.ortho_basis_at(x - dx, False))/(2*h)
+        assert np.abs(j[d] - fd).max() < 1e-6
+
+
+def test_planted_jacobi_weight_mistake_is_caught():
+    """A wrong Jacobi weight in the tet basis breaks orthonormality."""
+    class Planted(ob.TetBasis):
+        def _ortho(self, p, q, r):
+            real = ob.jacobi
+            ob.jacobi = lambda n, a, b, z: real(n, a if a < 2 else a + 1, b, z)
+            try:
+                return super()._ortho(p, q, r)
+            finally:
+                ob.jacobi = real
+    qp, qw = _rule('tet', 6)
+    phi = Planted(3, qp).ortho_basis_at(qp, do_clean=False)
+    g = (phi*qw) @ phi.T
+    assert np.abs(g - np.eye(len(g))).max() > 1e-3
+
+
+def _face_rule(kind, n):
+    return _rule(kind, n)
+
+
+@pytest.mark.parametrize('name', ['tet', 'pri'])
+@pytest.mark.parametrize('p', [1, 2, 3, 4])
+def test_oracle_operators_exact(name, p):
+    o = _oracle(name, p)
+    rng = np.random.default_rng(p)
+    f, grad = _randpoly(rng, 3, p, name)
+    u = f(o.upts)
+
+    # M0: interpolation to flux points; M4: gradient at solution points
+    assert np.abs(o.m0 @ u - f(o.fpts)).max() < 1e-11
+    g = (o.m4 @ u).reshape(3, -1).T
+    assert np.abs(g - grad(o.upts)).max() < 1e-10
+
+    # Divergence of a polynomial flux with a consistent normal flux
+    F = np.stack([f(o.upts)*(d + 1) for d in range(3)])
+    fn = (o.norm_fpts*f(o.fpts)[:, None]*np.arange(1, 4)).sum(axis=1)
+    div = o.opmat('M1 - M3*M2') @ F.reshape(-1) + o.opmat('M3') @ fn
+    assert np.abs(div - grad(o.upts) @ np.arange(1, 4)).max() < 1e-10
+
+    # Lifting identity for M3 (and M6 = n (x) M3) with v of degree p
+    qp, qw = _rule(name, p + 2)
+    lu = o.ubasis.nodal_basis_at(qp)                    # (nq, nupts)
+    v, _ = _randpoly(rng, 3, p, name)
+    lhs = (lu @ o.m3).T @ (qw*v(qp))                    # (nfpts,)
+    lhs6 = [(lu @ o.m6[d*o.nupts:(d + 1)*o.nupts]).T @ (qw*v(qp))
+            for d in range(3)]
+    rhs = []
+    for (kind, proj, _), nf in zip(o.faces, o.nfacefpts):
+        fq, fw = _face_rule(kind, p + 2)
+        L = o._facebasis(kind).nodal_basis_at(fq)       # (nq, nf)
+        rhs.append(L.T @ (fw*v(ob._proj(proj, fq))))
+    rhs = np.concatenate(rhs)
+    assert np.abs(lhs - rhs).max() < 1e-11
+    for d in range(3):
+        assert np.abs(lhs6[d] - o.norm_fpts[:, d]*rhs).max() < 1e-11
+
+
+@pytest.mark.parametrize('name,cls', [('tet', TetShape), ('pri', PriShape)])
+@pytest.mark.parametrize('p', [1, 2, 3, 4])
+def test_product_operators_match_oracle(name, cls, p):
+    s, o = cls(p), _oracle(name, p)
+    assert np.array_equal(s.upts, o.upts)
+    # quad-face Gauss-Legendre points: the oracle takes numpy's rule
+    assert np.abs(s.fpts - o.fpts).max() < 1e-15
+    assert np.array_equal(s.norm_fpts, o.norm_fpts)
+    for op in OPS:
+        a, b = s.opmat(op), o.opmat(op)
+        assert a.shape == b.shape
+        assert np.array_equal(a != 0, b != 0), op
+        assert np.abs(a - b).max() < 1e-12*max(1, np.abs(b).max()), op
+
+
+def test_operator_shapes_tet_pri_p4():
+    t, pr = TetShape(4), PriShape(4)
+    assert t.opmat('M0').shape == (60, 35)
+    assert t.opmat('M1 - M3*M2').shape == (35, 105)
+    assert t.opmat('M3').shape == (35, 60)
+    assert pr.opmat('M0').shape == (105, 75)
+    assert pr.opmat('M1 - M3*M2').shape == (75, 225)
+    assert pr.opmat('M3').shape == (75, 105)
+
+
+@pytest.mark.parametrize('layout', [(2, 1, 1), (3, 2, 1), (2, 0, 2),
+                                    (2, 2, 0)])
+def test_mixed_connectivity_against_bruteforce(layout):
+    """Node-free product pairing (structured face keys + fuzzy-sorted flux
+    points) against the oracle's geometric brute force, on stacked tet/prism
+    meshes with tet-prism interfaces at both ends of the prism slab."""
+    from oracle.simplex import pair_faces_mixed, _Etype
+    from pyfr_amd.inifile import Inifile
+    from pyfr_amd.mesh import SimplexBoxMesh
+    from pyfr_amd.solvers import SHAPES, FluidElements
+    n, nt, npr = layout
+    p = 2
+    m = SimplexBoxMesh.stacked(n, nt, npr)
+    cfg = Inifile.from_dict({'solver': {'order': p},
+                             'constants': {'gamma': 1.4}})
+    eles = {et: FluidElements(SHAPES[et], v, cfg)
+            for et, v in m.spts.items()}
+    lhs, rhs = m.con
+    nfaces = sum(len(ets.basis.nfacefpts)*ets.neles for ets in eles.values())
+    assert 2*len(lhs) == nfaces
+    ours = set()
+    for (tl, el, fl), (tr, er, fr) in zip(lhs, rhs):
+        pl = eles[tl].srtd_face_fpts[fl][el]
+        pr = eles[tr].srtd_face_fpts[fr][er]
+        for a, b in zip(pl, pr):
+            ours.add(tuple(sorted(((tl, el, int(a)), (tr, er, int(b))))))
+    L = 2*np.pi
+    ets = {et: _Etype(et, p, _tables(et, p), v.transpose(0, 2, 1))
+           for et, v in m.spts.items()}
+    idx = {et: np.arange(v.shape[1]) for et, v in m.spts.items()}
+    ref = set()
+    for tl, el, il, tr, er, ir in pair_faces_mixed(ets, [((L, L, L), idx)]):
+        for a, b in zip(il, ir):
+            ref.add(tuple(sorted(((tl, el, int(a)), (tr, er, int(b))))))
+    assert ours == ref
+
+
+def test_mixed_mesh_geometry_matches_oracle():
+    from oracle.simplex import _Etype
+    from pyfr_amd.geometry import AffineGeometry
+    from pyfr_amd.mesh import SimplexBoxMesh
+    m = SimplexBoxMesh.stacked(3, 2, 1)
+    for et, v in m.spts.items():
+        g = AffineGeometry(v, et)
+        o = _Etype(et, 3, _tables(et, 3), v.transpose(0, 2, 1))
+        sh = o.sh
+        assert np.abs(g.ploc_at(sh.upts) - o.ploc(sh.upts)).max() < 1e-13
+        assert np.abs(g.rcpdjac_at(sh.upts) - o.rcpdjac).max() < 1e-13
+        assert np.abs(g.smat_at(sh.upts).transpose(1, 0, 2, 3) -
+                      o.smats).max() < 1e-13
+        assert np.abs(g.pnorm_at(sh.fpts, sh.norm_fpts) - o.pnorm).max() \
+            < 1e-13
+    # cell volumes: each cube splits into pieces of total volume h^3
+    h = 2*np.pi/3
+    vol = sum(AffineGeometry(v, et).djac.sum()*(4/3 if et == 'tet' else 4)
+              for et, v in m.spts.items())
+    assert abs(vol - 27*h**3) < 1e-9
+
+
+def test_disjoint_bench_mesh_counts():
+    from pyfr_amd.mesh import SimplexBoxMesh
+    m = SimplexBoxMesh.disjoint(4, 2)
+    assert m.spts['tet'].shape == (4, 6*4**3, 3)
+    assert m.spts['pri'].shape == (6, 2*2**3, 3)
+    # every face paired once, never across the two boxes
+    lhs, rhs = m.con
+    assert 2*len(lhs) == 4*6*4**3 + 5*2*2**3
+    assert all(a[0] == b[0] for a, b in zip(lhs, rhs))
+
+
+def _density_wave_error(n, p, kind, plant=False):
+    from oracle.simplex import OracleMixedSystem
+    from pyfr_amd.mesh import SimplexBoxMesh
+    U = np.array([0.5, 0.3, -0.2])
+    m = SimplexBoxMesh.stacked(n, n if kind == 'tet' else 0,
+                               0 if kind == 'tet' else n)
+    L = 2*np.pi
+    verts = {et: v.transpose(0, 2, 1) for et, v in m.spts.items()}
+    idx = {et: np.arange(v.shape[1]) for et, v in m.spts.items()}
+    o = OracleMixedSystem(verts, p, {et: _tables(et, p) for et in verts},
+                          [((L, L, L), idx)])
+    if plant:
+        for et in o.et.values():
+            et.M3 = -et.M3
+    fns = {'rho': lambda x, y, z: 1 + 0.2*np.sin(x + y + z),
+           'u': lambda x, y, z: U[0] + 0*x, 'v': lambda x, y, z: U[1] + 0*x,
+           'w': lambda x, y, z: U[2] + 0*x, 'p': lambda x, y, z: 1 + 0*x}
+    r = o.rhs(o.ics(fns))
+    e2 = cnt = 0
+    for et in verts:
+        x, y, z = o.et[et].ploc(o.et[et].sh.upts).transpose(1, 0, 2)
+        ex = -U.sum()*0.2*np.cos(x + y + z)
+        e2 += ((r[et][:, 0] - ex)**2).sum()
+        cnt += ex.size
+    return np.sqrt(e2/cnt)
+
+
+@pytest.mark.parametrize('kind', ['tet', 'pri'])
+def test_oracle_simplex_rhs_converges(kind):
+    """The oracle's tet/prism Euler RHS for an advected density wave
+    converges at order p (p = 2, n = 4 -> 8: measured 1.95 tets, 2.05
+    prisms); a sign error planted in its correction operator does not."""
+    e4, e8 = _density_wave_error(4, 2, kind), _density_wave_error(8, 2, kind)
+    assert np.log2(e4/e8) > 2 - 0.7
+    p4 = _density_wave_error(4, 2, kind, True)
+    p8 = _density_wave_error(8, 2, kind, True)
+    assert np.log2(p4/p8) < 2 - 0.7
