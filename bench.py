@@ -36,18 +36,36 @@ CONFIGS = {
                  dt=2e-4, workload='ns-hex64-p3-rk45-fp64'),
     'cfg0': dict(system='euler', n=8, order=2, scheme='rk4', dt=1e-3,
                  workload='euler-hex8-p2-rk4-fp64'),
+    # dense-operator mixed mesh: a periodic box of 48^3 cells x 6 tets and a
+    # periodic 16^3-cell x 2-prism slab, p=4, FP32, 8 random-phase modes
+    'cfg3': dict(system='euler', ntet=48, npri=16, order=4, scheme='rk4',
+                 dt=1e-3, precision='single',
+                 workload='euler-tet48^3x6+pri16^3x2-p4-rk4-fp32'),
 }
+
+MFMA_PEAK_TFS = {'f32': 157.3, 'f64': 78.6}
 
 
 FUSED = True
 
 
 def make_cfg(c, precision='double'):
-    from tests.fr_cases import make_cfg as mk
-    cfg = mk(c['system'], c['order'], 'rusanov', precision=precision,
-             scheme=c['scheme'], dt=c['dt'])
+    from tests.fr_cases import make_cfg as mk, make_simplex_cfg, mode_ics
+    if 'ntet' in c:
+        cfg = make_simplex_cfg(c['order'], precision, c['scheme'], c['dt'],
+                               mode_ics(seed=0, nmodes=8, amp=0.05))
+    else:
+        cfg = mk(c['system'], c['order'], 'rusanov', precision=precision,
+                 scheme=c['scheme'], dt=c['dt'])
     cfg.set('backend-hip', 'fused', str(FUSED))
     return cfg
+
+
+def make_mesh(c):
+    from pyfr_amd.mesh import BoxMesh, SimplexBoxMesh
+    if 'ntet' in c:
+        return SimplexBoxMesh.disjoint(c['ntet'], c['npri'])
+    return BoxMesh(c['n'])
 
 
 # -- algorithmic bytes per launch (every input read once, every output
@@ -85,20 +103,82 @@ def kernel_bytes(k, isz, nvars=5):
         return acc.nrow*acc.ncol*(2*8 + 2*4)
     if name.startswith('mul2_ep'):
         ep = int(name[-1])
-        a0, a1, b0, b1, out = k.mats
+        a0, a1, b0, b1, out = k.mats[:5]
         n = out.ncol
         by = (b0.nrow + b1.nrow + out.nrow)*n*isz
         if ep >= 1:
-            by += out.nrow*(n // nvars)*isz          # 1/|J|
+            rcp = k.mats[5]                          # 1/|J| per point or
+            by += rcp.nrow*(n // nvars)*isz          # per element
         if ep == 2:
             by += 2*out.nrow*n*isz                   # r1 read + write
         if ep == 3:
             by += 2*out.nrow*n*8                     # FP64 solution r/w
         return by
+    if name.startswith('tdivf_flux_ep'):
+        ep = int(name[-1])
+        a0, a1, u, sm, fn, out, rcp = k.mats
+        n = out.ncol
+        by = (u.nrow + fn.nrow + out.nrow)*n*isz
+        by += (sm.nrow + rcp.nrow)*(n // nvars)*isz   # S, 1/|J| per element
+        if ep == 2:
+            by += 2*out.nrow*n*isz
+        if ep == 3:
+            by += 2*out.nrow*n*8
+        return by
     if name == 'axnpby':
         x = k.mats
         return (len(x) + 1)*x[0].nrow*x[0].ncol*isz
     return 0
+
+
+def rhs_kernel_list(be, integ, c, precision):
+    """The kernels of one RHS evaluation plus its stage update, in launch
+    order, as the timed steps run them (after at least one step)."""
+    sysm, dt = integ.system, c['dt']
+    mixed = precision == 'mixed'
+    if c['scheme'] == 'rk45' and sysm.fused_stages:
+        # the graph of a middle RK45 stage (RHS fused with its update)
+        key = [k for k in sysm._kcache if len(k) == 5 and k[3] == 1][0]
+        return flatten(sysm._kcache[key][0].klist)
+    g = sysm._rhs_graphs(*((1, 1) if mixed else (0, 1)))[0]
+    kerns = flatten(g.klist)
+    if c['scheme'] == 'rk45' and not mixed:
+        rk = integ._rkvdh2_kerns(0, 0, 1)
+        for k in rk:
+            k.bind(dt=dt)
+        kerns += rk
+    elif mixed:
+        kerns += [mixed_stage_kernel(be, integ, dt)]
+    return kerns
+
+
+def kernel_label(k):
+    """Timing label: the kernel name plus the operator shape for operator
+    applications (different element types stay apart)."""
+    if k.name == 'mul':
+        return f'mul{k.mats[0].nrow}x{k.mats[0].ncol}'
+    if k.name.startswith('mul2_ep') or k.name.startswith('tdivf_flux'):
+        a0, a1 = k.mats[:2]
+        return f'{k.name}_{a0.nrow}x{a0.ncol}+{a1.ncol}'
+    return k.name
+
+
+def kernel_flops(k):
+    """Algorithmic flops of dense operator applications (2 m k per
+    column); 0 for everything else."""
+    if getattr(k, 'kind', None) != 'dense':
+        return 0
+    if k.name == 'mul':
+        a, b = k.mats[:2]
+        return 2*a.nrow*a.ncol*b.ncol
+    a0, a1, b0 = k.mats[:3]
+    return 2*a0.nrow*(a0.ncol + a1.ncol)*b0.ncol   # mul2 / tdivf_flux
+
+
+def mesh_label(c):
+    if 'ntet' in c:
+        return f'{c["ntet"]}^3x6 tet + {c["npri"]}^3x2 pri (periodic boxes)'
+    return f'{c["n"]}^3 hex'
 
 
 def mixed_stage_kernel(be, integ, dt):
@@ -145,26 +225,32 @@ def time_kernels(be, kerns, reps):
 
 
 def cpu_baseline(c, budget_s=12.0):
-    """The oracle (NumPy, one process) on a bounded sample of the same
-    workload: same system, order and field on a 12^3 box."""
-    os.environ.setdefault('OMP_NUM_THREADS', '1')
+    """The oracle's C/OpenMP restatement of the reference OpenMP backend's
+    RHS (oracle/crhs.c, blocked 8-wide SoA layout, sparse operators) on a
+    bounded sample of the same workload: same system, order and field on a
+    24^3 box (12^3 for cfg0's order), all host cores this process may use."""
+    from oracle.crhs import CRhs
     from oracle.rhs import OracleHexSystem
     from tests.fr_cases import oracle_consts
-    n = 12 if c['order'] >= 3 else 8
+    ncores = len(os.sched_getaffinity(0))
+    os.environ['OMP_NUM_THREADS'] = str(ncores)
+    n = 24 if c['order'] >= 3 else 12
     orc = OracleHexSystem(n, c['order'], c['system'],
                           constants=oracle_consts(c['system']))
-    u = orc.ics()
-    orc.rhs(u)
+    crhs = CRhs(orc)
+    crhs.set_u(orc.ics())
+    crhs.rhs_inplace()
     t0, nrhs = time.perf_counter(), 0
     while time.perf_counter() - t0 < budget_s:
-        orc.rhs(u)
+        crhs.rhs_inplace()
         nrhs += 1
     dt = time.perf_counter() - t0
     return {'value': orc.ndofs*nrhs/dt/1e9, 'unit': 'GDOF*RHS/s',
-            'cores': 1, 'kind': 'port',
-            'sample': f'oracle NumPy FP64 {c["system"]} RHS, {n}^3 hex '
+            'cores': ncores, 'kind': 'port',
+            'sample': f'oracle C/OpenMP FP64 {c["system"]} RHS (restatement '
+                      f'of the reference OpenMP kernels), {n}^3 hex '
                       f'p={c["order"]} ({orc.ndofs} DOF), {nrhs} RHS in '
-                      f'{dt:.1f} s'}
+                      f'{dt:.1f} s on {ncores} threads'}
 
 
 def serving_report(nreq, seed=0, rate=50.0):
@@ -262,11 +348,17 @@ def main():
                     help='skip the precision-form measurement')
     ap.add_argument('--no-serving', action='store_true',
                     help='skip the serving-form measurement')
+    ap.add_argument('--no-dense', action='store_true',
+                    help='skip the dense tet/prism mixed-mesh measurement')
     ap.add_argument('--kernel-reps', type=int, default=5)
     ap.add_argument('--serving', action='store_true',
                     help='serving form: continuous batching of a Poisson '
                     'stream of independent solves vs one at a time')
     ap.add_argument('--serving-requests', type=int, default=48)
+    ap.add_argument('--counter-probe', action='store_true',
+                    help='for rocprofv3 --pmc: one launch of each kernel of '
+                    'one RHS (+ stage), then a 1 GiB device copy as the '
+                    'traffic calibration')
     ap.add_argument('--accuracy', action='store_true',
                     help='precision form: compare mixed against FP64 at '
                     'steps 100 and 500 instead of timing')
@@ -283,25 +375,28 @@ def main():
     c = CONFIGS[args.config]
     if args.accuracy:
         return accuracy_report(c)
+    if args.counter_probe:
+        return counter_probe(c, c.get('precision', args.precision))
     if args.serving:
         print(json.dumps(serving_report(args.serving_requests)))
         return
 
-    res = measure(c, args.precision, args.steps, args.warmup,
-                  args.kernel_reps)
+    prec = c.get('precision', args.precision)
+    res = measure(c, prec, args.steps, args.warmup, args.kernel_reps)
     line = {
         'metric': 'GDOF*RHS/s', 'value': round(res['gdofs'], 4),
         'unit': 'GDOF*RHS/s', 'n_gpus': 1, 'steps': args.steps,
         'warmup': args.warmup, 'ms_per_step': round(res['ms_per_step'], 3),
         'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
         'dtype': res['dtype'],
-        'data': 'synthetic (analytic Taylor-Green field)',
+        'data': ('synthetic (8 seeded random-phase modes)' if 'ntet' in c
+                 else 'synthetic (analytic Taylor-Green field)'),
         'config': res['config'], 'roofline': res['roofline'],
         'rhs_kernels': res['rhs_kernels'],
         'rhs_alg_bytes_per_dof': res['rhs_alg_bytes_per_dof'],
         'rhs_kernel_ms_sum': res['rhs_kernel_ms_sum'],
     }
-    if args.precision == 'double' and not args.no_mixed:
+    if prec == 'double' and not args.no_mixed:
         mx = measure(c, 'mixed', args.steps, args.warmup, args.kernel_reps)
         line['precision_form'] = {
             'value': round(mx['gdofs'], 4), 'unit': 'GDOF*RHS/s',
@@ -323,8 +418,59 @@ def main():
                 'p50_latency_s': round(sv['sequential']['p50_s'], 3),
                 'p99_latency_s': round(sv['sequential']['p99_s'], 3)},
             'throughput_ratio': round(sv['throughput_ratio'], 3)}
-    line['cpu_baseline'] = None if args.no_cpu_baseline else cpu_baseline(c)
+    if not args.no_dense and args.config == 'cfg1':
+        line['dense_mixed_mesh'] = dense_report(args)
+    line['cpu_baseline'] = (None if args.no_cpu_baseline or 'ntet' in c
+                            else cpu_baseline(c))
     print(json.dumps(line))
+
+
+def counter_probe(c, precision):
+    """Launch each kernel of one RHS (+ stage update) once, in order, then
+    copy 1 GiB device to device: the per-dispatch FETCH_SIZE / WRITE_SIZE
+    of the copy calibrate the counters against a known byte count."""
+    from pyfr_amd.backend import AMDBackend
+    from pyfr_amd.solvers import get_integrator
+    cfg = make_cfg(c, precision)
+    be = AMDBackend(cfg)
+    integ = get_integrator(be, make_mesh(c), cfg)
+    integ.run_steps(1, c['dt'])
+    be.wait()
+    kerns = rhs_kernel_list(be, integ, c, precision)
+    for k in kerns:
+        print(f'# dispatch {k.name}', file=sys.stderr, flush=True)
+        k.run(be.stream)
+    be.wait()
+    n = 1 << 30
+    src, dst = ct.c_void_p(), ct.c_void_p()
+    be.lib.malloc(ct.byref(src), n)
+    be.lib.malloc(ct.byref(dst), n)
+    be.lib.memset(src.value, 1, n, be.stream)
+    be.wait()
+    print(f'# dispatch copy {n} bytes', file=sys.stderr, flush=True)
+    be.lib.memcpy_dtod(dst.value, src.value, n, be.stream)
+    be.wait()
+    be.lib.free(src.value)
+    be.lib.free(dst.value)
+    print(json.dumps({'kernels': [k.name for k in kerns],
+                      'alg_bytes': [kernel_bytes(
+                          k, np.dtype(be.fpdtype).itemsize) for k in kerns],
+                      'copy_bytes': n}))
+
+
+def dense_report(args):
+    """The dense-operator mixed mesh (tets + prisms, p=4, FP32) as a
+    secondary measurement of the same JSON line."""
+    c = CONFIGS['cfg3']
+    r = measure(c, c['precision'], max(5, args.steps//2), args.warmup,
+                args.kernel_reps)
+    return {'workload': r['config']['workload'], 'value': round(r['gdofs'], 4),
+            'unit': 'GDOF*RHS/s', 'dtype': r['dtype'],
+            'ms_per_step': round(r['ms_per_step'], 3),
+            'config': r['config'], 'roofline': r['roofline'],
+            'rhs_kernels': r['rhs_kernels'],
+            'rhs_alg_bytes_per_dof': r['rhs_alg_bytes_per_dof'],
+            'tuned_variants': r.get('tuned_variants')}
 
 
 def measure(c, precision, steps, warmup, kernel_reps):
@@ -337,7 +483,7 @@ def measure(c, precision, steps, warmup, kernel_reps):
     cfg = make_cfg(c, precision)
     t0 = time.perf_counter()
     be = AMDBackend(cfg)
-    integ = get_integrator(be, BoxMesh(c['n']), cfg)
+    integ = get_integrator(be, make_mesh(c), cfg)
     sysm = integ.system
     tsetup = time.perf_counter() - t0
     print(f'# {precision}: setup {tsetup:.1f} s, {sysm.gndofs} DOF',
@@ -356,37 +502,29 @@ def measure(c, precision, steps, warmup, kernel_reps):
     print(f'# {precision}: {steps} steps in {t1 - t0:.3f} s: '
           f'{gdofs:.3f} GDOF*RHS/s', file=sys.stderr, flush=True)
 
-    mixed = precision == 'mixed'
-    if c['scheme'] == 'rk45' and sysm.fused_stages:
-        # the graph of a middle RK45 stage (RHS fused with its update)
-        key = [k for k in sysm._kcache if len(k) == 5 and k[3] == 1][0]
-        kerns = flatten(sysm._kcache[key][0].klist)
-    else:
-        g = sysm._rhs_graphs(*((1, 1) if mixed else (0, 1)))[0]
-        kerns = flatten(g.klist)
-        if c['scheme'] == 'rk45' and not mixed:
-            rk = integ._rkvdh2_kerns(0, 0, 1)
-            for k in rk:
-                k.bind(dt=dt)
-            kerns += rk
-        elif mixed:
-            kerns += [mixed_stage_kernel(be, integ, dt)]
+    kerns = rhs_kernel_list(be, integ, c, precision)
     isz = np.dtype(be.fpdtype).itemsize
     times = time_kernels(be, kerns, kernel_reps)
     per = {}
     for k, ms in zip(kerns, times):
-        nm = k.name
-        if nm == 'mul':
-            nm = f'mul{k.mats[0].nrow}x{k.mats[0].ncol}'
-        d = per.setdefault(nm, {'ms': 0.0, 'bytes': 0, 'launches': 0})
+        nm = kernel_label(k)
+        d = per.setdefault(nm, {'ms': 0.0, 'bytes': 0, 'flops': 0,
+                                'launches': 0})
         d['ms'] += ms
         d['bytes'] += kernel_bytes(k, isz)
+        d['flops'] += kernel_flops(k)
         d['launches'] += 1
+    mpeak = MFMA_PEAK_TFS['f64' if isz == 8 else 'f32']
     for nm, d in per.items():
         d['GBs'] = d['bytes']/(d['ms']*1e-3)/1e9 if d['ms'] > 0 else 0.0
         d['frac'] = d['GBs']/HBM_PEAK_GBS
-        print(f'#   {nm:14s} {d["ms"]:8.3f} ms/RHS  {d["GBs"]:8.1f} GB/s '
-              f'({100*d["frac"]:5.1f}% of peak)', file=sys.stderr)
+        msg = (f'#   {nm:22s} {d["ms"]:8.3f} ms/RHS  {d["GBs"]:8.1f} GB/s '
+               f'({100*d["frac"]:5.1f}% of peak)')
+        if d['flops']:
+            d['TFs'] = d['flops']/(d['ms']*1e-3)/1e12
+            d['mfma_frac'] = d['TFs']/mpeak
+            msg += f'  {d["TFs"]:6.1f} TFLOP/s ({100*d["mfma_frac"]:4.1f}%)'
+        print(msg, file=sys.stderr)
     dom = max(per, key=lambda n: per[n]['ms'])
     dd = per[dom]
     roof = {'bound': 'hbm', 'kernel': dom,
@@ -394,6 +532,12 @@ def measure(c, precision, steps, warmup, kernel_reps):
             'unit': 'GB/s', 'frac': round(dd['frac'], 4),
             'traffic': None, 'bytes_per_launch': dd['bytes']//dd['launches'],
             'avg_launch_ms': round(dd['ms']/dd['launches'], 4)}
+    if dd.get('mfma_frac', 0) > dd['frac']:
+        roof.update({'bound': 'mfma', 'achieved': round(dd['TFs'], 2),
+                     'peak': mpeak, 'unit': 'TFLOP/s',
+                     'frac': round(dd['mfma_frac'], 4),
+                     'flops_per_launch': dd['flops']//dd['launches'],
+                     'hbm_GBs': round(dd['GBs'], 1)})
     dtype = {'double': 'f64', 'single': 'f32',
              'mixed': 'f32 RHS / f64 state'}[precision]
     out = {
@@ -402,7 +546,7 @@ def measure(c, precision, steps, warmup, kernel_reps):
                        '-fp64', '-' + {'double': 'fp64', 'single': 'fp32',
                                        'mixed': 'mixed'}[precision]),
                    'kernels': 'fused' if FUSED else 'reference-decomposition',
-                   'mesh': f'{c["n"]}^3 hex', 'order': c['order'],
+                   'mesh': mesh_label(c), 'order': c['order'],
                    'dof': sysm.gndofs, 'scheme': c['scheme'], 'dt': dt,
                    'rhs_per_step': integ.nrhs_per_step},
         'roofline': roof,
@@ -412,6 +556,10 @@ def measure(c, precision, steps, warmup, kernel_reps):
                                        / sysm.gndofs, 1),
         'rhs_kernel_ms_sum': round(sum(d['ms'] for d in per.values()), 3),
     }
+    tuned = [t for p in be._providers if hasattr(p, 'tuning_report')
+             for t in p.tuning_report()]
+    if tuned:
+        out['tuned_variants'] = tuned
     del integ, sysm, kerns
     be.wait()
     return out

@@ -126,6 +126,29 @@ int frhip_mul2_plan_tune(void *plan, int ncol, const void *b0, long ld0,
 int frhip_mul_plan_set_variant(void *plan, int variant);
 int frhip_mul2_plan_set_variant(void *plan, int variant);
 
+/* ---- flux-fused divergence for straight-sided tets and prisms (Euler):
+ *      tdisf (pyfr/solvers/euler/kernels/tflux.mako:8-30) + tdivtpcorf +
+ *      tdivtconf + negdivconf (pyfr/solvers/baseadvec/elements.py:84-119)
+ *      [+ rkvdh2 stage] in one matrix-core pass; the transformed flux is
+ *      computed per lane from u and the element's S and never stored.
+ * a0 = M1 - M3*M2 (m x 3 nupts), a1 = M3 (m x nfpts), m <= 80.
+ * u (nupts, 5, neles) and fn (nfpts, 5, neles) AoSoA; smats (9, neles) with
+ * leading dimension lds (row 3i + k = S[i][k]); ep and the RK arguments as
+ * for frhip_mul2 (0 store, 1 negdivconf, 2 / 3 RK45 stage); ncol = 5 *
+ * padded neles.  The tuner times 1, 2 and 4 wavefronts per block. */
+int frhip_tdivf_plan_create(void **plan, int m, int nupts, int nfpts,
+                            const double *a0, const double *a1, int dtype);
+int frhip_tdivf_plan_destroy(void *plan);
+int frhip_tdivf_plan_tune(void *plan, int ncol, const void *u, long ldu,
+                          const void *smats, long lds, const void *fn,
+                          long ldf, double gamma, int reps, int *variant);
+int frhip_tdivf_plan_set_variant(void *plan, int variant);
+int frhip_tdivf_flux_euler(void *plan, int ncol, const void *u, long ldu,
+                           const void *smats, long lds, const void *fn,
+                           long ldf, void *c, long ldc, int ep,
+                           const void *rcp, long ldr, void *xa, double dta,
+                           double dtb, int last, double gamma, void *stream);
+
 /* ---- pointwise kernels (element-major 2-D over (npts, neles)) ---- */
 
 /* Geometry modes of the tflux kernels (the `linear` argument). */

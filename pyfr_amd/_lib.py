@@ -57,6 +57,13 @@ _FUNCS = {
     'frhip_mul2_plan_set_variant': [_P, _I],
     'frhip_mul2': [_P, _I, _P, _L, _P, _L, _P, _L, _I, _P, _L, _P, _D, _D,
                    _I, _P],
+    'frhip_tdivf_plan_create': [ct.POINTER(_P), _I, _I, _I, _P, _P, _I],
+    'frhip_tdivf_plan_destroy': [_P],
+    'frhip_tdivf_plan_tune': [_P, _I, _P, _L, _P, _L, _P, _L, _D, _I,
+                              ct.POINTER(_I)],
+    'frhip_tdivf_plan_set_variant': [_P, _I],
+    'frhip_tdivf_flux_euler': [_P, _I, _P, _L, _P, _L, _P, _L, _P, _L, _I,
+                               _P, _L, _P, _D, _D, _I, _D, _P],
     'frhip_tflux_euler': [_I, _I, _I, _I, _P, _L, _P, _L, _P, _L, _P, _D,
                           _P],
     'frhip_tflux_ns': [_I, _I, _I, _I, _P, _L, _P, _L, _P, _L, _P, _L, _P,

@@ -431,6 +431,7 @@ class MulProvider:
         self.backend = backend
         self._plans = {}
         self._plans2 = {}
+        self._plans3 = {}
         self._tuned = {}
 
     def _tune(self, fn, plan, ncol, *bargs):
@@ -526,8 +527,10 @@ class MulProvider:
 
         def fn(stream, dta=0.0, dtb=0.0, last=0):
             lib.mul2(*args, dta, dtb, int(last), stream)
+        mats = (a0, a1, b0, b1, out) + (() if rcpdjac is None else
+                                        (rcpdjac,))
         k = Kernel(fn, rtargs={'dta': 0.0, 'dtb': 0.0, 'last': 0},
-                   mats=(a0, a1, b0, b1, out), name=f'mul2_ep{ep}')
+                   mats=mats, name=f'mul2_ep{ep}')
         k.kind = 'structured' if kind == 1 else 'dense'
         return k
 
@@ -543,12 +546,80 @@ class MulProvider:
         ep = {None: 1, 'rk45': 2, 'rk45-mixed': 3}[stage]
         return self._fused(a0, a1, b0, b1, out, ep, rcpdjac, xa)
 
+    def tuning_report(self):
+        """The variant the tuner chose for each dense plan and shape
+        (0: scalar kernel; 1, 2, 4: matrix-core kernel, wavefronts per
+        block)."""
+        shapes = {}
+        for plan, _, a in self._plans.values():
+            shapes[plan] = ('mul', f'{a.nrow}x{a.ncol}')
+        for plan, _, a0, a1 in self._plans2.values():
+            shapes[plan] = ('mul2', f'{a0.nrow}x{a0.ncol}+{a1.ncol}')
+        for plan, a0, a1 in self._plans3.values():
+            shapes[plan] = ('tdivf_flux', f'{a0.nrow}x{a0.ncol}+{a1.ncol}')
+        out = []
+        for key, v in self._tuned.items():
+            kind, shape = shapes.get(key[0], ('?', '?'))
+            out.append({'kernel': kind, 'operator': shape, 'ncol': key[1],
+                        'variant': v})
+        return out
+
+    def tdivf_flux(self, a0, a1, u, smats, fn, out, rcpdjac, gamma, xa=None,
+                   stage=None):
+        """tdisf + tdivtpcorf + tdivtconf + negdivconf (+ rkvdh2 stage) for
+        straight-sided elements (Euler): the flux of u is formed in the
+        kernel from one S per element (smats (9, neles))."""
+        for a in (a0, a1):
+            if 'const' not in a.tags:
+                raise NotSuitableError('tdivf_flux requires const operators')
+        nupts = u.nrow
+        if (a0.ncol != 3*nupts or a1.ncol != fn.nrow or
+                a0.nrow != out.nrow or out.ncol != u.ncol or
+                fn.ncol != u.ncol or smats.nrow != 9):
+            raise ValueError('Incompatible matrices for tdivf_flux')
+        if out.ncol % (64*5) or a0.nrow > 80:
+            raise NotSuitableError('tdivf_flux needs 5-variable AoSoA data '
+                                   'and at most 80 operator rows')
+        lib, dt = self.backend.lib, out.dtype
+        key = (id(a0), id(a1), dt)
+        if key not in self._plans3:
+            h0 = np.ascontiguousarray(a0._host, dtype=np.float64)
+            h1 = np.ascontiguousarray(a1._host, dtype=np.float64)
+            plan = ct.c_void_p()
+            lib.tdivf_plan_create(ct.byref(plan), h0.shape[0], nupts,
+                                  h1.shape[1], h0.ctypes.data,
+                                  h1.ctypes.data, _dtcode(dt))
+            self._plans3[key] = (plan.value, a0, a1)
+        plan = self._plans3[key][0]
+        ncol = out.ncol
+        if self.backend.autotune and (plan, ncol) not in self._tuned:
+            self.backend.wait()
+            v = ct.c_int()
+            lib.tdivf_plan_tune(plan, ncol, u.data, u.leaddim, smats.data,
+                                smats.leaddim, fn.data, fn.leaddim, gamma, 5,
+                                ct.byref(v))
+            self._tuned[(plan, ncol)] = v.value
+        ep = {None: 1, 'rk45': 2, 'rk45-mixed': 3}[stage]
+        args = (plan, ncol, u.data, u.leaddim, smats.data, smats.leaddim,
+                fn.data, fn.leaddim, out.data, out.leaddim, ep, rcpdjac.data,
+                _rstride(rcpdjac), None if xa is None else xa.data)
+
+        def fn_(stream, dta=0.0, dtb=0.0, last=0):
+            lib.tdivf_flux_euler(*args, dta, dtb, int(last), gamma, stream)
+        k = Kernel(fn_, rtargs={'dta': 0.0, 'dtb': 0.0, 'last': 0},
+                   mats=(a0, a1, u, smats, fn, out, rcpdjac),
+                   name=f'tdivf_flux_ep{ep}')
+        k.kind = 'dense'
+        return k
+
     def __del__(self):
         try:
             for plan, _, _ in self._plans.values():
                 self.backend.lib.mul_plan_destroy(plan)
             for plan, _, _, _ in self._plans2.values():
                 self.backend.lib.mul2_plan_destroy(plan)
+            for plan, _, _ in self._plans3.values():
+                self.backend.lib.tdivf_plan_destroy(plan)
         except Exception:
             pass
 
@@ -765,6 +836,8 @@ class AMDBackend:
         self.use_graphs = cfg.getbool('backend-hip', 'graphs', True)
         # fuse element-local operator chains where a fused kernel exists
         self.fused = cfg.getbool('backend-hip', 'fused', True)
+        # straight-sided Euler elements: fold tdisf into the divergence
+        self.fuse_flux = cfg.getbool('backend-hip', 'fuse-flux', True)
         # time the variants of dense operator plans at kernel construction
         self.autotune = cfg.getbool('backend-hip', 'autotune', True)
 
@@ -845,13 +918,16 @@ class AMDBackend:
 
     # -- kernels
     def kernel(self, name, *args, **kwargs):
+        declined = False
         for prov in self._providers:
             meth = getattr(prov, name, None)
             if meth:
                 try:
                     return meth(*args, **kwargs)
                 except NotSuitableError:
-                    continue
+                    declined = True
+        if declined:
+            raise NotSuitableError(f'No provider suits kernel {name!r} here')
         raise KeyError(f'Kernel {name!r} has no providers')
 
     def ordered_meta_kernel(self, kerns):

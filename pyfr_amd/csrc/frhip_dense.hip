@@ -11,6 +11,9 @@
 //  * the operator is pre-arranged on the host in MFMA A-fragment order, so
 //    each (m tile, k step) operand is one coalesced 256/512 B load that all
 //    wavefronts share through the L2;
+//  * the result layouts differ by type: register r of lane group kr holds
+//    tile row 4 kr + r for v_mfma_f32_16x16x4f32 and kr + 4 r for
+//    v_mfma_f64_16x16x4f64 (checked against NumPy by the GPU tests);
 //  * the 64 columns are split into four 16-column tiles interleaved with
 //    stride 4 (tile t holds columns 4j + t), so one lane's B operands for
 //    the four tiles are four consecutive columns of one row: a single
@@ -25,6 +28,7 @@
 // thread per column) covers shapes the MFMA kernel does not take and is one
 // of the variants the plan-time autotuner times.
 #include "frhip_dense.h"
+#include "frhip_physics.h"
 
 #include <algorithm>
 #include <cstring>
@@ -142,13 +146,14 @@ __global__ __launch_bounds__(256) void k_dense_mfma(
         bc = bn;
     }
 
-    // Lane (kr, cq) holds rows 16 mt + 4 kr + r of columns c0 + cq + t
+    // Lane (kr, cq), result register r holds row 16 mt + 4 kr + r (f32) or
+    // 16 mt + kr + 4 r (f64) of columns c0 + cq + t
     const long e0 = (c0/(64L*nvar))*64 + cq;
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int row = 16*mt + 4*kr + r;
+            const int row = 16*mt + (sizeof(T) == 8 ? kr + 4*r : 4*kr + r);
             if (row >= m) continue;
             const long off = row*ldc + c0 + cq;
             if constexpr (EP == 0) {
@@ -168,6 +173,150 @@ __global__ __launch_bounds__(256) void k_dense_mfma(
                 for (int t = 0; t < 4; ++t)
                     epi1<T, EP>(acc[mt][t][r], c, off + t, rcp,
                                 row*ldr + e0 + t, xa, dta, dtb, alpha, beta);
+            }
+        }
+}
+
+// ---- flux-fused divergence (affine elements, Euler) ----------------------
+//
+// tdisf + tdivtpcorf + tdivtconf + negdivconf in one pass: the transformed
+// flux F = S f(u) never goes to HBM.  Each lane evaluates the flux of its
+// wavefront's variable at one solution point for its four columns and feeds
+// it straight into the MFMA B operand.  The operator's F columns are
+// regrouped on the host into chunks of 4 points x 3 directions (k step
+// 3q + d holds points 4q..4q+3 of direction d), so lane group kr computes
+// point 4q + kr once per chunk and uses it for three k steps.  S is one
+// matrix per element (straight-sided tets and prisms), held in registers.
+// The five variable wavefronts of an element block re-read u from the L2;
+// HBM sees u, fn and the output once.
+template <typename T, int MT, int EP>
+__global__ __launch_bounds__(256) void k_tdivf_flux(
+    int m, int nupts, int nq, int ks1, int k1, const T *__restrict__ af,
+    long ncol, const T *__restrict__ u, long ldu, const T *__restrict__ sm,
+    long lds, const T *__restrict__ b1, long ld1, T *__restrict__ c,
+    long ldc, const T *__restrict__ rcp, long ldr, void *__restrict__ xa,
+    double dta, double dtb, T gm1)
+{
+    using V = typename V4<T>::type;
+    const int lane = threadIdx.x & 63;
+    const long c0 = ((long)blockIdx.x*(blockDim.x >> 6) +
+                     (threadIdx.x >> 6))*64;
+    if (c0 >= ncol) return;
+    const int kr = lane >> 4, cq = (lane & 15)*4;
+    const int v = (int)((c0 >> 6) % 5);
+    const long eb = c0/320, ecol = eb*64 + cq;
+    const int KS = 3*nq + ks1;
+
+    V S[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            S[i][k] = *reinterpret_cast<const V *>(sm + (3*i + k)*lds + ecol);
+
+    V acc[MT][4];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+            acc[mt][t] = V{0, 0, 0, 0};
+
+    const T *ub = u + eb*320 + cq;
+    const T *afl = af + lane;
+    for (int q = 0; q < nq; ++q) {
+        const int j = min(4*q + kr, nupts - 1);
+        V uw[5];
+#pragma unroll
+        for (int w = 0; w < 5; ++w)
+            uw[w] = *reinterpret_cast<const V *>(ub + j*ldu + w*64);
+
+        // this variable's flux in the three directions, transformed by S
+        // (flux.mako:3-26, tflux.mako:24-28)
+        V b[3];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const T invrho = T(1)/uw[0][t], E = uw[4][t];
+            const T rv[3] = {uw[1][t], uw[2][t], uw[3][t]};
+            const T vel[3] = {invrho*rv[0], invrho*rv[1], invrho*rv[2]};
+            const T p = gm1*(E - T(0.5)*invrho*((rv[0])*(rv[0]) +
+                                                 (rv[1])*(rv[1]) +
+                                                 (rv[2])*(rv[2])));
+            T fv[3];
+            if (v == 0) {
+                fv[0] = rv[0]; fv[1] = rv[1]; fv[2] = rv[2];
+            }
+            else if (v == 4) {
+                fv[0] = (E + p)*vel[0]; fv[1] = (E + p)*vel[1];
+                fv[2] = (E + p)*vel[2];
+            }
+            else {
+                const T vj = v == 1 ? vel[0] : v == 2 ? vel[1] : vel[2];
+                fv[0] = v == 1 ? rv[0]*vj + p : rv[0]*vj;
+                fv[1] = v == 2 ? rv[1]*vj + p : rv[1]*vj;
+                fv[2] = v == 3 ? rv[2]*vj + p : rv[2]*vj;
+            }
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+                b[d][t] = S[d][0][t]*fv[0] + S[d][1][t]*fv[1] +
+                          S[d][2][t]*fv[2];
+        }
+
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const int ks = 3*q + d;
+            T a[MT];
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt)
+                a[mt] = afl[((long)mt*KS + ks)*64];
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                    acc[mt][t] = mfma4(a[mt], b[d][t], acc[mt][t]);
+        }
+    }
+
+    // common normal fluxes at the flux points
+    auto brow = [&](int s1) -> const V * {
+        const int r = min(4*s1 + kr, k1 - 1);
+        return reinterpret_cast<const V *>(b1 + r*ld1 + c0 + cq);
+    };
+    V bc = *brow(0);
+    for (int s1 = 0; s1 < ks1; ++s1) {
+        const V bn = (s1 + 1 < ks1) ? *brow(s1 + 1) : bc;
+        const int ks = 3*nq + s1;
+        T a[MT];
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt)
+            a[mt] = afl[((long)mt*KS + ks)*64];
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+                acc[mt][t] = mfma4(a[mt], bc[t], acc[mt][t]);
+        bc = bn;
+    }
+
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = 16*mt + (sizeof(T) == 8 ? kr + 4*r : 4*kr + r);
+            if (row >= m) continue;
+            const long off = row*ldc + c0 + cq;
+            if constexpr (EP == 1) {
+                const V rv = {rcp[row*ldr + ecol], rcp[row*ldr + ecol + 1],
+                              rcp[row*ldr + ecol + 2],
+                              rcp[row*ldr + ecol + 3]};
+                const V o = {-rv[0]*acc[mt][0][r], -rv[1]*acc[mt][1][r],
+                             -rv[2]*acc[mt][2][r], -rv[3]*acc[mt][3][r]};
+                *reinterpret_cast<V *>(c + off) = o;
+            }
+            else {
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                    epi1<T, EP>(acc[mt][t][r], c, off + t, rcp,
+                                row*ldr + ecol + t, xa, dta, dtb, T(1), T(0));
             }
         }
 }
@@ -402,4 +551,240 @@ int dense_tune(DensePlan *p, long ncol, const void *b0, long ld0,
     return 0;
 }
 
+// ---- flux-fused plans -----------------------------------------------------
+
+struct FluxPlan {
+    DensePlan d;        // m, k0 = 3 nupts, k1 = nfpts, fragments
+    int nupts, nq;
+};
+
+template <typename T, int EP>
+static void launch_flux_ep(const FluxPlan *p, int wpb, long ncol,
+                           const void *u, long ldu, const void *sm, long lds,
+                           const void *fn, long ldf, void *c, long ldc,
+                           const void *rcp, long ldr, void *xa, double dta,
+                           double dtb, double gamma, hipStream_t s)
+{
+    dim3 grid(cdiv(ncol/64, wpb)), block(64*wpb);
+#define FR_MT(MTV)                                                          \
+    case MTV:                                                               \
+        hipLaunchKernelGGL((k_tdivf_flux<T, MTV, EP>), grid, block, 0, s,   \
+                           p->d.m, p->nupts, p->nq, p->d.ks1, p->d.k1,      \
+                           (const T *)p->d.afrag, ncol, (const T *)u, ldu,  \
+                           (const T *)sm, lds, (const T *)fn, ldf, (T *)c,  \
+                           ldc, (const T *)rcp, ldr, xa, dta, dtb,          \
+                           (T)(gamma - 1));                                 \
+        break;
+    switch (p->d.mt) {
+        FR_MT(1) FR_MT(2) FR_MT(3) FR_MT(4) FR_MT(5)
+    }
+#undef FR_MT
+}
+
+template <typename T>
+static int launch_flux_t(const FluxPlan *p, int wpb, int ep, long ncol,
+                         const void *u, long ldu, const void *sm, long lds,
+                         const void *fn, long ldf, void *c, long ldc,
+                         const void *rcp, long ldr, void *xa, double dta,
+                         double dtb, double gamma, hipStream_t s)
+{
+#define FR_EP(E)                                                            \
+    case E:                                                                 \
+        launch_flux_ep<T, E>(p, wpb, ncol, u, ldu, sm, lds, fn, ldf, c,     \
+                             ldc, rcp, ldr, xa, dta, dtb, gamma, s);        \
+        return 0;
+    switch (ep) {
+        FR_EP(0) FR_EP(1) FR_EP(2) FR_EP(4)
+        case 3: case 5:
+            if constexpr (sizeof(T) == 4) {
+                if (ep == 3)
+                    launch_flux_ep<T, 3>(p, wpb, ncol, u, ldu, sm, lds, fn,
+                                         ldf, c, ldc, rcp, ldr, xa, dta, dtb,
+                                         gamma, s);
+                else
+                    launch_flux_ep<T, 5>(p, wpb, ncol, u, ldu, sm, lds, fn,
+                                         ldf, c, ldc, rcp, ldr, xa, dta, dtb,
+                                         gamma, s);
+                return 0;
+            }
+            break;
+    }
+#undef FR_EP
+    set_error("invalid flux-fused epilogue");
+    return 2;
+}
+
+static int flux_launch(const FluxPlan *p, int ep, long ncol, const void *u,
+                       long ldu, const void *sm, long lds, const void *fn,
+                       long ldf, void *c, long ldc, const void *rcp, long ldr,
+                       void *xa, double dta, double dtb, double gamma,
+                       hipStream_t s)
+{
+    const size_t va = 4*(p->d.dtype == FRHIP_F64 ? 8 : 4);
+    auto al = [va](const void *q) {
+        return (reinterpret_cast<uintptr_t>(q) % va) == 0;
+    };
+    FR_CHECK(ncol % 320 == 0, "ncol must cover whole 64-element blocks of 5 "
+             "variables");
+    FR_CHECK(ldu >= ncol && ldf >= ncol && ldc >= ncol && lds*5 >= ncol,
+             "invalid leading dimensions");
+    FR_CHECK(ldu % 4 == 0 && lds % 4 == 0 && ldf % 4 == 0 && ldc % 4 == 0 &&
+             al(u) && al(sm) && al(fn) && al(c),
+             "flux-fused operands must be 4-element aligned");
+    FR_CHECK(ep == 0 || rcp, "epilogue needs 1/|J|");
+    FR_CHECK(ep < 2 || xa, "RK epilogue needs the r1 / accumulator array");
+    const int wpb = p->d.variant > 0 ? p->d.variant : 4;
+    const int rc = p->d.dtype == FRHIP_F64
+        ? launch_flux_t<double>(p, wpb, ep, ncol, u, ldu, sm, lds, fn, ldf, c,
+                                ldc, rcp, ldr, xa, dta, dtb, gamma, s)
+        : launch_flux_t<float>(p, wpb, ep, ncol, u, ldu, sm, lds, fn, ldf, c,
+                               ldc, rcp, ldr, xa, dta, dtb, gamma, s);
+    if (rc) return rc;
+    FR_LAUNCHED();
+    return 0;
+}
+
 }  // namespace frhip
+
+using namespace frhip;
+
+extern "C" {
+
+int frhip_tdivf_plan_create(void **plan, int m, int nupts, int nfpts,
+                            const double *a0, const double *a1, int dtype)
+{
+    FR_API_BEGIN
+    FR_CHECK(plan && a0 && a1, "null argument");
+    FR_CHECK(m > 0 && m <= 80 && nupts > 0 && nfpts > 0,
+             "invalid operator shape (at most 80 rows)");
+    FR_CHECK(dtype == FRHIP_F64 || dtype == FRHIP_F32, "invalid dtype");
+    const int k0 = 3*nupts, nq = (nupts + 3)/4, ks0 = 3*nq;
+    const int ks1 = (nfpts + 3)/4, KS = ks0 + ks1, mt = (m + 15)/16;
+    std::vector<double> frag((size_t)mt*KS*64, 0.0);
+    for (int t = 0; t < mt; ++t)
+        for (int s = 0; s < KS; ++s)
+            for (int l = 0; l < 64; ++l) {
+                const int r = 16*t + (l & 15), kr = l >> 4;
+                double v = 0.0;
+                if (r < m) {
+                    if (s < ks0) {
+                        const int q = s/3, d = s % 3, j = 4*q + kr;
+                        if (j < nupts) v = a0[(long)r*k0 + d*nupts + j];
+                    }
+                    else {
+                        const int kp = 4*(s - ks0) + kr;
+                        if (kp < nfpts) v = a1[(long)r*nfpts + kp];
+                    }
+                }
+                frag[((size_t)t*KS + s)*64 + l] = v;
+            }
+    const size_t esz = dtype == FRHIP_F64 ? 8 : 4;
+    std::vector<char> buf(frag.size()*esz);
+    for (size_t i = 0; i < frag.size(); ++i) {
+        if (esz == 8) std::memcpy(&buf[i*8], &frag[i], 8);
+        else {
+            const float f = (float)frag[i];
+            std::memcpy(&buf[i*4], &f, 4);
+        }
+    }
+    void *d = nullptr;
+    FR_HIP(hipMalloc(&d, buf.size()));
+    hipError_t e = hipMemcpy(d, buf.data(), buf.size(),
+                             hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(d);
+        FR_HIP(e);
+    }
+    FluxPlan *p = new FluxPlan{{m, k0, nfpts, dtype, mt, ks0, ks1, d, nullptr,
+                                4}, nupts, nq};
+    *plan = p;
+    return 0;
+    FR_API_END
+}
+
+int frhip_tdivf_plan_destroy(void *plan)
+{
+    FR_API_BEGIN
+    FluxPlan *p = static_cast<FluxPlan *>(plan);
+    if (p) {
+        if (p->d.afrag) (void)hipFree(p->d.afrag);
+        delete p;
+    }
+    return 0;
+    FR_API_END
+}
+
+int frhip_tdivf_flux_euler(void *plan, int ncol, const void *u, long ldu,
+                           const void *smats, long lds, const void *fn,
+                           long ldf, void *c, long ldc, int ep,
+                           const void *rcp, long ldr, void *xa, double dta,
+                           double dtb, int last, double gamma, void *stream)
+{
+    FR_API_BEGIN
+    FluxPlan *p = static_cast<FluxPlan *>(plan);
+    FR_CHECK(p, "null plan");
+    FR_CHECK(ep >= 0 && ep <= 3, "invalid epilogue");
+    FR_CHECK(ep != 3 || p->d.dtype == FRHIP_F32,
+             "the mixed RK epilogue needs FP32 operands");
+    if (ncol == 0) return 0;
+    const int epi = (ep >= 2 && last) ? ep + 2 : ep;
+    return flux_launch(p, epi, ncol, u, ldu, smats, lds, fn, ldf, c, ldc, rcp,
+                       ldr, xa, dta, dtb, gamma, as_stream(stream));
+    FR_API_END
+}
+
+int frhip_tdivf_plan_tune(void *plan, int ncol, const void *u, long ldu,
+                          const void *smats, long lds, const void *fn,
+                          long ldf, double gamma, int reps, int *variant)
+{
+    FR_API_BEGIN
+    FluxPlan *p = static_cast<FluxPlan *>(plan);
+    FR_CHECK(p && reps > 0, "invalid arguments");
+    const size_t esz = p->d.dtype == FRHIP_F64 ? 8 : 4;
+    void *c = nullptr;
+    FR_HIP(hipMalloc(&c, (size_t)p->d.m*ncol*esz));
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    float best = 1e30f;
+    int bestv = 4, rc = 0;
+    for (int v : {1, 2, 4}) {
+        FluxPlan q = *p;
+        q.d.variant = v;
+        for (int it = 0; it < reps + 1 && !rc; ++it) {
+            if (it == 1) (void)hipEventRecord(e0, nullptr);
+            rc = flux_launch(&q, 0, ncol, u, ldu, smats, lds, fn, ldf, c,
+                             ncol, nullptr, 0, nullptr, 0, 0, gamma, nullptr);
+        }
+        if (rc) break;
+        (void)hipEventRecord(e1, nullptr);
+        (void)hipEventSynchronize(e1);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (ms < best) {
+            best = ms;
+            bestv = v;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(c);
+    if (rc) return rc;
+    p->d.variant = bestv;
+    if (variant) *variant = bestv;
+    return 0;
+    FR_API_END
+}
+
+int frhip_tdivf_plan_set_variant(void *plan, int variant)
+{
+    FR_API_BEGIN
+    FluxPlan *p = static_cast<FluxPlan *>(plan);
+    FR_CHECK(p, "null plan");
+    FR_CHECK(variant == 1 || variant == 2 || variant == 4, "invalid variant");
+    p->d.variant = variant;
+    return 0;
+    FR_API_END
+}
+
+}  // extern "C"

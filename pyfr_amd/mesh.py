@@ -279,9 +279,17 @@ def sorted_face_fpts(plocfpts, facefpts, tol=1e-6):
         c = plocfpts[ffpts].transpose(1, 2, 0)           # (E, 3, nfp)
         rel = c - c[:, :, :1]
         q = np.round(rel/1e-9).astype(np.int64).reshape(len(c), -1)
-        uniq, first, inv = np.unique(q, axis=0, return_index=True,
-                                     return_inverse=True)
+        # group identical layouts through a 64-bit hash of the rounded
+        # coordinates, then check every element against its group's
+        # representative exactly
+        w = np.random.default_rng(0).integers(1, 2**62, q.shape[1])
+        with np.errstate(over='ignore'):
+            h = (q*w).sum(axis=1)
+        _, first, inv = np.unique(h, return_index=True, return_inverse=True)
+        inv = inv.ravel()
+        if not np.array_equal(q, q[first[inv]]):
+            raise RuntimeError('flux-point layout hash collision')
         perms = np.array([_fuzzysort(rel[i], list(range(len(ffpts))), 0, tol)
                           for i in first])
-        out.append(ffpts[perms[inv.ravel()]])
+        out.append(ffpts[perms[inv]])
     return out

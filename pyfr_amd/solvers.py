@@ -353,6 +353,20 @@ class EulerElements(FluidElements):
                 xa=None if xa is None else self.scal_upts[xa], stage=stage)
         self.kernels['tdivf'] = tdivf
 
+        # straight-sided elements, Euler: tdisf folded in as well
+        if self.affine and type(self) is EulerElements and be.fuse_flux:
+            gamma = self.cfg.getfloat('constants', 'gamma')
+
+            def tdivf_flux(uin, fout, xa=None, stage=None):
+                return be.kernel(
+                    'tdivf_flux', self.opmat('M1 - M3*M2'),
+                    self.opmat('M3'), self.scal_upts[uin],
+                    self._smats_affine, self._scal_fpts,
+                    self.scal_upts[fout], self._rcpdjac_upts, gamma,
+                    xa=None if xa is None else self.scal_upts[xa],
+                    stage=stage)
+            self.kernels['tdivf_flux'] = tdivf_flux
+
 
 class NavierStokesElements(EulerElements):
     @property
@@ -741,10 +755,17 @@ class EulerSystem(BaseSystem):
     intinterscls = EulerIntInters
 
     def _fused_kernels(self, uin, fout, xa, stage):
+        from pyfr_amd.backend import NotSuitableError
         eles = list(self.ele_map.values())
         ks = [e.kernels['disu'](uin) for e in eles]
         ks += [i.kernels['comm_flux']() for i in self._int_inters]
         for e in eles:
+            if 'tdivf_flux' in e.kernels:
+                try:
+                    ks.append(e.kernels['tdivf_flux'](uin, fout, xa, stage))
+                    continue
+                except NotSuitableError:
+                    pass
             ks += [e.kernels['tdisf'](uin),
                    e.kernels['tdivf'](fout, xa, stage)]
         return ks

@@ -20,12 +20,14 @@ def backend_factory():
 
     made = []
 
-    def make(precision='double', graphs=True, fused=True):
+    def make(precision='double', graphs=True, fused=True, fuse_flux=True):
         """precision: 'double', 'single' or 'mixed'; fused: use the fused
-        operator kernels where available."""
+        operator kernels where available; fuse_flux: also fold tdisf into
+        them for straight-sided Euler elements."""
         cfg = Inifile.from_dict({'backend': {'precision': precision},
                                  'backend-hip': {'graphs': str(graphs),
-                                                 'fused': str(fused)}})
+                                                 'fused': str(fused),
+                                                 'fuse-flux': str(fuse_flux)}})
         be = AMDBackend(cfg)
         made.append(be)
         return be

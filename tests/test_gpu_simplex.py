@@ -177,12 +177,15 @@ def _noise_floor(orc, u, seed=0):
     return rel_l2(_cat(orc.rhs(up), names), _cat(orc.rhs(u), names))
 
 
-@pytest.mark.parametrize('fused', [True, False])
+@pytest.mark.parametrize('fused', ['flux', 'div', 'none'])
 @pytest.mark.parametrize('n,nt,npr,order', [(3, 2, 1, 1), (3, 2, 1, 2),
                                             (3, 2, 1, 3), (3, 2, 1, 4),
                                             (2, 2, 0, 4), (2, 0, 2, 3)])
 def test_simplex_rhs_parity(backend_factory, n, nt, npr, order, fused):
-    be = backend_factory(fused=fused)
+    """fused: 'flux' = tdisf folded into the divergence kernel, 'div' =
+    the fused divergence on a stored flux, 'none' = the reference's kernel
+    sequence."""
+    be = backend_factory(fused=fused != 'none', fuse_flux=fused == 'flux')
     print(f'\n[simplex rhs] n={n} tets={nt} prisms={npr} p={order} '
           f'fused={fused}', flush=True)
     integ, orc, u0 = _mixed(be, n, nt, npr, order)

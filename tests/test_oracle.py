@@ -239,3 +239,22 @@ def test_pairing_bruteforce(n):
     assert len(pairs) == 3*n**3
     used = [(p[0], p[1]) for p in pairs] + [(p[3], p[4]) for p in pairs]
     assert len(set(used)) == 6*n**3
+
+
+@pytest.mark.parametrize('system,n,order', [('euler', 4, 2),
+                                            ('navier-stokes', 4, 3),
+                                            ('navier-stokes', 3, 1)])
+def test_c_restatement_matches_numpy(system, n, order):
+    """The oracle's C/OpenMP RHS (the CPU baseline) agrees with the NumPy
+    oracle to the RHS rounding floor."""
+    import subprocess
+    from oracle.crhs import LIB, CRhs
+    from tests.fr_cases import oracle_consts
+    if not os.path.exists(LIB):
+        subprocess.run(['make', '-s', '-C', os.path.dirname(LIB) + '/..'],
+                       check=True)
+    orc = OracleHexSystem(n, order, system, constants=oracle_consts(system))
+    u = orc.ics()
+    ref = orc.rhs(u)
+    got = CRhs(orc).rhs(u)
+    assert np.linalg.norm(got - ref)/np.linalg.norm(ref) < 2e-12
