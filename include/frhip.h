@@ -86,9 +86,11 @@ int frhip_mul(void *plan, int ncol, const void *b, long ldb, void *c,
               long ldc, double alpha, double beta, void *stream);
 
 /* Autotune a dense plan for (ncol, ldb): times the scalar kernel and the
- * MFMA kernel at 1, 2 and 4 wavefronts per block on the given input, writing
- * only to a scratch output, and keeps the fastest (*variant: 0 scalar, else
- * wavefronts per block; -1 for structured plans, which have no variants).
+ * MFMA kernel at 1, 2 and 4 wavefronts per block, each with the operator
+ * read through the L2 or staged in LDS once per block, on the given input,
+ * writing only to a scratch output, and keeps the fastest (*variant: 0
+ * scalar, 1/2/4 wavefronts per block, +16 for the LDS-staged form; -1 for
+ * structured plans, which have no variants).
  * Synchronous (uses the null stream); the input must be ready.  Replaces
  * the reference's rocBLAS solution-index sweep (pyfr/backends/hip/
  * rocblas.py:139-177). */
@@ -120,7 +122,7 @@ int frhip_mul2(void *plan, int ncol, const void *b0, long ld0, const void *b1,
 /* As frhip_mul_plan_tune for a two-input plan. */
 int frhip_mul2_plan_tune(void *plan, int ncol, const void *b0, long ld0,
                          const void *b1, long ld1, int reps, int *variant);
-/* Pin a dense plan's variant (0, 1, 2 or 4 as reported by the tuners), e.g.
+/* Pin a dense plan's variant (a code the tuners report), e.g.
  * from a stored tuning result.  Operands the MFMA kernel cannot take (ncol
  * not a multiple of 64, misaligned pointers) always use the scalar kernel. */
 int frhip_mul_plan_set_variant(void *plan, int variant);
@@ -135,7 +137,9 @@ int frhip_mul2_plan_set_variant(void *plan, int variant);
  * u (nupts, 5, neles) and fn (nfpts, 5, neles) AoSoA; smats (9, neles) with
  * leading dimension lds (row 3i + k = S[i][k]); ep and the RK arguments as
  * for frhip_mul2 (0 store, 1 negdivconf, 2 / 3 RK45 stage); ncol = 5 *
- * padded neles.  The tuner times 1, 2 and 4 wavefronts per block. */
+ * padded neles.  The tuner times 1, 2 and 4 wavefronts per block, with
+ * and without the LDS-staged operator (variant codes as above, without the
+ * scalar kernel). */
 int frhip_tdivf_plan_create(void **plan, int m, int nupts, int nfpts,
                             const double *a0, const double *a1, int dtype);
 int frhip_tdivf_plan_destroy(void *plan);

@@ -10,12 +10,14 @@ struct DensePlan {
     int mt, ks0, ks1;   // 16-row tiles of A; 4-row k steps of each part
     void *afrag;        // [A0 | A1] in MFMA A-fragment order
     void *arow;         // [A0 | A1] row-major (scalar kernel)
-    int variant;        // 0: scalar kernel; 1, 2, 4: MFMA, wavefronts/block
+    int variant;        // 0: scalar kernel; 1, 2, 4: MFMA, wavefronts per
+                        // block; 16 + (1, 2, 4): operator staged in LDS
 };
 
 int dense_plan_create(DensePlan **out, int m, int k0, int k1,
                       const double *a0, const double *a1, int dtype);
 void dense_plan_destroy(DensePlan *p);
+bool dense_variant_ok(const DensePlan *p, int variant);
 
 // ep: 0 store, 1 negdivconf, 2/4 RK45 stage / last stage, 3/5 mixed RK45
 // stage / last stage, 6 c += acc, 7 c = alpha*acc + beta*c, 8 c = alpha*acc.

@@ -36,6 +36,9 @@
 
 namespace frhip {
 
+// B rows in flight per wavefront in the k loops
+constexpr int FR_PD = 3;
+
 template <typename T> struct V4;
 template <> struct V4<float> {
     typedef float type __attribute__((ext_vector_type(4)));
@@ -97,7 +100,7 @@ __device__ __forceinline__ void epi1(T acc, T *__restrict__ c, long off,
 
 // ---- MFMA kernel ----------------------------------------------------------
 template <typename T, int MT, int EP>
-__global__ __launch_bounds__(256) void k_dense_mfma(
+__device__ __forceinline__ void dense_tiles(
     int m, int ks0, int ks1, int k0, int k1, const T *__restrict__ af,
     long ncol, const T *__restrict__ b0, long ld0, const T *__restrict__ b1,
     long ld1, T *__restrict__ c, long ldc, const T *__restrict__ rcp,
@@ -106,11 +109,14 @@ __global__ __launch_bounds__(256) void k_dense_mfma(
 {
     using V = typename V4<T>::type;
     const int lane = threadIdx.x & 63;
-    const long c0 = ((long)blockIdx.x*(blockDim.x >> 6) +
-                     (threadIdx.x >> 6))*64;
-    if (c0 >= ncol) return;
     const int kr = lane >> 4, cq = (lane & 15)*4;
     const int KS = ks0 + ks1;
+    const int W = blockDim.x >> 6;
+    const T *afl = af + lane;
+    {
+    const long g = (long)blockIdx.x*W + (threadIdx.x >> 6);
+    if (g*64 >= ncol) return;
+    const long c0 = g*64;
 
     V acc[MT][4];
 #pragma unroll
@@ -130,20 +136,28 @@ __global__ __launch_bounds__(256) void k_dense_mfma(
         return reinterpret_cast<const V *>(b1 + r*ld1 + c0 + cq);
     };
 
-    const T *afl = af + lane;
-    V bc = *brow(0);
-    for (int ks = 0; ks < KS; ++ks) {
-        const V bn = (ks + 1 < KS) ? *brow(ks + 1) : bc;
-        T a[MT];
+    // B rows stream through a ring of FR_PD registers: the row of step
+    // ks + FR_PD is requested while step ks computes (past the end the
+    // last row is re-read, an L2 hit)
+    V bq[FR_PD];
 #pragma unroll
-        for (int mt = 0; mt < MT; ++mt)
-            a[mt] = afl[((long)mt*KS + ks)*64];
+    for (int i = 0; i < FR_PD; ++i) bq[i] = *brow(min(i, KS - 1));
+    for (int kb = 0; kb < KS; kb += FR_PD) {
 #pragma unroll
-        for (int mt = 0; mt < MT; ++mt)
+        for (int i = 0; i < FR_PD; ++i) {
+            const int ks = kb + i;
+            if (ks >= KS) break;
+            T a[MT];
 #pragma unroll
-            for (int t = 0; t < 4; ++t)
-                acc[mt][t] = mfma4(a[mt], bc[t], acc[mt][t]);
-        bc = bn;
+            for (int mt = 0; mt < MT; ++mt)
+                a[mt] = afl[((long)mt*KS + ks)*64];
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                    acc[mt][t] = mfma4(a[mt], bq[i][t], acc[mt][t]);
+            bq[i] = *brow(min(ks + FR_PD, KS - 1));
+        }
     }
 
     // Lane (kr, cq), result register r holds row 16 mt + 4 kr + r (f32) or
@@ -175,6 +189,36 @@ __global__ __launch_bounds__(256) void k_dense_mfma(
                                 row*ldr + e0 + t, xa, dta, dtb, alpha, beta);
             }
         }
+    }
+}
+
+// The block's copy of the operator fragments in LDS (dynamic shared
+// memory), filled cooperatively before any wavefront of the block exits, so
+// the operator is read from the L2 once per block instead of once per
+// wavefront and k step.  (A persistent grid looping over column groups was
+// tried: the loop doubled the VGPR count and halved occupancy.)
+template <typename T>
+__device__ __forceinline__ const T *stage_frags(const T *__restrict__ af,
+                                                int n)
+{
+    extern __shared__ __attribute__((aligned(16))) char fr_smem[];
+    T *sa = reinterpret_cast<T *>(fr_smem);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) sa[i] = af[i];
+    __syncthreads();
+    return sa;
+}
+
+template <typename T, int MT, int EP, bool LDSA>
+__global__ __launch_bounds__(256) void k_dense_mfma(
+    int m, int ks0, int ks1, int k0, int k1, const T *__restrict__ af,
+    long ncol, const T *__restrict__ b0, long ld0, const T *__restrict__ b1,
+    long ld1, T *__restrict__ c, long ldc, const T *__restrict__ rcp,
+    long ldr, int nvar, void *__restrict__ xa, double dta, double dtb,
+    T alpha, T beta)
+{
+    const T *a = LDSA ? stage_frags(af, MT*(ks0 + ks1)*64) : af;
+    dense_tiles<T, MT, EP>(m, ks0, ks1, k0, k1, a, ncol, b0, ld0, b1, ld1, c,
+                           ldc, rcp, ldr, nvar, xa, dta, dtb, alpha, beta);
 }
 
 // ---- flux-fused divergence (affine elements, Euler) ----------------------
@@ -190,7 +234,7 @@ __global__ __launch_bounds__(256) void k_dense_mfma(
 // The five variable wavefronts of an element block re-read u from the L2;
 // HBM sees u, fn and the output once.
 template <typename T, int MT, int EP>
-__global__ __launch_bounds__(256) void k_tdivf_flux(
+__device__ __forceinline__ void flux_tiles(
     int m, int nupts, int nq, int ks1, int k1, const T *__restrict__ af,
     long ncol, const T *__restrict__ u, long ldu, const T *__restrict__ sm,
     long lds, const T *__restrict__ b1, long ld1, T *__restrict__ c,
@@ -199,13 +243,16 @@ __global__ __launch_bounds__(256) void k_tdivf_flux(
 {
     using V = typename V4<T>::type;
     const int lane = threadIdx.x & 63;
-    const long c0 = ((long)blockIdx.x*(blockDim.x >> 6) +
-                     (threadIdx.x >> 6))*64;
-    if (c0 >= ncol) return;
     const int kr = lane >> 4, cq = (lane & 15)*4;
+    const int KS = 3*nq + ks1;
+    const int W = blockDim.x >> 6;
+    const T *afl = af + lane;
+    {
+    const long g = (long)blockIdx.x*W + (threadIdx.x >> 6);
+    if (g*64 >= ncol) return;
+    const long c0 = g*64;
     const int v = (int)((c0 >> 6) % 5);
     const long eb = c0/320, ecol = eb*64 + cq;
-    const int KS = 3*nq + ks1;
 
     V S[3][3];
 #pragma unroll
@@ -222,13 +269,20 @@ __global__ __launch_bounds__(256) void k_tdivf_flux(
             acc[mt][t] = V{0, 0, 0, 0};
 
     const T *ub = u + eb*320 + cq;
-    const T *afl = af + lane;
+    // u of chunk q + 1 is requested while chunk q computes
+    V un[5];
+#pragma unroll
+    for (int w = 0; w < 5; ++w)
+        un[w] = *reinterpret_cast<const V *>(ub + min(kr, nupts - 1)*ldu +
+                                             w*64);
     for (int q = 0; q < nq; ++q) {
-        const int j = min(4*q + kr, nupts - 1);
         V uw[5];
 #pragma unroll
+        for (int w = 0; w < 5; ++w) uw[w] = un[w];
+        const int jn = min(4*(q + 1) + kr, nupts - 1);
+#pragma unroll
         for (int w = 0; w < 5; ++w)
-            uw[w] = *reinterpret_cast<const V *>(ub + j*ldu + w*64);
+            un[w] = *reinterpret_cast<const V *>(ub + jn*ldu + w*64);
 
         // this variable's flux in the three directions, transformed by S
         // (flux.mako:3-26, tflux.mako:24-28)
@@ -281,20 +335,26 @@ __global__ __launch_bounds__(256) void k_tdivf_flux(
         const int r = min(4*s1 + kr, k1 - 1);
         return reinterpret_cast<const V *>(b1 + r*ld1 + c0 + cq);
     };
-    V bc = *brow(0);
-    for (int s1 = 0; s1 < ks1; ++s1) {
-        const V bn = (s1 + 1 < ks1) ? *brow(s1 + 1) : bc;
-        const int ks = 3*nq + s1;
-        T a[MT];
+    V bq[FR_PD];
 #pragma unroll
-        for (int mt = 0; mt < MT; ++mt)
-            a[mt] = afl[((long)mt*KS + ks)*64];
+    for (int i = 0; i < FR_PD; ++i) bq[i] = *brow(min(i, ks1 - 1));
+    for (int kb = 0; kb < ks1; kb += FR_PD) {
 #pragma unroll
-        for (int mt = 0; mt < MT; ++mt)
+        for (int i = 0; i < FR_PD; ++i) {
+            const int s1 = kb + i;
+            if (s1 >= ks1) break;
+            const int ks = 3*nq + s1;
+            T a[MT];
 #pragma unroll
-            for (int t = 0; t < 4; ++t)
-                acc[mt][t] = mfma4(a[mt], bc[t], acc[mt][t]);
-        bc = bn;
+            for (int mt = 0; mt < MT; ++mt)
+                a[mt] = afl[((long)mt*KS + ks)*64];
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                    acc[mt][t] = mfma4(a[mt], bq[i][t], acc[mt][t]);
+            bq[i] = *brow(min(s1 + FR_PD, ks1 - 1));
+        }
     }
 
 #pragma unroll
@@ -319,6 +379,20 @@ __global__ __launch_bounds__(256) void k_tdivf_flux(
                                 row*ldr + ecol + t, xa, dta, dtb, T(1), T(0));
             }
         }
+    }
+}
+
+template <typename T, int MT, int EP, bool LDSA>
+__global__ __launch_bounds__(256) void k_tdivf_flux(
+    int m, int nupts, int nq, int ks1, int k1, const T *__restrict__ af,
+    long ncol, const T *__restrict__ u, long ldu, const T *__restrict__ sm,
+    long lds, const T *__restrict__ b1, long ld1, T *__restrict__ c,
+    long ldc, const T *__restrict__ rcp, long ldr, void *__restrict__ xa,
+    double dta, double dtb, T gm1)
+{
+    const T *a = LDSA ? stage_frags(af, MT*(3*nq + ks1)*64) : af;
+    flux_tiles<T, MT, EP>(m, nupts, nq, ks1, k1, a, ncol, u, ldu, sm, lds,
+                          b1, ld1, c, ldc, rcp, ldr, xa, dta, dtb, gm1);
 }
 
 // ---- scalar kernel --------------------------------------------------------
@@ -346,6 +420,21 @@ __global__ __launch_bounds__(256) void k_dense_fma(
 }
 
 // ---- dispatch -------------------------------------------------------------
+
+// Variant codes: 0 scalar kernel; 1, 2, 4 MFMA with that many wavefronts
+// per block and the operator read through the L2; 16 + (1, 2, 4) the same
+// with the operator staged in LDS once per block.
+constexpr int FR_LDSV = 16;
+constexpr size_t FR_LDS_MAX = 65536;
+
+static bool variant_valid(int v, size_t ldsbytes, bool scalar_ok)
+{
+    const int w = v & ~FR_LDSV;
+    if (v == 0) return scalar_ok;
+    if (w != 1 && w != 2 && w != 4) return false;
+    return !(v & FR_LDSV) || ldsbytes <= FR_LDS_MAX;
+}
+
 template <typename T, int EP>
 static int launch_ep(const DensePlan *p, int variant, long ncol,
                      const void *b0, long ld0, const void *b1, long ld1,
@@ -363,16 +452,22 @@ static int launch_ep(const DensePlan *p, int variant, long ncol,
                            (T)alpha, (T)beta);
         return 0;
     }
-    const int wpb = variant;                 // wavefronts per block
-    dim3 grid(cdiv(ncol/64, wpb)), block(64*wpb);
+    const int wpb = variant & ~FR_LDSV;      // wavefronts per block
+    const bool lds = variant & FR_LDSV;
+    const size_t ldsb = lds ? (size_t)p->mt*(p->ks0 + p->ks1)*64*sizeof(T)
+                            : 0;
+    dim3 block(64*wpb);
 #define FR_MT(MTV)                                                          \
-    case MTV:                                                               \
-        hipLaunchKernelGGL((k_dense_mfma<T, MTV, EP>), grid, block, 0, s,   \
-                           p->m, p->ks0, p->ks1, p->k0, p->k1,              \
-                           (const T *)p->afrag, ncol, pb0, ld0, pb1, ld1,   \
-                           (T *)c, ldc, prcp, ldr, nvar, xa, dta, dtb,      \
-                           (T)alpha, (T)beta);                              \
-        return 0;
+    case MTV: {                                                             \
+        auto k = lds ? k_dense_mfma<T, MTV, EP, true>                       \
+                     : k_dense_mfma<T, MTV, EP, false>;                     \
+        dim3 grid(cdiv(ncol/64, wpb));                                      \
+        hipLaunchKernelGGL(k, grid, block, ldsb, s, p->m, p->ks0, p->ks1,   \
+                           p->k0, p->k1, (const T *)p->afrag, ncol, pb0,    \
+                           ld0, pb1, ld1, (T *)c, ldc, prcp, ldr, nvar, xa, \
+                           dta, dtb, (T)alpha, (T)beta);                    \
+        return 0;                                                           \
+    }
     switch (p->mt) {
         FR_MT(1) FR_MT(2) FR_MT(3) FR_MT(4) FR_MT(5) FR_MT(6) FR_MT(7)
     }
@@ -477,6 +572,13 @@ int dense_plan_create(DensePlan **out, int m, int k0, int k1,
     return 0;
 }
 
+bool dense_variant_ok(const DensePlan *p, int variant)
+{
+    const size_t esz = p->dtype == FRHIP_F64 ? 8 : 4;
+    return variant_valid(variant, (size_t)p->mt*(p->ks0 + p->ks1)*64*esz,
+                         true);
+}
+
 void dense_plan_destroy(DensePlan *p)
 {
     if (!p) return;
@@ -493,6 +595,10 @@ int dense_launch(const DensePlan *p, int ep, long ncol, const void *b0,
     int variant = p->variant;
     if (variant != 0 && !mfma_ok(p, ncol, b0, ld0, b1, ld1, c, ldc))
         variant = 0;
+    const size_t esz = p->dtype == FRHIP_F64 ? 8 : 4;
+    if (!variant_valid(variant, (size_t)p->mt*(p->ks0 + p->ks1)*64*esz,
+                       true))
+        variant &= ~FR_LDSV;
     const int rc = p->dtype == FRHIP_F64
         ? launch_t<double>(p, variant, ep, ncol, b0, ld0, b1, ld1, c, ldc,
                            rcp, ldr, nvar, xa, dta, dtb, alpha, beta, s)
@@ -521,10 +627,12 @@ int dense_tune(DensePlan *p, long ncol, const void *b0, long ld0,
     hipEvent_t e0, e1;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
-    const int cands[] = {0, 1, 2, 4};
+    const int cands[] = {0, 1, 2, 4, FR_LDSV + 1, FR_LDSV + 2, FR_LDSV + 4};
+    const size_t ldsb = (size_t)p->mt*(p->ks0 + p->ks1)*64*esz;
     float best = 1e30f;
     int bestv = p->variant, rc = 0;
     for (int v : cands) {
+        if (!variant_valid(v, ldsb, true)) continue;
         DensePlan q = *p;
         q.variant = v;
         for (int it = 0; it < reps + 1 && !rc; ++it) {
@@ -565,16 +673,24 @@ static void launch_flux_ep(const FluxPlan *p, int wpb, long ncol,
                            const void *rcp, long ldr, void *xa, double dta,
                            double dtb, double gamma, hipStream_t s)
 {
-    dim3 grid(cdiv(ncol/64, wpb)), block(64*wpb);
+    const bool stage = wpb & FR_LDSV;
+    const int w = wpb & ~FR_LDSV;
+    const size_t ldsb = stage ? (size_t)p->d.mt*(p->d.ks0 + p->d.ks1)*64*
+                                    sizeof(T) : 0;
+    dim3 block(64*w);
 #define FR_MT(MTV)                                                          \
-    case MTV:                                                               \
-        hipLaunchKernelGGL((k_tdivf_flux<T, MTV, EP>), grid, block, 0, s,   \
-                           p->d.m, p->nupts, p->nq, p->d.ks1, p->d.k1,      \
-                           (const T *)p->d.afrag, ncol, (const T *)u, ldu,  \
-                           (const T *)sm, lds, (const T *)fn, ldf, (T *)c,  \
-                           ldc, (const T *)rcp, ldr, xa, dta, dtb,          \
+    case MTV: {                                                             \
+        auto k = stage ? k_tdivf_flux<T, MTV, EP, true>                     \
+                       : k_tdivf_flux<T, MTV, EP, false>;                   \
+        dim3 grid(cdiv(ncol/64, w));                                        \
+        hipLaunchKernelGGL(k, grid, block, ldsb, s, p->d.m, p->nupts,       \
+                           p->nq, p->d.ks1, p->d.k1, (const T *)p->d.afrag, \
+                           ncol, (const T *)u, ldu, (const T *)sm, lds,     \
+                           (const T *)fn, ldf, (T *)c, ldc,                 \
+                           (const T *)rcp, ldr, xa, dta, dtb,               \
                            (T)(gamma - 1));                                 \
-        break;
+        break;                                                              \
+    }
     switch (p->d.mt) {
         FR_MT(1) FR_MT(2) FR_MT(3) FR_MT(4) FR_MT(5)
     }
@@ -633,7 +749,11 @@ static int flux_launch(const FluxPlan *p, int ep, long ncol, const void *u,
              "flux-fused operands must be 4-element aligned");
     FR_CHECK(ep == 0 || rcp, "epilogue needs 1/|J|");
     FR_CHECK(ep < 2 || xa, "RK epilogue needs the r1 / accumulator array");
-    const int wpb = p->d.variant > 0 ? p->d.variant : 4;
+    const size_t esz = p->d.dtype == FRHIP_F64 ? 8 : 4;
+    int wpb = p->d.variant > 0 ? p->d.variant : 4;
+    if (!variant_valid(wpb, (size_t)p->d.mt*(p->d.ks0 + p->d.ks1)*64*esz,
+                       false))
+        wpb &= ~FR_LDSV;
     const int rc = p->d.dtype == FRHIP_F64
         ? launch_flux_t<double>(p, wpb, ep, ncol, u, ldu, sm, lds, fn, ldf, c,
                                 ldc, rcp, ldr, xa, dta, dtb, gamma, s)
@@ -748,7 +868,9 @@ int frhip_tdivf_plan_tune(void *plan, int ncol, const void *u, long ldu,
     (void)hipEventCreate(&e1);
     float best = 1e30f;
     int bestv = 4, rc = 0;
-    for (int v : {1, 2, 4}) {
+    const size_t ldsb = (size_t)p->d.mt*(p->d.ks0 + p->d.ks1)*64*esz;
+    for (int v : {1, 2, 4, FR_LDSV + 1, FR_LDSV + 2, FR_LDSV + 4}) {
+        if (!variant_valid(v, ldsb, false)) continue;
         FluxPlan q = *p;
         q.d.variant = v;
         for (int it = 0; it < reps + 1 && !rc; ++it) {
@@ -781,7 +903,10 @@ int frhip_tdivf_plan_set_variant(void *plan, int variant)
     FR_API_BEGIN
     FluxPlan *p = static_cast<FluxPlan *>(plan);
     FR_CHECK(p, "null plan");
-    FR_CHECK(variant == 1 || variant == 2 || variant == 4, "invalid variant");
+    const size_t esz = p->d.dtype == FRHIP_F64 ? 8 : 4;
+    FR_CHECK(variant_valid(variant,
+                           (size_t)p->d.mt*(p->d.ks0 + p->d.ks1)*64*esz,
+                           false), "invalid variant");
     p->d.variant = variant;
     return 0;
     FR_API_END

@@ -172,7 +172,7 @@ class SimplexBoxMesh:
     conform within and across element types.
 
     ``spts``: {etype: (nverts, neles, 3)}; elements are ordered by block,
-    then cell (x fastest), then split index.  ``con``: (lhs, rhs) lists of
+    then split index, then cell (x fastest).  ``con``: (lhs, rhs) lists of
     (etype, eidx, fidx), each face pair once, left side the smaller
     (etype index, face index, element index) key, sorted by left side.
     """
@@ -193,8 +193,11 @@ class SimplexBoxMesh:
                                          indexing='ij')
                 cell = np.stack([ii.ravel(), jj.ravel(), kk.ravel()], 1)
                 off = CELL_SPLITS[et]                   # (ns, nv, 3)
-                lat = cell[:, None, None, :] + off[None]   # (C, ns, nv, 3)
-                lat = lat.reshape(-1, off.shape[1], 3)     # (E, nv, 3)
+                # split-major order: consecutive elements have the same
+                # orientation, so their neighbours' flux points sit in the
+                # same rows and interface gathers stay coalesced
+                lat = off[:, None] + cell[None, :, None, :]  # (ns, C, nv, 3)
+                lat = lat.reshape(-1, off.shape[1], 3)       # (E, nv, 3)
                 verts[et].append(lat*h)
                 nodes[et].append((lat, np.full(len(lat), noff),
                                   np.array([nx, ny, nz])))
