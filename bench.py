@@ -152,6 +152,21 @@ def rhs_kernel_list(be, integ, c, precision):
     return kerns
 
 
+def measured_traffic(c, precision, label):
+    """HBM bytes per launch of a kernel from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary of the same workload (None if it holds no
+    figure for it)."""
+    fn = os.path.join(ROOT, 'profiles', 'pmc-traffic_rhs-kernels.json')
+    try:
+        data = json.load(open(fn))['workloads']
+    except (OSError, KeyError, ValueError):
+        return None
+    wl = c['workload'].replace('-fp64', '-' + {
+        'double': 'fp64', 'single': 'fp32', 'mixed': 'mixed'}[precision])
+    ent = data.get(wl, {}).get(label)
+    return ent['traffic_bytes'] if ent else None
+
+
 def kernel_label(k):
     """Timing label: the kernel name plus the operator shape for operator
     applications (different element types stay apart)."""
@@ -453,6 +468,7 @@ def counter_probe(c, precision):
     be.lib.free(src.value)
     be.lib.free(dst.value)
     print(json.dumps({'kernels': [k.name for k in kerns],
+                      'labels': [kernel_label(k) for k in kerns],
                       'alg_bytes': [kernel_bytes(
                           k, np.dtype(be.fpdtype).itemsize) for k in kerns],
                       'copy_bytes': n}))
@@ -532,6 +548,7 @@ def measure(c, precision, steps, warmup, kernel_reps):
             'unit': 'GB/s', 'frac': round(dd['frac'], 4),
             'traffic': None, 'bytes_per_launch': dd['bytes']//dd['launches'],
             'avg_launch_ms': round(dd['ms']/dd['launches'], 4)}
+    roof['traffic'] = measured_traffic(c, precision, dom)
     if dd.get('mfma_frac', 0) > dd['frac']:
         roof.update({'bound': 'mfma', 'achieved': round(dd['TFs'], 2),
                      'peak': mpeak, 'unit': 'TFLOP/s',

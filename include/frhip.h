@@ -28,7 +28,9 @@ extern "C" {
 #endif
 
 enum { FRHIP_F64 = 0, FRHIP_F32 = 1 };
-enum { FRHIP_RS_RUSANOV = 0, FRHIP_RS_HLLC = 1 };
+/* Riemann solvers (pyfr/solvers/euler/kernels/rsolvers/*.mako). */
+enum { FRHIP_RS_RUSANOV = 0, FRHIP_RS_HLLC = 1, FRHIP_RS_HLL = 2,
+       FRHIP_RS_ROE = 3, FRHIP_RS_ROEM = 4 };
 
 /* ---- runtime: replaces the ctypes HIP driver wrapper
  *      pyfr/backends/hip/driver.py:402-482 (hipSetDevice, hipMalloc, hipFree,

@@ -83,7 +83,186 @@ def hllc(ul, ur, n, gamma):
     return nf
 
 
-RSOLVERS = {'rusanov': rusanov, 'hllc': hllc}
+def hll(ul, ur, n, gamma):
+    """``pyfr/solvers/euler/kernels/rsolvers/hll.mako:4-53`` (including its
+    left wave-speed estimate min(nv - a, nvl - ar))."""
+    fl, pl, vl = inviscid_flux(ul, gamma)
+    fr, pr, vr = inviscid_flux(ur, gamma)
+    nvl = n[0]*vl[0] + n[1]*vl[1] + n[2]*vl[2]
+    nvr = n[0]*vr[0] + n[1]*vr[1] + n[2]*vr[2]
+    ar = np.sqrt(gamma*pr/ur[0])
+    srl, srr = np.sqrt(ul[0]), np.sqrt(ur[0])
+    nv = (srl*nvl + srr*nvr)/(srl + srr)
+    H = (srl*(pr + ur[4]) + srr*(pl + ul[4]))/(srl*ur[0] + srr*ul[0])
+    inv_rar = 1/(srl + srr)
+    va = [(vl[i]*srl + vr[i]*srr)*inv_rar for i in range(3)]
+    qq = va[0]*va[0] + va[1]*va[1] + va[2]*va[2]
+    a = np.sqrt((gamma - 1)*(H - 0.5*qq))
+    sl = np.minimum(nv - a, nvl - ar)
+    sr = np.maximum(nv + a, nvr + ar)
+    rcpsrsl = 1/(sr - sl)
+    nf = np.empty_like(ul)
+    for i in range(5):
+        nf_fl = n[0]*fl[0, i] + n[1]*fl[1, i] + n[2]*fl[2, i]
+        nf_fr = n[0]*fr[0, i] + n[1]*fr[1, i] + n[2]*fr[2, i]
+        nf_sub = (sr*nf_fl - sl*nf_fr + sl*sr*(ur[i] - ul[i]))*rcpsrsl
+        nf[i] = np.where(0 <= sl, nf_fl, np.where(0 >= sr, nf_fr, nf_sub))
+    return nf
+
+
+def _transform_to(n, u):
+    """Rotate u[1:4] into the frame whose first axis is n (Moler & Hughes;
+    ``pyfr/solvers/baseadvec/kernels/transform.mako``, 3-D branch)."""
+    t = np.empty((3,) + np.shape(u[1]))
+    b1 = np.abs(n[0]) < 0.99
+    b2 = ~b1 & (np.abs(n[1]) < np.abs(n[2]))
+    b3 = ~b1 & ~b2
+    with np.errstate(divide='ignore', invalid='ignore'):
+        h1, h2, h3 = 1/(1 + n[0]), 1/(1 - n[1]), 1/(1 - n[2])
+    u0, u1, u2 = u[1], u[2], u[3]
+    t[0] = n[0]*u0 + n[1]*u1 + n[2]*u2
+    t[1] = np.where(
+        b1, (n[0] + h1*n[2]*n[2])*u1 - n[1]*u0 - h1*n[1]*n[2]*u2,
+        np.where(b2, (1 - h2*n[0]*n[0])*u0 + n[0]*u1 - h2*n[0]*n[2]*u2,
+                 (1 - h3*n[1]*n[1])*u1 - h3*n[0]*n[1]*u0 + n[1]*u2))
+    t[2] = np.where(
+        b1, (n[0] + h1*n[1]*n[1])*u2 - n[2]*u0 - h1*n[1]*n[2]*u1,
+        np.where(b2, n[2]*u1 - h2*n[0]*n[2]*u0 + (1 - h2*n[2]*n[2])*u2,
+                 (1 - h3*n[0]*n[0])*u0 - h3*n[0]*n[1]*u1 + n[0]*u2))
+    return t
+
+
+def _transform_from(n, t):
+    """Inverse of ``_transform_to`` (``transform.mako``, 3-D branch)."""
+    u = np.empty_like(t)
+    b1 = np.abs(n[0]) < 0.99
+    b2 = ~b1 & (np.abs(n[1]) < np.abs(n[2]))
+    with np.errstate(divide='ignore', invalid='ignore'):
+        h1, h2, h3 = 1/(1 + n[0]), 1/(1 - n[1]), 1/(1 - n[2])
+    t0, t1, t2 = t
+    u[0] = np.where(b1, n[0]*t0 - n[1]*t1 - n[2]*t2,
+                    np.where(b2, n[0]*t0 + (1 - h2*n[0]*n[0])*t1
+                             - h2*n[0]*n[2]*t2,
+                             n[0]*t0 - h3*n[0]*n[1]*t1
+                             + (1 - h3*n[0]*n[0])*t2))
+    u[1] = np.where(b1, n[1]*t0 + (n[0] + h1*n[2]*n[2])*t1
+                    - h1*n[1]*n[2]*t2,
+                    np.where(b2, n[1]*t0 + n[0]*t1 + n[2]*t2,
+                             n[1]*t0 + (1 - h3*n[1]*n[1])*t1
+                             - h3*n[0]*n[1]*t2))
+    u[2] = np.where(b1, n[2]*t0 - h1*n[1]*n[2]*t1 + (n[0] + h1*n[1]*n[1])*t2,
+                    np.where(b2, n[2]*t0 - h2*n[0]*n[2]*t1
+                             + (1 - h2*n[2]*n[2])*t2,
+                             n[2]*t0 + n[1]*t1 + n[0]*t2))
+    return u
+
+
+def _flux_1d(s, gamma):
+    """``flux.mako: inviscid_flux_1d``."""
+    invrho = 1.0/s[0]
+    E = s[4]
+    v = [invrho*s[1], invrho*s[2], invrho*s[3]]
+    p = (gamma - 1)*(E - 0.5*invrho*(s[1]*s[1] + s[2]*s[2] + s[3]*s[3]))
+    f = [s[1], s[1]*v[0] + p, s[1]*v[1], s[1]*v[2], (E + p)*v[0]]
+    return f, p, v
+
+
+def roe(ul, ur, n, gamma):
+    """Roe with a Harten entropy fix (eps = 0.001) in the face frame;
+    ``rsolvers/roe.mako:4-76`` through ``rsolve1d.mako:1-21``."""
+    utl = np.stack([ul[0], *_transform_to(n, ul), ul[4]])
+    utr = np.stack([ur[0], *_transform_to(n, ur), ur[4]])
+    fl, pl, vl = _flux_1d(utl, gamma)
+    fr, pr, vr = _flux_1d(utr, gamma)
+    srl, srr = np.sqrt(utl[0]), np.sqrt(utr[0])
+    roa = srl*srr
+    ha = (srl*(pr + utr[4]) + srr*(pl + utl[4]))/(srl*utr[0] + srr*utl[0])
+    inv_rar = 1/(srl + srr)
+    va = [(vl[i]*srl + vr[i]*srr)*inv_rar for i in range(3)]
+    qq = va[0]*va[0] + va[1]*va[1] + va[2]*va[2]
+    a = np.sqrt((gamma - 1)*(ha - 0.5*qq))
+    eps = 0.001
+    l1, l2, l3 = np.abs(va[0] - a), np.abs(va[0]), np.abs(va[0] + a)
+    l1 = np.where(l1 < eps, (0.5/eps)*(l1*l1 + eps**2), l1)
+    l3 = np.where(l3 < eps, (0.5/eps)*(l3*l3 + eps**2), l3)
+    dv = [vr[i] - vl[i] for i in range(3)]
+    dro = utr[0] - utl[0]
+    dp = pr - pl
+    inv_a2 = 1/(2*a*a)
+    ntf = np.empty_like(utl)
+    v1 = (dp - roa*a*dv[0])*inv_a2
+    v2 = dro - dp*2*inv_a2
+    v3 = (dp + roa*a*dv[0])*inv_a2
+    ntf[0] = 0.5*(fl[0] + fr[0]) - (l1*v1 + l2*v2 + l3*v3)
+    for i in range(3):
+        if i == 0:
+            v1 = (dp - roa*a*dv[0])*inv_a2*(va[i] - a)
+            v2 = (dro - dp*2*inv_a2)*va[i]
+            v3 = (dp + roa*a*dv[0])*inv_a2*(va[i] + a)
+        else:
+            v1 = (dp - roa*a*dv[0])*inv_a2*va[i]
+            v2 = (dro - dp*2*inv_a2)*va[i] + roa*dv[i]
+            v3 = (dp + roa*a*dv[0])*inv_a2*va[i]
+        ntf[i + 1] = 0.5*(fl[i + 1] + fr[i + 1]) - (l1*v1 + l2*v2 + l3*v3)
+    v1 = (dp - roa*a*dv[0])*inv_a2*(ha - a*va[0])
+    v2 = (roa*((va[0]*dv[0] + va[1]*dv[1] + va[2]*dv[2]) - va[0]*dv[0]) +
+          (dro - dp*2*inv_a2)*qq*0.5)
+    v3 = (dp + roa*a*dv[0])*inv_a2*(ha + a*va[0])
+    ntf[4] = 0.5*(fl[4] + fr[4]) - (l1*v1 + l2*v2 + l3*v3)
+    nf = np.empty_like(ntf)
+    nf[0], nf[4] = ntf[0], ntf[4]
+    nf[1:4] = _transform_from(n, ntf[1:4])
+    return nf
+
+
+def roem(ul, ur, n, gamma):
+    """RoeM (Kim, Kim & Rho 2003); ``rsolvers/roem.mako:4-86``."""
+    fl, pl, vl = inviscid_flux(ul, gamma)
+    fr, pr, vr = inviscid_flux(ur, gamma)
+    hl = (ul[4] + pl)/ul[0]
+    hr = (ur[4] + pr)/ur[0]
+    contral = n[0]*vl[0] + n[1]*vl[1] + n[2]*vl[2]
+    contrar = n[0]*vr[0] + n[1]*vr[1] + n[2]*vr[2]
+    drho, dp, dh = ur[0] - ul[0], pr - pl, hr - hl
+    dcontra = contrar - contral
+    dv = [vr[i] - vl[i] for i in range(3)]
+    rrr = np.sqrt(ur[0]/ul[0])
+    ratl = 1.0/(1.0 + rrr)
+    ratr = rrr*ratl
+    ra = rrr*ul[0]
+    ha = hl*ratl + hr*ratr
+    va = [vl[i]*ratl + vr[i]*ratr for i in range(3)]
+    qq = va[0]*va[0] + va[1]*va[1] + va[2]*va[2]
+    contraa = n[0]*va[0] + n[1]*va[1] + n[2]*va[2]
+    aa = np.sqrt((gamma - 1)*(ha - 0.5*qq))
+    rcp_aa = 1.0/aa
+    abs_ma = np.abs(contraa*rcp_aa) + 1e-15
+    b1 = np.maximum(0.0, np.maximum(contraa + aa, contrar + aa))
+    b2 = np.minimum(0.0, np.minimum(contraa - aa, contral - aa))
+    b1b2 = b1*b2
+    rcp_b1_b2 = 1.0/(b1 - b2)
+    b1, b2, b1b2 = b1*rcp_b1_b2, b2*rcp_b1_b2, b1b2*rcp_b1_b2
+    sdst = np.where(pl < pr, pl/pr, pr/pl)
+    h = 1.0 - sdst
+    f = np.power(abs_ma, h)
+    g = f/(1.0 + abs_ma)
+    du = [ur[i] - ul[i] for i in range(4)] + [ur[0]*hr - ul[0]*hl]
+    bdq = [None]*5
+    bdq[0] = drho - f*dp*rcp_aa*rcp_aa
+    bdq[4] = bdq[0]*ha + ra*dh
+    for i in range(3):
+        bdq[i + 1] = bdq[0]*va[i] + ra*(dv[i] - n[i]*dcontra)
+    nf = np.empty_like(ul)
+    for i in range(5):
+        nf[i] = (n[0]*(b1*fl[0, i] - b2*fr[0, i]) +
+                 n[1]*(b1*fl[1, i] - b2*fr[1, i]) +
+                 n[2]*(b1*fl[2, i] - b2*fr[2, i]) +
+                 b1b2*(du[i] - g*bdq[i]))
+    return nf
+
+
+RSOLVERS = {'rusanov': rusanov, 'hllc': hllc, 'hll': hll, 'roe': roe,
+            'roem': roem}
 
 
 def viscous_flux_add(u, g, f, c):

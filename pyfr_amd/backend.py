@@ -740,11 +740,10 @@ class PointwiseProvider:
     @staticmethod
     def _rs(tplargs):
         rs = tplargs['rsolver']
-        if rs == 'rusanov':
-            return 0
-        if rs == 'hllc':
-            return 1
-        raise NotSuitableError(f'Riemann solver {rs} not available')
+        ids = {'rusanov': 0, 'hllc': 1, 'hll': 2, 'roe': 3, 'roem': 4}
+        if rs not in ids:
+            raise NotSuitableError(f'Riemann solver {rs} not available')
+        return ids[rs]
 
     def intcflux(self, tplargs, dims, ul, ur, nl, gradul=None, gradur=None,
                  artvisc=None, **kw):

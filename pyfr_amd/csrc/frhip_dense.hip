@@ -284,6 +284,15 @@ __device__ __forceinline__ void flux_tiles(
         for (int w = 0; w < 5; ++w)
             un[w] = *reinterpret_cast<const V *>(ub + jn*ldu + w*64);
 
+        // the chunk's operator fragments, requested before the flux
+        // arithmetic so that their latency overlaps it
+        T a[3][MT];
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt)
+                a[d][mt] = afl[((long)mt*KS + 3*q + d)*64];
+
         // this variable's flux in the three directions, transformed by S
         // (flux.mako:3-26, tflux.mako:24-28)
         V b[3];
@@ -316,18 +325,12 @@ __device__ __forceinline__ void flux_tiles(
         }
 
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            const int ks = 3*q + d;
-            T a[MT];
-#pragma unroll
-            for (int mt = 0; mt < MT; ++mt)
-                a[mt] = afl[((long)mt*KS + ks)*64];
+        for (int d = 0; d < 3; ++d)
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
                 for (int t = 0; t < 4; ++t)
-                    acc[mt][t] = mfma4(a[mt], b[d][t], acc[mt][t]);
-        }
+                    acc[mt][t] = mfma4(a[d][mt], b[d][t], acc[mt][t]);
     }
 
     // common normal fluxes at the flux points
@@ -364,7 +367,12 @@ __device__ __forceinline__ void flux_tiles(
             const int row = 16*mt + (sizeof(T) == 8 ? kr + 4*r : 4*kr + r);
             if (row >= m) continue;
             const long off = row*ldc + c0 + cq;
-            if constexpr (EP == 1) {
+            if constexpr (EP == 0) {
+                const V o = {acc[mt][0][r], acc[mt][1][r], acc[mt][2][r],
+                             acc[mt][3][r]};
+                *reinterpret_cast<V *>(c + off) = o;
+            }
+            else if constexpr (EP == 1) {
                 const V rv = {rcp[row*ldr + ecol], rcp[row*ldr + ecol + 1],
                               rcp[row*ldr + ecol + 2],
                               rcp[row*ldr + ecol + 3]};

@@ -80,12 +80,220 @@ __device__ __forceinline__ void rsolve_hllc(const T ul[5], const T ur[5],
     }
 }
 
+// HLL; pyfr/solvers/euler/kernels/rsolvers/hll.mako:4-53, including its
+// left wave-speed estimate min(nv - a, nvl - ar).
 template <typename T>
-__device__ __forceinline__ void rsolve(int rs, const T ul[5], const T ur[5],
+__device__ __forceinline__ void rsolve_hll(const T ul[5], const T ur[5],
+                                           const T n[3], T nf[5], T gm1,
+                                           T gamma)
+{
+    T fl[3][5], fr[3][5], vl[3], vr[3], pl, pr;
+    inviscid_flux(ul, fl, pl, vl, gm1);
+    inviscid_flux(ur, fr, pr, vr, gm1);
+    const T nvl = n[0]*vl[0] + n[1]*vl[1] + n[2]*vl[2];
+    const T nvr = n[0]*vr[0] + n[1]*vr[1] + n[2]*vr[2];
+    const T ar = sqrt(gamma*pr/ur[0]);
+    const T srl = sqrt(ul[0]), srr = sqrt(ur[0]);
+    const T nv = (srl*nvl + srr*nvr)/(srl + srr);
+    const T H = (srl*(pr + ur[4]) + srr*(pl + ul[4]))/(srl*ur[0] + srr*ul[0]);
+    const T inv_rar = T(1)/(srl + srr);
+    T va[3];
+    for (int i = 0; i < 3; ++i) va[i] = (vl[i]*srl + vr[i]*srr)*inv_rar;
+    const T qq = va[0]*va[0] + va[1]*va[1] + va[2]*va[2];
+    const T a = sqrt(gm1*(H - T(0.5)*qq));
+    const T sl = fmin(nv - a, nvl - ar);
+    const T sr = fmax(nv + a, nvr + ar);
+    const T rcpsrsl = T(1)/(sr - sl);
+    for (int i = 0; i < 5; ++i) {
+        const T nf_fl = n[0]*fl[0][i] + n[1]*fl[1][i] + n[2]*fl[2][i];
+        const T nf_fr = n[0]*fr[0][i] + n[1]*fr[1][i] + n[2]*fr[2][i];
+        const T nf_sub = (sr*nf_fl - sl*nf_fr + sl*sr*(ur[i] - ul[i]))*rcpsrsl;
+        nf[i] = (T(0) <= sl) ? nf_fl : (T(0) >= sr) ? nf_fr : nf_sub;
+    }
+}
+
+// Rotation of the momentum into the frame whose first axis is n, and back
+// (Moler & Hughes); pyfr/solvers/baseadvec/kernels/transform.mako, 3-D.
+template <typename T>
+__device__ __forceinline__ void transform_to(const T n[3], const T u[3],
+                                             T t[3])
+{
+    if (fabs(n[0]) < T(0.99)) {
+        const T h = T(1)/(T(1) + n[0]);
+        t[0] = n[0]*u[0] + n[1]*u[1] + n[2]*u[2];
+        t[1] = (n[0] + h*n[2]*n[2])*u[1] - n[1]*u[0] - h*n[1]*n[2]*u[2];
+        t[2] = (n[0] + h*n[1]*n[1])*u[2] - n[2]*u[0] - h*n[1]*n[2]*u[1];
+    }
+    else if (fabs(n[1]) < fabs(n[2])) {
+        const T h = T(1)/(T(1) - n[1]);
+        t[0] = n[0]*u[0] + n[1]*u[1] + n[2]*u[2];
+        t[1] = (T(1) - h*n[0]*n[0])*u[0] + n[0]*u[1] - h*n[0]*n[2]*u[2];
+        t[2] = n[2]*u[1] - h*n[0]*n[2]*u[0] + (T(1) - h*n[2]*n[2])*u[2];
+    }
+    else {
+        const T h = T(1)/(T(1) - n[2]);
+        t[0] = n[0]*u[0] + n[1]*u[1] + n[2]*u[2];
+        t[1] = (T(1) - h*n[1]*n[1])*u[1] - h*n[0]*n[1]*u[0] + n[1]*u[2];
+        t[2] = (T(1) - h*n[0]*n[0])*u[0] - h*n[0]*n[1]*u[1] + n[0]*u[2];
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void transform_from(const T n[3], const T t[3],
+                                               T u[3])
+{
+    if (fabs(n[0]) < T(0.99)) {
+        const T h = T(1)/(T(1) + n[0]);
+        u[0] = n[0]*t[0] - n[1]*t[1] - n[2]*t[2];
+        u[1] = n[1]*t[0] + (n[0] + h*n[2]*n[2])*t[1] - h*n[1]*n[2]*t[2];
+        u[2] = n[2]*t[0] - h*n[1]*n[2]*t[1] + (n[0] + h*n[1]*n[1])*t[2];
+    }
+    else if (fabs(n[1]) < fabs(n[2])) {
+        const T h = T(1)/(T(1) - n[1]);
+        u[0] = n[0]*t[0] + (T(1) - h*n[0]*n[0])*t[1] - h*n[0]*n[2]*t[2];
+        u[1] = n[1]*t[0] + n[0]*t[1] + n[2]*t[2];
+        u[2] = n[2]*t[0] - h*n[0]*n[2]*t[1] + (T(1) - h*n[2]*n[2])*t[2];
+    }
+    else {
+        const T h = T(1)/(T(1) - n[2]);
+        u[0] = n[0]*t[0] - h*n[0]*n[1]*t[1] + (T(1) - h*n[0]*n[0])*t[2];
+        u[1] = n[1]*t[0] + (T(1) - h*n[1]*n[1])*t[1] - h*n[0]*n[1]*t[2];
+        u[2] = n[2]*t[0] + n[1]*t[1] + n[0]*t[2];
+    }
+}
+
+// 1-D inviscid flux along the first axis; flux.mako: inviscid_flux_1d.
+template <typename T>
+__device__ __forceinline__ void inviscid_flux_1d(const T s[5], T f[5], T &p,
+                                                 T v[3], T gm1)
+{
+    const T invrho = T(1)/s[0], E = s[4];
+    for (int i = 0; i < 3; ++i) v[i] = invrho*s[i + 1];
+    p = gm1*(E - T(0.5)*invrho*(s[1]*s[1] + s[2]*s[2] + s[3]*s[3]));
+    f[0] = s[1];
+    f[4] = (E + p)*v[0];
+    f[1] = s[1]*v[0] + p;
+    f[2] = s[1]*v[1];
+    f[3] = s[1]*v[2];
+}
+
+// Roe with Harten's entropy fix (eps = 0.001), solved in the face frame;
+// rsolvers/roe.mako:4-76 through rsolvers/rsolve1d.mako:1-21.
+template <typename T>
+__device__ __forceinline__ void rsolve_roe(const T ul[5], const T ur[5],
+                                           const T n[3], T nf[5], T gm1)
+{
+    T tl[5], tr[5];
+    tl[0] = ul[0]; tr[0] = ur[0]; tl[4] = ul[4]; tr[4] = ur[4];
+    transform_to(n, ul + 1, tl + 1);
+    transform_to(n, ur + 1, tr + 1);
+
+    T fl[5], fr[5], vl[3], vr[3], pl, pr;
+    inviscid_flux_1d(tl, fl, pl, vl, gm1);
+    inviscid_flux_1d(tr, fr, pr, vr, gm1);
+    const T srl = sqrt(tl[0]), srr = sqrt(tr[0]);
+    const T roa = srl*srr;
+    const T ha = (srl*(pr + tr[4]) + srr*(pl + tl[4]))/(srl*tr[0] + srr*tl[0]);
+    const T inv_rar = T(1)/(srl + srr);
+    T va[3], dv[3];
+    for (int i = 0; i < 3; ++i) va[i] = (vl[i]*srl + vr[i]*srr)*inv_rar;
+    const T qq = va[0]*va[0] + va[1]*va[1] + va[2]*va[2];
+    const T a = sqrt(gm1*(ha - T(0.5)*qq));
+    T l1 = fabs(va[0] - a), l2 = fabs(va[0]), l3 = fabs(va[0] + a);
+    // the reference's literals: 0.5/eps and eps**2 for eps = 0.001
+    const T eps = T(0.001), hre = T(500.0), eps2 = T(1.0000000000000002e-06);
+    l1 = (l1 < eps) ? hre*(l1*l1 + eps2) : l1;
+    l3 = (l3 < eps) ? hre*(l3*l3 + eps2) : l3;
+    for (int i = 0; i < 3; ++i) dv[i] = vr[i] - vl[i];
+    const T dro = tr[0] - tl[0], dp = pr - pl;
+    const T inv_a2 = T(1)/(T(2)*a*a);
+
+    T nt[5];
+    T v1 = (dp - roa*a*dv[0])*inv_a2;
+    T v2 = dro - dp*T(2)*inv_a2;
+    T v3 = (dp + roa*a*dv[0])*inv_a2;
+    nt[0] = T(0.5)*(fl[0] + fr[0]) - (l1*v1 + l2*v2 + l3*v3);
+    for (int i = 0; i < 3; ++i) {
+        if (i == 0) {
+            v1 = (dp - roa*a*dv[0])*inv_a2*(va[i] - a);
+            v2 = (dro - dp*T(2)*inv_a2)*va[i];
+            v3 = (dp + roa*a*dv[0])*inv_a2*(va[i] + a);
+        }
+        else {
+            v1 = (dp - roa*a*dv[0])*inv_a2*va[i];
+            v2 = (dro - dp*T(2)*inv_a2)*va[i] + roa*dv[i];
+            v3 = (dp + roa*a*dv[0])*inv_a2*va[i];
+        }
+        nt[i + 1] = T(0.5)*(fl[i + 1] + fr[i + 1]) - (l1*v1 + l2*v2 + l3*v3);
+    }
+    v1 = (dp - roa*a*dv[0])*inv_a2*(ha - a*va[0]);
+    v2 = roa*((va[0]*dv[0] + va[1]*dv[1] + va[2]*dv[2]) - va[0]*dv[0]) +
+         (dro - dp*T(2)*inv_a2)*qq*T(0.5);
+    v3 = (dp + roa*a*dv[0])*inv_a2*(ha + a*va[0]);
+    nt[4] = T(0.5)*(fl[4] + fr[4]) - (l1*v1 + l2*v2 + l3*v3);
+
+    nf[0] = nt[0];
+    nf[4] = nt[4];
+    transform_from(n, nt + 1, nf + 1);
+}
+
+// RoeM (Kim, Kim and Rho, JCP 185(2)); rsolvers/roem.mako:4-86.
+template <typename T>
+__device__ __forceinline__ void rsolve_roem(const T ul[5], const T ur[5],
+                                            const T n[3], T nf[5], T gm1)
+{
+    T fl[3][5], fr[3][5], vl[3], vr[3], pl, pr;
+    inviscid_flux(ul, fl, pl, vl, gm1);
+    inviscid_flux(ur, fr, pr, vr, gm1);
+    const T hl = (ul[4] + pl)/ul[0], hr = (ur[4] + pr)/ur[0];
+    const T contral = n[0]*vl[0] + n[1]*vl[1] + n[2]*vl[2];
+    const T contrar = n[0]*vr[0] + n[1]*vr[1] + n[2]*vr[2];
+    const T drho = ur[0] - ul[0], dp = pr - pl, dh = hr - hl;
+    const T dcontra = contrar - contral;
+    T dv[3], va[3];
+    for (int i = 0; i < 3; ++i) dv[i] = vr[i] - vl[i];
+    const T rrr = sqrt(ur[0]/ul[0]);
+    const T ratl = T(1.0)/(T(1.0) + rrr), ratr = rrr*ratl;
+    const T ra = rrr*ul[0], ha = hl*ratl + hr*ratr;
+    for (int i = 0; i < 3; ++i) va[i] = vl[i]*ratl + vr[i]*ratr;
+    const T qq = va[0]*va[0] + va[1]*va[1] + va[2]*va[2];
+    const T contraa = n[0]*va[0] + n[1]*va[1] + n[2]*va[2];
+    const T aa = sqrt(gm1*(ha - T(0.5)*qq));
+    const T rcp_aa = T(1.0)/aa;
+    const T abs_ma = fabs(contraa*rcp_aa) + T(1e-15);
+    T b1 = fmax(T(0.0), fmax(contraa + aa, contrar + aa));
+    T b2 = fmin(T(0.0), fmin(contraa - aa, contral - aa));
+    T b1b2 = b1*b2;
+    const T rcp_b1_b2 = T(1.0)/(b1 - b2);
+    b1 = b1*rcp_b1_b2;
+    b2 = b2*rcp_b1_b2;
+    b1b2 = b1b2*rcp_b1_b2;
+    const T sdst = (pl < pr) ? pl/pr : pr/pl;
+    const T h = T(1.0) - sdst;
+    const T f = pow(abs_ma, h);
+    const T g = f/(T(1.0) + abs_ma);
+    T du[5], bdq[5];
+    for (int i = 0; i < 4; ++i) du[i] = ur[i] - ul[i];
+    du[4] = ur[0]*hr - ul[0]*hl;
+    bdq[0] = drho - f*dp*rcp_aa*rcp_aa;
+    bdq[4] = bdq[0]*ha + ra*dh;
+    for (int i = 0; i < 3; ++i)
+        bdq[i + 1] = bdq[0]*va[i] + ra*(dv[i] - n[i]*dcontra);
+    for (int i = 0; i < 5; ++i)
+        nf[i] = n[0]*(b1*fl[0][i] - b2*fr[0][i]) +
+                n[1]*(b1*fl[1][i] - b2*fr[1][i]) +
+                n[2]*(b1*fl[2][i] - b2*fr[2][i]) + b1b2*(du[i] - g*bdq[i]);
+}
+
+template <typename T, int RS>
+__device__ __forceinline__ void rsolve(const T ul[5], const T ur[5],
                                        const T n[3], T nf[5], T gm1, T gamma,
                                        T qgamma)
 {
-    if (rs == FRHIP_RS_HLLC) rsolve_hllc(ul, ur, n, nf, gm1, gamma);
+    if constexpr (RS == FRHIP_RS_HLLC) rsolve_hllc(ul, ur, n, nf, gm1, gamma);
+    else if constexpr (RS == FRHIP_RS_HLL) rsolve_hll(ul, ur, n, nf, gm1, gamma);
+    else if constexpr (RS == FRHIP_RS_ROE) rsolve_roe(ul, ur, n, nf, gm1);
+    else if constexpr (RS == FRHIP_RS_ROEM) rsolve_roem(ul, ur, n, nf, gm1);
     else rsolve_rusanov(ul, ur, n, nf, gm1, qgamma);
 }
 
@@ -333,9 +541,9 @@ __global__ __launch_bounds__(256) void k_negdivconf(
 // ---------------------------------------------------------------------------
 
 // intcflux (Euler); pyfr/solvers/euler/kernels/intcflux.mako:6-22.
-template <typename T>
+template <typename T, int RS>
 __global__ __launch_bounds__(256) void k_intcflux_euler(
-    int n, int rs, T *__restrict__ base, const int32_t *__restrict__ lmap,
+    int n, T *__restrict__ base, const int32_t *__restrict__ lmap,
     const int32_t *__restrict__ rmap, const T *__restrict__ nl, long ldn,
     T gamma, T gm1, T qgamma)
 {
@@ -351,7 +559,7 @@ __global__ __launch_bounds__(256) void k_intcflux_euler(
     const T mag = sqrt((n0)*(n0) + (n1)*(n1) + (n2)*(n2));
     const T nn[3] = {(T(1)/mag)*n0, (T(1)/mag)*n1, (T(1)/mag)*n2};
     T fn[5];
-    rsolve(rs, ul, ur, nn, fn, gm1, gamma, qgamma);
+    rsolve<T, RS>(ul, ur, nn, fn, gm1, gamma, qgamma);
     for (int v = 0; v < 5; ++v) {
         base[li + v*SOA] = mag*fn[v];
         base[ri + v*SOA] = -mag*fn[v];
@@ -383,9 +591,9 @@ __global__ __launch_bounds__(256) void k_intconu(
 }
 
 // intcflux (NS); pyfr/solvers/navstokes/kernels/intcflux.mako:10-54.
-template <typename T>
+template <typename T, int RS>
 __global__ __launch_bounds__(256) void k_intcflux_ns(
-    int n, int rs, T *__restrict__ base, const int32_t *__restrict__ lmap,
+    int n, T *__restrict__ base, const int32_t *__restrict__ lmap,
     const int32_t *__restrict__ rmap, const T *__restrict__ gb,
     const int32_t *__restrict__ glmap, const int32_t *__restrict__ grmap,
     long gstride, const T *__restrict__ nl, long ldn, NSConsts<T> c0,
@@ -405,7 +613,7 @@ __global__ __launch_bounds__(256) void k_intcflux_ns(
     const T nn[3] = {(T(1)/mag)*n0, (T(1)/mag)*n1, (T(1)/mag)*n2};
 
     T fi[5];
-    rsolve(rs, ul, ur, nn, fi, c.gm1, c.gamma, c.qgamma);
+    rsolve<T, RS>(ul, ur, nn, fi, c.gm1, c.gamma, c.qgamma);
 
     T fvl[3][5] = {}, fvr[3][5] = {};
     if (c.beta != T(-0.5)) {
@@ -766,26 +974,47 @@ int frhip_negdivconf(int dtype, int npts, int neles, void *t, long ldt,
     FR_API_END
 }
 
+#define FR_RS_OK(rs)                                                        \
+    FR_CHECK((rs) >= FRHIP_RS_RUSANOV && (rs) <= FRHIP_RS_ROEM,             \
+             "invalid Riemann solver")
+
+// Expand M(RS) for the run-time solver id (one instantiation per solver)
+#define FR_RS_DISPATCH(rs, M)                                               \
+    switch (rs) {                                                           \
+        case FRHIP_RS_RUSANOV: M(FRHIP_RS_RUSANOV); break;                  \
+        case FRHIP_RS_HLLC: M(FRHIP_RS_HLLC); break;                        \
+        case FRHIP_RS_HLL: M(FRHIP_RS_HLL); break;                          \
+        case FRHIP_RS_ROE: M(FRHIP_RS_ROE); break;                          \
+        default: M(FRHIP_RS_ROEM); break;                                   \
+    }
+
 int frhip_intcflux_euler(int dtype, int rsolver, int n, void *base,
                          const int32_t *lmap, const int32_t *rmap,
                          const void *nl, long ldn, double gamma, void *stream)
 {
     FR_API_BEGIN
     FR_DTYPE_OK(dtype);
-    FR_CHECK(rsolver == FRHIP_RS_RUSANOV || rsolver == FRHIP_RS_HLLC,
-             "invalid Riemann solver");
+    FR_RS_OK(rsolver);
     if (n == 0) return 0;
     dim3 grid(cdiv(n, 256)), block(256);
-    if (dtype == FRHIP_F64)
-        hipLaunchKernelGGL(k_intcflux_euler<double>, grid, block, 0,
-                           as_stream(stream), n, rsolver, (double *)base, lmap,
-                           rmap, (const double *)nl, ldn, gamma, gamma - 1,
-                           0.25*gamma);
-    else
-        hipLaunchKernelGGL(k_intcflux_euler<float>, grid, block, 0,
-                           as_stream(stream), n, rsolver, (float *)base, lmap,
-                           rmap, (const float *)nl, ldn, (float)gamma,
-                           (float)(gamma - 1), (float)(0.25*gamma));
+#define FR_L64(RS)                                                          \
+    hipLaunchKernelGGL((k_intcflux_euler<double, RS>), grid, block, 0,      \
+                       as_stream(stream), n, (double *)base, lmap, rmap,    \
+                       (const double *)nl, ldn, gamma, gamma - 1,           \
+                       0.25*gamma)
+#define FR_L32(RS)                                                          \
+    hipLaunchKernelGGL((k_intcflux_euler<float, RS>), grid, block, 0,       \
+                       as_stream(stream), n, (float *)base, lmap, rmap,     \
+                       (const float *)nl, ldn, (float)gamma,                \
+                       (float)(gamma - 1), (float)(0.25*gamma))
+    if (dtype == FRHIP_F64) {
+        FR_RS_DISPATCH(rsolver, FR_L64)
+    }
+    else {
+        FR_RS_DISPATCH(rsolver, FR_L32)
+    }
+#undef FR_L64
+#undef FR_L32
     FR_LAUNCHED();
     return 0;
     FR_API_END
@@ -823,21 +1052,30 @@ int frhip_intcflux_ns(int dtype, int rsolver, int n, void *ubase,
 {
     FR_API_BEGIN
     FR_DTYPE_OK(dtype);
+    FR_RS_OK(rsolver);
     FR_CHECK(consts, "null constants");
     if (n == 0) return 0;
     dim3 grid(cdiv(n, 256)), block(256);
-    if (dtype == FRHIP_F64)
-        hipLaunchKernelGGL(k_intcflux_ns<double>, grid, block, 0,
-                           as_stream(stream), n, rsolver, (double *)ubase,
-                           lmap, rmap, (const double *)gbase, glmap, grmap,
-                           gstride, (const double *)nl, ldn,
-                           ns_consts<double>(consts), (const double *)mu_p);
-    else
-        hipLaunchKernelGGL(k_intcflux_ns<float>, grid, block, 0,
-                           as_stream(stream), n, rsolver, (float *)ubase, lmap,
-                           rmap, (const float *)gbase, glmap, grmap, gstride,
-                           (const float *)nl, ldn, ns_consts<float>(consts),
-                           (const float *)mu_p);
+#define FR_L64(RS)                                                          \
+    hipLaunchKernelGGL((k_intcflux_ns<double, RS>), grid, block, 0,         \
+                       as_stream(stream), n, (double *)ubase, lmap, rmap,   \
+                       (const double *)gbase, glmap, grmap, gstride,        \
+                       (const double *)nl, ldn, ns_consts<double>(consts),  \
+                       (const double *)mu_p)
+#define FR_L32(RS)                                                          \
+    hipLaunchKernelGGL((k_intcflux_ns<float, RS>), grid, block, 0,          \
+                       as_stream(stream), n, (float *)ubase, lmap, rmap,    \
+                       (const float *)gbase, glmap, grmap, gstride,         \
+                       (const float *)nl, ldn, ns_consts<float>(consts),    \
+                       (const float *)mu_p)
+    if (dtype == FRHIP_F64) {
+        FR_RS_DISPATCH(rsolver, FR_L64)
+    }
+    else {
+        FR_RS_DISPATCH(rsolver, FR_L32)
+    }
+#undef FR_L64
+#undef FR_L32
     FR_LAUNCHED();
     return 0;
     FR_API_END

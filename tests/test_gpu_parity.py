@@ -86,6 +86,10 @@ def _systems(be, system, n, order, rsolver='rusanov', scheme='rk4'):
     ('navier-stokes', 4, 3, 'rusanov'),
     ('navier-stokes', 4, 2, 'hllc'),
     ('navier-stokes', 3, 1, 'rusanov'),
+    ('euler', 4, 2, 'hll'),
+    ('euler', 4, 2, 'roe'),
+    ('euler', 4, 3, 'roem'),
+    ('navier-stokes', 3, 2, 'roe'),
 ])
 def test_rhs_parity(backend_factory, system, n, order, rsolver, fused):
     be = backend_factory(fused=fused)

@@ -258,3 +258,58 @@ def test_c_restatement_matches_numpy(system, n, order):
     ref = orc.rhs(u)
     got = CRhs(orc).rhs(u)
     assert np.linalg.norm(got - ref)/np.linalg.norm(ref) < 2e-12
+
+
+def _random_states(rng, n, mach=0.5):
+    rho = rng.uniform(0.5, 2.0, n)
+    p = rng.uniform(0.5, 2.0, n)
+    v = rng.normal(0, mach, (3, n))
+    return np.stack([rho, *(rho*v), p/0.4 + 0.5*rho*(v**2).sum(0)])
+
+
+@pytest.mark.parametrize('name', ['rusanov', 'hllc', 'hll', 'roe', 'roem'])
+def test_riemann_solver_properties(name):
+    """Consistency F(u, u, n) = f(u).n for every solver; conservation
+    F(ul, ur, n) = -F(ur, ul, -n) where the scheme has it (the reference's
+    HLL estimates the left wave speed with the right sound speed, so it is
+    symmetric only for states of equal sound speed); exact upwinding for
+    supersonic flow along n."""
+    from oracle.physics import RSOLVERS, inviscid_flux
+    rs = RSOLVERS[name]
+    rng = np.random.default_rng(11)
+    N = 300
+    ul, ur = _random_states(rng, N), _random_states(rng, N)
+    n = rng.normal(size=(3, N))
+    n /= np.linalg.norm(n, axis=0)
+    f, _, _ = inviscid_flux(ul, 1.4)
+    fn = (n[:, None, :]*f).sum(0)
+    assert np.abs(rs(ul, ul, n, 1.4) - fn).max() < 1e-12
+    if name == 'hll':
+        # equal density and pressure on both sides: al == ar
+        ur2 = ul.copy()
+        v2 = rng.normal(0, 0.5, (3, N))
+        ur2[1:4] = ul[0]*v2
+        pl = 0.4*(ul[4] - 0.5*(ul[1:4]**2).sum(0)/ul[0])
+        ur2[4] = pl/0.4 + 0.5*ul[0]*(v2**2).sum(0)
+        assert np.abs(rs(ul, ur2, n, 1.4) + rs(ur2, ul, -n, 1.4)).max() < 1e-12
+    else:
+        assert np.abs(rs(ul, ur, n, 1.4) + rs(ur, ul, -n, 1.4)).max() < 1e-12
+    # supersonic along n: both states moving at Mach > 3 along n
+    sl, sr = _random_states(rng, N, 0.05), _random_states(rng, N, 0.05)
+    for s in (sl, sr):
+        s[4] -= 0.5*(s[1:4]**2).sum(0)/s[0]
+        s[1:4] += 4.0*s[0]*n
+        s[4] += 0.5*(s[1:4]**2).sum(0)/s[0]
+    f, _, _ = inviscid_flux(sl, 1.4)
+    fn = (n[:, None, :]*f).sum(0)
+    fr, _, _ = inviscid_flux(sr, 1.4)
+    fnr = (n[:, None, :]*fr).sum(0)
+    got = rs(sl, sr, n, 1.4)
+    if name == 'roe':
+        # the reference's Roe flux is 1/2 (fl + fr) - sum |l| alpha r, with
+        # no 1/2 on the dissipation; with every eigenvalue positive and the
+        # Roe property sum l alpha r = fr - fl it reduces to (3 fl - fr)/2
+        assert np.abs(got - (1.5*fn - 0.5*fnr)).max() < \
+            1e-10*np.abs(fn).max()
+    elif name != 'rusanov':
+        assert np.abs(got - fn).max() < 1e-10*np.abs(fn).max()
