@@ -312,8 +312,10 @@ def accuracy_report(c, checkpoints=(100, 500)):
     from pyfr_amd.mesh import BoxMesh
     from pyfr_amd.solvers import get_integrator
     res = {}
-    for prec in ('double', 'mixed'):
-        cfg = make_cfg(c, prec)
+    for prec in ('double', 'mixed', 'mixed-bf16'):
+        cfg = make_cfg(c, prec.split('-')[0])
+        if prec == 'mixed-bf16':
+            cfg.set('backend-hip', 'operator-inputs', 'bf16')
         be = AMDBackend(cfg)
         integ = get_integrator(be, BoxMesh(c['n']), cfg)
         res[prec] = {'sums0': integ.system.conservation_sums(0).tolist()}
@@ -331,19 +333,20 @@ def accuracy_report(c, checkpoints=(100, 500)):
     names = ['rho', 'rhou', 'rhov', 'rhow', 'E']
     out = {'workload': c['workload'].replace('-fp64', '-mixed-vs-fp64'),
            'checkpoints': {}}
-    for cp in checkpoints:
-        a, b = res['mixed'][cp], res['double'][cp]
-        d = {}
-        for v, nm in enumerate(names):
-            da, db = a[:, v], b[:, v]
-            d[nm] = {'rel_l2': float(np.linalg.norm(da - db) /
-                                     np.linalg.norm(db)),
-                     'max_abs': float(np.abs(da - db).max()),
-                     'ref_max_abs': float(np.abs(db).max())}
-        d['state_rel_l2'] = float(np.linalg.norm(a - b)/np.linalg.norm(b))
-        d['sums_fp64'] = res['double'][f'sums{cp}']
-        d['sums_mixed'] = res['mixed'][f'sums{cp}']
-        out['checkpoints'][cp] = d
+    for form in ('mixed', 'mixed-bf16'):
+        for cp in checkpoints:
+            a, b = res[form][cp], res['double'][cp]
+            d = {}
+            for v, nm in enumerate(names):
+                da, db = a[:, v], b[:, v]
+                d[nm] = {'rel_l2': float(np.linalg.norm(da - db) /
+                                         np.linalg.norm(db)),
+                         'max_abs': float(np.abs(da - db).max()),
+                         'ref_max_abs': float(np.abs(db).max())}
+            d['state_rel_l2'] = float(np.linalg.norm(a - b)/np.linalg.norm(b))
+            d['sums_fp64'] = res['double'][f'sums{cp}']
+            d['sums_' + form] = res[form][f'sums{cp}']
+            out['checkpoints'][f'{form}@{cp}'] = d
     out['sums_initial'] = res['double']['sums0']
     print(json.dumps(out))
 

@@ -28,6 +28,9 @@ extern "C" {
 #endif
 
 enum { FRHIP_F64 = 0, FRHIP_F32 = 1 };
+/* Operator plans only: FP32 data with both matrix-core operands rounded to
+ * bfloat16 and FP32 accumulation (the precision form's BF16 experiment). */
+enum { FRHIP_F32_BF16OPS = 2 };
 /* Riemann solvers (pyfr/solvers/euler/kernels/rsolvers/*.mako). */
 enum { FRHIP_RS_RUSANOV = 0, FRHIP_RS_HLLC = 1, FRHIP_RS_HLL = 2,
        FRHIP_RS_ROE = 3, FRHIP_RS_ROEM = 4 };
@@ -80,7 +83,8 @@ int frhip_graph_destroy(void *graph_exec);
  * operators), that kernel is used with A's nonzeros uploaded; otherwise the
  * plan is dense (tet/prism operators): matrix-core (MFMA) kernels with the
  * operator in fragment order, or a scalar kernel.  *kind receives 1
- * (structured) or 2 (dense). */
+ * (structured) or 2 (dense).  dtype FRHIP_F32_BF16OPS always makes a BF16
+ * matrix-core plan (*kind = 3) over FP32 data. */
 int frhip_mul_plan_create(void **plan, int m, int k, const double *a_host,
                           int dtype, int *kind);
 int frhip_mul_plan_destroy(void *plan);
@@ -141,7 +145,8 @@ int frhip_mul2_plan_set_variant(void *plan, int variant);
  * for frhip_mul2 (0 store, 1 negdivconf, 2 / 3 RK45 stage); ncol = 5 *
  * padded neles.  The tuner times 1, 2 and 4 wavefronts per block, with
  * and without the LDS-staged operator (variant codes as above, without the
- * scalar kernel). */
+ * scalar kernel), and variant 32: one block of five wavefronts per
+ * 64-element block, the flux evaluated once per point into LDS. */
 int frhip_tdivf_plan_create(void **plan, int m, int nupts, int nfpts,
                             const double *a0, const double *a1, int dtype);
 int frhip_tdivf_plan_destroy(void *plan);

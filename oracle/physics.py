@@ -170,8 +170,9 @@ def _flux_1d(s, gamma):
 def roe(ul, ur, n, gamma):
     """Roe with a Harten entropy fix (eps = 0.001) in the face frame;
     ``rsolvers/roe.mako:4-76`` through ``rsolve1d.mako:1-21``."""
-    utl = np.stack([ul[0], *_transform_to(n, ul), ul[4]])
-    utr = np.stack([ur[0], *_transform_to(n, ur), ur[4]])
+    with np.errstate(divide='ignore', invalid='ignore'):
+        utl = np.stack([ul[0], *_transform_to(n, ul), ul[4]])
+        utr = np.stack([ur[0], *_transform_to(n, ur), ur[4]])
     fl, pl, vl = _flux_1d(utl, gamma)
     fr, pr, vr = _flux_1d(utr, gamma)
     srl, srr = np.sqrt(utl[0]), np.sqrt(utr[0])
@@ -211,7 +212,8 @@ def roe(ul, ur, n, gamma):
     ntf[4] = 0.5*(fl[4] + fr[4]) - (l1*v1 + l2*v2 + l3*v3)
     nf = np.empty_like(ntf)
     nf[0], nf[4] = ntf[0], ntf[4]
-    nf[1:4] = _transform_from(n, ntf[1:4])
+    with np.errstate(divide='ignore', invalid='ignore'):
+        nf[1:4] = _transform_from(n, ntf[1:4])
     return nf
 
 

@@ -482,7 +482,7 @@ class MulProvider:
             plan, ncol, bp, ldb, cp, ldc, alpha, beta, stream),
             rtargs={'alpha': alpha, 'beta': beta}, mats=(a, b, out),
             name='mul')
-        k.kind = 'structured' if kind == 1 else 'dense'
+        k.kind = {1: 'structured', 2: 'dense', 3: 'bf16'}[kind]
         return k
 
     # -- fused two-operator applications (no reference counterpart: they fuse
@@ -495,10 +495,12 @@ class MulProvider:
             if h0.shape[0] != h1.shape[0]:
                 raise ValueError('Operators must have the same row count')
             plan, kind = ct.c_void_p(), ct.c_int()
+            code = _dtcode(dtype)
+            if self.backend.ops_bf16 and code == 1:
+                code = 2                       # FRHIP_F32_BF16OPS
             self.backend.lib.mul2_plan_create(
                 ct.byref(plan), h0.shape[0], h0.shape[1], h1.shape[1],
-                h0.ctypes.data, h1.ctypes.data, _dtcode(dtype),
-                ct.byref(kind))
+                h0.ctypes.data, h1.ctypes.data, code, ct.byref(kind))
             self._plans2[key] = (plan.value, kind.value, a0, a1)
         return self._plans2[key][:2]
 
@@ -531,7 +533,7 @@ class MulProvider:
                                         (rcpdjac,))
         k = Kernel(fn, rtargs={'dta': 0.0, 'dtb': 0.0, 'last': 0},
                    mats=mats, name=f'mul2_ep{ep}')
-        k.kind = 'structured' if kind == 1 else 'dense'
+        k.kind = {1: 'structured', 2: 'dense', 3: 'bf16'}[kind]
         return k
 
     def mul2(self, a0, a1, b0, b1, out):
@@ -835,6 +837,15 @@ class AMDBackend:
         self.use_graphs = cfg.getbool('backend-hip', 'graphs', True)
         # fuse element-local operator chains where a fused kernel exists
         self.fused = cfg.getbool('backend-hip', 'fused', True)
+        # 'bf16': the fused gradient/divergence operator applications run
+        # with both matrix-core operands rounded to bfloat16 (FP32 data and
+        # accumulation); an accuracy experiment of the precision form
+        ops = cfg.get('backend-hip', 'operator-inputs', 'native')
+        if ops not in {'native', 'bf16'}:
+            raise ValueError('operator-inputs must be native or bf16')
+        if ops == 'bf16' and self.fpdtype != np.float32:
+            raise ValueError('bf16 operator inputs need FP32 arithmetic')
+        self.ops_bf16 = ops == 'bf16'
         # straight-sided Euler elements: fold tdisf into the divergence
         self.fuse_flux = cfg.getbool('backend-hip', 'fuse-flux', True)
         # time the variants of dense operator plans at kernel construction

@@ -14,6 +14,20 @@ struct DensePlan {
                         // block; 16 + (1, 2, 4): operator staged in LDS
 };
 
+// BF16-operand plans (FP32 data and accumulation): k steps of 16
+struct Bf16Plan {
+    int m, k0, k1, mt, kb0, kb1;
+    void *afrag;        // [A0 | A1] as bf16 MFMA A fragments (4 per lane)
+};
+
+int bf16_plan_create(Bf16Plan **out, int m, int k0, int k1, const double *a0,
+                     const double *a1);
+void bf16_plan_destroy(Bf16Plan *p);
+int bf16_launch(const Bf16Plan *p, int ep, long ncol, const void *b0,
+                long ld0, const void *b1, long ld1, void *c, long ldc,
+                const void *rcp, long ldr, int nvar, void *xa, double dta,
+                double dtb, double alpha, double beta, hipStream_t s);
+
 int dense_plan_create(DensePlan **out, int m, int k0, int k1,
                       const double *a0, const double *a1, int dtype);
 void dense_plan_destroy(DensePlan *p);

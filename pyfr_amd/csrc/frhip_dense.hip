@@ -403,6 +403,146 @@ __global__ __launch_bounds__(256) void k_tdivf_flux(
                           b1, ld1, c, ldc, rcp, ldr, xa, dta, dtb, gm1);
 }
 
+// Block-cooperative form: one block of five wavefronts per 64-element
+// block, wavefront v owning variable v.  Per chunk of 4 solution points,
+// 256 threads each evaluate the full transformed flux (3 directions x 5
+// variables) at one (element, point) and write it to LDS; after one barrier
+// every wavefront reads its variable's B operands from there.  u is read
+// from HBM once and the flux evaluated once per point (the per-wavefront
+// form above reads u and evaluates the flux five times).  Double-buffered:
+// the barrier of chunk q also orders chunk q - 1's reads before chunk
+// q + 1's writes to the same buffer.
+template <typename T, int MT, int EP>
+__global__ __launch_bounds__(320) void k_tdivf_flux_blk(
+    int m, int nupts, int nq, int ks1, int k1, const T *__restrict__ af,
+    long ncol, const T *__restrict__ u, long ldu, const T *__restrict__ sm,
+    long lds, const T *__restrict__ b1, long ld1, T *__restrict__ c,
+    long ldc, const T *__restrict__ rcp, long ldr, void *__restrict__ xa,
+    double dta, double dtb, T gm1)
+{
+    using V = typename V4<T>::type;
+    extern __shared__ __attribute__((aligned(16))) char fr_smem[];
+    T *fbuf = reinterpret_cast<T *>(fr_smem);       // [2][4][3][5][64]
+    constexpr int FB = 4*3*5*64;
+
+    const int tid = threadIdx.x, lane = tid & 63, v = tid >> 6;
+    const int kr = lane >> 4, cq = (lane & 15)*4;
+    const long eb = blockIdx.x;
+    const long c0 = (eb*5 + v)*64;
+    const int KS = 3*nq + ks1;
+
+    // flux-evaluation role: element e, chunk point jj (threads 0..255)
+    const int fe = tid & 63, fj = tid >> 6;
+    const long egl = eb*64 + fe;
+    T S[3][3];
+    if (fj < 4)
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) S[i][k] = sm[(3*i + k)*lds + egl];
+    const T *ub = u + eb*320 + fe;
+
+    V acc[MT][4];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+            acc[mt][t] = V{0, 0, 0, 0};
+    const T *afl = af + lane;
+
+    for (int q = 0; q < nq; ++q) {
+        T *fb = fbuf + (q & 1)*FB;
+        if (fj < 4) {
+            const int j = min(4*q + fj, nupts - 1);
+            T uu[5];
+#pragma unroll
+            for (int w = 0; w < 5; ++w) uu[w] = ub[j*ldu + w*64];
+            T f[3][5], p, vel[3];
+            inviscid_flux(uu, f, p, vel, gm1);
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+#pragma unroll
+                for (int w = 0; w < 5; ++w)
+                    fb[((fj*3 + d)*5 + w)*64 + fe] =
+                        S[d][0]*f[0][w] + S[d][1]*f[1][w] + S[d][2]*f[2][w];
+        }
+        T a[3][MT];
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt)
+                a[d][mt] = afl[((long)mt*KS + 3*q + d)*64];
+        __syncthreads();
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const V bd = *reinterpret_cast<const V *>(
+                fb + ((kr*3 + d)*5 + v)*64 + cq);
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                    acc[mt][t] = mfma4(a[d][mt], bd[t], acc[mt][t]);
+        }
+    }
+
+    // common normal fluxes at the flux points (this wavefront's variable)
+    auto brow = [&](int s1) -> const V * {
+        const int r = min(4*s1 + kr, k1 - 1);
+        return reinterpret_cast<const V *>(b1 + r*ld1 + c0 + cq);
+    };
+    V bq[FR_PD];
+#pragma unroll
+    for (int i = 0; i < FR_PD; ++i) bq[i] = *brow(min(i, ks1 - 1));
+    for (int kb = 0; kb < ks1; kb += FR_PD) {
+#pragma unroll
+        for (int i = 0; i < FR_PD; ++i) {
+            const int s1 = kb + i;
+            if (s1 >= ks1) break;
+            const int ks = 3*nq + s1;
+            T a[MT];
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt)
+                a[mt] = afl[((long)mt*KS + ks)*64];
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                    acc[mt][t] = mfma4(a[mt], bq[i][t], acc[mt][t]);
+            bq[i] = *brow(min(s1 + FR_PD, ks1 - 1));
+        }
+    }
+
+    const long ecol = eb*64 + cq;
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = 16*mt + (sizeof(T) == 8 ? kr + 4*r : 4*kr + r);
+            if (row >= m) continue;
+            const long off = row*ldc + c0 + cq;
+            if constexpr (EP == 0) {
+                const V o = {acc[mt][0][r], acc[mt][1][r], acc[mt][2][r],
+                             acc[mt][3][r]};
+                *reinterpret_cast<V *>(c + off) = o;
+            }
+            else if constexpr (EP == 1) {
+                const V rv = {rcp[row*ldr + ecol], rcp[row*ldr + ecol + 1],
+                              rcp[row*ldr + ecol + 2],
+                              rcp[row*ldr + ecol + 3]};
+                const V o = {-rv[0]*acc[mt][0][r], -rv[1]*acc[mt][1][r],
+                             -rv[2]*acc[mt][2][r], -rv[3]*acc[mt][3][r]};
+                *reinterpret_cast<V *>(c + off) = o;
+            }
+            else {
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                    epi1<T, EP>(acc[mt][t][r], c, off + t, rcp,
+                                row*ldr + ecol + t, xa, dta, dtb, T(1), T(0));
+            }
+        }
+    (void)ncol;
+}
+
 // ---- scalar kernel --------------------------------------------------------
 template <typename T, int EP>
 __global__ __launch_bounds__(256) void k_dense_fma(
@@ -433,6 +573,9 @@ __global__ __launch_bounds__(256) void k_dense_fma(
 // per block and the operator read through the L2; 16 + (1, 2, 4) the same
 // with the operator staged in LDS once per block.
 constexpr int FR_LDSV = 16;
+// flux-fused kernels only: the block-cooperative form (five wavefronts per
+// element block, flux evaluated once into LDS)
+constexpr int FR_BLKV = 32;
 constexpr size_t FR_LDS_MAX = 65536;
 
 static bool variant_valid(int v, size_t ldsbytes, bool scalar_ok)
@@ -667,6 +810,190 @@ int dense_tune(DensePlan *p, long ncol, const void *b0, long ld0,
     return 0;
 }
 
+// ---- BF16 operands on the matrix cores (precision-form experiment) ------
+//
+// C = [A0 | A1] @ [B0; B1] with FP32 data in HBM, both MFMA operands rounded
+// to bfloat16 (round to nearest even) and FP32 accumulation:
+// v_mfma_f32_16x16x16bf16_1k, k steps of 16.  Lane group kr supplies rows
+// 16s + 4kr .. 16s + 4kr + 3 of B for its four interleaved column tiles, so
+// it loads four B rows of 4 columns and transposes them in registers.
+
+typedef short s4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ short to_bf16(float f)
+{
+    unsigned x = __float_as_uint(f);
+    x += 0x7fffu + ((x >> 16) & 1u);
+    return (short)(x >> 16);
+}
+
+static unsigned short host_bf16(float f)
+{
+    unsigned x;
+    std::memcpy(&x, &f, 4);
+    x += 0x7fffu + ((x >> 16) & 1u);
+    return (unsigned short)(x >> 16);
+}
+
+template <int MT, int EP>
+__global__ __launch_bounds__(256) void k_dense_bf16(
+    int m, int kb0, int kb1, int k0, int k1, const s4v *__restrict__ af,
+    long ncol, const float *__restrict__ b0, long ld0,
+    const float *__restrict__ b1, long ld1, float *__restrict__ c, long ldc,
+    const float *__restrict__ rcp, long ldr, int nvar,
+    void *__restrict__ xa, double dta, double dtb, float alpha, float beta)
+{
+    using V = V4<float>::type;
+    const int lane = threadIdx.x & 63;
+    const long c0 = ((long)blockIdx.x*(blockDim.x >> 6) +
+                     (threadIdx.x >> 6))*64;
+    if (c0 >= ncol) return;
+    const int kr = lane >> 4, cq = (lane & 15)*4;
+    const int KS = kb0 + kb1;
+
+    V acc[MT][4];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+            acc[mt][t] = V{0, 0, 0, 0};
+
+    for (int s = 0; s < KS; ++s) {
+        const bool p0 = s < kb0;
+        const float *bp = p0 ? b0 : b1;
+        const long ld = p0 ? ld0 : ld1;
+        const int kk = p0 ? k0 : k1;
+        const int r0 = 16*(p0 ? s : s - kb0) + 4*kr;
+        V rows[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            rows[i] = *reinterpret_cast<const V *>(
+                bp + min(r0 + i, kk - 1)*ld + c0 + cq);
+        s4v bop[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+            bop[t] = s4v{to_bf16(rows[0][t]), to_bf16(rows[1][t]),
+                         to_bf16(rows[2][t]), to_bf16(rows[3][t])};
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+            const s4v a = af[((long)mt*KS + s)*64 + lane];
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+                acc[mt][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(
+                    a, bop[t], acc[mt][t], 0, 0, 0);
+        }
+    }
+
+    const long e0 = (c0/(64L*nvar))*64 + cq;
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = 16*mt + 4*kr + r;
+            if (row >= m) continue;
+            const long off = row*ldc + c0 + cq;
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+                epi1<float, EP>(acc[mt][t][r], c, off + t, rcp,
+                                row*ldr + e0 + t, xa, dta, dtb, alpha, beta);
+        }
+}
+
+int bf16_plan_create(Bf16Plan **out, int m, int k0, int k1, const double *a0,
+                     const double *a1)
+{
+    FR_CHECK(m > 0 && k0 > 0 && k1 >= 0, "invalid operator shape");
+    const int mt = (m + 15)/16;
+    FR_CHECK(mt <= 8 || mt == 12,
+             "bf16 operator plans take at most 128 (or exactly 177-192) rows");
+    const int kb0 = (k0 + 15)/16, kb1 = (k1 + 15)/16, KS = kb0 + kb1;
+    std::vector<unsigned short> frag((size_t)mt*KS*64*4, 0);
+    for (int t = 0; t < mt; ++t)
+        for (int s = 0; s < KS; ++s)
+            for (int l = 0; l < 64; ++l)
+                for (int i = 0; i < 4; ++i) {
+                    const int r = 16*t + (l & 15);
+                    const int kp = 16*(s < kb0 ? s : s - kb0) + 4*(l >> 4) + i;
+                    double v = 0.0;
+                    if (r < m) {
+                        if (s < kb0 && kp < k0) v = a0[(long)r*k0 + kp];
+                        else if (s >= kb0 && kp < k1) v = a1[(long)r*k1 + kp];
+                    }
+                    frag[(((size_t)t*KS + s)*64 + l)*4 + i] =
+                        host_bf16((float)v);
+                }
+    void *d = nullptr;
+    FR_HIP(hipMalloc(&d, frag.size()*2));
+    hipError_t e = hipMemcpy(d, frag.data(), frag.size()*2,
+                             hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(d);
+        FR_HIP(e);
+    }
+    *out = new Bf16Plan{m, k0, k1, mt, kb0, kb1, d};
+    return 0;
+}
+
+void bf16_plan_destroy(Bf16Plan *p)
+{
+    if (!p) return;
+    if (p->afrag) (void)hipFree(p->afrag);
+    delete p;
+}
+
+template <int EP>
+static void launch_bf16_ep(const Bf16Plan *p, long ncol, const void *b0,
+                           long ld0, const void *b1, long ld1, void *c,
+                           long ldc, const void *rcp, long ldr, int nvar,
+                           void *xa, double dta, double dtb, double alpha,
+                           double beta, hipStream_t s)
+{
+    dim3 grid(cdiv(ncol/64, 4)), block(256);
+#define FR_MT(MTV)                                                          \
+    case MTV:                                                               \
+        hipLaunchKernelGGL((k_dense_bf16<MTV, EP>), grid, block, 0, s,      \
+                           p->m, p->kb0, p->kb1, p->k0, p->k1,              \
+                           (const s4v *)p->afrag, ncol, (const float *)b0,  \
+                           ld0, (const float *)b1, ld1, (float *)c, ldc,    \
+                           (const float *)rcp, ldr, nvar, xa, dta, dtb,     \
+                           (float)alpha, (float)beta);                      \
+        break;
+    switch (p->mt) {
+        FR_MT(1) FR_MT(2) FR_MT(3) FR_MT(4) FR_MT(5) FR_MT(6) FR_MT(7)
+        FR_MT(8) FR_MT(12)
+    }
+#undef FR_MT
+}
+
+int bf16_launch(const Bf16Plan *p, int ep, long ncol, const void *b0,
+                long ld0, const void *b1, long ld1, void *c, long ldc,
+                const void *rcp, long ldr, int nvar, void *xa, double dta,
+                double dtb, double alpha, double beta, hipStream_t s)
+{
+    const size_t va = 16;
+    auto al = [va](const void *q) {
+        return q == nullptr || (reinterpret_cast<uintptr_t>(q) % va) == 0;
+    };
+    FR_CHECK(ncol % 64 == 0 && ld0 % 4 == 0 && (p->k1 == 0 || ld1 % 4 == 0)
+             && ldc % 4 == 0 && al(b0) && al(b1) && al(c),
+             "bf16 plans need 64-column groups and 4-element alignment");
+#define FR_EP(E)                                                            \
+    case E:                                                                 \
+        launch_bf16_ep<E>(p, ncol, b0, ld0, b1, ld1, c, ldc, rcp, ldr,      \
+                          nvar, xa, dta, dtb, alpha, beta, s);              \
+        break;
+    switch (ep) {
+        FR_EP(0) FR_EP(1) FR_EP(2) FR_EP(3) FR_EP(4) FR_EP(5) FR_EP(6)
+        FR_EP(7) FR_EP(8)
+        default:
+            set_error("invalid bf16 epilogue");
+            return 2;
+    }
+#undef FR_EP
+    FR_LAUNCHED();
+    return 0;
+}
+
 // ---- flux-fused plans -----------------------------------------------------
 
 struct FluxPlan {
@@ -681,6 +1008,24 @@ static void launch_flux_ep(const FluxPlan *p, int wpb, long ncol,
                            const void *rcp, long ldr, void *xa, double dta,
                            double dtb, double gamma, hipStream_t s)
 {
+    if (wpb == FR_BLKV) {
+        constexpr size_t fbb = 2*4*3*5*64*sizeof(T);
+#define FR_MTB(MTV)                                                         \
+        case MTV:                                                           \
+            hipLaunchKernelGGL((k_tdivf_flux_blk<T, MTV, EP>),              \
+                               dim3(cdiv(ncol, 320)), dim3(320), fbb, s,    \
+                               p->d.m, p->nupts, p->nq, p->d.ks1, p->d.k1,  \
+                               (const T *)p->d.afrag, ncol, (const T *)u,   \
+                               ldu, (const T *)sm, lds, (const T *)fn, ldf, \
+                               (T *)c, ldc, (const T *)rcp, ldr, xa, dta,   \
+                               dtb, (T)(gamma - 1));                        \
+            break;
+        switch (p->d.mt) {
+            FR_MTB(1) FR_MTB(2) FR_MTB(3) FR_MTB(4) FR_MTB(5)
+        }
+#undef FR_MTB
+        return;
+    }
     const bool stage = wpb & FR_LDSV;
     const int w = wpb & ~FR_LDSV;
     const size_t ldsb = stage ? (size_t)p->d.mt*(p->d.ks0 + p->d.ks1)*64*
@@ -759,7 +1104,8 @@ static int flux_launch(const FluxPlan *p, int ep, long ncol, const void *u,
     FR_CHECK(ep < 2 || xa, "RK epilogue needs the r1 / accumulator array");
     const size_t esz = p->d.dtype == FRHIP_F64 ? 8 : 4;
     int wpb = p->d.variant > 0 ? p->d.variant : 4;
-    if (!variant_valid(wpb, (size_t)p->d.mt*(p->d.ks0 + p->d.ks1)*64*esz,
+    if (wpb != FR_BLKV &&
+        !variant_valid(wpb, (size_t)p->d.mt*(p->d.ks0 + p->d.ks1)*64*esz,
                        false))
         wpb &= ~FR_LDSV;
     const int rc = p->d.dtype == FRHIP_F64
@@ -877,8 +1223,9 @@ int frhip_tdivf_plan_tune(void *plan, int ncol, const void *u, long ldu,
     float best = 1e30f;
     int bestv = 4, rc = 0;
     const size_t ldsb = (size_t)p->d.mt*(p->d.ks0 + p->d.ks1)*64*esz;
-    for (int v : {1, 2, 4, FR_LDSV + 1, FR_LDSV + 2, FR_LDSV + 4}) {
-        if (!variant_valid(v, ldsb, false)) continue;
+    for (int v : {1, 2, 4, FR_LDSV + 1, FR_LDSV + 2, FR_LDSV + 4,
+                  FR_BLKV}) {
+        if (v != FR_BLKV && !variant_valid(v, ldsb, false)) continue;
         FluxPlan q = *p;
         q.d.variant = v;
         for (int it = 0; it < reps + 1 && !rc; ++it) {
@@ -912,7 +1259,8 @@ int frhip_tdivf_plan_set_variant(void *plan, int variant)
     FluxPlan *p = static_cast<FluxPlan *>(plan);
     FR_CHECK(p, "null plan");
     const size_t esz = p->d.dtype == FRHIP_F64 ? 8 : 4;
-    FR_CHECK(variant_valid(variant,
+    FR_CHECK(variant == FR_BLKV ||
+             variant_valid(variant,
                            (size_t)p->d.mt*(p->d.ks0 + p->d.ks1)*64*esz,
                            false), "invalid variant");
     p->d.variant = variant;

@@ -202,6 +202,52 @@ def test_simplex_rhs_parity(backend_factory, n, nt, npr, order, fused):
     assert err < tol
 
 
+@pytest.mark.parametrize('precision', ['double', 'single'])
+@pytest.mark.parametrize('variant', [1, 2, 4, 17, 18, 20, 32])
+def test_flux_variants_parity(backend_factory, precision, variant):
+    """Every variant of the flux-fused divergence kernel the tuner may pick
+    (1/2/4 waves per block, +16 LDS-staged operator, 32 block-cooperative
+    flux through LDS) against the oracle, at p = 3 (tet 20 / prism 24
+    solution points) and p = 4 (35 / 40)."""
+    from pyfr_amd._lib import FrHipError
+    from pyfr_amd.backend import MulProvider
+    # no graphs: a kernel reads its plan's variant at each launch
+    be = backend_factory(precision=precision, graphs=False, fused=True,
+                         fuse_flux=True)
+    mp = next(p for p in be._providers if isinstance(p, MulProvider))
+    for order in (3, 4):
+        print(f'\n[flux variant] {precision} v={variant} p={order}',
+              flush=True)
+        integ, orc, u0 = _mixed(be, 3, 2, 1, order)
+        sysm = integ.system
+        names = sysm.ele_types
+        sysm.rhs(0.0, 0, 1)       # builds (and tunes) the kernels
+        plans = [p for p, _, _ in mp._plans3.values()]
+        assert plans
+        taken = 0
+        for plan in plans:
+            try:
+                be.lib.tdivf_plan_set_variant(plan, variant)
+                taken += 1
+            except FrHipError:
+                # LDS-staged operators larger than the LDS budget are
+                # refused (the FP64 ones at p = 4); the rest must run
+                assert variant & 16, variant
+        if not taken:
+            continue
+        for b in sysm.ele_banks:      # the second RHS must rewrite it all
+            b[1].set(np.full(b[1].ioshape, np.nan))
+        sysm.rhs(0.0, 0, 1)
+        got = dict(zip(names, sysm.ele_scal_upts(1)))
+        ref = orc.rhs(u0)
+        tol = (max(1e-12, 10*_noise_floor(orc, u0))
+               if precision == 'double' else 1e-4)
+        err = rel_l2(_cat(got, names), _cat(ref, names))
+        print(f'[flux variant] rel L2 {err:.3e} (tol {tol:.3e})', flush=True)
+        assert err < tol
+        del integ, sysm
+
+
 def test_simplex_rk4_trajectory(backend_factory):
     be = backend_factory()
     integ, orc, u0 = _mixed(be, 3, 2, 1, 3, dt=2e-3)
@@ -251,3 +297,44 @@ def test_simplex_fp32_rhs_and_trajectory(backend_factory):
     err = rel_l2(_cat(dict(zip(names, integ.soln)), names), _cat(u, names))
     print(f'[simplex fp32 rk4] 10 steps rel L2 {err:.3e}', flush=True)
     assert err < 1e-5
+
+
+def _bf16(x):
+    """Round to bfloat16 (nearest even), returned as float64."""
+    a = np.asarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    a = (a + 0x7fff + ((a >> 16) & 1)) >> 16 << 16
+    return a.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+@pytest.mark.parametrize('shape', [(64, 192, 96), (192, 64, 96),
+                                   (35, 105, 60)])
+def test_bf16_operand_plans(backend_factory, shape):
+    """BF16-operand matrix-core plans (the precision form's BF16 run): the
+    product equals NumPy's on the bfloat16-rounded operands to FP32
+    accumulation error, and differs from the unrounded product by about
+    the bfloat16 rounding."""
+    import ctypes as ct
+    be = backend_factory(precision='single')
+    m, k0, k1 = shape
+    neles = 128
+    A0, A1, b0, b1, c, _ = _operands(be, m, k0, k1, neles, 7)
+    h0, h1 = np.ascontiguousarray(A0), np.ascontiguousarray(A1)
+    plan, kind = ct.c_void_p(), ct.c_int()
+    be.lib.mul2_plan_create(ct.byref(plan), m, k0, k1, h0.ctypes.data,
+                            h1.ctypes.data, 2, ct.byref(kind))
+    try:
+        assert kind.value == 3
+        print(f'\n[bf16 plan] {shape}', flush=True)
+        be.lib.mul2(plan.value, c.ncol, b0.data, b0.leaddim, b1.data,
+                    b1.leaddim, c.data, c.leaddim, 0, None, 0, None, 0.0,
+                    0.0, 0, be.stream)
+        be.wait()
+    finally:
+        be.lib.mul2_plan_destroy(plan.value)
+    B0, B1 = b0.get().reshape(k0, -1), b1.get().reshape(k1, -1)
+    exact = (A0 @ B0 + A1 @ B1).reshape(m, 5, neles)
+    rounded = (_bf16(A0) @ _bf16(B0) + _bf16(A1) @ _bf16(B1)).reshape(
+        m, 5, neles)
+    got = c.get()
+    assert rel_l2(got, rounded) < 2e-6
+    assert 1e-4 < rel_l2(got, exact) < 2e-2
