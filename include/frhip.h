@@ -31,7 +31,7 @@ enum { FRHIP_F64 = 0, FRHIP_F32 = 1 };
 /* Operator plans only: FP32 data with both matrix-core operands rounded to
  * bfloat16 and FP32 accumulation (the precision form's BF16 experiment). */
 enum { FRHIP_F32_BF16OPS = 2 };
-/* Riemann solvers (pyfr/solvers/euler/kernels/rsolvers/*.mako). */
+/* Riemann solvers (pyfr/solvers/euler/kernels/rsolvers/<name>.mako). */
 enum { FRHIP_RS_RUSANOV = 0, FRHIP_RS_HLLC = 1, FRHIP_RS_HLL = 2,
        FRHIP_RS_ROE = 3, FRHIP_RS_ROEM = 4 };
 
@@ -221,6 +221,49 @@ int frhip_intcflux_ns(int dtype, int rsolver, int n, void *ubase,
                       const int32_t *grmap, long gstride, const void *nl,
                       long ldn, const double *consts, const void *mu_p,
                       void *stream);
+
+/* ---- boundary interfaces, 1-D over n boundary flux points ----
+ * Types: the `type` keys of [soln-bcs-*] (pyfr/solvers/euler/inters.py:57-102,
+ * pyfr/solvers/navstokes/inters.py:70-219); the first four exist for Euler and
+ * NS, the rest for NS only. */
+enum { FRHIP_BC_SUP_IN_FA = 0, FRHIP_BC_SUP_OUT_FN = 1,
+       FRHIP_BC_CHAR_RIEM_INV = 2, FRHIP_BC_SLP_ADIA_WALL = 3,
+       FRHIP_BC_NO_SLP_ADIA_WALL = 4, FRHIP_BC_NO_SLP_ISOT_WALL = 5,
+       FRHIP_BC_SUB_IN_FRV = 6, FRHIP_BC_SUB_IN_FTPTTANG = 7,
+       FRHIP_BC_SUB_OUT_FP = 8 };
+
+/* bcv: the boundary's prescribed values per point (rows of n, leading dim
+ * ldb, in dtype), in place of the reference's ploc-dependent template
+ * expressions: sup-in-fa and char-riem-inv {rho, p, u, v, w}; sub-in-frv
+ * {rho, u, v, w}; no-slp-isot-wall {u, v, w}; sub-out-fp {p}; none for the
+ * other types (bcv may be NULL).  consts = the 8 constants of
+ * frhip_intcflux_ns followed by {cpTw, cpTt, pt, vc0, vc1, vc2} (the
+ * isothermal-wall temperature, and sub-in-ftpttang's total enthalpy, total
+ * pressure and flow direction); the Euler kernel reads gamma only.
+ *
+ * Euler bccflux: pyfr/solvers/euler/kernels/bccflux.mako:7-25 with the
+ * type's bc_rsolve_state (euler/kernels/bcs/<type>.mako). */
+int frhip_bccflux_euler(int dtype, int bctype, int rsolver, int n,
+                        void *base, const int32_t *lmap, const void *nl,
+                        long ldn, const void *bcv, long ldb,
+                        const double *consts, void *stream);
+
+/* NS bcconu: pyfr/solvers/navstokes/kernels/bcconu.mako:6-15 (the type's
+ * bc_ldg_state, navstokes/kernels/bcs/<type>.mako), ulout = LDG state. */
+int frhip_bcconu(int dtype, int bctype, int n, const void *ubase,
+                 const int32_t *ulin, void *cbase, const int32_t *ulout,
+                 const void *nl, long ldn, const void *bcv, long ldb,
+                 const double *consts, void *stream);
+
+/* NS bccflux: pyfr/solvers/navstokes/kernels/bccflux.mako:10-20 with the
+ * type's common-flux state (bcs/ghost.mako, bcs/ghost-imperm.mako for the
+ * no-slip walls, bcs/slp-adia-wall.mako:21-37).  Gradient view as in
+ * frhip_intcflux_ns (unused, may be NULL, for slp-adia-wall). */
+int frhip_bccflux_ns(int dtype, int bctype, int rsolver, int n, void *ubase,
+                     const int32_t *lmap, const void *gbase,
+                     const int32_t *glmap, long gstride, const void *nl,
+                     long ldn, const void *bcv, long ldb,
+                     const double *consts, void *stream);
 
 /* ---- BLAS extensions: pyfr/backends/hip/blasext.py:8-136 ---- */
 

@@ -47,33 +47,61 @@ def _wrap(x, length):
     return y
 
 
-def pair_faces_bruteforce(fpts_phys, nfaces, nfacefpts, length):
+def pair_faces_bruteforce(fpts_phys, nfaces, nfacefpts, length,
+                          periodic=(True, True, True), boundaries=None):
     """Pair element faces and their flux points.
 
     fpts_phys: (nfpts, 3, E) physical flux-point coordinates (faces' points
     contiguous, ``nfacefpts`` per face).  Returns a list of
     (eL, fL, idxL, eR, fR, idxR) where idxL/idxR are flux-point indices
     (into the element's fpts) of matched points, in matched order.
+
+    Coordinates wrap only along the ``periodic`` axes.  Unpaired faces are
+    an error unless ``boundaries`` (a dict) is given: it then receives, per
+    box side name 'x0', 'x1', 'y0', ... (axis, coordinate 0 or ``length``),
+    a list of (e, f, idx) of the faces lying on that side.
     """
     nfpts, _, neles = fpts_phys.shape
     off = np.cumsum([0] + list(nfacefpts))
+    per = np.array(periodic, dtype=bool)
+
+    def wrap(x):
+        y = x.copy()
+        y[..., per] = _wrap(x[..., per], length)
+        return y
+
     keys = {}
     for e in range(neles):
         for f in range(nfaces):
             pts = fpts_phys[off[f]:off[f + 1], :, e]
-            c = _wrap(pts.mean(axis=0), length)
+            c = wrap(pts.mean(axis=0))
             key = tuple(np.round(c, 6))
             keys.setdefault(key, []).append((e, f))
 
     pairs = []
     for key, faces in keys.items():
+        if len(faces) == 1 and boundaries is not None:
+            (e, f), = faces
+            side = [f'{"xyz"[a]}{int(np.isclose(key[a], length))}'
+                    for a in range(3) if not per[a] and
+                    (np.isclose(key[a], 0) or np.isclose(key[a], length))]
+            if len(side) != 1:
+                raise ValueError(f'Unpaired interior face at {key}')
+            boundaries.setdefault(side[0], []).append(
+                (e, f, off[f] + np.arange(off[f + 1] - off[f])))
+            continue
         if len(faces) != 2:
             raise ValueError(f'Unpaired face at {key}: {faces}')
-        (eL, fL), (eR, fR) = sorted(faces)
-        pl = _wrap(fpts_phys[off[fL]:off[fL + 1], :, eL], length)
-        pr = _wrap(fpts_phys[off[fR]:off[fR + 1], :, eR], length)
+        # left = smaller (face, element) key, as the reference's
+        # _construct_con keeps the pair with lkey < rkey, lkey = (face codec
+        # index, element index) (pyfr/readers/native.py:404-406, :466-470);
+        # HLL is not antisymmetric under a side swap, so this matters
+        (eL, fL), (eR, fR) = sorted(faces, key=lambda q: (q[1], q[0]))
+        pl = wrap(fpts_phys[off[fL]:off[fL + 1], :, eL])
+        pr = wrap(fpts_phys[off[fR]:off[fR + 1], :, eR])
         d = np.abs(pl[:, None, :] - pr[None, :, :])
-        d = np.minimum(d, length - d).sum(axis=2)
+        d[..., per] = np.minimum(d[..., per], length - d[..., per])
+        d = d.sum(axis=2)
         jr = np.argmin(d, axis=1)
         if np.any(d[np.arange(len(pl)), jr] > 1e-8) or \
            len(set(jr.tolist())) != len(pl):

@@ -12,6 +12,7 @@ Navier-Stokes with gradient fusion (``pyfr/solvers/baseadvecdiff/system.py:53-23
 import numpy as np
 
 from oracle.basis import HexShapeOracle
+from oracle.bcs import bccflux_euler, bccflux_ns, bcconu
 from oracle.geom import (linear_smats_djac, pnorm_at, rcpdjac_at, smats_at,
                          trilinear_ploc)
 from oracle.mesh import box_hex_verts, pair_faces_bruteforce
@@ -36,7 +37,12 @@ class OracleHexSystem:
     """
 
     def __init__(self, n, order, system='euler', rsolver='rusanov',
-                 constants=None, length=2*np.pi, verts=None):
+                 constants=None, length=2*np.pi, verts=None,
+                 periodic=(True, True, True), bcs=None):
+        """periodic: per-axis periodicity; the sides of the other axes
+        ('x0', 'x1', 'y0', ...) are boundaries, each needing an entry
+        ``bcs[side] = (type, vals)`` (``oracle.bcs``; vals may hold
+        callables of the flux-point coordinates (x, y, z))."""
         self.n, self.order, self.system = n, order, system
         self.rsolver = RSOLVERS[rsolver]
         self.c = dict(constants or {'gamma': 1.4})
@@ -71,7 +77,19 @@ class OracleHexSystem:
 
         # Physical flux-point locations and the interface pairing
         self.plocf = trilinear_ploc(self.verts, sh.fpts)
-        pairs = pair_faces_bruteforce(self.plocf, 6, sh.nfacefpts, length)
+        bfaces = {}
+        pairs = pair_faces_bruteforce(self.plocf, 6, sh.nfacefpts, length,
+                                      periodic, bfaces)
+        self.bcs = []
+        for side, faces in sorted(bfaces.items()):
+            bctype, vals = (bcs or {})[side]
+            bf = np.concatenate([f[2] for f in faces])
+            be = np.concatenate([np.full(len(f[2]), f[0]) for f in faces])
+            x, y, z = self.plocf[bf, :, be].T
+            v = {k: (fn(x, y, z) if callable(fn) else fn)
+                 for k, fn in vals.items()}
+            self.bcs.append((side, bctype, bf, be,
+                             self.pnorm[bf, be].T, v))
         self.lf = np.concatenate([p[2] for p in pairs])
         self.le = np.concatenate([np.full(len(p[2]), p[0]) for p in pairs])
         self.rf = np.concatenate([p[5] for p in pairs])
@@ -116,10 +134,17 @@ class OracleHexSystem:
         uf[self.lf, :, self.le] = (mag*fn).T
         uf[self.rf, :, self.re] = (-mag*fn).T
 
+    def _bccflux_euler(self, uf):
+        for _, bctype, bf, be, nl, vals in self.bcs:
+            ul = uf[bf, :, be].T
+            uf[bf, :, be] = bccflux_euler(bctype, self.rsolver, ul, nl, vals,
+                                          self.c).T
+
     # -- Euler RHS
     def rhs_euler(self, u):
         uf = self._mul(self.M0, u)                          # disu
         self._intcflux_euler(uf)                            # intcflux
+        self._bccflux_euler(uf)                             # bccflux
         F = self._transform(self._tflux_inv(u))             # tdisf
         div = self._mul(self.M132, F.reshape(3*self.nupts, 5, -1))
         div = self._mul(self.M3, uf) + div                  # tdivtconf
@@ -138,6 +163,8 @@ class OracleHexSystem:
         cu = ur*(0.5 + beta) + ul*(0.5 - beta)
         comm[self.lf, :, self.le] = cu.T
         comm[self.rf, :, self.re] = cu.T
+        for _, bctype, bf, be, nl, vals in self.bcs:          # bcconu
+            comm[bf, :, be] = bcconu(bctype, uf[bf, :, be].T, nl, vals, c).T
 
         grad = self._mul(self.M460, u)                      # tgradpcoru
         grad = self._mul(self.M6, comm) + grad              # tgradcoru
@@ -179,8 +206,13 @@ class OracleHexSystem:
             if tau != 0.0:
                 fv = fv + tau*(ul[i] - ur[i])
             out[i] = fi[i] + fv
+        bcout = [bccflux_ns(bctype, self.rsolver, uf[bf, :, be].T,
+                            gf[:, bf, :, be].transpose(1, 2, 0), nl, vals, c)
+                 for _, bctype, bf, be, nl, vals in self.bcs]
         uf[self.lf, :, self.le] = (mag*out).T
         uf[self.rf, :, self.re] = (-mag*out).T
+        for (_, _, bf, be, _, _), o in zip(self.bcs, bcout):  # bccflux
+            uf[bf, :, be] = o.T
 
         div = self._mul(self.M132, F.reshape(3*Nu, 5, -1))  # tdivtpcorf
         div = self._mul(self.M3, uf) + div                  # tdivtconf

@@ -79,7 +79,8 @@ def pair_faces_mixed(etypes, blocks):
     coordinates.  blocks: list of (periods (3,), {etype: element indices});
     faces pair only within a block.  Returns a list of
     (tL, eL, idxL, tR, eR, idxR) with flux-point index arrays in matched
-    order; the left side is the smaller (etype order, element, face)."""
+    order; the left side is the smaller (etype order, face, element) key, the
+    reference's lkey < rkey rule (pyfr/readers/native.py:404-406, :466-470)."""
     order = {n: i for i, n in enumerate(etypes)}
     pairs = []
     for periods, members in blocks:
@@ -99,7 +100,7 @@ def pair_faces_mixed(etypes, blocks):
             if len(faces) != 2:
                 raise ValueError(f'Unpaired face at {key}: {faces}')
             (_, tL, eL, fL), (_, tR, eR, fR) = sorted(
-                faces, key=lambda q: (q[0], q[2], q[3]))
+                faces, key=lambda q: (q[0], q[3], q[2]))
             sl, sr = etypes[tL].sh, etypes[tR].sh
             offl = np.cumsum([0] + sl.nfacefpts)
             offr = np.cumsum([0] + sr.nfacefpts)

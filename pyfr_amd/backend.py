@@ -682,6 +682,9 @@ class PointwiseProvider:
         'pyfr.solvers.euler.kernels.intcflux': 'intcflux',
         'pyfr.solvers.navstokes.kernels.intcflux': 'intcflux',
         'pyfr.solvers.navstokes.kernels.intconu': 'intconu',
+        'pyfr.solvers.euler.kernels.bccflux': 'bccflux',
+        'pyfr.solvers.navstokes.kernels.bccflux': 'bccflux',
+        'pyfr.solvers.navstokes.kernels.bcconu': 'bcconu',
         'pyfr.solvers.baseadvec.kernels.negdivconf': 'negdivconf',
         'pyfr.integrators.explicit.kernels.rkvdh2': 'rkvdh2',
     }
@@ -784,6 +787,69 @@ class PointwiseProvider:
                 urout.mapping.data, tplargs['c']['ldg-beta'])
         return Kernel(lambda stream: lib.intconu(*args, stream),
                       views=(ulin, urin, ulout, urout), name='intconu')
+
+    # -- boundary interfaces -------------------------------------------------
+    BC_TYPES = {'sup-in-fa': 0, 'sup-out-fn': 1, 'char-riem-inv': 2,
+                'slp-adia-wall': 3, 'no-slp-adia-wall': 4,
+                'no-slp-isot-wall': 5, 'sub-in-frv': 6,
+                'sub-in-ftpttang': 7, 'sub-out-fp': 8}
+
+    def _bc(self, tplargs, euler):
+        bct = tplargs['bctype']
+        if bct not in self.BC_TYPES or (euler and self.BC_TYPES[bct] > 3):
+            raise NotSuitableError(f'Boundary type {bct} not available')
+        c = tplargs['c']
+        vc = list(c.get('vc', (0.0, 0.0, 0.0)))
+        consts = (ct.c_double*14)(
+            c['gamma'], c.get('mu', 0.0), c.get('Pr', 1.0),
+            c.get('ldg-beta', 0.0), c.get('ldg-tau', 0.0),
+            1.0 if tplargs.get('visc_corr') == 'sutherland' else 0.0,
+            c.get('cpTref', 1.0), c.get('cpTs', 0.0), c.get('cpTw', 0.0),
+            c.get('cpTt', 0.0), c.get('pt', 0.0), *vc)
+        return self.BC_TYPES[bct], consts
+
+    def bccflux(self, tplargs, dims, ul, nl, gradul=None, artvisc=None,
+                bcvals=None, **kw):
+        """Boundary common flux (``euler/kernels/bccflux.mako``,
+        ``navstokes/kernels/bccflux.mako``).  ``bcvals``: the boundary's
+        prescribed values per point (rows as in ``frhip.h``), which the
+        reference splices into the kernel as ploc expressions."""
+        lib = self.backend.lib
+        (n,) = dims
+        dt = _dtcode(ul.refdtype)
+        rs = self._rs(tplargs)
+        euler = gradul is None and 'visc_corr' not in tplargs
+        bct, consts = self._bc(tplargs, euler)
+        bv = None if bcvals is None else bcvals.data
+        ldb = 0 if bcvals is None else bcvals.leaddim
+        if euler:
+            args = (dt, bct, rs, n, ul.basedata, ul.mapping.data, nl.data,
+                    nl.leaddim, bv, ldb, consts)
+            return Kernel(lambda stream: lib.bccflux_euler(*args, stream),
+                          views=(ul,), mats=(nl, bcvals), name='bccflux')
+        if artvisc is not None:
+            raise NotSuitableError('artificial viscosity is not supported')
+        g = (None, None, 0) if gradul is None else \
+            (gradul.basedata, gradul.mapping.data, gradul.rstrides_val)
+        args = (dt, bct, rs, n, ul.basedata, ul.mapping.data, *g, nl.data,
+                nl.leaddim, bv, ldb, consts)
+        return Kernel(lambda stream: lib.bccflux_ns(*args, stream),
+                      views=(ul, gradul), mats=(nl, bcvals),
+                      name='bccflux_ns')
+
+    def bcconu(self, tplargs, dims, ulin, ulout, nlin, bcvals=None, **kw):
+        """``navstokes/kernels/bcconu.mako:6-15``."""
+        lib = self.backend.lib
+        (n,) = dims
+        bct, consts = self._bc(tplargs, False)
+        bv = None if bcvals is None else bcvals.data
+        ldb = 0 if bcvals is None else bcvals.leaddim
+        args = (_dtcode(ulin.refdtype), bct, n, ulin.basedata,
+                ulin.mapping.data, ulout.basedata, ulout.mapping.data,
+                nlin.data, nlin.leaddim, bv, ldb, consts)
+        return Kernel(lambda stream: lib.bcconu(*args, stream),
+                      views=(ulin, ulout), mats=(nlin, bcvals),
+                      name='bcconu')
 
     # -- rkvdh2 --------------------------------------------------------------
     def rkvdh2(self, tplargs, dims, r1, r2, rold=None, rerr=None, dt=None,

@@ -23,12 +23,18 @@ HEX_FACE_AXES = [(2, -1), (1, -1), (0, +1), (1, +1), (0, -1), (2, +1)]
 
 
 class BoxMesh:
-    """n^3 (or nx*ny*nz) periodic box of linear hexes on [0, L)^3."""
+    """n^3 (or nx*ny*nz) box of linear hexes on [0, L)^3, periodic along
+    the ``periodic`` axes.  The two sides of every other axis are
+    boundaries named 'x0'/'x1', 'y0'/'y1', 'z0'/'z1' (low/high side):
+    ``bcon[name]`` lists their (etype, eidx, fidx) faces, the counterpart of
+    a mesh file's ``bcon_<name>`` arrays."""
     ndims = 3
 
-    def __init__(self, n, length=2*np.pi, origin=0.0):
+    def __init__(self, n, length=2*np.pi, origin=0.0,
+                 periodic=(True, True, True)):
         nx, ny, nz = (n, n, n) if np.isscalar(n) else n
         self.shape = (nx, ny, nz)
+        self.periodic = tuple(bool(p) for p in periodic)
         self.length = length
         hx, hy, hz = length/nx, length/ny, length/nz
         E = nx*ny*nz
@@ -52,20 +58,33 @@ class BoxMesh:
         nf = len(HEX_FACE_AXES)
         dims = np.array(self.shape)
         keys = np.empty((nf, E, 4), dtype=np.int64)
+        bnd = np.zeros((nf, E), dtype=bool)
+        self.bcon = {}
         for f, (nax, side) in enumerate(HEX_FACE_AXES):
             oth = [a for a in range(3) if a != nax]
+            plane = ijk[nax] + (side > 0)
+            if self.periodic[nax]:
+                plane = plane % dims[nax]
+            else:
+                b = (plane == 0) | (plane == dims[nax])
+                bnd[f] = b
+                name = f'{"xyz"[nax]}{int(side > 0)}'
+                self.bcon.setdefault(name, []).extend(
+                    ('hex', int(e), f) for e in np.flatnonzero(b))
             keys[f, :, 0] = nax
-            keys[f, :, 1] = (ijk[nax] + (side > 0)) % dims[nax]
+            keys[f, :, 1] = plane
             keys[f, :, 2] = ijk[oth[0]]
             keys[f, :, 3] = ijk[oth[1]]
         keys = keys.reshape(nf*E, 4)
         fidx = np.repeat(np.arange(nf), E)
         eidx = np.tile(np.arange(E), nf)
+        keep = ~bnd.ravel()
+        keys, fidx, eidx = keys[keep], fidx[keep], eidx[keep]
         order = np.lexsort(keys.T[::-1])
         sk = keys[order]
         same = np.all(sk[1:] == sk[:-1], axis=1)
         starts = np.flatnonzero(same)
-        if 2*len(starts) != nf*E or np.any(same[1:] & same[:-1]):
+        if 2*len(starts) != len(keys) or np.any(same[1:] & same[:-1]):
             raise ValueError('Faces are not all paired exactly once')
         a, b = order[starts], order[starts + 1]
         # left = smaller (fidx, eidx) key, i.e. smaller flat index

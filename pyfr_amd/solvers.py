@@ -11,9 +11,10 @@ written against ``pyfr/backends/base``:
   ``pyfr/solvers/euler/elements.py:149-206``,
   ``pyfr/solvers/navstokes/elements.py:31-128``;
 * interfaces   -- ``pyfr/solvers/base/inters.py``,
-  ``pyfr/solvers/baseadvec/inters.py:8-29``,
-  ``pyfr/solvers/baseadvecdiff/inters.py:7-35``,
-  ``pyfr/solvers/euler/inters.py:28-37``, ``pyfr/solvers/navstokes/inters.py:29-47``;
+  ``pyfr/solvers/baseadvec/inters.py:8-116``,
+  ``pyfr/solvers/baseadvecdiff/inters.py:7-76``,
+  ``pyfr/solvers/euler/inters.py:28-102``,
+  ``pyfr/solvers/navstokes/inters.py:29-219`` (internal and boundary);
 * systems      -- ``pyfr/solvers/base/system.py``,
   ``pyfr/solvers/baseadvec/system.py:57-136``,
   ``pyfr/solvers/baseadvecdiff/system.py:53-231``;
@@ -21,12 +22,14 @@ written against ``pyfr/backends/base``:
   ``pyfr/integrators/base.py:282-296``, ``:401-426``,
   ``pyfr/integrators/explicit/controllers.py:42-60``.
 
-Scope: periodic meshes of hexahedra, straight-sided tetrahedra and right
-prisms on one GPU (no boundary conditions, no MPI interfaces, no
-anti-aliasing, no source terms).
+Scope: meshes of hexahedra, straight-sided tetrahedra and right prisms on
+one GPU, periodic or with the reference's fixed-value boundary conditions
+(not the controlled char-riem-inv variants nor time-dependent boundary
+values); no MPI interfaces, no anti-aliasing, no source terms.
 """
 from functools import cached_property
 import math
+import re
 import time
 
 import numpy as np
@@ -264,6 +267,10 @@ class FluidElements:
         fpts_idx = self.srtd_face_fpts[fidx][eidxs]
         pn = self._pnorm_fpts[fpts_idx, eidxs[:, None]]
         return pn.reshape(-1, self.ndims),
+
+    def get_ploc_for_inters(self, eidxs, fidx):
+        fpts_idx = self.srtd_face_fpts[fidx][eidxs]
+        return self.plocfpts[fpts_idx, eidxs[:, None]].reshape(-1, self.ndims),
 
     def _inters_map(self, mid, rmap, eidxs, fidx, lda=None):
         nfp = self.nfacefpts[fidx]
@@ -574,6 +581,137 @@ class NavierStokesIntInters(EulerIntInters):
         self._perm = self._get_perm_for_field(side, scal)
 
 
+# Boundary values each type reads from its [soln-bcs-*] section, as
+# expressions of the constants and x, y, z (pyfr/solvers/euler/inters.py:
+# 57-102, pyfr/solvers/navstokes/inters.py:111-219), and their defaults
+BC_VALUES = {'sup-in-fa': ('rho', 'p', 'u', 'v', 'w'),
+             'char-riem-inv': ('rho', 'p', 'u', 'v', 'w'),
+             'sub-in-frv': ('rho', 'u', 'v', 'w'),
+             'no-slp-isot-wall': ('u', 'v', 'w'),
+             'sub-out-fp': ('p',)}
+BC_DEFAULTS = {'sub-in-frv': {'u': 0, 'v': 0, 'w': 0},
+               'no-slp-isot-wall': {'u': 0, 'v': 0, 'w': 0}}
+
+
+class EulerBCInters(EulerIntInters):
+    """Boundary interfaces (``pyfr/solvers/baseadvec/inters.py:52-116``,
+    ``euler/inters.py:40-102``).  The reference evaluates the boundary
+    values inside the kernel from expressions of ``ploc`` (and ``t``);
+    here they are evaluated once on the host at the boundary flux points
+    and passed as per-point rows, so time-dependent values are declined."""
+    bctypes = ('sup-in-fa', 'sup-out-fn', 'char-riem-inv', 'slp-adia-wall')
+    bccflux_mod = 'pyfr.solvers.euler.kernels.bccflux'
+
+    def __init__(self, be, lhs, elemap, cfgsect, cfg):
+        from pyfr_amd.backend import NotSuitableError
+        self._be = be
+        self.elemap = elemap
+        self.cfg, self.cfgsect = cfg, cfgsect
+        self.name = cfgsect.removeprefix('soln-bcs-')
+        self.type = cfg.get(cfgsect, 'type')
+        if self.type not in self.bctypes:
+            raise NotSuitableError(f'Boundary type {self.type} is not '
+                                   'available')
+        first = next(iter(elemap.values()))
+        self.ndims, self.nvars = first.ndims, first.nvars
+        self.lhs = Interside(lhs)
+        self.ninters = len(lhs)
+        self.ninterfpts = sum(elemap[et].nfacefpts[f]*len(e)
+                              for et, f, e in self.lhs.items())
+        self.c = cfg.items_as('constants', float)
+        self.kernels = {}
+        scal = {t: e._scal_fpts for t, e in elemap.items()}
+        self._perm = self._get_perm_for_field(self.lhs, scal)
+        self._pnorm_lhs = self._const_mat(self.lhs, 'get_pnorms_for_inters')
+        self._bc_consts()
+        self._bcvals = self._eval_values()
+        rsolver = cfg.get('solver-interfaces', 'riemann-solver')
+        self._tplargs = dict(ndims=self.ndims, nvars=self.nvars,
+                             rsolver=rsolver, c=self.c, bctype=self.type)
+        self._kernels()
+
+    def _kernels(self):
+        be = self._be
+        be.pointwise.register(self.bccflux_mod)
+        self.kernels['comm_flux'] = lambda: be.kernel(
+            'bccflux', tplargs=self._tplargs, dims=[self.ninterfpts],
+            ul=self.scal_lhs, nl=self._pnorm_lhs, bcvals=self._bcvals)
+
+    def _eval_opt(self, k):
+        """A scalar option (the reference's ``_eval_opts``)."""
+        env = self.cfg.items_as('constants', float)
+        return float(_npeval(self.cfg.getexpr(self.cfgsect, k), env))
+
+    def _bc_consts(self):
+        pass
+
+    def _eval_values(self):
+        from pyfr_amd.backend import NotSuitableError
+        keys = BC_VALUES.get(self.type)
+        if not keys:
+            return None
+        m = _get_inter_arrays(self.lhs, 'get_ploc_for_inters', self.elemap,
+                              self._perm)
+        ploc = m[0] if m else np.empty((0, self.ndims))
+        env = self.cfg.items_as('constants', float)
+        env |= dict(zip('xyz', ploc.T))
+        dflt = BC_DEFAULTS.get(self.type, {})
+        rows = []
+        for k in keys:
+            expr = (self.cfg.getexpr(self.cfgsect, k, dflt[k]) if k in dflt
+                    else self.cfg.getexpr(self.cfgsect, k))
+            if re.search(r'(?<![\w.])t(?![\w.])', expr):
+                raise NotSuitableError('time-dependent boundary values are '
+                                       'not supported')
+            rows.append(np.broadcast_to(_npeval(expr, env),
+                                        (self.ninterfpts,)))
+        return self._be.const_matrix(np.array(rows, dtype=float))
+
+
+class NavierStokesBCInters(EulerBCInters):
+    """``baseadvecdiff/inters.py:60-76``, ``navstokes/inters.py:70-219``."""
+    bctypes = ('sup-in-fa', 'sup-out-fn', 'char-riem-inv', 'slp-adia-wall',
+               'no-slp-adia-wall', 'no-slp-isot-wall', 'sub-in-frv',
+               'sub-in-ftpttang', 'sub-out-fp')
+    bccflux_mod = 'pyfr.solvers.navstokes.kernels.bccflux'
+
+    def _bc_consts(self):
+        self.c |= self.cfg.items_as('solver-interfaces', float)
+        if self.type == 'no-slp-isot-wall':
+            self.c['cpTw'] = self._eval_opt('cpTw')
+        elif self.type == 'sub-in-ftpttang':
+            # navstokes/inters.py:177-203
+            gamma = self.cfg.getfloat('constants', 'gamma')
+            self.c['cpTt'] = self._eval_opt('cpTt')
+            self.c['pt'] = self._eval_opt('pt')
+            self.c['Rdcp'] = (gamma - 1.0)/gamma
+            theta = self._eval_opt('theta')*np.pi/180.0
+            phi = self._eval_opt('phi')*np.pi/180.0
+            vc = np.array([np.cos(theta), np.sin(theta), 1.0])
+            vc[:2] *= np.sin(phi)
+            vc[2] *= np.cos(phi)
+            self.c['vc'] = tuple(float(v) for v in vc)
+
+    def _kernels(self):
+        be = self._be
+        self._tplargs['visc_corr'] = self.cfg.get(
+            'solver', 'viscosity-correction', 'none')
+        self._vect_lhs = self._view(self.lhs, 'get_vect_fpts_for_inters',
+                                    (self.ndims, self.nvars))
+        self._comm_lhs = self._view(self.lhs, 'get_comm_fpts_for_inters',
+                                    (self.nvars,))
+        be.pointwise.register('pyfr.solvers.navstokes.kernels.bcconu')
+        be.pointwise.register(self.bccflux_mod)
+        self.kernels['con_u'] = lambda: be.kernel(
+            'bcconu', tplargs=self._tplargs, dims=[self.ninterfpts],
+            ulin=self.scal_lhs, ulout=self._comm_lhs, nlin=self._pnorm_lhs,
+            bcvals=self._bcvals)
+        self.kernels['comm_flux'] = lambda: be.kernel(
+            'bccflux', tplargs=self._tplargs, dims=[self.ninterfpts],
+            ul=self.scal_lhs, gradul=self._vect_lhs, nl=self._pnorm_lhs,
+            bcvals=self._bcvals)
+
+
 # ---------------------------------------------------------------------------
 # Systems
 # ---------------------------------------------------------------------------
@@ -618,9 +756,18 @@ class BaseSystem:
         for e in elemap.values():
             e.post_commit()
 
-        # Interfaces
+        # Interfaces; boundaries from the mesh's bcon_<name> face lists and
+        # the [soln-bcs-<name>] sections (pyfr/solvers/base/system.py:
+        # 235-262)
         lhs, rhs = mesh.con
         self._int_inters = [self.intinterscls(backend, lhs, rhs, elemap, cfg)]
+        self._bc_inters = []
+        for name, faces in sorted(getattr(mesh, 'bcon', {}).items()):
+            sect = f'soln-bcs-{name}'
+            if not cfg.hasopt(sect, 'type'):
+                raise ValueError(f'No [{sect}] type for boundary {name}')
+            self._bc_inters.append(
+                self.bcinterscls(backend, faces, elemap, sect, cfg))
         self._commit()
         self._kcache = {}
 
@@ -629,6 +776,14 @@ class BaseSystem:
         for i in self._int_inters:
             i.scal_lhs = i._view(i.lhs, '_scal_fpts_for_inters', (self.nvars,))
             i.scal_rhs = i._view(i.rhs, '_scal_fpts_for_inters', (self.nvars,))
+        for b in self._bc_inters:
+            b.scal_lhs = b._view(b.lhs, '_scal_fpts_for_inters', (self.nvars,))
+
+    @property
+    def _inters(self):
+        """Internal then boundary interfaces (the reference runs their
+        kernels of one kind together in one graph stage)."""
+        return self._int_inters + self._bc_inters
 
     @property
     def gndofs(self):
@@ -753,12 +908,13 @@ class EulerSystem(BaseSystem):
     name = 'euler'
     elementscls = EulerElements
     intinterscls = EulerIntInters
+    bcinterscls = EulerBCInters
 
     def _fused_kernels(self, uin, fout, xa, stage):
         from pyfr_amd.backend import NotSuitableError
         eles = list(self.ele_map.values())
         ks = [e.kernels['disu'](uin) for e in eles]
-        ks += [i.kernels['comm_flux']() for i in self._int_inters]
+        ks += [i.kernels['comm_flux']() for i in self._inters]
         for e in eles:
             if 'tdivf_flux' in e.kernels:
                 try:
@@ -777,7 +933,7 @@ class EulerSystem(BaseSystem):
         g = be.graph()
         eles = list(self.ele_map.values())
         ks = [e.kernels['disu'](uin) for e in eles]
-        ks += [i.kernels['comm_flux']() for i in self._int_inters]
+        ks += [i.kernels['comm_flux']() for i in self._inters]
         for e in eles:
             ks += [e.kernels['tdisf'](uin), e.kernels['tdivtpcorf'](fout),
                    e.kernels['tdivtconf'](fout), e.kernels['negdivconf'](fout)]
@@ -790,15 +946,16 @@ class NavierStokesSystem(BaseSystem):
     name = 'navier-stokes'
     elementscls = NavierStokesElements
     intinterscls = NavierStokesIntInters
+    bcinterscls = NavierStokesBCInters
 
     def _fused_kernels(self, uin, fout, xa, stage):
         eles = list(self.ele_map.values())
         ks = [e.kernels['disu'](uin) for e in eles]
-        ks += [i.kernels['con_u']() for i in self._int_inters]
+        ks += [i.kernels['con_u']() for i in self._inters]
         for e in eles:
             ks += [e.kernels['tgradf'](uin), e.kernels['tdisf_fused'](uin),
                    e.kernels['gradcoru_fpts']()]
-        ks += [i.kernels['comm_flux']() for i in self._int_inters]
+        ks += [i.kernels['comm_flux']() for i in self._inters]
         for e in eles:
             ks += [e.kernels['tdivf'](fout, xa, stage)]
         return ks
@@ -810,13 +967,13 @@ class NavierStokesSystem(BaseSystem):
         g = be.graph()
         eles = list(self.ele_map.values())
         ks = [e.kernels['disu'](uin) for e in eles]
-        ks += [i.kernels['con_u']() for i in self._int_inters]
+        ks += [i.kernels['con_u']() for i in self._inters]
         for e in eles:
             ks += [e.kernels['tgradpcoru_upts'](uin),
                    e.kernels['tgradcoru_upts'](),
                    e.kernels['tdisf_fused'](uin),
                    e.kernels['gradcoru_fpts']()]
-        ks += [i.kernels['comm_flux']() for i in self._int_inters]
+        ks += [i.kernels['comm_flux']() for i in self._inters]
         for e in eles:
             ks += [e.kernels['tdivtpcorf'](fout), e.kernels['tdivtconf'](fout),
                    e.kernels['negdivconf'](fout)]

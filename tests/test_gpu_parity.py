@@ -87,6 +87,8 @@ def _systems(be, system, n, order, rsolver='rusanov', scheme='rk4'):
     ('navier-stokes', 4, 2, 'hllc'),
     ('navier-stokes', 3, 1, 'rusanov'),
     ('euler', 4, 2, 'hll'),
+    ('euler', 3, 3, 'hll'),
+    ('navier-stokes', 3, 3, 'hll'),
     ('euler', 4, 2, 'roe'),
     ('euler', 4, 3, 'roem'),
     ('navier-stokes', 3, 2, 'roe'),
