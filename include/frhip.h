@@ -33,7 +33,7 @@ enum { FRHIP_F64 = 0, FRHIP_F32 = 1 };
 enum { FRHIP_F32_BF16OPS = 2 };
 /* Riemann solvers (pyfr/solvers/euler/kernels/rsolvers/<name>.mako). */
 enum { FRHIP_RS_RUSANOV = 0, FRHIP_RS_HLLC = 1, FRHIP_RS_HLL = 2,
-       FRHIP_RS_ROE = 3, FRHIP_RS_ROEM = 4 };
+       FRHIP_RS_ROE = 3, FRHIP_RS_ROEM = 4, FRHIP_RS_EXACT = 5 };
 
 /* ---- runtime: replaces the ctypes HIP driver wrapper
  *      pyfr/backends/hip/driver.py:402-482 (hipSetDevice, hipMalloc, hipFree,

@@ -263,8 +263,124 @@ def roem(ul, ur, n, gamma):
     return nf
 
 
+def _star_flux(p, ps, rs, cs, g):
+    """Pressure function of one side and its derivative (Toro 2009 eqs.
+    4.86-4.87, zero covolume); ``rsolvers/exact.mako:14-30``."""
+    gprtg, trgm = (g + 1)/(2*g), 2/(g - 1)
+    trgp, gmrgp = 2/(g + 1), (g - 1)/(g + 1)
+    pr = p/ps
+    pr_g = np.power(pr, -gprtg)
+    f_raref, fd_raref = trgm*cs*(pr_g*pr - 1), pr_g/(rs*cs)
+    pbs_inv = 1/(p + gmrgp*ps)
+    sapb = np.sqrt(trgp*pbs_inv/rs)
+    f_shock = (p - ps)*sapb
+    fd_shock = sapb - 0.5*f_shock*pbs_inv
+    raref = p <= ps
+    return (np.where(raref, f_raref, f_shock),
+            np.where(raref, fd_raref, fd_shock))
+
+
+def _exact_1d(ul, ur, g, p_min, kmax=3):
+    """``rsolvers/exact.mako:49-206`` in the face frame: initial guess
+    (primitive-variable / two-rarefaction / two-shock), kmax Newton steps
+    on the star pressure (negative iterates replaced by p_min), then the
+    sampled state at x/t = 0 and its 1-D flux."""
+    hgm, grgm, trgm = 0.5*(g - 1), g/(g - 1), 2/(g - 1)
+    trgp, gmrtg, gprtg = 2/(g + 1), (g - 1)/(2*g), (g + 1)/(2*g)
+    tgrgm, gmrgp = (2*g)/(g - 1), (g - 1)/(g + 1)
+
+    def prim(s):
+        invrho = 1/s[0]
+        v = invrho*s[1:4]
+        p = (g - 1)*(s[4] - 0.5*invrho*(s[1]*s[1] + s[2]*s[2] + s[3]*s[3]))
+        return p, v, np.sqrt(g*p*invrho)
+
+    pl, vl, cl = prim(ul)
+    pr, vr, cr = prim(ur)
+    rl, rr = ul[0], ur[0]
+    bpv = np.maximum(0, 0.125*(vl[0] - vr[0])*(rl + rr)*(cl + cr) +
+                     0.5*(pl + pr))
+    pmin, pmax = np.minimum(pl, pr), np.maximum(pl, pr)
+    pre = np.power(pl/pr, gmrtg)
+    um = ((pre*vl[0]*cr + vr[0]*cl + trgm*(pre - 1)*cl*cr) /
+          (pre*cr + cl))
+    ptl = 1 - hgm*(um - vl[0])/cl
+    ptr = 1 + hgm*(um - vr[0])/cr
+    p_rr = 0.5*(pl*np.power(ptl, tgrgm) + pr*np.power(ptr, tgrgm))
+    gl = np.sqrt(trgp/(rl*(gmrgp*pl + bpv)))
+    gr = np.sqrt(trgp/(rr*(gmrgp*pr + bpv)))
+    p_ss = (gl*pl + gr*pr - vr[0] + vl[0])/(gl + gr)
+    p0 = np.where((pmax <= 2*pmin) & (pmin <= bpv) & (bpv <= pmax), bpv,
+                  np.where(bpv < pmin, p_rr, p_ss))
+    for _ in range(kmax):
+        fsl, fdl = _star_flux(p0, pl, rl, cl, g)
+        fsr, fdr = _star_flux(p0, pr, rr, cr, g)
+        p1 = p0 - (fsl + fsr + vr[0] - vl[0])/(fdl + fdr)
+        p0 = np.where(p1 < 0, p_min, p1)
+    # u* from the last iteration's side functions, evaluated at the
+    # pressure before the final update (exact.mako:91-97)
+    us = 0.5*(vl[0] + vr[0] + fsr - fsl)
+
+    # sampled primitive state (rho, u_n, p) by the decision tree
+    def st(r, u, p):
+        return np.array([r, u, p])
+    L, R = st(rl, vl[0], pl), st(rr, vr[0], pr)
+    # left of the contact
+    cl_fan = trgp + gmrgp*vl[0]/cl
+    left_raref = np.where(
+        vl[0] >= cl, L,
+        np.where(us < cl*np.power(p0/pl, gmrtg),
+                 st(rl*np.power(p0/pl, 1/g), us, p0),
+                 st(rl*np.power(cl_fan, trgm), trgp*cl + gmrgp*vl[0],
+                    pl*np.power(cl_fan, tgrgm))))
+    p0pl = p0/pl
+    sl = vl[0] - cl*np.sqrt(gprtg*p0pl + gmrtg)
+    left_shock = np.where(0 <= sl, L,
+                          st(rl*(p0pl + gmrgp)/(p0pl*gmrgp + 1), us, p0))
+    wl = np.where(p0 <= pl, left_raref, left_shock)
+    # right of the contact
+    p0pr = p0/pr
+    sr = vr[0] + cr*np.sqrt(gprtg*p0pr + gmrtg)
+    right_shock = np.where(sr <= 0, R,
+                           st(rr*(p0pr + gmrgp)/(p0pr*gmrgp + 1), us, p0))
+    cr_fan = trgp - gmrgp*vr[0]/cr
+    right_raref = np.where(
+        vr[0] + cr <= 0, R,
+        np.where(us + cr*np.power(p0pr, gmrtg) >= 0,
+                 st(rr*np.power(p0pr, 1/g), us, p0),
+                 st(rr*np.power(cr_fan, trgm), gmrgp*vr[0] - trgp*cr,
+                    pr*np.power(cr_fan, tgrgm))))
+    wr = np.where(p0 > pr, right_shock, right_raref)
+    left = 0 <= us
+    w = np.where(left, wl, wr)
+    vt = [np.where(left, vl[i], vr[i]) for i in (1, 2)]
+    nf = np.empty_like(ul)
+    nf[0] = w[0]*w[1]
+    nf[1] = nf[0]*w[1] + w[2]
+    nf[2] = nf[0]*vt[0]
+    nf[3] = nf[0]*vt[1]
+    nf[4] = (grgm*w[2] + 0.5*w[0]*(w[1]*w[1] + vt[0]*vt[0] +
+                                   vt[1]*vt[1]))*w[1]
+    return nf
+
+
+def exact(ul, ur, n, gamma, p_min=5*np.finfo(np.float64).eps):
+    """Exact Riemann solver (Toro 2009 ch. 4) in the face frame;
+    ``rsolvers/exact.mako`` through ``rsolve1d.mako:1-21``.  p_min: the
+    reference's ``[solver-interfaces] p-min`` default, 5 x machine epsilon
+    of the working precision (``pyfr/solvers/euler/inters.py:12-26``)."""
+    with np.errstate(divide='ignore', invalid='ignore', over='ignore'):
+        utl = np.stack([ul[0], *_transform_to(n, ul), ul[4]])
+        utr = np.stack([ur[0], *_transform_to(n, ur), ur[4]])
+        ntf = _exact_1d(utl, utr, gamma, p_min)
+        nf = np.empty_like(ntf)
+        nf[0], nf[4] = ntf[0], ntf[4]
+        nf[1:4] = _transform_from(n, ntf[1:4])
+    return nf
+
+
 RSOLVERS = {'rusanov': rusanov, 'hllc': hllc, 'hll': hll, 'roe': roe,
-            'roem': roem}
+            'roem': roem, 'exact': exact}
 
 
 def viscous_flux_add(u, g, f, c):

@@ -743,11 +743,18 @@ class PointwiseProvider:
 
     # -- interfaces ----------------------------------------------------------
     @staticmethod
-    def _rs(tplargs):
+    def _rs(tplargs, dtype):
         rs = tplargs['rsolver']
-        ids = {'rusanov': 0, 'hllc': 1, 'hll': 2, 'roe': 3, 'roem': 4}
+        ids = {'rusanov': 0, 'hllc': 1, 'hll': 2, 'roe': 3, 'roem': 4,
+               'exact': 5}
         if rs not in ids:
             raise NotSuitableError(f'Riemann solver {rs} not available')
+        # the exact solver's pressure floor is compiled in as the
+        # reference's default, 5 x machine epsilon
+        pm = tplargs.get('p_min')
+        if rs == 'exact' and pm is not None and \
+                pm != 5*float(np.finfo(dtype).eps):
+            raise NotSuitableError('only the default p-min is supported')
         return ids[rs]
 
     def intcflux(self, tplargs, dims, ul, ur, nl, gradul=None, gradur=None,
@@ -757,7 +764,7 @@ class PointwiseProvider:
         if ul.basedata != ur.basedata:
             raise ValueError('Interface views must share storage')
         dt = _dtcode(ul.refdtype)
-        rs = self._rs(tplargs)
+        rs = self._rs(tplargs, ul.refdtype)
         c = tplargs['c']
         if gradul is None:
             args = (dt, rs, n, ul.basedata, ul.mapping.data, ur.mapping.data,
@@ -817,7 +824,7 @@ class PointwiseProvider:
         lib = self.backend.lib
         (n,) = dims
         dt = _dtcode(ul.refdtype)
-        rs = self._rs(tplargs)
+        rs = self._rs(tplargs, ul.refdtype)
         euler = gradul is None and 'visc_corr' not in tplargs
         bct, consts = self._bc(tplargs, euler)
         bv = None if bcvals is None else bcvals.data

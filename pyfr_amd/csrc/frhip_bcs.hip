@@ -319,7 +319,7 @@ int frhip_bccflux_euler(int dtype, int bctype, int rsolver, int n,
     auto go = [&](auto tag) {
         using T = decltype(tag);
         const BCConsts<T> k = bc_consts<T>(consts);
-        dispatch<0, FRHIP_RS_ROEM + 1>(rsolver, [&](auto rs) {
+        dispatch<0, FRHIP_RS_EXACT + 1>(rsolver, [&](auto rs) {
             dispatch<0, FRHIP_BC_SLP_ADIA_WALL + 1>(bctype, [&](auto bc) {
                 hipLaunchKernelGGL(
                     (k_bccflux_euler<T, decltype(rs)::value,
@@ -386,7 +386,7 @@ int frhip_bccflux_ns(int dtype, int bctype, int rsolver, int n, void *base,
     auto go = [&](auto tag) {
         using T = decltype(tag);
         const BCConsts<T> k = bc_consts<T>(consts);
-        dispatch<0, FRHIP_RS_ROEM + 1>(rsolver, [&](auto rs) {
+        dispatch<0, FRHIP_RS_EXACT + 1>(rsolver, [&](auto rs) {
             dispatch<0, FRHIP_BC_SUB_OUT_FP + 1>(bctype, [&](auto bc) {
                 hipLaunchKernelGGL(
                     (k_bccflux_ns<T, decltype(rs)::value,

@@ -4,6 +4,8 @@
 
 #include "frhip_common.h"
 
+#include <limits>
+
 namespace frhip {
 
 // Inviscid Euler flux; pyfr/solvers/euler/kernels/flux.mako:3-26.
@@ -298,6 +300,149 @@ __device__ __forceinline__ void rsolve_roem(const T ul[5], const T ur[5],
                 n[2]*(b1*fl[2][i] - b2*fr[2][i]) + b1b2*(du[i] - g*bdq[i]);
 }
 
+// Exact Riemann solver (Toro 2009 ch. 4, zero covolume) in the face frame;
+// rsolvers/exact.mako:14-206 through rsolve1d.mako:1-21.  The gamma ratios
+// are formed in double, as the reference forms its template literals; p_min
+// is the reference's default 5 x machine epsilon of T
+// (pyfr/solvers/euler/inters.py:12-26).
+template <typename T>
+__device__ __forceinline__ void exact_star_flux(T p, T ps, T rs, T cs, T &f,
+                                                T &fd, T gprtg, T trgm,
+                                                T trgp, T gmrgp)
+{
+    if (p <= ps) {
+        const T pr = p/ps;
+        const T pr_g = pow(pr, -gprtg);
+        fd = pr_g/(rs*cs);
+        f = trgm*cs*(pr_g*pr - 1);
+    }
+    else {
+        const T pbs_inv = T(1)/(p + gmrgp*ps);
+        const T sapb = sqrt(trgp*pbs_inv/rs);
+        f = (p - ps)*sapb;
+        fd = sapb - T(0.5)*f*pbs_inv;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void rsolve_exact(const T ul[5], const T ur[5],
+                                             const T n[3], T nf[5], T gm1,
+                                             T gamma)
+{
+    const double g = (double)gamma, gm = g - 1.0;
+    const T hgm = T(0.5*gm), grgm = T(g/gm), trgm = T(2/gm);
+    const T trgp = T(2/(g + 1)), gmrtg = T(gm/(2*g));
+    const T gprtg = T((g + 1)/(2*g)), tgrgm = T((2*g)/gm);
+    const T gmrgp = T(gm/(g + 1)), rgamma = T(1/g);
+    const T p_min = T(5)*std::numeric_limits<T>::epsilon();
+
+    T tl[5], tr[5];
+    tl[0] = ul[0]; tr[0] = ur[0]; tl[4] = ul[4]; tr[4] = ur[4];
+    transform_to(n, ul + 1, tl + 1);
+    transform_to(n, ur + 1, tr + 1);
+
+    // primitives (exact.mako:32-45)
+    T vl[3], vr[3];
+    const T il = T(1)/tl[0], ir = T(1)/tr[0];
+    for (int i = 0; i < 3; ++i) { vl[i] = il*tl[i + 1]; vr[i] = ir*tr[i + 1]; }
+    const T pl = gm1*(tl[4] - T(0.5)*il*(tl[1]*tl[1] + tl[2]*tl[2] +
+                                         tl[3]*tl[3]));
+    const T pr = gm1*(tr[4] - T(0.5)*ir*(tr[1]*tr[1] + tr[2]*tr[2] +
+                                         tr[3]*tr[3]));
+    const T cl = sqrt(gamma*pl*il), cr = sqrt(gamma*pr*ir);
+    const T rl = tl[0], rr = tr[0];
+
+    // initial guess
+    const T bpv = fmax(T(0), T(0.125)*(vl[0] - vr[0])*(rl + rr)*(cl + cr) +
+                             T(0.5)*(pl + pr));
+    const T pmin = fmin(pl, pr), pmax = fmax(pl, pr);
+    T p0;
+    if (pmax <= 2*pmin && pmin <= bpv && bpv <= pmax)
+        p0 = bpv;
+    else if (bpv < pmin) {
+        const T pre = pow(pl/pr, gmrtg);
+        const T um = (pre*vl[0]*cr + vr[0]*cl + trgm*(pre - 1)*cl*cr) /
+                     (pre*cr + cl);
+        const T ptl = 1 - hgm*(um - vl[0])/cl;
+        const T ptr = 1 + hgm*(um - vr[0])/cr;
+        p0 = T(0.5)*(pl*pow(ptl, tgrgm) + pr*pow(ptr, tgrgm));
+    }
+    else {
+        const T gl = sqrt(trgp/(rl*(gmrgp*pl + bpv)));
+        const T gr = sqrt(trgp/(rr*(gmrgp*pr + bpv)));
+        p0 = (gl*pl + gr*pr - vr[0] + vl[0])/(gl + gr);
+    }
+
+    // three Newton steps on the star pressure
+    T fsl, fdl, fsr, fdr;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        exact_star_flux(p0, pl, rl, cl, fsl, fdl, gprtg, trgm, trgp, gmrgp);
+        exact_star_flux(p0, pr, rr, cr, fsr, fdr, gprtg, trgm, trgp, gmrgp);
+        const T p1 = p0 - (fsl + fsr + vr[0] - vl[0])/(fdl + fdr);
+        p0 = (p1 < 0) ? p_min : p1;
+    }
+    const T us = T(0.5)*(vl[0] + vr[0] + fsr - fsl);
+
+    // sample x/t = 0: w = (rho, u_n, p) and the tangential velocity
+    T w0, w1, w4, wt1, wt2;
+    if (0 <= us) {
+        wt1 = vl[1]; wt2 = vl[2];
+        if (p0 <= pl) {
+            if (vl[0] >= cl) { w0 = rl; w1 = vl[0]; w4 = pl; }
+            else if (us < cl*pow(p0/pl, gmrtg)) {
+                w0 = rl*pow(p0/pl, rgamma); w1 = us; w4 = p0;
+            }
+            else {
+                const T c = trgp + gmrgp*vl[0]/cl;
+                w0 = rl*pow(c, trgm); w1 = trgp*cl + gmrgp*vl[0];
+                w4 = pl*pow(c, tgrgm);
+            }
+        }
+        else {
+            const T p0p = p0/pl;
+            const T sl = vl[0] - cl*sqrt(gprtg*p0p + gmrtg);
+            if (0 <= sl) { w0 = rl; w1 = vl[0]; w4 = pl; }
+            else {
+                w0 = rl*(p0p + gmrgp)/(p0p*gmrgp + 1); w1 = us; w4 = p0;
+            }
+        }
+    }
+    else {
+        wt1 = vr[1]; wt2 = vr[2];
+        if (p0 > pr) {
+            const T p0p = p0/pr;
+            const T sr = vr[0] + cr*sqrt(gprtg*p0p + gmrtg);
+            if (sr <= 0) { w0 = rr; w1 = vr[0]; w4 = pr; }
+            else {
+                w0 = rr*(p0p + gmrgp)/(p0p*gmrgp + 1); w1 = us; w4 = p0;
+            }
+        }
+        else if (vr[0] + cr <= 0) { w0 = rr; w1 = vr[0]; w4 = pr; }
+        else {
+            const T p0p = p0/pr;
+            if (us + cr*pow(p0p, gmrtg) >= 0) {
+                w0 = rr*pow(p0p, rgamma); w1 = us; w4 = p0;
+            }
+            else {
+                const T c = trgp - gmrgp*vr[0]/cr;
+                w0 = rr*pow(c, trgm); w1 = gmrgp*vr[0] - trgp*cr;
+                w4 = pr*pow(c, tgrgm);
+            }
+        }
+    }
+
+    T nt[5];
+    nt[0] = w0*w1;
+    nt[1] = nt[0]*w1 + w4;
+    nt[2] = nt[0]*wt1;
+    nt[3] = nt[0]*wt2;
+    nt[4] = (grgm*w4 + T(0.5)*w0*(w1*w1 + wt1*wt1 + wt2*wt2))*w1;
+    nf[0] = nt[0];
+    nf[4] = nt[4];
+    transform_from(n, nt + 1, nf + 1);
+}
+
 template <typename T, int RS>
 __device__ __forceinline__ void rsolve(const T ul[5], const T ur[5],
                                        const T n[3], T nf[5], T gm1, T gamma,
@@ -307,6 +452,8 @@ __device__ __forceinline__ void rsolve(const T ul[5], const T ur[5],
     else if constexpr (RS == FRHIP_RS_HLL) rsolve_hll(ul, ur, n, nf, gm1, gamma);
     else if constexpr (RS == FRHIP_RS_ROE) rsolve_roe(ul, ur, n, nf, gm1);
     else if constexpr (RS == FRHIP_RS_ROEM) rsolve_roem(ul, ur, n, nf, gm1);
+    else if constexpr (RS == FRHIP_RS_EXACT)
+        rsolve_exact(ul, ur, n, nf, gm1, gamma);
     else rsolve_rusanov(ul, ur, n, nf, gm1, qgamma);
 }
 
@@ -399,7 +546,7 @@ inline NSConsts<T> ns_consts(const double *c)
     FR_CHECK((dt) == FRHIP_F64 || (dt) == FRHIP_F32, "invalid dtype")
 
 #define FR_RS_OK(rs)                                                        \
-    FR_CHECK((rs) >= FRHIP_RS_RUSANOV && (rs) <= FRHIP_RS_ROEM,             \
+    FR_CHECK((rs) >= FRHIP_RS_RUSANOV && (rs) <= FRHIP_RS_EXACT,            \
              "invalid Riemann solver")
 
 // Expand M(RS) for the run-time solver id (one instantiation per solver)
@@ -409,5 +556,6 @@ inline NSConsts<T> ns_consts(const double *c)
         case FRHIP_RS_HLLC: M(FRHIP_RS_HLLC); break;                        \
         case FRHIP_RS_HLL: M(FRHIP_RS_HLL); break;                          \
         case FRHIP_RS_ROE: M(FRHIP_RS_ROE); break;                          \
-        default: M(FRHIP_RS_ROEM); break;                                   \
+        case FRHIP_RS_ROEM: M(FRHIP_RS_ROEM); break;                        \
+        default: M(FRHIP_RS_EXACT); break;                                  \
     }

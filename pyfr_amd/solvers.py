@@ -470,6 +470,14 @@ class NavierStokesElements(EulerElements):
 # Interfaces
 # ---------------------------------------------------------------------------
 
+def _p_min(be, cfg):
+    """The exact solver's pressure floor: ``[solver-interfaces] p-min``,
+    default 5 x machine epsilon of the working precision
+    (``pyfr/solvers/euler/inters.py:12-26``)."""
+    return cfg.getfloat('solver-interfaces', 'p-min',
+                        5*float(np.finfo(be.fpdtype).eps))
+
+
 def _get_inter_arrays(interside, meth, elemap, perm=Ellipsis):
     """``pyfr/solvers/base/inters.py:6-17``."""
     parts, reorder = [], []
@@ -505,7 +513,8 @@ class EulerIntInters:
 
         rsolver = cfg.get('solver-interfaces', 'riemann-solver')
         self._tplargs = dict(ndims=self.ndims, nvars=self.nvars,
-                             rsolver=rsolver, c=self.c)
+                             rsolver=rsolver, c=self.c,
+                             p_min=_p_min(be, cfg))
         be.pointwise.register('pyfr.solvers.euler.kernels.intcflux')
         self.kernels['comm_flux'] = lambda: be.kernel(
             'intcflux', tplargs=self._tplargs, dims=[self.ninterfpts],
@@ -556,7 +565,7 @@ class NavierStokesIntInters(EulerIntInters):
         visc_corr = cfg.get('solver', 'viscosity-correction', 'none')
         self._tplargs = dict(ndims=self.ndims, nvars=self.nvars,
                              rsolver=self._tplargs['rsolver'], c=self.c,
-                             visc_corr=visc_corr)
+                             visc_corr=visc_corr, p_min=_p_min(be, cfg))
         self._vect_lhs = self._view(self.lhs, 'get_vect_fpts_for_inters',
                                     (self.ndims, self.nvars))
         self._vect_rhs = self._view(self.rhs, 'get_vect_fpts_for_inters',
@@ -627,7 +636,8 @@ class EulerBCInters(EulerIntInters):
         self._bcvals = self._eval_values()
         rsolver = cfg.get('solver-interfaces', 'riemann-solver')
         self._tplargs = dict(ndims=self.ndims, nvars=self.nvars,
-                             rsolver=rsolver, c=self.c, bctype=self.type)
+                             rsolver=rsolver, c=self.c, bctype=self.type,
+                             p_min=_p_min(be, cfg))
         self._kernels()
 
     def _kernels(self):

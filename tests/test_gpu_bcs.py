@@ -84,6 +84,8 @@ def _floor(orc, u, seed=0):
     ('ns-inflow', 3, 2, 'rusanov', 0.1),
     ('ns-inflow', 3, 3, 'hll', 0.0),
     ('ns-slip', 3, 2, 'rusanov', 0.0),
+    ('euler-walls', 3, 2, 'exact', 0.0),
+    ('ns-walls', 3, 2, 'exact', 0.1),
 ])
 def test_bc_rhs_parity(backend_factory, case, n, order, rsolver, tau, fused):
     be = backend_factory(fused=fused)

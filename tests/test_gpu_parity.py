@@ -89,6 +89,8 @@ def _systems(be, system, n, order, rsolver='rusanov', scheme='rk4'):
     ('euler', 4, 2, 'hll'),
     ('euler', 3, 3, 'hll'),
     ('navier-stokes', 3, 3, 'hll'),
+    ('euler', 4, 2, 'exact'),
+    ('navier-stokes', 3, 3, 'exact'),
     ('euler', 4, 2, 'roe'),
     ('euler', 4, 3, 'roem'),
     ('navier-stokes', 3, 2, 'roe'),

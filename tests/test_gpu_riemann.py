@@ -1,0 +1,89 @@
+"""Pointwise GPU parity of the interface Riemann solvers (FP64): the
+intcflux kernel on random state pairs, from smooth to strong jumps
+(density ratio up to 25, pressure ratio up to 1000, Mach up to 3, any face
+orientation; pairs whose exact problem has a vacuum are excluded), against
+the oracle's solvers.  Tolerance 1e-11 of the
+largest flux magnitude per solver and state set: the fluxes are continuous
+in the states, so branch choices made differently by the last ulp cannot
+move them further."""
+import ctypes as ct
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SOLVERS = {'rusanov': 0, 'hllc': 1, 'hll': 2, 'roe': 3, 'roem': 4,
+           'exact': 5}
+
+
+def _states(rng, n, jump):
+    rho = rng.uniform(1.0, 1.0 + jump, n)*np.where(rng.random(n) < 0.5,
+                                                   1, 0.2)
+    p = np.exp(rng.uniform(0, np.log(1 + 40*jump), n))
+    c = np.sqrt(1.4*p/rho)
+    v = rng.normal(0, 1, (3, n))*c*rng.uniform(0, 3*min(jump, 1), n)
+    return np.stack([rho, *(rho*v), p/0.4 + 0.5*rho*(v**2).sum(0)])
+
+
+def _prim(u, n):
+    v = u[1:4]/u[0]
+    p = 0.4*(u[4] - 0.5*u[0]*(v*v).sum(0))
+    return (v*n).sum(0), np.sqrt(1.4*p/u[0])
+
+
+def _pairs(rng, N, jump):
+    """N state pairs and face normals, keeping only pairs that satisfy the
+    pressure-positivity condition 2/(gamma - 1) (cl + cr) > un_r - un_l
+    with a 20 % margin (Toro 2009 eq. 4.40): the exact problem of the other
+    pairs has a vacuum, where the exact solver has no answer."""
+    ul, ur = _states(rng, 2*N, jump), _states(rng, 2*N, jump)
+    nl = rng.normal(size=(3, 2*N))*rng.uniform(0.5, 2.0, 2*N)
+    n = nl/np.sqrt((nl**2).sum(0))
+    unl, cl = _prim(ul, n)
+    unr, cr = _prim(ur, n)
+    keep = np.flatnonzero(5.0*(cl + cr) > 1.2*(unr - unl))[:N]
+    assert len(keep) == N
+    return ul[:, keep], ur[:, keep], nl[:, keep]
+
+
+@pytest.mark.parametrize('name', list(SOLVERS))
+@pytest.mark.parametrize('jump', [0.05, 1.0, 25.0])
+def test_intcflux_pointwise(backend_factory, name, jump):
+    import torch
+    from oracle.physics import RSOLVERS
+    be = backend_factory()
+    rng = np.random.default_rng(int(jump*100) + SOLVERS[name])
+    N = 4096
+    ul, ur, nl = _pairs(rng, N, jump)
+    nb = N//64
+
+    def aosoa(x):
+        a = np.zeros((nb, 5, 64))
+        for v in range(5):
+            a[:, v] = x[v].reshape(nb, 64)
+        return a.ravel()
+    k = np.arange(N)
+    soa = (k//64)*320 + k % 64
+    base = torch.tensor(np.concatenate([aosoa(ul), aosoa(ur)]),
+                        device='cuda')
+    lm = torch.tensor(soa.astype(np.int32), device='cuda')
+    rm = torch.tensor((soa + nb*320).astype(np.int32), device='cuda')
+    nt = torch.tensor(np.ascontiguousarray(nl), device='cuda')
+    print(f'\n[riemann] {name} jump={jump}', flush=True)
+    be.lib.intcflux_euler(0, SOLVERS[name], N, base.data_ptr(),
+                          lm.data_ptr(), rm.data_ptr(), nt.data_ptr(), N,
+                          1.4, None)
+    torch.cuda.synchronize()
+    out = base.cpu().numpy()
+    got_l = np.stack([out[soa + 64*v] for v in range(5)])
+    got_r = np.stack([out[nb*320 + soa + 64*v] for v in range(5)])
+    mag = np.sqrt((nl**2).sum(0))
+    with np.errstate(all='ignore'):
+        ref = mag*RSOLVERS[name](ul, ur, nl/mag, 1.4)
+    assert np.all(np.isfinite(ref))
+    scale = np.abs(ref).max()
+    err = np.abs(got_l - ref).max()/scale
+    print(f'[riemann] max err / max flux {err:.2e}', flush=True)
+    assert err < 1e-11
+    assert np.array_equal(got_r, -got_l)

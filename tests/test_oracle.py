@@ -267,7 +267,8 @@ def _random_states(rng, n, mach=0.5):
     return np.stack([rho, *(rho*v), p/0.4 + 0.5*rho*(v**2).sum(0)])
 
 
-@pytest.mark.parametrize('name', ['rusanov', 'hllc', 'hll', 'roe', 'roem'])
+@pytest.mark.parametrize('name', ['rusanov', 'hllc', 'hll', 'roe', 'roem',
+                                  'exact'])
 def test_riemann_solver_properties(name):
     """Consistency F(u, u, n) = f(u).n for every solver; conservation
     F(ul, ur, n) = -F(ur, ul, -n) where the scheme has it (the reference's
@@ -313,3 +314,41 @@ def test_riemann_solver_properties(name):
             1e-10*np.abs(fn).max()
     elif name != 'rusanov':
         assert np.abs(got - fn).max() < 1e-10*np.abs(fn).max()
+
+
+@pytest.mark.parametrize('rho_l,u_l,p_l,rho_r,u_r,p_r,w', [
+    # Toro (2009) Table 4.3, tests 1 (Sod), 3 and 5; w = (rho, u, p) of the
+    # exact solution at x/t = 0: the left star state (rho*L, u*, p*) for
+    # tests 1 and 3; for test 5 the left shock moves right (S_L = +0.79),
+    # so x/t = 0 sees the unperturbed left state
+    (1.0, 0.0, 1.0, 0.125, 0.0, 0.1, (0.42632, 0.92745, 0.30313)),
+    (1.0, 0.0, 1000.0, 1.0, 0.0, 0.01, (0.57506, 19.5975, 460.894)),
+    (5.99924, 19.5975, 460.894, 5.99242, -6.19633, 46.0950,
+     (5.99924, 19.5975, 460.894)),
+])
+def test_exact_solver_star_states(rho_l, u_l, p_l, rho_r, u_r, p_r, w):
+    """Known answers: the exact solver's interface flux equals the flux of
+    the state Toro's exact solution has at x/t = 0 (to the table's 5-6
+    digits); the face may have any orientation, and a tangential velocity
+    is carried through."""
+    from oracle.physics import exact
+    g = 1.4
+    rs, us, ps = w
+    want = np.array([rs*us, rs*us*us + ps, 0, 0,
+                     (g/(g - 1)*ps + 0.5*rs*us*us)*us])
+    rng = np.random.default_rng(5)
+    n = rng.normal(size=3)
+    n /= np.linalg.norm(n)
+    t = np.cross(n, rng.normal(size=3))
+    t /= np.linalg.norm(t)
+    vt = 0.3*t
+
+    def con(r, u, p):
+        v = u*n + vt
+        return np.array([r, *(r*v), p/(g - 1) + 0.5*r*(v @ v)])[:, None]
+    f = exact(con(rho_l, u_l, p_l), con(rho_r, u_r, p_r), n[:, None], g)[:, 0]
+    # back to the face frame: normal and tangential momentum flux
+    got = np.array([f[0], f[1:4] @ n, 0, 0, f[4] - 0.5*(vt @ vt)*f[0]])
+    assert np.allclose((f[1:4] - (f[1:4] @ n)*n), f[0]*vt, rtol=1e-10,
+                       atol=1e-10*abs(f[0]))
+    assert np.allclose(got, want, rtol=2e-5, atol=2e-5*np.abs(want).max())
