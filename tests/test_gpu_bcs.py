@@ -37,7 +37,7 @@ CASES = {
 SCALARS = ('cpTw', 'cpTt', 'pt', 'theta', 'phi')
 
 
-def _setup(be, case, n, order, rsolver, tau=0.0, scheme='rk4'):
+def _setup(be, case, n, order, rsolver, tau=0.0, scheme='rk4', mesh=None):
     from oracle.rhs import OracleHexSystem
     from pyfr_amd.mesh import BoxMesh
     from pyfr_amd.solvers import get_integrator
@@ -61,7 +61,7 @@ def _setup(be, case, n, order, rsolver, tau=0.0, scheme='rk4'):
             for k in 'uvw':
                 vals.setdefault(k, 0.0)
         obcs[side] = (bctype, vals)
-    mesh = BoxMesh(n, periodic=periodic)
+    mesh = mesh or BoxMesh(n, periodic=periodic)
     integ = get_integrator(be, mesh, cfg)
     orc = OracleHexSystem(n, order, system, rsolver, constants=oc,
                           periodic=periodic, bcs=obcs)
@@ -130,3 +130,30 @@ def test_bc_declines_time_dependent_values(backend_factory):
     cfg.set('soln-bcs-x0', 'u', '0.3 + 0.1*t')
     with pytest.raises(NotSuitableError):
         get_integrator(be, BoxMesh(3, periodic=(False, True, True)), cfg)
+
+
+@pytest.mark.parametrize('version', ['2.2', '4.1'])
+def test_bc_rhs_parity_gmsh_mesh(backend_factory, version):
+    """The same boundaries on a mesh read from a .msh file (periodic z as a
+    periodic_z_l/r surface pair, the x and y sides as named surfaces)."""
+    import io
+    from pyfr_amd.gmsh import GmshMesh
+    from pyfr_amd.mesh import BoxMesh
+    from tests.msh_files import box_sides, mesh_to_msh
+    n = 3
+    raw = BoxMesh(n, periodic=(False, False, False))
+    spec = {'x0': 'x0', 'x1': 'x1', 'y0': 'y0', 'y1': 'y1',
+            'z0': 'periodic_z_l', 'z1': 'periodic_z_r'}
+    mesh = GmshMesh(io.StringIO(mesh_to_msh(raw, box_sides(raw, spec),
+                                            version)))
+    be = backend_factory()
+    print(f'\n[bc rhs gmsh] {version}', flush=True)
+    integ, orc = _setup(be, 'ns-walls', n, 3, 'rusanov', 0.1, mesh=mesh)
+    u0 = orc.ics()
+    integ.system.rhs(0.0, 0, 1)
+    got = integ.system.ele_scal_upts(1)[0]
+    ref = orc.rhs(u0)
+    tol = max(1e-12, 10*_floor(orc, u0))
+    err = rel_l2(got, ref)
+    print(f'[bc rhs gmsh] rel L2 {err:.3e} (tol {tol:.3e})', flush=True)
+    assert err < tol
