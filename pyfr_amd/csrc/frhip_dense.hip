@@ -32,6 +32,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <utility>
 #include <vector>
 
 namespace frhip {
@@ -567,6 +568,162 @@ __global__ __launch_bounds__(256) void k_dense_fma(
     }
 }
 
+// 4x4x1 (16-block) MFMA form, FP32: one lane per element column.
+//
+// v_mfma_f32_4x4x1f32 computes 16 independent 4x4 outer products; block b
+// takes its B value from lane 4b + j and, with the A-broadcast controls
+// CBSZ = 4 and ABID = k, every block takes A from lanes 4k..4k+3 (lane
+// 4b + j, result register i = A_k[i] * B_b[j]; checked on the device).
+// Each lane's B value is its own column's datum, so lane 4b + j owns
+// column 4b + j and register i of accumulator g holds output row 4g + i:
+// the m-row operator is padded to 4 rows only (36 for the 35-row tet
+// operator, against 48 with 16-row tiles).  One operator VGPR holds 16
+// (k, 4-row group) entries, selected by ABID: fragment (chunk q, group g)
+// has lane 4k + i = A[4g + i][k-th column of chunk q].  Flux chunks cover
+// 5 solution points x 3 directions (15 k), evaluated per lane for its own
+// element; fn chunks cover 16 rows.
+typedef float FrV4f __attribute__((ext_vector_type(4)));
+
+template <int G, int KK>
+__device__ __forceinline__ void mfma4x4_k(FrV4f (&acc)[G], const float (&a)[G],
+                                          float b)
+{
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+        acc[g] = __builtin_amdgcn_mfma_f32_4x4x1f32(a[g], b, acc[g], 4, KK,
+                                                    0);
+}
+
+template <int G, int N, int... KK>
+__device__ __forceinline__ void mfma4x4_chunk(
+    FrV4f (&acc)[G], const float (&a)[G], const float (&b)[N],
+    std::integer_sequence<int, KK...>)
+{
+    (mfma4x4_k<G, KK>(acc, a, b[KK]), ...);
+}
+
+template <int G, int EP>
+__global__ __launch_bounds__(256) void k_tdivf_flux4(
+    int m, int nupts, int nc0, int nc1, int k1, const float *__restrict__ af,
+    long ncol, const float *__restrict__ u, long ldu,
+    const float *__restrict__ sm, long lds, const float *__restrict__ b1,
+    long ld1, float *__restrict__ c, long ldc, const float *__restrict__ rcp,
+    long ldr, void *__restrict__ xa, double dta, double dtb, float gm1)
+{
+    const int lane = threadIdx.x & 63;
+    const long w = (long)blockIdx.x*(blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (w*64 >= ncol) return;
+    const long c0 = w*64;
+    const int v = (int)((c0 >> 6) % 5);
+    const long eb = c0/320, ecol = eb*64 + lane;
+
+    float S[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) S[i][k] = sm[(3*i + k)*lds + ecol];
+
+    FrV4f acc[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) acc[g] = FrV4f{0, 0, 0, 0};
+
+    const float *ub = u + eb*320 + lane;
+    const float *afl = af + lane;
+    // u of chunk q + 1 is requested while chunk q computes; points past
+    // the end re-read the last point, whose operator columns are zero
+    float un[5][5];
+#pragma unroll
+    for (int pp = 0; pp < 5; ++pp)
+#pragma unroll
+        for (int x = 0; x < 5; ++x)
+            un[pp][x] = ub[min(pp, nupts - 1)*ldu + x*64];
+    // the operator fragments of chunk q + 1 likewise
+    float an[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) an[g] = afl[(long)g*64];
+    for (int q = 0; q < nc0; ++q) {
+        float a[G];
+        const long qn = min(q + 1, nc0 + nc1 - 1);
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            a[g] = an[g];
+            an[g] = afl[(qn*G + g)*64];
+        }
+        float uc[5][5];
+#pragma unroll
+        for (int pp = 0; pp < 5; ++pp)
+#pragma unroll
+            for (int x = 0; x < 5; ++x) {
+                uc[pp][x] = un[pp][x];
+                un[pp][x] = ub[min(5*(q + 1) + pp, nupts - 1)*ldu + x*64];
+            }
+        // this variable's transformed flux at the chunk's 5 points
+        // (flux.mako:3-26, tflux.mako:24-28)
+        float b[15];
+#pragma unroll
+        for (int pp = 0; pp < 5; ++pp) {
+            const float *uw = uc[pp];
+            const float invrho = 1.0f/uw[0], E = uw[4];
+            const float rv[3] = {uw[1], uw[2], uw[3]};
+            const float vel[3] = {invrho*rv[0], invrho*rv[1], invrho*rv[2]};
+            const float p = gm1*(E - 0.5f*invrho*((rv[0])*(rv[0]) +
+                                                  (rv[1])*(rv[1]) +
+                                                  (rv[2])*(rv[2])));
+            float fv[3];
+            if (v == 0) {
+                fv[0] = rv[0]; fv[1] = rv[1]; fv[2] = rv[2];
+            }
+            else if (v == 4) {
+                fv[0] = (E + p)*vel[0]; fv[1] = (E + p)*vel[1];
+                fv[2] = (E + p)*vel[2];
+            }
+            else {
+                const float vj = v == 1 ? vel[0] : v == 2 ? vel[1] : vel[2];
+                fv[0] = v == 1 ? rv[0]*vj + p : rv[0]*vj;
+                fv[1] = v == 2 ? rv[1]*vj + p : rv[1]*vj;
+                fv[2] = v == 3 ? rv[2]*vj + p : rv[2]*vj;
+            }
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+                b[3*pp + d] = S[d][0]*fv[0] + S[d][1]*fv[1] + S[d][2]*fv[2];
+        }
+        mfma4x4_chunk(acc, a, b, std::make_integer_sequence<int, 15>{});
+    }
+
+    // common normal fluxes at the flux points, 16 rows per chunk (rows past
+    // the end re-read the last row; their operator columns are zero)
+    const float *bl = b1 + c0 + lane;
+    for (int q = 0; q < nc1; ++q) {
+        float a[G], b[16];
+        const long qn = min(nc0 + q + 1, nc0 + nc1 - 1);
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            a[g] = an[g];
+            an[g] = afl[(qn*G + g)*64];
+        }
+#pragma unroll
+        for (int kk = 0; kk < 16; ++kk)
+            b[kk] = bl[min(16*q + kk, k1 - 1)*ld1];
+        mfma4x4_chunk(acc, a, b, std::make_integer_sequence<int, 16>{});
+    }
+
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int row = 4*g + i;
+            if (row >= m) continue;
+            const long off = row*ldc + c0 + lane;
+            if constexpr (EP == 0)
+                c[off] = acc[g][i];
+            else if constexpr (EP == 1)
+                c[off] = -rcp[row*ldr + ecol]*acc[g][i];
+            else
+                epi1<float, EP>(acc[g][i], c, off, rcp, row*ldr + ecol, xa,
+                                dta, dtb, 1.0f, 0.0f);
+        }
+}
+
 // ---- dispatch -------------------------------------------------------------
 
 // Variant codes: 0 scalar kernel; 1, 2, 4 MFMA with that many wavefronts
@@ -576,6 +733,8 @@ constexpr int FR_LDSV = 16;
 // flux-fused kernels only: the block-cooperative form (five wavefronts per
 // element block, flux evaluated once into LDS)
 constexpr int FR_BLKV = 32;
+// flux-fused kernels only, FP32: the 4x4x1 MFMA form (k_tdivf_flux4)
+constexpr int FR_V4 = 64;
 constexpr size_t FR_LDS_MAX = 65536;
 
 static bool variant_valid(int v, size_t ldsbytes, bool scalar_ok)
@@ -999,7 +1158,17 @@ int bf16_launch(const Bf16Plan *p, int ep, long ncol, const void *b0,
 struct FluxPlan {
     DensePlan d;        // m, k0 = 3 nupts, k1 = nfpts, fragments
     int nupts, nq;
+    void *af4;          // FP32: k_tdivf_flux4 fragments, or null
+    int g4, nc0, nc1;   // 4-row groups, flux and fn chunks
 };
+
+// Row-group counts with a k_tdivf_flux4 instantiation: tets p = 1..4
+// (4, 10, 20, 35 rows) and prisms p = 1..4 (6, 18, 40, 75 rows)
+static bool flux4_ok(int g)
+{
+    return g == 1 || g == 2 || g == 3 || g == 5 || g == 9 || g == 10 ||
+           g == 19;
+}
 
 template <typename T, int EP>
 static void launch_flux_ep(const FluxPlan *p, int wpb, long ncol,
@@ -1008,6 +1177,27 @@ static void launch_flux_ep(const FluxPlan *p, int wpb, long ncol,
                            const void *rcp, long ldr, void *xa, double dta,
                            double dtb, double gamma, hipStream_t s)
 {
+    if constexpr (sizeof(T) == 4) {
+        if (wpb == FR_V4) {
+            const dim3 grid(cdiv(ncol/64, 4)), block(256);
+#define FR_G4(GV)                                                           \
+            case GV:                                                        \
+                hipLaunchKernelGGL((k_tdivf_flux4<GV, EP>), grid, block, 0, \
+                                   s, p->d.m, p->nupts, p->nc0, p->nc1,     \
+                                   p->d.k1, (const float *)p->af4, ncol,    \
+                                   (const float *)u, ldu, (const float *)sm,\
+                                   lds, (const float *)fn, ldf, (float *)c, \
+                                   ldc, (const float *)rcp, ldr, xa, dta,   \
+                                   dtb, (float)(gamma - 1));                \
+                break;
+            switch (p->g4) {
+                FR_G4(1) FR_G4(2) FR_G4(3) FR_G4(5) FR_G4(9) FR_G4(10)
+                FR_G4(19)
+            }
+#undef FR_G4
+            return;
+        }
+    }
     if (wpb == FR_BLKV) {
         constexpr size_t fbb = 2*4*3*5*64*sizeof(T);
 #define FR_MTB(MTV)                                                         \
@@ -1104,7 +1294,9 @@ static int flux_launch(const FluxPlan *p, int ep, long ncol, const void *u,
     FR_CHECK(ep < 2 || xa, "RK epilogue needs the r1 / accumulator array");
     const size_t esz = p->d.dtype == FRHIP_F64 ? 8 : 4;
     int wpb = p->d.variant > 0 ? p->d.variant : 4;
-    if (wpb != FR_BLKV &&
+    if (wpb == FR_V4 && !p->af4)
+        wpb = 4;
+    if (wpb != FR_BLKV && wpb != FR_V4 &&
         !variant_valid(wpb, (size_t)p->d.mt*(p->d.ks0 + p->d.ks1)*64*esz,
                        false))
         wpb &= ~FR_LDSV;
@@ -1169,8 +1361,39 @@ int frhip_tdivf_plan_create(void **plan, int m, int nupts, int nfpts,
         (void)hipFree(d);
         FR_HIP(e);
     }
+    // 4x4x1 fragments (FP32): chunk q < nc0 holds points 5q..5q+4 x 3
+    // directions (k = 3 point + direction), chunk nc0 + q fn rows 16q..+15
+    void *d4 = nullptr;
+    const int g4 = (m + 3)/4, nc0 = (nupts + 4)/5, nc1 = (nfpts + 15)/16;
+    if (dtype == FRHIP_F32 && flux4_ok(g4)) {
+        std::vector<float> f4((size_t)(nc0 + nc1)*g4*64, 0.0f);
+        for (int q = 0; q < nc0 + nc1; ++q)
+            for (int g = 0; g < g4; ++g)
+                for (int l = 0; l < 64; ++l) {
+                    const int r = 4*g + (l & 3), kk = l >> 2;
+                    double v = 0.0;
+                    if (r < m && q < nc0 && kk < 15) {
+                        const int j = 5*q + kk/3, dd = kk % 3;
+                        if (j < nupts) v = a0[(long)r*k0 + dd*nupts + j];
+                    }
+                    else if (r < m && q >= nc0) {
+                        const int kp = 16*(q - nc0) + kk;
+                        if (kp < nfpts) v = a1[(long)r*nfpts + kp];
+                    }
+                    f4[((size_t)q*g4 + g)*64 + l] = (float)v;
+                }
+        hipError_t e4 = hipMalloc(&d4, f4.size()*4);
+        if (e4 == hipSuccess)
+            e4 = hipMemcpy(d4, f4.data(), f4.size()*4,
+                           hipMemcpyHostToDevice);
+        if (e4 != hipSuccess) {
+            (void)hipFree(d);
+            if (d4) (void)hipFree(d4);
+            FR_HIP(e4);
+        }
+    }
     FluxPlan *p = new FluxPlan{{m, k0, nfpts, dtype, mt, ks0, ks1, d, nullptr,
-                                4}, nupts, nq};
+                                4}, nupts, nq, d4, g4, nc0, nc1};
     *plan = p;
     return 0;
     FR_API_END
@@ -1182,6 +1405,7 @@ int frhip_tdivf_plan_destroy(void *plan)
     FluxPlan *p = static_cast<FluxPlan *>(plan);
     if (p) {
         if (p->d.afrag) (void)hipFree(p->d.afrag);
+        if (p->af4) (void)hipFree(p->af4);
         delete p;
     }
     return 0;
@@ -1224,8 +1448,10 @@ int frhip_tdivf_plan_tune(void *plan, int ncol, const void *u, long ldu,
     int bestv = 4, rc = 0;
     const size_t ldsb = (size_t)p->d.mt*(p->d.ks0 + p->d.ks1)*64*esz;
     for (int v : {1, 2, 4, FR_LDSV + 1, FR_LDSV + 2, FR_LDSV + 4,
-                  FR_BLKV}) {
-        if (v != FR_BLKV && !variant_valid(v, ldsb, false)) continue;
+                  FR_BLKV, FR_V4}) {
+        if (v == FR_V4 ? !p->af4
+                       : v != FR_BLKV && !variant_valid(v, ldsb, false))
+            continue;
         FluxPlan q = *p;
         q.d.variant = v;
         for (int it = 0; it < reps + 1 && !rc; ++it) {
@@ -1259,7 +1485,7 @@ int frhip_tdivf_plan_set_variant(void *plan, int variant)
     FluxPlan *p = static_cast<FluxPlan *>(plan);
     FR_CHECK(p, "null plan");
     const size_t esz = p->d.dtype == FRHIP_F64 ? 8 : 4;
-    FR_CHECK(variant == FR_BLKV ||
+    FR_CHECK(variant == FR_BLKV || (variant == FR_V4 && p->af4) ||
              variant_valid(variant,
                            (size_t)p->d.mt*(p->d.ks0 + p->d.ks1)*64*esz,
                            false), "invalid variant");
