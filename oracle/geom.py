@@ -6,6 +6,8 @@
 * ``kopriva_smats_djacs`` restates the stored (curved-element) metric,
   ``pyfr/solvers/base/elements.py:382-440``, used for p < 2 and for the
   physical normals and 1/|J| of every element (``:309-328``, ``:364-380``).
+* ``hex_ploc`` maps reference points through shape points of any order
+  (curved elements), the reference's ``sbasis``.
 """
 import itertools as it
 
@@ -14,11 +16,50 @@ import numpy as np
 from oracle.basis import HexBasis
 
 
+def hex_std_ele(k):
+    """Shape points of an order-k hex, ``TensorProdShape.std_ele``
+    (``pyfr/shapes.py:331-333``): equispaced, x fastest."""
+    pts1d = np.linspace(-1, 1, k + 1)
+    return np.array([p[::-1] for p in it.product(pts1d, repeat=3)])
+
+
 def hex_linspts():
     """Vertices of the reference hex, ``TensorProdShape.std_ele(1)``
     (``pyfr/shapes.py:278-282``)."""
-    pts1d = np.linspace(-1, 1, 2)
-    return np.array([p[::-1] for p in it.product(pts1d, repeat=3)])
+    return hex_std_ele(1)
+
+
+def hex_sorder(nspts):
+    """Shape order k of a hex with nspts = (k + 1)^3 nodes."""
+    k = round(nspts ** (1/3)) - 1
+    if (k + 1)**3 != nspts:
+        raise ValueError(f'Invalid hex node count {nspts}')
+    return k
+
+
+def hex_corners(spts):
+    """The 8 vertices (in ``HexShape`` order) of order-k shape points
+    (nspts, 3, E)."""
+    k = hex_sorder(spts.shape[0])
+    n = k + 1
+    idx = [((v & 1)*k) + n*((((v >> 1) & 1)*k) + n*(((v >> 2) & 1)*k))
+           for v in range(8)]
+    return spts[idx]
+
+
+def hex_ploc(spts, x):
+    """Physical location of reference points x for shape points spts
+    (nspts, 3, E) of any order: the shape basis ``sbasis`` of
+    ``pyfr/shapes.py:49-50`` (here the oracle's orthonormal-basis nodal
+    form), through ``ploc_at`` (``pyfr/solvers/base/elements.py:337-350``).
+    -> (npts, 3, E)."""
+    if spts.shape[0] == 8:
+        return trilinear_ploc(spts, x)
+    k = hex_sorder(spts.shape[0])
+    sb = HexBasis(k, hex_std_ele(k))
+    op = sb.nodal_basis_at(np.asarray(x))
+    ne = spts.shape[-1]
+    return (op @ spts.reshape(spts.shape[0], -1)).reshape(len(x), 3, ne)
 
 
 # Sign of each reference coordinate at each of the 8 vertices
@@ -83,7 +124,7 @@ def kopriva_smats_djacs(verts, order):
     mbasis = HexBasis(mord, mpts)
 
     # Physical location of the pseudo grid points (nmpts, 3, E)
-    x = trilinear_ploc(verts, mpts)
+    x = hex_ploc(verts, mpts)
 
     jacop = np.rollaxis(mbasis.jac_nodal_basis_at(mpts), 2).reshape(-1, nmpts)
     jac = (jacop @ x.reshape(nmpts, -1)).reshape(nmpts, 3, 3, neles)

@@ -13,8 +13,8 @@ import numpy as np
 
 from oracle.basis import HexShapeOracle
 from oracle.bcs import bccflux_euler, bccflux_ns, bcconu
-from oracle.geom import (linear_smats_djac, pnorm_at, rcpdjac_at, smats_at,
-                         trilinear_ploc)
+from oracle.geom import (hex_corners, hex_ploc, linear_smats_djac, pnorm_at,
+                         rcpdjac_at, smats_at)
 from oracle.mesh import box_hex_verts, pair_faces_bruteforce
 from oracle.physics import RSOLVERS, inviscid_flux, pri_to_con, viscous_flux_add
 
@@ -38,11 +38,14 @@ class OracleHexSystem:
 
     def __init__(self, n, order, system='euler', rsolver='rusanov',
                  constants=None, length=2*np.pi, verts=None,
-                 periodic=(True, True, True), bcs=None):
+                 periodic=(True, True, True), bcs=None, curved=None):
         """periodic: per-axis periodicity; the sides of the other axes
         ('x0', 'x1', 'y0', ...) are boundaries, each needing an entry
         ``bcs[side] = (type, vals)`` (``oracle.bcs``; vals may hold
-        callables of the flux-point coordinates (x, y, z))."""
+        callables of the flux-point coordinates (x, y, z)).  verts:
+        (nspts, 3, E) shape points of any order; curved: per-element mask
+        of the elements that take the stored metric at p >= 2 (the rest use
+        the linear-element metric of their 8 vertices)."""
         self.n, self.order, self.system = n, order, system
         self.rsolver = RSOLVERS[rsolver]
         self.c = dict(constants or {'gamma': 1.4})
@@ -64,19 +67,23 @@ class OracleHexSystem:
         # Geometry; p < 2 => every element uses the stored ("curved")
         # metric (pyfr/solvers/base/elements.py:224-227)
         self.linear = order >= 2
-        if self.linear:
-            s, d = linear_smats_djac(self.verts, sh.upts)
-            self.smats_u = s                        # (Nu, 3, 3, E)
-            self.rcpdjac_l = 1/d                    # (Nu, E)
+        cmask = (np.zeros(self.neles, dtype=bool) if curved is None
+                 else np.asarray(curved, dtype=bool))
+        if not self.linear:
+            cmask[:] = True
+        sm = smats_at(self.verts, order, sh.upts).transpose(1, 0, 2, 3)
+        rd = rcpdjac_at(self.verts, order, sh.upts)
+        if cmask.all():
+            self.smats_u, self.rcpdjac_l = sm, rd
         else:
-            sm = smats_at(self.verts, order, sh.upts)   # (3, Nu, 3, E)
-            self.smats_u = sm.transpose(1, 0, 2, 3)
-            self.rcpdjac_l = rcpdjac_at(self.verts, order, sh.upts)
+            s, d = linear_smats_djac(hex_corners(self.verts), sh.upts)
+            self.smats_u = np.where(cmask, sm, s)   # (Nu, 3, 3, E)
+            self.rcpdjac_l = np.where(cmask, rd, 1/d)
         self.rcpdjac = rcpdjac_at(self.verts, order, sh.upts)
         self.pnorm = pnorm_at(self.verts, order, sh.fpts, sh.norm_fpts)
 
         # Physical flux-point locations and the interface pairing
-        self.plocf = trilinear_ploc(self.verts, sh.fpts)
+        self.plocf = hex_ploc(self.verts, sh.fpts)
         bfaces = {}
         pairs = pair_faces_bruteforce(self.plocf, 6, sh.nfacefpts, length,
                                       periodic, bfaces)
@@ -98,7 +105,7 @@ class OracleHexSystem:
 
     # -- initial conditions (pyfr/solvers/base/elements.py:86-129)
     def ics(self, ics=TGV_ICS, vscale=1.0):
-        x, y, z = trilinear_ploc(self.verts, self.shape.upts).transpose(1, 0, 2)
+        x, y, z = hex_ploc(self.verts, self.shape.upts).transpose(1, 0, 2)
         pri = [ics['rho'](x, y, z), vscale*ics['u'](x, y, z),
                vscale*ics['v'](x, y, z), vscale*ics['w'](x, y, z),
                ics['p'](x, y, z)]

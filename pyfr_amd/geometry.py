@@ -1,6 +1,8 @@
 """Metric terms of hex elements (setup, host side).
 
-Restates the reference's metric pipeline for linear (trilinear-mapped) hexes:
+Restates the reference's metric pipeline for hexes of any shape order (the
+trilinear map of 8 vertices, or curved elements with (k + 1)^3 shape
+points):
 
 * physical locations by the shape-point basis (``elements.py:331-350``);
 * S = |J| J^-1 at pseudo-grid points in Kopriva's invariant curl form and |J|
@@ -67,6 +69,22 @@ class TensorLagrange:
         return np.stack([d0, d1, d2])
 
 
+def hex_shape_order(nspts):
+    """Shape order k of a hex with (k + 1)^3 shape points."""
+    k = round(nspts ** (1/3)) - 1
+    if k < 1 or (k + 1)**3 != nspts:
+        raise ValueError(f'Invalid hex shape-point count {nspts}')
+    return k
+
+
+def hex_corner_index(k):
+    """Indices of the 8 vertices (``HexShape`` order) among the order-k
+    shape points (x fastest)."""
+    n = k + 1
+    return [((v & 1)*k) + n*((((v >> 1) & 1)*k) + n*(((v >> 2) & 1)*k))
+            for v in range(8)]
+
+
 def shape_basis():
     """Trilinear shape-point basis of a linear hex: nodes are the vertices
     in ``HexShape`` order (x fastest)."""
@@ -79,10 +97,11 @@ class HexGeometry:
     def __init__(self, eles, order):
         self.eles = np.asarray(eles, dtype=float)
         self.nspts, self.neles, self.ndims = self.eles.shape
-        if self.nspts != 8:
-            raise ValueError('Only linear (8-node) hexes are supported')
+        self.sorder = hex_shape_order(self.nspts)
         self.order = order
-        self.sbasis = shape_basis()
+        # shape basis on the order-k equispaced shape points (pyfr/shapes.py:
+        # 49-50, :331-333); k = 1 is the trilinear map of the vertices
+        self.sbasis = TensorLagrange(np.linspace(-1, 1, self.sorder + 1))
         mord = max(order, 1)
         self.mbasis = TensorLagrange(np.linspace(-1, 1, mord + 1))
         self._smats_djacs = None

@@ -23,33 +23,63 @@ HEX_FACE_AXES = [(2, -1), (1, -1), (0, +1), (1, +1), (0, -1), (2, +1)]
 
 
 class BoxMesh:
-    """n^3 (or nx*ny*nz) box of linear hexes on [0, L)^3, periodic along
-    the ``periodic`` axes.  The two sides of every other axis are
-    boundaries named 'x0'/'x1', 'y0'/'y1', 'z0'/'z1' (low/high side):
-    ``bcon[name]`` lists their (etype, eidx, fidx) faces, the counterpart of
-    a mesh file's ``bcon_<name>`` arrays."""
+    """n^3 (or nx*ny*nz) box of hexes on [0, L)^3, periodic along the
+    ``periodic`` axes.  The two sides of every other axis are boundaries
+    named 'x0'/'x1', 'y0'/'y1', 'z0'/'z1' (low/high side): ``bcon[name]``
+    lists their (etype, eidx, fidx) faces, the counterpart of a mesh file's
+    ``bcon_<name>`` arrays.
+
+    sorder, warp: elements of shape order k = sorder ((k + 1)^3 equispaced
+    shape points, ``HexShape.std_ele``) whose points are moved by
+    ``warp(x, y, z) -> (x, y, z)`` (which must respect the periodic axes).
+    As the reference's reader does (``pyfr/readers/base.py:445-482``),
+    elements whose points lie within ``lintol`` (relative to their extent)
+    of the trilinear map of their vertices are snapped to it and count as
+    linear; the others are curved.  Curved elements come first
+    (``pyfr/readers/base.py:22-45``), ``spts_curved`` flags them."""
     ndims = 3
 
     def __init__(self, n, length=2*np.pi, origin=0.0,
-                 periodic=(True, True, True)):
+                 periodic=(True, True, True), sorder=1, warp=None,
+                 lintol=1e-5):
         nx, ny, nz = (n, n, n) if np.isscalar(n) else n
         self.shape = (nx, ny, nz)
         self.periodic = tuple(bool(p) for p in periodic)
         self.length = length
-        hx, hy, hz = length/nx, length/ny, length/nz
+        h = np.array([length/nx, length/ny, length/nz])
         E = nx*ny*nz
         e = np.arange(E)
-        ii, jj, kk = e % nx, (e // nx) % ny, e // (nx*ny)
-        verts = np.empty((8, E, 3))
-        for v in range(8):
-            bx, by, bz = v & 1, (v >> 1) & 1, (v >> 2) & 1
-            verts[v, :, 0] = origin + (ii + bx)*hx
-            verts[v, :, 1] = origin + (jj + by)*hy
-            verts[v, :, 2] = origin + (kk + bz)*hz
-        self.spts = {'hex': verts}
+        ijk = np.stack([e % nx, (e // nx) % ny, e // (nx*ny)])
+        k1 = sorder + 1
+        q = np.arange(k1**3)
+        off = np.stack([q % k1, (q // k1) % k1, q // (k1*k1)], 1)/sorder
+        spts = origin + (ijk.T[None] + off[:, None])*h      # (nsp, E, 3)
+        curved = np.zeros(E, dtype=bool)
+        if warp is not None:
+            spts = np.stack(warp(spts[..., 0], spts[..., 1], spts[..., 2]),
+                            axis=-1)
+            if sorder > 1:
+                # trilinear map of the vertices at the shape points
+                xi = off
+                cidx = [((v & 1)*sorder) + k1*((((v >> 1) & 1)*sorder) +
+                                               k1*(((v >> 2) & 1)*sorder))
+                        for v in range(8)]
+                w = np.ones((len(q), 8))
+                for v in range(8):
+                    for a in range(3):
+                        b = (v >> a) & 1
+                        w[:, v] *= xi[:, a] if b else 1 - xi[:, a]
+                lin = np.einsum('qv,vec->qec', w, spts[cidx])
+                dev = np.abs(spts - lin).max(axis=(0, 2))
+                ext = np.ptp(spts, axis=0).max(axis=1)
+                curved = dev >= lintol*ext
+                spts[:, ~curved] = lin[:, ~curved]
+        perm = np.argsort(~curved, kind='stable')
+        self.spts = {'hex': np.ascontiguousarray(spts[:, perm])}
+        self.spts_curved = {'hex': curved[perm]}
         self.etypes = ['hex']
         self.neles = E
-        self.con = self._connect(np.stack([ii, jj, kk]))
+        self.con = self._connect(ijk[:, perm])
 
     def _connect(self, ijk):
         """Pair faces through a structured key: (normal axis, periodic plane

@@ -34,7 +34,7 @@ import time
 
 import numpy as np
 
-from pyfr_amd.geometry import AffineGeometry, HexGeometry
+from pyfr_amd.geometry import AffineGeometry, HexGeometry, hex_corner_index
 from pyfr_amd.mesh import sorted_face_fpts, sorted_face_fpts_structured
 from pyfr_amd.shapes import SHAPES as HEX_SHAPES
 from pyfr_amd.simplex import SIMPLEX_SHAPES
@@ -230,9 +230,11 @@ class FluidElements:
     # -- constant geometric matrices
     @cached_property
     def _verts_linear(self):
-        # ploc_at('linspts') = the vertices (nspts, ndims, neles)
-        v = self.eles[:, self.linoff:].transpose(0, 2, 1)
-        return self._be.const_matrix(np.ascontiguousarray(v), tags={'align'})
+        # ploc_at('linspts') = the vertices (8, ndims, neles) of the linear
+        # region (the corners of higher-order shape points)
+        v = self.eles[hex_corner_index(self.geom.sorder), self.linoff:]
+        return self._be.const_matrix(np.ascontiguousarray(
+            v.transpose(0, 2, 1)), tags={'align'})
 
     @cached_property
     def _upts_const(self):
@@ -318,7 +320,7 @@ class EulerElements(FluidElements):
         # pyfr/solvers/euler/elements.py:149-206
         be.pointwise.register('pyfr.solvers.euler.kernels.tflux')
         tplargs = {'ndims': self.ndims, 'nvars': self.nvars,
-                   'nverts': self.nspts,
+                   'nverts': 8 if self.name == 'hex' else self.nspts,
                    'c': self.cfg.items_as('constants', float)}
         r, s = self.mesh_regions, self._slice_mat
 
@@ -425,7 +427,7 @@ class NavierStokesElements(EulerElements):
         if visc_corr not in {'sutherland', 'none'}:
             raise ValueError('Invalid viscosity-correction option')
         tplargs = {'ndims': self.ndims, 'nvars': self.nvars,
-                   'nverts': self.nspts,
+                   'nverts': 8 if self.name == 'hex' else self.nspts,
                    'c': self.cfg.items_as('constants', float),
                    'visc_corr': visc_corr, 'shock_capturing': 'none'}
         r, s = self.mesh_regions, self._slice_mat
@@ -746,8 +748,14 @@ class BaseSystem:
         for etype, spts in mesh.spts.items():
             elemap[etype] = self.elementscls(SHAPES[etype], spts, cfg)
         ics = {et: e.set_ics_from_cfg() for et, e in elemap.items()}
+        # curved elements come first; the linear region starts after the
+        # last one (pyfr/solvers/base/system.py:241-246)
+        curved = getattr(mesh, 'spts_curved', {})
         for et, e in elemap.items():
-            e.set_backend(backend, nonce, linoff=0)
+            c = curved.get(et)
+            linoff = 0 if c is None else int(np.max(np.nonzero(c)[0],
+                                                    initial=-1) + 1)
+            e.set_backend(backend, nonce, linoff=linoff)
         self.ele_map = elemap
         self.ele_types = list(elemap)
         self.ele_ndofs = [e.neles*e.nupts*e.nvars for e in elemap.values()]
