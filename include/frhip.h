@@ -190,6 +190,24 @@ int frhip_tflux_ns(int dtype, int linear, int npts, int neles,
                    const void *rcpdjac, long ldr, const double *consts,
                    const void *mu_e, void *stream);
 
+/* gradcoru: the corrected reference gradient (3 rows per point) made
+ * physical in place, g <- |J|^-1 S^T g;
+ * pyfr/solvers/baseadvecdiff/kernels/gradcoru.mako:10-23.  Geometry as for
+ * frhip_tflux_ns. */
+int frhip_gradcoru(int dtype, int linear, int npts, int neles, void *gradu,
+                   long ldg, const void *geo, long ldgeo, const void *upts,
+                   const void *rcpdjac, long ldr, void *stream);
+
+/* NS tflux from a physical gradient (the unfused / anti-aliased path):
+ * f holds the gradient on entry and the transformed flux on exit;
+ * pyfr/solvers/navstokes/kernels/tflux.mako:13-50 without 'fused'.
+ * consts and mu_e as for frhip_tflux_ns. */
+int frhip_tflux_ns_grad(int dtype, int linear, int npts, int neles,
+                        const void *u, long ldu, void *f, long ldf,
+                        const void *geo, long ldgeo, const void *upts,
+                        const double *consts, const void *mu_e,
+                        void *stream);
+
 /* negdivconf: pyfr/solvers/baseadvec/kernels/negdivconf.mako:7-22 (no source
  * terms): t[v] = -rcpdjac * t[v]; rcpdjac (npts, neles) with leading
  * dimension ldr, or ldr = 0 for one value per element. */

@@ -38,14 +38,18 @@ class OracleHexSystem:
 
     def __init__(self, n, order, system='euler', rsolver='rusanov',
                  constants=None, length=2*np.pi, verts=None,
-                 periodic=(True, True, True), bcs=None, curved=None):
+                 periodic=(True, True, True), bcs=None, curved=None,
+                 antialias=(), nq1=None):
         """periodic: per-axis periodicity; the sides of the other axes
         ('x0', 'x1', 'y0', ...) are boundaries, each needing an entry
         ``bcs[side] = (type, vals)`` (``oracle.bcs``; vals may hold
         callables of the flux-point coordinates (x, y, z)).  verts:
         (nspts, 3, E) shape points of any order; curved: per-element mask
         of the elements that take the stored metric at p >= 2 (the rest use
-        the linear-element metric of their 8 vertices)."""
+        the linear-element metric of their 8 vertices).  antialias:
+        ('flux',) evaluates the flux at the element quadrature points
+        (``pyfr/solvers/baseadvec/elements.py:78-94``,
+        ``baseadvecdiff/elements.py:103-116``)."""
         self.n, self.order, self.system = n, order, system
         self.rsolver = RSOLVERS[rsolver]
         self.c = dict(constants or {'gamma': 1.4})
@@ -63,6 +67,12 @@ class OracleHexSystem:
         self.M3 = sh.opmat('M3')
         self.M460 = sh.opmat('M4 - M6*M0')
         self.M6 = sh.opmat('M6')
+        self.fluxaa = 'flux' in antialias
+        if self.fluxaa:
+            sh.set_quadrature(nq1)
+            self.nqpts = len(sh.qpts)
+            self.M7 = sh.opmat('M7')
+            self.M1329 = sh.opmat('(M1 - M3*M2)*M9')
 
         # Geometry; p < 2 => every element uses the stored ("curved")
         # metric (pyfr/solvers/base/elements.py:224-227)
@@ -80,6 +90,14 @@ class OracleHexSystem:
             self.smats_u = np.where(cmask, sm, s)   # (Nu, 3, 3, E)
             self.rcpdjac_l = np.where(cmask, rd, 1/d)
         self.rcpdjac = rcpdjac_at(self.verts, order, sh.upts)
+        if self.fluxaa:
+            # S at the quadrature points: stored metric for curved
+            # elements, the vertex formula for linear ones
+            sq = smats_at(self.verts, order, sh.qpts).transpose(1, 0, 2, 3)
+            if not cmask.all():
+                sl, _ = linear_smats_djac(hex_corners(self.verts), sh.qpts)
+                sq = np.where(cmask, sq, sl)
+            self.smats_q = sq
         self.pnorm = pnorm_at(self.verts, order, sh.fpts, sh.norm_fpts)
 
         # Physical flux-point locations and the interface pairing
@@ -120,10 +138,10 @@ class OracleHexSystem:
         f, _, _ = inviscid_flux(u.transpose(1, 0, 2), self.c['gamma'])
         return f                                    # (3, 5, Nu, E)
 
-    def _transform(self, f):
+    def _transform(self, f, s=None):
         # F[i] = sum_k s[i][k] f[k]; tflux.mako:24-28
-        s = self.smats_u                            # (Nu, 3, 3, E)
-        F = np.zeros((3, self.nupts, 5, self.neles))
+        s = self.smats_u if s is None else s        # (Npts, 3, 3, E)
+        F = np.zeros((3, s.shape[0], 5, self.neles))
         for i in range(3):
             acc = 0
             for k in range(3):
@@ -152,8 +170,13 @@ class OracleHexSystem:
         uf = self._mul(self.M0, u)                          # disu
         self._intcflux_euler(uf)                            # intcflux
         self._bccflux_euler(uf)                             # bccflux
-        F = self._transform(self._tflux_inv(u))             # tdisf
-        div = self._mul(self.M132, F.reshape(3*self.nupts, 5, -1))
+        if self.fluxaa:
+            uq = self._mul(self.M7, u)                      # qptsu
+            F = self._transform(self._tflux_inv(uq), self.smats_q)
+            div = self._mul(self.M1329, F.reshape(3*self.nqpts, 5, -1))
+        else:
+            F = self._transform(self._tflux_inv(u))         # tdisf
+            div = self._mul(self.M132, F.reshape(3*self.nupts, 5, -1))
         div = self._mul(self.M3, uf) + div                  # tdivtconf
         return -self.rcpdjac[:, None, :]*div                # negdivconf
 
@@ -185,9 +208,19 @@ class OracleHexSystem:
             for k in range(3):
                 acc = acc + s[:, k, i][:, None, :]*grad[k]
             gp[i] = self.rcpdjac_l[:, None, :]*acc
-        f = self._tflux_inv(u)                              # (3, 5, Nu, E)
-        viscous_flux_add(u.transpose(1, 0, 2), gp.transpose(0, 2, 1, 3), f, c)
-        F = self._transform(f)
+        if self.fluxaa:
+            # gradcoru_qpts, qptsu and the flux at the quadrature points
+            uq = self._mul(self.M7, u)
+            gq = np.stack([self._mul(self.M7, gp[d]) for d in range(3)])
+            f = self._tflux_inv(uq)
+            viscous_flux_add(uq.transpose(1, 0, 2), gq.transpose(0, 2, 1, 3),
+                             f, c)
+            F = self._transform(f, self.smats_q)
+        else:
+            f = self._tflux_inv(u)                          # (3, 5, Nu, E)
+            viscous_flux_add(u.transpose(1, 0, 2), gp.transpose(0, 2, 1, 3),
+                             f, c)
+            F = self._transform(f)
 
         gf = np.stack([self._mul(self.M0, gp[d]) for d in range(3)])
 
@@ -221,7 +254,10 @@ class OracleHexSystem:
         for (_, _, bf, be, _, _), o in zip(self.bcs, bcout):  # bccflux
             uf[bf, :, be] = o.T
 
-        div = self._mul(self.M132, F.reshape(3*Nu, 5, -1))  # tdivtpcorf
+        if self.fluxaa:
+            div = self._mul(self.M1329, F.reshape(3*self.nqpts, 5, -1))
+        else:
+            div = self._mul(self.M132, F.reshape(3*Nu, 5, -1))  # tdivtpcorf
         div = self._mul(self.M3, uf) + div                  # tdivtconf
         return -self.rcpdjac[:, None, :]*div                # negdivconf
 

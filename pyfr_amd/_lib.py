@@ -68,6 +68,9 @@ _FUNCS = {
                           _P],
     'frhip_tflux_ns': [_I, _I, _I, _I, _P, _L, _P, _L, _P, _L, _P, _L, _P,
                        _P, _L, _P, _P, _P],
+    'frhip_gradcoru': [_I, _I, _I, _I, _P, _L, _P, _L, _P, _P, _L, _P],
+    'frhip_tflux_ns_grad': [_I, _I, _I, _I, _P, _L, _P, _L, _P, _L, _P, _P,
+                            _P, _P],
     'frhip_negdivconf': [_I, _I, _I, _P, _L, _P, _L, _P],
     'frhip_intcflux_euler': [_I, _I, _I, _P, _PI32, _PI32, _P, _L, _D, _P],
     'frhip_intconu': [_I, _I, _P, _PI32, _PI32, _P, _PI32, _PI32, _D, _P],

@@ -686,6 +686,7 @@ class PointwiseProvider:
         'pyfr.solvers.navstokes.kernels.bccflux': 'bccflux',
         'pyfr.solvers.navstokes.kernels.bcconu': 'bcconu',
         'pyfr.solvers.baseadvec.kernels.negdivconf': 'negdivconf',
+        'pyfr.solvers.baseadvecdiff.kernels.gradcoru': 'gradcoru',
         'pyfr.integrators.explicit.kernels.rkvdh2': 'rkvdh2',
     }
 
@@ -711,12 +712,20 @@ class PointwiseProvider:
         geo = verts if linear else smats
         uptsp = _ptr(upts) if linear else None
         c = tplargs['c']
-        if 'fused' in ktype:
+        ns = 'visc_corr' in tplargs
+        if ns:
             consts = (ct.c_double*8)(
                 c['gamma'], c.get('mu', 0.0), c.get('Pr', 1.0),
                 c.get('ldg-beta', 0.0), c.get('ldg-tau', 0.0),
                 1.0 if tplargs.get('visc_corr') == 'sutherland' else 0.0,
                 c.get('cpTref', 1.0), c.get('cpTs', 0.0))
+        if ns and 'fused' not in ktype:
+            # f holds the physical gradient on entry (tflux.mako, unfused)
+            args = (dt, mode, npts, neles, u.data, u.leaddim, f.data,
+                    f.leaddim, geo.data, geo.leaddim, uptsp, consts, None)
+            return Kernel(lambda stream: lib.tflux_ns_grad(*args, stream),
+                          mats=(u, f, geo), name='tflux_ns_grad')
+        if 'fused' in ktype:
             rp = None if linear else _ptr(rcpdjac)
             ldr = 0 if linear else _rstride(rcpdjac)
             args = (dt, mode, npts, neles, u.data, u.leaddim, f.data,
@@ -728,6 +737,22 @@ class PointwiseProvider:
                 f.leaddim, geo.data, geo.leaddim, uptsp, c['gamma'])
         return Kernel(lambda stream: lib.tflux_euler(*args, stream),
                       mats=(u, f, geo), name='tflux')
+
+    # -- gradcoru (baseadvecdiff/kernels/gradcoru.mako) -------------------------
+    def gradcoru(self, tplargs, dims, gradu, smats=None, rcpdjac=None,
+                 verts=None, upts=None, **kw):
+        lib = self.backend.lib
+        npts, neles = dims
+        ktype = tplargs['ktype']
+        mode = 1 if 'linear' in ktype else 2 if 'affine' in ktype else 0
+        geo = verts if mode == 1 else smats
+        uptsp = _ptr(upts) if mode == 1 else None
+        rp = None if mode == 1 else _ptr(rcpdjac)
+        ldr = 0 if mode == 1 else _rstride(rcpdjac)
+        args = (_dtcode(gradu.dtype), mode, npts, neles, gradu.data,
+                gradu.leaddim, geo.data, geo.leaddim, uptsp, rp, ldr)
+        return Kernel(lambda stream: lib.gradcoru(*args, stream),
+                      mats=(gradu, geo), name='gradcoru')
 
     # -- negdivconf ----------------------------------------------------------
     def negdivconf(self, tplargs, dims, tdivtconf, rcpdjac, ploc=None, u=None,

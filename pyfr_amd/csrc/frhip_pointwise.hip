@@ -171,6 +171,67 @@ __global__ __launch_bounds__(256) void k_tflux_ns(
     }
 }
 
+// gradcoru: the corrected reference gradient made physical in place,
+// g <- |J|^-1 S^T g; pyfr/solvers/baseadvecdiff/kernels/gradcoru.mako:10-23
+// with transform_grad.mako:3-10 (the separate step of the unfused NS path).
+template <typename T, int GEO>
+__global__ __launch_bounds__(256) void k_gradcoru(
+    int npts, int neles, T *__restrict__ gradu, long ldgr,
+    const T *__restrict__ geo, long ldg, const T *__restrict__ upts,
+    const T *__restrict__ rcpdjac, long ldr)
+{
+    const int e = blockIdx.x*64 + threadIdx.x;
+    if (e >= neles) return;
+    T V[8][3];
+    load_geo<T, GEO>(geo, ldg, e, V);
+    for (int pt = blockIdx.y*4 + threadIdx.y; pt < npts; pt += 4*gridDim.y) {
+        T s[3][3], djac;
+        smats_at<T, GEO>(geo, ldg, e, pt, npts, upts, V, s, djac);
+        const T rcpd = GEO == 1 ? T(1)/djac : rcpdjac[pt*ldr + e];
+        T tg[3][5];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 5; ++j)
+                tg[i][j] = gradu[((long)i*npts + pt)*ldgr + soa_ix(e, j, 5)];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 5; ++j)
+                gradu[((long)i*npts + pt)*ldgr + soa_ix(e, j, 5)] =
+                    rcpd*(s[0][i]*tg[0][j] + s[1][i]*tg[1][j] +
+                          s[2][i]*tg[2][j]);
+    }
+}
+
+// tflux (NS) from a physical gradient: f holds the gradient on entry and
+// the transformed flux on exit; pyfr/solvers/navstokes/kernels/tflux.mako:
+// 13-50 without 'fused' (the anti-aliased path, at quadrature points).
+template <typename T, int GEO>
+__global__ __launch_bounds__(256) void k_tflux_ns_grad(
+    int npts, int neles, const T *__restrict__ u, long ldu,
+    T *__restrict__ f, long ldf, const T *__restrict__ geo, long ldg,
+    const T *__restrict__ upts, NSConsts<T> c0, const T *__restrict__ mu_e)
+{
+    const int e = blockIdx.x*64 + threadIdx.x;
+    if (e >= neles) return;
+    const NSConsts<T> c = mu_e ? c0.with_mu(mu_e[e]) : c0;
+    T V[8][3];
+    load_geo<T, GEO>(geo, ldg, e, V);
+    for (int pt = blockIdx.y*4 + threadIdx.y; pt < npts; pt += 4*gridDim.y) {
+        T s[3][3], djac;
+        smats_at<T, GEO>(geo, ldg, e, pt, npts, upts, V, s, djac);
+        T uu[5], g[3][5];
+        for (int v = 0; v < 5; ++v) uu[v] = u[pt*ldu + soa_ix(e, v, 5)];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 5; ++j)
+                g[i][j] = f[((long)i*npts + pt)*ldf + soa_ix(e, j, 5)];
+        T ft[3][5], p, vel[3];
+        inviscid_flux(uu, ft, p, vel, c.gm1);
+        viscous_flux_add(uu, g, ft, c);
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 5; ++j)
+                f[((long)i*npts + pt)*ldf + soa_ix(e, j, 5)] =
+                    s[i][0]*ft[0][j] + s[i][1]*ft[1][j] + s[i][2]*ft[2][j];
+    }
+}
+
 // negdivconf; pyfr/solvers/baseadvec/kernels/negdivconf.mako:7-22.
 template <typename T>
 __global__ __launch_bounds__(256) void k_negdivconf(
@@ -539,6 +600,77 @@ int frhip_tflux_euler(int dtype, int linear, int npts, int neles,
         else FR_LAUNCH_TF(float, 2);
     }
 #undef FR_LAUNCH_TF
+    FR_LAUNCHED();
+    return 0;
+    FR_API_END
+}
+
+int frhip_gradcoru(int dtype, int linear, int npts, int neles, void *gradu,
+                   long ldg, const void *geo, long ldgeo, const void *upts,
+                   const void *rcpdjac, long ldr, void *stream)
+{
+    FR_API_BEGIN
+    FR_DTYPE_OK(dtype);
+    FR_CHECK(npts > 0 && neles >= 0, "invalid extents");
+    FR_CHECK(linear >= 0 && linear <= 2, "invalid geometry mode");
+    if (neles == 0) return 0;
+    FR_CHECK(linear == FRHIP_GEO_HEX_LINEAR ? upts != nullptr
+                                            : rcpdjac != nullptr,
+             "missing geometry argument");
+    dim3 grid = elem_grid(npts, neles), block(64, 4);
+    hipStream_t s = as_stream(stream);
+#define FR_LAUNCH_GC(T, G)                                                  \
+    hipLaunchKernelGGL((k_gradcoru<T, G>), grid, block, 0, s, npts, neles,  \
+                       (T *)gradu, ldg, (const T *)geo, ldgeo,              \
+                       (const T *)upts, (const T *)rcpdjac, ldr)
+    if (dtype == FRHIP_F64) {
+        if (linear == 0) FR_LAUNCH_GC(double, 0);
+        else if (linear == 1) FR_LAUNCH_GC(double, 1);
+        else FR_LAUNCH_GC(double, 2);
+    }
+    else {
+        if (linear == 0) FR_LAUNCH_GC(float, 0);
+        else if (linear == 1) FR_LAUNCH_GC(float, 1);
+        else FR_LAUNCH_GC(float, 2);
+    }
+#undef FR_LAUNCH_GC
+    FR_LAUNCHED();
+    return 0;
+    FR_API_END
+}
+
+int frhip_tflux_ns_grad(int dtype, int linear, int npts, int neles,
+                        const void *u, long ldu, void *f, long ldf,
+                        const void *geo, long ldgeo, const void *upts,
+                        const double *consts, const void *mu_e, void *stream)
+{
+    FR_API_BEGIN
+    FR_DTYPE_OK(dtype);
+    FR_CHECK(npts > 0 && neles >= 0 && consts, "invalid arguments");
+    FR_CHECK(linear >= 0 && linear <= 2, "invalid geometry mode");
+    if (neles == 0) return 0;
+    FR_CHECK(linear != FRHIP_GEO_HEX_LINEAR || upts,
+             "linear tflux needs the point coordinates");
+    dim3 grid = elem_grid(npts, neles), block(64, 4);
+    hipStream_t s = as_stream(stream);
+#define FR_LAUNCH_NG(T, G)                                                  \
+    hipLaunchKernelGGL((k_tflux_ns_grad<T, G>), grid, block, 0, s, npts,    \
+                       neles, (const T *)u, ldu, (T *)f, ldf,               \
+                       (const T *)geo, ldgeo, (const T *)upts, c,           \
+                       (const T *)mu_e)
+    if (dtype == FRHIP_F64) {
+        auto c = ns_consts<double>(consts);
+        if (linear == 0) FR_LAUNCH_NG(double, 0);
+        else if (linear == 1) FR_LAUNCH_NG(double, 1);
+        else FR_LAUNCH_NG(double, 2);
+    }
+    else {
+        auto c = ns_consts<float>(consts);
+        if (linear == 0) FR_LAUNCH_NG(float, 0);
+        else if (linear == 1) FR_LAUNCH_NG(float, 1);
+        else FR_LAUNCH_NG(float, 2);
+    }
+#undef FR_LAUNCH_NG
     FR_LAUNCHED();
     return 0;
     FR_API_END

@@ -157,7 +157,10 @@ class HexShape:
     ndims = 3
     nfaces = 6
 
-    def __init__(self, order):
+    def __init__(self, order, nq1=None):
+        """nq1: points per direction of the Gauss-Legendre quadrature rule
+        of flux anti-aliasing (default order + 2, the reference's
+        ``(order + 2)**3`` hex rule, ``pyfr/shapes.py:147-176``)."""
         if order < 1:
             raise ValueError('Order must be >= 1')
         self.order = order
@@ -166,6 +169,8 @@ class HexShape:
         self.nupts = n**3
         self.nfacefpts = [n*n]*6
         self.nfpts = 6*n*n
+        self.nq1 = nq1 or order + 2
+        self.nqpts = self.nq1**3
 
     # -- point sets
     def uidx(self, i, j, k):
@@ -285,6 +290,42 @@ class HexShape:
     def m6(self):
         m = self.norm_fpts.T[:, None, :]*self.m3
         return m.reshape(-1, self.nfpts)
+
+    # -- flux anti-aliasing (pyfr/shapes.py:125-135): quadrature points, M7
+    #    (solution -> quadrature points), M8 (L2 projection back), M9
+    @cached_property
+    def qpts(self):
+        x, n = gauss_legendre(self.nq1)[0], self.nq1
+        return np.array([(x[i], x[j], x[k])
+                         for k in range(n) for j in range(n) for i in range(n)])
+
+    @cached_property
+    def m7(self):
+        L = lagrange_at(self.x1d, gauss_legendre(self.nq1)[0])  # (nq, n)
+        return np.kron(L, np.kron(L, L))
+
+    @cached_property
+    def m8(self):
+        """The L2 projection onto the order-p space: values at the solution
+        points of sum_k psi_k(x) sum_q w_q psi_k(x_q) f_q with the
+        orthonormal Legendre products psi_k, a Kronecker product of 1-D
+        projections for the tensor-product space."""
+        xq, wq = gauss_legendre(self.nq1)
+        P = np.zeros((self.n, self.nq1))
+        for i, xi in enumerate(self.x1d):
+            for q, (xqq, wqq) in enumerate(zip(xq, wq)):
+                P[i, q] = wqq*sum((k + 0.5)*_legendre(k, float(xi))[0] *
+                                  _legendre(k, float(xqq))[0]
+                                  for k in range(self.n))
+        return np.kron(P, np.kron(P, P))
+
+    @property
+    def m9(self):
+        m8, (a, b) = self.m8, self.m8.shape
+        out = np.zeros((3*a, 3*b))
+        for d in range(3):
+            out[d*a:(d + 1)*a, d*b:(d + 1)*b] = m8
+        return out
 
     def opmat(self, expr):
         """Evaluate an operator expression ('M0', 'M1 - M3*M2', ...) the way

@@ -96,7 +96,22 @@ class FluidElements:
         self.nvars = len(self.convars)
         self.kernels = {}
         order = cfg.getint('solver', 'order')
-        self.basis = basis = basiscls(order)
+        # anti-aliasing options (pyfr/shapes.py:40-44)
+        aa = cfg.get('solver', 'anti-alias', 'none')
+        self.antialias = {a.strip() for a in aa.split(',')} - {'none'}
+        if self.antialias - {'flux', 'surf-flux'}:
+            raise ValueError('Invalid anti-alias options')
+        if 'surf-flux' in self.antialias:
+            raise NotImplementedError('surface-flux anti-aliasing is not '
+                                      'supported')
+        self.fluxaa = 'flux' in self.antialias
+        if self.fluxaa and basiscls is not SHAPES['hex']:
+            raise NotImplementedError('flux anti-aliasing is supported for '
+                                      'hexes only')
+        if self.fluxaa:
+            self.basis = basis = basiscls(order, self._aa_nq1(cfg, order))
+        else:
+            self.basis = basis = basiscls(order)
         self.name = basis.name
         self.nupts, self.nfpts = basis.nupts, basis.nfpts
         self.nfacefpts = basis.nfacefpts
@@ -106,6 +121,24 @@ class FluidElements:
             self.geom = AffineGeometry(eles, self.name)
         else:
             self.geom = HexGeometry(eles, order)
+
+    @staticmethod
+    def _aa_nq1(cfg, order):
+        """Points per direction of the element quadrature rule
+        (``pyfr/shapes.py:147-176``): Gauss-Legendre, by default
+        (order + 2)^3 points, or [solver-elements-hex] quad-npts."""
+        sect = 'solver-elements-hex'
+        if cfg.hasopt(sect, 'quad-deg'):
+            raise NotImplementedError('quad-deg rule selection is not '
+                                      'supported; give quad-npts')
+        if cfg.get(sect, 'quad-pts', 'gauss-legendre') != 'gauss-legendre':
+            raise NotImplementedError('only Gauss-Legendre quadrature '
+                                      'points are supported')
+        npts = cfg.getint(sect, 'quad-npts', (order + 2)**3)
+        n1 = round(npts ** (1/3))
+        if n1**3 != npts or n1 < order + 1:
+            raise ValueError('Invalid quad-npts for a hex')
+        return n1
 
     # -- initial conditions (pyfr/solvers/base/elements.py:86-129)
     @staticmethod
@@ -154,7 +187,7 @@ class FluidElements:
 
     def set_backend(self, backend, nonce, linoff):
         self._be = backend
-        self.grad_fusion = not backend.blocks
+        self.grad_fusion = not (backend.blocks or self.fluxaa)
         if self.basis.order >= 2:
             self.linoff = linoff - linoff % -backend.csubsz
         else:
@@ -180,6 +213,10 @@ class FluidElements:
                                     (ndims, self.nupts, nvars, neles))
         elif 'vect_upts' in sbufs:
             self._grad_upts = self._vect_upts
+        if self.fluxaa:
+            nq = self.basis.nqpts
+            self._scal_qpts = alloc('scal_qpts', (nq, nvars, neles))
+            self._vect_qpts = alloc('vect_qpts', (ndims, nq, nvars, neles))
         self.scal_upts = []
 
     def post_commit(self):
@@ -239,6 +276,31 @@ class FluidElements:
     @cached_property
     def _upts_const(self):
         return self._be.const_matrix(self.basis.upts)
+
+    @cached_property
+    def _qpts_const(self):
+        return self._be.const_matrix(self.basis.qpts)
+
+    @cached_property
+    def _smats_curved_q(self):
+        s = self.geom.smat_at(self.basis.qpts)[..., :self.linoff]
+        return self._be.const_matrix(s, tags={'align'})
+
+    # the divergence operator on the flux (at quadrature points under flux
+    # anti-aliasing, pyfr/solvers/baseadvec/elements.py:84-94)
+    @property
+    def _divop(self):
+        return '(M1 - M3*M2)*M9' if self.fluxaa else 'M1 - M3*M2'
+
+    @property
+    def _flux_buf(self):
+        return self._vect_qpts if self.fluxaa else self._vect_upts
+
+    def _qptsu(self, uin):
+        """qptsu: M7 u at the quadrature points
+        (``pyfr/solvers/baseadvec/elements.py:78-82``)."""
+        return self._be.kernel('mul', self.opmat('M7'), self.scal_upts[uin],
+                               out=self._scal_qpts)
 
     @cached_property
     def _smats_curved(self):
@@ -306,7 +368,7 @@ class EulerElements(FluidElements):
         kernels['disu'] = lambda uin: be.kernel(
             'mul', self.opmat('M0'), self.scal_upts[uin], out=self._scal_fpts)
         kernels['tdivtpcorf'] = lambda fout: be.kernel(
-            'mul', self.opmat('M1 - M3*M2'), self._vect_upts,
+            'mul', self.opmat(self._divop), self._flux_buf,
             out=self.scal_upts[fout])
         kernels['tdivtconf'] = lambda fout: be.kernel(
             'mul', self.opmat('M3'), self._scal_fpts,
@@ -325,6 +387,8 @@ class EulerElements(FluidElements):
         r, s = self.mesh_regions, self._slice_mat
 
         def tdisf(uin):
+            if self.fluxaa:
+                return self._tdisf_qpts(uin, tplargs)
             ks = []
             if 'affine' in r:
                 ks.append(be.kernel(
@@ -349,6 +413,26 @@ class EulerElements(FluidElements):
         kernels['tdisf'] = tdisf
         self._add_fused_kernels()
 
+    def _tdisf_qpts(self, uin, tplargs):
+        """Flux anti-aliasing: u interpolated to the quadrature points and
+        the flux formed there (``baseadvec/elements.py:78-82``,
+        ``euler/elements.py:149-206`` with pts = qpts)."""
+        be, r, s = self._be, self.mesh_regions, self._slice_mat
+        nq = self.basis.nqpts
+        ks = [self._qptsu(uin)]
+        if 'curved' in r:
+            ks.append(be.kernel(
+                'tflux', tplargs=tplargs | {'ktype': 'curved'},
+                dims=[nq, r['curved']], u=s(self._scal_qpts, 'curved'),
+                f=s(self._vect_qpts, 'curved'), smats=self._smats_curved_q))
+        if 'linear' in r:
+            ks.append(be.kernel(
+                'tflux', tplargs=tplargs | {'ktype': 'linear'},
+                dims=[nq, r['linear']], u=s(self._scal_qpts, 'linear'),
+                f=s(self._vect_qpts, 'linear'), verts=self._verts_linear,
+                upts=self._qpts_const))
+        return be.ordered_meta_kernel(ks)
+
     def _add_fused_kernels(self):
         """Fused tdivtpcorf + tdivtconf + negdivconf (+ RK45 stage): one
         pass over the element's flux and common normal flux."""
@@ -356,8 +440,8 @@ class EulerElements(FluidElements):
 
         def tdivf(fout, xa=None, stage=None):
             return be.kernel(
-                'tdivf', self.opmat('M1 - M3*M2'), self.opmat('M3'),
-                self._vect_upts, self._scal_fpts, self.scal_upts[fout],
+                'tdivf', self.opmat(self._divop), self.opmat('M3'),
+                self._flux_buf, self._scal_fpts, self.scal_upts[fout],
                 self._rcpdjac_upts,
                 xa=None if xa is None else self.scal_upts[xa], stage=stage)
         self.kernels['tdivf'] = tdivf
@@ -392,7 +476,7 @@ class NavierStokesElements(EulerElements):
         kernels['disu'] = lambda uin: be.kernel(
             'mul', self.opmat('M0'), self.scal_upts[uin], out=self._scal_fpts)
         kernels['tdivtpcorf'] = lambda fout: be.kernel(
-            'mul', self.opmat('M1 - M3*M2'), self._vect_upts,
+            'mul', self.opmat(self._divop), self._flux_buf,
             out=self.scal_upts[fout])
         kernels['tdivtconf'] = lambda fout: be.kernel(
             'mul', self.opmat('M3'), self._scal_fpts,
@@ -459,6 +543,11 @@ class NavierStokesElements(EulerElements):
                     verts=self._verts_linear, upts=self._upts_const))
             return ks[0] if len(ks) == 1 else be.unordered_meta_kernel(ks)
 
+        if self.fluxaa:
+            be.pointwise.register('pyfr.solvers.baseadvecdiff.kernels.gradcoru')
+
+            def tdisf_fused(uin):
+                return self._ns_flux_qpts(uin, tplargs)
         kernels['tdisf_fused'] = tdisf_fused
 
         # Fused tgradpcoru + tgradcoru: grad = (M4 - M6 M0) u + M6 comm
@@ -466,6 +555,47 @@ class NavierStokesElements(EulerElements):
             'mul2', self.opmat('M4 - M6*M0'), self.opmat('M6'),
             self.scal_upts[uin], self._comm_fpts, self._grad_upts)
         self._add_fused_kernels()
+
+    def _ns_flux_qpts(self, uin, tplargs):
+        """Without gradient fusion (flux anti-aliasing): the corrected
+        gradient made physical in place at the solution points (gradcoru,
+        ``baseadvecdiff/elements.py:68-85``), interpolated to the
+        quadrature points (gradcoru_qpts, ``:103-116``) with u (qptsu), and
+        the unfused NS flux formed there (``navstokes/elements.py:63-128``).
+        Runs between tgradf and gradcoru_fpts, which then interpolates the
+        physical gradient."""
+        be, r, s = self._be, self.mesh_regions, self._slice_mat
+        nu, nq = self.nupts, self.basis.nqpts
+        gtpl = {'ndims': self.ndims, 'nvars': self.nvars, 'nverts': 8}
+        ks = []
+        if 'curved' in r:
+            ks.append(be.kernel(
+                'gradcoru', tplargs=gtpl | {'ktype': 'curved'},
+                dims=[nu, r['curved']], gradu=s(self._grad_upts, 'curved'),
+                smats=self._smats_curved, rcpdjac=self._rcpdjac_curved()))
+        if 'linear' in r:
+            ks.append(be.kernel(
+                'gradcoru', tplargs=gtpl | {'ktype': 'linear'},
+                dims=[nu, r['linear']], gradu=s(self._grad_upts, 'linear'),
+                verts=self._verts_linear, upts=self._upts_const))
+        gu, gq = self._grad_upts, self._vect_qpts
+        ks += [be.kernel('mul', self.opmat('M7'),
+                         gu.slice(i*nu, (i + 1)*nu),
+                         gq.slice(i*nq, (i + 1)*nq))
+               for i in range(self.ndims)]
+        ks.append(self._qptsu(uin))
+        if 'curved' in r:
+            ks.append(be.kernel(
+                'tflux', tplargs=tplargs | {'ktype': 'curved'},
+                dims=[nq, r['curved']], u=s(self._scal_qpts, 'curved'),
+                f=s(self._vect_qpts, 'curved'), smats=self._smats_curved_q))
+        if 'linear' in r:
+            ks.append(be.kernel(
+                'tflux', tplargs=tplargs | {'ktype': 'linear'},
+                dims=[nq, r['linear']], u=s(self._scal_qpts, 'linear'),
+                f=s(self._vect_qpts, 'linear'), verts=self._verts_linear,
+                upts=self._qpts_const))
+        return be.ordered_meta_kernel(ks)
 
 
 # ---------------------------------------------------------------------------
