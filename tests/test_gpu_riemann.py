@@ -47,10 +47,20 @@ def _pairs(rng, N, jump):
     return ul[:, keep], ur[:, keep], nl[:, keep]
 
 
+def _dev(lib, arr):
+    """A device copy of a host array through the library's own runtime
+    calls (no second HIP runtime in the process)."""
+    import ctypes as ct
+    a = np.ascontiguousarray(arr)
+    p = ct.c_void_p()
+    lib.malloc(ct.byref(p), a.nbytes)
+    lib.memcpy_htod(p, a.ctypes.data, a.nbytes, None)
+    return p
+
+
 @pytest.mark.parametrize('name', list(SOLVERS))
 @pytest.mark.parametrize('jump', [0.05, 1.0, 25.0])
 def test_intcflux_pointwise(backend_factory, name, jump):
-    import torch
     from oracle.physics import RSOLVERS
     be = backend_factory()
     rng = np.random.default_rng(int(jump*100) + SOLVERS[name])
@@ -65,17 +75,22 @@ def test_intcflux_pointwise(backend_factory, name, jump):
         return a.ravel()
     k = np.arange(N)
     soa = (k//64)*320 + k % 64
-    base = torch.tensor(np.concatenate([aosoa(ul), aosoa(ur)]),
-                        device='cuda')
-    lm = torch.tensor(soa.astype(np.int32), device='cuda')
-    rm = torch.tensor((soa + nb*320).astype(np.int32), device='cuda')
-    nt = torch.tensor(np.ascontiguousarray(nl), device='cuda')
+    lib = be.lib
+    host = np.concatenate([aosoa(ul), aosoa(ur)])
+    bufs = [_dev(lib, host), _dev(lib, soa.astype(np.int32)),
+            _dev(lib, (soa + nb*320).astype(np.int32)), _dev(lib, nl)]
+    base, lm, rm, nt = bufs
     print(f'\n[riemann] {name} jump={jump}', flush=True)
-    be.lib.intcflux_euler(0, SOLVERS[name], N, base.data_ptr(),
-                          lm.data_ptr(), rm.data_ptr(), nt.data_ptr(), N,
-                          1.4, None)
-    torch.cuda.synchronize()
-    out = base.cpu().numpy()
+    try:
+        lib.intcflux_euler(0, SOLVERS[name], N, base, lm, rm, nt, N, 1.4,
+                           None)
+        lib.device_sync()
+        out = np.empty_like(host)
+        lib.memcpy_dtoh(out.ctypes.data, base, out.nbytes, None)
+        lib.device_sync()
+    finally:
+        for b in bufs:
+            lib.free(b)
     got_l = np.stack([out[soa + 64*v] for v in range(5)])
     got_r = np.stack([out[nb*320 + soa + 64*v] for v in range(5)])
     mag = np.sqrt((nl**2).sum(0))
