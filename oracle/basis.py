@@ -247,7 +247,18 @@ class HexShapeOracle:
 
     @cached_property
     def m3(self):
-        return self.gbasis_at(self.upts)
+        m = self.gbasis_at(self.upts)
+        if getattr(self, 'iq_pts', None) is not None:
+            # shapes.py:107-111 with proj_l2 (:18-19) of the interface rule
+            # onto the face basis at the standard flux points
+            fb = QuadBasis(self.order, self.face_pts2)
+            fp = clean(fb.vdm.T @ (self.iq_wts*fb.ortho_basis_at(self.iq_pts)))
+            nq = len(self.iq_pts)
+            bd = np.zeros((len(fp)*6, nq*6))
+            for f in range(6):
+                bd[f*len(fp):(f + 1)*len(fp), f*nq:(f + 1)*nq] = fp
+            m = m @ bd
+        return m
 
     @cached_property
     def m4(self):
@@ -258,6 +269,22 @@ class HexShapeOracle:
     def m6(self):
         m = self.norm_fpts.T[:, None, :]*self.m3
         return m.reshape(-1, self.nfpts)
+
+    # Surface-flux anti-aliasing (pyfr/shapes.py:196-224, :297-303): the
+    # flux points of each face are the interface rule (Gauss-Legendre,
+    # nf1 x nf1 points; default (order + 2)^2, shapes.py:159-161); M3 as
+    # above.  The face basis of gbasis stays on the standard flux points.
+    def set_surface_quadrature(self, nf1=None):
+        self.iq_pts, self.iq_wts = gauss_legendre_quad(nf1 or self.order + 2)
+        nq = len(self.iq_pts)
+        self.nfacefpts = [nq]*6
+        self.fpts = np.vstack([_proj(p, self.iq_pts) for p, _ in HEX_FACES])
+        self.nfpts = 6*nq
+        self.norm_fpts = np.vstack([[nrm]*nq for _, nrm in HEX_FACES]
+                                   ).astype(float)
+        self.facefpts = [list(range(f*nq, (f + 1)*nq)) for f in range(6)]
+        for k in ('m0', 'm2', 'm3', 'm6'):
+            self.__dict__.pop(k, None)
 
     # Flux anti-aliasing (pyfr/shapes.py:125-135, :18-19): the element
     # quadrature rule (Gauss-Legendre, nq1 points per direction; default

@@ -39,7 +39,7 @@ class OracleHexSystem:
     def __init__(self, n, order, system='euler', rsolver='rusanov',
                  constants=None, length=2*np.pi, verts=None,
                  periodic=(True, True, True), bcs=None, curved=None,
-                 antialias=(), nq1=None):
+                 antialias=(), nq1=None, nf1=None):
         """periodic: per-axis periodicity; the sides of the other axes
         ('x0', 'x1', 'y0', ...) are boundaries, each needing an entry
         ``bcs[side] = (type, vals)`` (``oracle.bcs``; vals may hold
@@ -49,13 +49,17 @@ class OracleHexSystem:
         the linear-element metric of their 8 vertices).  antialias:
         ('flux',) evaluates the flux at the element quadrature points
         (``pyfr/solvers/baseadvec/elements.py:78-94``,
-        ``baseadvecdiff/elements.py:103-116``)."""
+        ``baseadvecdiff/elements.py:103-116``); ('surf-flux',) puts the
+        flux points at the nf1 x nf1 face quadrature rule, with M3 fed by
+        its L2 projection (``pyfr/shapes.py:103-113``)."""
         self.n, self.order, self.system = n, order, system
         self.rsolver = RSOLVERS[rsolver]
         self.c = dict(constants or {'gamma': 1.4})
         self.length = length
 
         self.shape = sh = HexShapeOracle(order)
+        if 'surf-flux' in antialias:
+            sh.set_surface_quadrature(nf1)
         self.nupts, self.nfpts = sh.nupts, sh.nfpts
         self.verts = box_hex_verts(n, length) if verts is None else verts
         self.neles = self.verts.shape[-1]

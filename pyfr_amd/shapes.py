@@ -151,24 +151,44 @@ def flush(a, tol=1e-10):
     return a
 
 
+def l2_projection_1d(x, nq):
+    """(len(x), nq): values at the nodes x of the L2 projection, onto the
+    polynomials of degree < len(x), of data at the nq-point Gauss-Legendre
+    rule: sum_k (k + 1/2) P_k(x_i) sum_q w_q P_k(x_q) f_q."""
+    xq, wq = gauss_legendre(nq)
+    P = np.zeros((len(x), nq))
+    for i, xi in enumerate(x):
+        for q, (xqq, wqq) in enumerate(zip(xq, wq)):
+            P[i, q] = wqq*sum((k + 0.5)*_legendre(k, float(xi))[0] *
+                              _legendre(k, float(xqq))[0]
+                              for k in range(len(x)))
+    return P
+
+
 class HexShape:
     """Hex FR element of order p with GL solution and GL flux points."""
     name = 'hex'
     ndims = 3
     nfaces = 6
 
-    def __init__(self, order, nq1=None):
+    def __init__(self, order, nq1=None, nf1=None):
         """nq1: points per direction of the Gauss-Legendre quadrature rule
         of flux anti-aliasing (default order + 2, the reference's
-        ``(order + 2)**3`` hex rule, ``pyfr/shapes.py:147-176``)."""
+        ``(order + 2)**3`` hex rule, ``pyfr/shapes.py:147-176``).  nf1:
+        surface-flux anti-aliasing, the flux points of each face are the
+        nf1 x nf1 Gauss-Legendre rule (``pyfr/shapes.py:196-224``,
+        ``:297-303``; the reference's default is nf1 = order + 2)."""
         if order < 1:
             raise ValueError('Order must be >= 1')
         self.order = order
         self.n = n = order + 1
         self.x1d, self.w1d = gauss_legendre(n)
         self.nupts = n**3
-        self.nfacefpts = [n*n]*6
-        self.nfpts = 6*n*n
+        self.nf1 = nf = nf1 or n
+        self.surf_flux = nf1 is not None
+        self.xf1d = gauss_legendre(nf)[0]
+        self.nfacefpts = [nf*nf]*6
+        self.nfpts = 6*nf*nf
         self.nq1 = nq1 or order + 2
         self.nqpts = self.nq1**3
 
@@ -190,7 +210,7 @@ class HexShape:
 
     @cached_property
     def fpts(self):
-        x, n = self.x1d, self.n
+        x, n = self.xf1d, self.nf1
         pts = []
         for nax, side, sax, tax in HEX_FACES:
             for b in range(n):
@@ -203,13 +223,14 @@ class HexShape:
     @cached_property
     def norm_fpts(self):
         out = np.zeros((self.nfpts, 3))
+        m = self.nf1**2
         for f, (nax, side, _, _) in enumerate(HEX_FACES):
-            out[f*self.n**2:(f + 1)*self.n**2, nax] = side
+            out[f*m:(f + 1)*m, nax] = side
         return out
 
     @cached_property
     def facefpts(self):
-        m = self.n**2
+        m = self.nf1**2
         return [list(range(f*m, (f + 1)*m)) for f in range(6)]
 
     def fpt_line(self, f, a, b):
@@ -242,12 +263,33 @@ class HexShape:
     @cached_property
     def m0(self):
         n = self.n
+        if self.surf_flux:
+            return self._m0_surf
         m = np.zeros((self.nfpts, self.nupts))
         for f in range(6):
             for b in range(n):
                 for a in range(n):
                     line, _, side = self.fpt_line(f, a, b)
                     m[f*n*n + a + n*b, line] = self.lend[side]
+        return m
+
+    @cached_property
+    def _m0_surf(self):
+        """Interpolation to face points that are not on the solution-point
+        lines: the normal end value times the two in-plane 1-D Lagrange
+        interpolants (the tensor form of ``ubasis.nodal_basis_at(fpts)``,
+        ``pyfr/shapes.py:89-91``)."""
+        n, nf = self.n, self.nf1
+        L = lagrange_at(self.x1d, self.xf1d)             # (nf, n)
+        m = np.zeros((self.nfpts, self.nupts))
+        for f, (nax, side, sax, tax) in enumerate(HEX_FACES):
+            for b, a in it.product(range(nf), repeat=2):
+                row = f*nf*nf + a + nf*b
+                for l, s, t in it.product(range(n), repeat=3):
+                    c = [0, 0, 0]
+                    c[nax], c[sax], c[tax] = l, s, t
+                    m[row, self.uidx(*c)] = (self.lend[side][l]*L[a, s] *
+                                             L[b, t])
         return m
 
     @cached_property
@@ -273,12 +315,23 @@ class HexShape:
     @cached_property
     def m3(self):
         n = self.n
-        m = np.zeros((self.nupts, self.nfpts))
+        m = np.zeros((self.nupts, 6*n*n))
         for f in range(6):
             for b in range(n):
                 for a in range(n):
                     line, _, side = self.fpt_line(f, a, b)
                     m[line, f*n*n + a + n*b] = self.hend[side]
+        if self.surf_flux:
+            # correction from the standard face points, fed by the L2
+            # projection of each face's quadrature-point values onto the
+            # order-p face basis: m3 @ block_diag(proj_l2(iqrule, facebasis))
+            # (pyfr/shapes.py:103-113, :18-19)
+            P = l2_projection_1d(self.x1d, self.nf1)
+            fp = np.kron(P, P)
+            bd = np.zeros((6*n*n, self.nfpts))
+            for f in range(6):
+                bd[f*n*n:(f + 1)*n*n, f*fp.shape[1]:(f + 1)*fp.shape[1]] = fp
+            m = m @ bd
         return m
 
     @cached_property
@@ -310,13 +363,7 @@ class HexShape:
         points of sum_k psi_k(x) sum_q w_q psi_k(x_q) f_q with the
         orthonormal Legendre products psi_k, a Kronecker product of 1-D
         projections for the tensor-product space."""
-        xq, wq = gauss_legendre(self.nq1)
-        P = np.zeros((self.n, self.nq1))
-        for i, xi in enumerate(self.x1d):
-            for q, (xqq, wqq) in enumerate(zip(xq, wq)):
-                P[i, q] = wqq*sum((k + 0.5)*_legendre(k, float(xi))[0] *
-                                  _legendre(k, float(xqq))[0]
-                                  for k in range(self.n))
+        P = l2_projection_1d(self.x1d, self.nq1)
         return np.kron(P, np.kron(P, P))
 
     @property

@@ -25,7 +25,8 @@ written against ``pyfr/backends/base``:
 Scope: meshes of hexahedra, straight-sided tetrahedra and right prisms on
 one GPU, periodic or with the reference's fixed-value boundary conditions
 (not the controlled char-riem-inv variants nor time-dependent boundary
-values); no MPI interfaces, no anti-aliasing, no source terms.
+values); anti-aliasing (flux, surf-flux) on hexes only; no MPI interfaces,
+no source terms.
 """
 from functools import cached_property
 import math
@@ -101,15 +102,16 @@ class FluidElements:
         self.antialias = {a.strip() for a in aa.split(',')} - {'none'}
         if self.antialias - {'flux', 'surf-flux'}:
             raise ValueError('Invalid anti-alias options')
-        if 'surf-flux' in self.antialias:
-            raise NotImplementedError('surface-flux anti-aliasing is not '
-                                      'supported')
         self.fluxaa = 'flux' in self.antialias
-        if self.fluxaa and basiscls is not SHAPES['hex']:
-            raise NotImplementedError('flux anti-aliasing is supported for '
-                                      'hexes only')
-        if self.fluxaa:
-            self.basis = basis = basiscls(order, self._aa_nq1(cfg, order))
+        if self.antialias and basiscls is not SHAPES['hex']:
+            raise NotImplementedError('anti-aliasing is supported for hexes '
+                                      'only')
+        if self.antialias:
+            nq1 = (self._gl_n1(cfg, 'solver-elements-hex', 3, order)
+                   if self.fluxaa else None)
+            nf1 = (self._gl_n1(cfg, 'solver-interfaces-quad', 2, order)
+                   if 'surf-flux' in self.antialias else None)
+            self.basis = basis = basiscls(order, nq1, nf1)
         else:
             self.basis = basis = basiscls(order)
         self.name = basis.name
@@ -123,21 +125,22 @@ class FluidElements:
             self.geom = HexGeometry(eles, order)
 
     @staticmethod
-    def _aa_nq1(cfg, order):
-        """Points per direction of the element quadrature rule
-        (``pyfr/shapes.py:147-176``): Gauss-Legendre, by default
-        (order + 2)^3 points, or [solver-elements-hex] quad-npts."""
-        sect = 'solver-elements-hex'
+    def _gl_n1(cfg, sect, ndims, order):
+        """Points per direction of a Gauss-Legendre anti-aliasing rule
+        (``pyfr/shapes.py:147-185``): the element rule of flux
+        anti-aliasing ([solver-elements-hex], ndims 3) or the face rule of
+        surface-flux anti-aliasing ([solver-interfaces-quad], ndims 2); by
+        default (order + 2)^ndims points, or quad-npts."""
         if cfg.hasopt(sect, 'quad-deg'):
             raise NotImplementedError('quad-deg rule selection is not '
                                       'supported; give quad-npts')
         if cfg.get(sect, 'quad-pts', 'gauss-legendre') != 'gauss-legendre':
             raise NotImplementedError('only Gauss-Legendre quadrature '
                                       'points are supported')
-        npts = cfg.getint(sect, 'quad-npts', (order + 2)**3)
-        n1 = round(npts ** (1/3))
-        if n1**3 != npts or n1 < order + 1:
-            raise ValueError('Invalid quad-npts for a hex')
+        npts = cfg.getint(sect, 'quad-npts', (order + 2)**ndims)
+        n1 = round(npts ** (1/ndims))
+        if n1**ndims != npts or n1 < order + 1:
+            raise ValueError(f'Invalid quad-npts in [{sect}]')
         return n1
 
     # -- initial conditions (pyfr/solvers/base/elements.py:86-129)
