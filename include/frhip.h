@@ -230,15 +230,20 @@ int frhip_intconu(int dtype, int n, const void *ubase, const int32_t *ulin,
 
 /* NS intcflux: pyfr/solvers/navstokes/kernels/intcflux.mako:10-54.
  * gbase/glmap/grmap address the physical gradient at the flux points with row
- * stride gstride between dimensions.  consts = {gamma, mu, Pr, beta, tau,
- * sutherland(0/1), cpTref, cpTs}.  mu_p: optional per-pair viscosity (n
- * values), NULL for the reference behaviour. */
+ * stride gstride between dimensions, or, where glstr / grstr are given, the
+ * per-point strides of the left / right view (n values each; a view over
+ * several element types, whose gradient banks differ in row count: the
+ * reference's per-point View.rstrides, pyfr/backends/base/types.py:313-320).
+ * consts = {gamma, mu, Pr, beta, tau, sutherland(0/1), cpTref, cpTs}.  mu_p:
+ * optional per-pair viscosity (n values), NULL for the reference
+ * behaviour. */
 int frhip_intcflux_ns(int dtype, int rsolver, int n, void *ubase,
                       const int32_t *lmap, const int32_t *rmap,
                       const void *gbase, const int32_t *glmap,
-                      const int32_t *grmap, long gstride, const void *nl,
-                      long ldn, const double *consts, const void *mu_p,
-                      void *stream);
+                      const int32_t *grmap, long gstride,
+                      const int32_t *glstr, const int32_t *grstr,
+                      const void *nl, long ldn, const double *consts,
+                      const void *mu_p, void *stream);
 
 /* ---- boundary interfaces, 1-D over n boundary flux points ----
  * Types: the `type` keys of [soln-bcs-*] (pyfr/solvers/euler/inters.py:57-102,
@@ -275,13 +280,15 @@ int frhip_bcconu(int dtype, int bctype, int n, const void *ubase,
 
 /* NS bccflux: pyfr/solvers/navstokes/kernels/bccflux.mako:10-20 with the
  * type's common-flux state (bcs/ghost.mako, bcs/ghost-imperm.mako for the
- * no-slip walls, bcs/slp-adia-wall.mako:21-37).  Gradient view as in
- * frhip_intcflux_ns (unused, may be NULL, for slp-adia-wall). */
+ * no-slip walls, bcs/slp-adia-wall.mako:21-37).  Gradient view (gstride or
+ * per-point glstr) as in frhip_intcflux_ns (unused, may be NULL, for
+ * slp-adia-wall). */
 int frhip_bccflux_ns(int dtype, int bctype, int rsolver, int n, void *ubase,
                      const int32_t *lmap, const void *gbase,
-                     const int32_t *glmap, long gstride, const void *nl,
-                     long ldn, const void *bcv, long ldb,
-                     const double *consts, void *stream);
+                     const int32_t *glmap, long gstride,
+                     const int32_t *glstr, const void *nl, long ldn,
+                     const void *bcv, long ldb, const double *consts,
+                     void *stream);
 
 /* ---- BLAS extensions: pyfr/backends/hip/blasext.py:8-136 ---- */
 

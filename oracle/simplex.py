@@ -1,9 +1,12 @@
-"""Oracle: Euler FR right-hand side on meshes of straight-sided tetrahedra
-and prisms (test infrastructure).
+"""Oracle: Euler and Navier-Stokes FR right-hand sides on meshes of
+straight-sided tetrahedra and prisms (test infrastructure).
 
-Same kernel decomposition as the hex oracle (``pyfr/solvers/baseadvec/
-system.py:57-136``): disu -> intcflux -> tdisf -> tdivtpcorf -> tdivtconf ->
-negdivconf, with the TetShape / PriShape operators of ``oracle.basis``.
+Same kernel decompositions as the hex oracle: Euler ``pyfr/solvers/
+baseadvec/system.py:57-136`` (disu -> intcflux -> tdisf -> tdivtpcorf ->
+tdivtconf -> negdivconf) and NS ``pyfr/solvers/baseadvecdiff/system.py:
+53-231`` (disu -> intconu -> tgradpcoru/tgradcoru -> tdisf_fused ->
+gradcoru_fpts -> intcflux -> divergence), with the TetShape / PriShape
+operators of ``oracle.basis`` and the boundary conditions of ``oracle.bcs``.
 
 Geometry follows the reference's pipeline for linear elements
 (``pyfr/solvers/base/elements.py:331-440``): physical points and the
@@ -16,7 +19,9 @@ periods), independent of the product's node-based connectivity.
 import numpy as np
 
 from oracle.basis import PriBasis, SimplexShapeOracle, TetBasis
-from oracle.physics import RSOLVERS, inviscid_flux, pri_to_con
+from oracle.bcs import bccflux_euler, bccflux_ns, bcconu
+from oracle.physics import RSOLVERS, inviscid_flux, pri_to_con, \
+    viscous_flux_add
 
 STD_VERTS = {
     'tet': np.array([(-1, -1, -1), (1, -1, -1), (-1, 1, -1), (-1, -1, 1)],
@@ -39,6 +44,8 @@ class _Etype:
         self.M0 = sh.opmat('M0')
         self.M132 = sh.opmat('M1 - M3*M2')
         self.M3 = sh.opmat('M3')
+        self.M460 = sh.opmat('M4 - M6*M0')
+        self.M6 = sh.opmat('M6')
 
         # Jacobian at the solution points: j[a][b] = dX_b/dxi_a
         dl = lb.jac_nodal_basis_at(sh.upts)          # (3, nv, Nu)
@@ -69,22 +76,34 @@ class _Etype:
 
 
 def _wrap(x, period):
-    y = np.mod(x, period)
-    y[np.isclose(y, period, atol=1e-9, rtol=0)] = 0.0
+    """x modulo period along axes with a finite period (inf: no wrap)."""
+    period = np.broadcast_to(period, x.shape)
+    fin = np.isfinite(period)
+    y = np.array(x, dtype=float)
+    y[fin] = np.mod(y[fin], period[fin])
+    y[fin & np.isclose(y, np.where(fin, period, 0), atol=1e-9, rtol=0)] = 0.0
     return y
 
 
-def pair_faces_mixed(etypes, blocks):
+def pair_faces_mixed(etypes, blocks, periodic=(True, True, True),
+                     boundaries=None):
     """Brute-force pairing of faces (and their flux points) by physical
     coordinates.  blocks: list of (periods (3,), {etype: element indices});
-    faces pair only within a block.  Returns a list of
-    (tL, eL, idxL, tR, eR, idxR) with flux-point index arrays in matched
-    order; the left side is the smaller (etype order, face, element) key, the
-    reference's lkey < rkey rule (pyfr/readers/native.py:404-406, :466-470)."""
+    faces pair only within a block, wrapping only along periodic axes.
+    Returns a list of (tL, eL, idxL, tR, eR, idxR) with flux-point index
+    arrays in matched order; the left side is the smaller (etype order,
+    face, element) key, the reference's lkey < rkey rule
+    (pyfr/readers/native.py:404-406, :466-470).  An unpaired face must lie
+    on a non-periodic side of its block (coordinate 0 or the block's
+    extent, ``periods``); with ``boundaries`` (a dict) such faces are
+    recorded as boundaries[side] = [(etype, element, fpt indices)], side
+    'x0', 'x1', 'y0', ...; without it they are an error."""
     order = {n: i for i, n in enumerate(etypes)}
+    per = np.asarray(periodic, dtype=bool)
     pairs = []
     for periods, members in blocks:
         periods = np.asarray(periods, dtype=float)
+        wrap = np.where(per, periods, np.inf)
         keys = {}
         for tn, eidx in members.items():
             et = etypes[tn]
@@ -93,10 +112,23 @@ def pair_faces_mixed(etypes, blocks):
             for e in eidx:
                 for f in range(len(et.sh.nfacefpts)):
                     pts = pl[off[f]:off[f + 1], :, e]
-                    c = _wrap(pts.mean(axis=0), periods)
+                    c = _wrap(pts.mean(axis=0), wrap)
                     key = tuple(np.round(c, 6))
                     keys.setdefault(key, []).append((order[tn], tn, e, f))
         for key, faces in keys.items():
+            if len(faces) == 1 and boundaries is not None:
+                _, tn, e, f = faces[0]
+                side = None
+                for ax in np.flatnonzero(~per):
+                    for hi, x in enumerate((0.0, periods[ax])):
+                        if abs(key[ax] - x) < 1e-6:
+                            side = f'{"xyz"[ax]}{hi}'
+                if side is None:
+                    raise ValueError(f'Unpaired interior face at {key}')
+                off = np.cumsum([0] + etypes[tn].sh.nfacefpts)
+                boundaries.setdefault(side, []).append(
+                    (tn, e, np.arange(off[f], off[f + 1])))
+                continue
             if len(faces) != 2:
                 raise ValueError(f'Unpaired face at {key}: {faces}')
             (_, tL, eL, fL), (_, tR, eR, fR) = sorted(
@@ -105,11 +137,11 @@ def pair_faces_mixed(etypes, blocks):
             offl = np.cumsum([0] + sl.nfacefpts)
             offr = np.cumsum([0] + sr.nfacefpts)
             pl = _wrap(etypes[tL].ploc(sl.fpts[offl[fL]:offl[fL + 1]])[..., eL],
-                       periods)
+                       wrap)
             pr = _wrap(etypes[tR].ploc(sr.fpts[offr[fR]:offr[fR + 1]])[..., eR],
-                       periods)
+                       wrap)
             dd = np.abs(pl[:, None, :] - pr[None, :, :])
-            dd = np.minimum(dd, periods - dd).sum(axis=2)
+            dd = np.where(per, np.minimum(dd, periods - dd), dd).sum(axis=2)
             jr = np.argmin(dd, axis=1)
             if (np.any(dd[np.arange(len(pl)), jr] > 1e-8) or
                     len(set(jr.tolist())) != len(pl)):
@@ -120,23 +152,72 @@ def pair_faces_mixed(etypes, blocks):
 
 
 class OracleMixedSystem:
-    """Euler RHS on a mesh of tets and/or prisms.
+    """Euler or NS RHS on a mesh of tets and/or prisms.
 
     verts: {etype: (nverts, 3, E)}; tables: {etype: tables for
-    SimplexShapeOracle}; blocks as for ``pair_faces_mixed``."""
+    SimplexShapeOracle}; blocks as for ``pair_faces_mixed``.  constants:
+    gamma (Euler), plus mu, Pr, ldg-beta, ldg-tau (NS).  periodic: per-axis
+    periodicity; each side of a non-periodic axis needs ``bcs[side] =
+    (type, vals)`` as for the hex oracle (vals may hold callables of the
+    flux-point coordinates)."""
 
     def __init__(self, verts, order, tables, blocks, rsolver='rusanov',
-                 gamma=1.4):
+                 gamma=1.4, system='euler', constants=None,
+                 periodic=(True, True, True), bcs=None):
         self.names = list(verts)
         self.et = {n: _Etype(n, order, tables[n], verts[n])
                    for n in self.names}
-        self.gamma = gamma
+        self.system = system
+        self.c = dict(constants or {'gamma': gamma})
+        self.gamma = self.c['gamma']
         self.rsolver = RSOLVERS[rsolver]
-        pairs = pair_faces_mixed(self.et, blocks)
+        bfaces = {}
+        pairs = pair_faces_mixed(self.et, blocks, periodic, bfaces)
         self.npairs_faces = len(pairs)
-        # flat flux-point pairs (etype, element, fpt) per side
-        self.L = [(p[0], p[1], p[2]) for p in pairs]
-        self.R = [(p[3], p[4], p[5]) for p in pairs]
+
+        # one flat flux-point index over all element types: (type, fpt,
+        # element) -> off[type] + fpt*E + element
+        self._off, o = {}, 0
+        for n in self.names:
+            self._off[n] = o
+            o += self.et[n].nfpts*self.et[n].neles
+        self._nflat = o
+        g = self._gidx
+        self.gl = np.concatenate([g(p[0], p[2], p[1]) for p in pairs])
+        self.gr = np.concatenate([g(p[3], p[5], p[4]) for p in pairs])
+        pn = self._flat({n: self.et[n].pnorm.transpose(0, 2, 1)
+                         for n in self.names}, 3)    # (nflat, 3)
+        self.nl = pn[self.gl].T                      # (3, npts)
+        ploc = self._flat({n: self.et[n].ploc(self.et[n].sh.fpts)
+                           for n in self.names}, 3)
+        self.bcs = []
+        for side, faces in sorted(bfaces.items()):
+            bctype, vals = (bcs or {})[side]
+            gb = np.concatenate([g(t, i, e) for t, e, i in faces])
+            x, y, z = ploc[gb].T
+            v = {k: (fn(x, y, z) if callable(fn) else fn)
+                 for k, fn in vals.items()}
+            self.bcs.append((side, bctype, gb, pn[gb].T, v))
+
+    def _gidx(self, t, fpts, e):
+        return self._off[t] + np.asarray(fpts)*self.et[t].neles + e
+
+    def _flat(self, d, nv=5):
+        """{etype: (..., Nf, nv, E)} -> (..., nflat, nv)"""
+        return np.concatenate(
+            [np.moveaxis(d[n], -1, -2).reshape(*d[n].shape[:-3], -1, nv)
+             for n in self.names], axis=-2)
+
+    def _unflat(self, a, nv=5):
+        out = {}
+        for n in self.names:
+            et = self.et[n]
+            k = et.nfpts*et.neles
+            o = self._off[n]
+            b = a[..., o:o + k, :].reshape(*a.shape[:-2], et.nfpts,
+                                           et.neles, nv)
+            out[n] = np.moveaxis(b, -2, -1)
+        return out
 
     def ics(self, fns):
         """fns: dict of callables rho, u, v, w, p of (x, y, z)."""
@@ -152,36 +233,111 @@ class OracleMixedSystem:
         return (A @ B.reshape(A.shape[1], -1)).reshape(A.shape[0],
                                                         *B.shape[1:])
 
-    def rhs(self, u):
-        """u: {etype: (Nu, 5, E)} -> {etype: (Nu, 5, E)}"""
-        uf = {n: self._mul(self.et[n].M0, u[n]) for n in self.names}
-        # interfaces (intcflux.mako:6-22) with the left side's normal
-        lv, rv, nl = [], [], []
-        for (tl, el, il), (tr, er, ir) in zip(self.L, self.R):
-            lv.append(uf[tl][il, :, el])
-            rv.append(uf[tr][ir, :, er])
-            nl.append(self.et[tl].pnorm[il, el])
-        ul, ur = np.concatenate(lv).T, np.concatenate(rv).T
-        nl = np.concatenate(nl).T
-        mag = np.sqrt(nl[0]*nl[0] + nl[1]*nl[1] + nl[2]*nl[2])
-        fn = self.rsolver(ul, ur, nl/mag, self.gamma)
-        q = 0
-        for (tl, el, il), (tr, er, ir) in zip(self.L, self.R):
-            k = len(il)
-            uf[tl][il, :, el] = (mag[q:q + k]*fn[:, q:q + k]).T
-            uf[tr][ir, :, er] = (-mag[q:q + k]*fn[:, q:q + k]).T
-            q += k
+    def _transform(self, et, f):
+        """F[i] = sum_k S[i][k] f[k] (tflux.mako:24-28); f (3, 5, Nu, E)."""
+        F = np.zeros((3, et.nupts, 5, et.neles))
+        for i in range(3):
+            for k in range(3):
+                F[i] += et.smats[:, i, k][:, None, :]*f[k].transpose(1, 0, 2)
+        return F
 
+    def _divergence(self, u, F, UF):
+        uf = self._unflat(UF)
         out = {}
         for n in self.names:
             et = self.et[n]
-            f, _, _ = inviscid_flux(u[n].transpose(1, 0, 2), self.gamma)
-            F = np.zeros((3, et.nupts, 5, et.neles))
-            for i in range(3):
-                for k in range(3):
-                    F[i] += et.smats[:, i, k][:, None, :] * \
-                        f[k].transpose(1, 0, 2)
-            div = self._mul(et.M132, F.reshape(3*et.nupts, 5, -1))
+            div = self._mul(et.M132, F[n].reshape(3*et.nupts, 5, -1))
             div = self._mul(et.M3, uf[n]) + div
             out[n] = -et.rcpdjac[:, None, :]*div
         return out
+
+    def rhs(self, u):
+        """u: {etype: (Nu, 5, E)} -> {etype: (Nu, 5, E)}"""
+        if self.system == 'navier-stokes':
+            return self.rhs_ns(u)
+        UF = self._flat({n: self._mul(self.et[n].M0, u[n])
+                         for n in self.names})
+        # interfaces (intcflux.mako:6-22) with the left side's normal
+        ul, ur, nl = UF[self.gl].T, UF[self.gr].T, self.nl
+        mag = np.sqrt(nl[0]*nl[0] + nl[1]*nl[1] + nl[2]*nl[2])
+        fn = self.rsolver(ul, ur, nl/mag, self.gamma)
+        bout = [bccflux_euler(bt, self.rsolver, UF[gb].T, bn, v, self.c)
+                for _, bt, gb, bn, v in self.bcs]
+        UF[self.gl] = (mag*fn).T
+        UF[self.gr] = (-mag*fn).T
+        for (_, _, gb, _, _), o in zip(self.bcs, bout):
+            UF[gb] = o.T
+        F = {}
+        for n in self.names:
+            f, _, _ = inviscid_flux(u[n].transpose(1, 0, 2), self.gamma)
+            F[n] = self._transform(self.et[n], f)
+        return self._divergence(u, F, UF)
+
+    def rhs_ns(self, u):
+        """``pyfr/solvers/baseadvecdiff/system.py:53-231`` with the
+        interface kernels ``intconu.mako:4-19`` and
+        ``navstokes/kernels/intcflux.mako:10-54``."""
+        c = self.c
+        beta, tau = c['ldg-beta'], c['ldg-tau']
+        UF = self._flat({n: self._mul(self.et[n].M0, u[n])
+                         for n in self.names})                 # disu
+        ul, ur = UF[self.gl].T, UF[self.gr].T
+        COMM = np.zeros_like(UF)                               # intconu
+        cu = ur*(0.5 + beta) + ul*(0.5 - beta)
+        COMM[self.gl] = cu.T
+        COMM[self.gr] = cu.T
+        for _, bt, gb, bn, v in self.bcs:                      # bcconu
+            COMM[gb] = bcconu(bt, UF[gb].T, bn, v, c).T
+        comm = self._unflat(COMM)
+
+        F, GF = {}, []
+        for n in self.names:
+            et = self.et[n]
+            grad = self._mul(et.M460, u[n])                    # tgradpcoru
+            grad = self._mul(et.M6, comm[n]) + grad            # tgradcoru
+            grad = grad.reshape(3, et.nupts, 5, et.neles)
+            # transform_grad.mako:3-10
+            s = et.smats
+            gp = np.zeros_like(grad)
+            for i in range(3):
+                acc = 0
+                for k in range(3):
+                    acc = acc + s[:, k, i][:, None, :]*grad[k]
+                gp[i] = et.rcpdjac[:, None, :]*acc
+            f, _, _ = inviscid_flux(u[n].transpose(1, 0, 2), c['gamma'])
+            viscous_flux_add(u[n].transpose(1, 0, 2),
+                             gp.transpose(0, 2, 1, 3), f, c)
+            F[n] = self._transform(et, f)
+            GF.append(np.stack([self._mul(et.M0, gp[d])
+                                for d in range(3)]))   # gradcoru_fpts
+        GF = self._flat(dict(zip(self.names, GF)))             # (3, nflat, 5)
+
+        gl = GF[:, self.gl].transpose(0, 2, 1)                 # (3, 5, np)
+        gr = GF[:, self.gr].transpose(0, 2, 1)
+        nl = self.nl
+        mag = np.sqrt(nl[0]*nl[0] + nl[1]*nl[1] + nl[2]*nl[2])
+        nn = (1/mag)*nl
+        fi = self.rsolver(ul, ur, nn, c['gamma'])
+        fvl = viscous_flux_add(ul, gl, np.zeros((3, 5, ul.shape[1])), c)
+        fvr = viscous_flux_add(ur, gr, np.zeros((3, 5, ur.shape[1])), c)
+        out = np.empty_like(fi)
+        for i in range(5):
+            nfl = nn[0]*fvl[0, i] + nn[1]*fvl[1, i] + nn[2]*fvl[2, i]
+            nfr = nn[0]*fvr[0, i] + nn[1]*fvr[1, i] + nn[2]*fvr[2, i]
+            if beta == -0.5:
+                fv = nfr
+            elif beta == 0.5:
+                fv = nfl
+            else:
+                fv = (0.5 + beta)*nfl + (0.5 - beta)*nfr
+            if tau != 0.0:
+                fv = fv + tau*(ul[i] - ur[i])
+            out[i] = fi[i] + fv
+        bout = [bccflux_ns(bt, self.rsolver, UF[gb].T,
+                           GF[:, gb].transpose(0, 2, 1), bn, v, c)
+                for _, bt, gb, bn, v in self.bcs]
+        UF[self.gl] = (mag*out).T
+        UF[self.gr] = (-mag*out).T
+        for (_, _, gb, _, _), o in zip(self.bcs, bout):        # bccflux
+            UF[gb] = o.T
+        return self._divergence(u, F, UF)

@@ -75,13 +75,13 @@ _FUNCS = {
     'frhip_intcflux_euler': [_I, _I, _I, _P, _PI32, _PI32, _P, _L, _D, _P],
     'frhip_intconu': [_I, _I, _P, _PI32, _PI32, _P, _PI32, _PI32, _D, _P],
     'frhip_intcflux_ns': [_I, _I, _I, _P, _PI32, _PI32, _P, _PI32, _PI32,
-                          _L, _P, _L, _P, _P, _P],
+                          _L, _PI32, _PI32, _P, _L, _P, _P, _P],
     'frhip_bccflux_euler': [_I, _I, _I, _I, _P, _PI32, _P, _L, _P, _L, _P,
                             _P],
     'frhip_bcconu': [_I, _I, _I, _P, _PI32, _P, _PI32, _P, _L, _P, _L, _P,
                      _P],
-    'frhip_bccflux_ns': [_I, _I, _I, _I, _P, _PI32, _P, _PI32, _L, _P, _L,
-                         _P, _L, _P, _P],
+    'frhip_bccflux_ns': [_I, _I, _I, _I, _P, _PI32, _P, _PI32, _L, _PI32,
+                         _P, _L, _P, _L, _P, _P],
     'frhip_axnpby': [_I, _I, _L, _L, _I, _P, _P, _P],
     'frhip_axnpby_pool': [_I, _I, _L, _L, _I, _P, _P, ct.c_uint, _P, _I, _P],
     'frhip_rkvdh2': [_I, _I, _L, _L, _P, _P, _D, _D, _I, _P],

@@ -242,9 +242,13 @@ class View:
 
     ``mapping`` holds, per point, the element offset (from ``basedata``) of
     variable 0 in the AoSoA layout; variable v is at +64*v and, for vector
-    views, dimension d at +d*rstride."""
+    views, dimension d at +d*rstride.  The row stride is per point
+    (``rstrides_np``); ``rstrides_val`` is its value where all points
+    share it, else 0 (a view over element types with different point
+    counts), and ``rstrides_mat()`` the per-point device array."""
 
     def __init__(self, backend, matmap, rmap, cmap, rstridemap, vshape, tags):
+        self._backend, self._tags = backend, tags
         self.n = len(matmap)
         self.nvrow = vshape[-2] if len(vshape) == 2 else 1
         self.nvcol = vshape[-1] if len(vshape) >= 1 else 1
@@ -276,13 +280,35 @@ class View:
         self.mapping = backend.const_matrix(self.mapping_np[None, :],
                                             dtype=np.int32, tags=tags)
         if self.nvrow > 1:
-            rstrides = (np.asarray(rstridemap)*leaddim)
-            self.rstrides_val = int(np.atleast_1d(rstrides).flat[0])
-            if np.any(rstrides != self.rstrides_val):
-                raise ValueError('Non-uniform row strides are unsupported')
+            rstrides = np.broadcast_to(np.asarray(rstridemap)*leaddim,
+                                       (self.n,))
+            if rstrides.size and (mapping + (self.nvrow - 1)*rstrides).max() \
+                    >= 2**31:
+                raise ValueError('View offsets exceed the int32 index range')
+            self.rstrides_np = rstrides.astype(np.int32)
+            u = np.unique(self.rstrides_np)
+            self.rstrides_val = int(u[0]) if len(u) == 1 else 0
         else:
+            self.rstrides_np = None
             self.rstrides_val = 0
         self.rstrides = None
+
+    def rstrides_mat(self):
+        """The per-point row strides as a device int32 array (made on first
+        use)."""
+        if self.rstrides is None:
+            self.rstrides = self._backend.const_matrix(
+                self.rstrides_np[None, :], dtype=np.int32, tags=self._tags)
+        return self.rstrides
+
+    @staticmethod
+    def stride_args(*views):
+        """(uniform stride, None, ...) where the views share one stride at
+        every point, else (0, per-point stride arrays...)."""
+        vals = {v.rstrides_val for v in views}
+        if len(vals) == 1 and 0 not in vals:
+            return (vals.pop(), *[None]*len(views))
+        return (0, *[v.rstrides_mat().data for v in views])
 
 
 # ---------------------------------------------------------------------------
@@ -798,15 +824,14 @@ class PointwiseProvider:
                           views=(ul, ur), mats=(nl,), name='intcflux')
         if artvisc is not None:
             raise NotSuitableError('artificial viscosity is not supported')
-        if gradul.rstrides_val != gradur.rstrides_val:
-            raise ValueError('Gradient views must share a row stride')
+        gs = View.stride_args(gradul, gradur)
         consts = (ct.c_double*8)(
             c['gamma'], c['mu'], c['Pr'], c['ldg-beta'], c['ldg-tau'],
             1.0 if tplargs.get('visc_corr') == 'sutherland' else 0.0,
             c.get('cpTref', 1.0), c.get('cpTs', 0.0))
         args = (dt, rs, n, ul.basedata, ul.mapping.data, ur.mapping.data,
                 gradul.basedata, gradul.mapping.data, gradur.mapping.data,
-                gradul.rstrides_val, nl.data, nl.leaddim, consts, None)
+                *gs, nl.data, nl.leaddim, consts, None)
         return Kernel(lambda stream: lib.intcflux_ns(*args, stream),
                       views=(ul, ur, gradul, gradur), mats=(nl,),
                       name='intcflux_ns')
@@ -861,8 +886,9 @@ class PointwiseProvider:
                           views=(ul,), mats=(nl, bcvals), name='bccflux')
         if artvisc is not None:
             raise NotSuitableError('artificial viscosity is not supported')
-        g = (None, None, 0) if gradul is None else \
-            (gradul.basedata, gradul.mapping.data, gradul.rstrides_val)
+        g = (None, None, 0, None) if gradul is None else \
+            (gradul.basedata, gradul.mapping.data,
+             *View.stride_args(gradul))
         args = (dt, bct, rs, n, ul.basedata, ul.mapping.data, *g, nl.data,
                 nl.leaddim, bv, ldb, consts)
         return Kernel(lambda stream: lib.bccflux_ns(*args, stream),

@@ -248,8 +248,9 @@ template <typename T, int RS, int BC>
 __global__ __launch_bounds__(256) void k_bccflux_ns(
     int n, T *__restrict__ base, const int32_t *__restrict__ lmap,
     const T *__restrict__ gb, const int32_t *__restrict__ glmap,
-    long gstride, const T *__restrict__ nl, long ldn,
-    const T *__restrict__ bcv, long ldb, BCConsts<T> k)
+    long gstride, const int32_t *__restrict__ glstr,
+    const T *__restrict__ nl, long ldn, const T *__restrict__ bcv, long ldb,
+    BCConsts<T> k)
 {
     const int i = blockIdx.x*blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -267,11 +268,11 @@ __global__ __launch_bounds__(256) void k_bccflux_ns(
     else {
         constexpr bool imperm = BC == FRHIP_BC_NO_SLP_ADIA_WALL ||
                                 BC == FRHIP_BC_NO_SLP_ISOT_WALL;
-        const long gli = glmap[i];
+        const long gli = glmap[i], gs = glstr ? glstr[i] : gstride;
         T gl[3][5], gr[3][5], url[5], fvr[3][5] = {};
         for (int d = 0; d < 3; ++d)
             for (int v = 0; v < 5; ++v)
-                gl[d][v] = gb[gli + d*gstride + v*SOA];
+                gl[d][v] = gb[gli + d*gs + v*SOA];
         bc_state<T, BC, true>(ul, nn, bv, k, url);
         bc_grad_state<T, BC>(imperm ? url : ul, nn, gl, gr);
         viscous_flux_add(url, gr, fvr, k.ns);
@@ -368,9 +369,10 @@ int frhip_bcconu(int dtype, int bctype, int n, const void *ub,
 
 int frhip_bccflux_ns(int dtype, int bctype, int rsolver, int n, void *base,
                      const int32_t *lmap, const void *gb,
-                     const int32_t *glmap, long gstride, const void *nl,
-                     long ldn, const void *bcv, long ldb,
-                     const double *consts, void *stream)
+                     const int32_t *glmap, long gstride,
+                     const int32_t *glstr, const void *nl, long ldn,
+                     const void *bcv, long ldb, const double *consts,
+                     void *stream)
 {
     FR_API_BEGIN
     FR_DTYPE_OK(dtype);
@@ -392,7 +394,7 @@ int frhip_bccflux_ns(int dtype, int bctype, int rsolver, int n, void *base,
                     (k_bccflux_ns<T, decltype(rs)::value,
                                   decltype(bc)::value>),
                     grid, block, 0, as_stream(stream), n, (T *)base, lmap,
-                    (const T *)gb, glmap, gstride, (const T *)nl, ldn,
+                    (const T *)gb, glmap, gstride, glstr, (const T *)nl, ldn,
                     (const T *)bcv, ldb, k);
             });
         });

@@ -309,13 +309,17 @@ __global__ __launch_bounds__(256) void k_intcflux_ns(
     int n, T *__restrict__ base, const int32_t *__restrict__ lmap,
     const int32_t *__restrict__ rmap, const T *__restrict__ gb,
     const int32_t *__restrict__ glmap, const int32_t *__restrict__ grmap,
-    long gstride, const T *__restrict__ nl, long ldn, NSConsts<T> c0,
-    const T *__restrict__ mu_p)
+    long gstride, const int32_t *__restrict__ glstr,
+    const int32_t *__restrict__ grstr, const T *__restrict__ nl, long ldn,
+    NSConsts<T> c0, const T *__restrict__ mu_p)
 {
     const int i = blockIdx.x*blockDim.x + threadIdx.x;
     if (i >= n) return;
     const NSConsts<T> c = mu_p ? c0.with_mu(mu_p[i]) : c0;
     const long li = lmap[i], ri = rmap[i], gli = glmap[i], gri = grmap[i];
+    // per-point dimension strides of a view over several element types
+    const long gsl = glstr ? glstr[i] : gstride;
+    const long gsr = grstr ? grstr[i] : gstride;
     T ul[5], ur[5];
     for (int v = 0; v < 5; ++v) {
         ul[v] = base[li + v*SOA];
@@ -333,14 +337,14 @@ __global__ __launch_bounds__(256) void k_intcflux_ns(
         T gl[3][5];
         for (int d = 0; d < 3; ++d)
             for (int v = 0; v < 5; ++v)
-                gl[d][v] = gb[gli + d*gstride + v*SOA];
+                gl[d][v] = gb[gli + d*gsl + v*SOA];
         viscous_flux_add(ul, gl, fvl, c);
     }
     if (c.beta != T(0.5)) {
         T gr[3][5];
         for (int d = 0; d < 3; ++d)
             for (int v = 0; v < 5; ++v)
-                gr[d][v] = gb[gri + d*gstride + v*SOA];
+                gr[d][v] = gb[gri + d*gsr + v*SOA];
         viscous_flux_add(ur, gr, fvr, c);
     }
 
@@ -793,9 +797,10 @@ int frhip_intconu(int dtype, int n, const void *ubase, const int32_t *ulin,
 int frhip_intcflux_ns(int dtype, int rsolver, int n, void *ubase,
                       const int32_t *lmap, const int32_t *rmap,
                       const void *gbase, const int32_t *glmap,
-                      const int32_t *grmap, long gstride, const void *nl,
-                      long ldn, const double *consts, const void *mu_p,
-                      void *stream)
+                      const int32_t *grmap, long gstride,
+                      const int32_t *glstr, const int32_t *grstr,
+                      const void *nl, long ldn, const double *consts,
+                      const void *mu_p, void *stream)
 {
     FR_API_BEGIN
     FR_DTYPE_OK(dtype);
@@ -806,14 +811,16 @@ int frhip_intcflux_ns(int dtype, int rsolver, int n, void *ubase,
 #define FR_L64(RS)                                                          \
     hipLaunchKernelGGL((k_intcflux_ns<double, RS>), grid, block, 0,         \
                        as_stream(stream), n, (double *)ubase, lmap, rmap,   \
-                       (const double *)gbase, glmap, grmap, gstride,        \
-                       (const double *)nl, ldn, ns_consts<double>(consts),  \
+                       (const double *)gbase, glmap, grmap, gstride, glstr, \
+                       grstr, (const double *)nl, ldn,                      \
+                       ns_consts<double>(consts),                           \
                        (const double *)mu_p)
 #define FR_L32(RS)                                                          \
     hipLaunchKernelGGL((k_intcflux_ns<float, RS>), grid, block, 0,          \
                        as_stream(stream), n, (float *)ubase, lmap, rmap,    \
-                       (const float *)gbase, glmap, grmap, gstride,         \
-                       (const float *)nl, ldn, ns_consts<float>(consts),    \
+                       (const float *)gbase, glmap, grmap, gstride, glstr,  \
+                       grstr, (const float *)nl, ldn,                       \
+                       ns_consts<float>(consts),                            \
                        (const float *)mu_p)
     if (dtype == FRHIP_F64) {
         FR_RS_DISPATCH(rsolver, FR_L64)

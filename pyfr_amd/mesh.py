@@ -212,23 +212,27 @@ CELL_SPLITS = {'tet': _kuhn_offsets(), 'pri': _prism_offsets()}
 
 
 class SimplexBoxMesh:
-    """Periodic boxes of straight-sided tetrahedra and prisms.
+    """Boxes of straight-sided tetrahedra and prisms.
 
     blocks: list of (shape (nx, ny, nz), spacing h, layers) where layers is
     a list of (etype, nlayers) filling the block's z extent bottom to top;
-    every block is periodic on its own.  Cells split into 6 tets or 2
-    prisms (``CELL_SPLITS``); all splits share the cube diagonal, so faces
-    conform within and across element types.
+    each block is periodic on its own along the axes where ``periodic`` is
+    true, and the faces on its other sides are boundaries named
+    'x0'/'x1', 'y0'/'y1', 'z0'/'z1' (low/high side).  Cells split into 6
+    tets or 2 prisms (``CELL_SPLITS``); all splits share the cube diagonal,
+    so faces conform within and across element types.
 
     ``spts``: {etype: (nverts, neles, 3)}; elements are ordered by block,
     then split index, then cell (x fastest).  ``con``: (lhs, rhs) lists of
     (etype, eidx, fidx), each face pair once, left side the smaller
     (etype index, face index, element index) key, sorted by left side.
+    ``bcon``: {name: [(etype, eidx, fidx), ...]} in that key order.
     """
     ndims = 3
     etype_order = ('tet', 'pri')
 
-    def __init__(self, blocks):
+    def __init__(self, blocks, periodic=(True, True, True)):
+        self.periodic = tuple(bool(p) for p in periodic)
         verts = {et: [] for et in self.etype_order}
         nodes = {et: [] for et in self.etype_order}
         noff = 0
@@ -263,12 +267,13 @@ class SimplexBoxMesh:
         self.con = self._connect()
 
     @classmethod
-    def stacked(cls, n, ntet, npri, length=2*np.pi):
+    def stacked(cls, n, ntet, npri, length=2*np.pi,
+                periodic=(True, True, True)):
         """n x n x (ntet + npri) cells of size length/n: tets below, prisms
-        above, periodic in all three directions (tet-prism interfaces at
-        both ends of the prism slab)."""
+        above (tet-prism interfaces at both ends of the prism slab when
+        periodic in z)."""
         layers = [(et, k) for et, k in (('tet', ntet), ('pri', npri)) if k]
-        return cls([((n, n, ntet + npri), length/n, layers)])
+        return cls([((n, n, ntet + npri), length/n, layers)], periodic)
 
     @classmethod
     def disjoint(cls, ntet, npri, length=2*np.pi):
@@ -284,30 +289,53 @@ class SimplexBoxMesh:
     def _connect(self):
         """Pair faces through a structured key: (block, face kind, face
         centroid in lattice units, times 12 so that triangle and quad
-        centroids are integers, modulo the block's periods)."""
-        keys, owners = [], []
+        centroids are integers, modulo the block's periods along periodic
+        axes).  A face left unpaired must lie on a non-periodic side."""
+        per = np.array(self.periodic)
+        keys, owners, sides = [], [], []
         for ti, et in enumerate(self.etypes):
             E0 = 0
             for lat, blk, dims in self._nodes[et]:
                 E = len(lat)
                 for f, fv in enumerate(FACE_VERTS[et]):
-                    c = lat[:, list(fv)].sum(axis=1)*(12//len(fv))
+                    fl = lat[:, list(fv)]                   # (E, nfv, 3)
+                    c = fl.sum(axis=1)*(12//len(fv))
                     k = np.empty((E, 5), dtype=np.int64)
                     k[:, 0] = blk
                     k[:, 1] = len(fv)
-                    k[:, 2:] = c % (12*dims)
+                    k[:, 2:] = np.where(per, c % (12*dims), c)
                     keys.append(k)
                     owners.append(np.stack([np.full(E, ti), np.full(E, f),
                                             E0 + np.arange(E)], 1))
+                    # side code 2*axis + (high side), or -1
+                    sd = np.full(E, -1)
+                    for ax in np.flatnonzero(~per):
+                        sd[np.all(fl[:, :, ax] == 0, axis=1)] = 2*ax
+                        sd[np.all(fl[:, :, ax] == dims[ax], axis=1)] = \
+                            2*ax + 1
+                    sides.append(sd)
                 E0 += E
         keys, owners = np.concatenate(keys), np.concatenate(owners)
+        sides = np.concatenate(sides)
         order = np.lexsort(keys.T[::-1])
         sk = keys[order]
-        same = np.all(sk[1:] == sk[:-1], axis=1)
-        starts = np.flatnonzero(same)
-        if 2*len(starts) != len(keys) or np.any(same[1:] & same[:-1]):
+        first = np.r_[True, np.any(sk[1:] != sk[:-1], axis=1)]
+        starts = np.flatnonzero(first)
+        cnt = np.diff(np.r_[starts, len(keys)])
+        single = order[starts[cnt == 1]]
+        if cnt.max() > 2 or np.any(sides[single] < 0):
             raise ValueError('Faces are not all paired exactly once')
-        a, b = owners[order[starts]], owners[order[starts + 1]]
+        self.bcon = {}
+        bo = owners[single]
+        bs = sides[single]
+        for code in np.unique(bs):
+            sel = bo[bs == code]
+            sel = sel[np.lexsort(sel.T[::-1])]
+            name = f'{"xyz"[code//2]}{code % 2}'
+            self.bcon[name] = [(self.etypes[t], int(e), int(f))
+                               for t, f, e in sel]
+        pst = starts[cnt == 2]
+        a, b = owners[order[pst]], owners[order[pst + 1]]
         # left side = smaller (etype index, face, element) key
         less = ((a[:, 0] < b[:, 0]) |
                 ((a[:, 0] == b[:, 0]) & ((a[:, 1] < b[:, 1]) |

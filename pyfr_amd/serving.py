@@ -384,7 +384,7 @@ class Pool:
                         u.leaddim, 1.0, 0.0, s)
             lib.intcflux_ns(d, 0, mp['n'], self.scal_fpts.data,
                             mp['l'], mp['r'], self.vect_fpts.data,
-                            mp['l'], mp['r'], Nf*u.leaddim,
+                            mp['l'], mp['r'], Nf*u.leaddim, None, None,
                             mp['nl'], mp['ldn'], consts,
                             mp['mu'], s)
         mm('M1 - M3*M2', self.vect_upts, f)

@@ -490,6 +490,12 @@ class NavierStokesElements(EulerElements):
             dims=[self.nupts, self.neles], tdivtconf=self.scal_upts[fout],
             rcpdjac=self._rcpdjac_upts)
 
+        # |beta| = 1/2: intconu writes one side only, the other keeps the
+        # element's own trace (pyfr/solvers/baseadvecdiff/elements.py:42-45)
+        if abs(self.cfg.getfloat('solver-interfaces', 'ldg-beta')) == 0.5:
+            kernels['copy_fpts'] = lambda: be.kernel(
+                'copy', self._comm_fpts, self._scal_fpts)
+
         # pyfr/solvers/baseadvecdiff/elements.py:47-55
         kernels['tgradpcoru_upts'] = lambda uin: be.kernel(
             'mul', self.opmat('M4 - M6*M0'), self.scal_upts[uin],
@@ -1099,9 +1105,15 @@ class NavierStokesSystem(BaseSystem):
     intinterscls = NavierStokesIntInters
     bcinterscls = NavierStokesBCInters
 
+    @staticmethod
+    def _copy_fpts(eles):
+        return [e.kernels['copy_fpts']() for e in eles
+                if 'copy_fpts' in e.kernels]
+
     def _fused_kernels(self, uin, fout, xa, stage):
         eles = list(self.ele_map.values())
         ks = [e.kernels['disu'](uin) for e in eles]
+        ks += self._copy_fpts(eles)
         ks += [i.kernels['con_u']() for i in self._inters]
         for e in eles:
             ks += [e.kernels['tgradf'](uin), e.kernels['tdisf_fused'](uin),
@@ -1118,6 +1130,7 @@ class NavierStokesSystem(BaseSystem):
         g = be.graph()
         eles = list(self.ele_map.values())
         ks = [e.kernels['disu'](uin) for e in eles]
+        ks += self._copy_fpts(eles)
         ks += [i.kernels['con_u']() for i in self._inters]
         for e in eles:
             ks += [e.kernels['tgradpcoru_upts'](uin),

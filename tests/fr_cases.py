@@ -79,9 +79,10 @@ def eval_ics(ics):
 
 
 def make_simplex_cfg(order=2, precision='double', scheme='rk4', dt=1e-3,
-                     ics=None, rsolver='rusanov'):
-    cfg = make_cfg('euler', order, rsolver, precision=precision,
-                   scheme=scheme, dt=dt)
+                     ics=None, rsolver='rusanov', system='euler', beta=0.0,
+                     tau=0.0):
+    cfg = make_cfg(system, order, rsolver, precision=precision,
+                   scheme=scheme, dt=dt, beta=beta, tau=tau)
     for k, v in (ics or mode_ics()).items():
         cfg.set('soln-ics', k, v)
     return cfg

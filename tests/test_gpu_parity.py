@@ -113,6 +113,32 @@ def test_rhs_parity(backend_factory, system, n, order, rsolver, fused):
 
 
 @pytest.mark.parametrize('fused', [True, False])
+@pytest.mark.parametrize('beta,tau', [(-0.5, 0.0), (0.5, 0.0), (0.25, 0.1),
+                                      (0.5, 0.2)])
+def test_ns_ldg_parity(backend_factory, beta, tau, fused):
+    """LDG beta and tau (navstokes/kernels/intconu.mako, intcflux.mako);
+    at |beta| = 1/2 intconu writes one side only and the other keeps the
+    element's own trace (the copy_fpts kernel)."""
+    from pyfr_amd.mesh import BoxMesh
+    from pyfr_amd.solvers import get_integrator
+    be = backend_factory(fused=fused)
+    print(f'\n[ldg] beta={beta} tau={tau} fused={fused}', flush=True)
+    cfg = make_cfg('navier-stokes', 3, beta=beta, tau=tau)
+    cfg.set('backend-hip', 'fused', str(be.fused))
+    cfg.set('constants', 'mu', '0.02')
+    integ = get_integrator(be, BoxMesh(3), cfg)
+    c = oracle_consts('navier-stokes', beta, tau) | {'mu': 0.02}
+    orc = OracleHexSystem(3, 3, 'navier-stokes', constants=c)
+    u0 = orc.ics()
+    integ.system.rhs(0.0, 0, 1)
+    ref = orc.rhs(u0)
+    tol = max(1e-12, 10*rhs_noise_floor(orc, u0))
+    err = rel_l2(integ.system.ele_scal_upts(1)[0], ref)
+    print(f'[ldg] rel L2 {err:.3e} (tol {tol:.3e})', flush=True)
+    assert err < tol
+
+
+@pytest.mark.parametrize('fused', [True, False])
 def test_cfg0_rk4_100_steps(backend_factory, fused):
     """Plain method, CPU-runnable config: 8^3 hex p=2 Euler, Rusanov, RK4,
     dt=1e-3 to t=0.1.  The reference's fixed-dt controller takes 101 steps

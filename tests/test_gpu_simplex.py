@@ -340,3 +340,111 @@ def test_bf16_operand_plans(backend_factory, shape):
     got = c.get()
     assert rel_l2(got, rounded) < 2e-6
     assert 1e-4 < rel_l2(got, exact) < 2e-2
+
+
+# -- Navier-Stokes and boundary conditions on tets / prisms -----------------
+
+# side -> (type, {option: expression}) on the p = 1/1.4, sound speed 1
+# state of mode_ics
+FREE = {'rho': '1 + 0.05*sin(z)', 'u': '0.2', 'v': '0.1*cos(z)',
+        'w': '-0.05', 'p': '1/1.4 + 0.02*cos(y)'}
+SIMPLEX_BCS = {
+    'euler': {'x0': ('char-riem-inv', FREE), 'x1': ('slp-adia-wall', {}),
+              'z0': ('slp-adia-wall', {}), 'z1': ('sup-out-fn', {})},
+    'navier-stokes': {
+        'x0': ('no-slp-isot-wall', {'u': '0.05*sin(z)', 'v': '0',
+                                    'w': '0', 'cpTw': '3.4'}),
+        'x1': ('no-slp-adia-wall', {}), 'z0': ('char-riem-inv', FREE),
+        'z1': ('sub-out-fp', {'p': '0.7'})},
+}
+MU = 0.02
+
+
+def _mixed_ns_bc(be, n, nt, npr, order, system, bcs=False, rsolver='rusanov',
+                 beta=0.0, tau=0.0):
+    from oracle.simplex import OracleMixedSystem
+    from pyfr_amd.mesh import SimplexBoxMesh
+    from pyfr_amd.solvers import get_integrator
+    ics = mode_ics(seed=5)
+    cfg = make_simplex_cfg(order, be.precision, ics=ics, rsolver=rsolver,
+                           system=system, beta=beta, tau=tau)
+    cfg.set('backend-hip', 'fused', str(be.fused))
+    c = {'gamma': 1.4}
+    if system == 'navier-stokes':
+        cfg.set('constants', 'mu', repr(MU))
+        c |= {'mu': MU, 'Pr': 0.71, 'ldg-beta': beta, 'ldg-tau': tau}
+    periodic = (False, True, False) if bcs else (True, True, True)
+    obcs = {}
+    if bcs:
+        for side, (bctype, opts) in SIMPLEX_BCS[system].items():
+            sect = f'soln-bcs-{side}'
+            cfg.set(sect, 'type', bctype)
+            vals = {}
+            for k, v in opts.items():
+                cfg.set(sect, k, v)
+                if k in ('cpTw',):
+                    c[k] = float(eval(v))
+                else:
+                    vals[k] = eval_ics({k: v})[k]
+            obcs[side] = (bctype, vals)
+    mesh = SimplexBoxMesh.stacked(n, nt, npr, periodic=periodic)
+    integ = get_integrator(be, mesh, cfg)
+    L = 2*np.pi
+    verts = {et: v.transpose(0, 2, 1) for et, v in mesh.spts.items()}
+    idx = {et: np.arange(v.shape[1]) for et, v in mesh.spts.items()}
+    orc = OracleMixedSystem(verts, order, {et: _tables(et, order)
+                                           for et in verts},
+                            [((L, L, (nt + npr)*L/n), idx)], rsolver=rsolver,
+                            system=system, constants=c, periodic=periodic,
+                            bcs=obcs)
+    return integ, orc, orc.ics(eval_ics(ics))
+
+
+def _rhs_check(integ, orc, u0, tag):
+    sysm = integ.system
+    names = sysm.ele_types
+    got0 = dict(zip(names, sysm.ele_scal_upts(0)))
+    assert rel_l2(_cat(got0, names), _cat(u0, names)) < 1e-14
+    sysm.rhs(0.0, 0, 1)
+    got = dict(zip(names, sysm.ele_scal_upts(1)))
+    ref = orc.rhs(u0)
+    tol = max(1e-12, 10*_noise_floor(orc, u0))
+    err = rel_l2(_cat(got, names), _cat(ref, names))
+    print(f'[{tag}] rel L2 {err:.3e} (tol {tol:.3e})', flush=True)
+    assert err < tol
+
+
+@pytest.mark.parametrize('fused', [True, False])
+@pytest.mark.parametrize('n,nt,npr,order,beta,tau', [
+    (2, 1, 1, 1, 0.0, 0.0), (2, 1, 1, 2, 0.0, 0.0), (3, 2, 1, 3, 0.0, 0.0),
+    (2, 1, 1, 4, 0.0, 0.0), (2, 1, 1, 3, 0.25, 0.1), (2, 2, 0, 3, 0.5, 0.0),
+    (2, 1, 1, 2, -0.5, 0.1)])
+def test_simplex_ns_rhs_parity(backend_factory, n, nt, npr, order, beta,
+                               tau, fused):
+    """NS on periodic tet/prism meshes: LDG gradient (M4 - M6 M0, M6), the
+    affine fused NS flux, gradients at the flux points and the NS interface
+    flux, with mu large enough that the viscous terms matter."""
+    be = backend_factory(fused=fused)
+    print(f'\n[simplex ns] n={n} tets={nt} prisms={npr} p={order} '
+          f'beta={beta} tau={tau} fused={fused}', flush=True)
+    integ, orc, u0 = _mixed_ns_bc(be, n, nt, npr, order, 'navier-stokes',
+                                  beta=beta, tau=tau)
+    _rhs_check(integ, orc, u0, 'simplex ns')
+
+
+@pytest.mark.parametrize('fused', [True, False])
+@pytest.mark.parametrize('system,order,rsolver', [
+    ('euler', 2, 'rusanov'), ('euler', 3, 'hllc'), ('euler', 4, 'roe'),
+    ('navier-stokes', 2, 'rusanov'), ('navier-stokes', 3, 'hll'),
+    ('navier-stokes', 4, 'rusanov')])
+def test_simplex_bc_rhs_parity(backend_factory, system, order, rsolver,
+                               fused):
+    """Boundary conditions on a tet/prism mesh periodic in y only: x sides
+    and the tet bottom / prism top are boundaries (triangles and quads)."""
+    be = backend_factory(fused=fused)
+    print(f'\n[simplex bc] {system} p={order} {rsolver} fused={fused}',
+          flush=True)
+    integ, orc, u0 = _mixed_ns_bc(be, 3, 2, 1, order, system, bcs=True,
+                                  rsolver=rsolver)
+    assert len(integ.system._bc_inters) == 4
+    _rhs_check(integ, orc, u0, 'simplex bc')
