@@ -532,6 +532,34 @@ class SimplexShapeOracle:
         m = self.norm_fpts.T[:, None, :]*self.m3
         return m.reshape(-1, self.nfpts)
 
+    # Flux anti-aliasing (pyfr/shapes.py:125-135, :18-19): an element
+    # quadrature rule (points, weights), M7 interpolation to it, M8 the L2
+    # projection basis.vdm.T @ (w * psi(qpts)), M9 = blockdiag(M8 x 3)
+    def set_quadrature(self, qpts, qwts):
+        self.qpts = np.asarray(qpts, dtype=float)
+        self.qwts = np.asarray(qwts, dtype=float)
+        for k in ('m7', 'm8'):
+            self.__dict__.pop(k, None)
+
+    @cached_property
+    def m7(self):
+        return self.ubasis.nodal_basis_at(self.qpts)
+
+    @cached_property
+    def m8(self):
+        ub = self.ubasis
+        return ub.vdm.T @ (self.qwts*ub.ortho_basis_at(self.qpts))
+
+    @property
+    def m9(self):
+        m8, (a, b) = self.m8, self.m8.shape
+        out = np.zeros((3*a, 3*b))
+        for d in range(3):
+            out[d*a:(d + 1)*a, d*b:(d + 1)*b] = m8
+        return out
+
     def opmat(self, expr):
         env = {f'M{i}': getattr(self, f'm{i}') for i in (0, 1, 2, 3, 4, 6)}
+        if hasattr(self, 'qpts'):
+            env |= {'M7': self.m7, 'M8': self.m8, 'M9': self.m9}
         return clean(eval(expr.replace('*', '@'), {'__builtins__': {}}, env))

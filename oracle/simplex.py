@@ -34,6 +34,8 @@ LINBASIS = {'tet': TetBasis, 'pri': PriBasis}
 
 class _Etype:
     def __init__(self, name, order, tables, verts):
+        """tables as for SimplexShapeOracle, plus 'qrule' = (points,
+        weights) of the element quadrature rule for flux anti-aliasing."""
         self.name = name
         self.sh = sh = SimplexShapeOracle(name, order, tables)
         self.verts = verts                          # (nv, 3, E)
@@ -46,21 +48,16 @@ class _Etype:
         self.M3 = sh.opmat('M3')
         self.M460 = sh.opmat('M4 - M6*M0')
         self.M6 = sh.opmat('M6')
-
-        # Jacobian at the solution points: j[a][b] = dX_b/dxi_a
-        dl = lb.jac_nodal_basis_at(sh.upts)          # (3, nv, Nu)
-        j = np.einsum('avp,vbe->pabe', dl, verts)    # (Nu, 3, 3, E)
-        s = np.empty_like(j)
-        for i, (a, b) in enumerate([(1, 2), (2, 0), (0, 1)]):
-            s[:, i, 0] = j[:, a, 1]*j[:, b, 2] - j[:, a, 2]*j[:, b, 1]
-            s[:, i, 1] = j[:, a, 2]*j[:, b, 0] - j[:, a, 0]*j[:, b, 2]
-            s[:, i, 2] = j[:, a, 0]*j[:, b, 1] - j[:, a, 1]*j[:, b, 0]
-        d = (j[:, 0, 0]*s[:, 0, 0] + j[:, 0, 1]*s[:, 0, 1] +
-             j[:, 0, 2]*s[:, 0, 2])
+        self.smats, d = self._smats_djac(sh.upts)    # (Nu, 3, 3, E)
         if np.any(d <= 0):
             raise ValueError('negative Jacobian')
-        self.smats = s                               # (Nu, 3, 3, E)
         self.rcpdjac = 1/d                           # (Nu, E)
+        if 'qrule' in tables:
+            sh.set_quadrature(*tables['qrule'])
+            self.nqpts = len(sh.qpts)
+            self.M7 = sh.opmat('M7')
+            self.M1329 = sh.opmat('(M1 - M3*M2)*M9')
+            self.smats_q, _ = self._smats_djac(sh.qpts)
 
         # Normals at the flux points from the Jacobian there
         dlf = lb.jac_nodal_basis_at(sh.fpts)
@@ -69,6 +66,20 @@ class _Etype:
         for i, (a, b) in enumerate([(1, 2), (2, 0), (0, 1)]):
             sf[:, i] = np.cross(jf[:, a], jf[:, b], axis=1)
         self.pnorm = np.einsum('pi,pike->pek', sh.norm_fpts, sf)  # (Nf,E,3)
+
+    def _smats_djac(self, pts):
+        """S = |J| J^-1 by cofactors and |J| at reference points from the
+        Jacobian j[a][b] = dX_b/dxi_a of the linear map."""
+        dl = self._lb.jac_nodal_basis_at(pts)        # (3, nv, npts)
+        j = np.einsum('avp,vbe->pabe', dl, self.verts)
+        s = np.empty_like(j)
+        for i, (a, b) in enumerate([(1, 2), (2, 0), (0, 1)]):
+            s[:, i, 0] = j[:, a, 1]*j[:, b, 2] - j[:, a, 2]*j[:, b, 1]
+            s[:, i, 1] = j[:, a, 2]*j[:, b, 0] - j[:, a, 0]*j[:, b, 2]
+            s[:, i, 2] = j[:, a, 0]*j[:, b, 1] - j[:, a, 1]*j[:, b, 0]
+        d = (j[:, 0, 0]*s[:, 0, 0] + j[:, 0, 1]*s[:, 0, 1] +
+             j[:, 0, 2]*s[:, 0, 2])
+        return s, d
 
     def ploc(self, pts):
         return np.einsum('pv,vbe->pbe', self._lb.nodal_basis_at(pts),
@@ -163,8 +174,13 @@ class OracleMixedSystem:
 
     def __init__(self, verts, order, tables, blocks, rsolver='rusanov',
                  gamma=1.4, system='euler', constants=None,
-                 periodic=(True, True, True), bcs=None):
+                 periodic=(True, True, True), bcs=None, antialias=()):
+        """antialias: ('flux',) forms the flux at each type's element
+        quadrature rule (``tables[etype]['qrule']``) and projects it back
+        (``pyfr/solvers/baseadvec/elements.py:78-94``,
+        ``baseadvecdiff/elements.py:103-116``)."""
         self.names = list(verts)
+        self.fluxaa = 'flux' in antialias
         self.et = {n: _Etype(n, order, tables[n], verts[n])
                    for n in self.names}
         self.system = system
@@ -233,12 +249,14 @@ class OracleMixedSystem:
         return (A @ B.reshape(A.shape[1], -1)).reshape(A.shape[0],
                                                         *B.shape[1:])
 
-    def _transform(self, et, f):
-        """F[i] = sum_k S[i][k] f[k] (tflux.mako:24-28); f (3, 5, Nu, E)."""
-        F = np.zeros((3, et.nupts, 5, et.neles))
+    def _transform(self, et, f, s=None):
+        """F[i] = sum_k S[i][k] f[k] (tflux.mako:24-28); f (3, 5, Np, E)
+        at the solution points, or with s at other points."""
+        s = et.smats if s is None else s
+        F = np.zeros((3, s.shape[0], 5, et.neles))
         for i in range(3):
             for k in range(3):
-                F[i] += et.smats[:, i, k][:, None, :]*f[k].transpose(1, 0, 2)
+                F[i] += s[:, i, k][:, None, :]*f[k].transpose(1, 0, 2)
         return F
 
     def _divergence(self, u, F, UF):
@@ -246,7 +264,10 @@ class OracleMixedSystem:
         out = {}
         for n in self.names:
             et = self.et[n]
-            div = self._mul(et.M132, F[n].reshape(3*et.nupts, 5, -1))
+            if self.fluxaa:
+                div = self._mul(et.M1329, F[n].reshape(3*et.nqpts, 5, -1))
+            else:
+                div = self._mul(et.M132, F[n].reshape(3*et.nupts, 5, -1))
             div = self._mul(et.M3, uf[n]) + div
             out[n] = -et.rcpdjac[:, None, :]*div
         return out
@@ -269,8 +290,14 @@ class OracleMixedSystem:
             UF[gb] = o.T
         F = {}
         for n in self.names:
-            f, _, _ = inviscid_flux(u[n].transpose(1, 0, 2), self.gamma)
-            F[n] = self._transform(self.et[n], f)
+            et = self.et[n]
+            if self.fluxaa:                                    # qptsu
+                uq = self._mul(et.M7, u[n])
+                f, _, _ = inviscid_flux(uq.transpose(1, 0, 2), self.gamma)
+                F[n] = self._transform(et, f, et.smats_q)
+            else:
+                f, _, _ = inviscid_flux(u[n].transpose(1, 0, 2), self.gamma)
+                F[n] = self._transform(et, f)
         return self._divergence(u, F, UF)
 
     def rhs_ns(self, u):
@@ -304,10 +331,19 @@ class OracleMixedSystem:
                 for k in range(3):
                     acc = acc + s[:, k, i][:, None, :]*grad[k]
                 gp[i] = et.rcpdjac[:, None, :]*acc
-            f, _, _ = inviscid_flux(u[n].transpose(1, 0, 2), c['gamma'])
-            viscous_flux_add(u[n].transpose(1, 0, 2),
-                             gp.transpose(0, 2, 1, 3), f, c)
-            F[n] = self._transform(et, f)
+            if self.fluxaa:
+                # gradcoru_qpts and qptsu, the flux at the quadrature points
+                uq = self._mul(et.M7, u[n])
+                gq = np.stack([self._mul(et.M7, gp[d]) for d in range(3)])
+                f, _, _ = inviscid_flux(uq.transpose(1, 0, 2), c['gamma'])
+                viscous_flux_add(uq.transpose(1, 0, 2),
+                                 gq.transpose(0, 2, 1, 3), f, c)
+                F[n] = self._transform(et, f, et.smats_q)
+            else:
+                f, _, _ = inviscid_flux(u[n].transpose(1, 0, 2), c['gamma'])
+                viscous_flux_add(u[n].transpose(1, 0, 2),
+                                 gp.transpose(0, 2, 1, 3), f, c)
+                F[n] = self._transform(et, f)
             GF.append(np.stack([self._mul(et.M0, gp[d])
                                 for d in range(3)]))   # gradcoru_fpts
         GF = self._flat(dict(zip(self.names, GF)))             # (3, nflat, 5)

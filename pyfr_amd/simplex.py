@@ -158,13 +158,27 @@ def _flush(a, tol=1e-10):
     return a
 
 
+# Element quadrature rule of flux anti-aliasing per order: the rule the
+# reference's default selection picks (pyfr/shapes.py:146-176: the fewest
+# points of degree >= 2p + 1 for tets, >= 2(p + 2) for prisms)
+AA_RULES = {
+    'tet': {1: ('witherden-vincent', 8), 2: ('witherden-vincent', 14),
+            3: ('witherden-vincent', 35), 4: ('witherden-vincent', 59)},
+    'pri': {1: ('witherden-vincent', 28), 2: ('witherden', 46),
+            3: ('witherden', 82), 4: ('witherden', 128)},
+}
+
+
 class _SimplexShape:
     ndims = 3
     name = None
     faces = None
 
-    def __init__(self, order):
+    def __init__(self, order, qrule=None):
+        """qrule: (rule name, npts) of the element quadrature rule of flux
+        anti-aliasing, or None."""
         self.order = order
+        self.qrule = qrule
         p = order
         self.nupts = len(self.upts)
         self.ubasis = NodalBasis(self.name, p, self.upts)
@@ -251,8 +265,43 @@ class _SimplexShape:
         m = self.norm_fpts.T[:, None, :]*self.m3
         return m.reshape(-1, self.nfpts)
 
+    # -- flux anti-aliasing (pyfr/shapes.py:125-135, :18-19)
+    @cached_property
+    def _eqrule(self):
+        pts, wts = TABLES[(self.name, *self.qrule)]
+        return np.array(pts), np.array(wts)
+
+    @property
+    def qpts(self):
+        return self._eqrule[0]
+
+    @property
+    def nqpts(self):
+        return len(self._eqrule[0])
+
+    @cached_property
+    def m7(self):
+        return self.ubasis.nodal_basis_at(self.qpts)
+
+    @cached_property
+    def m8(self):
+        """L2 projection: values at the solution points of
+        sum_b phi_b(x) sum_q w_q phi_b(x_q) f_q for the orthonormal phi_b."""
+        qp, qw = self._eqrule
+        ob = self.ubasis.ortho
+        return ob.at(self.upts).T @ (qw*ob.at(qp))
+
+    @property
+    def m9(self):
+        m8, (a, b) = self.m8, self.m8.shape
+        out = np.zeros((3*a, 3*b))
+        for d in range(3):
+            out[d*a:(d + 1)*a, d*b:(d + 1)*b] = m8
+        return out
+
     def opmat(self, expr):
-        env = {f'M{i}': getattr(self, f'm{i}') for i in (0, 1, 2, 3, 4, 6)}
+        ids = (0, 1, 2, 3, 4, 6) + ((7, 8, 9) if self.qrule else ())
+        env = {f'M{i}': getattr(self, f'm{i}') for i in ids}
         return _flush(eval(expr.replace('*', '@'), {'__builtins__': {}}, env))
 
     @cached_property

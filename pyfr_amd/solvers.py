@@ -104,9 +104,12 @@ class FluidElements:
             raise ValueError('Invalid anti-alias options')
         self.fluxaa = 'flux' in self.antialias
         if self.antialias and basiscls is not SHAPES['hex']:
-            raise NotImplementedError('anti-aliasing is supported for hexes '
-                                      'only')
-        if self.antialias:
+            if 'surf-flux' in self.antialias:
+                raise NotImplementedError('surface-flux anti-aliasing is '
+                                          'supported for hexes only')
+            self.basis = basis = basiscls(
+                order, self._simplex_qrule(cfg, basiscls.name, order))
+        elif self.antialias:
             nq1 = (self._gl_n1(cfg, 'solver-elements-hex', 3, order)
                    if self.fluxaa else None)
             nf1 = (self._gl_n1(cfg, 'solver-interfaces-quad', 2, order)
@@ -123,6 +126,32 @@ class FluidElements:
             self.geom = AffineGeometry(eles, self.name)
         else:
             self.geom = HexGeometry(eles, order)
+
+    @staticmethod
+    def _simplex_qrule(cfg, name, order):
+        """(rule, npts) of a tet / prism element quadrature rule
+        (``pyfr/shapes.py:146-185``): by default the rule the reference's
+        selection picks for order 1..4 (``simplex.AA_RULES``), or
+        [solver-elements-<name>] quad-pts / quad-npts naming one of the
+        tabulated rules."""
+        from pyfr_amd.quadtables import TABLES
+        from pyfr_amd.simplex import AA_RULES
+        sect = f'solver-elements-{name}'
+        if cfg.hasopt(sect, 'quad-deg'):
+            raise NotImplementedError('quad-deg rule selection is not '
+                                      'supported; give quad-pts and '
+                                      'quad-npts')
+        if cfg.hasopt(sect, 'quad-pts') or cfg.hasopt(sect, 'quad-npts'):
+            key = (name, cfg.get(sect, 'quad-pts'),
+                   cfg.getint(sect, 'quad-npts'))
+            if key not in TABLES or TABLES[key][1][0] is None:
+                raise NotImplementedError(f'quadrature rule {key[1:]} is '
+                                          f'not tabulated for {name}')
+            return key[1:]
+        if order not in AA_RULES[name]:
+            raise NotImplementedError(f'no default {name} quadrature rule '
+                                      f'at order {order}')
+        return AA_RULES[name][order]
 
     @staticmethod
     def _gl_n1(cfg, sect, ndims, order):
@@ -423,6 +452,11 @@ class EulerElements(FluidElements):
         be, r, s = self._be, self.mesh_regions, self._slice_mat
         nq = self.basis.nqpts
         ks = [self._qptsu(uin)]
+        if 'affine' in r:
+            ks.append(be.kernel(
+                'tflux', tplargs=tplargs | {'ktype': 'affine'},
+                dims=[nq, self.neles], u=self._scal_qpts, f=self._vect_qpts,
+                smats=self._smats_affine))
         if 'curved' in r:
             ks.append(be.kernel(
                 'tflux', tplargs=tplargs | {'ktype': 'curved'},
@@ -450,7 +484,8 @@ class EulerElements(FluidElements):
         self.kernels['tdivf'] = tdivf
 
         # straight-sided elements, Euler: tdisf folded in as well
-        if self.affine and type(self) is EulerElements and be.fuse_flux:
+        if (self.affine and type(self) is EulerElements and be.fuse_flux and
+                not self.fluxaa):
             gamma = self.cfg.getfloat('constants', 'gamma')
 
             def tdivf_flux(uin, fout, xa=None, stage=None):
@@ -577,6 +612,11 @@ class NavierStokesElements(EulerElements):
         nu, nq = self.nupts, self.basis.nqpts
         gtpl = {'ndims': self.ndims, 'nvars': self.nvars, 'nverts': 8}
         ks = []
+        if 'affine' in r:
+            ks.append(be.kernel(
+                'gradcoru', tplargs=gtpl | {'ktype': 'affine'},
+                dims=[nu, self.neles], gradu=self._grad_upts,
+                smats=self._smats_affine, rcpdjac=self._rcpdjac_upts))
         if 'curved' in r:
             ks.append(be.kernel(
                 'gradcoru', tplargs=gtpl | {'ktype': 'curved'},
@@ -593,6 +633,11 @@ class NavierStokesElements(EulerElements):
                          gq.slice(i*nq, (i + 1)*nq))
                for i in range(self.ndims)]
         ks.append(self._qptsu(uin))
+        if 'affine' in r:
+            ks.append(be.kernel(
+                'tflux', tplargs=tplargs | {'ktype': 'affine'},
+                dims=[nq, self.neles], u=self._scal_qpts, f=self._vect_qpts,
+                smats=self._smats_affine))
         if 'curved' in r:
             ks.append(be.kernel(
                 'tflux', tplargs=tplargs | {'ktype': 'curved'},

@@ -361,7 +361,7 @@ MU = 0.02
 
 
 def _mixed_ns_bc(be, n, nt, npr, order, system, bcs=False, rsolver='rusanov',
-                 beta=0.0, tau=0.0):
+                 beta=0.0, tau=0.0, aa=False):
     from oracle.simplex import OracleMixedSystem
     from pyfr_amd.mesh import SimplexBoxMesh
     from pyfr_amd.solvers import get_integrator
@@ -369,6 +369,8 @@ def _mixed_ns_bc(be, n, nt, npr, order, system, bcs=False, rsolver='rusanov',
     cfg = make_simplex_cfg(order, be.precision, ics=ics, rsolver=rsolver,
                            system=system, beta=beta, tau=tau)
     cfg.set('backend-hip', 'fused', str(be.fused))
+    if aa:
+        cfg.set('solver', 'anti-alias', 'flux')
     c = {'gamma': 1.4}
     if system == 'navier-stokes':
         cfg.set('constants', 'mu', repr(MU))
@@ -392,11 +394,16 @@ def _mixed_ns_bc(be, n, nt, npr, order, system, bcs=False, rsolver='rusanov',
     L = 2*np.pi
     verts = {et: v.transpose(0, 2, 1) for et, v in mesh.spts.items()}
     idx = {et: np.arange(v.shape[1]) for et, v in mesh.spts.items()}
-    orc = OracleMixedSystem(verts, order, {et: _tables(et, order)
-                                           for et in verts},
+    tabs = {et: _tables(et, order) for et in verts}
+    if aa:
+        from pyfr_amd.quadtables import TABLES
+        from pyfr_amd.simplex import AA_RULES
+        for et in tabs:
+            tabs[et]['qrule'] = TABLES[(et, *AA_RULES[et][order])]
+    orc = OracleMixedSystem(verts, order, tabs,
                             [((L, L, (nt + npr)*L/n), idx)], rsolver=rsolver,
                             system=system, constants=c, periodic=periodic,
-                            bcs=obcs)
+                            bcs=obcs, antialias=('flux',) if aa else ())
     return integ, orc, orc.ics(eval_ics(ics))
 
 
@@ -448,3 +455,23 @@ def test_simplex_bc_rhs_parity(backend_factory, system, order, rsolver,
                                   rsolver=rsolver)
     assert len(integ.system._bc_inters) == 4
     _rhs_check(integ, orc, u0, 'simplex bc')
+
+
+@pytest.mark.parametrize('fused', ['flux', 'div', 'none'])
+@pytest.mark.parametrize('system,n,nt,npr,order', [
+    ('euler', 2, 1, 1, 1), ('euler', 2, 1, 1, 2), ('euler', 3, 2, 1, 3),
+    ('euler', 2, 1, 1, 4), ('navier-stokes', 2, 1, 1, 2),
+    ('navier-stokes', 2, 1, 1, 3), ('navier-stokes', 2, 2, 0, 4)])
+def test_simplex_flux_aa_rhs_parity(backend_factory, system, n, nt, npr,
+                                    order, fused):
+    """Flux anti-aliasing on tets and prisms: u (and, for NS, the physical
+    gradient) at the element quadrature rule, the flux formed there with
+    the affine metric, (M1 - M3 M2) M9 divergence (dense operators);
+    'flux' also checks that the flux-folded kernel steps aside."""
+    be = backend_factory(fused=fused != 'none', fuse_flux=fused == 'flux')
+    print(f'\n[simplex aa] {system} n={n} tets={nt} prisms={npr} '
+          f'p={order} fused={fused}', flush=True)
+    integ, orc, u0 = _mixed_ns_bc(be, n, nt, npr, order, system, aa=True)
+    for e in integ.system.ele_map.values():
+        assert e.fluxaa and 'tdivf_flux' not in e.kernels
+    _rhs_check(integ, orc, u0, 'simplex aa')
