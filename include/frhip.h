@@ -290,6 +290,45 @@ int frhip_bccflux_ns(int dtype, int bctype, int rsolver, int n, void *ubase,
                      const void *bcv, long ldb, const double *consts,
                      void *stream);
 
+/* ---- boundary values that depend on time ----
+ * The reference splices each [soln-bcs-*] value expression into the kernel
+ * source and binds t at run time (pyfr/solvers/base/inters.py:54-71,
+ * pyfr/solvers/euler/inters.py:57-102).  Here each expression is a stack
+ * program of (op, arg) int32 pairs, evaluated in double at every boundary
+ * point from its coordinates and t:
+ *   PUSHC arg: push pool[arg];  PUSHV arg: push x, y, z or t (arg 0..3);
+ *   binary ops pop b then a and push a op b; unary ops replace the top.
+ * Functions are those the reference's expressions may use (pyfr/nputil.py
+ * npeval: pow, min, max, atan2, sin, cos, tan, exp, log, sqrt, abs, tanh,
+ * asin, acos, atan). */
+enum { FRHIP_BCX_PUSHC = 0, FRHIP_BCX_PUSHV = 1, FRHIP_BCX_ADD = 2,
+       FRHIP_BCX_SUB = 3, FRHIP_BCX_MUL = 4, FRHIP_BCX_DIV = 5,
+       FRHIP_BCX_POW = 6, FRHIP_BCX_MIN = 7, FRHIP_BCX_MAX = 8,
+       FRHIP_BCX_ATAN2 = 9, FRHIP_BCX_NEG = 10, FRHIP_BCX_SIN = 11,
+       FRHIP_BCX_COS = 12, FRHIP_BCX_TAN = 13, FRHIP_BCX_EXP = 14,
+       FRHIP_BCX_LOG = 15, FRHIP_BCX_SQRT = 16, FRHIP_BCX_ABS = 17,
+       FRHIP_BCX_TANH = 18, FRHIP_BCX_ASIN = 19, FRHIP_BCX_ACOS = 20,
+       FRHIP_BCX_ATAN = 21 };
+
+/* A validated program of nrows (<= 8) expressions: expression r is the
+ * pairs code[2*offs[r] .. 2*offs[r+1]); pool holds its constants.  Checked
+ * on the host (opcodes, indices, stack depth <= 16, one result each) and
+ * copied to the device with a device time scalar (initially 0). */
+int frhip_bc_program_create(void **prog, int nrows, const int32_t *code,
+                            const int32_t *offs, const double *pool,
+                            int npool);
+int frhip_bc_program_destroy(void *prog);
+
+/* Set the program's t on `stream`, ahead of the work that evaluates it
+ * (a captured graph keeps reading the same device scalar). */
+int frhip_bc_program_set_time(void *prog, double t, void *stream);
+
+/* out[r][i] = expression r at point i (coordinates ploc[0..2][i], row
+ * stride ldp) and the program's t, for n points; out has nrows rows of
+ * stride ldo, in the working precision. */
+int frhip_bc_eval(void *prog, int dtype, int n, const void *ploc, long ldp,
+                  void *out, long ldo, void *stream);
+
 /* ---- BLAS extensions: pyfr/backends/hip/blasext.py:8-136 ---- */
 
 /* axnpby: x0 = a0*x0 + sum_i a_i*x_i over (nrow, nvar*neles_pad) arrays

@@ -9,6 +9,8 @@ Navier-Stokes with gradient fusion (``pyfr/solvers/baseadvecdiff/system.py:53-23
   disu -> intconu -> tgradpcoru -> tgradcoru -> tdisf_fused -> gradcoru_fpts
   -> intcflux -> tdivtpcorf -> tdivtconf -> negdivconf
 """
+import inspect
+
 import numpy as np
 
 from oracle.basis import HexShapeOracle
@@ -109,21 +111,35 @@ class OracleHexSystem:
         bfaces = {}
         pairs = pair_faces_bruteforce(self.plocf, 6, sh.nfacefpts, length,
                                       periodic, bfaces)
-        self.bcs = []
+        self._bcdefs = []
         for side, faces in sorted(bfaces.items()):
             bctype, vals = (bcs or {})[side]
             bf = np.concatenate([f[2] for f in faces])
             be = np.concatenate([np.full(len(f[2]), f[0]) for f in faces])
-            x, y, z = self.plocf[bf, :, be].T
-            v = {k: (fn(x, y, z) if callable(fn) else fn)
-                 for k, fn in vals.items()}
-            self.bcs.append((side, bctype, bf, be,
-                             self.pnorm[bf, be].T, v))
+            self._bcdefs.append((side, bctype, bf, be, vals))
+        self._tdep = any(callable(fn) and _takes_time(fn)
+                         for *_, vals in self._bcdefs
+                         for fn in vals.values())
+        self._set_bcs(0.0)
         self.lf = np.concatenate([p[2] for p in pairs])
         self.le = np.concatenate([np.full(len(p[2]), p[0]) for p in pairs])
         self.rf = np.concatenate([p[5] for p in pairs])
         self.re = np.concatenate([np.full(len(p[5]), p[3]) for p in pairs])
         self.nl = self.pnorm[self.lf, self.le].T    # (3, npairs)
+
+    def _set_bcs(self, t):
+        """Boundary values at the boundary flux points; callables of
+        (x, y, z) or of (x, y, z, t)."""
+        self.bcs = []
+        for side, bctype, bf, be, vals in self._bcdefs:
+            x, y, z = self.plocf[bf, :, be].T
+            v = {}
+            for k, fn in vals.items():
+                if callable(fn):
+                    v[k] = fn(x, y, z, t) if _takes_time(fn) else fn(x, y, z)
+                else:
+                    v[k] = fn
+            self.bcs.append((side, bctype, bf, be, self.pnorm[bf, be].T, v))
 
     # -- initial conditions (pyfr/solvers/base/elements.py:86-129)
     def ics(self, ics=TGV_ICS, vscale=1.0):
@@ -265,7 +281,10 @@ class OracleHexSystem:
         div = self._mul(self.M3, uf) + div                  # tdivtconf
         return -self.rcpdjac[:, None, :]*div                # negdivconf
 
-    def rhs(self, u):
+    def rhs(self, u, t=0.0):
+        """The RHS at time t (boundary values that depend on t)."""
+        if self._tdep:
+            self._set_bcs(t)
         return self.rhs_euler(u) if self.system == 'euler' else self.rhs_ns(u)
 
     @property
@@ -274,6 +293,11 @@ class OracleHexSystem:
 
 
 # -- time integration (pyfr/integrators/...)
+
+def _takes_time(fn):
+    """A boundary-value callable of (x, y, z, t) rather than (x, y, z)."""
+    return len(inspect.signature(fn).parameters) == 4
+
 
 def clamp_dt(dt_want, t, tcurr, lookahead=10, dtmin=1e-12):
     """``pyfr/integrators/base.py:289-296``."""
@@ -284,19 +308,23 @@ def clamp_dt(dt_want, t, tcurr, lookahead=10, dtmin=1e-12):
     return max(remaining/nsteps, dtmin)
 
 
-def rk4_step(rhs, u, dt):
+def rk4_step(rhs, u, dt, t=None):
     """One RK4 step with the reference's register choreography and axnpby
-    special cases (``steppers.py:67-108``, ``hip/kernels/axnpby.mako:21-43``)."""
+    special cases (``steppers.py:67-108``, ``hip/kernels/axnpby.mako:21-43``).
+    With t, rhs is called as rhs(u, stage time) (stage times t, t + dt/2,
+    t + dt/2, t + dt)."""
+    f = rhs if t is None else (lambda v, c: rhs(v, t + c*dt))
+    g = (lambda v, c: f(v)) if t is None else f
     r0 = u
-    r1 = rhs(r0)
+    r1 = g(r0, 0.0)
     r2 = (1.0*r0 + (dt/2.0)*r1)
-    r2 = rhs(r2)
+    r2 = g(r2, 0.5)
     r1 = ((dt/6.0)*r1 + 1.0*r0 + (dt/3.0)*r2)
     r2 = ((dt/2.0)*r2 + 1.0*r0)
-    r2 = rhs(r2)
+    r2 = g(r2, 0.5)
     r1 = r1 + ((dt/3.0)*r2)
     r2 = (dt*r2 + 1.0*r0)
-    r2 = rhs(r2)
+    r2 = g(r2, 1.0)
     r1 = r1 + ((dt/6.0)*r2)
     return r1
 
@@ -308,13 +336,17 @@ RK45_B = [1153189308089 / 22510343858157, 1772645290293 / 4653164025191,
           5198255086312 / 14908931495163]
 
 
-def rk45_step(rhs, u, dt):
+def rk45_step(rhs, u, dt, t=None):
     """Low-storage 2-register RK45 (``steppers.py:111-190``, ``:218-243``;
-    ``kernels/rkvdh2.mako:4-31``)."""
+    ``kernels/rkvdh2.mako:4-31``).  With t, rhs is called as rhs(u, stage
+    time), stage i at t + c_i dt with c = [0] + [sum(b[:i]) + a[i]]
+    (``steppers.py:130``)."""
     a, b = RK45_A, RK45_B
+    c = [0.0] + [sum(b[:i]) + ai for i, ai in enumerate(a)]
     r1, r2 = u.copy(), None
     for i in range(5):
-        r2 = rhs(r1 if i == 0 else r2)
+        v = r1 if i == 0 else r2
+        r2 = rhs(v) if t is None else rhs(v, t + c[i]*dt)
         if i < 4:
             n1 = r1 + (dt*a[i])*r2
             n2 = r1 + (dt*b[i])*r2
@@ -325,15 +357,15 @@ def rk45_step(rhs, u, dt):
     return r2
 
 
-def advance(rhs, u, dt, tend, scheme='rk4', tstart=0.0):
+def advance(rhs, u, dt, tend, scheme='rk4', tstart=0.0, timed=False):
     """Fixed-dt ``none`` controller with Kahan-summed time
     (``explicit/controllers.py:48-60``, ``integrators/base.py:282-296``).
-    Returns (u, nsteps)."""
+    timed: call rhs(u, stage time).  Returns (u, nsteps)."""
     step = rk4_step if scheme == 'rk4' else rk45_step
     tcurr, comp, nsteps = tstart, 0.0, 0
     while tcurr < tend:
         h = clamp_dt(dt, tend, tcurr)
-        u = step(rhs, u, h)
+        u = step(rhs, u, h, tcurr if timed else None)
         y = h - comp
         t = tcurr + y
         comp = (t - tcurr) - y

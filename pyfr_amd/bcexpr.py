@@ -1,0 +1,103 @@
+"""Compile boundary-value expressions of (x, y, z, t) into the stack
+programs ``frhip_bc_eval`` evaluates (opcodes ``FRHIP_BCX_*`` of
+``include/frhip.h``).
+
+The reference splices each ``[soln-bcs-*]`` value expression into its
+kernel source, with ``ploc`` and a run-time ``t``
+(``pyfr/solvers/base/inters.py:54-71``).  The expressions use the
+``npeval`` vocabulary (``pyfr/nputil.py:82-117``): numbers, the
+``[constants]``, ``x``, ``y``, ``z``, ``t``, ``+ - * /``, unary minus and
+the functions below; ``**`` is refused, as there.
+"""
+import ast
+
+import numpy as np
+
+PUSHC, PUSHV = 0, 1
+BINOPS = {ast.Add: 2, ast.Sub: 3, ast.Mult: 4, ast.Div: 5}
+FUNCS2 = {'pow': 6, 'min': 7, 'max': 8, 'atan2': 9}
+NEG = 10
+FUNCS1 = {'sin': 11, 'cos': 12, 'tan': 13, 'exp': 14, 'log': 15, 'sqrt': 16,
+          'abs': 17, 'tanh': 18, 'asin': 19, 'acos': 20, 'atan': 21}
+VARS = {'x': 0, 'y': 1, 'z': 2, 't': 3}
+MAX_STACK = 16
+
+
+class ExprError(ValueError):
+    pass
+
+
+def _emit(node, consts, code, pool):
+    """Post-order emission; returns the stack depth the node needs."""
+    if isinstance(node, ast.Expression):
+        return _emit(node.body, consts, code, pool)
+    if isinstance(node, ast.Constant) and isinstance(node.value,
+                                                     (int, float)):
+        pool.append(float(node.value))
+        code.append((PUSHC, len(pool) - 1))
+        return 1
+    if isinstance(node, ast.Name):
+        if node.id in VARS:
+            code.append((PUSHV, VARS[node.id]))
+        elif node.id == 'pi':
+            pool.append(float(np.pi))
+            code.append((PUSHC, len(pool) - 1))
+        elif node.id in consts:
+            pool.append(float(consts[node.id]))
+            code.append((PUSHC, len(pool) - 1))
+        else:
+            raise ExprError(f'Unknown name {node.id!r}')
+        return 1
+    if isinstance(node, ast.UnaryOp) and isinstance(node.op,
+                                                    (ast.USub, ast.UAdd)):
+        d = _emit(node.operand, consts, code, pool)
+        if isinstance(node.op, ast.USub):
+            code.append((NEG, 0))
+        return d
+    if isinstance(node, ast.BinOp) and type(node.op) in BINOPS:
+        da = _emit(node.left, consts, code, pool)
+        db = _emit(node.right, consts, code, pool)
+        code.append((BINOPS[type(node.op)], 0))
+        return max(da, db + 1)
+    if (isinstance(node, ast.Call) and isinstance(node.func, ast.Name) and
+            not node.keywords):
+        fn, args = node.func.id, node.args
+        if fn in FUNCS1 and len(args) == 1:
+            d = _emit(args[0], consts, code, pool)
+            code.append((FUNCS1[fn], 0))
+            return d
+        if fn in FUNCS2 and len(args) == 2:
+            da = _emit(args[0], consts, code, pool)
+            db = _emit(args[1], consts, code, pool)
+            code.append((FUNCS2[fn], 0))
+            return max(da, db + 1)
+    raise ExprError(f'Unsupported expression element: {ast.dump(node)}')
+
+
+def compile_exprs(exprs, consts):
+    """exprs: list of expression strings; consts: {name: value}.  Returns
+    (code (ncode, 2) int32, offs (len(exprs) + 1) int32, pool float64)."""
+    code, offs, pool = [], [0], []
+    for e in exprs:
+        if '**' in e or '^' in e:
+            raise ExprError('Direct exponentiation is not supported; use pow')
+        try:
+            tree = ast.parse(e.strip(), mode='eval')
+        except SyntaxError as ex:
+            raise ExprError(f'Invalid expression {e!r}') from ex
+        depth = _emit(tree, consts, code, pool)
+        if depth > MAX_STACK:
+            raise ExprError(f'Expression too deep: {e!r}')
+        offs.append(len(code))
+    return (np.array(code, dtype=np.int32).reshape(-1, 2),
+            np.array(offs, dtype=np.int32), np.array(pool, dtype=float))
+
+
+def uses_time(expr):
+    """True if the expression refers to t."""
+    try:
+        tree = ast.parse(expr.strip(), mode='eval')
+    except SyntaxError:
+        return False
+    return any(isinstance(n, ast.Name) and n.id == 't'
+               for n in ast.walk(tree))

@@ -9,8 +9,9 @@
 //   *.mako (adds no-slp-adia-wall, no-slp-isot-wall, sub-in-frv,
 //   sub-in-ftpttang, sub-out-fp).
 // The boundary's prescribed values (which the reference splices into the
-// kernel source as expressions of ploc) arrive as per-point rows bcv,
-// evaluated on the host; constants derived from gamma and the BC options
+// kernel source as expressions of ploc and t) arrive as per-point rows bcv,
+// evaluated on the host once, or, for values that depend on t, by the
+// stack-program kernel below at each RHS; constants derived from gamma and the BC options
 // are formed in double on the host, like the reference's template literals.
 #include "frhip_common.h"
 #include "frhip_physics.h"
@@ -297,6 +298,91 @@ static void dispatch(int id, F &&f)
     }
 }
 
+
+// ---- boundary values as expressions of (x, y, z, t) ------------------------
+// The reference splices each value expression into the kernel source with
+// t a run-time argument (pyfr/solvers/base/inters.py:54-71 _exp_opts,
+// pyfr/solvers/euler/inters.py:57-102).  Here an expression is a stack
+// program of (op, arg) pairs evaluated in double at each boundary point; t
+// lives in a device scalar the host sets on the solver stream before each
+// RHS, so captured graphs stay valid.
+constexpr int BCX_STACK = 16;
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_bc_eval(
+    int n, int nrows, const int32_t *__restrict__ code,
+    const int32_t *__restrict__ offs, const double *__restrict__ pool,
+    const T *__restrict__ ploc, long ldp, const double *__restrict__ tp,
+    T *__restrict__ out, long ldo)
+{
+    const int i = blockIdx.x*blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double var[4] = {double(ploc[i]), double(ploc[ldp + i]),
+                           double(ploc[2*ldp + i]), *tp};
+    for (int r = 0; r < nrows; ++r) {
+        double stk[BCX_STACK];
+        int sp = 0;
+        for (int pc = offs[r]; pc < offs[r + 1]; ++pc) {
+            const int op = code[2*pc], arg = code[2*pc + 1];
+            // programs are validated on the host (depth <= BCX_STACK,
+            // operand counts, indices); the guards keep any access in range
+            if (op <= FRHIP_BCX_PUSHV) {
+                if (sp < BCX_STACK)
+                    stk[sp++] = op == FRHIP_BCX_PUSHC ? pool[arg]
+                                                      : var[arg & 3];
+                continue;
+            }
+            if (op <= FRHIP_BCX_ATAN2) {
+                if (sp < 2) continue;
+                const double b = stk[--sp], a = stk[sp - 1];
+                double c;
+                switch (op) {
+                    case FRHIP_BCX_ADD: c = a + b; break;
+                    case FRHIP_BCX_SUB: c = a - b; break;
+                    case FRHIP_BCX_MUL: c = a*b; break;
+                    case FRHIP_BCX_DIV: c = a/b; break;
+                    case FRHIP_BCX_POW: c = pow(a, b); break;
+                    case FRHIP_BCX_MIN: c = fmin(a, b); break;
+                    case FRHIP_BCX_MAX: c = fmax(a, b); break;
+                    default: c = atan2(a, b); break;
+                }
+                stk[sp - 1] = c;
+                continue;
+            }
+            if (sp < 1) continue;
+            const double a = stk[sp - 1];
+            double c;
+            switch (op) {
+                case FRHIP_BCX_NEG: c = -a; break;
+                case FRHIP_BCX_SIN: c = sin(a); break;
+                case FRHIP_BCX_COS: c = cos(a); break;
+                case FRHIP_BCX_TAN: c = tan(a); break;
+                case FRHIP_BCX_EXP: c = exp(a); break;
+                case FRHIP_BCX_LOG: c = log(a); break;
+                case FRHIP_BCX_SQRT: c = sqrt(a); break;
+                case FRHIP_BCX_ABS: c = fabs(a); break;
+                case FRHIP_BCX_TANH: c = tanh(a); break;
+                case FRHIP_BCX_ASIN: c = asin(a); break;
+                case FRHIP_BCX_ACOS: c = acos(a); break;
+                default: c = atan(a); break;
+            }
+            stk[sp - 1] = c;
+        }
+        out[r*ldo + i] = T(sp > 0 ? stk[0] : 0.0);
+    }
+}
+
+__global__ void k_set_scalar(double *p, double v)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+struct BCProgram {
+    int nrows, ncode, npool;
+    int32_t *code, *offs;   // device
+    double *pool, *t;       // device
+};
+
 }  // namespace frhip
 
 using namespace frhip;
@@ -401,6 +487,126 @@ int frhip_bccflux_ns(int dtype, int bctype, int rsolver, int n, void *base,
     };
     if (dtype == FRHIP_F64) go(double());
     else go(float());
+    FR_LAUNCHED();
+    return 0;
+    FR_API_END
+}
+
+int frhip_bc_program_create(void **prog, int nrows, const int32_t *code,
+                            const int32_t *offs, const double *pool,
+                            int npool)
+{
+    FR_API_BEGIN
+    FR_CHECK(prog && code && offs && nrows > 0 && nrows <= 8,
+             "invalid program arguments");
+    FR_CHECK(offs[0] == 0, "program offsets must start at 0");
+    for (int r = 0; r < nrows; ++r) {
+        FR_CHECK(offs[r + 1] > offs[r], "empty expression");
+        int sp = 0;
+        for (int pc = offs[r]; pc < offs[r + 1]; ++pc) {
+            const int op = code[2*pc], arg = code[2*pc + 1];
+            if (op == FRHIP_BCX_PUSHC) {
+                FR_CHECK(pool && arg >= 0 && arg < npool,
+                         "constant index out of range");
+                ++sp;
+            }
+            else if (op == FRHIP_BCX_PUSHV) {
+                FR_CHECK(arg >= 0 && arg < 4, "variable index out of range");
+                ++sp;
+            }
+            else if (op <= FRHIP_BCX_ATAN2) {
+                FR_CHECK(sp >= 2, "stack underflow");
+                --sp;
+            }
+            else {
+                FR_CHECK(op <= FRHIP_BCX_ATAN, "invalid opcode");
+                FR_CHECK(sp >= 1, "stack underflow");
+            }
+            FR_CHECK(sp <= BCX_STACK, "expression too deep");
+        }
+        FR_CHECK(sp == 1, "expression leaves a malformed stack");
+    }
+    auto *p = new BCProgram{nrows, offs[nrows], npool, nullptr, nullptr,
+                            nullptr, nullptr};
+    auto fail = [&]() {
+        (void)hipFree(p->code);
+        (void)hipFree(p->offs);
+        (void)hipFree(p->pool);
+        (void)hipFree(p->t);
+        delete p;
+    };
+    const size_t nc = (size_t)2*p->ncode*sizeof(int32_t);
+    if (hipMalloc(&p->code, nc) != hipSuccess ||
+        hipMalloc(&p->offs, (nrows + 1)*sizeof(int32_t)) != hipSuccess ||
+        hipMalloc(&p->pool, (npool > 0 ? npool : 1)*sizeof(double)) !=
+            hipSuccess ||
+        hipMalloc(&p->t, sizeof(double)) != hipSuccess ||
+        hipMemcpy(p->code, code, nc, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(p->offs, offs, (nrows + 1)*sizeof(int32_t),
+                  hipMemcpyHostToDevice) != hipSuccess ||
+        (npool > 0 && hipMemcpy(p->pool, pool, npool*sizeof(double),
+                                hipMemcpyHostToDevice) != hipSuccess)) {
+        fail();
+        FR_CHECK(false, "device allocation for a boundary program failed");
+    }
+    const double zero = 0.0;
+    if (hipMemcpy(p->t, &zero, sizeof(double), hipMemcpyHostToDevice) !=
+            hipSuccess) {
+        fail();
+        FR_CHECK(false, "device copy for a boundary program failed");
+    }
+    *prog = p;
+    return 0;
+    FR_API_END
+}
+
+int frhip_bc_program_destroy(void *prog)
+{
+    FR_API_BEGIN
+    auto *p = static_cast<BCProgram *>(prog);
+    if (!p) return 0;
+    (void)hipFree(p->code);
+    (void)hipFree(p->offs);
+    (void)hipFree(p->pool);
+    (void)hipFree(p->t);
+    delete p;
+    return 0;
+    FR_API_END
+}
+
+int frhip_bc_program_set_time(void *prog, double t, void *stream)
+{
+    FR_API_BEGIN
+    auto *p = static_cast<BCProgram *>(prog);
+    FR_CHECK(p, "null program");
+    hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(64), 0, as_stream(stream),
+                       p->t, t);
+    FR_LAUNCHED();
+    return 0;
+    FR_API_END
+}
+
+int frhip_bc_eval(void *prog, int dtype, int n, const void *ploc, long ldp,
+                  void *out, long ldo, void *stream)
+{
+    FR_API_BEGIN
+    auto *p = static_cast<BCProgram *>(prog);
+    FR_CHECK(p, "null program");
+    FR_DTYPE_OK(dtype);
+    FR_CHECK(n >= 0 && ldp >= n && ldo >= n, "invalid extents");
+    if (n == 0) return 0;
+    FR_CHECK(ploc && out, "null arrays");
+    const dim3 grid(cdiv(n, 256)), block(256);
+    if (dtype == FRHIP_F64)
+        hipLaunchKernelGGL(k_bc_eval<double>, grid, block, 0,
+                           as_stream(stream), n, p->nrows, p->code, p->offs,
+                           p->pool, (const double *)ploc, ldp, p->t,
+                           (double *)out, ldo);
+    else
+        hipLaunchKernelGGL(k_bc_eval<float>, grid, block, 0,
+                           as_stream(stream), n, p->nrows, p->code, p->offs,
+                           p->pool, (const float *)ploc, ldp, p->t,
+                           (float *)out, ldo);
     FR_LAUNCHED();
     return 0;
     FR_API_END

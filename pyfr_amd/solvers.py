@@ -791,9 +791,13 @@ BC_DEFAULTS = {'sub-in-frv': {'u': 0, 'v': 0, 'w': 0},
 class EulerBCInters(EulerIntInters):
     """Boundary interfaces (``pyfr/solvers/baseadvec/inters.py:52-116``,
     ``euler/inters.py:40-102``).  The reference evaluates the boundary
-    values inside the kernel from expressions of ``ploc`` (and ``t``);
-    here they are evaluated once on the host at the boundary flux points
-    and passed as per-point rows, so time-dependent values are declined."""
+    values inside the kernel from expressions of ``ploc`` (and ``t``).
+    Here values that do not depend on t are evaluated once on the host at
+    the boundary flux points and passed as per-point rows; if any depends
+    on t, the expressions are compiled to stack programs
+    (``pyfr_amd.bcexpr``) that a kernel evaluates into those rows at each
+    RHS, from a device time scalar set before the RHS (``set_time``)."""
+    _bcprog = None
     bctypes = ('sup-in-fa', 'sup-out-fn', 'char-riem-inv', 'slp-adia-wall')
     bccflux_mod = 'pyfr.solvers.euler.kernels.bccflux'
 
@@ -829,9 +833,39 @@ class EulerBCInters(EulerIntInters):
     def _kernels(self):
         be = self._be
         be.pointwise.register(self.bccflux_mod)
-        self.kernels['comm_flux'] = lambda: be.kernel(
+        self.kernels['comm_flux'] = lambda: self._with_values(be.kernel(
             'bccflux', tplargs=self._tplargs, dims=[self.ninterfpts],
-            ul=self.scal_lhs, nl=self._pnorm_lhs, bcvals=self._bcvals)
+            ul=self.scal_lhs, nl=self._pnorm_lhs, bcvals=self._bcvals))
+
+    def _with_values(self, kern):
+        """kern preceded, for time-dependent values, by their evaluation."""
+        if self._bcprog is None:
+            return kern
+        from pyfr_amd.backend import Kernel, _dtcode
+        lib, prog, n = self._be.lib, self._bcprog, self.ninterfpts
+        pl, bv = self._ploc_mat, self._bcvals
+        args = (prog, _dtcode(bv.dtype), n, pl.data, pl.leaddim, bv.data,
+                bv.leaddim)
+        ev = Kernel(lambda stream: lib.bc_eval(*args, stream), mats=(pl, bv),
+                    name='bc_eval')
+        return self._be.ordered_meta_kernel([ev, kern])
+
+    @property
+    def time_dependent(self):
+        return self._bcprog is not None
+
+    def set_time(self, t):
+        """Set t for the next RHS, on the solver stream."""
+        if self._bcprog is not None:
+            self._be.lib.bc_program_set_time(self._bcprog, float(t),
+                                             self._be.stream)
+
+    def __del__(self):
+        try:
+            if self._bcprog is not None:
+                self._be.lib.bc_program_destroy(self._bcprog)
+        except Exception:
+            pass
 
     def _eval_opt(self, k):
         """A scalar option (the reference's ``_eval_opts``)."""
@@ -842,25 +876,34 @@ class EulerBCInters(EulerIntInters):
         pass
 
     def _eval_values(self):
-        from pyfr_amd.backend import NotSuitableError
+        import ctypes as ct
+        from pyfr_amd.bcexpr import compile_exprs, uses_time
         keys = BC_VALUES.get(self.type)
         if not keys:
             return None
         m = _get_inter_arrays(self.lhs, 'get_ploc_for_inters', self.elemap,
                               self._perm)
         ploc = m[0] if m else np.empty((0, self.ndims))
-        env = self.cfg.items_as('constants', float)
-        env |= dict(zip('xyz', ploc.T))
+        consts = self.cfg.items_as('constants', float)
         dflt = BC_DEFAULTS.get(self.type, {})
-        rows = []
-        for k in keys:
-            expr = (self.cfg.getexpr(self.cfgsect, k, dflt[k]) if k in dflt
-                    else self.cfg.getexpr(self.cfgsect, k))
-            if re.search(r'(?<![\w.])t(?![\w.])', expr):
-                raise NotSuitableError('time-dependent boundary values are '
-                                       'not supported')
-            rows.append(np.broadcast_to(_npeval(expr, env),
-                                        (self.ninterfpts,)))
+        exprs = [str(self.cfg.getexpr(self.cfgsect, k, dflt[k]))
+                 if k in dflt else self.cfg.getexpr(self.cfgsect, k)
+                 for k in keys]
+        if any(uses_time(e) for e in exprs):
+            code, offs, pool = compile_exprs(exprs, consts)
+            prog = ct.c_void_p()
+            self._be.lib.bc_program_create(
+                ct.byref(prog), len(exprs), code.ctypes.data,
+                offs.ctypes.data, pool.ctypes.data if len(pool) else None,
+                len(pool))
+            self._bcprog = prog.value
+            self._ploc_mat = self._be.const_matrix(
+                np.ascontiguousarray(ploc.T), tags={'align'})
+            return self._be.matrix((len(exprs), self.ninterfpts),
+                                   tags={'align'})
+        env = consts | dict(zip('xyz', ploc.T))
+        rows = [np.broadcast_to(_npeval(e, env), (self.ninterfpts,))
+                for e in exprs]
         return self._be.const_matrix(np.array(rows, dtype=float))
 
 
@@ -898,10 +941,12 @@ class NavierStokesBCInters(EulerBCInters):
                                     (self.nvars,))
         be.pointwise.register('pyfr.solvers.navstokes.kernels.bcconu')
         be.pointwise.register(self.bccflux_mod)
-        self.kernels['con_u'] = lambda: be.kernel(
+        # time-dependent values are evaluated ahead of con_u, which runs
+        # before comm_flux in every RHS
+        self.kernels['con_u'] = lambda: self._with_values(be.kernel(
             'bcconu', tplargs=self._tplargs, dims=[self.ninterfpts],
             ulin=self.scal_lhs, ulout=self._comm_lhs, nlin=self._pnorm_lhs,
-            bcvals=self._bcvals)
+            bcvals=self._bcvals))
         self.kernels['comm_flux'] = lambda: be.kernel(
             'bccflux', tplargs=self._tplargs, dims=[self.ninterfpts],
             ul=self.scal_lhs, gradul=self._vect_lhs, nl=self._pnorm_lhs,
@@ -993,8 +1038,15 @@ class BaseSystem:
 
     def rhs(self, t, uinbank, foutbank):
         """``pyfr/solvers/base/system.py:420-430``."""
+        self._set_time(t)
         for g in self._rhs_graphs(uinbank, foutbank):
             self.backend.run_graph(g)
+
+    def _set_time(self, t):
+        """t for boundary values that depend on it (the reference binds it
+        into its kernels before each RHS)."""
+        for b in self._bc_inters:
+            b.set_time(t)
 
     def _rhs_graphs(self, uin, fout):
         key = (uin, fout)
@@ -1041,6 +1093,7 @@ class BaseSystem:
         if key not in self._kcache:
             self._kcache[key] = self._try_fused(
                 uin, fout, xa, 'rk45-mixed' if mixed else 'rk45')
+        self._set_time(t)
         for g in self._kcache[key]:
             g.final.bind(dta=dta, dtb=dtb, last=int(last))
             self.backend.run_graph(g)

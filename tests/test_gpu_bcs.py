@@ -1,6 +1,7 @@
 """GPU parity of the boundary-condition kernels (bccflux, bcconu): one RHS
 on hex boxes with non-periodic sides against the oracle, for every Euler
-and NS boundary type, with boundary values given as expressions of x, y, z.
+and NS boundary type, with boundary values given as expressions of x, y, z
+and, evaluated on the device at each RHS, of t.
 Tolerance as for the periodic RHS: max(1e-12, 10 x the oracle's own
 1e-16-input-noise floor) in relative L2; RK trajectories 1e-12."""
 import numpy as np
@@ -117,19 +118,128 @@ def test_bc_rk45_trajectory(backend_factory):
     assert err < 1e-12
 
 
-def test_bc_declines_time_dependent_values(backend_factory):
-    from pyfr_amd.backend import NotSuitableError
+# side -> (type, {option: expression of x, y, z, t}); the oracle's values
+# are the same expressions as callables of (x, y, z, t)
+TIMED = {
+    'euler': ('euler', (False, True, True), {
+        'x0': ('sup-in-fa', {'rho': '1 + 0.05*sin(z)*cos(3*t)',
+                             'u': '0.3 + 0.1*sin(2*t)', 'v': '0.1*cos(z)',
+                             'w': '-0.05*t', 'p': '100/1.4 + 0.2*cos(y + t)'}),
+        'x1': ('char-riem-inv', {'rho': '1', 'u': '0.3*exp(-t)', 'v': '0',
+                                 'w': '0', 'p': 'pow(1 + t, 0.5)*100/1.4'})}),
+    'navier-stokes': ('navier-stokes', (False, False, True), {
+        'x0': ('no-slp-isot-wall', {'u': '0.1*sin(z)*min(t, 0.5)',
+                                    'v': '0', 'w': '0', 'cpTw': '250'}),
+        'x1': ('sup-out-fn', {}),
+        'y0': ('sub-in-frv', {'rho': '1', 'u': '0.2*tanh(4*t)',
+                              'v': '0.05*sin(z - t)'}),
+        'y1': ('sub-out-fp', {'p': '71 + atan(t)'})}),
+}
+NPSYMS = {'__builtins__': {}, 'exp': np.exp, 'sin': np.sin, 'cos': np.cos,
+          'tanh': np.tanh, 'pow': np.power, 'min': np.minimum,
+          'atan': np.arctan}
+
+
+def _timed_setup(be, system, n, order, scheme='rk4'):
+    from oracle.rhs import OracleHexSystem
     from pyfr_amd.mesh import BoxMesh
     from pyfr_amd.solvers import get_integrator
+    sysname, periodic, bcs = TIMED[system]
+    cfg = make_cfg(sysname, order, scheme=scheme, dt=2e-4)
+    cfg.set('backend-hip', 'fused', str(be.fused))
+    oc = oracle_consts(sysname)
+    obcs = {}
+    for side, (bctype, opts) in bcs.items():
+        sect = f'soln-bcs-{side}'
+        cfg.set(sect, 'type', bctype)
+        vals = {}
+        for k, v in opts.items():
+            cfg.set(sect, k, v)
+            if k in SCALARS:
+                oc[k] = float(eval(v))
+            else:
+                vals[k] = (lambda e: lambda x, y, z, t: eval(
+                    e, NPSYMS, {'x': x, 'y': y, 'z': z, 't': t}) + 0*x)(v)
+        if bctype in ('no-slp-isot-wall', 'sub-in-frv'):
+            for k in 'uvw':
+                vals.setdefault(k, 0.0)
+        obcs[side] = (bctype, vals)
+    integ = get_integrator(be, BoxMesh(n, periodic=periodic), cfg)
+    orc = OracleHexSystem(n, order, sysname, constants=oc,
+                          periodic=periodic, bcs=obcs)
+    return integ, orc
+
+
+@pytest.mark.parametrize('fused', [True, False])
+@pytest.mark.parametrize('system', ['euler', 'navier-stokes'])
+def test_bc_time_dependent_rhs(backend_factory, system, fused):
+    """Boundary values that depend on t: compiled expression programs
+    evaluated on the device at each RHS, t set before the (captured) RHS;
+    two RHS at different t through the same graph."""
+    be = backend_factory(fused=fused)
+    print(f'\n[bc timed] {system} fused={fused}', flush=True)
+    integ, orc = _timed_setup(be, system, 3, 2)
+    b = integ.system._bc_inters
+    assert any(i.time_dependent for i in b)
+    u0 = orc.ics()
+    for t in (0.37, 1.25):
+        integ.system.rhs(t, 0, 1)
+        got = integ.system.ele_scal_upts(1)[0]
+        ref = orc.rhs(u0, t)
+        tol = max(1e-12, 10*_floor(orc, u0))
+        err = rel_l2(got, ref)
+        print(f'[bc timed] t={t} rel L2 {err:.3e} (tol {tol:.3e})',
+              flush=True)
+        assert err < tol
+        assert rel_l2(orc.rhs(u0, 0.0), ref) > 1e-6   # t matters
+
+
+@pytest.mark.parametrize('scheme', ['rk4', 'rk45'])
+def test_bc_time_dependent_trajectory(backend_factory, scheme):
+    """RK stage times reach the boundary values: 10 steps against the
+    oracle stepping with the reference's stage times."""
+    from oracle.rhs import advance
     be = backend_factory()
-    cfg = make_cfg('euler', 2)
-    for side in ('x0', 'x1'):
-        cfg.set(f'soln-bcs-{side}', 'type', 'sup-in-fa')
-        for k, v in FREE.items():
-            cfg.set(f'soln-bcs-{side}', k, v)
-    cfg.set('soln-bcs-x0', 'u', '0.3 + 0.1*t')
-    with pytest.raises(NotSuitableError):
-        get_integrator(be, BoxMesh(3, periodic=(False, True, True)), cfg)
+    integ, orc = _timed_setup(be, 'navier-stokes', 3, 2, scheme=scheme)
+    integ.dt = 2e-3
+    integ.advance_to(10*2e-3)
+    u_ref, nsteps = advance(orc.rhs, orc.ics(), 2e-3, 10*2e-3, scheme,
+                            timed=True)
+    u_untimed, _ = advance(orc.rhs, orc.ics(), 2e-3, 10*2e-3, scheme)
+    assert integ.nsteps == nsteps
+    err = rel_l2(integ.soln[0], u_ref)
+    print(f'\n[bc timed {scheme}] {nsteps} steps rel L2 {err:.3e} '
+          f'(vs t = 0 values: {rel_l2(u_untimed, u_ref):.3e})', flush=True)
+    assert err < 1e-12
+    assert rel_l2(u_untimed, u_ref) > 1e3*max(err, 1e-15)
+
+
+def test_bc_program_validation(backend_factory):
+    """Malformed programs are refused on the host before any device
+    work: bad opcode, stack underflow, constant index, depth, leftovers."""
+    import ctypes as ct
+    from pyfr_amd._lib import FrHipError
+    be = backend_factory()
+    pool = np.array([1.0, 2.0])
+
+    def make(code, offs):
+        code = np.asarray(code, dtype=np.int32).reshape(-1, 2)
+        offs = np.asarray(offs, dtype=np.int32)
+        p = ct.c_void_p()
+        be.lib.bc_program_create(ct.byref(p), len(offs) - 1,
+                                 code.ctypes.data, offs.ctypes.data,
+                                 pool.ctypes.data, len(pool))
+        be.lib.bc_program_destroy(p.value)
+
+    make([(0, 0), (1, 3), (2, 0)], [0, 3])          # 1.0 + t: accepted
+    for code, offs in [([(0, 0), (99, 0)], [0, 2]),  # bad opcode
+                       ([(0, 0), (2, 0)], [0, 2]),   # underflow
+                       ([(0, 5)], [0, 1]),           # constant index
+                       ([(1, 4)], [0, 1]),           # variable index
+                       ([(0, 0), (0, 1)], [0, 2]),   # two results
+                       ([(0, 0)]*17 + [(2, 0)]*16, [0, 33])]:  # depth 17
+        with pytest.raises(FrHipError):
+            make(code, offs)
 
 
 @pytest.mark.parametrize('version', ['2.2', '4.1'])
