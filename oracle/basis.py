@@ -489,6 +489,22 @@ class SimplexShapeOracle:
         cls = TriBasis if kind == 'tri' else QuadBasis
         return cls(self.order, self._frule[kind])
 
+    # Surface-flux anti-aliasing (pyfr/shapes.py:103-113, :196-224,
+    # :297-303): the flux points of each face are its interface rule
+    # iqrules[kind] = (points, weights); M3 is fed by proj_l2 of that rule
+    # onto the face basis, which stays on the standard flux points
+    def set_surface_quadrature(self, iqrules):
+        self._iqrules = {k: tuple(np.asarray(x, dtype=float) for x in v)
+                         for k, v in iqrules.items()}
+        self.nfacefpts = [len(self._iqrules[k][0]) for k, _, _ in self.faces]
+        self.nfpts = sum(self.nfacefpts)
+        self.fpts = np.vstack([_proj(p, self._iqrules[k][0])
+                               for k, p, _ in self.faces])
+        self.norm_fpts = np.vstack([[nm]*c for (_, _, nm), c in
+                                    zip(self.faces, self.nfacefpts)]) * 1.0
+        for k in ('m0', 'm2', 'm3', 'm6'):
+            self.__dict__.pop(k, None)
+
     @cached_property
     def gbasis_coeffs(self):
         coeffs = []
@@ -519,8 +535,22 @@ class SimplexShapeOracle:
 
     @cached_property
     def m3(self):
-        return clean((self.gbasis_coeffs @
-                      self.ubasis.ortho_basis_at(self.upts)).T)
+        m = clean((self.gbasis_coeffs @
+                   self.ubasis.ortho_basis_at(self.upts)).T)
+        if getattr(self, '_iqrules', None):
+            fps = []
+            for kind, _, _ in self.faces:
+                fb = self._facebasis(kind)
+                qp, qw = self._iqrules[kind]
+                fps.append(clean(fb.vdm.T @ (qw*fb.ortho_basis_at(qp))))
+            bd = np.zeros((sum(f.shape[0] for f in fps),
+                           sum(f.shape[1] for f in fps)))
+            r = c = 0
+            for f in fps:
+                bd[r:r + f.shape[0], c:c + f.shape[1]] = f
+                r, c = r + f.shape[0], c + f.shape[1]
+            m = m @ bd
+        return m
 
     @cached_property
     def m4(self):

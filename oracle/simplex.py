@@ -35,9 +35,13 @@ LINBASIS = {'tet': TetBasis, 'pri': PriBasis}
 class _Etype:
     def __init__(self, name, order, tables, verts):
         """tables as for SimplexShapeOracle, plus 'qrule' = (points,
-        weights) of the element quadrature rule for flux anti-aliasing."""
+        weights) of the element quadrature rule for flux anti-aliasing and
+        'iqrules' = {face kind: (points, weights)} for surface-flux
+        anti-aliasing."""
         self.name = name
         self.sh = sh = SimplexShapeOracle(name, order, tables)
+        if 'iqrules' in tables:
+            sh.set_surface_quadrature(tables['iqrules'])
         self.verts = verts                          # (nv, 3, E)
         self.neles = verts.shape[-1]
         self.nupts, self.nfpts = sh.nupts, sh.nfpts

@@ -167,6 +167,11 @@ AA_RULES = {
     'pri': {1: ('witherden-vincent', 28), 2: ('witherden', 46),
             3: ('witherden', 82), 4: ('witherden', 128)},
 }
+# Triangle-face rule of surface-flux anti-aliasing per order (the fewest
+# points among the symmetric rules of degree >= 2(p + 1),
+# pyfr/shapes.py:159-185); quad faces take (p + 2)^2 Gauss-Legendre
+SF_TRI_RULES = {1: ('williams-shunn', 6), 2: ('witherden-vincent', 12),
+                3: ('witherden-vincent', 16), 4: ('witherden-vincent', 25)}
 
 
 class _SimplexShape:
@@ -174,11 +179,14 @@ class _SimplexShape:
     name = None
     faces = None
 
-    def __init__(self, order, qrule=None):
+    def __init__(self, order, qrule=None, iqrules=None):
         """qrule: (rule name, npts) of the element quadrature rule of flux
-        anti-aliasing, or None."""
+        anti-aliasing, or None.  iqrules: surface-flux anti-aliasing, the
+        face rules {'tri': (rule name, npts), 'quad': Gauss-Legendre points
+        per direction}; the flux points of each face are that rule's."""
         self.order = order
         self.qrule = qrule
+        self.iqrules = iqrules
         p = order
         self.nupts = len(self.upts)
         self.ubasis = NodalBasis(self.name, p, self.upts)
@@ -188,15 +196,33 @@ class _SimplexShape:
         self.facefpts = [list(range(nf[i], nf[i + 1]))
                          for i in range(len(self.faces))]
 
-    # face flux points: Williams-Shunn (tri) / Gauss-Legendre (quad)
-    def _face_rule(self, kind):
+    # standard face flux points: Williams-Shunn (tri) / Gauss-Legendre
+    # (quad); the face bases live on them
+    def _std_face_pts(self, kind):
         p = self.order
         if kind == 'tri':
             pts, _ = TABLES[('tri', 'williams-shunn', (p + 1)*(p + 2)//2)]
-            return np.array(pts), None
+            return np.array(pts)
         g, _ = gauss_legendre(p + 1)
         return np.array([(g[i], g[j]) for j in range(p + 1)
-                         for i in range(p + 1)]), None
+                         for i in range(p + 1)])
+
+    def _iq_rule(self, kind):
+        """(points, weights) of the interface rule of surface-flux
+        anti-aliasing."""
+        if kind == 'tri':
+            pts, wts = TABLES[('tri', *self.iqrules['tri'])]
+            return np.array(pts), np.array(wts)
+        n = self.iqrules['quad']
+        g, w = gauss_legendre(n)
+        return (np.array([(g[i], g[j]) for j in range(n) for i in range(n)]),
+                np.array([w[i]*w[j] for j in range(n) for i in range(n)]))
+
+    # the flux points of each face: the standard set, or the interface rule
+    def _face_rule(self, kind):
+        if self.iqrules:
+            return self._iq_rule(kind)[0], None
+        return self._std_face_pts(kind), None
 
     @cached_property
     def fpts(self):
@@ -212,7 +238,7 @@ class _SimplexShape:
     def facebases(self):
         out = {}
         for kind in {k for k, _, _ in self.faces}:
-            out[kind] = NodalBasis(kind, self.order, self._face_rule(kind)[0])
+            out[kind] = NodalBasis(kind, self.order, self._std_face_pts(kind))
         return out
 
     @cached_property
@@ -253,7 +279,23 @@ class _SimplexShape:
 
     @cached_property
     def m3(self):
-        return (self.gbasis_coeffs @ self.ubasis.ortho.at(self.upts)).T
+        m = (self.gbasis_coeffs @ self.ubasis.ortho.at(self.upts)).T
+        if self.iqrules:
+            # fed by the L2 projection of each face's interface-rule values
+            # onto its face basis, proj_l2 (pyfr/shapes.py:103-113, :18-19)
+            blocks = []
+            for kind, _, _ in self.faces:
+                fb = self.facebases[kind]
+                qp, qw = self._iq_rule(kind)
+                blocks.append(fb.ortho.at(fb.pts).T @ (qw*fb.ortho.at(qp)))
+            bd = np.zeros((sum(b.shape[0] for b in blocks),
+                           sum(b.shape[1] for b in blocks)))
+            r = c = 0
+            for b in blocks:
+                bd[r:r + b.shape[0], c:c + b.shape[1]] = b
+                r, c = r + b.shape[0], c + b.shape[1]
+            m = m @ bd
+        return m
 
     @cached_property
     def m4(self):

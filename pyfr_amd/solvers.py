@@ -104,11 +104,11 @@ class FluidElements:
             raise ValueError('Invalid anti-alias options')
         self.fluxaa = 'flux' in self.antialias
         if self.antialias and basiscls is not SHAPES['hex']:
-            if 'surf-flux' in self.antialias:
-                raise NotImplementedError('surface-flux anti-aliasing is '
-                                          'supported for hexes only')
-            self.basis = basis = basiscls(
-                order, self._simplex_qrule(cfg, basiscls.name, order))
+            qrule = (self._simplex_qrule(cfg, basiscls.name, order)
+                     if self.fluxaa else None)
+            iq = (self._simplex_iqrules(cfg, order)
+                  if 'surf-flux' in self.antialias else None)
+            self.basis = basis = basiscls(order, qrule, iq)
         elif self.antialias:
             nq1 = (self._gl_n1(cfg, 'solver-elements-hex', 3, order)
                    if self.fluxaa else None)
@@ -152,6 +152,35 @@ class FluidElements:
             raise NotImplementedError(f'no default {name} quadrature rule '
                                       f'at order {order}')
         return AA_RULES[name][order]
+
+    @classmethod
+    def _simplex_iqrules(cls, cfg, order):
+        """Face rules of surface-flux anti-aliasing on tets and prisms
+        (``pyfr/shapes.py:159-185``): triangles by default the rule the
+        reference's selection picks (``simplex.SF_TRI_RULES``), or
+        [solver-interfaces-tri] quad-pts / quad-npts naming a tabulated
+        rule; quads Gauss-Legendre as for hexes."""
+        from pyfr_amd.quadtables import TABLES
+        from pyfr_amd.simplex import SF_TRI_RULES
+        sect = 'solver-interfaces-tri'
+        if cfg.hasopt(sect, 'quad-deg'):
+            raise NotImplementedError('quad-deg rule selection is not '
+                                      'supported; give quad-pts and '
+                                      'quad-npts')
+        if cfg.hasopt(sect, 'quad-pts') or cfg.hasopt(sect, 'quad-npts'):
+            key = ('tri', cfg.get(sect, 'quad-pts'),
+                   cfg.getint(sect, 'quad-npts'))
+            if key not in TABLES or TABLES[key][1][0] is None:
+                raise NotImplementedError(f'quadrature rule {key[1:]} is '
+                                          'not tabulated for tri')
+            tri = key[1:]
+        elif order in SF_TRI_RULES:
+            tri = SF_TRI_RULES[order]
+        else:
+            raise NotImplementedError(f'no default tri quadrature rule at '
+                                      f'order {order}')
+        return {'tri': tri,
+                'quad': cls._gl_n1(cfg, 'solver-interfaces-quad', 2, order)}
 
     @staticmethod
     def _gl_n1(cfg, sect, ndims, order):

@@ -361,7 +361,7 @@ MU = 0.02
 
 
 def _mixed_ns_bc(be, n, nt, npr, order, system, bcs=False, rsolver='rusanov',
-                 beta=0.0, tau=0.0, aa=False):
+                 beta=0.0, tau=0.0, aa=()):
     from oracle.simplex import OracleMixedSystem
     from pyfr_amd.mesh import SimplexBoxMesh
     from pyfr_amd.solvers import get_integrator
@@ -370,7 +370,7 @@ def _mixed_ns_bc(be, n, nt, npr, order, system, bcs=False, rsolver='rusanov',
                            system=system, beta=beta, tau=tau)
     cfg.set('backend-hip', 'fused', str(be.fused))
     if aa:
-        cfg.set('solver', 'anti-alias', 'flux')
+        cfg.set('solver', 'anti-alias', ', '.join(aa))
     c = {'gamma': 1.4}
     if system == 'navier-stokes':
         cfg.set('constants', 'mu', repr(MU))
@@ -395,15 +395,24 @@ def _mixed_ns_bc(be, n, nt, npr, order, system, bcs=False, rsolver='rusanov',
     verts = {et: v.transpose(0, 2, 1) for et, v in mesh.spts.items()}
     idx = {et: np.arange(v.shape[1]) for et, v in mesh.spts.items()}
     tabs = {et: _tables(et, order) for et in verts}
-    if aa:
-        from pyfr_amd.quadtables import TABLES
-        from pyfr_amd.simplex import AA_RULES
-        for et in tabs:
+    from pyfr_amd.quadtables import TABLES
+    from pyfr_amd.shapes import gauss_legendre
+    from pyfr_amd.simplex import AA_RULES, SF_TRI_RULES
+    for et in tabs:
+        if 'flux' in aa:
             tabs[et]['qrule'] = TABLES[(et, *AA_RULES[et][order])]
+        if 'surf-flux' in aa:
+            g, w = gauss_legendre(order + 2)
+            r = range(order + 2)
+            tabs[et]['iqrules'] = {
+                'tri': TABLES[('tri', *SF_TRI_RULES[order])],
+                'quad': (np.array([(g[i], g[j]) for j in r for i in r]),
+                         np.array([w[i]*w[j] for j in r for i in r]))}
     orc = OracleMixedSystem(verts, order, tabs,
                             [((L, L, (nt + npr)*L/n), idx)], rsolver=rsolver,
                             system=system, constants=c, periodic=periodic,
-                            bcs=obcs, antialias=('flux',) if aa else ())
+                            bcs=obcs, antialias=tuple(a for a in aa
+                                                      if a == 'flux'))
     return integ, orc, orc.ics(eval_ics(ics))
 
 
@@ -471,7 +480,29 @@ def test_simplex_flux_aa_rhs_parity(backend_factory, system, n, nt, npr,
     be = backend_factory(fused=fused != 'none', fuse_flux=fused == 'flux')
     print(f'\n[simplex aa] {system} n={n} tets={nt} prisms={npr} '
           f'p={order} fused={fused}', flush=True)
-    integ, orc, u0 = _mixed_ns_bc(be, n, nt, npr, order, system, aa=True)
+    integ, orc, u0 = _mixed_ns_bc(be, n, nt, npr, order, system,
+                                  aa=('flux',))
     for e in integ.system.ele_map.values():
         assert e.fluxaa and 'tdivf_flux' not in e.kernels
     _rhs_check(integ, orc, u0, 'simplex aa')
+
+
+@pytest.mark.parametrize('fused', [True, False])
+@pytest.mark.parametrize('system,order,aa,bcs', [
+    ('euler', 1, ('surf-flux',), False), ('euler', 2, ('surf-flux',), False),
+    ('euler', 3, ('surf-flux',), False), ('euler', 4, ('surf-flux',), False),
+    ('navier-stokes', 2, ('surf-flux',), False),
+    ('navier-stokes', 3, ('flux', 'surf-flux'), False),
+    ('euler', 2, ('flux', 'surf-flux'), True),
+    ('navier-stokes', 2, ('surf-flux',), True)])
+def test_simplex_surf_flux_rhs_parity(backend_factory, system, order, aa,
+                                      bcs, fused):
+    """Surface-flux anti-aliasing on tets and prisms: flux points at the
+    face rules (triangles and quads), dense M0 / M3, interfaces and
+    boundaries on the new points; alone and with flux anti-aliasing."""
+    be = backend_factory(fused=fused)
+    print(f'\n[simplex sf] {system} p={order} aa={aa} bcs={bcs} '
+          f'fused={fused}', flush=True)
+    integ, orc, u0 = _mixed_ns_bc(be, 3 if bcs else 2, 2 if bcs else 1, 1,
+                                  order, system, bcs=bcs, aa=aa)
+    _rhs_check(integ, orc, u0, 'simplex sf')

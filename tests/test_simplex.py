@@ -544,3 +544,100 @@ def test_oracle_simplex_flux_aa(layout, p):
            {'rho': 1.1, 'u': 0.3, 'v': -0.2, 'w': 0.1, 'p': 0.9}.items()}
     ra = oa.rhs(oa.ics(fns))
     assert max(np.abs(a).max() for a in ra.values()) < 1e-12
+
+
+# -- surface-flux anti-aliasing on tets / prisms ----------------------------
+
+def _iq_tables(p):
+    from pyfr_amd.shapes import gauss_legendre
+    from pyfr_amd.simplex import SF_TRI_RULES
+    g, w = gauss_legendre(p + 2)
+    quad = (np.array([(g[i], g[j]) for j in range(p + 2)
+                      for i in range(p + 2)]),
+            np.array([w[i]*w[j] for j in range(p + 2) for i in range(p + 2)]))
+    return {'tri': TABLES[('tri', *SF_TRI_RULES[p])], 'quad': quad}
+
+
+@pytest.mark.parametrize('p', [1, 2, 3, 4])
+def test_sf_tri_rules_exact(p):
+    """The default triangle-face rules integrate to degree 2(p + 1)."""
+    pts, wts = (np.array(x, dtype=float) for x in _iq_tables(p)['tri'])
+    deg = 2*(p + 1)
+    qp, qw = _rule('tri', deg + 2)
+    for e in it.product(range(deg + 1), repeat=2):
+        if sum(e) <= deg:
+            f = lambda x: x[:, 0]**e[0]*x[:, 1]**e[1]
+            assert abs(wts @ f(pts) - qw @ f(qp)) < 1e-13, e
+
+
+@pytest.mark.parametrize('name', ['tet', 'pri'])
+@pytest.mark.parametrize('p', [1, 2, 3, 4])
+def test_sf_operators(name, p):
+    """Product operators against the oracle's reference-form construction;
+    for the oracle: face data in the order-p face space give the standard
+    correction (the projection reproduces them), and each interface-rule
+    value is integrated with its weight."""
+    from pyfr_amd.simplex import SF_TRI_RULES, SIMPLEX_SHAPES
+    iq = _iq_tables(p)
+    o = ob.SimplexShapeOracle(name, p, _tables(name, p))
+    o.set_surface_quadrature(iq)
+    sh = SIMPLEX_SHAPES[name](p, None, {'tri': SF_TRI_RULES[p],
+                                        'quad': p + 2})
+    for e in ('M0', 'M3', 'M1 - M3*M2', 'M4 - M6*M0', 'M6'):
+        a, b = o.opmat(e), sh.opmat(e)
+        assert np.abs(a - b).max() < 1e-11*max(1, np.abs(a).max()), e
+
+    std = _oracle(name, p)
+    rng = np.random.default_rng(p)
+    gs, gq = np.empty(std.nfpts), np.empty(o.nfpts)
+    offs, offq = np.cumsum([0] + std.nfacefpts), np.cumsum([0] + o.nfacefpts)
+    for f, (kind, _, _) in enumerate(o.faces):
+        g, _ = _randpoly(rng, 2, p, 'pri' if kind == 'quad' else 'tri')
+        if kind == 'quad':
+            # the quad face space is the tensor one
+            c = rng.standard_normal((p + 1, p + 1))
+            g = lambda x, c=c: sum(c[i, j]*x[:, 0]**i*x[:, 1]**j
+                                   for i in range(p + 1)
+                                   for j in range(p + 1))
+        gs[offs[f]:offs[f + 1]] = g(std._frule[kind])
+        gq[offq[f]:offq[f + 1]] = g(np.asarray(iq[kind][0], dtype=float))
+    assert np.abs(o.m3 @ gq - std.m3 @ gs).max() < 1e-11*np.abs(gs).max()
+
+    w = np.array(_tables(name, p)['upts'][1])
+    col = w @ o.m3
+    ref = np.concatenate([np.asarray(iq[k][1]) for k, _, _ in o.faces])
+    assert np.abs(np.abs(col) - ref).max() < 1e-12
+
+
+@pytest.mark.parametrize('layout,p', [((2, 2, 0), 2), ((2, 0, 2), 2),
+                                      ((2, 1, 1), 3), ((2, 1, 1), 1)])
+def test_oracle_simplex_sf_rhs_identity(layout, p):
+    """Uniform density and pressure, a different constant velocity in each
+    element: each face sees two constant states, so the correction input is
+    a constant on each face, which the projection reproduces: the
+    surface-flux RHS equals the standard one."""
+    from oracle.simplex import OracleMixedSystem
+    from pyfr_amd.mesh import SimplexBoxMesh
+    n, nt, npr = layout
+    m = SimplexBoxMesh.stacked(n, nt, npr)
+    L = 2*np.pi
+    verts = {et: v.transpose(0, 2, 1) for et, v in m.spts.items()}
+    idx = {et: np.arange(v.shape[1]) for et, v in m.spts.items()}
+    blocks = [((L, L, (nt + npr)*L/n), idx)]
+    o = OracleMixedSystem(verts, p, {et: _tables(et, p) for et in verts},
+                          blocks)
+    tabs = {et: dict(_tables(et, p), iqrules=_iq_tables(p)) for et in verts}
+    osf = OracleMixedSystem(verts, p, tabs, blocks)
+    rng = np.random.default_rng(p)
+    u = {}
+    for et, oe in o.et.items():
+        v = 0.3*rng.standard_normal((3, oe.neles))
+        uu = np.empty((oe.nupts, 5, oe.neles))
+        uu[:, 0] = 1.2
+        uu[:, 1:4] = 1.2*v[None]
+        uu[:, 4] = (1/1.4)/0.4 + 0.6*(v**2).sum(axis=0)[None]
+        u[et] = uu
+    r, rs = o.rhs(u), osf.rhs(u)
+    for et in r:
+        assert np.abs(r[et]).max() > 1e-2
+        assert np.abs(rs[et] - r[et]).max() < 1e-11*np.abs(r[et]).max()
