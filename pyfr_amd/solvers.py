@@ -23,9 +23,9 @@ written against ``pyfr/backends/base``:
   ``pyfr/integrators/explicit/controllers.py:42-60``.
 
 Scope: meshes of hexahedra, straight-sided tetrahedra and right prisms on
-one GPU, periodic or with the reference's fixed-value boundary conditions
-(not the controlled char-riem-inv variants nor time-dependent boundary
-values); anti-aliasing (flux, surf-flux) on hexes only; no MPI interfaces,
+one GPU, periodic or with the reference's boundary conditions (values may
+depend on x, y, z and t; not the controlled char-riem-inv variants);
+anti-aliasing (flux, surf-flux) on every element type; no MPI interfaces,
 no source terms.
 """
 from functools import cached_property

@@ -506,3 +506,86 @@ def test_simplex_surf_flux_rhs_parity(backend_factory, system, order, aa,
     integ, orc, u0 = _mixed_ns_bc(be, 3 if bcs else 2, 2 if bcs else 1, 1,
                                   order, system, bcs=bcs, aa=aa)
     _rhs_check(integ, orc, u0, 'simplex sf')
+
+
+@pytest.mark.parametrize('precision', ['single', 'double'])
+def test_flux_fused_in_place(backend_factory, precision):
+    """The RK steppers compute RHS in place (u and the output the same
+    bank).  The flux-fused kernel reads u and writes its output in one
+    launch, so only forms with one block per element block and a barrier
+    before the writes are safe; an in-place call runs one of those
+    whatever the plan's variant.  Checked on 8,192 element blocks (more
+    blocks than the device holds at once) for every variant: the in-place
+    result equals, bit for bit, the out-of-place result of the form the
+    in-place call runs."""
+    import ctypes as ct
+    from pyfr_amd._lib import FrHipError
+    be = backend_factory(precision=precision)
+    lib = be.lib
+    dt = np.float32 if precision == 'single' else np.float64
+    code = 1 if precision == 'single' else 0
+    m, nu, nf = 20, 20, 40                   # tet p = 3 shapes
+    neles = 64*8192
+    ncol = 5*neles
+    rng = np.random.default_rng(7)
+    a0 = rng.standard_normal((m, 3*nu))
+    a1 = rng.standard_normal((m, nf))
+    plan = ct.c_void_p()
+    lib.tdivf_plan_create(ct.byref(plan), m, nu, nf, a0.ctypes.data,
+                          a1.ctypes.data, code)
+    u = np.empty((nu, neles//64, 5, 64), dtype=dt)
+    u[:, :, 0] = 1 + 0.1*rng.random((nu, neles//64, 64))
+    u[:, :, 1:4] = 0.1*rng.standard_normal((nu, neles//64, 3, 64))
+    u[:, :, 4] = 2.5 + 0.1*rng.random((nu, neles//64, 64))
+    u = u.reshape(nu, ncol)
+    sm = np.zeros((9, neles), dtype=dt)
+    sm[0] = sm[4] = sm[8] = 1.0
+    sm += (0.01*rng.standard_normal((9, neles))).astype(dt)
+    fn = rng.standard_normal((nf, ncol)).astype(dt)
+    rcp = (1 + 0.1*rng.random((1, neles))).astype(dt)
+    sz = u.nbytes
+    bufs = {}
+
+    def dev(name, arr):
+        p = ct.c_void_p()
+        lib.malloc(ct.byref(p), arr.nbytes)
+        lib.memcpy_htod(p, arr.ctypes.data, arr.nbytes, None)
+        bufs[name] = p
+        return p
+
+    du, dsm, dfn, drcp = (dev('u', u), dev('sm', sm), dev('fn', fn),
+                          dev('rcp', rcp))
+    dc = ct.c_void_p()
+    lib.malloc(ct.byref(dc), sz)
+    bufs['c'] = dc
+    dw = ct.c_void_p()
+    lib.malloc(ct.byref(dw), sz)
+    bufs['w'] = dw
+
+    def run(src, dst):
+        lib.tdivf_flux_euler(plan, ncol, src, ncol, dsm, neles, dfn, ncol,
+                             dst, ncol, 1, drcp, 0, None, 0.0, 0.0, 0, 1.4,
+                             None)
+        lib.stream_sync(None)
+        out = np.empty_like(u)
+        lib.memcpy_dtoh(out.ctypes.data, dst, sz, None)
+        return out
+
+    safe = 64 if precision == 'single' else 32
+    try:
+        ref = {}
+        for v in (safe, 32):
+            lib.tdivf_plan_set_variant(plan, v)
+            ref[v] = run(du, dc)
+        for v in (1, 2, 4, 32, safe):
+            print(f'\n[in place] {precision} variant {v}', flush=True)
+            lib.tdivf_plan_set_variant(plan, v)
+            lib.memcpy_dtod(dw, du, sz, None)
+            got = run(dw, dw)
+            want = ref[v if v in (32, safe) else safe]
+            assert np.isfinite(got).all()
+            assert np.array_equal(got, want), v
+    finally:
+        for p in bufs.values():
+            lib.free(p)
+        lib.tdivf_plan_destroy(plan)
