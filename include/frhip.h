@@ -147,9 +147,8 @@ int frhip_mul2_plan_set_variant(void *plan, int variant);
  * and without the LDS-staged operator (variant codes as above, without the
  * scalar kernel), variant 32: one block of five wavefronts per
  * 64-element block, the flux evaluated once per point into LDS, and (FP32)
- * variant 64: the 4x4x1 MFMA form, also one block per element block.  c
- * may be u itself (an RHS in place, as the RK steppers call it): such a
- * call runs variant 64 (FP32) or 32, whose barrier orders every read of u
+ * variant 64: the 4x4x1 MFMA form.  c may be u itself (an RHS in place):
+ * such a call runs variant 32, whose block barriers order every read of u
  * before any write of c, whatever the plan's variant. */
 int frhip_tdivf_plan_create(void **plan, int m, int nupts, int nfpts,
                             const double *a0, const double *a1, int dtype);

@@ -602,21 +602,23 @@ __device__ __forceinline__ void mfma4x4_chunk(
     (mfma4x4_k<G, KK>(acc, a, b[KK]), ...);
 }
 
-// One block of five wavefronts per 64-element block, wavefront v owning
-// variable v.  The barrier before the epilogue orders every wavefront's
-// reads of u before any write of c, so c may alias u (an RHS computed in
-// place, as the RK steppers do).
+// One wavefront per (64-element block, variable); the wavefronts of an
+// element block may sit in different blocks, so c must not alias u (an RHS
+// in place runs the block-cooperative form instead).
 template <int G, int EP>
-__global__ __launch_bounds__(320) void k_tdivf_flux4(
+__global__ __launch_bounds__(256) void k_tdivf_flux4(
     int m, int nupts, int nc0, int nc1, int k1, const float *__restrict__ af,
-    long ncol, const float *u, long ldu,
+    long ncol, const float *__restrict__ u, long ldu,
     const float *__restrict__ sm, long lds, const float *__restrict__ b1,
-    long ld1, float *c, long ldc, const float *__restrict__ rcp,
+    long ld1, float *__restrict__ c, long ldc, const float *__restrict__ rcp,
     long ldr, void *__restrict__ xa, double dta, double dtb, float gm1)
 {
-    const int lane = threadIdx.x & 63, v = threadIdx.x >> 6;
-    const long eb = blockIdx.x;
-    const long c0 = (eb*5 + v)*64, ecol = eb*64 + lane;
+    const int lane = threadIdx.x & 63;
+    const long w = (long)blockIdx.x*(blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (w*64 >= ncol) return;
+    const long c0 = w*64;
+    const int v = (int)((c0 >> 6) % 5);
+    const long eb = c0/320, ecol = eb*64 + lane;
 
     float S[3][3];
 #pragma unroll
@@ -707,7 +709,6 @@ __global__ __launch_bounds__(320) void k_tdivf_flux4(
             b[kk] = bl[min(16*q + kk, k1 - 1)*ld1];
         mfma4x4_chunk(acc, a, b, std::make_integer_sequence<int, 16>{});
     }
-    __syncthreads();
 
 #pragma unroll
     for (int g = 0; g < G; ++g)
@@ -1181,7 +1182,7 @@ static void launch_flux_ep(const FluxPlan *p, int wpb, long ncol,
 {
     if constexpr (sizeof(T) == 4) {
         if (wpb == FR_V4) {
-            const dim3 grid(ncol/320), block(320);
+            const dim3 grid(cdiv(ncol/64, 4)), block(256);
 #define FR_G4(GV)                                                           \
             case GV:                                                        \
                 hipLaunchKernelGGL((k_tdivf_flux4<GV, EP>), grid, block, 0, \
@@ -1302,10 +1303,11 @@ static int flux_launch(const FluxPlan *p, int ep, long ncol, const void *u,
         !variant_valid(wpb, (size_t)p->d.mt*(p->d.ks0 + p->d.ks1)*64*esz,
                        false))
         wpb &= ~FR_LDSV;
-    // an RHS in place (c == u): only the forms with one block per element
-    // block, whose barrier orders all reads of u before any write of c
-    if (c == u && wpb != FR_BLKV && wpb != FR_V4)
-        wpb = p->af4 ? FR_V4 : FR_BLKV;
+    // an RHS in place (c == u): only the block-cooperative form keeps an
+    // element block's five wavefronts in one block, whose barriers order
+    // all reads of u before any write of c
+    if (c == u)
+        wpb = FR_BLKV;
     const int rc = p->d.dtype == FRHIP_F64
         ? launch_flux_t<double>(p, wpb, ep, ncol, u, ldu, sm, lds, fn, ldf, c,
                                 ldc, rcp, ldr, xa, dta, dtb, gamma, s)

@@ -1359,25 +1359,31 @@ class BaseIntegrator:
 
 
 class RK4Integrator(BaseIntegrator):
-    """``pyfr/integrators/explicit/steppers.py:60-108``."""
-    nregs = 3
+    """``pyfr/integrators/explicit/steppers.py:60-108``, with one register
+    more than the reference: its stages 2-4 compute the RHS in place
+    (``rhs(r2, r2)``); here they write r3, so the flux-fused kernels never
+    read and write one bank.  Each combination keeps the reference's
+    coefficients and operand order (``add(dt/2, r2, 1, r0)`` with r2 = k
+    becomes ``add(0, r2, dt/2, r3, 1, r0)``: the same products and sum in
+    the axnpby kernel), so the arithmetic is the reference's."""
+    nregs = 4
     nrhs_per_step = 4
 
     def step(self, t, dt):
         add, rhs = self._add, self._rhs
-        r0, r1, r2 = 0, 1, 2
+        r0, r1, r2, r3 = 0, 1, 2, 3
         if r0 != self.idxcurr:
             r0, r1 = r1, r0
         rhs(t, r0, r1)
         add(0.0, r2, 1.0, r0, dt/2.0, r1)
-        rhs(t + dt/2.0, r2, r2)
-        add(dt/6.0, r1, 1.0, r0, dt/3.0, r2)
-        add(dt/2.0, r2, 1.0, r0)
-        rhs(t + dt/2.0, r2, r2)
-        add(1.0, r1, dt/3.0, r2)
-        add(dt, r2, 1.0, r0)
-        rhs(t + dt, r2, r2)
-        add(1.0, r1, dt/6.0, r2)
+        rhs(t + dt/2.0, r2, r3)
+        add(dt/6.0, r1, 1.0, r0, dt/3.0, r3)
+        add(0.0, r2, dt/2.0, r3, 1.0, r0)
+        rhs(t + dt/2.0, r2, r3)
+        add(1.0, r1, dt/3.0, r3)
+        add(0.0, r2, dt, r3, 1.0, r0)
+        rhs(t + dt, r2, r3)
+        add(1.0, r1, dt/6.0, r3)
         return r1
 
 
