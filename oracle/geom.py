@@ -116,16 +116,23 @@ def kopriva_smats_djacs(verts, order):
 
     Returns (smats (3, nmpts, 3, E), djacs (nmpts, E), mbasis, mpts).
     """
-    neles = verts.shape[-1]
     mord = max(order, 1)
     pts1d = np.linspace(-1, 1, mord + 1)
     mpts = np.array([p[::-1] for p in it.product(pts1d, repeat=3)])
-    nmpts = len(mpts)
     mbasis = HexBasis(mord, mpts)
 
     # Physical location of the pseudo grid points (nmpts, 3, E)
     x = hex_ploc(verts, mpts)
+    smats, djacs = kopriva_metric(x, mbasis, mpts)
+    return smats, djacs, mbasis, mpts
 
+
+def kopriva_metric(x, mbasis, mpts):
+    """The invariant curl form of S and |J| at the pseudo-grid points mpts
+    from their physical locations x (nmpts, 3, E), with the nodal basis
+    mbasis on mpts (any element type; ``elements.py:382-440``).
+    Returns (smats (3, nmpts, 3, E), djacs (nmpts, E))."""
+    nmpts, neles = len(mpts), x.shape[-1]
     jacop = np.rollaxis(mbasis.jac_nodal_basis_at(mpts), 2).reshape(-1, nmpts)
     jac = (jacop @ x.reshape(nmpts, -1)).reshape(nmpts, 3, 3, neles)
     jac = jac.transpose(1, 2, 0, 3)
@@ -144,7 +151,7 @@ def kopriva_smats_djacs(verts, order):
     x0, x1, x2 = jac
     x1cx2 = np.cross(x1, x2, axisa=0, axisb=0, axisc=1)
     djacs = np.einsum('ij...,ji...->j...', x0, x1cx2)
-    return smats, djacs, mbasis, mpts
+    return smats, djacs
 
 
 def smats_at(verts, order, pts):

@@ -20,6 +20,7 @@ import numpy as np
 
 from oracle.basis import PriBasis, SimplexShapeOracle, TetBasis
 from oracle.bcs import bccflux_euler, bccflux_ns, bcconu
+from oracle.geom import kopriva_metric
 from oracle.physics import RSOLVERS, inviscid_flux, pri_to_con, \
     viscous_flux_add
 
@@ -32,6 +33,27 @@ STD_VERTS = {
 LINBASIS = {'tet': TetBasis, 'pri': PriBasis}
 
 
+def std_ele(name, k):
+    """Equispaced order-k shape points: ``TetShape.std_ele``
+    (``pyfr/shapes.py:502-510``) and ``PriShape.std_ele`` (``:555-562``)."""
+    x = np.linspace(-1, 1, k + 1)
+    if name == 'tet':
+        return np.array([(p, q, r) for i, r in enumerate(x)
+                         for j, q in enumerate(x[:k + 1 - i])
+                         for p in x[:k + 1 - i - j]])
+    return np.array([(p, q, r) for r in x for i, q in enumerate(x)
+                     for p in x[:k + 1 - i]])
+
+
+def shape_order(name, nspts):
+    for k in range(1, 16):
+        n = ((k + 1)*(k + 2)*(k + 3)//6 if name == 'tet'
+             else (k + 1)**2*(k + 2)//2)
+        if n == nspts:
+            return k
+    raise ValueError('Invalid shape point count')
+
+
 class _Etype:
     def __init__(self, name, order, tables, verts):
         """tables as for SimplexShapeOracle, plus 'qrule' = (points,
@@ -42,11 +64,22 @@ class _Etype:
         self.sh = sh = SimplexShapeOracle(name, order, tables)
         if 'iqrules' in tables:
             sh.set_surface_quadrature(tables['iqrules'])
-        self.verts = verts                          # (nv, 3, E)
+        self.verts = verts                          # (nspts, 3, E)
         self.neles = verts.shape[-1]
         self.nupts, self.nfpts = sh.nupts, sh.nfpts
-        lb = LINBASIS[name](1, STD_VERTS[name])
-        self._lb = lb
+        # shape basis on the shape points (pyfr/shapes.py:49-50); curved
+        # elements (order-k shape points) take the stored metric in the
+        # invariant form on std_ele(max(p, 1)) (pyfr/shapes.py:52-61,
+        # pyfr/solvers/base/elements.py:382-440)
+        k = shape_order(name, verts.shape[0])
+        self.curved = k > 1
+        self._lb = LINBASIS[name](k, std_ele(name, k))
+        if self.curved:
+            mord = max(order, 1)
+            mpts = std_ele(name, mord)
+            self._mb = LINBASIS[name](mord, mpts)
+            x = self.ploc(mpts)
+            self._sm, self._dj = kopriva_metric(x, self._mb, mpts)
         self.M0 = sh.opmat('M0')
         self.M132 = sh.opmat('M1 - M3*M2')
         self.M3 = sh.opmat('M3')
@@ -63,17 +96,18 @@ class _Etype:
             self.M1329 = sh.opmat('(M1 - M3*M2)*M9')
             self.smats_q, _ = self._smats_djac(sh.qpts)
 
-        # Normals at the flux points from the Jacobian there
-        dlf = lb.jac_nodal_basis_at(sh.fpts)
-        jf = np.einsum('avp,vbe->pabe', dlf, verts)
-        sf = np.empty_like(jf)
-        for i, (a, b) in enumerate([(1, 2), (2, 0), (0, 1)]):
-            sf[:, i] = np.cross(jf[:, a], jf[:, b], axis=1)
+        # Normals S^T n at the flux points (elements.py:364-380)
+        sf, _ = self._smats_djac(sh.fpts)
         self.pnorm = np.einsum('pi,pike->pek', sh.norm_fpts, sf)  # (Nf,E,3)
 
     def _smats_djac(self, pts):
-        """S = |J| J^-1 by cofactors and |J| at reference points from the
-        Jacobian j[a][b] = dX_b/dxi_a of the linear map."""
+        """S = |J| J^-1 and |J| at reference points: by cofactors of the
+        Jacobian j[a][b] = dX_b/dxi_a of the linear map, or interpolated
+        from the pseudo-grid points for curved elements."""
+        if self.curved:
+            m0 = self._mb.nodal_basis_at(pts)
+            s = np.einsum('pm,imje->pije', m0, self._sm)
+            return s, m0 @ self._dj
         dl = self._lb.jac_nodal_basis_at(pts)        # (3, nv, npts)
         j = np.einsum('avp,vbe->pabe', dl, self.verts)
         s = np.empty_like(j)

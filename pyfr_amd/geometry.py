@@ -220,3 +220,54 @@ class AffineGeometry:
         """Physical normals S^T n (``elements.py:364-380``): (npts, E, 3)."""
         return np.einsum('pi,eik->pek', np.asarray(norm, dtype=float),
                          self.smats)
+
+
+class SimplexLagrange:
+    """Nodal basis of the order-k tet / prism space through points pts,
+    with the layouts of ``TensorLagrange``."""
+
+    def __init__(self, kind, k, pts):
+        from pyfr_amd.simplex import NodalBasis
+        self.pts = np.asarray(pts, dtype=float)
+        self._nb = NodalBasis(kind, k, self.pts)
+
+    def nodal_basis_at(self, pts):
+        """(npts, nnodes)"""
+        return self._nb.nodal_basis_at(np.atleast_2d(pts))
+
+    def jac_nodal_basis_at(self, pts):
+        """(3, npts, nnodes)"""
+        return self._nb.jac_nodal_basis_at(np.atleast_2d(pts)).swapaxes(1, 2)
+
+
+def simplex_shape_order(name, nspts):
+    """Shape order k of a tet ((k+1)(k+2)(k+3)/6 points) or prism
+    ((k+1)^2(k+2)/2 points)."""
+    for k in range(1, 16):
+        n = ((k + 1)*(k + 2)*(k + 3)//6 if name == 'tet'
+             else (k + 1)**2*(k + 2)//2)
+        if n == nspts:
+            return k
+    raise ValueError(f'Invalid {name} shape point count {nspts}')
+
+
+class CurvedSimplexGeometry(HexGeometry):
+    """Metric terms of curved tets and prisms with order-k shape points
+    (``std_ele(k)`` order): the same pipeline as for hexes, with the shape
+    basis the order-k nodal basis on the shape points and the pseudo-grid
+    basis the order-max(p, 1) one on ``std_ele(max(p, 1))``
+    (``pyfr/shapes.py:49-61``, ``pyfr/solvers/base/elements.py:382-440``)."""
+
+    def __init__(self, eles, order, name):
+        from pyfr_amd.simplex import SIMPLEX_SHAPES
+        self.eles = np.asarray(eles, dtype=float)
+        self.nspts, self.neles, self.ndims = self.eles.shape
+        self.name = name
+        self.sorder = simplex_shape_order(name, self.nspts)
+        self.order = order
+        shape = SIMPLEX_SHAPES[name]
+        self.sbasis = SimplexLagrange(name, self.sorder,
+                                      shape.std_ele(self.sorder))
+        mord = max(order, 1)
+        self.mbasis = SimplexLagrange(name, mord, shape.std_ele(mord))
+        self._smats_djacs = None

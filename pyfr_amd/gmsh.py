@@ -17,10 +17,9 @@ reference's reader and nodal assembler (``pyfr/readers/gmsh.py:28-489``,
 * elements of a type keep their file order, grouped by (Gmsh type,
   physical entities) sorted, as the reference's volume merge stacks them.
 
-Scope: straight-sided first-order elements (Gmsh types 4, 5, 6; surfaces 2
-and 3).  Higher-order and curved elements, pyramids and 2-D meshes are
-declined, since the solver's geometry paths take linear hexes and affine
-tets and prisms.  The ``.pyfrm`` (HDF5) writer is not built: h5py is not
+Scope: first-order elements (Gmsh types 4, 5, 6; surfaces 2 and 3).
+Higher-order node orderings (curved elements come from the generated
+meshes), pyramids and 2-D meshes are declined.  The ``.pyfrm`` (HDF5) writer is not built: h5py is not
 available here, and the solver takes the mesh object directly.
 """
 from collections import defaultdict

@@ -227,11 +227,16 @@ class SimplexBoxMesh:
     (etype, eidx, fidx), each face pair once, left side the smaller
     (etype index, face index, element index) key, sorted by left side.
     ``bcon``: {name: [(etype, eidx, fidx), ...]} in that key order.
+
+    sorder, warp: order-k shape points (``std_ele(k)`` order) of each
+    element's affine map, moved by ``warp(x, y, z) -> (x, y, z)`` (curved
+    elements; the warp must respect the periods).
     """
     ndims = 3
     etype_order = ('tet', 'pri')
 
-    def __init__(self, blocks, periodic=(True, True, True)):
+    def __init__(self, blocks, periodic=(True, True, True), sorder=1,
+                 warp=None):
         self.periodic = tuple(bool(p) for p in periodic)
         verts = {et: [] for et in self.etype_order}
         nodes = {et: [] for et in self.etype_order}
@@ -260,20 +265,36 @@ class SimplexBoxMesh:
         for et in self.etype_order:
             if verts[et]:
                 v = np.concatenate(verts[et])           # (E, nv, 3)
-                self.spts[et] = np.ascontiguousarray(v.transpose(1, 0, 2))
+                sp = np.ascontiguousarray(v.transpose(1, 0, 2))
+                if sorder > 1 or warp is not None:
+                    sp = self._shape_points(et, sp, sorder, warp)
+                self.spts[et] = sp
                 self._nodes[et] = nodes[et]
         self.etypes = list(self.spts)
         self.neles = sum(v.shape[1] for v in self.spts.values())
         self.con = self._connect()
 
+    @staticmethod
+    def _shape_points(et, verts, k, warp):
+        """(nspts, E, 3) order-k shape points of the affine map of verts
+        (nverts, E, 3), warped."""
+        from pyfr_amd.geometry import AffineGeometry
+        from pyfr_amd.simplex import SIMPLEX_SHAPES
+        ref = SIMPLEX_SHAPES[et].std_ele(k)
+        x = AffineGeometry(verts, et).ploc_at(ref)       # (nspts, 3, E)
+        if warp is not None:
+            x = np.stack(warp(x[:, 0], x[:, 1], x[:, 2]), axis=1)
+        return np.ascontiguousarray(x.transpose(0, 2, 1))
+
     @classmethod
     def stacked(cls, n, ntet, npri, length=2*np.pi,
-                periodic=(True, True, True)):
+                periodic=(True, True, True), sorder=1, warp=None):
         """n x n x (ntet + npri) cells of size length/n: tets below, prisms
         above (tet-prism interfaces at both ends of the prism slab when
         periodic in z)."""
         layers = [(et, k) for et, k in (('tet', ntet), ('pri', npri)) if k]
-        return cls([((n, n, ntet + npri), length/n, layers)], periodic)
+        return cls([((n, n, ntet + npri), length/n, layers)], periodic,
+                   sorder, warp)
 
     @classmethod
     def disjoint(cls, ntet, npri, length=2*np.pi):

@@ -35,7 +35,8 @@ import time
 
 import numpy as np
 
-from pyfr_amd.geometry import AffineGeometry, HexGeometry, hex_corner_index
+from pyfr_amd.geometry import (AffineGeometry, CurvedSimplexGeometry,
+                               HexGeometry, hex_corner_index)
 from pyfr_amd.mesh import sorted_face_fpts, sorted_face_fpts_structured
 from pyfr_amd.shapes import SHAPES as HEX_SHAPES
 from pyfr_amd.simplex import SIMPLEX_SHAPES
@@ -120,10 +121,21 @@ class FluidElements:
         self.name = basis.name
         self.nupts, self.nfpts = basis.nupts, basis.nfpts
         self.nfacefpts = basis.nfacefpts
-        # tets and prisms here are straight-sided: one S and |J| per element
-        self.affine = self.name in SIMPLEX_SHAPES
-        if self.affine:
-            self.geom = AffineGeometry(eles, self.name)
+        # straight-sided tets and right prisms (vertices only) have one S
+        # and |J| per element; other tets and prisms (order-k shape points,
+        # or prisms whose map is not affine) take the stored metric at every
+        # point, all elements of the type in the curved region
+        self.affine = False
+        self.curved_simplex = False
+        if self.name in SIMPLEX_SHAPES:
+            try:
+                if self.nspts != basiscls.nverts:
+                    raise ValueError('curved')
+                self.geom = AffineGeometry(eles, self.name)
+                self.affine = True
+            except ValueError:
+                self.geom = CurvedSimplexGeometry(eles, order, self.name)
+                self.curved_simplex = True
         else:
             self.geom = HexGeometry(eles, order)
 
@@ -230,7 +242,7 @@ class FluidElements:
 
     @cached_property
     def srtd_face_fpts(self):
-        if self.affine:
+        if self.name != 'hex':
             return sorted_face_fpts(self.plocfpts, self.basis.facefpts)
         return sorted_face_fpts_structured(self.plocfpts, self.basis.facefpts)
 
@@ -249,7 +261,7 @@ class FluidElements:
     def set_backend(self, backend, nonce, linoff):
         self._be = backend
         self.grad_fusion = not (backend.blocks or self.fluxaa)
-        if self.basis.order >= 2:
+        if self.basis.order >= 2 and not self.curved_simplex:
             self.linoff = linoff - linoff % -backend.csubsz
         else:
             self.linoff = self.neles

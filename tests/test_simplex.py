@@ -641,3 +641,89 @@ def test_oracle_simplex_sf_rhs_identity(layout, p):
     for et in r:
         assert np.abs(r[et]).max() > 1e-2
         assert np.abs(rs[et] - r[et]).max() < 1e-11*np.abs(r[et]).max()
+
+
+# -- curved tets and prisms -------------------------------------------------
+
+def _warp(x, y, z):
+    return (x + 0.12*np.sin(y)*np.sin(z), y + 0.08*np.sin(x)*np.cos(z),
+            z + 0.05*np.sin(x + y))
+
+
+def _curved_oracle(n, nt, npr, p, sorder=2, warp=_warp, **kw):
+    from oracle.simplex import OracleMixedSystem
+    from pyfr_amd.mesh import SimplexBoxMesh
+    m = SimplexBoxMesh.stacked(n, nt, npr, sorder=sorder, warp=warp)
+    L = 2*np.pi
+    verts = {et: v.transpose(0, 2, 1) for et, v in m.spts.items()}
+    idx = {et: np.arange(v.shape[1]) for et, v in m.spts.items()}
+    o = OracleMixedSystem(verts, p, {et: _tables(et, p) for et in verts},
+                          [((L, L, (nt + npr)*L/n), idx)], **kw)
+    return m, o
+
+
+@pytest.mark.parametrize('p', [1, 2, 3])
+def test_curved_simplex_geometry_matches_oracle(p):
+    from pyfr_amd.geometry import CurvedSimplexGeometry
+    m, o = _curved_oracle(2, 1, 1, p)
+    for et, v in m.spts.items():
+        g = CurvedSimplexGeometry(v, p, et)
+        oe = o.et[et]
+        assert oe.curved
+        sh = oe.sh
+        assert np.abs(g.ploc_at(sh.upts) - oe.ploc(sh.upts)).max() < 1e-13
+        assert np.abs(g.smat_at(sh.upts).transpose(1, 0, 2, 3) -
+                      oe.smats).max() < 1e-12
+        assert np.abs(g.rcpdjac_at(sh.upts) - oe.rcpdjac).max() < 1e-12
+        assert np.abs(g.pnorm_at(sh.fpts, sh.norm_fpts) - oe.pnorm).max() \
+            < 1e-12
+
+
+@pytest.mark.parametrize('p', [1, 3])
+def test_curved_simplex_straight_equals_affine(p):
+    """Straight elements given as order-2 shape points take the stored
+    metric; it equals the affine one, and so does the RHS."""
+    from tests.fr_cases import eval_ics, mode_ics
+    _, oc = _curved_oracle(2, 1, 1, p, warp=None)
+    _, oa = _curved_oracle(2, 1, 1, p, sorder=1, warp=None)
+    assert all(e.curved for e in oc.et.values())
+    u = oa.ics(eval_ics(mode_ics(seed=6)))
+    rc, ra = oc.rhs(u), oa.rhs(u)
+    for et in ra:
+        assert np.abs(rc[et] - ra[et]).max() < 1e-11*np.abs(ra[et]).max()
+
+
+@pytest.mark.parametrize('layout', [(2, 2, 0), (2, 0, 2), (2, 1, 1)])
+def test_curved_simplex_free_stream_and_conservation(layout):
+    """On a warped mesh the invariant-form metric keeps a uniform state
+    steady (a metric taken point by point from the exact Jacobian, planted
+    in its place, does not) and the totals are conserved."""
+    from tests.fr_cases import eval_ics, mode_ics
+    n, nt, npr = layout
+    _, o = _curved_oracle(n, nt, npr, 2)
+    fns = {k: (lambda v: lambda x, y, z: v + 0*x)(v) for k, v in
+           {'rho': 1.1, 'u': 0.3, 'v': -0.2, 'w': 0.1, 'p': 0.9}.items()}
+    r = o.rhs(o.ics(fns))
+    assert max(np.abs(a).max() for a in r.values()) < 1e-11
+    u = o.ics(eval_ics(mode_ics(seed=2)))
+    r = o.rhs(u)
+    for v in range(5):
+        tot = sum((np.array(_tables(et, 2)['upts'][1])[:, None] *
+                   r[et][:, v]/oe.rcpdjac).sum() for et, oe in o.et.items())
+        assert abs(tot) < 1e-10
+    # plant: S and |J| by cofactors of the exact Jacobian at every point
+    for oe in o.et.values():
+        sh = oe.sh
+        lb = oe._lb
+        for pts, attr in ((sh.upts, 'smats'), (sh.fpts, None)):
+            dl = lb.jac_nodal_basis_at(pts)
+            j = np.einsum('avp,vbe->pabe', dl, oe.verts)
+            s = np.empty_like(j)
+            for i, (a, b) in enumerate([(1, 2), (2, 0), (0, 1)]):
+                s[:, i] = np.cross(j[:, a], j[:, b], axis=1)
+            if attr:
+                oe.smats = s
+            else:
+                oe.pnorm = np.einsum('pi,pike->pek', sh.norm_fpts, s)
+    r = o.rhs(o.ics(fns))
+    assert max(np.abs(a).max() for a in r.values()) > 1e-6
