@@ -1107,7 +1107,8 @@ class BaseSystem:
         g = self.backend.graph()
         g.add_all(ks)
         g.commit()
-        g.final = ks[-1]
+        # every element type's stage kernel takes the stage arguments
+        g.stage_kernels = [k for k in ks if 'dtb' in k.rtargs]
         return [g]
 
     @property
@@ -1136,7 +1137,8 @@ class BaseSystem:
                 uin, fout, xa, 'rk45-mixed' if mixed else 'rk45')
         self._set_time(t)
         for g in self._kcache[key]:
-            g.final.bind(dta=dta, dtb=dtb, last=int(last))
+            for k in g.stage_kernels:
+                k.bind(dta=dta, dtb=dtb, last=int(last))
             self.backend.run_graph(g)
 
     def ele_scal_upts(self, idx):

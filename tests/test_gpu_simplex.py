@@ -263,9 +263,10 @@ def test_simplex_rk4_trajectory(backend_factory):
     assert err < 1e-12
 
 
-def _rk4_dict(rhs, u, dt):
-    """oracle.rhs.rk4_step over dicts of per-type arrays."""
+def _rk4_dict(rhs, u, dt, step=None):
+    """oracle.rhs.rk4_step (or step) over dicts of per-type arrays."""
     from oracle.rhs import rk4_step
+    step = step or rk4_step
     names = list(u)
     shapes = [u[n].shape for n in names]
     sizes = [u[n].size for n in names]
@@ -279,7 +280,27 @@ def _rk4_dict(rhs, u, dt):
             out[n] = v[o:o + sz].reshape(sh)
             o += sz
         return out
-    return unflat(rk4_step(lambda v: flat(rhs(unflat(v))), flat(u), dt))
+    return unflat(step(lambda v: flat(rhs(unflat(v))), flat(u), dt))
+
+
+@pytest.mark.parametrize('fuse_flux', [True, False])
+def test_simplex_rk45_trajectory(backend_factory, fuse_flux):
+    """RK45 with the stage update fused into each element type's
+    divergence kernel on a mixed tet/prism mesh: every type's kernel takes
+    the stage arguments."""
+    from oracle.rhs import rk45_step
+    be = backend_factory(fuse_flux=fuse_flux)
+    integ, orc, u0 = _mixed(be, 3, 2, 1, 3, scheme='rk45', dt=2e-3)
+    assert integ.system.fused_stages and len(integ.system.ele_types) == 2
+    names = integ.system.ele_types
+    integ.run_steps(10, 2e-3)
+    u = u0
+    for _ in range(10):
+        u = _rk4_dict(orc.rhs, u, 2e-3, rk45_step)
+    err = rel_l2(_cat(dict(zip(names, integ.soln)), names), _cat(u, names))
+    print(f'\n[simplex rk45] fuse_flux={fuse_flux} 10 steps rel L2 '
+          f'{err:.3e}', flush=True)
+    assert err < 1e-12
 
 
 def test_simplex_fp32_rhs_and_trajectory(backend_factory):
