@@ -149,7 +149,15 @@ int frhip_mul2_plan_set_variant(void *plan, int variant);
  * 64-element block, the flux evaluated once per point into LDS, and (FP32)
  * variant 64: the 4x4x1 MFMA form.  c may be u itself (an RHS in place):
  * such a call runs variant 32, whose block barriers order every read of u
- * before any write of c, whatever the plan's variant. */
+ * before any write of c, whatever the plan's variant.
+ * ep 10..13: RK4 stages 1..4 on four registers, with the update axnpby of
+ * pyfr/integrators/explicit/steppers.py:67-108 in the epilogue (k = the
+ * stage RHS, dta = dt; xa points to a device array of the two pointers
+ * {r1, r0}; c receives the next stage's input):
+ *   10: r1 = k;                        c = r0 + dt/2 k
+ *   11: r1 = dt/6 r1 + r0 + dt/3 k;    c = dt/2 k + r0
+ *   12: r1 = r1 + dt/3 k;              c = dt k + r0
+ *   13: r1 = r1 + dt/6 k               (c unused). */
 int frhip_tdivf_plan_create(void **plan, int m, int nupts, int nfpts,
                             const double *a0, const double *a1, int dtype);
 int frhip_tdivf_plan_destroy(void *plan);

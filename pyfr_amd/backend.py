@@ -627,10 +627,16 @@ class MulProvider:
                                 smats.leaddim, fn.data, fn.leaddim, gamma, 5,
                                 ct.byref(v))
             self._tuned[(plan, ncol)] = v.value
-        ep = {None: 1, 'rk45': 2, 'rk45-mixed': 3}[stage]
+        ep = {None: 1, 'rk45': 2, 'rk45-mixed': 3, 'rk4-1': 10, 'rk4-2': 11,
+              'rk4-3': 12, 'rk4-4': 13}[stage]
+        if ep >= 10:
+            # RK4 stages: xa = (r1, r0), passed as a device pointer pair
+            xap = self._ptr_pair(*xa)
+        else:
+            xap = None if xa is None else xa.data
         args = (plan, ncol, u.data, u.leaddim, smats.data, smats.leaddim,
                 fn.data, fn.leaddim, out.data, out.leaddim, ep, rcpdjac.data,
-                _rstride(rcpdjac), None if xa is None else xa.data)
+                _rstride(rcpdjac), xap)
 
         def fn_(stream, dta=0.0, dtb=0.0, last=0):
             lib.tdivf_flux_euler(*args, dta, dtb, int(last), gamma, stream)
@@ -640,8 +646,26 @@ class MulProvider:
         k.kind = 'dense'
         return k
 
+    def _ptr_pair(self, r1, r0):
+        """A device array holding the two bank pointers {r1, r0} (made once
+        per pair, kept for the provider's lifetime)."""
+        if not hasattr(self, '_pairs'):
+            self._pairs = {}
+        key = (r1.data, r0.data)
+        if key not in self._pairs:
+            lib = self.backend.lib
+            h = np.array(key, dtype=np.uint64)
+            p = ct.c_void_p()
+            lib.malloc(ct.byref(p), h.nbytes)
+            lib.memcpy_htod(p, h.ctypes.data, h.nbytes, None)
+            lib.stream_sync(None)
+            self._pairs[key] = (p.value, r1, r0)
+        return self._pairs[key][0]
+
     def __del__(self):
         try:
+            for p, _, _ in getattr(self, '_pairs', {}).values():
+                self.backend.lib.free(p)
             for plan, _, _ in self._plans.values():
                 self.backend.lib.mul_plan_destroy(plan)
             for plan, _, _, _ in self._plans2.values():

@@ -89,6 +89,39 @@ __device__ __forceinline__ void epi1(T acc, T *__restrict__ c, long off,
             T *r1 = static_cast<T *>(xa) + off;
             *r1 = *r1 + T(dtb)*t;
         }
+        else if constexpr (EP >= 10) {
+            // RK4 stage EP - 9 with four registers (dta = dt): xa points to
+            // the device pair {r1, r0}; c receives the next stage's input.
+            // Each update is the axnpby of pyfr/integrators/explicit/
+            // steppers.py:67-108 it replaces, operands in the same order.
+            T *const *pp = static_cast<T *const *>(xa);
+            T *r1 = pp[0] + off;
+            const T r0 = pp[1][off];
+            if constexpr (EP == 10) {           // r1 = k; r2 = r0 + dt/2 k
+                *r1 = t;
+                T a = T(1)*r0;
+                a = a + T(dta/2.0)*t;
+                c[off] = a;
+            }
+            else if constexpr (EP == 11) {      // r1 = dt/6 r1 + r0 + dt/3 k
+                T a = T(dta/6.0)*(*r1);         // r3 = dt/2 k + r0
+                a = a + T(1)*r0;
+                a = a + T(dta/3.0)*t;
+                *r1 = a;
+                T b = T(dta/2.0)*t;
+                b = b + T(1)*r0;
+                c[off] = b;
+            }
+            else if constexpr (EP == 12) {      // r1 += dt/3 k; r2 = dt k + r0
+                *r1 += T(dta/3.0)*t;
+                T b = T(dta)*t;
+                b = b + T(1)*r0;
+                c[off] = b;
+            }
+            else {                              // r1 += dt/6 k
+                *r1 += T(dta/6.0)*t;
+            }
+        }
         else {
             double *r1 = static_cast<double *>(xa) + off;
             const double t1 = *r1, td = (double)t;
@@ -1257,6 +1290,7 @@ static int launch_flux_t(const FluxPlan *p, int wpb, int ep, long ncol,
         return 0;
     switch (ep) {
         FR_EP(0) FR_EP(1) FR_EP(2) FR_EP(4)
+        FR_EP(10) FR_EP(11) FR_EP(12) FR_EP(13)
         case 3: case 5:
             if constexpr (sizeof(T) == 4) {
                 if (ep == 3)
@@ -1306,7 +1340,7 @@ static int flux_launch(const FluxPlan *p, int ep, long ncol, const void *u,
     // an RHS in place (c == u): only the block-cooperative form keeps an
     // element block's five wavefronts in one block, whose barriers order
     // all reads of u before any write of c
-    if (c == u)
+    if (c == u && ep != 13)
         wpb = FR_BLKV;
     const int rc = p->d.dtype == FRHIP_F64
         ? launch_flux_t<double>(p, wpb, ep, ncol, u, ldu, sm, lds, fn, ldf, c,
@@ -1429,11 +1463,13 @@ int frhip_tdivf_flux_euler(void *plan, int ncol, const void *u, long ldu,
     FR_API_BEGIN
     FluxPlan *p = static_cast<FluxPlan *>(plan);
     FR_CHECK(p, "null plan");
-    FR_CHECK(ep >= 0 && ep <= 3, "invalid epilogue");
+    FR_CHECK((ep >= 0 && ep <= 3) || (ep >= 10 && ep <= 13),
+             "invalid epilogue");
     FR_CHECK(ep != 3 || p->d.dtype == FRHIP_F32,
              "the mixed RK epilogue needs FP32 operands");
+    FR_CHECK(ep < 10 || xa, "the RK4 epilogues need the {r1, r0} pair");
     if (ncol == 0) return 0;
-    const int epi = (ep >= 2 && last) ? ep + 2 : ep;
+    const int epi = (ep >= 2 && ep <= 3 && last) ? ep + 2 : ep;
     return flux_launch(p, epi, ncol, u, ldu, smats, lds, fn, ldf, c, ldc, rcp,
                        ldr, xa, dta, dtb, gamma, as_stream(stream));
     FR_API_END

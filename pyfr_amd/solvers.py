@@ -539,6 +539,18 @@ class EulerElements(FluidElements):
                     stage=stage)
             self.kernels['tdivf_flux'] = tdivf_flux
 
+            def tdivf_flux_rk4(stage, uin, nxt, r1, r0):
+                """RK4 stage `stage` (1..4) with its update fused: u in
+                uin, the next stage's input to nxt, r1 and r0 the
+                accumulator and the step's initial state."""
+                sb = self.scal_upts
+                return be.kernel(
+                    'tdivf_flux', self.opmat('M1 - M3*M2'),
+                    self.opmat('M3'), sb[uin], self._smats_affine,
+                    self._scal_fpts, sb[nxt], self._rcpdjac_upts, gamma,
+                    xa=(sb[r1], sb[r0]), stage=f'rk4-{stage}')
+            self.kernels['tdivf_flux_rk4'] = tdivf_flux_rk4
+
 
 class NavierStokesElements(EulerElements):
     @property
@@ -1141,6 +1153,35 @@ class BaseSystem:
                 k.bind(dta=dta, dtb=dtb, last=int(last))
             self.backend.run_graph(g)
 
+    @property
+    def fused_rk4(self):
+        """True if RK4 stage updates can be fused into the RHS (every
+        element type on the flux-fused divergence kernel)."""
+        return (self.backend.fused and
+                all('tdivf_flux_rk4' in e.kernels
+                    for e in self.ele_map.values()))
+
+    def rhs_rk4_stage(self, t, stage, uin, nxt, r1, r0, dt):
+        """RHS of RK4 stage `stage` at t with the stage's register updates
+        in the divergence kernel's epilogue (``steppers.py:67-108``)."""
+        key = ('rk4', stage, uin, nxt, r1, r0)
+        if key not in self._kcache:
+            eles = list(self.ele_map.values())
+            ks = [e.kernels['disu'](uin) for e in eles]
+            ks += [i.kernels['comm_flux']() for i in self._inters]
+            ks += [e.kernels['tdivf_flux_rk4'](stage, uin, nxt, r1, r0)
+                   for e in eles]
+            g = self.backend.graph()
+            g.add_all(ks)
+            g.commit()
+            g.stage_kernels = [k for k in ks if 'dtb' in k.rtargs]
+            self._kcache[key] = [g]
+        self._set_time(t)
+        for g in self._kcache[key]:
+            for k in g.stage_kernels:
+                k.bind(dta=dt)
+            self.backend.run_graph(g)
+
     def ele_scal_upts(self, idx):
         return [b[idx].get() for b in self.ele_banks]
 
@@ -1379,7 +1420,10 @@ class RK4Integrator(BaseIntegrator):
     read and write one bank.  Each combination keeps the reference's
     coefficients and operand order (``add(dt/2, r2, 1, r0)`` with r2 = k
     becomes ``add(0, r2, dt/2, r3, 1, r0)``: the same products and sum in
-    the axnpby kernel), so the arithmetic is the reference's."""
+    the axnpby kernel), so the arithmetic is the reference's.  Where every
+    element type runs the flux-fused divergence kernel, each stage's
+    updates are done in that kernel's epilogue instead, the stage inputs
+    alternating between r2 and r3."""
     nregs = 4
     nrhs_per_step = 4
 
@@ -1388,6 +1432,17 @@ class RK4Integrator(BaseIntegrator):
         r0, r1, r2, r3 = 0, 1, 2, 3
         if r0 != self.idxcurr:
             r0, r1 = r1, r0
+        if self.system.fused_rk4:
+            # the same updates in each stage's divergence epilogue; the
+            # stage inputs alternate between r2 and r3, so no stage reads
+            # and writes one bank
+            st = self.system.rhs_rk4_stage
+            st(t, 1, r0, r2, r1, r0, dt)
+            st(t + dt/2.0, 2, r2, r3, r1, r0, dt)
+            st(t + dt/2.0, 3, r3, r2, r1, r0, dt)
+            st(t + dt, 4, r2, r2, r1, r0, dt)
+            self.nrhsevals += 4
+            return r1
         rhs(t, r0, r1)
         add(0.0, r2, 1.0, r0, dt/2.0, r1)
         rhs(t + dt/2.0, r2, r3)

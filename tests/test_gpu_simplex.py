@@ -283,6 +283,27 @@ def _rk4_dict(rhs, u, dt, step=None):
     return unflat(step(lambda v: flat(rhs(unflat(v))), flat(u), dt))
 
 
+@pytest.mark.parametrize('precision', ['double', 'single'])
+def test_simplex_rk4_fused_stages(backend_factory, precision):
+    """RK4 with each stage's register updates in the flux-fused divergence
+    epilogue against the reference kernel sequence (separate axnpby),
+    10 steps on a mixed mesh."""
+    be = backend_factory(precision=precision)
+    integ, _, _ = _mixed(be, 3, 2, 1, 3, dt=2e-3)
+    assert integ.system.fused_rk4
+    integ.run_steps(10, 2e-3)
+    ref_be = backend_factory(precision=precision, fused=False)
+    ref, _, _ = _mixed(ref_be, 3, 2, 1, 3, dt=2e-3)
+    assert not ref.system.fused_rk4
+    ref.run_steps(10, 2e-3)
+    names = integ.system.ele_types
+    err = rel_l2(_cat(dict(zip(names, integ.soln)), names),
+                 _cat(dict(zip(names, ref.soln)), names))
+    print(f'\n[rk4 fused stages] {precision} rel L2 {err:.3e}', flush=True)
+    assert err < (1e-14 if precision == 'double' else 1e-5)
+    assert integ.nrhsevals == ref.nrhsevals == 40
+
+
 @pytest.mark.parametrize('fuse_flux', [True, False])
 def test_simplex_rk45_trajectory(backend_factory, fuse_flux):
     """RK45 with the stage update fused into each element type's
