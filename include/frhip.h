@@ -334,6 +334,12 @@ int frhip_bc_program_destroy(void *prog);
  * (a captured graph keeps reading the same device scalar). */
 int frhip_bc_program_set_time(void *prog, double t, void *stream);
 
+/* Set the program's constant pool[idx] on `stream` (run-time parameters of
+ * an expression, e.g. the pressure t*im + ic of the controlled
+ * char-riem-inv types, pyfr/solvers/euler/kernels/bcs/
+ * char-riem-inv-controlled.mako, which the reference binds as ic, im). */
+int frhip_bc_program_set_const(void *prog, int idx, double v, void *stream);
+
 /* out[r][i] = expression r at point i (coordinates ploc[0..2][i], row
  * stride ldp) and the program's t, for n points; out has nrows rows of
  * stride ldo, in the working precision. */

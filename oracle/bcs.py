@@ -22,10 +22,12 @@ import numpy as np
 
 from oracle.physics import viscous_flux_add
 
-EULER_TYPES = ('sup-in-fa', 'sup-out-fn', 'char-riem-inv', 'slp-adia-wall')
+EULER_TYPES = ('sup-in-fa', 'sup-out-fn', 'char-riem-inv', 'slp-adia-wall',
+               'char-riem-inv-mass-flow', 'char-riem-inv-pressure')
 NS_TYPES = ('sup-in-fa', 'sup-out-fn', 'char-riem-inv', 'slp-adia-wall',
             'no-slp-adia-wall', 'no-slp-isot-wall', 'sub-in-frv',
-            'sub-in-ftpttang', 'sub-out-fp')
+            'sub-in-ftpttang', 'sub-out-fp', 'char-riem-inv-mass-flow',
+            'char-riem-inv-pressure')
 
 
 def _ke(u):
@@ -197,6 +199,13 @@ _GRAD = {'sup-in-fa': 'zero', 'sup-out-fn': 'copy', 'char-riem-inv': 'zero',
          'sub-in-ftpttang': 'copy', 'sub-out-fp': 'zero'}
 CFLUX_STATE = {'no-slp-adia-wall': 'ghost-imperm',
                'no-slp-isot-wall': 'ghost-imperm', 'slp-adia-wall': None}
+# the controlled types are char-riem-inv with p = t*im + ic
+# (euler/kernels/bcs/char-riem-inv-controlled.mako; navstokes/kernels/bcs/
+# char-riem-inv-mass-flow.mako, char-riem-inv-pressure.mako): their
+# caller passes that p in vals
+for _t in ('char-riem-inv-mass-flow', 'char-riem-inv-pressure'):
+    _RS[_t], _LDG[_t], _GRAD[_t] = _RS['char-riem-inv'], _LDG[
+        'char-riem-inv'], _GRAD['char-riem-inv']
 
 
 def bc_state(bctype, ul, n, vals, c):

@@ -112,14 +112,20 @@ class OracleHexSystem:
         pairs = pair_faces_bruteforce(self.plocf, 6, sh.nfacefpts, length,
                                       periodic, bfaces)
         self._bcdefs = []
+        self._ctrls = {}
         for side, faces in sorted(bfaces.items()):
             bctype, vals = (bcs or {})[side]
             bf = np.concatenate([f[2] for f in faces])
             be = np.concatenate([np.full(len(f[2]), f[0]) for f in faces])
+            if bctype in ('char-riem-inv-mass-flow',
+                          'char-riem-inv-pressure'):
+                vals = dict(vals)
+                self._ctrls[side] = _Controller(
+                    self, bctype, vals.pop('ctrl'), faces)
             self._bcdefs.append((side, bctype, bf, be, vals))
-        self._tdep = any(callable(fn) and _takes_time(fn)
-                         for *_, vals in self._bcdefs
-                         for fn in vals.values())
+        self._tdep = bool(self._ctrls) or any(
+            callable(fn) and _takes_time(fn) for *_, vals in self._bcdefs
+            for fn in vals.values())
         self._set_bcs(0.0)
         self.lf = np.concatenate([p[2] for p in pairs])
         self.le = np.concatenate([np.full(len(p[2]), p[0]) for p in pairs])
@@ -129,11 +135,15 @@ class OracleHexSystem:
 
     def _set_bcs(self, t):
         """Boundary values at the boundary flux points; callables of
-        (x, y, z) or of (x, y, z, t)."""
+        (x, y, z) or of (x, y, z, t); a controlled boundary's pressure
+        t*im + ic."""
         self.bcs = []
         for side, bctype, bf, be, vals in self._bcdefs:
             x, y, z = self.plocf[bf, :, be].T
             v = {}
+            if side in self._ctrls:
+                c = self._ctrls[side]
+                v['p'] = t*c.interp_m + c.interp_c
             for k, fn in vals.items():
                 if callable(fn):
                     v[k] = fn(x, y, z, t) if _takes_time(fn) else fn(x, y, z)
@@ -282,7 +292,11 @@ class OracleHexSystem:
         return -self.rcpdjac[:, None, :]*div                # negdivconf
 
     def rhs(self, u, t=0.0):
-        """The RHS at time t (boundary values that depend on t)."""
+        """The RHS at time t (boundary values that depend on t; the
+        controlled boundaries update from u first, as the reference's
+        preparefns do before each RHS)."""
+        for c in self._ctrls.values():
+            c.prepare(u, t)
         if self._tdep:
             self._set_bcs(t)
         return self.rhs_euler(u) if self.system == 'euler' else self.rhs_ns(u)
@@ -293,6 +307,74 @@ class OracleHexSystem:
 
 
 # -- time integration (pyfr/integrators/...)
+
+class _Controller:
+    """char-riem-inv-mass-flow / -pressure control
+    (``pyfr/solvers/euler/inters.py:108-294``) with the surface integral of
+    ``pyfr/quadrules/surface.py:8-55`` on the boundary's faces: a
+    Gauss-Legendre rule of degree >= quad-deg, the solution interpolated by
+    the nodal basis, the physical (unnormalised) normals of the stored
+    metric.  ctrl: target, alpha, eta, nsteps, tstart, qdeg and, for the
+    mass flow, the initial pressure p."""
+
+    def __init__(self, orc, bctype, ctrl, faces):
+        from oracle.basis import HEX_FACES, _proj, gauss_legendre_quad
+        self.mass = bctype.endswith('mass-flow')
+        self.target, self.alpha, self.eta = (ctrl['target'], ctrl['alpha'],
+                                             ctrl['eta'])
+        self.nsteps, self.tstart = ctrl.get('nsteps', 100), ctrl.get(
+            'tstart', 0.0)
+        self.gamma = orc.c['gamma']
+        self.interp_c = ctrl['p'] if self.mass else self.target
+        self.interp_m = 0.0
+        self.meas_avg, self.tprev, self.counter = 0.0, None, 0
+        n1 = -(-(ctrl['qdeg'] + 1)//2)
+        qp2, qw = gauss_legendre_quad(n1)
+        sh = orc.shape
+        nfp = sh.nfacefpts[0]
+        self.terms = []
+        self.area = 0.0
+        for f in faces:
+            e, fidx = f[0], int(f[2][0]) // nfp
+            proj, norm = HEX_FACES[fidx]
+            ppts = _proj(proj, qp2)
+            m0 = sh.ubasis.nodal_basis_at(ppts)               # (nq, Nu)
+            pn = pnorm_at(orc.verts[..., e:e + 1], orc.order, ppts,
+                          np.array([norm]*len(ppts), dtype=float))[:, 0]
+            self.terms.append((e, m0, qw, pn))                # pn (nq, 3)
+            self.area += (np.sqrt((pn**2).sum(axis=1))*qw).sum()
+
+    def _measure(self, u):
+        tot = 0.0
+        for e, m0, qw, pn in self.terms:
+            uf = m0 @ u[:, :, e]                              # (nq, 5)
+            if self.mass:
+                tot += np.einsum('qd,qd,q', uf[:, 1:4], pn, qw)
+            else:
+                rho, E = uf[:, 0], uf[:, 4]
+                p = (self.gamma - 1)*(E - 0.5*(uf[:, 1:4]**2).sum(axis=1)/rho)
+                tot += (p*np.sqrt((pn**2).sum(axis=1))*qw).sum()
+        return tot if self.mass else tot/self.area
+
+    def prepare(self, u, t):
+        update = self.counter % self.nsteps == 0
+        if (update or not self.tprev) and t >= self.tstart:
+            meas = self._measure(u)
+            if not self.tprev:
+                self.meas_avg, self.tprev = meas, t
+            else:
+                a = self.alpha
+                self.meas_avg = a*meas + (1 - a)*self.meas_avg
+                dt, self.tprev = t - self.tprev, t
+                p0 = self.interp_m*t + self.interp_c
+                if self.mass:
+                    p1 = p0 + dt*self.eta*(1 - self.target/self.meas_avg)
+                else:
+                    p1 = p0 + dt*self.eta*(self.target - self.meas_avg)
+                self.interp_m = (p1 - p0)/dt
+                self.interp_c = p0 - self.interp_m*t
+        self.counter += 1
+
 
 def _takes_time(fn):
     """A boundary-value callable of (x, y, z, t) rather than (x, y, z)."""

@@ -85,6 +85,7 @@ _FUNCS = {
     'frhip_bc_program_create': [ct.POINTER(_P), _I, _P, _P, _P, _I],
     'frhip_bc_program_destroy': [_P],
     'frhip_bc_program_set_time': [_P, _D, _P],
+    'frhip_bc_program_set_const': [_P, _I, _D, _P],
     'frhip_bc_eval': [_P, _I, _I, _P, _L, _P, _L, _P],
     'frhip_axnpby': [_I, _I, _L, _L, _I, _P, _P, _P],
     'frhip_axnpby_pool': [_I, _I, _L, _L, _I, _P, _P, ct.c_uint, _P, _I, _P],

@@ -74,10 +74,15 @@ def _emit(node, consts, code, pool):
     raise ExprError(f'Unsupported expression element: {ast.dump(node)}')
 
 
-def compile_exprs(exprs, consts):
+def compile_exprs(exprs, consts, params=None):
     """exprs: list of expression strings; consts: {name: value}.  Returns
-    (code (ncode, 2) int32, offs (len(exprs) + 1) int32, pool float64)."""
+    (code (ncode, 2) int32, offs (len(exprs) + 1) int32, pool float64).
+    params: {name: initial value} of run-time parameters; with it, also
+    returns {name: [pool indices]} (set with frhip_bc_program_set_const)."""
     code, offs, pool = [], [0], []
+    pidx = {}
+    if params:
+        consts = dict(consts, **params)
     for e in exprs:
         if '**' in e or '^' in e:
             raise ExprError('Direct exponentiation is not supported; use pow')
@@ -85,12 +90,44 @@ def compile_exprs(exprs, consts):
             tree = ast.parse(e.strip(), mode='eval')
         except SyntaxError as ex:
             raise ExprError(f'Invalid expression {e!r}') from ex
+        n0 = len(code)
         depth = _emit(tree, consts, code, pool)
         if depth > MAX_STACK:
             raise ExprError(f'Expression too deep: {e!r}')
+        if params:
+            # the pool entries the parameters' names were emitted as
+            pushes = [arg for op, arg in code[n0:] if op == PUSHC]
+            for name, idx in zip(_pool_names(tree, consts), pushes):
+                if name in params:
+                    pidx.setdefault(name, []).append(idx)
         offs.append(len(code))
-    return (np.array(code, dtype=np.int32).reshape(-1, 2),
-            np.array(offs, dtype=np.int32), np.array(pool, dtype=float))
+    out = (np.array(code, dtype=np.int32).reshape(-1, 2),
+           np.array(offs, dtype=np.int32), np.array(pool, dtype=float))
+    return out + (pidx,) if params is not None else out
+
+
+def _pool_names(node, consts):
+    """For each constant push of the post-order emission, the name it came
+    from (None for numbers and pi)."""
+    if isinstance(node, ast.Expression):
+        return _pool_names(node.body, consts)
+    if isinstance(node, ast.Constant):
+        return [None]
+    if isinstance(node, ast.Name):
+        if node.id in VARS:
+            return []
+        return [None] if node.id == 'pi' else [node.id]
+    if isinstance(node, ast.UnaryOp):
+        return _pool_names(node.operand, consts)
+    if isinstance(node, ast.BinOp):
+        return (_pool_names(node.left, consts) +
+                _pool_names(node.right, consts))
+    if isinstance(node, ast.Call):
+        out = []
+        for a in node.args:
+            out += _pool_names(a, consts)
+        return out
+    return []
 
 
 def uses_time(expr):

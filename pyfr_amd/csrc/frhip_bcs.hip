@@ -586,6 +586,19 @@ int frhip_bc_program_set_time(void *prog, double t, void *stream)
     FR_API_END
 }
 
+int frhip_bc_program_set_const(void *prog, int idx, double v, void *stream)
+{
+    FR_API_BEGIN
+    auto *p = static_cast<BCProgram *>(prog);
+    FR_CHECK(p, "null program");
+    FR_CHECK(idx >= 0 && idx < p->npool, "constant index out of range");
+    hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(64), 0, as_stream(stream),
+                       p->pool + idx, v);
+    FR_LAUNCHED();
+    return 0;
+    FR_API_END
+}
+
 int frhip_bc_eval(void *prog, int dtype, int n, const void *ploc, long ldp,
                   void *out, long ldo, void *stream)
 {

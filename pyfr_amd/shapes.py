@@ -233,6 +233,13 @@ class HexShape:
         m = self.nf1**2
         return [list(range(f*m, (f + 1)*m)) for f in range(6)]
 
+    def nodal_basis_at(self, pts):
+        """(npts, nupts): the solution's nodal basis at reference points."""
+        pts = np.atleast_2d(pts)
+        L = [lagrange_at(self.x1d, pts[:, d]) for d in range(3)]
+        return np.einsum('pk,pj,pi->pkji', L[2], L[1], L[0]).reshape(
+            len(pts), -1)
+
     def fpt_line(self, f, a, b):
         """Solution points on the line normal to face f through its flux point
         (a, b), ordered along the normal axis, and that axis."""

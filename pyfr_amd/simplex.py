@@ -263,6 +263,10 @@ class _SimplexShape:
             rows.append(np.einsum('i,ik,ji->kj', qw, L, M))
         return np.vstack(rows)
 
+    def nodal_basis_at(self, pts):
+        """(npts, nupts): the solution's nodal basis at reference points."""
+        return self.ubasis.nodal_basis_at(pts)
+
     @cached_property
     def m0(self):
         return self.ubasis.nodal_basis_at(self.fpts)

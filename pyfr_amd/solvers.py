@@ -24,7 +24,7 @@ written against ``pyfr/backends/base``:
 
 Scope: meshes of hexahedra, straight-sided tetrahedra and right prisms on
 one GPU, periodic or with the reference's boundary conditions (values may
-depend on x, y, z and t; not the controlled char-riem-inv variants);
+depend on x, y, z and t; the controlled char-riem-inv types included);
 anti-aliasing (flux, surf-flux) on every element type; no MPI interfaces,
 no source terms.
 """
@@ -837,8 +837,152 @@ BC_VALUES = {'sup-in-fa': ('rho', 'p', 'u', 'v', 'w'),
              'sub-in-frv': ('rho', 'u', 'v', 'w'),
              'no-slp-isot-wall': ('u', 'v', 'w'),
              'sub-out-fp': ('p',)}
+BC_VALUES['char-riem-inv-mass-flow'] = BC_VALUES['char-riem-inv']
+BC_VALUES['char-riem-inv-pressure'] = BC_VALUES['char-riem-inv']
 BC_DEFAULTS = {'sub-in-frv': {'u': 0, 'v': 0, 'w': 0},
                'no-slp-isot-wall': {'u': 0, 'v': 0, 'w': 0}}
+# controlled char-riem-inv types: their target option
+CONTROLLED = {'char-riem-inv-mass-flow': 'mass-flow-rate',
+              'char-riem-inv-pressure': 'pressure'}
+
+
+def surface_rule(cfg, sect, kind):
+    """(points, weights) of a face quadrature rule of a boundary's surface
+    integrals (``pyfr/quadrules/surface.py:40-55``): the rule named by
+    [solver-interfaces-<kind>] flux-pts (or the section's
+    quad-pts-<kind>) with the fewest points of degree >= quad-deg(-<kind>).
+    Quads: Gauss-Legendre; triangles: the tabulated Williams-Shunn rules."""
+    from pyfr_amd.quadtables import TABLES
+    from pyfr_amd.shapes import gauss_legendre
+    rname = cfg.get(f'solver-interfaces-{kind}', 'flux-pts',
+                    'gauss-legendre' if kind == 'quad' else 'williams-shunn')
+    rname = cfg.get(sect, f'quad-pts-{kind}', rname)
+    qdeg = (cfg.getint(sect, f'quad-deg-{kind}')
+            if cfg.hasopt(sect, f'quad-deg-{kind}')
+            else cfg.getint(sect, 'quad-deg'))
+    if kind == 'quad' and rname == 'gauss-legendre':
+        n = max(1, -(-(qdeg + 1)//2))
+        g, w = gauss_legendre(n)
+        return (np.array([(g[i], g[j]) for j in range(n) for i in range(n)]),
+                np.array([w[i]*w[j] for j in range(n) for i in range(n)]))
+    # stated degrees of the tabulated Williams-Shunn triangle rules, and
+    # the rules of that family that are not tabulated
+    ws = {3: 2, 6: 4, 10: 5, 15: 7, 36: 12}
+    missing = {21: 8, 28: 10, 45: 14}
+    if kind == 'tri' and rname == 'williams-shunn':
+        cands = sorted((n, d) for n, d in (ws | missing).items() if d >= qdeg)
+        if cands and cands[0][0] in ws:
+            pts, wts = TABLES[('tri', 'williams-shunn', cands[0][0])]
+            return np.array(pts), np.array(wts)
+    raise NotImplementedError(f'no tabulated {kind} rule {rname} of degree '
+                              f'{qdeg}')
+
+
+class ControlledBC:
+    """The controller of the char-riem-inv-mass-flow / -pressure types
+    (``pyfr/solvers/euler/inters.py:108-294``): every ``nsteps`` RHS
+    evaluations (from ``tstart``) it measures the boundary's mass flow
+    (or mean pressure) by a surface integral of the input state
+    (``pyfr/quadrules/surface.py:8-55``), averages it exponentially
+    (alpha), corrects the boundary pressure (eta) and interpolates it
+    linearly in t; the kernel reads p = t*im + ic."""
+
+    def __init__(self, bc, target_opt):
+        cfg, sect = bc.cfg, bc.cfgsect
+        self.bc = bc
+        self.kind = target_opt
+        self.target, self.alpha, self.eta = (
+            bc._eval_opt(k) for k in (target_opt, 'alpha', 'eta'))
+        self.tstart = cfg.getfloat(sect, 'tstart', 0.0)
+        self.nsteps = cfg.getint(sect, 'nsteps', 100)
+        self.interp_c = (bc._eval_opt('p') if target_opt == 'mass-flow-rate'
+                         else self.target)
+        self.interp_m = 0.0
+        self.meas_avg = 0.0
+        self.tprev = None
+        self.counter = 0
+        self.pool_idx = {}
+        self._bound = None
+        self.gamma = cfg.getfloat('constants', 'gamma')
+        # surface quadrature per (etype, face): points, weights, m0, normals
+        self.groups = []
+        area = 0.0
+        for etype, fidx, eidxs in bc.lhs.items():
+            eles = bc.elemap[etype]
+            kind, proj = _face_kind_proj(eles.basis, fidx)
+            fp, qw = surface_rule(cfg, sect, kind)
+            ppts = np.array([proj(s, t) for s, t in fp], dtype=float)
+            m0 = eles.basis.nodal_basis_at(ppts)           # (nq, nupts)
+            norm = eles.basis.norm_fpts[eles.basis.facefpts[fidx][0]]
+            pn = eles.geom.pnorm_at(ppts, np.array([norm]*len(ppts)))
+            pn = pn[:, np.asarray(eidxs)].transpose(2, 0, 1)  # (3, nq, ne)
+            self.groups.append((etype, np.asarray(eidxs), m0, qw, pn))
+            area += (np.sqrt((pn**2).sum(axis=0))*qw[:, None]).sum()
+        self.area = area
+
+    def params(self):
+        return {'ic': self.interp_c, 'im': self.interp_m}
+
+    def _measure(self, solns):
+        tot = 0.0
+        for etype, eidxs, m0, qw, pn in self.groups:
+            u = solns[etype][:, :, eidxs]                  # (nupts, 5, ne)
+            uf = np.einsum('qu,uve->vqe', m0, u)            # (5, nq, ne)
+            if self.kind == 'mass-flow-rate':
+                tot += np.einsum('hij,hij', uf[1:-1], pn*qw[None, :, None])
+            else:
+                rho, E = uf[0], uf[-1]
+                p = (self.gamma - 1)*(E - 0.5*(uf[1:-1]**2).sum(axis=0)/rho)
+                nmag = np.sqrt((pn**2).sum(axis=0))
+                tot += np.einsum('ij,ij', p, nmag*qw[:, None])
+        return tot if self.kind == 'mass-flow-rate' else tot/self.area
+
+    def _correction(self, p0, dt):
+        if self.kind == 'mass-flow-rate':
+            return p0 + dt*self.eta*(1 - self.target/self.meas_avg)
+        return p0 + dt*self.eta*(self.target - self.meas_avg)
+
+    def prepare(self, system, uin, t):
+        update = self.counter % self.nsteps == 0
+        if (update or not self.tprev) and t >= self.tstart:
+            solns = dict(zip(system.ele_types, system.ele_scal_upts(uin)))
+            meas = self._measure(solns)
+            if not self.tprev:
+                self.meas_avg = meas
+                self.tprev = t
+            else:
+                a = self.alpha
+                self.meas_avg = a*meas + (1 - a)*self.meas_avg
+                dt = t - self.tprev
+                self.tprev = t
+                p0 = self.interp_m*t + self.interp_c
+                p1 = self._correction(p0, dt)
+                self.interp_m = (p1 - p0)/dt
+                self.interp_c = p0 - self.interp_m*t
+        vals = (self.interp_c, self.interp_m)
+        if vals != self._bound:
+            lib, prog = self.bc._be.lib, self.bc._bcprog
+            for name, v in zip(('ic', 'im'), vals):
+                for i in self.pool_idx.get(name, []):
+                    lib.bc_program_set_const(prog, int(i), float(v),
+                                             self.bc._be.stream)
+            self._bound = vals
+        self.counter += 1
+
+
+def _face_kind_proj(basis, fidx):
+    """(face kind, projection (s, t) -> reference point) of face fidx."""
+    if basis.name == 'hex':
+        from pyfr_amd.shapes import HEX_FACES
+        nax, side, sax, tax = HEX_FACES[fidx]
+
+        def proj(s, t):
+            p = [0.0, 0.0, 0.0]
+            p[nax], p[sax], p[tax] = float(side), s, t
+            return p
+        return 'quad', proj
+    kind, proj, _ = basis.faces[fidx]
+    return kind, proj
 
 
 class EulerBCInters(EulerIntInters):
@@ -851,7 +995,9 @@ class EulerBCInters(EulerIntInters):
     (``pyfr_amd.bcexpr``) that a kernel evaluates into those rows at each
     RHS, from a device time scalar set before the RHS (``set_time``)."""
     _bcprog = None
-    bctypes = ('sup-in-fa', 'sup-out-fn', 'char-riem-inv', 'slp-adia-wall')
+    _ctrl = None
+    bctypes = ('sup-in-fa', 'sup-out-fn', 'char-riem-inv', 'slp-adia-wall',
+               'char-riem-inv-mass-flow', 'char-riem-inv-pressure')
     bccflux_mod = 'pyfr.solvers.euler.kernels.bccflux'
 
     def __init__(self, be, lhs, elemap, cfgsect, cfg):
@@ -876,10 +1022,15 @@ class EulerBCInters(EulerIntInters):
         self._perm = self._get_perm_for_field(self.lhs, scal)
         self._pnorm_lhs = self._const_mat(self.lhs, 'get_pnorms_for_inters')
         self._bc_consts()
+        if self.type in CONTROLLED:
+            self._ctrl = ControlledBC(self, CONTROLLED[self.type])
         self._bcvals = self._eval_values()
         rsolver = cfg.get('solver-interfaces', 'riemann-solver')
+        # the controlled types are char-riem-inv with a run-time pressure
+        # (pyfr/solvers/euler/kernels/bcs/char-riem-inv-controlled.mako)
+        ktype = 'char-riem-inv' if self.type in CONTROLLED else self.type
         self._tplargs = dict(ndims=self.ndims, nvars=self.nvars,
-                             rsolver=rsolver, c=self.c, bctype=self.type,
+                             rsolver=rsolver, c=self.c, bctype=ktype,
                              p_min=_p_min(be, cfg))
         self._kernels()
 
@@ -913,6 +1064,12 @@ class EulerBCInters(EulerIntInters):
             self._be.lib.bc_program_set_time(self._bcprog, float(t),
                                              self._be.stream)
 
+    def prepare(self, system, uin, t):
+        """Before each RHS: the controlled types' update from the input
+        bank (``pyfr/solvers/euler/inters.py:161-196``)."""
+        if self._ctrl is not None:
+            self._ctrl.prepare(system, uin, t)
+
     def __del__(self):
         try:
             if self._bcprog is not None:
@@ -940,10 +1097,15 @@ class EulerBCInters(EulerIntInters):
         consts = self.cfg.items_as('constants', float)
         dflt = BC_DEFAULTS.get(self.type, {})
         exprs = [str(self.cfg.getexpr(self.cfgsect, k, dflt[k]))
-                 if k in dflt else self.cfg.getexpr(self.cfgsect, k)
-                 for k in keys]
+                 if k in dflt else
+                 't*im + ic' if k == 'p' and self._ctrl else
+                 self.cfg.getexpr(self.cfgsect, k) for k in keys]
         if any(uses_time(e) for e in exprs):
-            code, offs, pool = compile_exprs(exprs, consts)
+            params = self._ctrl.params() if self._ctrl else None
+            res = compile_exprs(exprs, consts, params)
+            code, offs, pool = res[:3]
+            if self._ctrl:
+                self._ctrl.pool_idx = res[3]
             prog = ct.c_void_p()
             self._be.lib.bc_program_create(
                 ct.byref(prog), len(exprs), code.ctypes.data,
@@ -962,7 +1124,8 @@ class EulerBCInters(EulerIntInters):
 
 class NavierStokesBCInters(EulerBCInters):
     """``baseadvecdiff/inters.py:60-76``, ``navstokes/inters.py:70-219``."""
-    bctypes = ('sup-in-fa', 'sup-out-fn', 'char-riem-inv', 'slp-adia-wall',
+    bctypes = ('char-riem-inv-mass-flow', 'char-riem-inv-pressure',
+               'sup-in-fa', 'sup-out-fn', 'char-riem-inv', 'slp-adia-wall',
                'no-slp-adia-wall', 'no-slp-isot-wall', 'sub-in-frv',
                'sub-in-ftpttang', 'sub-out-fp')
     bccflux_mod = 'pyfr.solvers.navstokes.kernels.bccflux'
@@ -1091,14 +1254,18 @@ class BaseSystem:
 
     def rhs(self, t, uinbank, foutbank):
         """``pyfr/solvers/base/system.py:420-430``."""
-        self._set_time(t)
+        self._set_time(t, uinbank)
         for g in self._rhs_graphs(uinbank, foutbank):
             self.backend.run_graph(g)
 
-    def _set_time(self, t):
-        """t for boundary values that depend on it (the reference binds it
-        into its kernels before each RHS)."""
+    def _set_time(self, t, uin=None):
+        """Before each RHS: the controlled boundaries' update from the input
+        bank uin, then t for boundary values that depend on it (the
+        reference's preparefns and kernel binding,
+        pyfr/solvers/base/system.py:408-418)."""
         for b in self._bc_inters:
+            if uin is not None:
+                b.prepare(self, uin, t)
             b.set_time(t)
 
     def _rhs_graphs(self, uin, fout):
@@ -1147,7 +1314,7 @@ class BaseSystem:
         if key not in self._kcache:
             self._kcache[key] = self._try_fused(
                 uin, fout, xa, 'rk45-mixed' if mixed else 'rk45')
-        self._set_time(t)
+        self._set_time(t, uin)
         for g in self._kcache[key]:
             for k in g.stage_kernels:
                 k.bind(dta=dta, dtb=dtb, last=int(last))
@@ -1176,7 +1343,7 @@ class BaseSystem:
             g.commit()
             g.stage_kernels = [k for k in ks if 'dtb' in k.rtargs]
             self._kcache[key] = [g]
-        self._set_time(t)
+        self._set_time(t, uin)
         for g in self._kcache[key]:
             for k in g.stage_kernels:
                 k.bind(dta=dt)

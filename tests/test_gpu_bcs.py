@@ -267,3 +267,86 @@ def test_bc_rhs_parity_gmsh_mesh(backend_factory, version):
     err = rel_l2(got, ref)
     print(f'[bc rhs gmsh] rel L2 {err:.3e} (tol {tol:.3e})', flush=True)
     assert err < tol
+
+
+def _controlled_setup(be, system, scheme):
+    from oracle.rhs import OracleHexSystem
+    from pyfr_amd.mesh import BoxMesh
+    from pyfr_amd.solvers import get_integrator
+    cfg = make_cfg(system, 2, scheme=scheme, dt=2e-4)
+    cfg.set('backend-hip', 'fused', str(be.fused))
+    sides = {
+        'x0': ('char-riem-inv-mass-flow', 'mass-flow-rate', -5.0,
+               {'rho': '1', 'u': '0.3', 'v': '0', 'w': '0.05',
+                'p': '71.4'}, 5, 2.0),
+        'x1': ('char-riem-inv-pressure', 'pressure', 71.0,
+               {'rho': '1', 'u': '0.3', 'v': '0', 'w': '0'}, 4, 1.0)}
+    obcs = {}
+    for side, (bctype, topt, target, vals, qdeg, eta) in sides.items():
+        sect = f'soln-bcs-{side}'
+        cfg.set(sect, 'type', bctype)
+        for k, v in vals.items():
+            cfg.set(sect, k, v)
+        for k, v in ((topt, target), ('alpha', 0.5), ('eta', eta),
+                     ('nsteps', 3), ('quad-deg', qdeg)):
+            cfg.set(sect, k, str(v))
+        ov = {k: float(eval(v)) for k, v in vals.items() if k != 'p'}
+        ov['ctrl'] = {'target': target, 'alpha': 0.5, 'eta': eta,
+                      'nsteps': 3, 'qdeg': qdeg,
+                      'p': float(eval(vals.get('p', '0')))}
+        obcs[side] = (bctype, ov)
+    periodic = (False, True, True)
+    integ = get_integrator(be, BoxMesh(3, periodic=periodic), cfg)
+    orc = OracleHexSystem(3, 2, system, constants=oracle_consts(system),
+                          periodic=periodic, bcs=obcs)
+    return integ, orc
+
+
+@pytest.mark.parametrize('scheme', ['rk4', 'rk45'])
+@pytest.mark.parametrize('system', ['euler', 'navier-stokes'])
+def test_bc_controlled_trajectory(backend_factory, system, scheme):
+    """char-riem-inv-mass-flow and char-riem-inv-pressure: the controller
+    measures the boundary's mass flow / mean pressure by a surface
+    integral of each RHS's input state every nsteps (3) RHS evaluations
+    and moves the boundary pressure; 10 steps against the oracle stepping
+    with the reference's stage times.  (With nsteps = 2, RK4 would update
+    at its third stage, whose time equals the second's: the reference's
+    controller then divides by a zero time step.)"""
+    from oracle.rhs import advance
+    be = backend_factory()
+    integ, orc = _controlled_setup(be, system, scheme)
+    integ.dt = 2e-4
+    integ.advance_to(10*2e-4)
+    u_ref, nsteps = advance(orc.rhs, orc.ics(), 2e-4, 10*2e-4, scheme,
+                            timed=True)
+    assert integ.nsteps == nsteps
+    err = rel_l2(integ.soln[0], u_ref)
+    ctrls = [b._ctrl for b in integ.system._bc_inters]
+    print(f'\n[bc controlled {system} {scheme}] rel L2 {err:.3e}; '
+          + ', '.join(f'{c.kind} p = {c.interp_c:.6f} + {c.interp_m:.4f} t'
+                      for c in ctrls), flush=True)
+    assert err < 1e-12
+    for c, side in zip(ctrls, ('x0', 'x1')):
+        oc = orc._ctrls[side]
+        assert c.interp_m != 0.0
+        assert abs(c.interp_m - oc.interp_m) <= 1e-9*abs(oc.interp_m)
+        assert abs(c.interp_c - oc.interp_c) <= 1e-12*abs(oc.interp_c)
+
+
+def test_bc_controlled_surface_integral(backend_factory):
+    """The product's surface integral of a uniform flow: mass flow
+    -rho u A through the x0 side, mean pressure p."""
+    be = backend_factory()
+    integ, _ = _controlled_setup(be, 'euler', 'rk4')
+    L = 2*np.pi
+    sysm = integ.system
+    u = {}
+    for et, e in sysm.ele_map.items():
+        a = np.empty((e.nupts, 5, e.neles))
+        a[:, 0], a[:, 1], a[:, 2:4] = 1.2, 1.2*0.3, 0.0
+        a[:, 4] = 0.9/0.4 + 0.5*1.2*0.09
+        u[et] = a
+    mf, pr = (b._ctrl for b in sysm._bc_inters)
+    assert abs(mf._measure(u) + 1.2*0.3*L*L) < 1e-12*L*L
+    assert abs(pr._measure(u) - 0.9) < 1e-13
+    assert abs(pr.area - L*L) < 1e-12*L*L

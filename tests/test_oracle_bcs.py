@@ -223,3 +223,37 @@ def test_ns_bc_flux_types_run():
         vals = _fs_vals()
         f = bccflux_ns(t, rusanov, ul, gl, 1.7*n, vals, C)
         assert f.shape == ul.shape and np.all(np.isfinite(f))
+
+
+def test_controlled_surface_integral_and_update():
+    """The controlled char-riem-inv oracle: its surface integral of a
+    uniform flow (mass flow -rho u A through the x0 side, mean pressure p,
+    area A) and one controller update by hand
+    (pyfr/solvers/euler/inters.py:165-186)."""
+    from oracle.rhs import OracleHexSystem
+    L = 2*np.pi
+    ctrl = {'target': -1.0, 'alpha': 0.5, 'eta': 2.0, 'nsteps': 1,
+            'qdeg': 5, 'p': 0.8}
+    bcs = {'x0': ('char-riem-inv-mass-flow',
+                  {'rho': 1.0, 'u': 0.3, 'v': 0.0, 'w': 0.0, 'ctrl': ctrl}),
+           'x1': ('char-riem-inv-pressure',
+                  {'rho': 1.0, 'u': 0.3, 'v': 0.0, 'w': 0.0,
+                   'ctrl': dict(ctrl, target=0.7)})}
+    o = OracleHexSystem(3, 2, 'euler', periodic=(False, True, True),
+                        bcs=bcs)
+    u = np.empty((o.nupts, 5, o.neles))
+    u[:, 0], u[:, 1], u[:, 2:4] = 1.2, 1.2*0.3, 0.0
+    u[:, 4] = 0.9/0.4 + 0.5*1.2*0.09
+    mf, pr = o._ctrls['x0'], o._ctrls['x1']
+    assert abs(mf._measure(u) + 1.2*0.3*L*L) < 1e-12*L*L
+    assert abs(pr._measure(u) - 0.9) < 1e-13
+    assert abs(pr.area - L*L) < 1e-12*L*L
+    # first measurements at t = 0.1 (tprev set), the update at t = 0.3
+    mf.prepare(u, 0.1)
+    mf.prepare(u, 0.3)
+    meas = -1.2*0.3*L*L
+    p1 = 0.8 + 0.2*2.0*(1 + 1.0/meas)
+    # the new line passes through (t, p0) and reaches p1 a step later
+    assert abs(mf.interp_m - (p1 - 0.8)/0.2) < 1e-12
+    assert abs(mf.interp_c + mf.interp_m*0.3 - 0.8) < 1e-12
+    assert abs(mf.interp_c + mf.interp_m*0.5 - p1) < 1e-12
