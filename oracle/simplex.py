@@ -18,7 +18,8 @@ periods), independent of the product's node-based connectivity.
 """
 import numpy as np
 
-from oracle.basis import PriBasis, SimplexShapeOracle, TetBasis
+from oracle.basis import HexBasis, HexShapeOracle, PriBasis, \
+    SimplexShapeOracle, TetBasis
 from oracle.bcs import bccflux_euler, bccflux_ns, bcconu
 from oracle.geom import kopriva_metric
 from oracle.physics import RSOLVERS, inviscid_flux, pri_to_con, \
@@ -30,13 +31,15 @@ STD_VERTS = {
     'pri': np.array([(-1, -1, -1), (1, -1, -1), (-1, 1, -1), (-1, -1, 1),
                      (1, -1, 1), (-1, 1, 1)], dtype=float),
 }
-LINBASIS = {'tet': TetBasis, 'pri': PriBasis}
+LINBASIS = {'tet': TetBasis, 'pri': PriBasis, 'hex': HexBasis}
 
 
 def std_ele(name, k):
     """Equispaced order-k shape points: ``TetShape.std_ele``
     (``pyfr/shapes.py:502-510``) and ``PriShape.std_ele`` (``:555-562``)."""
     x = np.linspace(-1, 1, k + 1)
+    if name == 'hex':
+        return np.array([(p, q, r) for r in x for q in x for p in x])
     if name == 'tet':
         return np.array([(p, q, r) for i, r in enumerate(x)
                          for j, q in enumerate(x[:k + 1 - i])
@@ -47,8 +50,8 @@ def std_ele(name, k):
 
 def shape_order(name, nspts):
     for k in range(1, 16):
-        n = ((k + 1)*(k + 2)*(k + 3)//6 if name == 'tet'
-             else (k + 1)**2*(k + 2)//2)
+        n = {'tet': (k + 1)*(k + 2)*(k + 3)//6, 'hex': (k + 1)**3,
+             'pri': (k + 1)**2*(k + 2)//2}[name]
         if n == nspts:
             return k
     raise ValueError('Invalid shape point count')
@@ -61,7 +64,17 @@ class _Etype:
         'iqrules' = {face kind: (points, weights)} for surface-flux
         anti-aliasing."""
         self.name = name
-        self.sh = sh = SimplexShapeOracle(name, order, tables)
+        if name == 'hex':
+            # GL solution and flux points (HexShapeOracle); its face points
+            # are the tensor GL points the prisms' quad faces also carry
+            if set(tables) - {'nq1', 'nf1'}:
+                raise ValueError('hex tables take only nq1 / nf1')
+            sh = HexShapeOracle(order)
+            if 'nf1' in tables:
+                sh.set_surface_quadrature(tables['nf1'])
+        else:
+            sh = SimplexShapeOracle(name, order, tables)
+        self.sh = sh
         if 'iqrules' in tables:
             sh.set_surface_quadrature(tables['iqrules'])
         self.verts = verts                          # (nspts, 3, E)
@@ -89,8 +102,11 @@ class _Etype:
         if np.any(d <= 0):
             raise ValueError('negative Jacobian')
         self.rcpdjac = 1/d                           # (Nu, E)
-        if 'qrule' in tables:
-            sh.set_quadrature(*tables['qrule'])
+        if 'qrule' in tables or 'nq1' in tables:
+            if name == 'hex':
+                sh.set_quadrature(tables['nq1'])
+            else:
+                sh.set_quadrature(*tables['qrule'])
             self.nqpts = len(sh.qpts)
             self.M7 = sh.opmat('M7')
             self.M1329 = sh.opmat('(M1 - M3*M2)*M9')

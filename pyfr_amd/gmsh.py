@@ -27,7 +27,7 @@ import re
 
 import numpy as np
 
-from pyfr_amd.mesh import FACE_VERTS, HEX_FACE_AXES, fuzzysort_batched
+from pyfr_amd.mesh import FACE_VERTS, fuzzysort_batched
 
 # Gmsh element type -> (PyFR type, node count); first order only
 GMSH_TYPES = {2: ('tri', 3), 3: ('quad', 4), 4: ('tet', 4), 5: ('hex', 8),
@@ -40,14 +40,7 @@ _DECLINED = {8, 26, 27, 9, 21, 23, 10, 36, 37, 11, 29, 30, 12, 92, 93,
              13, 90, 91, 7, 14, 118, 119}
 
 
-def _hex_face_verts():
-    out = []
-    for ax, side in HEX_FACE_AXES:
-        out.append(tuple(v for v in range(8) if ((v >> ax) & 1) == (side > 0)))
-    return out
-
-
-FACES = {'hex': _hex_face_verts(), **FACE_VERTS}
+FACES = FACE_VERTS
 # codec order of the volume element types (Gmsh type number)
 ETYPE_ORDER = ['tet', 'hex', 'pri']
 

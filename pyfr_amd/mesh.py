@@ -178,7 +178,12 @@ def sorted_face_fpts_structured(plocfpts, facefpts, tol=1e-6):
 TET_FACE_VERTS = [(0, 1, 2), (0, 1, 3), (0, 2, 3), (1, 2, 3)]
 PRI_FACE_VERTS = [(0, 1, 2), (3, 4, 5), (0, 1, 4, 3), (1, 2, 5, 4),
                   (0, 2, 5, 3)]
-FACE_VERTS = {'tet': TET_FACE_VERTS, 'pri': PRI_FACE_VERTS}
+# hex faces: the vertices on each (normal axis, side) of HEX_FACE_AXES,
+# in HexShape (x fastest) order
+HEX_FACE_VERTS = [tuple(v for v in range(8) if ((v >> ax) & 1) == (side > 0))
+                  for ax, side in HEX_FACE_AXES]
+FACE_VERTS = {'tet': TET_FACE_VERTS, 'pri': PRI_FACE_VERTS,
+              'hex': HEX_FACE_VERTS}
 
 
 def _kuhn_offsets():
@@ -208,19 +213,40 @@ def _prism_offsets():
                      for t in tris])                    # (2, 6, 3)
 
 
-CELL_SPLITS = {'tet': _kuhn_offsets(), 'pri': _prism_offsets()}
+def _prism_x_offsets():
+    """Two right prisms per cube extruded along x, split along the
+    (0,0)-(1,1) diagonal of the yz faces: each cube's z faces are whole
+    quad faces (one per prism), conforming with hexes above and below."""
+    tris = [[(0, 0), (1, 0), (1, 1)], [(0, 0), (1, 1), (0, 1)]]
+    return np.array([[(x, y, z) for x in (0, 1) for (y, z) in t]
+                     for t in tris])                    # (2, 6, 3)
+
+
+def _hex_offsets():
+    """One hex per cube, vertices in HexShape order (x fastest)."""
+    return np.array([[(x, y, z) for z in (0, 1) for y in (0, 1)
+                      for x in (0, 1)]])                # (1, 8, 3)
+
+
+CELL_SPLITS = {'tet': _kuhn_offsets(), 'pri': _prism_offsets(),
+               'prx': _prism_x_offsets(), 'hex': _hex_offsets()}
+# element type of each cell split
+SPLIT_ETYPE = {'tet': 'tet', 'pri': 'pri', 'prx': 'pri', 'hex': 'hex'}
 
 
 class SimplexBoxMesh:
-    """Boxes of straight-sided tetrahedra and prisms.
+    """Boxes of straight-sided tetrahedra, prisms and hexahedra.
 
     blocks: list of (shape (nx, ny, nz), spacing h, layers) where layers is
-    a list of (etype, nlayers) filling the block's z extent bottom to top;
+    a list of (split, nlayers) filling the block's z extent bottom to top;
     each block is periodic on its own along the axes where ``periodic`` is
     true, and the faces on its other sides are boundaries named
     'x0'/'x1', 'y0'/'y1', 'z0'/'z1' (low/high side).  Cells split into 6
-    tets or 2 prisms (``CELL_SPLITS``); all splits share the cube diagonal,
-    so faces conform within and across element types.
+    tets ('tet'), 2 prisms extruded along z ('pri') or along x ('prx'), or
+    stay one hex ('hex') (``CELL_SPLITS``).  Tet and 'pri' layers share the
+    cube diagonal, so their faces conform; 'prx' and hex layers meet on
+    whole quad faces.  A tet or 'pri' layer next to a hex or 'prx' layer
+    leaves faces unpaired (an error).
 
     ``spts``: {etype: (nverts, neles, 3)}; elements are ordered by block,
     then split index, then cell (x fastest).  ``con``: (lhs, rhs) lists of
@@ -233,7 +259,7 @@ class SimplexBoxMesh:
     elements; the warp must respect the periods).
     """
     ndims = 3
-    etype_order = ('tet', 'pri')
+    etype_order = ('hex', 'tet', 'pri')
 
     def __init__(self, blocks, periodic=(True, True, True), sorder=1,
                  warp=None):
@@ -245,12 +271,13 @@ class SimplexBoxMesh:
             if sum(n for _, n in layers) != nz:
                 raise ValueError('layers must fill the block')
             k0 = 0
-            for et, nl in layers:
+            for split, nl in layers:
+                et = SPLIT_ETYPE[split]
                 kk, jj, ii = np.meshgrid(np.arange(k0, k0 + nl),
                                          np.arange(ny), np.arange(nx),
                                          indexing='ij')
                 cell = np.stack([ii.ravel(), jj.ravel(), kk.ravel()], 1)
-                off = CELL_SPLITS[et]                   # (ns, nv, 3)
+                off = CELL_SPLITS[split]                # (ns, nv, 3)
                 # split-major order: consecutive elements have the same
                 # orientation, so their neighbours' flux points sit in the
                 # same rows and interface gathers stay coalesced
@@ -267,6 +294,9 @@ class SimplexBoxMesh:
                 v = np.concatenate(verts[et])           # (E, nv, 3)
                 sp = np.ascontiguousarray(v.transpose(1, 0, 2))
                 if sorder > 1 or warp is not None:
+                    if et == 'hex':
+                        raise ValueError('Hex layers take no shape order '
+                                         'or warp')
                     sp = self._shape_points(et, sp, sorder, warp)
                 self.spts[et] = sp
                 self._nodes[et] = nodes[et]
@@ -295,6 +325,15 @@ class SimplexBoxMesh:
         layers = [(et, k) for et, k in (('tet', ntet), ('pri', npri)) if k]
         return cls([((n, n, ntet + npri), length/n, layers)], periodic,
                    sorder, warp)
+
+    @classmethod
+    def hex_prism(cls, n, nhex, npri, length=2*np.pi,
+                  periodic=(True, True, True)):
+        """n x n x (nhex + npri) cells of size length/n: hexes below,
+        prisms extruded along x above (hex-prism interfaces on quad faces at
+        both ends of the prism slab when periodic in z)."""
+        layers = [(s, k) for s, k in (('hex', nhex), ('prx', npri)) if k]
+        return cls([((n, n, nhex + npri), length/n, layers)], periodic)
 
     @classmethod
     def disjoint(cls, ntet, npri, length=2*np.pi):

@@ -1,0 +1,140 @@
+"""GPU parity on meshes of hexes and prisms together: hexes below a slab of
+prisms extruded along x, meeting on whole quad faces (a hex core with
+prism layers).  The hexes run the tensor-product kernels, the prisms the
+dense matrix-core kernels; the interface kernels pair flux points across
+the two types.  Euler and NS RHS (periodic, and with boundaries on hex and
+prism faces), RK4 and RK45 trajectories, FP64 and FP32, against the
+oracle's mixed-mesh system with its hex element type.
+
+Tolerances as for the tet/prism tests: FP64 RHS max(1e-12, 10 x the
+oracle's 1e-16-input-noise floor), trajectories 1e-12; FP32 1e-4."""
+import numpy as np
+import pytest
+
+from tests.fr_cases import eval_ics, make_simplex_cfg, mode_ics, rel_l2
+from tests.test_gpu_simplex import MU, SIMPLEX_BCS, _cat, _noise_floor, \
+    _rk4_dict
+from tests.test_hex_prism import _tables
+
+pytestmark = pytest.mark.gpu
+
+
+def _hex_prism(be, n, nh, npr, order, system='euler', bcs=False,
+               rsolver='rusanov', beta=0.0, tau=0.0, scheme='rk4', dt=1e-3):
+    from oracle.simplex import OracleMixedSystem
+    from pyfr_amd.mesh import SimplexBoxMesh
+    from pyfr_amd.solvers import get_integrator
+    ics = mode_ics(seed=6)
+    cfg = make_simplex_cfg(order, be.precision, scheme, dt, ics=ics,
+                           rsolver=rsolver, system=system, beta=beta,
+                           tau=tau)
+    cfg.set('backend-hip', 'fused', str(be.fused))
+    c = {'gamma': 1.4}
+    if system == 'navier-stokes':
+        cfg.set('constants', 'mu', repr(MU))
+        c |= {'mu': MU, 'Pr': 0.71, 'ldg-beta': beta, 'ldg-tau': tau}
+    periodic = (False, True, False) if bcs else (True, True, True)
+    obcs = {}
+    if bcs:
+        for side, (bctype, opts) in SIMPLEX_BCS[system].items():
+            sect = f'soln-bcs-{side}'
+            cfg.set(sect, 'type', bctype)
+            vals = {}
+            for k, v in opts.items():
+                cfg.set(sect, k, v)
+                if k == 'cpTw':
+                    c[k] = float(eval(v))
+                else:
+                    vals[k] = eval_ics({k: v})[k]
+            obcs[side] = (bctype, vals)
+    mesh = SimplexBoxMesh.hex_prism(n, nh, npr, periodic=periodic)
+    integ = get_integrator(be, mesh, cfg)
+    L = 2*np.pi
+    verts = {et: v.transpose(0, 2, 1) for et, v in mesh.spts.items()}
+    idx = {et: np.arange(v.shape[1]) for et, v in mesh.spts.items()}
+    orc = OracleMixedSystem(verts, order, {et: _tables(et, order)
+                                           for et in verts},
+                            [((L, L, (nh + npr)*L/n), idx)],
+                            rsolver=rsolver, system=system, constants=c,
+                            periodic=periodic, bcs=obcs)
+    return integ, orc, orc.ics(eval_ics(ics))
+
+
+def _rhs_err(integ, orc, u0):
+    sysm = integ.system
+    names = sysm.ele_types
+    assert names == ['hex', 'pri']
+    got0 = dict(zip(names, sysm.ele_scal_upts(0)))
+    assert rel_l2(_cat(got0, names), _cat(u0, names)) < 1e-7
+    sysm.rhs(0.0, 0, 1)
+    got = dict(zip(names, sysm.ele_scal_upts(1)))
+    ref = orc.rhs(u0)
+    return (rel_l2(_cat(got, names), _cat(ref, names)),
+            max(1e-12, 10*_noise_floor(orc, u0)))
+
+
+@pytest.mark.parametrize('fused', ['flux', 'div', 'none'])
+@pytest.mark.parametrize('order', [1, 2, 3, 4])
+def test_euler_rhs_parity(backend_factory, order, fused):
+    be = backend_factory(fused=fused != 'none', fuse_flux=fused == 'flux')
+    print(f'\n[hex-prism euler] p={order} fused={fused}', flush=True)
+    integ, orc, u0 = _hex_prism(be, 3, 2, 1, order)
+    err, tol = _rhs_err(integ, orc, u0)
+    print(f'[hex-prism euler] rel L2 {err:.3e} (tol {tol:.3e})', flush=True)
+    assert err < tol
+
+
+@pytest.mark.parametrize('fused', [True, False])
+@pytest.mark.parametrize('order,beta,tau', [(2, 0.0, 0.0), (3, 0.5, 0.1),
+                                            (4, -0.5, 0.0)])
+def test_ns_rhs_parity(backend_factory, order, beta, tau, fused):
+    be = backend_factory(fused=fused)
+    print(f'\n[hex-prism ns] p={order} beta={beta} tau={tau} '
+          f'fused={fused}', flush=True)
+    integ, orc, u0 = _hex_prism(be, 2, 1, 1, order, 'navier-stokes',
+                                beta=beta, tau=tau)
+    err, tol = _rhs_err(integ, orc, u0)
+    print(f'[hex-prism ns] rel L2 {err:.3e} (tol {tol:.3e})', flush=True)
+    assert err < tol
+
+
+@pytest.mark.parametrize('system,order,rsolver', [
+    ('euler', 2, 'hllc'), ('euler', 3, 'roe'),
+    ('navier-stokes', 2, 'rusanov'), ('navier-stokes', 3, 'hll')])
+def test_bc_rhs_parity(backend_factory, system, order, rsolver):
+    """Periodic in y only: the x sides cut both hexes and prisms (quad
+    faces of both types), the bottom is hex faces, the top prism faces."""
+    be = backend_factory()
+    print(f'\n[hex-prism bc] {system} p={order} {rsolver}', flush=True)
+    integ, orc, u0 = _hex_prism(be, 3, 2, 1, order, system, bcs=True,
+                                rsolver=rsolver)
+    assert len(integ.system._bc_inters) == 4
+    err, tol = _rhs_err(integ, orc, u0)
+    print(f'[hex-prism bc] rel L2 {err:.3e} (tol {tol:.3e})', flush=True)
+    assert err < tol
+
+
+@pytest.mark.parametrize('scheme', ['rk4', 'rk45'])
+def test_trajectory(backend_factory, scheme):
+    from oracle.rhs import rk4_step, rk45_step
+    be = backend_factory()
+    integ, orc, u0 = _hex_prism(be, 3, 2, 1, 3, scheme=scheme, dt=2e-3)
+    names = integ.system.ele_types
+    print(f'\n[hex-prism {scheme}] 10 steps', flush=True)
+    integ.run_steps(10, 2e-3)
+    step = rk4_step if scheme == 'rk4' else rk45_step
+    u = u0
+    for _ in range(10):
+        u = _rk4_dict(orc.rhs, u, 2e-3, step)
+    err = rel_l2(_cat(dict(zip(names, integ.soln)), names), _cat(u, names))
+    print(f'[hex-prism {scheme}] rel L2 {err:.3e}', flush=True)
+    assert err < 1e-12
+
+
+def test_fp32_rhs(backend_factory):
+    be = backend_factory(precision='single')
+    print('\n[hex-prism fp32] p=4', flush=True)
+    integ, orc, u0 = _hex_prism(be, 3, 2, 1, 4)
+    err, _ = _rhs_err(integ, orc, u0)
+    print(f'[hex-prism fp32] rel L2 {err:.3e}', flush=True)
+    assert err < 1e-4
