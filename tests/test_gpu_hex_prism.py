@@ -3,8 +3,9 @@ prisms extruded along x, meeting on whole quad faces (a hex core with
 prism layers).  The hexes run the tensor-product kernels, the prisms the
 dense matrix-core kernels; the interface kernels pair flux points across
 the two types.  Euler and NS RHS (periodic, and with boundaries on hex and
-prism faces), RK4 and RK45 trajectories, FP64 and FP32, against the
-oracle's mixed-mesh system with its hex element type.
+prism faces), flux and surface-flux anti-aliasing, RK4 and RK45
+trajectories, FP64 and FP32, against the oracle's mixed-mesh system with
+its hex element type.
 
 Tolerances as for the tet/prism tests: FP64 RHS max(1e-12, 10 x the
 oracle's 1e-16-input-noise floor), trajectories 1e-12; FP32 1e-4."""
@@ -20,7 +21,8 @@ pytestmark = pytest.mark.gpu
 
 
 def _hex_prism(be, n, nh, npr, order, system='euler', bcs=False,
-               rsolver='rusanov', beta=0.0, tau=0.0, scheme='rk4', dt=1e-3):
+               rsolver='rusanov', beta=0.0, tau=0.0, scheme='rk4', dt=1e-3,
+               aa=()):
     from oracle.simplex import OracleMixedSystem
     from pyfr_amd.mesh import SimplexBoxMesh
     from pyfr_amd.solvers import get_integrator
@@ -29,6 +31,8 @@ def _hex_prism(be, n, nh, npr, order, system='euler', bcs=False,
                            rsolver=rsolver, system=system, beta=beta,
                            tau=tau)
     cfg.set('backend-hip', 'fused', str(be.fused))
+    if aa:
+        cfg.set('solver', 'anti-alias', ', '.join(aa))
     c = {'gamma': 1.4}
     if system == 'navier-stokes':
         cfg.set('constants', 'mu', repr(MU))
@@ -52,11 +56,12 @@ def _hex_prism(be, n, nh, npr, order, system='euler', bcs=False,
     L = 2*np.pi
     verts = {et: v.transpose(0, 2, 1) for et, v in mesh.spts.items()}
     idx = {et: np.arange(v.shape[1]) for et, v in mesh.spts.items()}
-    orc = OracleMixedSystem(verts, order, {et: _tables(et, order)
+    orc = OracleMixedSystem(verts, order, {et: _tables(et, order, aa)
                                            for et in verts},
                             [((L, L, (nh + npr)*L/n), idx)],
                             rsolver=rsolver, system=system, constants=c,
-                            periodic=periodic, bcs=obcs)
+                            periodic=periodic, bcs=obcs,
+                            antialias=tuple(a for a in aa if a == 'flux'))
     return integ, orc, orc.ics(eval_ics(ics))
 
 
@@ -111,6 +116,27 @@ def test_bc_rhs_parity(backend_factory, system, order, rsolver):
     assert len(integ.system._bc_inters) == 4
     err, tol = _rhs_err(integ, orc, u0)
     print(f'[hex-prism bc] rel L2 {err:.3e} (tol {tol:.3e})', flush=True)
+    assert err < tol
+
+
+@pytest.mark.parametrize('fused', [True, False])
+@pytest.mark.parametrize('system,order,aa,bcs', [
+    ('euler', 2, ('flux',), False), ('euler', 3, ('surf-flux',), False),
+    ('euler', 2, ('flux', 'surf-flux'), True),
+    ('navier-stokes', 2, ('flux',), False),
+    ('navier-stokes', 3, ('surf-flux',), True),
+    ('navier-stokes', 2, ('flux', 'surf-flux'), False)])
+def test_antialias_rhs_parity(backend_factory, system, order, aa, bcs,
+                              fused):
+    """Each type on its own default rules: Gauss-Legendre (p + 2)^3 in the
+    hexes, the reference's default prism rule, and (p + 2)^2 GL points on
+    every quad face, hex and prism alike, so the faces still pair."""
+    be = backend_factory(fused=fused)
+    print(f'\n[hex-prism aa] {system} p={order} aa={aa} bcs={bcs} '
+          f'fused={fused}', flush=True)
+    integ, orc, u0 = _hex_prism(be, 3, 2, 1, order, system, bcs=bcs, aa=aa)
+    err, tol = _rhs_err(integ, orc, u0)
+    print(f'[hex-prism aa] rel L2 {err:.3e} (tol {tol:.3e})', flush=True)
     assert err < tol
 
 

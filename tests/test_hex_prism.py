@@ -21,28 +21,46 @@ from pyfr_amd.mesh import SimplexBoxMesh
 from tests.fr_cases import eval_ics, mode_ics, oracle_consts
 
 
-def _tables(name, p):
+def _tables(name, p, aa=()):
+    """Oracle tables of a hex or prism at order p, with the default
+    anti-aliasing rules for aa: Gauss-Legendre (p + 2) per direction for
+    the hex element and all quad faces, the reference's default prism and
+    triangle rules."""
     from pyfr_amd.quadtables import TABLES
+    from pyfr_amd.shapes import gauss_legendre
+    from pyfr_amd.simplex import AA_RULES, SF_TRI_RULES
     if name == 'hex':
-        return {}
+        return ({'nq1': p + 2} if 'flux' in aa else {}) | (
+            {'nf1': p + 2} if 'surf-flux' in aa else {})
     nu = (p + 1)**2*(p + 2)//2
-    return {'upts': TABLES[('pri', 'williams-shunn~gauss-legendre', nu)],
-            'tri_fpts': TABLES[('tri', 'williams-shunn',
-                                (p + 1)*(p + 2)//2)][0],
-            'tri_q36': TABLES[('tri', 'williams-shunn', 36)]}
+    t = {'upts': TABLES[('pri', 'williams-shunn~gauss-legendre', nu)],
+         'tri_fpts': TABLES[('tri', 'williams-shunn',
+                             (p + 1)*(p + 2)//2)][0],
+         'tri_q36': TABLES[('tri', 'williams-shunn', 36)]}
+    if 'flux' in aa:
+        t['qrule'] = TABLES[('pri', *AA_RULES['pri'][p])]
+    if 'surf-flux' in aa:
+        g, w = gauss_legendre(p + 2)
+        r = range(p + 2)
+        t['iqrules'] = {
+            'tri': TABLES[('tri', *SF_TRI_RULES[p])],
+            'quad': (np.array([(g[i], g[j]) for j in r for i in r]),
+                     np.array([w[i]*w[j] for j in r for i in r]))}
+    return t
 
 
 def _oracle(mesh, order, lz, system='euler', periodic=(True, True, True),
-            bcs=None):
+            bcs=None, aa=()):
     from oracle.simplex import OracleMixedSystem
     L = 2*np.pi
     verts = {et: v.transpose(0, 2, 1) for et, v in mesh.spts.items()}
     idx = {et: np.arange(v.shape[1]) for et, v in mesh.spts.items()}
-    return OracleMixedSystem(verts, order, {et: _tables(et, order)
+    return OracleMixedSystem(verts, order, {et: _tables(et, order, aa)
                                             for et in verts},
                              [((L, L, lz), idx)], system=system,
                              constants=oracle_consts(system),
-                             periodic=periodic, bcs=bcs)
+                             periodic=periodic, bcs=bcs,
+                             antialias=tuple(a for a in aa if a == 'flux'))
 
 
 def test_connectivity():
@@ -90,14 +108,17 @@ def test_gmsh_round_trip(version):
         assert sorted(m.bcon[name]) == sorted(ref.bcon[side])
 
 
-@pytest.mark.parametrize('system,order', [('euler', 1), ('euler', 3),
-                                          ('navier-stokes', 2)])
-def test_oracle_hex_type_matches_hex_oracle(system, order):
+@pytest.mark.parametrize('system,order,aa', [
+    ('euler', 1, ()), ('euler', 3, ()), ('navier-stokes', 2, ()),
+    ('euler', 2, ('flux',)), ('euler', 2, ('surf-flux',)),
+    ('navier-stokes', 2, ('flux', 'surf-flux'))])
+def test_oracle_hex_type_matches_hex_oracle(system, order, aa):
     from oracle.rhs import OracleHexSystem
     n = 3
     mesh = SimplexBoxMesh.hex_prism(n, n, 0)
-    orc = _oracle(mesh, order, 2*np.pi, system)
-    ref = OracleHexSystem(n, order, system, constants=oracle_consts(system))
+    orc = _oracle(mesh, order, 2*np.pi, system, aa=aa)
+    ref = OracleHexSystem(n, order, system, constants=oracle_consts(system),
+                          antialias=aa, nq1=order + 2, nf1=order + 2)
     assert np.array_equal(mesh.spts['hex'].transpose(0, 2, 1), ref.verts)
     u = orc.ics(eval_ics(mode_ics(seed=2)))
     r = orc.rhs(u)['hex']
@@ -106,12 +127,13 @@ def test_oracle_hex_type_matches_hex_oracle(system, order):
     assert np.abs(r - rr).max() < 1e-12*np.abs(rr).max()
 
 
-@pytest.mark.parametrize('system,order', [('euler', 2), ('euler', 4),
-                                          ('navier-stokes', 3)])
-def test_oracle_free_stream_and_conservation(system, order):
+@pytest.mark.parametrize('system,order,aa', [
+    ('euler', 2, ()), ('euler', 4, ()), ('navier-stokes', 3, ()),
+    ('euler', 2, ('flux', 'surf-flux')), ('navier-stokes', 2, ('surf-flux',))])
+def test_oracle_free_stream_and_conservation(system, order, aa):
     n, nh, npr = 3, 2, 1
     mesh = SimplexBoxMesh.hex_prism(n, nh, npr)
-    orc = _oracle(mesh, order, (nh + npr)*2*np.pi/n, system)
+    orc = _oracle(mesh, order, (nh + npr)*2*np.pi/n, system, aa=aa)
     uu = {}
     for et, e in orc.et.items():
         uu[et] = np.empty((e.nupts, 5, e.neles))
