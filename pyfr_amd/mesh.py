@@ -255,8 +255,9 @@ class SimplexBoxMesh:
     ``bcon``: {name: [(etype, eidx, fidx), ...]} in that key order.
 
     sorder, warp: order-k shape points (``std_ele(k)`` order) of each
-    element's affine map, moved by ``warp(x, y, z) -> (x, y, z)`` (curved
-    elements; the warp must respect the periods).
+    element's affine (hex: trilinear) map, moved by ``warp(x, y, z) -> (x,
+    y, z)`` (curved elements; the warp must respect the periods).  Hexes
+    of order k > 1 are all flagged curved in ``spts_curved``.
     """
     ndims = 3
     etype_order = ('hex', 'tet', 'pri')
@@ -294,24 +295,37 @@ class SimplexBoxMesh:
                 v = np.concatenate(verts[et])           # (E, nv, 3)
                 sp = np.ascontiguousarray(v.transpose(1, 0, 2))
                 if sorder > 1 or warp is not None:
-                    if et == 'hex':
-                        raise ValueError('Hex layers take no shape order '
-                                         'or warp')
                     sp = self._shape_points(et, sp, sorder, warp)
                 self.spts[et] = sp
                 self._nodes[et] = nodes[et]
         self.etypes = list(self.spts)
         self.neles = sum(v.shape[1] for v in self.spts.values())
+        # hexes of shape order k > 1 are all curved (their stored metric);
+        # tets and prisms are curved by their shape point count alone
+        if 'hex' in self.spts and sorder > 1:
+            self.spts_curved = {'hex': np.ones(self.spts['hex'].shape[1],
+                                               dtype=bool)}
         self.con = self._connect()
 
     @staticmethod
     def _shape_points(et, verts, k, warp):
         """(nspts, E, 3) order-k shape points of the affine map of verts
         (nverts, E, 3), warped."""
-        from pyfr_amd.geometry import AffineGeometry
-        from pyfr_amd.simplex import SIMPLEX_SHAPES
-        ref = SIMPLEX_SHAPES[et].std_ele(k)
-        x = AffineGeometry(verts, et).ploc_at(ref)       # (nspts, 3, E)
+        if et == 'hex':
+            # trilinear map of the 8 vertices at the (k + 1)^3 equispaced
+            # points of HexShape.std_ele(k) (x fastest)
+            t = np.linspace(0, 1, k + 1)
+            xi = np.array([(a, b, c) for c in t for b in t for a in t])
+            w = np.ones((len(xi), 8))
+            for v in range(8):
+                for a in range(3):
+                    w[:, v] *= xi[:, a] if (v >> a) & 1 else 1 - xi[:, a]
+            x = np.einsum('qv,veb->qbe', w, verts)       # (nspts, 3, E)
+        else:
+            from pyfr_amd.geometry import AffineGeometry
+            from pyfr_amd.simplex import SIMPLEX_SHAPES
+            ref = SIMPLEX_SHAPES[et].std_ele(k)
+            x = AffineGeometry(verts, et).ploc_at(ref)   # (nspts, 3, E)
         if warp is not None:
             x = np.stack(warp(x[:, 0], x[:, 1], x[:, 2]), axis=1)
         return np.ascontiguousarray(x.transpose(0, 2, 1))
@@ -328,12 +342,13 @@ class SimplexBoxMesh:
 
     @classmethod
     def hex_prism(cls, n, nhex, npri, length=2*np.pi,
-                  periodic=(True, True, True)):
+                  periodic=(True, True, True), sorder=1, warp=None):
         """n x n x (nhex + npri) cells of size length/n: hexes below,
         prisms extruded along x above (hex-prism interfaces on quad faces at
         both ends of the prism slab when periodic in z)."""
         layers = [(s, k) for s, k in (('hex', nhex), ('prx', npri)) if k]
-        return cls([((n, n, nhex + npri), length/n, layers)], periodic)
+        return cls([((n, n, nhex + npri), length/n, layers)], periodic,
+                   sorder, warp)
 
     @classmethod
     def disjoint(cls, ntet, npri, length=2*np.pi):

@@ -3,7 +3,8 @@ prisms extruded along x, meeting on whole quad faces (a hex core with
 prism layers).  The hexes run the tensor-product kernels, the prisms the
 dense matrix-core kernels; the interface kernels pair flux points across
 the two types.  Euler and NS RHS (periodic, and with boundaries on hex and
-prism faces), flux and surface-flux anti-aliasing, RK4 and RK45
+prism faces), flux and surface-flux anti-aliasing, curved (warped)
+meshes of both types, RK4 and RK45
 trajectories, FP64 and FP32, against the oracle's mixed-mesh system with
 its hex element type.
 
@@ -15,14 +16,14 @@ import pytest
 from tests.fr_cases import eval_ics, make_simplex_cfg, mode_ics, rel_l2
 from tests.test_gpu_simplex import MU, SIMPLEX_BCS, _cat, _noise_floor, \
     _rk4_dict
-from tests.test_hex_prism import _tables
+from tests.test_hex_prism import _tables, warp
 
 pytestmark = pytest.mark.gpu
 
 
 def _hex_prism(be, n, nh, npr, order, system='euler', bcs=False,
                rsolver='rusanov', beta=0.0, tau=0.0, scheme='rk4', dt=1e-3,
-               aa=()):
+               aa=(), sorder=1, warp=None):
     from oracle.simplex import OracleMixedSystem
     from pyfr_amd.mesh import SimplexBoxMesh
     from pyfr_amd.solvers import get_integrator
@@ -51,7 +52,8 @@ def _hex_prism(be, n, nh, npr, order, system='euler', bcs=False,
                 else:
                     vals[k] = eval_ics({k: v})[k]
             obcs[side] = (bctype, vals)
-    mesh = SimplexBoxMesh.hex_prism(n, nh, npr, periodic=periodic)
+    mesh = SimplexBoxMesh.hex_prism(n, nh, npr, periodic=periodic,
+                                    sorder=sorder, warp=warp)
     integ = get_integrator(be, mesh, cfg)
     L = 2*np.pi
     verts = {et: v.transpose(0, 2, 1) for et, v in mesh.spts.items()}
@@ -137,6 +139,37 @@ def test_antialias_rhs_parity(backend_factory, system, order, aa, bcs,
     integ, orc, u0 = _hex_prism(be, 3, 2, 1, order, system, bcs=bcs, aa=aa)
     err, tol = _rhs_err(integ, orc, u0)
     print(f'[hex-prism aa] rel L2 {err:.3e} (tol {tol:.3e})', flush=True)
+    assert err < tol
+
+
+def _warp_bc(x, y, z):
+    """Moves no point off the non-periodic x and z sides."""
+    return (x, y + 0.08*np.sin(x)*np.sin(z), z)
+
+
+@pytest.mark.parametrize('fused', [True, False])
+@pytest.mark.parametrize('system,order,sorder,extra', [
+    ('euler', 1, 2, None), ('euler', 2, 2, None), ('euler', 3, 3, None),
+    ('navier-stokes', 2, 2, None), ('navier-stokes', 3, 2, None),
+    ('euler', 2, 2, 'bcs'), ('navier-stokes', 2, 2, 'bcs'),
+    ('euler', 2, 2, 'flux')])
+def test_curved_rhs_parity(backend_factory, system, order, sorder, extra,
+                           fused):
+    """A warped box of order-k hexes and prisms, all curved: the stored
+    invariant-form metric in both types, curved normals on both sides of
+    every hex-prism face; with boundaries and with flux anti-aliasing."""
+    be = backend_factory(fused=fused)
+    print(f'\n[hex-prism curved] {system} p={order} k={sorder} {extra} '
+          f'fused={fused}', flush=True)
+    bcs = extra == 'bcs'
+    integ, orc, u0 = _hex_prism(be, 3, 2, 1, order, system, bcs=bcs,
+                                aa=('flux',) if extra == 'flux' else (),
+                                sorder=sorder, warp=_warp_bc if bcs else warp)
+    e = integ.system.ele_map
+    # the linear region starts at or past the end (rounded up to a block)
+    assert e['hex'].linoff >= e['hex'].neles and e['pri'].curved_simplex
+    err, tol = _rhs_err(integ, orc, u0)
+    print(f'[hex-prism curved] rel L2 {err:.3e} (tol {tol:.3e})', flush=True)
     assert err < tol
 
 

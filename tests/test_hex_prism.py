@@ -7,8 +7,8 @@ element set per type and interfaces between them on their quad faces
 Pins: the mesh's connectivity (every face paired once, quad faces across
 the hex-prism planes, the Gmsh round trip); the oracle's hex element type
 inside its mixed-mesh system reproduces the hex oracle exactly on an
-all-hex box; on the hex-prism mesh a uniform state is steady and the
-weighted RHS totals vanish (a wrongly oriented element breaks the steady
+all-hex box, straight or curved; on the curved hex-prism mesh a uniform
+state is steady and the weighted RHS totals vanish (a wrongly oriented element breaks the steady
 state; an unpaired face, or flux points paired across unequal quadrature
 weights, breaks conservation)."""
 import collections
@@ -47,6 +47,12 @@ def _tables(name, p, aa=()):
             'quad': (np.array([(g[i], g[j]) for j in r for i in r]),
                      np.array([w[i]*w[j] for j in r for i in r]))}
     return t
+
+
+def warp(x, y, z):
+    """A smooth periodic warp of the box (every period 2 pi)."""
+    return (x + 0.12*np.sin(y)*np.sin(z), y + 0.08*np.sin(x)*np.cos(z),
+            z + 0.05*np.sin(x + y))
 
 
 def _oracle(mesh, order, lz, system='euler', periodic=(True, True, True),
@@ -127,12 +133,35 @@ def test_oracle_hex_type_matches_hex_oracle(system, order, aa):
     assert np.abs(r - rr).max() < 1e-12*np.abs(rr).max()
 
 
+@pytest.mark.parametrize('system,order,sorder', [
+    ('euler', 1, 2), ('euler', 2, 2), ('euler', 3, 3),
+    ('navier-stokes', 2, 2)])
+def test_oracle_curved_hex_type_matches_hex_oracle(system, order, sorder):
+    """Curved hexes (order-k shape points of a warped box, all flagged
+    curved): the mixed system's invariant metric on std_ele(max(p, 1))
+    gives the hex oracle's curved RHS."""
+    from oracle.rhs import OracleHexSystem
+    n = 3
+    mesh = SimplexBoxMesh.hex_prism(n, n, 0, sorder=sorder, warp=warp)
+    assert mesh.spts_curved['hex'].all()
+    orc = _oracle(mesh, order, 2*np.pi, system)
+    assert orc.et['hex'].curved
+    ref = OracleHexSystem(n, order, system, constants=oracle_consts(system),
+                          verts=mesh.spts['hex'].transpose(0, 2, 1),
+                          curved=np.ones(n**3, dtype=bool))
+    u = orc.ics(eval_ics(mode_ics(seed=2)))
+    r = orc.rhs(u)['hex']
+    rr = ref.rhs(u['hex'])
+    assert np.abs(rr).max() > 1e-2
+    assert np.abs(r - rr).max() < 1e-12*np.abs(rr).max()
+
+
 @pytest.mark.parametrize('system,order,aa', [
     ('euler', 2, ()), ('euler', 4, ()), ('navier-stokes', 3, ()),
     ('euler', 2, ('flux', 'surf-flux')), ('navier-stokes', 2, ('surf-flux',))])
 def test_oracle_free_stream_and_conservation(system, order, aa):
     n, nh, npr = 3, 2, 1
-    mesh = SimplexBoxMesh.hex_prism(n, nh, npr)
+    mesh = SimplexBoxMesh.hex_prism(n, nh, npr, sorder=2, warp=warp)
     orc = _oracle(mesh, order, (nh + npr)*2*np.pi/n, system, aa=aa)
     uu = {}
     for et, e in orc.et.items():
