@@ -108,7 +108,11 @@ int frhip_mul_plan_tune(void *plan, int ncol, const void *b, long ldb,
  *      with leading dimension ldr); 2 = negdivconf then the rkvdh2 stage on
  *      r1 = xa and r2 = c (r1' = r1 + dta*t, r2' = r1 + dtb*t; if last only
  *      r1' = r1 + dtb*t); 3 = as 2 with FP32 operands, an FP64 accumulated
- *      solution xa and the FP32 next-stage input written to c.
+ *      solution xa and the FP32 next-stage input written to c; 10..13 =
+ *      negdivconf then RK4 stage ep - 9 with four registers, as for
+ *      frhip_tdivf_flux_euler (xa = device pair {r1, r0}, dta = dt, c the
+ *      next stage's input; structured plans only, the updates of
+ *      pyfr/integrators/explicit/steppers.py:67-108).
  * They fuse kernels the reference runs back to back on the same element
  * data: tgradpcoru_upts + tgradcoru_upts (pyfr/solvers/baseadvecdiff/
  * elements.py:47-55) and tdivtpcorf + tdivtconf + negdivconf

@@ -542,21 +542,28 @@ class MulProvider:
         plan, kind = self._plan2(a0, a1, out.dtype)
         if out.ncol % (64*5):
             raise NotSuitableError('fused kernels need 5-variable AoSoA data')
+        if ep >= 10 and kind != 1:
+            raise NotSuitableError('RK4 stage epilogues need a structured '
+                                   'plan')
         lib = self.backend.lib
         ncol = out.ncol
         if kind == 2:
             self._tune(lib.mul2_plan_tune, plan, ncol, b0.data, b0.leaddim,
                        b1.data, b1.leaddim)
+        if ep >= 10:
+            # RK4 stages: xa = (r1, r0), passed as a device pointer pair
+            xap, xam = self._ptr_pair(*xa), tuple(xa)
+        else:
+            xap, xam = None if xa is None else xa.data, ()
         args = (plan, ncol, b0.data, b0.leaddim, b1.data, b1.leaddim,
                 out.data, out.leaddim, ep,
                 None if rcpdjac is None else rcpdjac.data,
-                0 if rcpdjac is None else _rstride(rcpdjac),
-                None if xa is None else xa.data)
+                0 if rcpdjac is None else _rstride(rcpdjac), xap)
 
         def fn(stream, dta=0.0, dtb=0.0, last=0):
             lib.mul2(*args, dta, dtb, int(last), stream)
         mats = (a0, a1, b0, b1, out) + (() if rcpdjac is None else
-                                        (rcpdjac,))
+                                        (rcpdjac,)) + xam
         k = Kernel(fn, rtargs={'dta': 0.0, 'dtb': 0.0, 'last': 0},
                    mats=mats, name=f'mul2_ep{ep}')
         k.kind = {1: 'structured', 2: 'dense', 3: 'bf16'}[kind]
@@ -569,9 +576,12 @@ class MulProvider:
     def tdivf(self, a0, a1, b0, b1, out, rcpdjac, xa=None, stage=None):
         """tdivtpcorf + tdivtconf + negdivconf, optionally followed by the
         rkvdh2 stage update (stage='rk45': xa = r1, out = r2; 'rk45-mixed':
-        xa = FP64 accumulated solution, out = FP32 stage input).  The stage
+        xa = FP64 accumulated solution, out = FP32 stage input), or RK4
+        stage k's updates (stage='rk4-k': xa = (r1, r0), out = the next
+        stage's input, dta = dt; structured plans only).  The stage
         coefficients are bound at run time: bind(dta=, dtb=, last=)."""
-        ep = {None: 1, 'rk45': 2, 'rk45-mixed': 3}[stage]
+        ep = {None: 1, 'rk45': 2, 'rk45-mixed': 3, 'rk4-1': 10, 'rk4-2': 11,
+              'rk4-3': 12, 'rk4-4': 13}[stage]
         return self._fused(a0, a1, b0, b1, out, ep, rcpdjac, xa)
 
     def tuning_report(self):

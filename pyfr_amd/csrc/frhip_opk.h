@@ -53,14 +53,16 @@ using FKF = void (*)(int, const float *, long, const float *, long, float *,
 
 // Fused two-input kernels: C = [A0 | A1] @ [B0; B1] with an epilogue
 // (0: store, 1: negdivconf, 2: + RK45 stage, 3: + mixed RK45 stage,
-// 4 / 5: the last stage of 2 / 3).  fd[3], fd[5] are null (FP32 only).
+// 4 / 5: the last stage of 2 / 3, 10-13: + RK4 stage 1-4 in slots 6-9).
+// fd[3], fd[5] are null (FP32 only); gradient kernels fill slot 0 only.
 struct FusedKernelEntry {
     int m, k0, k1, nnz;
     const int32_t *lin;      // row-major indices into the m x (k0 + k1) A
     const double *vals;
     const char *desc;
-    FKD fd[6];
-    FKF ff[6];
+    // by epilogue slot: ep 0-5, then ep 10-13 (null where not built)
+    FKD fd[10];
+    FKF ff[10];
 };
 
 extern const FusedKernelEntry g_fk_table[];

@@ -147,6 +147,10 @@ def emit_kernel(name, A):
 #         (xa) and FP32 next-stage input (c)
 #   EP 4, 5: the last stage of EP 2, 3 (r1 += dtb*t; EP 5 also stores the
 #         new solution rounded to float into c)
+#   EP 10-13: RK4 stage EP - 9 with four registers (dta = dt): xa points to
+#         the device pointer pair {r1, r0}, c receives the next stage's
+#         input (the axnpby updates of pyfr/integrators/explicit/
+#         steppers.py:67-108, operands in the same order)
 # The last stage is a template value, not a run-time flag: a uniform branch
 # per output row makes the compiler keep every row's RK operands live and
 # spill.  The trailing int argument is unused.
@@ -156,7 +160,10 @@ def emit_kernel(name, A):
 # reference's OpenMP backend applies (pyfr/solvers/baseadvec/system.py:126-132,
 # pyfr/solvers/baseadvecdiff/system.py:174-203).
 FUSED = [('grad', ['M4 - M6*M0', 'M6']), ('div', ['M1 - M3*M2', 'M3'])]
-FUSED_EPS = {'double': [0, 1, 2, 4], 'float': [0, 1, 2, 3, 4, 5]}
+FUSED_EPS = {'double': [0, 1, 2, 4, 10, 11, 12, 13],
+             'float': [0, 1, 2, 3, 4, 5, 10, 11, 12, 13]}
+# kernel table slot of each epilogue (FusedKernelEntry::fd / ff)
+EP_SLOTS = [0, 1, 2, 3, 4, 5, 10, 11, 12, 13]
 
 
 def emit_fused(name, A0, A1):
@@ -199,6 +206,31 @@ def emit_fused(name, A0, A1):
         L.append('        } else if constexpr (EP == 4) {')
         L.append(f'          T *r1 = static_cast<T *>(xa) + j + {r}*ldc;')
         L.append('          *r1 = *r1 + T(dtb)*t;')
+        L.append('        } else if constexpr (EP >= 10) {')
+        L.append('          T *const *pp = static_cast<T *const *>(xa);')
+        L.append(f'          T *r1 = pp[0] + j + {r}*ldc;')
+        L.append(f'          const T r0 = pp[1][j + {r}*ldc];')
+        L.append('          if constexpr (EP == 10) {')
+        L.append('            *r1 = t;')
+        L.append('            T a = T(1)*r0;')
+        L.append('            a = a + T(dta/2.0)*t;')
+        L.append(f'            c[{r}*ldc] = a;')
+        L.append('          } else if constexpr (EP == 11) {')
+        L.append('            T a = T(dta/6.0)*(*r1);')
+        L.append('            a = a + T(1)*r0;')
+        L.append('            a = a + T(dta/3.0)*t;')
+        L.append('            *r1 = a;')
+        L.append('            T b = T(dta/2.0)*t;')
+        L.append('            b = b + T(1)*r0;')
+        L.append(f'            c[{r}*ldc] = b;')
+        L.append('          } else if constexpr (EP == 12) {')
+        L.append('            *r1 += T(dta/3.0)*t;')
+        L.append('            T b = T(dta)*t;')
+        L.append('            b = b + T(1)*r0;')
+        L.append(f'            c[{r}*ldc] = b;')
+        L.append('          } else {')
+        L.append('            *r1 += T(dta/6.0)*t;')
+        L.append('          }')
         L.append('        } else {')
         L.append(f'          double *r1 = static_cast<double *>(xa) + j + '
                  f'{r}*ldc;')
@@ -303,14 +335,15 @@ def main(outdir):
             parts.append(f'extern const double val_{name}[{nnz}] = {{' +
                          ','.join(float(x).hex() for x in vals) + '};')
             for t in ('double', 'float'):
-                for ep in FUSED_EPS[t]:
+                # the gradient kernel only stores
+                for ep in FUSED_EPS[t] if fname == 'div' else [0]:
                     parts.append(f'template __global__ void {name}<{t}, {ep}>'
                                  f'(int, const {t} *, long, const {t} *, '
                                  f'long, {t} *, long, const {t} *, long, '
                                  f'void *, double, double, int);')
             parts.append('')
             ftable.append((name, A0.shape[0], A0.shape[1], A1.shape[1], nnz,
-                           f'hex p={p} fused {fname}'))
+                           f'hex p={p} fused {fname}', fname))
             decls.append(f'template <typename T, int EP> __global__ void '
                          f'{name}(int, const T *, long, const T *, long, T *, '
                          f'long, const T *, long, void *, double, double, '
@@ -325,11 +358,12 @@ def main(outdir):
 
     t = list(head) + decls + ['']
     t.append('const FusedKernelEntry g_fk_table[] = {')
-    for name, m, k0, k1, nnz, desc in ftable:
-        fnd = ', '.join(f'&{name}<double, {ep}>'
-                        if ep in FUSED_EPS['double'] else 'nullptr'
-                        for ep in range(6))
-        fnf = ', '.join(f'&{name}<float, {ep}>' for ep in range(6))
+    for name, m, k0, k1, nnz, desc, fname in ftable:
+        def fns(t):
+            eps = FUSED_EPS[t] if fname == 'div' else [0]
+            return ', '.join(f'&{name}<{t}, {ep}>' if ep in eps else 'nullptr'
+                             for ep in EP_SLOTS)
+        fnd, fnf = fns('double'), fns('float')
         t.append(f'    {{{m}, {k0}, {k1}, {nnz}, lin_{name}, val_{name}, '
                  f'"{desc}", {{{fnd}}}, {{{fnf}}}}},')
     t.append('};')

@@ -524,6 +524,20 @@ class EulerElements(FluidElements):
                 xa=None if xa is None else self.scal_upts[xa], stage=stage)
         self.kernels['tdivf'] = tdivf
 
+        # Euler hexes: RK4 stage updates in the divergence epilogue (tdisf
+        # stays its own kernel); builds raise NotSuitableError where the
+        # operators have no generated kernel
+        if (self.name == 'hex' and type(self) is EulerElements and
+                not self.fluxaa):
+            def tdivf_rk4(stage, uin, nxt, r1, r0):
+                sb = self.scal_upts
+                return [self.kernels['tdisf'](uin), be.kernel(
+                    'tdivf', self.opmat(self._divop), self.opmat('M3'),
+                    self._flux_buf, self._scal_fpts, sb[nxt],
+                    self._rcpdjac_upts, xa=(sb[r1], sb[r0]),
+                    stage=f'rk4-{stage}')]
+            self.kernels['tdivf_rk4'] = tdivf_rk4
+
         # straight-sided elements, Euler: tdisf folded in as well
         if (self.affine and type(self) is EulerElements and be.fuse_flux and
                 not self.fluxaa):
@@ -544,12 +558,12 @@ class EulerElements(FluidElements):
                 uin, the next stage's input to nxt, r1 and r0 the
                 accumulator and the step's initial state."""
                 sb = self.scal_upts
-                return be.kernel(
+                return [be.kernel(
                     'tdivf_flux', self.opmat('M1 - M3*M2'),
                     self.opmat('M3'), sb[uin], self._smats_affine,
                     self._scal_fpts, sb[nxt], self._rcpdjac_upts, gamma,
-                    xa=(sb[r1], sb[r0]), stage=f'rk4-{stage}')
-            self.kernels['tdivf_flux_rk4'] = tdivf_flux_rk4
+                    xa=(sb[r1], sb[r0]), stage=f'rk4-{stage}')]
+            self.kernels['tdivf_rk4'] = tdivf_flux_rk4
 
 
 class NavierStokesElements(EulerElements):
@@ -1322,11 +1336,21 @@ class BaseSystem:
 
     @property
     def fused_rk4(self):
-        """True if RK4 stage updates can be fused into the RHS (every
-        element type on the flux-fused divergence kernel)."""
-        return (self.backend.fused and
-                all('tdivf_flux_rk4' in e.kernels
-                    for e in self.ele_map.values()))
+        """True if RK4 stage updates can be fused into the RHS: every
+        element type on the flux-fused divergence kernel (tets, prisms) or
+        on a generated fused divergence kernel (hexes), tried once."""
+        if not hasattr(self, '_fused_rk4_ok'):
+            from pyfr_amd.backend import NotSuitableError
+            ok = (self.backend.fused and len(self.ele_banks[0]) >= 4 and
+                  all('tdivf_rk4' in e.kernels
+                      for e in self.ele_map.values()))
+            try:
+                for e in self.ele_map.values() if ok else ():
+                    e.kernels['tdivf_rk4'](1, 0, 2, 1, 0)
+            except NotSuitableError:
+                ok = False
+            self._fused_rk4_ok = ok
+        return self._fused_rk4_ok
 
     def rhs_rk4_stage(self, t, stage, uin, nxt, r1, r0, dt):
         """RHS of RK4 stage `stage` at t with the stage's register updates
@@ -1336,8 +1360,8 @@ class BaseSystem:
             eles = list(self.ele_map.values())
             ks = [e.kernels['disu'](uin) for e in eles]
             ks += [i.kernels['comm_flux']() for i in self._inters]
-            ks += [e.kernels['tdivf_flux_rk4'](stage, uin, nxt, r1, r0)
-                   for e in eles]
+            ks += [k for e in eles
+                   for k in e.kernels['tdivf_rk4'](stage, uin, nxt, r1, r0)]
             g = self.backend.graph()
             g.add_all(ks)
             g.commit()
@@ -1588,9 +1612,10 @@ class RK4Integrator(BaseIntegrator):
     coefficients and operand order (``add(dt/2, r2, 1, r0)`` with r2 = k
     becomes ``add(0, r2, dt/2, r3, 1, r0)``: the same products and sum in
     the axnpby kernel), so the arithmetic is the reference's.  Where every
-    element type runs the flux-fused divergence kernel, each stage's
-    updates are done in that kernel's epilogue instead, the stage inputs
-    alternating between r2 and r3."""
+    element type runs a fused divergence kernel with RK4 epilogues (the
+    flux-fused kernel of Euler tets and prisms, the generated kernel of
+    Euler hexes), each stage's updates are done in that kernel's epilogue
+    instead, the stage inputs alternating between r2 and r3."""
     nregs = 4
     nrhs_per_step = 4
 
