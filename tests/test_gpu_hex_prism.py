@@ -221,6 +221,28 @@ def test_rk4_fused_stages(backend_factory, precision, npr, order):
         assert rel_l2(got, _cat(u, names)) < 1e-12
 
 
+@pytest.mark.parametrize('precision', ['double', 'single'])
+def test_mul2_rk4_epilogue_needs_structured_plan(backend_factory, precision):
+    """A dense two-input plan refuses the RK4 epilogues on the host,
+    before any launch, and so does the provider (NotSuitableError)."""
+    from pyfr_amd._lib import FrHipError
+    from pyfr_amd.backend import NotSuitableError
+    from tests.test_gpu_simplex import _operands, _Plan2
+    be = backend_factory(precision=precision)
+    A0, A1, b0, b1, c, rng = _operands(be, 35, 105, 60, 320, 5)
+    p = _Plan2(be, A0, A1)
+    assert p.kind == 2
+    rcp = be.const_matrix(np.ones((1, 320)), tags={'align'})
+    print(f'\n[mul2 rk4 dense] {precision}', flush=True)
+    with pytest.raises(FrHipError, match='structured'):
+        be.lib.mul2(p.plan, c.ncol, b0.data, b0.leaddim, b1.data, b1.leaddim,
+                    c.data, c.leaddim, 10, rcp.data, 0, c.data, 1e-3, 0.0, 0,
+                    be.stream)
+    a0, a1 = be.const_matrix(A0), be.const_matrix(A1)
+    with pytest.raises(NotSuitableError):
+        be.kernel('tdivf', a0, a1, b0, b1, c, rcp, xa=(c, c), stage='rk4-1')
+
+
 def test_fp32_rhs(backend_factory):
     be = backend_factory(precision='single')
     print('\n[hex-prism fp32] p=4', flush=True)
