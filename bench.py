@@ -7,7 +7,8 @@ FP64, Rusanov + LDG (beta=0, tau=0) interfaces, fixed-dt RK45 (dt=2e-4),
 Taylor-Green initial field at Mach 0.1, Re 1600.  A step is one RK45 step
 (5 RHS evaluations + 5 stage updates) with all data resident in HBM.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--config cfg1|cfg0]
+  python bench.py [--gpus 1] [--steps K] [--warmup W]
+                  [--config cfg1|cfg0|cfg3|hex64rk4]
 
 Prints one JSON line: `value` is the FP64 plain GDOF*RHS/s; a roofline
 object for the dominant kernel (algorithmic bytes per launch / mean launch
@@ -36,6 +37,9 @@ CONFIGS = {
                  dt=2e-4, workload='ns-hex64-p3-rk45-fp64'),
     'cfg0': dict(system='euler', n=8, order=2, scheme='rk4', dt=1e-3,
                  workload='euler-hex8-p2-rk4-fp64'),
+    # cfg0's scheme at cfg1's size: Euler RK4 with the fused stage updates
+    'hex64rk4': dict(system='euler', n=64, order=2, scheme='rk4', dt=1e-4,
+                     workload='euler-hex64-p2-rk4-fp64'),
     # dense-operator mixed mesh: a periodic box of 48^3 cells x 6 tets and a
     # periodic 16^3-cell x 2-prism slab, p=4, FP32, 8 random-phase modes
     'cfg3': dict(system='euler', ntet=48, npri=16, order=4, scheme='rk4',
