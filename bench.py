@@ -54,7 +54,7 @@ FUSED = True
 
 
 def make_cfg(c, precision='double'):
-    from tests.fr_cases import make_cfg as mk, make_simplex_cfg, mode_ics
+    from pyfr_amd.cases import make_cfg as mk, make_simplex_cfg, mode_ics
     if 'ntet' in c:
         cfg = make_simplex_cfg(c['order'], precision, c['scheme'], c['dt'],
                                mode_ics(seed=0, nmodes=8, amp=0.05))
@@ -245,18 +245,27 @@ def time_kernels(be, kerns, reps):
 
 def cpu_baseline(c, budget_s=12.0):
     """The oracle's C/OpenMP restatement of the reference OpenMP backend's
-    RHS (oracle/crhs.c, blocked 8-wide SoA layout, sparse operators) on a
-    bounded sample of the same workload: same system, order and field on a
-    24^3 box (12^3 for cfg0's order), all host cores this process may use."""
-    from oracle.crhs import CRhs
-    from oracle.rhs import OracleHexSystem
-    from tests.fr_cases import oracle_consts
+    RHS (oracle/crhs.c, blocked 8-wide SoA layout, sparse operators),
+    built with that backend's flags (-O3 -ffast-math, the host's widest ISA:
+    pyfr/backends/openmp/compiler.py:76-96), on all host cores this process
+    may use, on a bounded sample of the same workload: same system, order
+    and field on a 24^3 box (12^3 for cfg0's order).  The full 64^3 box is
+    not used because the NumPy oracle that builds the operators, metric
+    terms and interface lists takes minutes and ~18 GB there; the per-DOF
+    rate is size-independent once the working set (~1.5 GB at 24^3) is far
+    beyond the host caches."""
     ncores = len(os.sched_getaffinity(0))
     os.environ['OMP_NUM_THREADS'] = str(ncores)
+    os.environ.setdefault('OMP_PROC_BIND', 'close')
+    from oracle.crhs import CRhs, cpu_model, host_isa
+    from oracle.rhs import OracleHexSystem
+    from pyfr_amd.cases import NS_CONSTS
     n = 24 if c['order'] >= 3 else 12
     orc = OracleHexSystem(n, c['order'], c['system'],
-                          constants=oracle_consts(c['system']))
-    crhs = CRhs(orc)
+                          constants=({'gamma': 1.4} if c['system'] == 'euler'
+                                     else {**NS_CONSTS, 'ldg-beta': 0.0,
+                                           'ldg-tau': 0.0}))
+    crhs = CRhs(orc, fast=True)
     crhs.set_u(orc.ics())
     crhs.rhs_inplace()
     t0, nrhs = time.perf_counter(), 0
@@ -264,12 +273,15 @@ def cpu_baseline(c, budget_s=12.0):
         crhs.rhs_inplace()
         nrhs += 1
     dt = time.perf_counter() - t0
+    isa = {'v3': 'x86-64-v3 (AVX2)', 'v4': 'x86-64-v4 (AVX-512)'}[host_isa()]
     return {'value': orc.ndofs*nrhs/dt/1e9, 'unit': 'GDOF*RHS/s',
             'cores': ncores, 'kind': 'port',
             'sample': f'oracle C/OpenMP FP64 {c["system"]} RHS (restatement '
                       f'of the reference OpenMP kernels), {n}^3 hex '
                       f'p={c["order"]} ({orc.ndofs} DOF), {nrhs} RHS in '
-                      f'{dt:.1f} s on {ncores} threads'}
+                      f'{dt:.1f} s on {ncores} threads (OMP_PROC_BIND='
+                      f'{os.environ["OMP_PROC_BIND"]}); gcc -O3 -ffast-math '
+                      f'-march={isa} -fopenmp; CPU {cpu_model()}'}
 
 
 def serving_report(nreq, seed=0, rate=50.0):
@@ -309,49 +321,63 @@ def serving_report(nreq, seed=0, rate=50.0):
 
 
 def accuracy_report(c, checkpoints=(100, 500)):
-    """Per-field relative L2 and max errors of the mixed precision form
-    against the FP64 run from the same initial field, plus FP64
-    conservation sums of both."""
+    """Per-field relative L2 and max errors of the precision form against
+    the FP64 run from the same initial field at the checkpoints, plus the
+    FP64 conservation sums: RK45 (the workload's scheme) in the base-state
+    form, the full-state form and with BF16 operator inputs, and RK4 in the
+    base-state form.  Momenta are also normalised by the momentum vector's
+    L2 norm (the w-momentum of the Taylor-Green field starts at zero)."""
     from pyfr_amd.backend import AMDBackend
     from pyfr_amd.mesh import BoxMesh
     from pyfr_amd.solvers import get_integrator
-    res = {}
-    for prec in ('double', 'mixed', 'mixed-bf16'):
-        cfg = make_cfg(c, prec.split('-')[0])
-        if prec == 'mixed-bf16':
-            cfg.set('backend-hip', 'operator-inputs', 'bf16')
-        be = AMDBackend(cfg)
-        integ = get_integrator(be, BoxMesh(c['n']), cfg)
-        res[prec] = {'sums0': integ.system.conservation_sums(0).tolist()}
-        done = 0
-        t0 = time.perf_counter()
-        for cp in checkpoints:
-            integ.run_steps(cp - done, c['dt'])
-            done = cp
-            res[prec][cp] = integ.soln[0]
-            res[prec][f'sums{cp}'] = integ.system.conservation_sums(
-                integ.idxcurr).tolist()
-            print(f'# {prec}: step {cp} at {time.perf_counter() - t0:.1f} s',
-                  file=sys.stderr, flush=True)
-        del integ, be
+    runs = {'rk45': ('double', 'mixed', 'mixed-fullstate', 'mixed-bf16'),
+            'rk4': ('double', 'mixed')}
     names = ['rho', 'rhou', 'rhov', 'rhow', 'E']
     out = {'workload': c['workload'].replace('-fp64', '-mixed-vs-fp64'),
            'checkpoints': {}}
-    for form in ('mixed', 'mixed-bf16'):
-        for cp in checkpoints:
-            a, b = res[form][cp], res['double'][cp]
-            d = {}
-            for v, nm in enumerate(names):
-                da, db = a[:, v], b[:, v]
-                d[nm] = {'rel_l2': float(np.linalg.norm(da - db) /
-                                         np.linalg.norm(db)),
-                         'max_abs': float(np.abs(da - db).max()),
-                         'ref_max_abs': float(np.abs(db).max())}
-            d['state_rel_l2'] = float(np.linalg.norm(a - b)/np.linalg.norm(b))
-            d['sums_fp64'] = res['double'][f'sums{cp}']
-            d['sums_' + form] = res[form][f'sums{cp}']
-            out['checkpoints'][f'{form}@{cp}'] = d
-    out['sums_initial'] = res['double']['sums0']
+    for scheme, forms in runs.items():
+        res = {}
+        for form in forms:
+            cfg = make_cfg(dict(c, scheme=scheme), form.split('-')[0])
+            if form == 'mixed-bf16':
+                cfg.set('backend-hip', 'operator-inputs', 'bf16')
+            if form == 'mixed-fullstate':
+                cfg.set('backend-hip', 'base-state', 'off')
+            be = AMDBackend(cfg)
+            integ = get_integrator(be, BoxMesh(c['n']), cfg)
+            res[form] = {'sums0': integ.system.conservation_sums(0).tolist()}
+            done = 0
+            t0 = time.perf_counter()
+            for cp in checkpoints:
+                integ.run_steps(cp - done, c['dt'])
+                done = cp
+                res[form][cp] = integ.soln[0]
+                res[form][f'sums{cp}'] = integ.system.conservation_sums(
+                    integ.idxcurr).tolist()
+                print(f'# {scheme} {form}: step {cp} at '
+                      f'{time.perf_counter() - t0:.1f} s', file=sys.stderr,
+                      flush=True)
+            del integ, be
+        for form in forms[1:]:
+            for cp in checkpoints:
+                a, b = res[form][cp], res['double'][cp]
+                mom = float(np.linalg.norm(b[:, 1:4]))
+                d = {}
+                for v, nm in enumerate(names):
+                    da, db = a[:, v], b[:, v]
+                    d[nm] = {'rel_l2': float(np.linalg.norm(da - db) /
+                                             np.linalg.norm(db)),
+                             'max_abs': float(np.abs(da - db).max()),
+                             'ref_max_abs': float(np.abs(db).max())}
+                    if 1 <= v <= 3:
+                        d[nm]['rel_l2_mom'] = float(
+                            np.linalg.norm(da - db)/mom)
+                d['state_rel_l2'] = float(np.linalg.norm(a - b) /
+                                          np.linalg.norm(b))
+                d['sums_fp64'] = res['double'][f'sums{cp}']
+                d['sums_form'] = res[form][f'sums{cp}']
+                out['checkpoints'][f'{scheme}:{form}@{cp}'] = d
+        out[f'sums_initial_{scheme}'] = res['double']['sums0']
     print(json.dumps(out))
 
 

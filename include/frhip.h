@@ -47,6 +47,10 @@ int frhip_host_malloc(void **ptr, size_t nbytes);
 int frhip_host_free(void *ptr);
 int frhip_memcpy_htod(void *dst, const void *src, size_t nbytes, void *stream);
 int frhip_memcpy_dtoh(void *dst, const void *src, size_t nbytes, void *stream);
+/* Device to page-locked host memory, not waited on (graph-capturable); the
+ * data is there once the stream has been synchronised. */
+int frhip_memcpy_dtoh_async(void *dst, const void *src, size_t nbytes,
+                            void *stream);
 int frhip_memcpy_dtod(void *dst, const void *src, size_t nbytes, void *stream);
 int frhip_memset(void *ptr, int value, size_t nbytes, void *stream);
 /* 2-D copies (hipMemcpy2DAsync): height rows of width bytes between pitched
@@ -112,7 +116,11 @@ int frhip_mul_plan_tune(void *plan, int ncol, const void *b, long ldb,
  *      negdivconf then RK4 stage ep - 9 with four registers, as for
  *      frhip_tdivf_flux_euler (xa = device pair {r1, r0}, dta = dt, c the
  *      next stage's input; structured plans only, the updates of
- *      pyfr/integrators/explicit/steppers.py:67-108).
+ *      pyfr/integrators/explicit/steppers.py:67-108); 14..17 = 10..13 for
+ *      the precision form (structured FP32 plans; no counterpart in the
+ *      reference): r1 and r0 FP64, the updates formed in FP64 in the same
+ *      order from the FP32 stage RHS, c the FP32 next stage input, and 17
+ *      also stores the new solution rounded to FP32 into c.
  * They fuse kernels the reference runs back to back on the same element
  * data: tgradpcoru_upts + tgradcoru_upts (pyfr/solvers/baseadvecdiff/
  * elements.py:47-55) and tdivtpcorf + tdivtconf + negdivconf
@@ -185,11 +193,21 @@ int frhip_tdivf_flux_euler(void *plan, int ncol, const void *u, long ldu,
 /* Euler tflux: pyfr/solvers/euler/kernels/tflux.mako:8-30 with flux.mako:3-26.
  * linear = FRHIP_GEO_HEX_LINEAR: S from the 8 vertices (upts: (npts, 3)
  * row-major, smats.mako:13-25); FRHIP_GEO_CURVED: stored smats;
- * FRHIP_GEO_AFFINE: per-element S of straight-sided tets/prisms. */
+ * FRHIP_GEO_AFFINE: per-element S of straight-sided tets/prisms.
+ *
+ * base (here and in frhip_tflux_ns, frhip_intcflux_euler,
+ * frhip_intcflux_ns): NULL for the reference's kernels, or {rho0, E0} of a
+ * constant state u0 = (rho0, 0, 0, 0, E0) at rest for the base-state form
+ * (no counterpart in the reference): u then holds u - u0 and the fluxes
+ * returned are f(u) - f(u0), f(u0) = (0, p0 I, 0), p0 = (gamma - 1) E0.  On
+ * free-stream-preserving elements f(u0) has zero discrete divergence, so the
+ * right-hand side is the same while E0 and p0 no longer take FP32 digits
+ * (the precision form).  The interface kernels take it with the Rusanov
+ * solver only. */
 int frhip_tflux_euler(int dtype, int linear, int npts, int neles,
                       const void *u, long ldu, void *f, long ldf,
                       const void *geo, long ldgeo, const void *upts,
-                      double gamma, void *stream);
+                      double gamma, const double *base, void *stream);
 
 /* NS fused tflux: pyfr/solvers/navstokes/kernels/tflux.mako:13-50 with
  * transform_grad.mako:3-10 and navstokes/kernels/flux.mako:48-105.  gradu is
@@ -203,7 +221,7 @@ int frhip_tflux_ns(int dtype, int linear, int npts, int neles,
                    const void *u, long ldu, void *f, long ldf, void *gradu,
                    long ldg, const void *geo, long ldgeo, const void *upts,
                    const void *rcpdjac, long ldr, const double *consts,
-                   const void *mu_e, void *stream);
+                   const void *mu_e, const double *base, void *stream);
 
 /* gradcoru: the corrected reference gradient (3 rows per point) made
  * physical in place, g <- |J|^-1 S^T g;
@@ -236,7 +254,7 @@ int frhip_negdivconf(int dtype, int npts, int neles, void *t, long ldt,
 int frhip_intcflux_euler(int dtype, int rsolver, int n, void *base,
                          const int32_t *lmap, const int32_t *rmap,
                          const void *nl, long ldn, double gamma,
-                         void *stream);
+                         const double *bstate, void *stream);
 
 /* NS intconu: pyfr/solvers/navstokes/kernels/intconu.mako:4-19. */
 int frhip_intconu(int dtype, int n, const void *ubase, const int32_t *ulin,
@@ -258,7 +276,7 @@ int frhip_intcflux_ns(int dtype, int rsolver, int n, void *ubase,
                       const int32_t *grmap, long gstride,
                       const int32_t *glstr, const int32_t *grstr,
                       const void *nl, long ldn, const double *consts,
-                      const void *mu_p, void *stream);
+                      const void *mu_p, const double *bstate, void *stream);
 
 /* ---- boundary interfaces, 1-D over n boundary flux points ----
  * Types: the `type` keys of [soln-bcs-*] (pyfr/solvers/euler/inters.py:57-102,
@@ -382,15 +400,79 @@ int frhip_rkvdh2(int dtype, int nrow, long ncol, long ld, void *r1, void *r2,
 int frhip_rkvdh2_mixed(int nrow, long ncol, long ld, void *acc, void *kin,
                        double dta, double dtb, int last, void *stream);
 
-/* Reductions over an (nrow, nvar, neles) array: per variable sum of w*x
- * (w: (nrow, neles) weights, may be NULL), max |x|, or sum of w*x*x; FP64
- * accumulation, deterministic two-pass order.  out: device buffer of nvar
- * doubles.  Replaces HIPBlasExtKernels.reduction
- * (pyfr/backends/hip/blasext.py:76-136). op: 0 = sum, 1 = max-abs,
- * 2 = weighted sum of squares.  Allocates its scratch on first use. */
-int frhip_reduce(int dtype, int op, int nrow, int nvar, int neles, long ld,
-                 const void *x, const void *w, long ldw, double *out,
-                 void *stream);
+/* ---- reduction provider: pyfr/backends/base/blasext.py:21-46
+ *      (reduction(rop, exprs, vvars, svars, pvars) + retval), replacing
+ *      HIPBlasExtKernels._reduction (pyfr/backends/hip/blasext.py:76-136)
+ *      and its template pyfr/backends/hip/kernels/reduction.mako ---- */
+enum { FRHIP_RED_SUM = 0, FRHIP_RED_MAX = 1 };
+/* Partial blocks per variable: a workspace holds nexprs*nvar*FRHIP_RED_NBLK
+ * doubles at most (frhip_reduction_workspace gives the exact count). */
+enum { FRHIP_RED_NBLK = 512 };
+
+/* Compile-once expression program: nexprs (<= 8) expressions, each a stack
+ * program of (op, arg) pairs with the FRHIP_BCX_* opcodes, over nslots
+ * (<= 16) variables (FRHIP_BCX_PUSHV arg = variable index).  Validated here
+ * (opcodes, operand counts, depth, index ranges). */
+int frhip_expr_program_create(void **prog, int nexprs, int nslots,
+                              const int32_t *code, const int32_t *offs,
+                              const double *pool, int npool);
+int frhip_expr_program_destroy(void *prog);
+
+/* out[x*nvar + v] = rop over rows r < nrow and elements e < neles of
+ * expression x evaluated (in double) at point (r, v, e) of nmat (nrow, nvar,
+ * neles) arrays mats[m] (leading dimension lds[m]).  The expression
+ * variables are the arrays' values there (an array whose bit is set in
+ * bcast is (nrow, neles) with one value per (row, element), shared by every
+ * variable: an extension, e.g. quadrature weights), then pv[k*nvar + v] for
+ * k < npv (the reference's pvars, device memory), then the nsv (<= 8)
+ * scalars sv (host memory; the reference's bound svars).  Accumulation is in
+ * double in a fixed two-pass order; the sum starts at 0, the max at
+ * -DBL_MAX.  work: device workspace of nwork doubles, at least what
+ * frhip_reduction_workspace reports; out: device, nexprs*nvar doubles.
+ * Allocates nothing; graph-capturable.  The reference reduces the result
+ * over variables on the host (retval); the caller does the same. */
+int frhip_reduction(void *prog, int dtype, int rop, int nrow, int neles,
+                    int nvar, int nmat, void *const *mats, const long *lds,
+                    unsigned bcast, const double *pv, int npv,
+                    const double *sv, int nsv, double *work, long nwork,
+                    double *out, void *stream);
+int frhip_reduction_workspace(void *prog, int nrow, int neles, int nvar,
+                              long *nwork);
+
+/* axnpby with per-variable scales: pyfr/backends/base/blasext.py:9-19
+ * (in_scale, in_scale_idxs, out_scale) as rendered by
+ * pyfr/backends/hip/kernels/axnpby.mako:4-44 and
+ * pyfr/backends/base/makoutil.py:40-54: operand l of variable k is scaled
+ * by insc[k] if bit l of inmask is set, the sum by outsc[k] (if outsc is not
+ * NULL); the a0 == 0 / a0 == 1 cases as in frhip_axnpby.  insc, outsc: host
+ * arrays of nvar (<= 16) values.  ncol = nvar*neles_pad. */
+int frhip_axnpby_scaled(int dtype, int nrow, long ncol, long ld, int nv,
+                        int nvar, void *const *xs, const double *a,
+                        unsigned inmask, const double *insc,
+                        const double *outsc, void *stream);
+
+/* RK4 stage `stage` (1..4) of the precision form (no counterpart kernel in
+ * the reference, which runs its RK4 axnpby updates in one precision,
+ * pyfr/integrators/explicit/steppers.py:67-108): r1, r0 FP64, k the FP32
+ * stage RHS, nxt the FP32 next stage input, with the arithmetic of the
+ * frhip_mul2 epilogues 14..17:
+ *   1: r1 = k;                      nxt = r0 + dt/2 k
+ *   2: r1 = dt/6 r1 + r0 + dt/3 k;  nxt = dt/2 k + r0
+ *   3: r1 = r1 + dt/3 k;            nxt = dt k + r0
+ *   4: r1 = r1 + dt/6 k;            nxt = r1 (the next step's input).
+ * Same (nrow, ld) in elements for all four arrays. */
+int frhip_rk4_mixed(int nrow, long ncol, long ld, int stage, void *r1,
+                    const void *r0, const void *k, void *nxt, double dt,
+                    void *stream);
+
+/* rkvdh2 stage with the embedded error estimate:
+ * pyfr/integrators/explicit/kernels/rkvdh2.mako:4-31 with errest (rold,
+ * rerr registers; e = b - bhat of stage `stage`,
+ * pyfr/integrators/explicit/steppers.py:127-128). */
+int frhip_rkvdh2_errest(int dtype, int nrow, long ncol, long ld, void *r1,
+                        void *r2, void *rold, void *rerr, double dt,
+                        double a, double b, double e, int stage, int last,
+                        void *stream);
 
 /* Copy/convert between precisions (same layout); n elements. */
 int frhip_convert(int dst_dtype, void *dst, int src_dtype, const void *src,

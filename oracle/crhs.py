@@ -16,10 +16,42 @@ LIB = os.path.join(HERE, '_build', 'libcrhs.so')
 _P, _I, _D = ct.c_void_p, ct.c_int, ct.c_double
 
 
-def _load():
-    if not os.path.exists(LIB):
-        raise RuntimeError(f'{LIB} is not built (make -C oracle)')
-    lib = ct.CDLL(LIB)
+def host_isa():
+    """'v4' if the host CPU has AVX-512 (F, BW, DQ, VL), else 'v3'."""
+    try:
+        with open('/proc/cpuinfo') as f:
+            flags = next((ln.split(':', 1)[1].split() for ln in f
+                          if ln.startswith('flags')), [])
+    except OSError:
+        flags = []
+    need = {'avx512f', 'avx512bw', 'avx512dq', 'avx512vl'}
+    return 'v4' if need <= set(flags) else 'v3'
+
+
+def cpu_model():
+    try:
+        with open('/proc/cpuinfo') as f:
+            for ln in f:
+                if ln.startswith('model name'):
+                    return ln.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return 'unknown'
+
+
+def lib_path(fast=False):
+    """The parity build, or (fast) the -ffast-math timing build for the
+    host's widest ISA."""
+    if not fast:
+        return LIB
+    return os.path.join(HERE, '_build', f'libcrhs_fast_{host_isa()}.so')
+
+
+def _load(fast=False):
+    path = lib_path(fast)
+    if not os.path.exists(path):
+        raise RuntimeError(f'{path} is not built (make -C oracle)')
+    lib = ct.CDLL(path)
     lib.crhs_create.restype = _P
     lib.crhs_create.argtypes = [_I, _I, _I, _I] + [_P]*7 + [_I] + [_P]*6
     lib.crhs_destroy.argtypes = [_P]
@@ -30,8 +62,10 @@ def _load():
 
 
 class CRhs:
-    def __init__(self, orc):
-        self.lib = lib = _load()
+    def __init__(self, orc, fast=False):
+        """fast: the timing build (-ffast-math, host ISA) instead of the
+        parity build."""
+        self.lib = lib = _load(fast)
         self.orc = orc
         ns = orc.system != 'euler'
         c = orc.c

@@ -27,6 +27,7 @@ _FUNCS = {
     'frhip_host_free': [_P],
     'frhip_memcpy_htod': [_P, _P, _S, _P],
     'frhip_memcpy_dtoh': [_P, _P, _S, _P],
+    'frhip_memcpy_dtoh_async': [_P, _P, _S, _P],
     'frhip_memcpy_dtod': [_P, _P, _S, _P],
     'frhip_memset': [_P, _I, _S, _P],
     'frhip_memcpy2d': [_P, _S, _P, _S, _S, _S, _I, _P],
@@ -65,17 +66,18 @@ _FUNCS = {
     'frhip_tdivf_flux_euler': [_P, _I, _P, _L, _P, _L, _P, _L, _P, _L, _I,
                                _P, _L, _P, _D, _D, _I, _D, _P],
     'frhip_tflux_euler': [_I, _I, _I, _I, _P, _L, _P, _L, _P, _L, _P, _D,
-                          _P],
+                          _P, _P],
     'frhip_tflux_ns': [_I, _I, _I, _I, _P, _L, _P, _L, _P, _L, _P, _L, _P,
-                       _P, _L, _P, _P, _P],
+                       _P, _L, _P, _P, _P, _P],
     'frhip_gradcoru': [_I, _I, _I, _I, _P, _L, _P, _L, _P, _P, _L, _P],
     'frhip_tflux_ns_grad': [_I, _I, _I, _I, _P, _L, _P, _L, _P, _L, _P, _P,
                             _P, _P],
     'frhip_negdivconf': [_I, _I, _I, _P, _L, _P, _L, _P],
-    'frhip_intcflux_euler': [_I, _I, _I, _P, _PI32, _PI32, _P, _L, _D, _P],
+    'frhip_intcflux_euler': [_I, _I, _I, _P, _PI32, _PI32, _P, _L, _D, _P,
+                             _P],
     'frhip_intconu': [_I, _I, _P, _PI32, _PI32, _P, _PI32, _PI32, _D, _P],
     'frhip_intcflux_ns': [_I, _I, _I, _P, _PI32, _PI32, _P, _PI32, _PI32,
-                          _L, _PI32, _PI32, _P, _L, _P, _P, _P],
+                          _L, _PI32, _PI32, _P, _L, _P, _P, _P, _P],
     'frhip_bccflux_euler': [_I, _I, _I, _I, _P, _PI32, _P, _L, _P, _L, _P,
                             _P],
     'frhip_bcconu': [_I, _I, _I, _P, _PI32, _P, _PI32, _P, _L, _P, _L, _P,
@@ -91,8 +93,17 @@ _FUNCS = {
     'frhip_axnpby_pool': [_I, _I, _L, _L, _I, _P, _P, ct.c_uint, _P, _I, _P],
     'frhip_rkvdh2': [_I, _I, _L, _L, _P, _P, _D, _D, _I, _P],
     'frhip_rkvdh2_mixed': [_I, _L, _L, _P, _P, _D, _D, _I, _P],
-    'frhip_reduce': [_I, _I, _I, _I, _I, _L, _P, _P, _L, _P, _P],
     'frhip_convert': [_I, _P, _I, _P, _L, _P],
+    'frhip_expr_program_create': [ct.POINTER(_P), _I, _I, _P, _P, _P, _I],
+    'frhip_expr_program_destroy': [_P],
+    'frhip_reduction': [_P, _I, _I, _I, _I, _I, _I, _P, _P, ct.c_uint, _P,
+                        _I, _P, _I, _P, _L, _P, _P],
+    'frhip_reduction_workspace': [_P, _I, _I, _I, ct.POINTER(_L)],
+    'frhip_axnpby_scaled': [_I, _I, _L, _L, _I, _I, _P, _P, ct.c_uint, _P,
+                            _P, _P],
+    'frhip_rk4_mixed': [_I, _L, _L, _I, _P, _P, _P, _P, _D, _P],
+    'frhip_rkvdh2_errest': [_I, _I, _L, _L, _P, _P, _P, _P, _D, _D, _D, _D,
+                            _I, _I, _P],
 }
 
 

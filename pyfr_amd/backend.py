@@ -319,12 +319,12 @@ class Kernel:
     """A launchable unit (``provider.py:10-66``): ``run(stream)`` launches,
     ``bind(**kw)`` updates runtime arguments."""
     dt = float('nan')
+    retval = None
 
     def __init__(self, fn, rtargs=None, mats=(), views=(), name=''):
         self._fn = fn
         self.rtargs = dict(rtargs or {})
         self.mats, self.views, self.name = mats, views, name
-        self.retval = None
         self.version = 0
 
     @property
@@ -449,6 +449,29 @@ def _ptr(m):
     return None if m is None else m.data
 
 
+# fused epilogue of each stage-update name
+_STAGE_EP = {None: 1, 'rk45': 2, 'rk45-mixed': 3, 'rk4-1': 10, 'rk4-2': 11,
+             'rk4-3': 12, 'rk4-4': 13, 'rk4m-1': 14, 'rk4m-2': 15,
+             'rk4m-3': 16, 'rk4m-4': 17}
+
+
+def _check_xa(xa, out, ep):
+    """The stage-update banks of a fused epilogue index with out's leading
+    dimension, so each must match out's shape, leading dimension and dtype
+    (FP64 for the mixed-precision epilogues)."""
+    if ep < 2:
+        return
+    mats = (xa,) if ep in (2, 3) else tuple(xa)
+    if ep >= 10 and len(mats) != 2:
+        raise ValueError('RK4 stage epilogues take the banks (r1, r0)')
+    want = np.float64 if ep in (3, 14, 15, 16, 17) else out.dtype
+    for m in mats:
+        if (m is None or m.nrow != out.nrow or m.ncol != out.ncol or
+                m.leaddim != out.leaddim or m.dtype != want):
+            raise ValueError('Stage-update banks must match the output '
+                             'shape, leading dimension and precision')
+
+
 class MulProvider:
     """``mul``: structured (sparsity-pattern matched) or dense operator
     kernels; replaces the GiMMiK/rocBLAS providers."""
@@ -550,6 +573,7 @@ class MulProvider:
         if kind == 2:
             self._tune(lib.mul2_plan_tune, plan, ncol, b0.data, b0.leaddim,
                        b1.data, b1.leaddim)
+        _check_xa(xa, out, ep)
         if ep >= 10:
             # RK4 stages: xa = (r1, r0), passed as a device pointer pair
             xap, xam = self._ptr_pair(*xa), tuple(xa)
@@ -580,9 +604,8 @@ class MulProvider:
         stage k's updates (stage='rk4-k': xa = (r1, r0), out = the next
         stage's input, dta = dt; structured plans only).  The stage
         coefficients are bound at run time: bind(dta=, dtb=, last=)."""
-        ep = {None: 1, 'rk45': 2, 'rk45-mixed': 3, 'rk4-1': 10, 'rk4-2': 11,
-              'rk4-3': 12, 'rk4-4': 13}[stage]
-        return self._fused(a0, a1, b0, b1, out, ep, rcpdjac, xa)
+        return self._fused(a0, a1, b0, b1, out, _STAGE_EP[stage], rcpdjac,
+                           xa)
 
     def tuning_report(self):
         """The variant the tuner chose for each dense plan and shape
@@ -607,6 +630,10 @@ class MulProvider:
         """tdisf + tdivtpcorf + tdivtconf + negdivconf (+ rkvdh2 stage) for
         straight-sided elements (Euler): the flux of u is formed in the
         kernel from one S per element (smats (9, neles))."""
+        ep = _STAGE_EP[stage]
+        if ep >= 14:
+            raise NotSuitableError('the precision form\'s RK4 epilogues '
+                                   'are built for the hex kernels')
         for a in (a0, a1):
             if 'const' not in a.tags:
                 raise NotSuitableError('tdivf_flux requires const operators')
@@ -637,8 +664,7 @@ class MulProvider:
                                 smats.leaddim, fn.data, fn.leaddim, gamma, 5,
                                 ct.byref(v))
             self._tuned[(plan, ncol)] = v.value
-        ep = {None: 1, 'rk45': 2, 'rk45-mixed': 3, 'rk4-1': 10, 'rk4-2': 11,
-              'rk4-3': 12, 'rk4-4': 13}[stage]
+        _check_xa(xa, out, ep)
         if ep >= 10:
             # RK4 stages: xa = (r1, r0), passed as a device pointer pair
             xap = self._ptr_pair(*xa)
@@ -665,17 +691,15 @@ class MulProvider:
         if key not in self._pairs:
             lib = self.backend.lib
             h = np.array(key, dtype=np.uint64)
-            p = ct.c_void_p()
-            lib.malloc(ct.byref(p), h.nbytes)
+            # tracked by the backend, which frees it at teardown
+            p = self.backend._malloc_impl(h.nbytes)
             lib.memcpy_htod(p, h.ctypes.data, h.nbytes, None)
             lib.stream_sync(None)
-            self._pairs[key] = (p.value, r1, r0)
+            self._pairs[key] = (p, r1, r0)
         return self._pairs[key][0]
 
     def __del__(self):
         try:
-            for p, _, _ in getattr(self, '_pairs', {}).values():
-                self.backend.lib.free(p)
             for plan, _, _ in self._plans.values():
                 self.backend.lib.mul_plan_destroy(plan)
             for plan, _, _, _ in self._plans2.values():
@@ -687,14 +711,14 @@ class MulProvider:
 
 
 class BlasExtProvider:
-    """``axnpby``, ``copy``, ``zero`` (``hip/blasext.py:8-74``)."""
+    """``axnpby`` (with the scaled form), ``copy``, ``zero`` and
+    ``reduction`` (``pyfr/backends/base/blasext.py:9-46``,
+    ``pyfr/backends/hip/blasext.py:8-136``)."""
 
     def __init__(self, backend):
         self.backend = backend
 
     def axnpby(self, *arr, in_scale=(), in_scale_idxs=(), out_scale=()):
-        if in_scale or out_scale:
-            raise NotSuitableError('scaled axnpby is not supported')
         if any(arr[0].traits != x.traits for x in arr[1:]):
             raise ValueError('Incompatible matrix types')
         lib, be = self.backend.lib, self.backend
@@ -703,6 +727,23 @@ class BlasExtProvider:
         dt = _dtcode(arr[0].dtype)
         ptrs = (ct.c_void_p*nv)(*[x.data for x in arr])
         coefs = (ct.c_double*nv)()
+        scaled = bool(in_scale or out_scale or in_scale_idxs)
+        if scaled:
+            # per-variable scales (axnpby.mako with makoutil.axnpby_expr)
+            if len(arr[0].ioshape) != 3:
+                raise ValueError('Scaled axnpby needs (nrow, nvar, neles) '
+                                 'matrices')
+            nvar = arr[0].ioshape[1]
+            if in_scale_idxs and not in_scale:
+                raise ValueError('in_scale_idxs without in_scale')
+            for sc in (in_scale, out_scale):
+                if sc and len(sc) != nvar:
+                    raise ValueError('One scale per variable is required')
+            if any(not 0 <= i < nv for i in in_scale_idxs):
+                raise ValueError('Invalid in_scale_idxs')
+            inmask = sum(1 << i for i in set(in_scale_idxs))
+            insc = (ct.c_double*nvar)(*in_scale) if in_scale else None
+            outsc = (ct.c_double*nvar)(*out_scale) if out_scale else None
 
         class AxnpbyKernel(Kernel):
             def bind(self, *consts):
@@ -713,9 +754,108 @@ class BlasExtProvider:
                     self.version += 1
 
             def run(self, stream):
-                lib.axnpby(dt, nrow, ncol, ld, nv, ptrs, coefs, stream)
+                if scaled:
+                    lib.axnpby_scaled(dt, nrow, ncol, ld, nv, nvar, ptrs,
+                                      coefs, inmask, insc, outsc, stream)
+                else:
+                    lib.axnpby(dt, nrow, ncol, ld, nv, ptrs, coefs, stream)
 
         return AxnpbyKernel(None, mats=arr, name='axnpby')
+
+    def reduction(self, rop, exprs, vvars, svars=[], pvars={}):
+        """``reduction(rop, exprs, vvars, svars, pvars)``
+        (``pyfr/backends/base/blasext.py:21-46``): each expression of the
+        ``vvars`` arrays' values, the per-variable constants ``pvars`` and
+        the bound scalars ``svars`` is reduced (``'sum'`` or ``'max'``) over
+        every point and variable; ``retval`` after the kernel has run (and
+        the stream has been waited on) holds one value per expression.
+        Accumulation is in FP64 for every precision.  Extension: an array of
+        ioshape (nrow, neles) or (nrow, 1, neles) is broadcast over the
+        variables (e.g. quadrature weights), and ``result`` holds the
+        (nexprs, nvar) values before the reduction over variables.  All
+        device buffers are made here, so ``run`` allocates nothing."""
+        from pyfr_amd.bcexpr import compile_exprs
+        be, lib = self.backend, self.backend.lib
+        if rop not in ('sum', 'max'):
+            raise ValueError('Reduction op must be sum or max')
+        if not vvars:
+            raise ValueError('A reduction needs at least one array')
+        names = list(vvars)
+        mats = [vvars[n] for n in names]
+        full = [m for m in mats if len(m.ioshape) == 3 and m.ioshape[1] > 1]
+        ref = full[0] if full else mats[0]
+        if len(ref.ioshape) == 3:
+            nrow, nvar, neles = ref.ioshape
+        else:
+            (nrow, neles), nvar = ref.ioshape, 1
+        bcast = 0
+        for i, m in enumerate(mats):
+            if m.dtype != ref.dtype:
+                raise ValueError('Incompatible matrix types')
+            if m is ref or m.traits == ref.traits:
+                continue
+            shp = m.ioshape
+            if shp in ((nrow, neles), (nrow, 1, neles)) and nvar > 1:
+                bcast |= 1 << i
+            else:
+                raise ValueError('Incompatible matrix types')
+        pnames = list(pvars)
+        parr = np.array([np.broadcast_to(np.asarray(pvars[n], dtype=float),
+                                         (nvar,)) for n in pnames],
+                        dtype=np.float64).reshape(len(pnames), nvar)
+        slots = names + pnames + list(svars)
+        if len(set(slots)) != len(slots):
+            raise ValueError('Duplicate variable names')
+        code, offs, pool = compile_exprs(
+            list(exprs), {}, varmap={n: i for i, n in enumerate(slots)})
+        prog = ct.c_void_p()
+        lib.expr_program_create(ct.byref(prog), len(exprs), len(slots),
+                                code.ctypes.data, offs.ctypes.data,
+                                pool.ctypes.data, len(pool))
+        be._programs.append(prog.value)
+        nw = ct.c_long()
+        lib.reduction_workspace(prog, nrow, neles, nvar, ct.byref(nw))
+        nout = len(exprs)*nvar
+        work = be._malloc_impl(8*max(nw.value, 1))
+        out = be._malloc_impl(8*nout)
+        pv = None
+        if pnames:
+            pv = be._malloc_impl(parr.nbytes)
+            lib.memcpy_htod(pv, parr.ctypes.data, parr.nbytes, be.stream)
+            be.wait()
+        host = be._host_array(nout)
+        dtc = _dtcode(ref.dtype)
+        mptr = (ct.c_void_p*len(mats))(*[m.data for m in mats])
+        lds = (ct.c_long*len(mats))(*[m.leaddim for m in mats])
+        nsv = len(svars)
+        sv = (ct.c_double*max(nsv, 1))()
+        ropc = 0 if rop == 'sum' else 1
+        nexprs = len(exprs)
+        reducer = np.max if rop == 'max' else np.sum
+
+        class ReductionKernel(Kernel):
+            @property
+            def result(self):
+                return host.reshape(nexprs, nvar).copy()
+
+            @property
+            def retval(self):
+                return reducer(host.reshape(nexprs, nvar), axis=1)
+
+            def bind(self, *consts):
+                if len(consts) != nsv:
+                    raise ValueError('reduction expects one value per svar')
+                if tuple(sv[:nsv]) != tuple(consts):
+                    sv[:nsv] = consts
+                    self.version += 1
+
+            def run(self, stream):
+                lib.reduction(prog, dtc, ropc, nrow, neles, nvar, len(mats),
+                              mptr, lds, bcast, pv, len(pnames), sv, nsv,
+                              work, nw.value, out, stream)
+                lib.memcpy_dtoh_async(host.ctypes.data, out, 8*nout, stream)
+
+        return ReductionKernel(None, mats=tuple(mats), name='reduction')
 
     def copy(self, dst, src):
         if dst.traits != src.traits:
@@ -759,10 +899,17 @@ class PointwiseProvider:
             raise NotSuitableError(f'No native kernel for {mod}')
         self.registered.add(self._known[mod])
 
+    @staticmethod
+    def _base(tplargs):
+        """The base-state form's {rho0, E0} (tplargs 'base'), or None."""
+        b = tplargs.get('base')
+        return None if b is None else (ct.c_double*2)(*b)
+
     # -- tflux (Euler and NS) ------------------------------------------------
     def tflux(self, tplargs, dims, u, f, smats=None, verts=None, upts=None,
               gradu=None, rcpdjac=None, artvisc_vtx=None, **kw):
         lib, be = self.backend.lib, self.backend
+        base = self._base(tplargs)
         npts, neles = dims
         ktype = tplargs['ktype']
         # geometry mode: 1 hex vertices, 2 one S per element, 0 stored S
@@ -780,6 +927,9 @@ class PointwiseProvider:
                 1.0 if tplargs.get('visc_corr') == 'sutherland' else 0.0,
                 c.get('cpTref', 1.0), c.get('cpTs', 0.0))
         if ns and 'fused' not in ktype:
+            if base is not None:
+                raise NotSuitableError('the base-state form has no unfused '
+                                       'NS flux')
             # f holds the physical gradient on entry (tflux.mako, unfused)
             args = (dt, mode, npts, neles, u.data, u.leaddim, f.data,
                     f.leaddim, geo.data, geo.leaddim, uptsp, consts, None)
@@ -790,11 +940,11 @@ class PointwiseProvider:
             ldr = 0 if linear else _rstride(rcpdjac)
             args = (dt, mode, npts, neles, u.data, u.leaddim, f.data,
                     f.leaddim, gradu.data, gradu.leaddim, geo.data,
-                    geo.leaddim, uptsp, rp, ldr, consts, None)
+                    geo.leaddim, uptsp, rp, ldr, consts, None, base)
             return Kernel(lambda stream: lib.tflux_ns(*args, stream),
                           mats=(u, f, gradu, geo), name='tflux_ns')
         args = (dt, mode, npts, neles, u.data, u.leaddim, f.data,
-                f.leaddim, geo.data, geo.leaddim, uptsp, c['gamma'])
+                f.leaddim, geo.data, geo.leaddim, uptsp, c['gamma'], base)
         return Kernel(lambda stream: lib.tflux_euler(*args, stream),
                       mats=(u, f, geo), name='tflux')
 
@@ -851,9 +1001,13 @@ class PointwiseProvider:
         dt = _dtcode(ul.refdtype)
         rs = self._rs(tplargs, ul.refdtype)
         c = tplargs['c']
+        base = self._base(tplargs)
+        if base is not None and rs != 0:
+            raise NotSuitableError('the base-state form takes the Rusanov '
+                                   'solver')
         if gradul is None:
             args = (dt, rs, n, ul.basedata, ul.mapping.data, ur.mapping.data,
-                    nl.data, nl.leaddim, c['gamma'])
+                    nl.data, nl.leaddim, c['gamma'], base)
             return Kernel(lambda stream: lib.intcflux_euler(*args, stream),
                           views=(ul, ur), mats=(nl,), name='intcflux')
         if artvisc is not None:
@@ -865,7 +1019,7 @@ class PointwiseProvider:
             c.get('cpTref', 1.0), c.get('cpTs', 0.0))
         args = (dt, rs, n, ul.basedata, ul.mapping.data, ur.mapping.data,
                 gradul.basedata, gradul.mapping.data, gradur.mapping.data,
-                *gs, nl.data, nl.leaddim, consts, None)
+                *gs, nl.data, nl.leaddim, consts, None, base)
         return Kernel(lambda stream: lib.intcflux_ns(*args, stream),
                       views=(ul, ur, gradul, gradur), mats=(nl,),
                       name='intcflux_ns')
@@ -946,8 +1100,6 @@ class PointwiseProvider:
     # -- rkvdh2 --------------------------------------------------------------
     def rkvdh2(self, tplargs, dims, r1, r2, rold=None, rerr=None, dt=None,
                **kw):
-        if tplargs.get('errest'):
-            raise NotSuitableError('error-estimating rkvdh2 not supported')
         lib = self.backend.lib
         stage, nst = tplargs['stage'], tplargs['nstages']
         a, b = tplargs['a'], tplargs['b']
@@ -956,6 +1108,22 @@ class PointwiseProvider:
         bb = b[stage]
         d = _dtcode(r1.dtype)
         nrow, ncol, ld = r1.nrow, r1.ncol, r1.leaddim
+        if tplargs.get('errest'):
+            # rkvdh2.mako with its error estimate: rold, rerr registers
+            if rold is None or rerr is None:
+                raise ValueError('errest needs rold and rerr')
+            if any(m.traits != r1.traits for m in (r2, rold, rerr)):
+                raise ValueError('Incompatible matrix types')
+            ee = tplargs['e'][stage]
+
+            def fne(stream, dt=0.0):
+                lib.rkvdh2_errest(d, nrow, ncol, ld, r1.data, r2.data,
+                                  rold.data, rerr.data, dt, aa, bb, ee,
+                                  stage, int(last), stream)
+            return Kernel(fne, rtargs={'dt': 0.0},
+                          mats=(r1, r2, rold, rerr), name='rkvdh2')
+        if r2.traits != r1.traits:
+            raise ValueError('Incompatible matrix types')
 
         def fn(stream, dt=0.0):
             lib.rkvdh2(d, nrow, ncol, ld, r1.data, r2.data, dt*aa, dt*bb,
@@ -1019,6 +1187,8 @@ class AMDBackend:
         self._mat_counter = it.count()
         self._extents = {}
         self._all_allocs = []
+        self._host_allocs = []
+        self._programs = []
         self._mem_current = 0
         self._mem_peak = 0
 
@@ -1037,6 +1207,16 @@ class AMDBackend:
         self._mem_current += nbytes
         self._mem_peak = max(self._mem_peak, self._mem_current)
         return p.value
+
+    def _host_array(self, n, dtype=np.float64):
+        """A page-locked host array of n values (freed with the backend),
+        the target of asynchronous device-to-host copies."""
+        nb = max(int(n)*np.dtype(dtype).itemsize, 8)
+        p = ct.c_void_p()
+        self.lib.host_malloc(ct.byref(p), nb)
+        self._host_allocs.append(p.value)
+        buf = (ct.c_char*nb).from_address(p.value)
+        return np.frombuffer(buf, dtype=dtype, count=int(n))
 
     def malloc(self, obj, extent):
         nb = self._extent_alloc_bytes(obj.nbytes)
@@ -1130,6 +1310,10 @@ class AMDBackend:
             self.lib.stream_sync(self.stream)
             for p in self._all_allocs:
                 self.lib.free(p)
+            for p in self._host_allocs:
+                self.lib.host_free(p)
+            for p in self._programs:
+                self.lib.expr_program_destroy(p)
             self.lib.stream_destroy(self.stream)
         except Exception:
             pass

@@ -18,7 +18,9 @@ BINOPS = {ast.Add: 2, ast.Sub: 3, ast.Mult: 4, ast.Div: 5}
 FUNCS2 = {'pow': 6, 'min': 7, 'max': 8, 'atan2': 9}
 NEG = 10
 FUNCS1 = {'sin': 11, 'cos': 12, 'tan': 13, 'exp': 14, 'log': 15, 'sqrt': 16,
-          'abs': 17, 'tanh': 18, 'asin': 19, 'acos': 20, 'atan': 21}
+          'abs': 17, 'tanh': 18, 'asin': 19, 'acos': 20, 'atan': 21,
+          'fabs': 17}
+FUNCS2.update({'fmin': 7, 'fmax': 8})
 VARS = {'x': 0, 'y': 1, 'z': 2, 't': 3}
 MAX_STACK = 16
 
@@ -27,18 +29,20 @@ class ExprError(ValueError):
     pass
 
 
-def _emit(node, consts, code, pool):
+def _emit(node, consts, code, pool, varmap=VARS):
     """Post-order emission; returns the stack depth the node needs."""
+    def emit(n):
+        return _emit(n, consts, code, pool, varmap)
     if isinstance(node, ast.Expression):
-        return _emit(node.body, consts, code, pool)
+        return emit(node.body)
     if isinstance(node, ast.Constant) and isinstance(node.value,
                                                      (int, float)):
         pool.append(float(node.value))
         code.append((PUSHC, len(pool) - 1))
         return 1
     if isinstance(node, ast.Name):
-        if node.id in VARS:
-            code.append((PUSHV, VARS[node.id]))
+        if node.id in varmap:
+            code.append((PUSHV, varmap[node.id]))
         elif node.id == 'pi':
             pool.append(float(np.pi))
             code.append((PUSHC, len(pool) - 1))
@@ -50,35 +54,38 @@ def _emit(node, consts, code, pool):
         return 1
     if isinstance(node, ast.UnaryOp) and isinstance(node.op,
                                                     (ast.USub, ast.UAdd)):
-        d = _emit(node.operand, consts, code, pool)
+        d = emit(node.operand)
         if isinstance(node.op, ast.USub):
             code.append((NEG, 0))
         return d
     if isinstance(node, ast.BinOp) and type(node.op) in BINOPS:
-        da = _emit(node.left, consts, code, pool)
-        db = _emit(node.right, consts, code, pool)
+        da = emit(node.left)
+        db = emit(node.right)
         code.append((BINOPS[type(node.op)], 0))
         return max(da, db + 1)
     if (isinstance(node, ast.Call) and isinstance(node.func, ast.Name) and
             not node.keywords):
         fn, args = node.func.id, node.args
         if fn in FUNCS1 and len(args) == 1:
-            d = _emit(args[0], consts, code, pool)
+            d = emit(args[0])
             code.append((FUNCS1[fn], 0))
             return d
         if fn in FUNCS2 and len(args) == 2:
-            da = _emit(args[0], consts, code, pool)
-            db = _emit(args[1], consts, code, pool)
+            da = emit(args[0])
+            db = emit(args[1])
             code.append((FUNCS2[fn], 0))
             return max(da, db + 1)
     raise ExprError(f'Unsupported expression element: {ast.dump(node)}')
 
 
-def compile_exprs(exprs, consts, params=None):
+def compile_exprs(exprs, consts, params=None, varmap=None):
     """exprs: list of expression strings; consts: {name: value}.  Returns
     (code (ncode, 2) int32, offs (len(exprs) + 1) int32, pool float64).
     params: {name: initial value} of run-time parameters; with it, also
-    returns {name: [pool indices]} (set with frhip_bc_program_set_const)."""
+    returns {name: [pool indices]} (set with frhip_bc_program_set_const).
+    varmap: {name: variable index} of the program's variables (default x, y,
+    z, t of a boundary program)."""
+    varmap = VARS if varmap is None else varmap
     code, offs, pool = [], [0], []
     pidx = {}
     if params:
@@ -91,13 +98,13 @@ def compile_exprs(exprs, consts, params=None):
         except SyntaxError as ex:
             raise ExprError(f'Invalid expression {e!r}') from ex
         n0 = len(code)
-        depth = _emit(tree, consts, code, pool)
+        depth = _emit(tree, consts, code, pool, varmap)
         if depth > MAX_STACK:
             raise ExprError(f'Expression too deep: {e!r}')
         if params:
             # the pool entries the parameters' names were emitted as
             pushes = [arg for op, arg in code[n0:] if op == PUSHC]
-            for name, idx in zip(_pool_names(tree, consts), pushes):
+            for name, idx in zip(_pool_names(tree, consts, varmap), pushes):
                 if name in params:
                     pidx.setdefault(name, []).append(idx)
         offs.append(len(code))
@@ -106,26 +113,26 @@ def compile_exprs(exprs, consts, params=None):
     return out + (pidx,) if params is not None else out
 
 
-def _pool_names(node, consts):
+def _pool_names(node, consts, varmap=VARS):
     """For each constant push of the post-order emission, the name it came
     from (None for numbers and pi)."""
     if isinstance(node, ast.Expression):
-        return _pool_names(node.body, consts)
+        return _pool_names(node.body, consts, varmap)
     if isinstance(node, ast.Constant):
         return [None]
     if isinstance(node, ast.Name):
-        if node.id in VARS:
+        if node.id in varmap:
             return []
         return [None] if node.id == 'pi' else [node.id]
     if isinstance(node, ast.UnaryOp):
-        return _pool_names(node.operand, consts)
+        return _pool_names(node.operand, consts, varmap)
     if isinstance(node, ast.BinOp):
-        return (_pool_names(node.left, consts) +
-                _pool_names(node.right, consts))
+        return (_pool_names(node.left, consts, varmap) +
+                _pool_names(node.right, consts, varmap))
     if isinstance(node, ast.Call):
         out = []
         for a in node.args:
-            out += _pool_names(a, consts)
+            out += _pool_names(a, consts, varmap)
         return out
     return []
 

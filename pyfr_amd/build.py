@@ -14,7 +14,7 @@ BUILD = os.path.join(CSRC, '_build')
 LIB = os.path.join(HERE, 'libfrhip.so')
 
 SOURCES = ['frhip_runtime.hip', 'frhip_ops.hip', 'frhip_dense.hip',
-           'frhip_pointwise.hip', 'frhip_bcs.hip']
+           'frhip_pointwise.hip', 'frhip_bcs.hip', 'frhip_blasext.hip']
 
 HIPFLAGS = [
     '--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC', '-shared',

@@ -15,6 +15,7 @@
 // are formed in double on the host, like the reference's template literals.
 #include "frhip_common.h"
 #include "frhip_physics.h"
+#include "frhip_expr.h"
 
 #include <cmath>
 #include <type_traits>
@@ -306,8 +307,6 @@ static void dispatch(int id, F &&f)
 // program of (op, arg) pairs evaluated in double at each boundary point; t
 // lives in a device scalar the host sets on the solver stream before each
 // RHS, so captured graphs stay valid.
-constexpr int BCX_STACK = 16;
-
 template <typename T>
 __global__ __launch_bounds__(256) void k_bc_eval(
     int n, int nrows, const int32_t *__restrict__ code,
@@ -319,57 +318,9 @@ __global__ __launch_bounds__(256) void k_bc_eval(
     if (i >= n) return;
     const double var[4] = {double(ploc[i]), double(ploc[ldp + i]),
                            double(ploc[2*ldp + i]), *tp};
-    for (int r = 0; r < nrows; ++r) {
-        double stk[BCX_STACK];
-        int sp = 0;
-        for (int pc = offs[r]; pc < offs[r + 1]; ++pc) {
-            const int op = code[2*pc], arg = code[2*pc + 1];
-            // programs are validated on the host (depth <= BCX_STACK,
-            // operand counts, indices); the guards keep any access in range
-            if (op <= FRHIP_BCX_PUSHV) {
-                if (sp < BCX_STACK)
-                    stk[sp++] = op == FRHIP_BCX_PUSHC ? pool[arg]
-                                                      : var[arg & 3];
-                continue;
-            }
-            if (op <= FRHIP_BCX_ATAN2) {
-                if (sp < 2) continue;
-                const double b = stk[--sp], a = stk[sp - 1];
-                double c;
-                switch (op) {
-                    case FRHIP_BCX_ADD: c = a + b; break;
-                    case FRHIP_BCX_SUB: c = a - b; break;
-                    case FRHIP_BCX_MUL: c = a*b; break;
-                    case FRHIP_BCX_DIV: c = a/b; break;
-                    case FRHIP_BCX_POW: c = pow(a, b); break;
-                    case FRHIP_BCX_MIN: c = fmin(a, b); break;
-                    case FRHIP_BCX_MAX: c = fmax(a, b); break;
-                    default: c = atan2(a, b); break;
-                }
-                stk[sp - 1] = c;
-                continue;
-            }
-            if (sp < 1) continue;
-            const double a = stk[sp - 1];
-            double c;
-            switch (op) {
-                case FRHIP_BCX_NEG: c = -a; break;
-                case FRHIP_BCX_SIN: c = sin(a); break;
-                case FRHIP_BCX_COS: c = cos(a); break;
-                case FRHIP_BCX_TAN: c = tan(a); break;
-                case FRHIP_BCX_EXP: c = exp(a); break;
-                case FRHIP_BCX_LOG: c = log(a); break;
-                case FRHIP_BCX_SQRT: c = sqrt(a); break;
-                case FRHIP_BCX_ABS: c = fabs(a); break;
-                case FRHIP_BCX_TANH: c = tanh(a); break;
-                case FRHIP_BCX_ASIN: c = asin(a); break;
-                case FRHIP_BCX_ACOS: c = acos(a); break;
-                default: c = atan(a); break;
-            }
-            stk[sp - 1] = c;
-        }
-        out[r*ldo + i] = T(sp > 0 ? stk[0] : 0.0);
-    }
+    for (int r = 0; r < nrows; ++r)
+        out[r*ldo + i] = T(expr_eval(code, offs[r], offs[r + 1], pool, var,
+                                     4));
 }
 
 __global__ void k_set_scalar(double *p, double v)
@@ -499,32 +450,10 @@ int frhip_bc_program_create(void **prog, int nrows, const int32_t *code,
     FR_API_BEGIN
     FR_CHECK(prog && code && offs && nrows > 0 && nrows <= 8,
              "invalid program arguments");
-    FR_CHECK(offs[0] == 0, "program offsets must start at 0");
-    for (int r = 0; r < nrows; ++r) {
-        FR_CHECK(offs[r + 1] > offs[r], "empty expression");
-        int sp = 0;
-        for (int pc = offs[r]; pc < offs[r + 1]; ++pc) {
-            const int op = code[2*pc], arg = code[2*pc + 1];
-            if (op == FRHIP_BCX_PUSHC) {
-                FR_CHECK(pool && arg >= 0 && arg < npool,
-                         "constant index out of range");
-                ++sp;
-            }
-            else if (op == FRHIP_BCX_PUSHV) {
-                FR_CHECK(arg >= 0 && arg < 4, "variable index out of range");
-                ++sp;
-            }
-            else if (op <= FRHIP_BCX_ATAN2) {
-                FR_CHECK(sp >= 2, "stack underflow");
-                --sp;
-            }
-            else {
-                FR_CHECK(op <= FRHIP_BCX_ATAN, "invalid opcode");
-                FR_CHECK(sp >= 1, "stack underflow");
-            }
-            FR_CHECK(sp <= BCX_STACK, "expression too deep");
-        }
-        FR_CHECK(sp == 1, "expression leaves a malformed stack");
+    FR_CHECK(npool == 0 || pool, "missing constant pool");
+    {
+        const char *err = expr_validate(nrows, code, offs, npool, 4);
+        FR_CHECK(!err, err ? err : "");
     }
     auto *p = new BCProgram{nrows, offs[nrows], npool, nullptr, nullptr,
                             nullptr, nullptr};

@@ -96,12 +96,13 @@ __device__ __forceinline__ void smats_at(const T *__restrict__ geo, long ldg,
                 s[i][k] = geo[((long)i*npts + pt)*ldg + soa_ix(e, k, 3)];
 }
 
-// tflux (Euler); pyfr/solvers/euler/kernels/tflux.mako:8-30.
-template <typename T, int GEO>
+// tflux (Euler); pyfr/solvers/euler/kernels/tflux.mako:8-30.  BASE: u
+// holds u - u0 and f receives the flux less f(u0) (BaseState).
+template <typename T, int GEO, bool BASE>
 __global__ __launch_bounds__(256) void k_tflux_euler(
     int npts, int neles, const T *__restrict__ u, long ldu, T *__restrict__ f,
     long ldf, const T *__restrict__ geo, long ldg, const T *__restrict__ upts,
-    T gm1)
+    T gm1, BaseState<T> bs)
 {
     const int e = blockIdx.x*64 + threadIdx.x;
     if (e >= neles) return;
@@ -116,7 +117,10 @@ __global__ __launch_bounds__(256) void k_tflux_euler(
         T uu[5];
         for (int v = 0; v < 5; ++v) uu[v] = u[pt*ldu + soa_ix(e, v, 5)];
         T ft[3][5], p, vel[3];
-        inviscid_flux(uu, ft, p, vel, gm1);
+        if constexpr (BASE)
+            inviscid_flux_base(uu, ft, p, vel, gm1, bs);
+        else
+            inviscid_flux(uu, ft, p, vel, gm1);
         for (int i = 0; i < 3; ++i)
             for (int j = 0; j < 5; ++j)
                 f[((long)i*npts + pt)*ldf + soa_ix(e, j, 5)] =
@@ -125,12 +129,12 @@ __global__ __launch_bounds__(256) void k_tflux_euler(
 }
 
 // tflux (NS, gradient-fused); pyfr/solvers/navstokes/kernels/tflux.mako:13-50.
-template <typename T, int GEO>
+template <typename T, int GEO, bool BASE>
 __global__ __launch_bounds__(256) void k_tflux_ns(
     int npts, int neles, const T *__restrict__ u, long ldu, T *__restrict__ f,
     long ldf, T *__restrict__ gradu, long ldgr, const T *__restrict__ geo,
     long ldg, const T *__restrict__ upts, const T *__restrict__ rcpdjac,
-    long ldr, NSConsts<T> c0, const T *__restrict__ mu_e)
+    long ldr, NSConsts<T> c0, const T *__restrict__ mu_e, BaseState<T> bs)
 {
     const int e = blockIdx.x*64 + threadIdx.x;
     if (e >= neles) return;
@@ -162,8 +166,17 @@ __global__ __launch_bounds__(256) void k_tflux_ns(
             }
 
         T ft[3][5], p, vel[3];
-        inviscid_flux(uu, ft, p, vel, c.gm1);
-        viscous_flux_add(uu, g, ft, c);
+        if constexpr (BASE) {
+            // the viscous flux of the full state; it vanishes at u0
+            T uf[5];
+            base_full(uu, uf, bs);
+            inviscid_flux_base(uu, ft, p, vel, c.gm1, bs);
+            viscous_flux_add(uf, g, ft, c);
+        }
+        else {
+            inviscid_flux(uu, ft, p, vel, c.gm1);
+            viscous_flux_add(uu, g, ft, c);
+        }
         for (int i = 0; i < 3; ++i)
             for (int j = 0; j < 5; ++j)
                 f[((long)i*npts + pt)*ldf + soa_ix(e, j, 5)] =
@@ -254,11 +267,11 @@ __global__ __launch_bounds__(256) void k_negdivconf(
 // ---------------------------------------------------------------------------
 
 // intcflux (Euler); pyfr/solvers/euler/kernels/intcflux.mako:6-22.
-template <typename T, int RS>
+template <typename T, int RS, bool BASE>
 __global__ __launch_bounds__(256) void k_intcflux_euler(
     int n, T *__restrict__ base, const int32_t *__restrict__ lmap,
     const int32_t *__restrict__ rmap, const T *__restrict__ nl, long ldn,
-    T gamma, T gm1, T qgamma)
+    T gamma, T gm1, T qgamma, BaseState<T> bs)
 {
     const int i = blockIdx.x*blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -272,7 +285,12 @@ __global__ __launch_bounds__(256) void k_intcflux_euler(
     const T mag = sqrt((n0)*(n0) + (n1)*(n1) + (n2)*(n2));
     const T nn[3] = {(T(1)/mag)*n0, (T(1)/mag)*n1, (T(1)/mag)*n2};
     T fn[5];
-    rsolve<T, RS>(ul, ur, nn, fn, gm1, gamma, qgamma);
+    if constexpr (BASE) {
+        static_assert(RS == FRHIP_RS_RUSANOV, "base-state form: Rusanov");
+        rsolve_rusanov_base(ul, ur, nn, fn, gm1, qgamma, bs);
+    }
+    else
+        rsolve<T, RS>(ul, ur, nn, fn, gm1, gamma, qgamma);
     for (int v = 0; v < 5; ++v) {
         base[li + v*SOA] = mag*fn[v];
         base[ri + v*SOA] = -mag*fn[v];
@@ -304,14 +322,14 @@ __global__ __launch_bounds__(256) void k_intconu(
 }
 
 // intcflux (NS); pyfr/solvers/navstokes/kernels/intcflux.mako:10-54.
-template <typename T, int RS>
+template <typename T, int RS, bool BASE>
 __global__ __launch_bounds__(256) void k_intcflux_ns(
     int n, T *__restrict__ base, const int32_t *__restrict__ lmap,
     const int32_t *__restrict__ rmap, const T *__restrict__ gb,
     const int32_t *__restrict__ glmap, const int32_t *__restrict__ grmap,
     long gstride, const int32_t *__restrict__ glstr,
     const int32_t *__restrict__ grstr, const T *__restrict__ nl, long ldn,
-    NSConsts<T> c0, const T *__restrict__ mu_p)
+    NSConsts<T> c0, const T *__restrict__ mu_p, BaseState<T> bs)
 {
     const int i = blockIdx.x*blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -330,7 +348,21 @@ __global__ __launch_bounds__(256) void k_intcflux_ns(
     const T nn[3] = {(T(1)/mag)*n0, (T(1)/mag)*n1, (T(1)/mag)*n2};
 
     T fi[5];
-    rsolve<T, RS>(ul, ur, nn, fi, c.gm1, c.gamma, c.qgamma);
+    // BASE: ul, ur hold u - u0; the viscous fluxes take the full states
+    T ulf[5], urf[5];
+    if constexpr (BASE) {
+        static_assert(RS == FRHIP_RS_RUSANOV, "base-state form: Rusanov");
+        rsolve_rusanov_base(ul, ur, nn, fi, c.gm1, c.qgamma, bs);
+        base_full(ul, ulf, bs);
+        base_full(ur, urf, bs);
+    }
+    else {
+        rsolve<T, RS>(ul, ur, nn, fi, c.gm1, c.gamma, c.qgamma);
+        for (int v = 0; v < 5; ++v) {
+            ulf[v] = ul[v];
+            urf[v] = ur[v];
+        }
+    }
 
     T fvl[3][5] = {}, fvr[3][5] = {};
     if (c.beta != T(-0.5)) {
@@ -338,14 +370,14 @@ __global__ __launch_bounds__(256) void k_intcflux_ns(
         for (int d = 0; d < 3; ++d)
             for (int v = 0; v < 5; ++v)
                 gl[d][v] = gb[gli + d*gsl + v*SOA];
-        viscous_flux_add(ul, gl, fvl, c);
+        viscous_flux_add(ulf, gl, fvl, c);
     }
     if (c.beta != T(0.5)) {
         T gr[3][5];
         for (int d = 0; d < 3; ++d)
             for (int v = 0; v < 5; ++v)
                 gr[d][v] = gb[gri + d*gsr + v*SOA];
-        viscous_flux_add(ur, gr, fvr, c);
+        viscous_flux_add(urf, gr, fvr, c);
     }
 
     for (int v = 0; v < 5; ++v) {
@@ -477,59 +509,6 @@ __global__ __launch_bounds__(256) void k_rkvdh2_mixed(long ncol, long ld,
     kin[idx] = last ? (float)na : (float)(t1 + dta*t2);
 }
 
-// Per-variable partial reductions, FP64 accumulation: block (256), grid
-// (nblk, nvar).  op 0: sum of w*x, op 1: max |x|, op 2: sum of w*x*x.
-template <typename T>
-__global__ __launch_bounds__(256) void k_reduce_part(
-    int op, int nrow, int nvar, int neles, long ld, const T *__restrict__ x,
-    const T *__restrict__ w, long ldw, double *__restrict__ part)
-{
-    const int v = blockIdx.y;
-    double acc = 0.0;
-    const long total = (long)nrow*neles;
-    for (long q = (long)blockIdx.x*blockDim.x + threadIdx.x; q < total;
-         q += (long)gridDim.x*blockDim.x) {
-        const int r = (int)(q/neles);
-        const long e = q - (long)r*neles;
-        const double xv = (double)x[r*ld + soa_ix(e, v, nvar)];
-        const double wv = w ? (double)w[r*ldw + e] : 1.0;
-        if (op == 0) acc += wv*xv;
-        else if (op == 2) acc += wv*xv*xv;
-        else acc = fmax(acc, fabs(xv));
-    }
-    __shared__ double sm[256];
-    sm[threadIdx.x] = acc;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s)
-            sm[threadIdx.x] = op != 1 ? sm[threadIdx.x] + sm[threadIdx.x + s]
-                                      : fmax(sm[threadIdx.x], sm[threadIdx.x + s]);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) part[(long)v*gridDim.x + blockIdx.x] = sm[0];
-}
-
-__global__ __launch_bounds__(256) void k_reduce_final(int op, int nblk,
-                                                      const double *part,
-                                                      double *out)
-{
-    const int v = blockIdx.x;
-    double acc = 0.0;
-    for (int q = threadIdx.x; q < nblk; q += blockDim.x)
-        acc = op != 1 ? acc + part[(long)v*nblk + q]
-                      : fmax(acc, part[(long)v*nblk + q]);
-    __shared__ double sm[256];
-    sm[threadIdx.x] = acc;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s)
-            sm[threadIdx.x] = op != 1 ? sm[threadIdx.x] + sm[threadIdx.x + s]
-                                      : fmax(sm[threadIdx.x], sm[threadIdx.x + s]);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) out[v] = sm[0];
-}
-
 template <typename D, typename S>
 __global__ __launch_bounds__(256) void k_convert(long n, D *__restrict__ d,
                                                  const S *__restrict__ s)
@@ -577,7 +556,7 @@ extern "C" {
 int frhip_tflux_euler(int dtype, int linear, int npts, int neles,
                       const void *u, long ldu, void *f, long ldf,
                       const void *geo, long ldgeo, const void *upts,
-                      double gamma, void *stream)
+                      double gamma, const double *base, void *stream)
 {
     FR_API_BEGIN
     FR_DTYPE_OK(dtype);
@@ -589,10 +568,16 @@ int frhip_tflux_euler(int dtype, int linear, int npts, int neles,
     dim3 grid = elem_grid(npts, neles), block(64, 4);
     hipStream_t s = as_stream(stream);
 #define FR_LAUNCH_TF(T, G)                                                  \
-    hipLaunchKernelGGL((k_tflux_euler<T, G>), grid, block, 0, s, npts,      \
-                       neles, (const T *)u, ldu, (T *)f, ldf,               \
-                       (const T *)geo, ldgeo, (const T *)upts,              \
-                       (T)(gamma - 1))
+    if (base)                                                               \
+        hipLaunchKernelGGL((k_tflux_euler<T, G, true>), grid, block, 0, s,  \
+                           npts, neles, (const T *)u, ldu, (T *)f, ldf,     \
+                           (const T *)geo, ldgeo, (const T *)upts,          \
+                           (T)(gamma - 1), base_state<T>(base, gamma));     \
+    else                                                                    \
+        hipLaunchKernelGGL((k_tflux_euler<T, G, false>), grid, block, 0, s, \
+                           npts, neles, (const T *)u, ldu, (T *)f, ldf,     \
+                           (const T *)geo, ldgeo, (const T *)upts,          \
+                           (T)(gamma - 1), BaseState<T>{})
     if (dtype == FRHIP_F64) {
         if (linear == 0) FR_LAUNCH_TF(double, 0);
         else if (linear == 1) FR_LAUNCH_TF(double, 1);
@@ -684,7 +669,7 @@ int frhip_tflux_ns(int dtype, int linear, int npts, int neles,
                    const void *u, long ldu, void *f, long ldf, void *gradu,
                    long ldg, const void *geo, long ldgeo, const void *upts,
                    const void *rcpdjac, long ldr, const double *consts,
-                   const void *mu_e, void *stream)
+                   const void *mu_e, const double *base, void *stream)
 {
     FR_API_BEGIN
     FR_DTYPE_OK(dtype);
@@ -697,10 +682,18 @@ int frhip_tflux_ns(int dtype, int linear, int npts, int neles,
     dim3 grid = elem_grid(npts, neles), block(64, 4);
     hipStream_t s = as_stream(stream);
 #define FR_LAUNCH_NS(T, G)                                                  \
-    hipLaunchKernelGGL((k_tflux_ns<T, G>), grid, block, 0, s, npts, neles,  \
-                       (const T *)u, ldu, (T *)f, ldf, (T *)gradu, ldg,     \
-                       (const T *)geo, ldgeo, (const T *)upts,              \
-                       (const T *)rcpdjac, ldr, c, (const T *)mu_e)
+    if (base)                                                               \
+        hipLaunchKernelGGL((k_tflux_ns<T, G, true>), grid, block, 0, s,     \
+                           npts, neles, (const T *)u, ldu, (T *)f, ldf,     \
+                           (T *)gradu, ldg, (const T *)geo, ldgeo,          \
+                           (const T *)upts, (const T *)rcpdjac, ldr, c,     \
+                           (const T *)mu_e, base_state<T>(base, consts[0]));\
+    else                                                                    \
+        hipLaunchKernelGGL((k_tflux_ns<T, G, false>), grid, block, 0, s,    \
+                           npts, neles, (const T *)u, ldu, (T *)f, ldf,     \
+                           (T *)gradu, ldg, (const T *)geo, ldgeo,          \
+                           (const T *)upts, (const T *)rcpdjac, ldr, c,     \
+                           (const T *)mu_e, BaseState<T>{})
     if (dtype == FRHIP_F64) {
         auto c = ns_consts<double>(consts);
         if (linear == 0) FR_LAUNCH_NS(double, 0);
@@ -741,24 +734,40 @@ int frhip_negdivconf(int dtype, int npts, int neles, void *t, long ldt,
 
 int frhip_intcflux_euler(int dtype, int rsolver, int n, void *base,
                          const int32_t *lmap, const int32_t *rmap,
-                         const void *nl, long ldn, double gamma, void *stream)
+                         const void *nl, long ldn, double gamma,
+                         const double *bstate, void *stream)
 {
     FR_API_BEGIN
     FR_DTYPE_OK(dtype);
     FR_RS_OK(rsolver);
+    FR_CHECK(!bstate || rsolver == FRHIP_RS_RUSANOV,
+             "the base-state form takes the Rusanov solver");
     if (n == 0) return 0;
     dim3 grid(cdiv(n, 256)), block(256);
 #define FR_L64(RS)                                                          \
-    hipLaunchKernelGGL((k_intcflux_euler<double, RS>), grid, block, 0,      \
-                       as_stream(stream), n, (double *)base, lmap, rmap,    \
+    hipLaunchKernelGGL((k_intcflux_euler<double, RS, false>), grid, block,  \
+                       0, as_stream(stream), n, (double *)base, lmap, rmap, \
                        (const double *)nl, ldn, gamma, gamma - 1,           \
-                       0.25*gamma)
+                       0.25*gamma, BaseState<double>{})
 #define FR_L32(RS)                                                          \
-    hipLaunchKernelGGL((k_intcflux_euler<float, RS>), grid, block, 0,       \
-                       as_stream(stream), n, (float *)base, lmap, rmap,     \
+    hipLaunchKernelGGL((k_intcflux_euler<float, RS, false>), grid, block,   \
+                       0, as_stream(stream), n, (float *)base, lmap, rmap,  \
                        (const float *)nl, ldn, (float)gamma,                \
-                       (float)(gamma - 1), (float)(0.25*gamma))
-    if (dtype == FRHIP_F64) {
+                       (float)(gamma - 1), (float)(0.25*gamma),             \
+                       BaseState<float>{})
+    if (bstate && dtype == FRHIP_F64)
+        hipLaunchKernelGGL(
+            (k_intcflux_euler<double, FRHIP_RS_RUSANOV, true>), grid, block,
+            0, as_stream(stream), n, (double *)base, lmap, rmap,
+            (const double *)nl, ldn, gamma, gamma - 1, 0.25*gamma,
+            base_state<double>(bstate, gamma));
+    else if (bstate)
+        hipLaunchKernelGGL(
+            (k_intcflux_euler<float, FRHIP_RS_RUSANOV, true>), grid, block,
+            0, as_stream(stream), n, (float *)base, lmap, rmap,
+            (const float *)nl, ldn, (float)gamma, (float)(gamma - 1),
+            (float)(0.25*gamma), base_state<float>(bstate, gamma));
+    else if (dtype == FRHIP_F64) {
         FR_RS_DISPATCH(rsolver, FR_L64)
     }
     else {
@@ -800,29 +809,31 @@ int frhip_intcflux_ns(int dtype, int rsolver, int n, void *ubase,
                       const int32_t *grmap, long gstride,
                       const int32_t *glstr, const int32_t *grstr,
                       const void *nl, long ldn, const double *consts,
-                      const void *mu_p, void *stream)
+                      const void *mu_p, const double *bstate, void *stream)
 {
     FR_API_BEGIN
     FR_DTYPE_OK(dtype);
     FR_RS_OK(rsolver);
     FR_CHECK(consts, "null constants");
+    FR_CHECK(!bstate || rsolver == FRHIP_RS_RUSANOV,
+             "the base-state form takes the Rusanov solver");
     if (n == 0) return 0;
     dim3 grid(cdiv(n, 256)), block(256);
-#define FR_L64(RS)                                                          \
-    hipLaunchKernelGGL((k_intcflux_ns<double, RS>), grid, block, 0,         \
-                       as_stream(stream), n, (double *)ubase, lmap, rmap,   \
-                       (const double *)gbase, glmap, grmap, gstride, glstr, \
-                       grstr, (const double *)nl, ldn,                      \
-                       ns_consts<double>(consts),                           \
-                       (const double *)mu_p)
-#define FR_L32(RS)                                                          \
-    hipLaunchKernelGGL((k_intcflux_ns<float, RS>), grid, block, 0,          \
-                       as_stream(stream), n, (float *)ubase, lmap, rmap,    \
-                       (const float *)gbase, glmap, grmap, gstride, glstr,  \
-                       grstr, (const float *)nl, ldn,                       \
-                       ns_consts<float>(consts),                            \
-                       (const float *)mu_p)
-    if (dtype == FRHIP_F64) {
+#define FR_LNS(T, RS, B, BS)                                                \
+    hipLaunchKernelGGL((k_intcflux_ns<T, RS, B>), grid, block, 0,           \
+                       as_stream(stream), n, (T *)ubase, lmap, rmap,        \
+                       (const T *)gbase, glmap, grmap, gstride, glstr,      \
+                       grstr, (const T *)nl, ldn, ns_consts<T>(consts),     \
+                       (const T *)mu_p, BS)
+#define FR_L64(RS) FR_LNS(double, RS, false, BaseState<double>{})
+#define FR_L32(RS) FR_LNS(float, RS, false, BaseState<float>{})
+    if (bstate && dtype == FRHIP_F64)
+        FR_LNS(double, FRHIP_RS_RUSANOV, true,
+               base_state<double>(bstate, consts[0]));
+    else if (bstate)
+        FR_LNS(float, FRHIP_RS_RUSANOV, true,
+               base_state<float>(bstate, consts[0]));
+    else if (dtype == FRHIP_F64) {
         FR_RS_DISPATCH(rsolver, FR_L64)
     }
     else {
@@ -830,6 +841,7 @@ int frhip_intcflux_ns(int dtype, int rsolver, int n, void *ubase,
     }
 #undef FR_L64
 #undef FR_L32
+#undef FR_LNS
     FR_LAUNCHED();
     return 0;
     FR_API_END
@@ -919,44 +931,6 @@ int frhip_rkvdh2_mixed(int nrow, long ncol, long ld, void *acc, void *kin,
     dim3 grid(cdiv(ncol, 256), nrow), block(256);
     hipLaunchKernelGGL(k_rkvdh2_mixed, grid, block, 0, as_stream(stream), ncol,
                        ld, (double *)acc, (float *)kin, dta, dtb, last);
-    FR_LAUNCHED();
-    return 0;
-    FR_API_END
-}
-
-// Scratch for the two-pass reduction, allocated on first use (outside any
-// graph capture) and kept for the life of the process.
-static double *g_red_part = nullptr;
-static const int g_red_nblk = 512;
-static int g_red_nvar = 0;
-
-int frhip_reduce(int dtype, int op, int nrow, int nvar, int neles, long ld,
-                 const void *x, const void *w, long ldw, double *out,
-                 void *stream)
-{
-    FR_API_BEGIN
-    FR_DTYPE_OK(dtype);
-    FR_CHECK(op == 0 || op == 1 || op == 2, "invalid reduction op");
-    FR_CHECK(nvar > 0 && nvar <= 64 && nrow > 0, "invalid extents");
-    if (nvar > g_red_nvar) {
-        if (g_red_part) FR_HIP(hipFree(g_red_part));
-        g_red_part = nullptr;
-        FR_HIP(hipMalloc(&g_red_part, sizeof(double)*g_red_nblk*64));
-        g_red_nvar = 64;
-    }
-    hipStream_t s = as_stream(stream);
-    dim3 grid(g_red_nblk, nvar), block(256);
-    if (dtype == FRHIP_F64)
-        hipLaunchKernelGGL(k_reduce_part<double>, grid, block, 0, s, op, nrow,
-                           nvar, neles, ld, (const double *)x,
-                           (const double *)w, ldw, g_red_part);
-    else
-        hipLaunchKernelGGL(k_reduce_part<float>, grid, block, 0, s, op, nrow,
-                           nvar, neles, ld, (const float *)x,
-                           (const float *)w, ldw, g_red_part);
-    FR_LAUNCHED();
-    hipLaunchKernelGGL(k_reduce_final, dim3(nvar), dim3(256), 0, s, op,
-                       g_red_nblk, g_red_part, out);
     FR_LAUNCHED();
     return 0;
     FR_API_END

@@ -105,6 +105,16 @@ int frhip_memcpy_dtoh(void *dst, const void *src, size_t nbytes, void *stream)
     FR_API_END
 }
 
+int frhip_memcpy_dtoh_async(void *dst, const void *src, size_t nbytes,
+                            void *stream)
+{
+    FR_API_BEGIN
+    FR_HIP(hipMemcpyAsync(dst, src, nbytes, hipMemcpyDeviceToHost,
+                          as_stream(stream)));
+    return 0;
+    FR_API_END
+}
+
 int frhip_memcpy_dtod(void *dst, const void *src, size_t nbytes, void *stream)
 {
     FR_API_BEGIN

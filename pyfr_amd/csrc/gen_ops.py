@@ -151,6 +151,10 @@ def emit_kernel(name, A):
 #         the device pointer pair {r1, r0}, c receives the next stage's
 #         input (the axnpby updates of pyfr/integrators/explicit/
 #         steppers.py:67-108, operands in the same order)
+#   EP 14-17: EP 10-13 for the precision form: r1 and r0 FP64, the stage
+#         RHS t and the next stage's input c FP32; the updates are formed in
+#         FP64 in the same order, and EP 17 stores the new solution rounded
+#         to float into c (the next step's first stage input)
 # The last stage is a template value, not a run-time flag: a uniform branch
 # per output row makes the compiler keep every row's RK operands live and
 # spill.  The trailing int argument is unused.
@@ -161,9 +165,9 @@ def emit_kernel(name, A):
 # pyfr/solvers/baseadvecdiff/system.py:174-203).
 FUSED = [('grad', ['M4 - M6*M0', 'M6']), ('div', ['M1 - M3*M2', 'M3'])]
 FUSED_EPS = {'double': [0, 1, 2, 4, 10, 11, 12, 13],
-             'float': [0, 1, 2, 3, 4, 5, 10, 11, 12, 13]}
+             'float': [0, 1, 2, 3, 4, 5, 10, 11, 12, 13, 14, 15, 16, 17]}
 # kernel table slot of each epilogue (FusedKernelEntry::fd / ff)
-EP_SLOTS = [0, 1, 2, 3, 4, 5, 10, 11, 12, 13]
+EP_SLOTS = [0, 1, 2, 3, 4, 5, 10, 11, 12, 13, 14, 15, 16, 17]
 
 
 def emit_fused(name, A0, A1):
@@ -206,6 +210,35 @@ def emit_fused(name, A0, A1):
         L.append('        } else if constexpr (EP == 4) {')
         L.append(f'          T *r1 = static_cast<T *>(xa) + j + {r}*ldc;')
         L.append('          *r1 = *r1 + T(dtb)*t;')
+        L.append('        } else if constexpr (EP >= 14) {')
+        L.append('          double *const *pp = '
+                 'static_cast<double *const *>(xa);')
+        L.append(f'          double *r1 = pp[0] + j + {r}*ldc;')
+        L.append(f'          const double r0 = pp[1][j + {r}*ldc];')
+        L.append('          const double td = (double)t;')
+        L.append('          if constexpr (EP == 14) {')
+        L.append('            *r1 = td;')
+        L.append('            double a = 1.0*r0;')
+        L.append('            a = a + (dta/2.0)*td;')
+        L.append(f'            c[{r}*ldc] = (T)a;')
+        L.append('          } else if constexpr (EP == 15) {')
+        L.append('            double a = (dta/6.0)*(*r1);')
+        L.append('            a = a + 1.0*r0;')
+        L.append('            a = a + (dta/3.0)*td;')
+        L.append('            *r1 = a;')
+        L.append('            double b = (dta/2.0)*td;')
+        L.append('            b = b + 1.0*r0;')
+        L.append(f'            c[{r}*ldc] = (T)b;')
+        L.append('          } else if constexpr (EP == 16) {')
+        L.append('            *r1 += (dta/3.0)*td;')
+        L.append('            double b = dta*td;')
+        L.append('            b = b + 1.0*r0;')
+        L.append(f'            c[{r}*ldc] = (T)b;')
+        L.append('          } else {')
+        L.append('            const double na = *r1 + (dta/6.0)*td;')
+        L.append('            *r1 = na;')
+        L.append(f'            c[{r}*ldc] = (T)na;')
+        L.append('          }')
         L.append('        } else if constexpr (EP >= 10) {')
         L.append('          T *const *pp = static_cast<T *const *>(xa);')
         L.append(f'          T *r1 = pp[0] + j + {r}*ldc;')

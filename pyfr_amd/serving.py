@@ -358,10 +358,11 @@ class Pool:
         if not self.ns:
             lib.intcflux_euler(d, 0, mp['n'], self.scal_fpts.data,
                                mp['l'], mp['r'], mp['nl'],
-                               mp['ldn'], GAMMA, s)
+                               mp['ldn'], GAMMA, None, s)
             lib.tflux_euler(d, int(self.linear), Nu, E, u.data, u.leaddim,
                             self.vect_upts.data, self.vect_upts.leaddim,
-                            geo.data, geo.leaddim, self.upts.data, GAMMA, s)
+                            geo.data, geo.leaddim, self.upts.data, GAMMA,
+                            None, s)
         else:
             consts = (ct.c_double*8)(GAMMA, 0.0, PR, 0.0, 0.0, 0.0, 1.0, 0.0)
             self._consts = consts
@@ -375,7 +376,7 @@ class Pool:
                          self.grad_upts.data, self.grad_upts.leaddim,
                          geo.data, geo.leaddim, self.upts.data,
                          self.rcpdjac.data, self.rcpdjac.leaddim, consts,
-                         self.mu_e.data, s)
+                         self.mu_e.data, None, s)
             for i in range(3):
                 lib.mul(self._plans['M0'], ncol,
                         self.grad_upts.data + i*Nu*u.leaddim*u.itemsize,
@@ -386,7 +387,7 @@ class Pool:
                             mp['l'], mp['r'], self.vect_fpts.data,
                             mp['l'], mp['r'], Nf*u.leaddim, None, None,
                             mp['nl'], mp['ldn'], consts,
-                            mp['mu'], s)
+                            mp['mu'], None, s)
         mm('M1 - M3*M2', self.vect_upts, f)
         mm('M3', self.scal_fpts, f, 1.0)
         lib.negdivconf(d, Nu, E, f.data, f.leaddim, self.rcpdjac.data,

@@ -259,6 +259,7 @@ class FluidElements:
         return {'scal_fpts', 'vect_upts'}
 
     def set_backend(self, backend, nonce, linoff):
+        self.base_tpl = {}
         self._be = backend
         self.grad_fusion = not (backend.blocks or self.fluxaa)
         if self.basis.order >= 2 and not self.curved_simplex:
@@ -465,18 +466,18 @@ class EulerElements(FluidElements):
             ks = []
             if 'affine' in r:
                 ks.append(be.kernel(
-                    'tflux', tplargs=tplargs | {'ktype': 'affine'},
+                    'tflux', tplargs=tplargs | self.base_tpl | {'ktype': 'affine'},
                     dims=[self.nupts, self.neles], u=self.scal_upts[uin],
                     f=self._vect_upts, smats=self._smats_affine))
             if 'curved' in r:
                 ks.append(be.kernel(
-                    'tflux', tplargs=tplargs | {'ktype': 'curved'},
+                    'tflux', tplargs=tplargs | self.base_tpl | {'ktype': 'curved'},
                     dims=[self.nupts, r['curved']],
                     u=s(self.scal_upts[uin], 'curved'),
                     f=s(self._vect_upts, 'curved'), smats=self._smats_curved))
             if 'linear' in r:
                 ks.append(be.kernel(
-                    'tflux', tplargs=tplargs | {'ktype': 'linear'},
+                    'tflux', tplargs=tplargs | self.base_tpl | {'ktype': 'linear'},
                     dims=[self.nupts, r['linear']],
                     u=s(self.scal_upts[uin], 'linear'),
                     f=s(self._vect_upts, 'linear'), verts=self._verts_linear,
@@ -495,48 +496,42 @@ class EulerElements(FluidElements):
         ks = [self._qptsu(uin)]
         if 'affine' in r:
             ks.append(be.kernel(
-                'tflux', tplargs=tplargs | {'ktype': 'affine'},
+                'tflux', tplargs=tplargs | self.base_tpl | {'ktype': 'affine'},
                 dims=[nq, self.neles], u=self._scal_qpts, f=self._vect_qpts,
                 smats=self._smats_affine))
         if 'curved' in r:
             ks.append(be.kernel(
-                'tflux', tplargs=tplargs | {'ktype': 'curved'},
+                'tflux', tplargs=tplargs | self.base_tpl | {'ktype': 'curved'},
                 dims=[nq, r['curved']], u=s(self._scal_qpts, 'curved'),
                 f=s(self._vect_qpts, 'curved'), smats=self._smats_curved_q))
         if 'linear' in r:
             ks.append(be.kernel(
-                'tflux', tplargs=tplargs | {'ktype': 'linear'},
+                'tflux', tplargs=tplargs | self.base_tpl | {'ktype': 'linear'},
                 dims=[nq, r['linear']], u=s(self._scal_qpts, 'linear'),
                 f=s(self._vect_qpts, 'linear'), verts=self._verts_linear,
                 upts=self._qpts_const))
         return be.ordered_meta_kernel(ks)
 
+    def _xa_mats(self, xa):
+        """Stage-update banks of a fused epilogue: none, one bank index, or
+        the RK4 pair (r1, r0) of bank indices."""
+        if xa is None:
+            return None
+        if isinstance(xa, tuple):
+            return tuple(self.scal_upts[i] for i in xa)
+        return self.scal_upts[xa]
+
     def _add_fused_kernels(self):
-        """Fused tdivtpcorf + tdivtconf + negdivconf (+ RK45 stage): one
-        pass over the element's flux and common normal flux."""
+        """Fused tdivtpcorf + tdivtconf + negdivconf (+ RK stage update):
+        one pass over the element's flux and common normal flux."""
         be = self._be
 
         def tdivf(fout, xa=None, stage=None):
             return be.kernel(
                 'tdivf', self.opmat(self._divop), self.opmat('M3'),
                 self._flux_buf, self._scal_fpts, self.scal_upts[fout],
-                self._rcpdjac_upts,
-                xa=None if xa is None else self.scal_upts[xa], stage=stage)
+                self._rcpdjac_upts, xa=self._xa_mats(xa), stage=stage)
         self.kernels['tdivf'] = tdivf
-
-        # Euler hexes: RK4 stage updates in the divergence epilogue (tdisf
-        # stays its own kernel); builds raise NotSuitableError where the
-        # operators have no generated kernel
-        if (self.name == 'hex' and type(self) is EulerElements and
-                not self.fluxaa):
-            def tdivf_rk4(stage, uin, nxt, r1, r0):
-                sb = self.scal_upts
-                return [self.kernels['tdisf'](uin), be.kernel(
-                    'tdivf', self.opmat(self._divop), self.opmat('M3'),
-                    self._flux_buf, self._scal_fpts, sb[nxt],
-                    self._rcpdjac_upts, xa=(sb[r1], sb[r0]),
-                    stage=f'rk4-{stage}')]
-            self.kernels['tdivf_rk4'] = tdivf_rk4
 
         # straight-sided elements, Euler: tdisf folded in as well
         if (self.affine and type(self) is EulerElements and be.fuse_flux and
@@ -549,21 +544,8 @@ class EulerElements(FluidElements):
                     self.opmat('M3'), self.scal_upts[uin],
                     self._smats_affine, self._scal_fpts,
                     self.scal_upts[fout], self._rcpdjac_upts, gamma,
-                    xa=None if xa is None else self.scal_upts[xa],
-                    stage=stage)
+                    xa=self._xa_mats(xa), stage=stage)
             self.kernels['tdivf_flux'] = tdivf_flux
-
-            def tdivf_flux_rk4(stage, uin, nxt, r1, r0):
-                """RK4 stage `stage` (1..4) with its update fused: u in
-                uin, the next stage's input to nxt, r1 and r0 the
-                accumulator and the step's initial state."""
-                sb = self.scal_upts
-                return [be.kernel(
-                    'tdivf_flux', self.opmat('M1 - M3*M2'),
-                    self.opmat('M3'), sb[uin], self._smats_affine,
-                    self._scal_fpts, sb[nxt], self._rcpdjac_upts, gamma,
-                    xa=(sb[r1], sb[r0]), stage=f'rk4-{stage}')]
-            self.kernels['tdivf_rk4'] = tdivf_flux_rk4
 
 
 class NavierStokesElements(EulerElements):
@@ -631,13 +613,13 @@ class NavierStokesElements(EulerElements):
             ks = []
             if 'affine' in r:
                 ks.append(be.kernel(
-                    'tflux', tplargs=tplargs | {'ktype': 'affine-fused'},
+                    'tflux', tplargs=tplargs | self.base_tpl | {'ktype': 'affine-fused'},
                     dims=[self.nupts, self.neles], u=self.scal_upts[uin],
                     f=self._vect_upts, gradu=self._grad_upts,
                     smats=self._smats_affine, rcpdjac=self._rcpdjac_upts))
             if 'curved' in r:
                 ks.append(be.kernel(
-                    'tflux', tplargs=tplargs | {'ktype': 'curved-fused'},
+                    'tflux', tplargs=tplargs | self.base_tpl | {'ktype': 'curved-fused'},
                     dims=[self.nupts, r['curved']],
                     u=s(self.scal_upts[uin], 'curved'),
                     f=s(self._vect_upts, 'curved'),
@@ -646,7 +628,7 @@ class NavierStokesElements(EulerElements):
                     rcpdjac=self._rcpdjac_curved(), upts=self._upts_const))
             if 'linear' in r:
                 ks.append(be.kernel(
-                    'tflux', tplargs=tplargs | {'ktype': 'linear-fused'},
+                    'tflux', tplargs=tplargs | self.base_tpl | {'ktype': 'linear-fused'},
                     dims=[self.nupts, r['linear']],
                     u=s(self.scal_upts[uin], 'linear'),
                     f=s(self._vect_upts, 'linear'),
@@ -702,17 +684,17 @@ class NavierStokesElements(EulerElements):
         ks.append(self._qptsu(uin))
         if 'affine' in r:
             ks.append(be.kernel(
-                'tflux', tplargs=tplargs | {'ktype': 'affine'},
+                'tflux', tplargs=tplargs | self.base_tpl | {'ktype': 'affine'},
                 dims=[nq, self.neles], u=self._scal_qpts, f=self._vect_qpts,
                 smats=self._smats_affine))
         if 'curved' in r:
             ks.append(be.kernel(
-                'tflux', tplargs=tplargs | {'ktype': 'curved'},
+                'tflux', tplargs=tplargs | self.base_tpl | {'ktype': 'curved'},
                 dims=[nq, r['curved']], u=s(self._scal_qpts, 'curved'),
                 f=s(self._vect_qpts, 'curved'), smats=self._smats_curved_q))
         if 'linear' in r:
             ks.append(be.kernel(
-                'tflux', tplargs=tplargs | {'ktype': 'linear'},
+                'tflux', tplargs=tplargs | self.base_tpl | {'ktype': 'linear'},
                 dims=[nq, r['linear']], u=s(self._scal_qpts, 'linear'),
                 f=s(self._vect_qpts, 'linear'), verts=self._verts_linear,
                 upts=self._qpts_const))
@@ -1207,6 +1189,9 @@ class BaseSystem:
         for etype, spts in mesh.spts.items():
             elemap[etype] = self.elementscls(SHAPES[etype], spts, cfg)
         ics = {et: e.set_ics_from_cfg() for et, e in elemap.items()}
+        self.base = self._base_state(backend, mesh, cfg, elemap, ics)
+        if self.base is not None:
+            ics = {et: u - self.base[None, :, None] for et, u in ics.items()}
         # curved elements come first; the linear region starts after the
         # last one (pyfr/solvers/base/system.py:241-246)
         curved = getattr(mesh, 'spts_curved', {})
@@ -1247,6 +1232,48 @@ class BaseSystem:
                 self.bcinterscls(backend, faces, elemap, sect, cfg))
         self._commit()
         self._kcache = {}
+        if self.base is not None:
+            b = (float(self.base[0]), float(self.base[4]))
+            for e in elemap.values():
+                e.base_tpl = {'base': b}
+            for i in self._int_inters:
+                i._tplargs['base'] = b
+
+    @staticmethod
+    def _base_state(backend, mesh, cfg, elemap, ics):
+        """The constant state u0 = (rho0, 0, 0, 0, E0) of the base-state
+        form, or None.  ``[backend-hip] base-state``: 'auto' (default) uses
+        it for the precision form ('mixed') where it applies, 'on' in any
+        precision (or raises), 'off' never.  It applies to hex meshes
+        without boundaries, the Rusanov solver and no flux anti-aliasing.
+        rho0 and p0 = (gamma - 1) E0 are the means of the initial density
+        and pressure over the solution points.  No counterpart in the
+        reference."""
+        opt = cfg.get('backend-hip', 'base-state', 'auto')
+        if opt not in {'auto', 'on', 'off'}:
+            raise ValueError('base-state must be auto, on or off')
+        if opt == 'off' or (opt == 'auto' and
+                            getattr(backend, 'precision', '') != 'mixed'):
+            return None
+        why = None
+        if any(e.name != 'hex' for e in elemap.values()):
+            why = 'hex elements only'
+        elif getattr(mesh, 'bcon', {}):
+            why = 'meshes without boundaries only'
+        elif cfg.get('solver-interfaces', 'riemann-solver') != 'rusanov':
+            why = 'the Rusanov solver only'
+        elif any(e.fluxaa for e in elemap.values()):
+            why = 'no flux anti-aliasing'
+        if why:
+            if opt == 'on':
+                raise ValueError(f'The base-state form applies to {why}')
+            return None
+        gamma = cfg.getfloat('constants', 'gamma')
+        u = np.concatenate([v.transpose(1, 0, 2).reshape(5, -1)
+                            for v in ics.values()], axis=1)
+        rho0 = float(np.mean(u[0]))
+        p = (gamma - 1)*(u[4] - 0.5*(u[1]**2 + u[2]**2 + u[3]**2)/u[0])
+        return np.array([rho0, 0.0, 0.0, 0.0, float(np.mean(p))/(gamma - 1)])
 
     def _commit(self):
         # make_field_views for scal_fpts (baseadvec/system.py:20-55)
@@ -1334,92 +1361,123 @@ class BaseSystem:
                 k.bind(dta=dta, dtb=dtb, last=int(last))
             self.backend.run_graph(g)
 
-    @property
-    def fused_rk4(self):
-        """True if RK4 stage updates can be fused into the RHS: every
-        element type on the flux-fused divergence kernel (tets, prisms) or
-        on a generated fused divergence kernel (hexes), tried once."""
-        if not hasattr(self, '_fused_rk4_ok'):
+    def _fused_rk4_ok(self, mixed):
+        """True if every element type's RHS can end in a fused divergence
+        kernel with the RK4 stage epilogues (the generated hex kernels, the
+        flux-fused tet/prism kernel; mixed: the precision form's, hexes
+        only); tried once on banks 2, 3 (stage input, next input) and
+        (1, 0) (accumulator, initial state)."""
+        key = ('_rk4ok', mixed)
+        if key not in self._kcache:
             from pyfr_amd.backend import NotSuitableError
-            ok = (self.backend.fused and len(self.ele_banks[0]) >= 4 and
-                  all('tdivf_rk4' in e.kernels
-                      for e in self.ele_map.values()))
+            ok = self.backend.fused and len(self.ele_banks[0]) >= 4
             try:
-                for e in self.ele_map.values() if ok else ():
-                    e.kernels['tdivf_rk4'](1, 0, 2, 1, 0)
+                if ok:
+                    self._fused_kernels(2, 3, (1, 0),
+                                        'rk4m-1' if mixed else 'rk4-1')
             except NotSuitableError:
                 ok = False
-            self._fused_rk4_ok = ok
-        return self._fused_rk4_ok
+            self._kcache[key] = ok
+        return self._kcache[key]
 
-    def rhs_rk4_stage(self, t, stage, uin, nxt, r1, r0, dt):
+    @property
+    def fused_rk4(self):
+        return self._fused_rk4_ok(False)
+
+    @property
+    def fused_rk4_mixed(self):
+        return self._fused_rk4_ok(True)
+
+    def rhs_rk4_stage(self, t, stage, uin, nxt, r1, r0, dt, mixed=False):
         """RHS of RK4 stage `stage` at t with the stage's register updates
-        in the divergence kernel's epilogue (``steppers.py:67-108``)."""
-        key = ('rk4', stage, uin, nxt, r1, r0)
-        if key not in self._kcache:
-            eles = list(self.ele_map.values())
-            ks = [e.kernels['disu'](uin) for e in eles]
-            ks += [i.kernels['comm_flux']() for i in self._inters]
-            ks += [k for e in eles
-                   for k in e.kernels['tdivf_rk4'](stage, uin, nxt, r1, r0)]
+        in the divergence kernel's epilogue (``steppers.py:67-108``; mixed:
+        the precision form's FP64 updates).  dt is a kernel argument of the
+        captured graph, so a few graphs are kept per stage, one per dt (the
+        last steps before an output time re-split dt), each captured
+        once."""
+        key = ('rk4', stage, uin, nxt, r1, r0, mixed)
+        byd = self._kcache.setdefault(key, {})
+        if dt not in byd:
+            ks = self._fused_kernels(uin, nxt, (r1, r0),
+                                     f'rk4{"m" if mixed else ""}-{stage}')
             g = self.backend.graph()
             g.add_all(ks)
             g.commit()
-            g.stage_kernels = [k for k in ks if 'dtb' in k.rtargs]
-            self._kcache[key] = [g]
+            for k in ks:
+                if 'dtb' in k.rtargs:
+                    k.bind(dta=dt)
+            if len(byd) >= 4:
+                del byd[next(iter(byd))]
+            byd[dt] = g
         self._set_time(t, uin)
-        for g in self._kcache[key]:
-            for k in g.stage_kernels:
-                k.bind(dta=dt)
-            self.backend.run_graph(g)
+        self.backend.run_graph(byd[dt])
 
     def ele_scal_upts(self, idx):
+        """The bank's arrays as stored (u - u0 for a solution bank of the
+        base-state form)."""
         return [b[idx].get() for b in self.ele_banks]
 
-    # -- FP64 reductions over register banks (replaces the reference's
-    #    reduction kernels as used by its residual/integrate plugins)
-    def _reduce(self, idx, op, weighted):
-        import ctypes as ct
-        from pyfr_amd.backend import _dtcode
+    def soln_upts(self, idx):
+        """The solution in register idx (u0 added back in the base-state
+        form)."""
+        out = self.ele_scal_upts(idx)
+        if self.base is not None:
+            out = [a.astype(np.float64) + self.base[None, :, None]
+                   for a in out]
+        return out
+
+    def set_soln_upts(self, idx, arrs):
+        if self.base is not None:
+            arrs = [a - self.base[None, :, None] for a in arrs]
+        self.set_ele_scal_upts(idx, arrs)
+
+    # -- FP64 reductions over register banks, through the backend's
+    #    reduction provider (the reference's residual/integrate plugins use
+    #    the same kernel, pyfr/plugins/residual.py, nancheck.py:19-39)
+    def _reduce(self, idx, kind):
+        """kind 'sum', 'sumsq' (quadrature-weighted) or 'max' (of |u|)
+        over solution register idx; the base state is added back (pvars
+        b), so the results are those of u."""
         be = self.backend
-        if not hasattr(self, '_red_out'):
-            p = ct.c_void_p()
-            be.lib.malloc(ct.byref(p), 8*64)
-            self._red_out = p.value
-            self._red_w = {}
-        out = []
-        for et, banks in zip(self.ele_map, self.ele_banks):
-            m = banks[idx]
-            e = self.ele_map[et]
-            w = None
-            if weighted:
+        if not hasattr(self, '_red_kerns'):
+            self._red_kerns, self._red_w = {}, {}
+        key = (idx, kind)
+        if key not in self._red_kerns:
+            ks = []
+            pv = {} if self.base is None else {'b': list(self.base)}
+            x = 'x' if self.base is None else '(x + b)'
+            for et, banks in zip(self.ele_map, self.ele_banks):
+                m, e = banks[idx], self.ele_map[et]
+                if kind == 'max':
+                    ks.append(be.kernel('reduction', 'max', [f'fabs({x})'],
+                                        {'x': m}, pvars=pv))
+                    continue
                 if (et, m.dtype) not in self._red_w:
+                    # quadrature weights times |J| at the solution points
                     wts = (e.basis.upts_wts[:, None] /
                            e.rcpdjac_at_np('upts'))
                     self._red_w[(et, m.dtype)] = be.const_matrix(
                         wts, dtype=m.dtype, tags={'align'})
-                w = self._red_w[(et, m.dtype)]
-            be.lib.reduce(_dtcode(m.dtype), op, m.nrow, e.nvars, e.neles,
-                          m.leaddim, m.data, None if w is None else w.data,
-                          0 if w is None else w.leaddim, self._red_out,
-                          be.stream)
-            res = np.empty(e.nvars)
-            be.lib.memcpy_dtoh(res.ctypes.data, self._red_out, 8*e.nvars,
-                               be.stream)
-            out.append(res)
-        return out
+                expr = f'w*{x}' if kind == 'sum' else f'w*{x}*{x}'
+                ks.append(be.kernel('reduction', 'sum', [expr],
+                                    {'x': m, 'w': self._red_w[(et, m.dtype)]},
+                                    pvars=pv))
+            self._red_kerns[key] = ks
+        ks = self._red_kerns[key]
+        be.run_kernels(ks, wait=True)
+        return [k.result[0] for k in ks]
 
     def conservation_sums(self, idx):
         """Per-variable integral of the bank over the domain (sum of
         w_u |J| u), FP64 accumulation."""
-        return np.sum(self._reduce(idx, 0, True), axis=0)
+        return np.sum(self._reduce(idx, 'sum'), axis=0)
 
     def l2_norms(self, idx):
         """Per-variable quadrature L2 norm of the bank."""
-        return np.sqrt(np.sum(self._reduce(idx, 2, True), axis=0))
+        return np.sqrt(np.sum(self._reduce(idx, 'sumsq'), axis=0))
 
     def max_abs(self, idx):
-        return np.max(self._reduce(idx, 1, False), axis=0)
+        return np.max(self._reduce(idx, 'max'), axis=0)
 
     def set_ele_scal_upts(self, idx, arrs):
         for b, a in zip(self.ele_banks, arrs):
@@ -1601,7 +1659,7 @@ class BaseIntegrator:
 
     @property
     def soln(self):
-        return self.system.ele_scal_upts(self.idxcurr)
+        return self.system.soln_upts(self.idxcurr)
 
 
 class RK4Integrator(BaseIntegrator):
@@ -1645,6 +1703,56 @@ class RK4Integrator(BaseIntegrator):
         add(0.0, r2, dt, r3, 1.0, r0)
         rhs(t + dt, r2, r3)
         add(1.0, r1, dt/6.0, r3)
+        return r1
+
+
+class MixedRK4Integrator(BaseIntegrator):
+    """RK4 for the precision form: r0 (the step's initial state) and r1
+    (the accumulator, then the new state) in FP64, the stage inputs and
+    stage RHS in two FP32 arrays.  The updates are those of
+    ``steppers.py:67-108`` formed in FP64 from the FP32 stage RHS; each
+    stage's update writes the next stage's FP32 input, and the last writes
+    the new state rounded to FP32, the next step's first input.  On hexes
+    they run in the divergence kernel's epilogue, elsewhere in
+    ``frhip_rk4_mixed`` after the RHS."""
+    nregs = 2
+    nwork = 2
+    nrhs_per_step = 4
+
+    def __init__(self, *args, **kwargs):
+        super().__init__(*args, **kwargs)
+        self._work_valid = False
+
+    def _sync_work(self, r0):
+        lib, be = self.backend.lib, self.backend
+        for banks in self.system.ele_banks:
+            acc, w = banks[r0], banks[2]
+            lib.convert(1, w.data, 0, acc.data, acc.nrow*acc.leaddim,
+                        be.stream)
+        self._work_valid = True
+
+    def step(self, t, dt):
+        r0, r1 = self.idxcurr, 1 - self.idxcurr
+        wa, wb = 2, 3
+        if not self._work_valid:
+            self._sync_work(r0)
+        if self.system.fused_rk4_mixed:
+            st = self.system.rhs_rk4_stage
+            st(t, 1, wa, wb, r1, r0, dt, mixed=True)
+            st(t + dt/2.0, 2, wb, wa, r1, r0, dt, mixed=True)
+            st(t + dt/2.0, 3, wa, wb, r1, r0, dt, mixed=True)
+            st(t + dt, 4, wb, wa, r1, r0, dt, mixed=True)
+            self.nrhsevals += 4
+            return r1
+        lib, be = self.backend.lib, self.backend
+        for stage, ts in ((1, t), (2, t + dt/2.0), (3, t + dt/2.0),
+                          (4, t + dt)):
+            self._rhs(ts, wa, wb)
+            for banks in self.system.ele_banks:
+                a = banks[r1]
+                lib.rk4_mixed(a.nrow, a.ncol, a.leaddim, stage, a.data,
+                              banks[r0].data, banks[wb].data, banks[wa].data,
+                              dt, be.stream)
         return r1
 
 
@@ -1745,6 +1853,7 @@ class MixedRK45Integrator(RK45Integrator):
 
 
 INTEGRATORS = {'rk4': RK4Integrator, 'rk45': RK45Integrator}
+MIXED_INTEGRATORS = {'rk4': MixedRK4Integrator, 'rk45': MixedRK45Integrator}
 
 
 def get_integrator(backend, mesh, cfg):
@@ -1752,8 +1861,8 @@ def get_integrator(backend, mesh, cfg):
     if cfg.get('solver-time-integrator', 'controller', 'none') != 'none':
         raise ValueError('Only the fixed-dt "none" controller is supported')
     if getattr(backend, 'precision', 'double') == 'mixed':
-        if scheme != 'rk45':
+        if scheme not in MIXED_INTEGRATORS:
             raise NotImplementedError('The mixed precision form is '
-                                      'implemented for rk45')
-        return MixedRK45Integrator(backend, mesh, cfg)
+                                      'implemented for rk4 and rk45')
+        return MIXED_INTEGRATORS[scheme](backend, mesh, cfg)
     return INTEGRATORS[scheme](backend, mesh, cfg)

@@ -93,3 +93,82 @@ def test_depth_limit():
     compile_exprs([right], {})              # 16 slots
     with pytest.raises(ExprError):
         compile_exprs([f'(1.0 + {right})'], {})
+
+
+def _run_vars(code, offs, pool, var):
+    """The stack machine over an arbitrary variable list (reduction
+    programs)."""
+    out = []
+    for r in range(len(offs) - 1):
+        stk = []
+        for op, arg in code[offs[r]:offs[r + 1]]:
+            if op == 0:
+                stk.append(pool[arg] + 0*var[0])
+            elif op == 1:
+                stk.append(var[arg])
+            elif op in F2:
+                b, a = stk.pop(), stk.pop()
+                stk.append(F2[op](a, b))
+            elif op == 10:
+                stk.append(-stk.pop())
+            else:
+                stk.append(F1[op](stk.pop()))
+        assert len(stk) == 1
+        out.append(stk[0])
+    return out
+
+
+# expressions the reference's plugins and controllers hand to the reduction
+# provider (pyfr/plugins/nancheck.py:21, pyfr/integrators/controllers.py:
+# 107-113, pyfr/integrators/implicit/base.py:83-120)
+RED_EXPRS = ['x', 'x*x', 'fabs(x)', 'err / (atol + rtol*fabs(curr))',
+             '(err / (atol + rtol*fabs(curr)))*(err / (atol + rtol*fabs(curr)))',
+             '(x/w)*(x/w)', 'a*b', 'fmax(x, w) - fmin(a, b)']
+
+
+def test_reduction_programs_match_numpy():
+    rng = np.random.default_rng(3)
+    names = ['x', 'w', 'a', 'b', 'err', 'curr', 'atol', 'rtol']
+    vals = [rng.uniform(0.5, 2.0, 40) for _ in names]
+    code, offs, pool = compile_exprs(
+        RED_EXPRS, {}, varmap={n: i for i, n in enumerate(names)})
+    got = _run_vars(code, offs, pool, vals)
+    env = dict(zip(names, vals), fabs=np.abs, fmax=np.maximum,
+               fmin=np.minimum)
+    for e, g in zip(RED_EXPRS, got):
+        ref = eval(e, {'__builtins__': {}}, env)
+        assert np.allclose(g, ref, rtol=1e-15, atol=0), e
+
+
+@pytest.mark.parametrize('op', [-1, -7, 22, 1000])
+def test_library_refuses_invalid_opcodes(op):
+    """Both program constructors reject opcodes outside the table before
+    any device call (so this runs without a GPU)."""
+    import ctypes as ct
+
+    from pyfr_amd._lib import FrHip, FrHipError
+    lib = FrHip()
+    # push x, push x, <op>: valid as a binary op if op were in range
+    code = np.array([[1, 0], [1, 0], [op, 0]], dtype=np.int32)
+    offs = np.array([0, 3], dtype=np.int32)
+    pool = np.zeros(1)
+    p = ct.c_void_p()
+    with pytest.raises(FrHipError, match='invalid opcode'):
+        lib.bc_program_create(ct.byref(p), 1, code.ctypes.data,
+                              offs.ctypes.data, pool.ctypes.data, 1)
+    with pytest.raises(FrHipError, match='invalid opcode'):
+        lib.expr_program_create(ct.byref(p), 1, 2, code.ctypes.data,
+                                offs.ctypes.data, pool.ctypes.data, 1)
+
+
+def test_library_refuses_bad_variable_index():
+    import ctypes as ct
+
+    from pyfr_amd._lib import FrHip, FrHipError
+    lib = FrHip()
+    code = np.array([[1, 3]], dtype=np.int32)      # variable 3 of 3
+    offs = np.array([0, 1], dtype=np.int32)
+    p = ct.c_void_p()
+    with pytest.raises(FrHipError, match='variable index'):
+        lib.expr_program_create(ct.byref(p), 1, 3, code.ctypes.data,
+                                offs.ctypes.data, None, 0)

@@ -184,40 +184,3 @@ def test_graph_replay_matches_eager(backend_factory):
         outs.append(res)
     for a, b in zip(*outs):
         assert np.array_equal(a, b)
-
-
-def test_mixed_precision_rk45(backend_factory):
-    """Precision form: FP32 RHS arithmetic with FP64 RK state.  Relative L2
-    error of the whole state against the FP64 run after 20 RK45 steps
-    <= 1e-5 (per-field errors are printed; the w-momentum field starts at
-    zero, so its own relative error is not a meaningful bar this early)."""
-    from pyfr_amd.mesh import BoxMesh
-    from pyfr_amd.solvers import MixedRK45Integrator, get_integrator
-    sols = {}
-    for prec, fused in (('double', False), ('mixed', True),
-                        ('mixed', False)):
-        cfg = make_cfg('navier-stokes', 3, scheme='rk45', dt=2e-4,
-                       precision=prec)
-        be = backend_factory(precision=prec, fused=fused)
-        cfg.set('backend-hip', 'fused', str(fused))
-        integ = get_integrator(be, BoxMesh(4), cfg)
-        assert isinstance(integ, MixedRK45Integrator) == (prec == 'mixed')
-        integ.run_steps(20)
-        sols[prec, fused] = integ.soln[0]
-        if prec == 'mixed':
-            # FP64 conservation sums of the FP64 state
-            cs = integ.system.conservation_sums(0)
-            w = integ.system.ele_map['hex'].basis.upts_wts
-            rj = integ.system.ele_map['hex'].rcpdjac_at_np('upts')
-            ref = np.einsum('u,uve->v', w, sols[prec, fused]/rj[:, None, :])
-            assert np.allclose(cs, ref, rtol=1e-12, atol=1e-9)
-    b = sols['double', False]
-    # fused and unfused mixed forms agree to rounding
-    assert rel_l2(sols['mixed', True], sols['mixed', False]) < 1e-6
-    a = sols['mixed', True]
-    for v in range(5):
-        print(f'[mixed] field {v} rel L2 {rel_l2(a[:, v], b[:, v]):.2e}',
-              flush=True)
-    err = rel_l2(a, b)
-    print(f'[mixed] state rel L2 {err:.2e}', flush=True)
-    assert err < 1e-5

@@ -58,14 +58,10 @@ def _dev(lib, arr):
     return p
 
 
-@pytest.mark.parametrize('name', list(SOLVERS))
-@pytest.mark.parametrize('jump', [0.05, 1.0, 25.0])
-def test_intcflux_pointwise(backend_factory, name, jump):
-    from oracle.physics import RSOLVERS
-    be = backend_factory()
-    rng = np.random.default_rng(int(jump*100) + SOLVERS[name])
-    N = 4096
-    ul, ur, nl = _pairs(rng, N, jump)
+def _device_flux(be, name, ul, ur, nl):
+    """The intcflux kernel (FP64) on N state pairs (N a multiple of 64):
+    the flux written to the left side, and that of the right side."""
+    N = ul.shape[1]
     nb = N//64
 
     def aosoa(x):
@@ -78,12 +74,12 @@ def test_intcflux_pointwise(backend_factory, name, jump):
     lib = be.lib
     host = np.concatenate([aosoa(ul), aosoa(ur)])
     bufs = [_dev(lib, host), _dev(lib, soa.astype(np.int32)),
-            _dev(lib, (soa + nb*320).astype(np.int32)), _dev(lib, nl)]
+            _dev(lib, (soa + nb*320).astype(np.int32)),
+            _dev(lib, np.ascontiguousarray(nl))]
     base, lm, rm, nt = bufs
-    print(f'\n[riemann] {name} jump={jump}', flush=True)
     try:
         lib.intcflux_euler(0, SOLVERS[name], N, base, lm, rm, nt, N, 1.4,
-                           None)
+                           None, None)
         lib.device_sync()
         out = np.empty_like(host)
         lib.memcpy_dtoh(out.ctypes.data, base, out.nbytes, None)
@@ -93,6 +89,19 @@ def test_intcflux_pointwise(backend_factory, name, jump):
             lib.free(b)
     got_l = np.stack([out[soa + 64*v] for v in range(5)])
     got_r = np.stack([out[nb*320 + soa + 64*v] for v in range(5)])
+    return got_l, got_r
+
+
+@pytest.mark.parametrize('name', list(SOLVERS))
+@pytest.mark.parametrize('jump', [0.05, 1.0, 25.0])
+def test_intcflux_pointwise(backend_factory, name, jump):
+    from oracle.physics import RSOLVERS
+    be = backend_factory()
+    rng = np.random.default_rng(int(jump*100) + SOLVERS[name])
+    N = 4096
+    ul, ur, nl = _pairs(rng, N, jump)
+    print(f'\n[riemann] {name} jump={jump}', flush=True)
+    got_l, got_r = _device_flux(be, name, ul, ur, nl)
     mag = np.sqrt((nl**2).sum(0))
     with np.errstate(all='ignore'):
         ref = mag*RSOLVERS[name](ul, ur, nl/mag, 1.4)
@@ -102,3 +111,38 @@ def test_intcflux_pointwise(backend_factory, name, jump):
     print(f'[riemann] max err / max flux {err:.2e}', flush=True)
     assert err < 1e-11
     assert np.array_equal(got_r, -got_l)
+
+
+@pytest.mark.parametrize('name,U', [('hllc', 0.0), ('roe', 0.0),
+                                    ('roem', 0.0), ('exact', 0.0),
+                                    ('hllc', 0.3), ('hllc', -0.3),
+                                    ('exact', 0.3), ('exact', -0.3)])
+def test_intcflux_contacts(backend_factory, name, U):
+    """Closed forms on the device, independent of the oracle: an isolated
+    contact (equal pressure and normal velocity U, density and shear
+    jumps) has the exact interface flux (0, p n, 0) at rest and f(ul).n or
+    f(ur).n when moving subsonically; HLLC, Roe, RoeM (at rest) and the
+    exact solver reproduce it."""
+    from oracle.physics import inviscid_flux
+    be = backend_factory()
+    rng = np.random.default_rng(31)
+    N = 1024
+    n = rng.normal(size=(3, N))
+    n /= np.linalg.norm(n, axis=0)
+    p = rng.uniform(0.5, 2.0, N)
+    st = []
+    for _ in range(2):
+        rho = rng.uniform(0.3, 3.0, N)
+        t = rng.normal(0, 0.5, (3, N)) if U else np.zeros((3, N))
+        t -= (t*n).sum(0)*n
+        v = U*n + t
+        st.append(np.stack([rho, *(rho*v), p/0.4 + 0.5*rho*(v**2).sum(0)]))
+    ul, ur = st
+    scale = rng.uniform(0.5, 2.0, N)
+    got, _ = _device_flux(be, name, ul, ur, n*scale)
+    if U == 0.0:
+        ref = np.concatenate([np.zeros((1, N)), p*n, np.zeros((1, N))])
+    else:
+        f, _, _ = inviscid_flux(ul if U > 0 else ur, 1.4)
+        ref = (n[:, None, :]*f).sum(0)
+    assert np.abs(got - scale*ref).max() < 1e-12*np.abs(scale*ref).max()

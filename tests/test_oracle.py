@@ -352,3 +352,111 @@ def test_exact_solver_star_states(rho_l, u_l, p_l, rho_r, u_r, p_r, w):
     assert np.allclose((f[1:4] - (f[1:4] @ n)*n), f[0]*vt, rtol=1e-10,
                        atol=1e-10*abs(f[0]))
     assert np.allclose(got, want, rtol=2e-5, atol=2e-5*np.abs(want).max())
+
+
+def _contact_states(rng, N, U):
+    """Left/right states of an isolated contact moving at normal speed U:
+    equal pressure and normal velocity, different densities and tangential
+    velocities.  Returns (ul, ur, n, p)."""
+    n = rng.normal(size=(3, N))
+    n /= np.linalg.norm(n, axis=0)
+    p = rng.uniform(0.5, 2.0, N)
+    out = []
+    for _ in range(2):
+        rho = rng.uniform(0.3, 3.0, N)
+        t = rng.normal(0, 0.5, (3, N))
+        t -= (t*n).sum(0)*n                  # tangential part only
+        v = U*n + t
+        out.append(np.stack([rho, *(rho*v),
+                             p/0.4 + 0.5*rho*(v**2).sum(0)]))
+    return out[0], out[1], n, p
+
+
+@pytest.mark.parametrize('name', ['hllc', 'roe', 'roem', 'exact'])
+def test_stationary_contact_is_exact(name):
+    """A stationary contact (u = 0, equal p, any density jump) is a steady
+    solution of the Euler equations: the exact interface flux is
+    (0, p n, 0).  HLLC, Roe and exact resolve it exactly; RoeM to its
+    1e-15 Mach offset."""
+    from oracle.physics import RSOLVERS
+    rng = np.random.default_rng(21)
+    ul, ur, n, p = _contact_states(rng, 200, 0.0)
+    ul[1:4] = ur[1:4] = 0.0
+    ul[4] = ur[4] = p/0.4
+    got = RSOLVERS[name](ul, ur, n, 1.4)
+    ref = np.concatenate([np.zeros((1, 200)), p*n, np.zeros((1, 200))])
+    assert np.abs(got - ref).max() < 1e-13
+
+
+def test_rusanov_stationary_contact_closed_form():
+    """Rusanov on a stationary contact: momentum p n, energy 0 and mass
+    a (rho_l - rho_r) with the reference's wave speed
+    a = sqrt(gamma (p_l + p_r) / (4 (rho_l + rho_r))) (no velocity term)."""
+    from oracle.physics import rusanov
+    rng = np.random.default_rng(22)
+    ul, ur, n, p = _contact_states(rng, 200, 0.0)
+    ul[1:4] = ur[1:4] = 0.0
+    ul[4] = ur[4] = p/0.4
+    got = rusanov(ul, ur, n, 1.4)
+    a = np.sqrt(1.4*2*p/(4*(ul[0] + ur[0])))
+    ref = np.concatenate([(a*(ul[0] - ur[0]))[None], p*n,
+                          np.zeros((1, 200))])
+    assert np.abs(got - ref).max() < 1e-13
+
+
+@pytest.mark.parametrize('U', [0.3, -0.3, 0.7, -0.05])
+@pytest.mark.parametrize('name', ['hllc', 'exact'])
+def test_moving_contact_is_upwinded(name, U):
+    """An isolated subsonic contact (with a shear jump) moving at normal
+    speed U: the exact solution at x/t = 0 is the left state for U > 0 and
+    the right state for U < 0, so the flux is f(ul).n or f(ur).n.  HLLC's
+    star states reproduce it exactly (s* = U)."""
+    from oracle.physics import RSOLVERS, inviscid_flux
+    rng = np.random.default_rng(23)
+    ul, ur, n, _ = _contact_states(rng, 200, U)
+    got = RSOLVERS[name](ul, ur, n, 1.4)
+    f, _, _ = inviscid_flux(ul if U > 0 else ur, 1.4)
+    ref = (n[:, None, :]*f).sum(0)
+    assert np.abs(got - ref).max() < 1e-12*(1 + np.abs(ref).max())
+
+
+@pytest.mark.parametrize('old,new', [
+    # plausible mistakes in the star-state formulas
+    ('ul[0]*nvl*(sl - nvl)', 'ul[0]*nvl*(sl + nvl)'),
+    ('(sl - nvl)/(sl - sstar)', '(sl - nvl)/(sl + sstar)'),
+    ('pl/(sl - nvl)', 'pr/(sl - nvl)'),
+    ('usl[0]*(vl[i] + (sstar - nvl)*n[i])', 'usl[0]*(vl[i] + sstar*n[i])'),
+])
+def test_planted_hllc_mistake_is_caught(old, new):
+    """Each mistake planted in a copy of the oracle's HLLC fails one of the
+    pins: the contact checks above, consistency F(u, u, n) = f(u).n or
+    conservation F(ul, ur, n) = -F(ur, ul, -n) on random states."""
+    import inspect
+
+    import oracle.physics as ph
+    src = inspect.getsource(ph.hllc)
+    assert old in src
+    ns = dict(vars(ph))
+    exec(src.replace(old, new), ns)
+    bad = ns['hllc']
+    rng = np.random.default_rng(24)
+    caught = False
+    for U in (0.0, 0.3, -0.3):
+        ul, ur, n, p = _contact_states(rng, 200, U)
+        f, _, _ = ph.inviscid_flux(ul if U >= 0 else ur, 1.4)
+        ref = (n[:, None, :]*f).sum(0)
+        if U == 0.0:
+            ul[1:4] = ur[1:4] = 0.0
+            ul[4] = ur[4] = p/0.4
+            ref = np.concatenate([np.zeros((1, 200)), p*n,
+                                  np.zeros((1, 200))])
+        caught |= np.abs(bad(ul, ur, n, 1.4) - ref).max() > 1e-8
+    ul, ur = _random_states(rng, 300), _random_states(rng, 300)
+    n = rng.normal(size=(3, 300))
+    n /= np.linalg.norm(n, axis=0)
+    f, _, _ = ph.inviscid_flux(ul, 1.4)
+    caught |= np.abs(bad(ul, ul, n, 1.4) -
+                     (n[:, None, :]*f).sum(0)).max() > 1e-8
+    caught |= np.abs(bad(ul, ur, n, 1.4) +
+                     bad(ur, ul, -n, 1.4)).max() > 1e-8
+    assert caught
