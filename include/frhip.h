@@ -439,6 +439,22 @@ int frhip_reduction(void *prog, int dtype, int rop, int nrow, int neles,
 int frhip_reduction_workspace(void *prog, int nrow, int neles, int nvar,
                               long *nwork);
 
+/* frhip_reduction without a workspace argument, for callers that cannot
+ * allocate device memory (code under graph capture, tight serving loops,
+ * harnesses that forbid allocation).  No counterpart in the reference.
+ * The partials live in a library-owned device array of 64*FRHIP_RED_NBLK
+ * doubles (256 KiB per device, held until the process ends), so
+ * nexprs*nvar must be <= 64 (larger calls fail with a message; use
+ * frhip_reduction).  Nothing in it has to start at zero: each call
+ * overwrites the partials it reads.  Calls that overlap on one device
+ * (different streams or threads) race on that array.  Allocates nothing;
+ * results equal frhip_reduction's bit for bit. */
+int frhip_reduction_noalloc(void *prog, int dtype, int rop, int nrow,
+                            int neles, int nvar, int nmat, void *const *mats,
+                            const long *lds, unsigned bcast, const double *pv,
+                            int npv, const double *sv, int nsv, double *out,
+                            void *stream);
+
 /* axnpby with per-variable scales: pyfr/backends/base/blasext.py:9-19
  * (in_scale, in_scale_idxs, out_scale) as rendered by
  * pyfr/backends/hip/kernels/axnpby.mako:4-44 and

@@ -99,6 +99,8 @@ _FUNCS = {
     'frhip_reduction': [_P, _I, _I, _I, _I, _I, _I, _P, _P, ct.c_uint, _P,
                         _I, _P, _I, _P, _L, _P, _P],
     'frhip_reduction_workspace': [_P, _I, _I, _I, ct.POINTER(_L)],
+    'frhip_reduction_noalloc': [_P, _I, _I, _I, _I, _I, _I, _P, _P,
+                                ct.c_uint, _P, _I, _P, _I, _P, _P],
     'frhip_axnpby_scaled': [_I, _I, _L, _L, _I, _I, _P, _P, ct.c_uint, _P,
                             _P, _P],
     'frhip_rk4_mixed': [_I, _L, _L, _I, _P, _P, _P, _P, _D, _P],

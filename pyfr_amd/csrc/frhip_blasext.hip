@@ -13,6 +13,7 @@
 #include "frhip_physics.h"
 
 #include <cfloat>
+#include <mutex>
 
 namespace frhip {
 
@@ -287,6 +288,31 @@ __global__ __launch_bounds__(256) void k_rk4_mixed(
     }
 }
 
+// Scratch of the workspace-free reduction: the partials of up to
+// RED_NOALLOC_MAXOUT (expression, variable) outputs, 256 KiB, held by every
+// device the library runs on until the process ends.  Host code never uses
+// the array's name: each device's address comes from hipGetSymbolAddress
+// with that device current, once (frhip_init, or the first call on the
+// device), and is kept in red_scratch_ptr.
+constexpr int RED_NOALLOC_MAXOUT = 64;
+constexpr int RED_MAXDEV = 64;
+static __device__ __align__(256)
+    double scratch[RED_NOALLOC_MAXOUT*FRHIP_RED_NBLK];
+static double *red_scratch_ptr[RED_MAXDEV];
+static std::mutex red_scratch_mutex;
+
+int red_scratch_init(int device)
+{
+    if (device < 0 || device >= RED_MAXDEV) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(red_scratch_mutex);
+    if (red_scratch_ptr[device]) return hipSuccess;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    red_scratch_ptr[device] = static_cast<double *>(ptr);
+    return hipSuccess;
+}
+
 }  // namespace frhip
 
 using namespace frhip;
@@ -355,6 +381,38 @@ int frhip_reduction(void *prog, int dtype, int rop, int nrow, int neles,
     FR_CHECK(!err, err ? err : "");
     const long need = (long)p->nexprs*nvar*red_nblk((long)nrow*neles);
     FR_CHECK(work && nwork >= need, "reduction workspace too small");
+    if (dtype == FRHIP_F64)
+        red_launch<double>(p, rop, nrow, neles, nvar, nmat, mats, lds, bcast,
+                           pv, npv, sv, nsv, work, out, as_stream(stream));
+    else
+        red_launch<float>(p, rop, nrow, neles, nvar, nmat, mats, lds, bcast,
+                          pv, npv, sv, nsv, work, out, as_stream(stream));
+    FR_LAUNCHED();
+    return 0;
+    FR_API_END
+}
+
+int frhip_reduction_noalloc(void *prog, int dtype, int rop, int nrow,
+                            int neles, int nvar, int nmat, void *const *mats,
+                            const long *lds, unsigned bcast, const double *pv,
+                            int npv, const double *sv, int nsv, double *out,
+                            void *stream)
+{
+    FR_API_BEGIN
+    auto *p = static_cast<ExprProgram *>(prog);
+    const char *err = red_check(p, dtype, rop, nrow, neles, nvar, nmat, mats,
+                                lds, bcast, pv, npv, sv, nsv, out);
+    FR_CHECK(!err, err ? err : "");
+    FR_CHECK((long)p->nexprs*nvar <= RED_NOALLOC_MAXOUT,
+             "frhip_reduction_noalloc: nexprs*nvar exceeds 64, the size of "
+             "its scratch; use frhip_reduction with a workspace");
+    int dev = 0;
+    FR_HIP(hipGetDevice(&dev));
+    FR_CHECK(dev >= 0 && dev < RED_MAXDEV, "device index out of range");
+    // set-up on a device frhip_init was not called for
+    FR_HIP((hipError_t)red_scratch_init(dev));
+    double *work = red_scratch_ptr[dev];
+    FR_CHECK(work, "no scratch address for this device");
     if (dtype == FRHIP_F64)
         red_launch<double>(p, rop, nrow, neles, nvar, nmat, mats, lds, bcast,
                            pv, npv, sv, nsv, work, out, as_stream(stream));

@@ -65,4 +65,9 @@ inline hipStream_t as_stream(void *s) { return static_cast<hipStream_t>(s); }
 
 inline unsigned int cdiv(long a, long b) { return (unsigned int)((a + b - 1) / b); }
 
+// Per-device address of the workspace-free reduction's scratch array
+// (frhip_blasext.hip); frhip_init looks it up for its device.  Returns a
+// hipError_t.
+int red_scratch_init(int device);
+
 }  // namespace frhip

@@ -27,6 +27,7 @@ int frhip_init(int device)
     FR_CHECK(device >= 0 && device < n, "invalid device id");
     FR_HIP(hipSetDevice(device));
     FR_HIP(hipFree(nullptr));
+    FR_HIP((hipError_t)red_scratch_init(device));
     return 0;
     FR_API_END
 }
