@@ -291,11 +291,12 @@ def serving_report(nreq, seed=0, rate=50.0):
     continuous batching and, as the baseline, one solve at a time."""
     from pyfr_amd.backend import AMDBackend
     from pyfr_amd.inifile import Inifile
-    from pyfr_amd.serving import ServingEngine, poisson_requests, summarize
+    from pyfr_amd.serving import (ServingEngine, poisson_requests,
+                                  sequential_bound, summarize)
     cfg = Inifile.from_dict({'backend': {'precision': 'double'}})
     be = AMDBackend(cfg)
     t0 = time.perf_counter()
-    eng = ServingEngine(be)
+    eng = ServingEngine(be, fused=FUSED)
     reqs = poisson_requests(seed, rate, nreq)
     for r in reqs:
         eng.box(r.N, r.order)
@@ -310,13 +311,25 @@ def serving_report(nreq, seed=0, rate=50.0):
     for mode in ('batched', 'sequential'):
         for r in reqs:
             r.admit_time = r.finish_time = r.result = None
+        for p in eng.pools.values():
+            p.ncaptures = p.nsteps_run = 0
+            p.host_s = 0.0
         done = eng.run(reqs, mode=mode)
         out[mode] = summarize(done)
+        steps = sum(p.nsteps_run for p in eng.pools.values())
+        out[mode]['graph_captures'] = sum(p.ncaptures
+                                          for p in eng.pools.values())
+        out[mode]['host_ms_per_pool_step'] = (
+            1e3*sum(p.host_s for p in eng.pools.values())/max(steps, 1))
+        if mode == 'sequential':
+            out['bound'] = sequential_bound(done)
         print(f'# {mode}: {out[mode]}', file=sys.stderr, flush=True)
+    print(f'# bound: {out["bound"]}', file=sys.stderr, flush=True)
     out['throughput_ratio'] = (out['batched']['gdof_rhs_per_s'] /
                                out['sequential']['gdof_rhs_per_s'])
     out['workload'] = (f'poisson-{rate:g}req/s-{nreq}solves-hex-N4..24-'
                        'p1..4-rk4x200-fp64')
+    out['kernels'] = 'fused' if FUSED else 'unfused'
     return out
 
 
@@ -402,7 +415,7 @@ def main():
     ap.add_argument('--serving', action='store_true',
                     help='serving form: continuous batching of a Poisson '
                     'stream of independent solves vs one at a time')
-    ap.add_argument('--serving-requests', type=int, default=48)
+    ap.add_argument('--serving-requests', type=int, default=200)
     ap.add_argument('--counter-probe', action='store_true',
                     help='for rocprofv3 --pmc: one launch of each kernel of '
                     'one RHS (+ stage), then a 1 GiB device copy as the '
@@ -465,7 +478,15 @@ def main():
                 'value': round(sv['sequential']['gdof_rhs_per_s'], 4),
                 'p50_latency_s': round(sv['sequential']['p50_s'], 3),
                 'p99_latency_s': round(sv['sequential']['p99_s'], 3)},
-            'throughput_ratio': round(sv['throughput_ratio'], 3)}
+            'throughput_ratio': round(sv['throughput_ratio'], 3),
+            'kernels': sv['kernels'],
+            'graph_captures': sv['batched']['graph_captures'],
+            'host_ms_per_pool_step': round(
+                sv['batched']['host_ms_per_pool_step'], 3),
+            'bound_from_one_at_a_time': {
+                k: (round(v, 3) if isinstance(v, float) else
+                    {kk: round(vv, 3) for kk, vv in v.items()})
+                for k, v in sv['bound'].items()}}
     if not args.no_dense and args.config == 'cfg1':
         line['dense_mixed_mesh'] = dense_report(args)
     line['cpu_baseline'] = (None if args.no_cpu_baseline or 'ntet' in c
