@@ -51,6 +51,9 @@ MFMA_PEAK_TFS = {'f32': 157.3, 'f64': 78.6}
 
 
 FUSED = True
+# [backend-hip] row-pad of the measured runs (elements; None: the
+# backend's default)
+ROW_PAD = None
 
 
 def make_cfg(c, precision='double'):
@@ -62,6 +65,8 @@ def make_cfg(c, precision='double'):
         cfg = mk(c['system'], c['order'], 'rusanov', precision=precision,
                  scheme=c['scheme'], dt=c['dt'])
     cfg.set('backend-hip', 'fused', str(FUSED))
+    if ROW_PAD is not None:
+        cfg.set('backend-hip', 'row-pad', str(ROW_PAD))
     return cfg
 
 
@@ -412,6 +417,8 @@ def main():
     ap.add_argument('--no-dense', action='store_true',
                     help='skip the dense tet/prism mixed-mesh measurement')
     ap.add_argument('--kernel-reps', type=int, default=5)
+    ap.add_argument('--row-pad', type=int, default=None,
+                    help='[backend-hip] row-pad of the arrays (elements)')
     ap.add_argument('--serving', action='store_true',
                     help='serving form: continuous batching of a Poisson '
                     'stream of independent solves vs one at a time')
@@ -431,8 +438,9 @@ def main():
               'DESIGN.md).', file=sys.stderr)
         sys.exit(2)
 
-    global FUSED
+    global FUSED, ROW_PAD
     FUSED = not args.unfused
+    ROW_PAD = args.row_pad
     c = CONFIGS[args.config]
     if args.accuracy:
         return accuracy_report(c)

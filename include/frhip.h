@@ -47,6 +47,10 @@ int frhip_host_malloc(void **ptr, size_t nbytes);
 int frhip_host_free(void *ptr);
 int frhip_memcpy_htod(void *dst, const void *src, size_t nbytes, void *stream);
 int frhip_memcpy_dtoh(void *dst, const void *src, size_t nbytes, void *stream);
+/* n doubles at ptr set to v by a kernel on `stream` (no memset node when
+ * captured): device-resident run-time arguments such as the RK4 step read
+ * by the fused epilogues. */
+int frhip_fill_f64(void *ptr, long n, double v, void *stream);
 /* Device to page-locked host memory, not waited on (graph-capturable); the
  * data is there once the stream has been synchronised. */
 int frhip_memcpy_dtoh_async(void *dst, const void *src, size_t nbytes,
@@ -114,13 +118,20 @@ int frhip_mul_plan_tune(void *plan, int ncol, const void *b, long ldb,
  *      r1' = r1 + dtb*t); 3 = as 2 with FP32 operands, an FP64 accumulated
  *      solution xa and the FP32 next-stage input written to c; 10..13 =
  *      negdivconf then RK4 stage ep - 9 with four registers, as for
- *      frhip_tdivf_flux_euler (xa = device pair {r1, r0}, dta = dt, c the
- *      next stage's input; structured plans only, the updates of
+ *      frhip_tdivf_flux_euler (xa = device pointers {r1, r0, dt}, dt a
+ *      device double holding the step, so that a captured graph stays
+ *      valid when it changes; dta unused; c the next stage's input;
+ *      structured plans only, the updates of
  *      pyfr/integrators/explicit/steppers.py:67-108); 14..17 = 10..13 for
  *      the precision form (structured FP32 plans; no counterpart in the
  *      reference): r1 and r0 FP64, the updates formed in FP64 in the same
  *      order from the FP32 stage RHS, c the FP32 next stage input, and 17
- *      also stores the new solution rounded to FP32 into c.
+ *      also stores the new solution rounded to FP32 into c; 20..23 = 10..13
+ *      with a time step per element (structured FP64 plans; no counterpart
+ *      in the reference, which has one dt per mesh): xa points to a device
+ *      array of three pointers {r1, r0, dt}, dt holding one value per
+ *      element (a pool of independent solves), and the coefficients are
+ *      dt[e]/2, dt[e]/3, dt[e]/6 as the pool axnpby forms them.
  * They fuse kernels the reference runs back to back on the same element
  * data: tgradpcoru_upts + tgradcoru_upts (pyfr/solvers/baseadvecdiff/
  * elements.py:47-55) and tdivtpcorf + tdivtconf + negdivconf
@@ -164,8 +175,9 @@ int frhip_mul2_plan_set_variant(void *plan, int variant);
  * before any write of c, whatever the plan's variant.
  * ep 10..13: RK4 stages 1..4 on four registers, with the update axnpby of
  * pyfr/integrators/explicit/steppers.py:67-108 in the epilogue (k = the
- * stage RHS, dta = dt; xa points to a device array of the two pointers
- * {r1, r0}; c receives the next stage's input):
+ * stage RHS; xa points to a device array of three pointers {r1, r0, dt},
+ * dt a device double holding the step, read by the kernel; c receives the
+ * next stage's input):
  *   10: r1 = k;                        c = r0 + dt/2 k
  *   11: r1 = dt/6 r1 + r0 + dt/3 k;    c = dt/2 k + r0
  *   12: r1 = r1 + dt/3 k;              c = dt k + r0

@@ -27,6 +27,7 @@ _FUNCS = {
     'frhip_host_free': [_P],
     'frhip_memcpy_htod': [_P, _P, _S, _P],
     'frhip_memcpy_dtoh': [_P, _P, _S, _P],
+    'frhip_fill_f64': [_P, _L, _D, _P],
     'frhip_memcpy_dtoh_async': [_P, _P, _S, _P],
     'frhip_memcpy_dtod': [_P, _P, _S, _P],
     'frhip_memset': [_P, _I, _S, _P],

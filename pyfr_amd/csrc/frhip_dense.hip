@@ -90,11 +90,15 @@ __device__ __forceinline__ void epi1(T acc, T *__restrict__ c, long off,
             *r1 = *r1 + T(dtb)*t;
         }
         else if constexpr (EP >= 10) {
-            // RK4 stage EP - 9 with four registers (dta = dt): xa points to
-            // the device pair {r1, r0}; c receives the next stage's input.
-            // Each update is the axnpby of pyfr/integrators/explicit/
-            // steppers.py:67-108 it replaces, operands in the same order.
+            // RK4 stage EP - 9 with four registers: xa points to the device
+            // pointers {r1, r0, dt} (dt a device double, read here so that
+            // captured graphs survive a change of step); c receives the
+            // next stage's input.  Each update is the axnpby of
+            // pyfr/integrators/explicit/steppers.py:67-108 it replaces,
+            // operands in the same order.
             T *const *pp = static_cast<T *const *>(xa);
+            dta = *static_cast<const double *>(
+                static_cast<void *const *>(xa)[2]);
             T *r1 = pp[0] + off;
             const T r0 = pp[1][off];
             if constexpr (EP == 10) {           // r1 = k; r2 = r0 + dt/2 k

@@ -60,9 +60,10 @@ struct FusedKernelEntry {
     const int32_t *lin;      // row-major indices into the m x (k0 + k1) A
     const double *vals;
     const char *desc;
-    // by epilogue slot: ep 0-5, then ep 10-17 (null where not built)
-    FKD fd[14];
-    FKF ff[14];
+    // by epilogue slot: ep 0-5, then ep 10-17, then ep 20-23 (null where
+    // not built)
+    FKD fd[18];
+    FKF ff[18];
 };
 
 extern const FusedKernelEntry g_fk_table[];

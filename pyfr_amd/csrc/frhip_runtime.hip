@@ -11,6 +11,12 @@ static thread_local std::string g_last_error;
 
 void set_error(const std::string &msg) { g_last_error = msg; }
 
+__global__ void k_fill_f64(double *p, long n, double v)
+{
+    const long i = (long)blockIdx.x*blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 }  // namespace frhip
 
 using namespace frhip;
@@ -102,6 +108,18 @@ int frhip_memcpy_dtoh(void *dst, const void *src, size_t nbytes, void *stream)
     FR_HIP(hipMemcpyAsync(dst, src, nbytes, hipMemcpyDeviceToHost,
                           as_stream(stream)));
     FR_HIP(hipStreamSynchronize(as_stream(stream)));
+    return 0;
+    FR_API_END
+}
+
+int frhip_fill_f64(void *ptr, long n, double v, void *stream)
+{
+    FR_API_BEGIN
+    FR_CHECK(ptr && n >= 0, "invalid fill arguments");
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_fill_f64, dim3(cdiv(n, 256)), dim3(256), 0,
+                       as_stream(stream), (double *)ptr, n, v);
+    FR_LAUNCHED();
     return 0;
     FR_API_END
 }

@@ -147,14 +147,17 @@ def emit_kernel(name, A):
 #         (xa) and FP32 next-stage input (c)
 #   EP 4, 5: the last stage of EP 2, 3 (r1 += dtb*t; EP 5 also stores the
 #         new solution rounded to float into c)
-#   EP 10-13: RK4 stage EP - 9 with four registers (dta = dt): xa points to
-#         the device pointer pair {r1, r0}, c receives the next stage's
+#   EP 10-13: RK4 stage EP - 9 with four registers: xa points to the device
+#         pointers {r1, r0, dt} (dt a device double, so a captured graph
+#         stays valid when the step changes), c receives the next stage's
 #         input (the axnpby updates of pyfr/integrators/explicit/
 #         steppers.py:67-108, operands in the same order)
 #   EP 14-17: EP 10-13 for the precision form: r1 and r0 FP64, the stage
 #         RHS t and the next stage's input c FP32; the updates are formed in
 #         FP64 in the same order, and EP 17 stores the new solution rounded
 #         to float into c (the next step's first stage input)
+#   EP 20-23: EP 10-13 with a time step per element (pools of independent
+#         solves): xa points to {r1, r0, dt} with dt one value per element
 # The last stage is a template value, not a run-time flag: a uniform branch
 # per output row makes the compiler keep every row's RK operands live and
 # spill.  The trailing int argument is unused.
@@ -164,13 +167,16 @@ def emit_kernel(name, A):
 # reference's OpenMP backend applies (pyfr/solvers/baseadvec/system.py:126-132,
 # pyfr/solvers/baseadvecdiff/system.py:174-203).
 FUSED = [('grad', ['M4 - M6*M0', 'M6']), ('div', ['M1 - M3*M2', 'M3'])]
-FUSED_EPS = {'double': [0, 1, 2, 4, 10, 11, 12, 13],
+FUSED_EPS = {'double': [0, 1, 2, 4, 10, 11, 12, 13, 20, 21, 22, 23],
              'float': [0, 1, 2, 3, 4, 5, 10, 11, 12, 13, 14, 15, 16, 17]}
 # kernel table slot of each epilogue (FusedKernelEntry::fd / ff)
-EP_SLOTS = [0, 1, 2, 3, 4, 5, 10, 11, 12, 13, 14, 15, 16, 17]
+EP_SLOTS = [0, 1, 2, 3, 4, 5, 10, 11, 12, 13, 14, 15, 16, 17, 20, 21, 22,
+            23]
 
 
-def emit_fused(name, A0, A1):
+def emit_fused(name, A0, A1, gb=8, attrs=''):
+    """gb: inputs loaded per batch of the k loop; attrs: function
+    attributes (occupancy experiments)."""
     m, k0 = A0.shape
     k1 = A1.shape[1]
     A = np.hstack([A0, A1])
@@ -178,7 +184,7 @@ def emit_fused(name, A0, A1):
     vid = {(r, c): lit(A[r, c]) for r, c in zip(rows, cols)}
     L = []
     L.append('template <typename T, int EP>')
-    L.append(f'__global__ __launch_bounds__(256) void {name}('
+    L.append(f'__global__ __launch_bounds__(256) {attrs} void {name}('
              'int ncol, const T *__restrict__ b0, long ld0, '
              'const T *__restrict__ b1, long ld1, T *__restrict__ c, '
              'long ldc, const T *__restrict__ rcp, long ldr, '
@@ -188,6 +194,9 @@ def emit_fused(name, A0, A1):
     L.append('    if (j >= ncol) return;')
     L.append('    b0 += j; b1 += j; c += j;')
     L.append('    const long e = (j/(64*5))*64 + (j & 63);')
+    L.append('    if constexpr (EP >= 10 && EP <= 17)')
+    L.append('        dta = *static_cast<const double *>(')
+    L.append('            static_cast<void *const *>(xa)[2]);')
 
     def bexpr(kk):
         return (f'ldg(b0 + {kk}*ld0)' if kk < k0 else
@@ -210,6 +219,32 @@ def emit_fused(name, A0, A1):
         L.append('        } else if constexpr (EP == 4) {')
         L.append(f'          T *r1 = static_cast<T *>(xa) + j + {r}*ldc;')
         L.append('          *r1 = *r1 + T(dtb)*t;')
+        L.append('        } else if constexpr (EP >= 20) {')
+        L.append('          T *const *pp = static_cast<T *const *>(xa);')
+        L.append(f'          T *r1 = pp[0] + j + {r}*ldc;')
+        L.append(f'          const T r0 = pp[1][j + {r}*ldc];')
+        L.append('          const T dte = pp[2][e];')
+        L.append('          if constexpr (EP == 20) {')
+        L.append('            *r1 = t;')
+        L.append('            T a = T(1)*r0;')
+        L.append('            a = a + (dte/T(2))*t;')
+        L.append(f'            c[{r}*ldc] = a;')
+        L.append('          } else if constexpr (EP == 21) {')
+        L.append('            T a = (dte/T(6))*(*r1);')
+        L.append('            a = a + T(1)*r0;')
+        L.append('            a = a + (dte/T(3))*t;')
+        L.append('            *r1 = a;')
+        L.append('            T b = (dte/T(2))*t;')
+        L.append('            b = b + T(1)*r0;')
+        L.append(f'            c[{r}*ldc] = b;')
+        L.append('          } else if constexpr (EP == 22) {')
+        L.append('            *r1 += (dte/T(3))*t;')
+        L.append('            T b = dte*t;')
+        L.append('            b = b + T(1)*r0;')
+        L.append(f'            c[{r}*ldc] = b;')
+        L.append('          } else {')
+        L.append('            *r1 += (dte/T(6))*t;')
+        L.append('          }')
         L.append('        } else if constexpr (EP >= 14) {')
         L.append('          double *const *pp = '
                  'static_cast<double *const *>(xa);')
@@ -276,7 +311,7 @@ def emit_fused(name, A0, A1):
         L.append('    }')
 
     usedk = sorted(set(cols.tolist()))
-    GB = 8
+    GB = gb
     if len(usedk) <= m:
         for kk in usedk:
             L.append(f'    const T v{kk} = {bexpr(kk)};')
@@ -358,7 +393,13 @@ def main(outdir):
         for fname, (op0, op1) in FUSED:
             A0, A1 = sh.opmat(op0), sh.opmat(op1)
             name = f'fk_hex{p}_{fname}'
-            src, lin, nnz, vals = emit_fused(name, A0, A1)
+            # the p = 3 FP64 divergence kernel needs 170 VGPRs, two past the
+            # 168 that give three waves per SIMD; capped there it spills two
+            # and streams 8 % faster (5.08 against 4.70 TB/s with the RK45
+            # stage, 64^3); no effect on the FP32 or smaller kernels
+            at = ('__attribute__((amdgpu_waves_per_eu(3)))'
+                  if fname == 'div' and A0.shape[0] <= 64 else '')
+            src, lin, nnz, vals = emit_fused(name, A0, A1, attrs=at)
             parts.append(f'// hex p={p} fused {fname}: [{op0} | {op1}] '
                          f'{A0.shape[0]}x({A0.shape[1]}+{A1.shape[1]}), '
                          f'nnz={nnz}')

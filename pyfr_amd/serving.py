@@ -22,10 +22,13 @@ module batches many independent solves onto one MI355X:
   state to the host), and repeats; requests arriving while others run join
   at the next step boundary.
 
-All kernels are the library's hex RHS kernels (``frhip_mul``,
-``frhip_tflux_*``, ``frhip_int*``, ``frhip_negdivconf``) plus
-``frhip_axnpby_pool`` for per-solve dt; each element and pair is computed
-exactly as when the solve runs alone, so results do not depend on the mix.
+All kernels are the library's hex RHS kernels.  By default a step runs the
+fused sequence: ``frhip_mul`` (disu), the interface kernels, ``frhip_mul2``
+(gradient, NS), ``frhip_tflux_*``, and the divergence ``frhip_mul2`` whose
+epilogue (ep 20-23) does the RK4 stage updates with each element's own dt;
+``fused=False`` runs the reference's unfused sequence with
+``frhip_axnpby_pool``.  Each element and pair is computed exactly as when
+the solve runs alone, so results do not depend on the mix.
 
 Problem definition (cfg0's analytic field at Mach Ma, sound speed 1):
 rho = 1, (u, v, w) = Ma (sin x cos y cos z, -cos x sin y cos z, 0),
@@ -156,8 +159,9 @@ class _BoxData:
 class Pool:
     """Device storage and kernels for one (system, order) class."""
 
-    def __init__(self, backend, system, order, ecap):
+    def __init__(self, backend, system, order, ecap, fused=True):
         self.be = be = backend
+        self.fused = fused
         self.lib = backend.lib
         self.system, self.order = system, order
         self.ns = system == 'navier-stokes'
@@ -198,9 +202,36 @@ class Pool:
         mul = be._providers[2]
         self._plans = {op: mul._plan(a, be.fpdtype)[0]
                        for op, a in self.ops.items()}
-        # interface arrays sized for a full pool, refilled on membership
-        # changes: int32 left/right offsets, normals (3, maxpairs), mu
-        self.maxpairs = self.ecap*3*(order + 1)**2
+        if fused:
+            # [M1 - M3*M2 | M3] and [M4 - M6*M0 | M6] as generated fused
+            # kernels; the RK4 stage updates with per-element dt in the
+            # divergence kernel's epilogue (ep 20-23), whose operands
+            # {r1, r0, dt} are device pointer triples, one per bank parity
+            self._plans2 = {}
+            for k, (a0, a1) in (('div', ('M1 - M3*M2', 'M3')),
+                                ('grad', ('M4 - M6*M0', 'M6'))):
+                plan, kind = mul._plan2(self.ops[a0], self.ops[a1],
+                                        be.fpdtype)
+                if kind != 1:
+                    raise ValueError(f'no generated fused kernel for hex '
+                                     f'p={order}')
+                self._plans2[k] = plan
+            self._triples = {}
+            for r1, r0 in ((1, 0), (0, 1)):
+                h = np.array([self.banks[r1].data, self.banks[r0].data,
+                              self.dt_e.data], dtype=np.uint64)
+                d = be._malloc_impl(h.nbytes)
+                self.lib.memcpy_htod(d, h.ctypes.data, h.nbytes, None)
+                self._triples[r1, r0] = d
+            self.lib.stream_sync(None)
+        # interface arrays indexed by element slot: a periodic N^3 box has
+        # 3 (p + 1)^2 flux-point pairs per element, so the pairs of a solve
+        # at element offset off fill [off*ppe, (off + n)*ppe).  Pairs of
+        # free slots are idle (both sides one free element's own point,
+        # touching nothing a solve reads), so admission and retirement
+        # write one solve's segment and never rebuild the rest
+        self.ppe = 3*(order + 1)**2
+        self.maxpairs = self.ecap*self.ppe
         isz = np.dtype(be.fpdtype).itemsize
         self._ibufs = {}
         for k, nb in (('l', 4*self.maxpairs), ('r', 4*self.maxpairs),
@@ -212,11 +243,18 @@ class Pool:
         s = ct.c_void_p()
         self.lib.stream_create(ct.byref(s))
         self.stream = s.value
+        self._put_pairs(0, self.ecap, None, None)
         ev = ct.c_void_p()
         self.lib.event_create(ct.byref(ev))
         self.event = ev.value
         self.inflight = 0           # steps enqueued since the last idle
         self.cur = 0                # bank holding the current solution
+        self.ncaptures = 0          # graph captures (membership changes)
+        self.nsteps_run = 0
+        self.host_s = 0.0           # host time spent enqueuing steps
+        # NS constants (viscosity per element / pair overrides mu)
+        self._consts = (ct.c_double*8)(GAMMA, 0.0, PR, 0.0, 0.0, 0.0, 1.0,
+                                       0.0)
 
     def _all_mats(self):
         out = [*self.banks, self.scal_fpts, self.vect_upts, self.rcpdjac,
@@ -296,6 +334,7 @@ class Pool:
         self._put(self.rcpdjac, off, box.rcpdjac)
         self._put(self.dt_e, off, np.full((1, n), req.dt))
         self._put(self.mu_e, off, np.full((1, n), req.mu))
+        self._put_pairs(off, n, box, req.mu)
         self.solves[req.rid] = [req, off, box, 0]
         self.version += 1
 
@@ -304,39 +343,52 @@ class Pool:
         req.result = self._get(self.banks[self.cur], off, box.neles)
         n = box.neles
         self.used[off//SOA:(off + n - n % -SOA)//SOA] = False
+        self._put_pairs(off, n, None, None)
         self.version += 1
         return req
 
-    # -- interface maps of the current membership ----------------------------
-    def _build_maps(self):
+    # -- interface maps, one segment per element slot -----------------------
+    def _put_pairs(self, off, n, box, mu):
+        """Write the pair segment of elements [off, off + n): the solve's
+        pairs (box, viscosity mu) or, with box None, idle pairs."""
         ld = self.scal_fpts.leaddim
-        lm, rm, nl, mup = [], [], [], []
-        for req, off, box, _ in sorted(self.solves.values(),
-                                       key=lambda s: s[1]):
+        k0, k1 = off*self.ppe, (off + n)*self.ppe
+        if box is None:
+            k = np.arange(k0, k1)
+            e, f = k//self.ppe, k % self.ppe
+            lm = rm = f*ld + (e//SOA)*SOA*5 + e % SOA
+            nl = np.zeros((3, k1 - k0))
+            nl[0] = 1.0
+            mup = np.zeros(k1 - k0)
+        else:
+            if box.npairs != k1 - k0:
+                raise ValueError('a periodic box has 3 (p+1)^2 pairs per '
+                                 'element')
             el, er = box.le + off, box.re + off
-            lm.append(box.lf*ld + (el//SOA)*SOA*5 + el % SOA)
-            rm.append(box.rf*ld + (er//SOA)*SOA*5 + er % SOA)
-            nl.append(box.nl)
-            mup.append(np.full(box.npairs, req.mu))
-        lm, rm = np.concatenate(lm), np.concatenate(rm)
-        if lm.max() >= 2**31:
+            lm = box.lf*ld + (el//SOA)*SOA*5 + el % SOA
+            rm = box.rf*ld + (er//SOA)*SOA*5 + er % SOA
+            nl, mup = box.nl, np.full(box.npairs, mu)
+        if max(lm.max(initial=0), rm.max(initial=0)) >= 2**31:
             raise ValueError('pool too large for int32 interface maps')
-        npairs = len(lm)
-        if npairs > self.maxpairs:
-            raise ValueError('interface pairs exceed the pool capacity')
         fdt = self.be.fpdtype
-        nlp = np.zeros((3, self.maxpairs), dtype=fdt)
-        nlp[:, :npairs] = np.concatenate(nl, axis=1)
+        isz = np.dtype(fdt).itemsize
+        b, st = self._ibufs, self.stream
         up = self.lib.memcpy_htod
-        st, b = self.stream, self._ibufs
         lm32, rm32 = lm.astype(np.int32), rm.astype(np.int32)
-        mup = np.concatenate(mup).astype(fdt)
-        up(b['l'], lm32.ctypes.data, lm32.nbytes, st)
-        up(b['r'], rm32.ctypes.data, rm32.nbytes, st)
-        up(b['nl'], nlp.ctypes.data, nlp.nbytes, st)
-        up(b['mu'], mup.ctypes.data, mup.nbytes, st)
-        self._maps = {'n': npairs, 'l': b['l'], 'r': b['r'], 'nl': b['nl'],
-                      'ldn': self.maxpairs, 'mu': b['mu']}
+        up(b['l'] + 4*k0, lm32.ctypes.data, lm32.nbytes, st)
+        up(b['r'] + 4*k0, rm32.ctypes.data, rm32.nbytes, st)
+        mup = np.ascontiguousarray(mup, dtype=fdt)
+        up(b['mu'] + isz*k0, mup.ctypes.data, mup.nbytes, st)
+        nlc = np.ascontiguousarray(nl, dtype=fdt)
+        self.lib.memcpy2d(b['nl'] + isz*k0, isz*self.maxpairs,
+                          nlc.ctypes.data, nlc.shape[1]*isz,
+                          nlc.shape[1]*isz, 3, 0, st)
+
+    def _build_maps(self):
+        """Pairs of the occupied extent [0, ehi) (solves and idle slots)."""
+        b = self._ibufs
+        self._maps = {'n': self.ehi*self.ppe, 'l': b['l'], 'r': b['r'],
+                      'nl': b['nl'], 'ldn': self.maxpairs, 'mu': b['mu']}
 
     # -- one RK4 step over the pool -----------------------------------------
     def _rhs(self, uin, fout, s):
@@ -364,8 +416,7 @@ class Pool:
                             geo.data, geo.leaddim, self.upts.data, GAMMA,
                             None, s)
         else:
-            consts = (ct.c_double*8)(GAMMA, 0.0, PR, 0.0, 0.0, 0.0, 1.0, 0.0)
-            self._consts = consts
+            consts = self._consts
             lib.intconu(d, mp['n'], self.scal_fpts.data, mp['l'],
                         mp['r'], self.comm.data, mp['l'],
                         mp['r'], 0.0, s)
@@ -393,6 +444,56 @@ class Pool:
         lib.negdivconf(d, Nu, E, f.data, f.leaddim, self.rcpdjac.data,
                        self.rcpdjac.leaddim, s)
 
+    def _rhs_stage(self, uin, stage, r1, r0, s):
+        """The fused RHS of RK4 stage `stage` (1..4) from bank uin with the
+        stage's updates in the divergence epilogue: r1 the accumulator, r0
+        the step's initial state, bank 2 the next stage's input (in place
+        for stages 2-4: the divergence kernel reads only the flux and the
+        common flux, both complete before it writes)."""
+        lib, be = self.lib, self.be
+        E = self.ehi
+        ncol = 5*E
+        d = _dtcode(be.fpdtype)
+        u, nxt = self.banks[uin], self.banks[2]
+        mp = self._maps
+        Nu, Nf = self.nupts, self.nfpts
+        geo = self.verts if self.linear else self.smats
+        sf, vu = self.scal_fpts, self.vect_upts
+        lib.mul(self._plans['M0'], ncol, u.data, u.leaddim, sf.data,
+                sf.leaddim, 1.0, 0.0, s)
+        if not self.ns:
+            lib.intcflux_euler(d, 0, mp['n'], sf.data, mp['l'], mp['r'],
+                               mp['nl'], mp['ldn'], GAMMA, None, s)
+            lib.tflux_euler(d, int(self.linear), Nu, E, u.data, u.leaddim,
+                            vu.data, vu.leaddim, geo.data, geo.leaddim,
+                            self.upts.data, GAMMA, None, s)
+        else:
+            consts = self._consts
+            gu, vf = self.grad_upts, self.vect_fpts
+            lib.intconu(d, mp['n'], sf.data, mp['l'], mp['r'],
+                        self.comm.data, mp['l'], mp['r'], 0.0, s)
+            lib.mul2(self._plans2['grad'], ncol, u.data, u.leaddim,
+                     self.comm.data, self.comm.leaddim, gu.data, gu.leaddim,
+                     0, None, 0, None, 0.0, 0.0, 0, s)
+            lib.tflux_ns(d, int(self.linear), Nu, E, u.data, u.leaddim,
+                         vu.data, vu.leaddim, gu.data, gu.leaddim, geo.data,
+                         geo.leaddim, self.upts.data, self.rcpdjac.data,
+                         self.rcpdjac.leaddim, consts, self.mu_e.data, None,
+                         s)
+            for i in range(3):
+                lib.mul(self._plans['M0'], ncol,
+                        gu.data + i*Nu*gu.leaddim*gu.itemsize, gu.leaddim,
+                        vf.data + i*Nf*vf.leaddim*vf.itemsize, vf.leaddim,
+                        1.0, 0.0, s)
+            lib.intcflux_ns(d, 0, mp['n'], sf.data, mp['l'], mp['r'],
+                            vf.data, mp['l'], mp['r'], Nf*vf.leaddim, None,
+                            None, mp['nl'], mp['ldn'], consts, mp['mu'],
+                            None, s)
+        lib.mul2(self._plans2['div'], ncol, vu.data, vu.leaddim, sf.data,
+                 sf.leaddim, nxt.data, nxt.leaddim, 19 + stage,
+                 self.rcpdjac.data, self.rcpdjac.leaddim,
+                 self._triples[r1, r0], 0.0, 0.0, 0, s)
+
     def _add(self, s, a, mask, *ri):
         lib, be = self.lib, self.be
         m = self.banks[0]
@@ -408,6 +509,10 @@ class Pool:
         r0, r1, r2 = self.cur, 1 - self.cur if self.cur < 2 else 0, 2
         if self.cur == 2:
             raise RuntimeError('invalid register state')
+        if self.fused:
+            for stage in (1, 2, 3, 4):
+                self._rhs_stage(r0 if stage == 1 else r2, stage, r1, r0, s)
+            return r1
         add = lambda a, m, *r: self._add(s, a, m, *r)
         self._rhs(r0, r1, s)
         add((0.0, 1.0, 2.0), 0b100, r2, r0, r1)
@@ -426,26 +531,34 @@ class Pool:
         the pool's stream)."""
         if not self.solves:
             return
+        t0 = time.perf_counter()
         if self._gversion != self.version:
+            # the kernels' arguments depend on the occupied extent only
             self._build_maps()
-            self._drop_graphs()
             self._gversion = self.version
         if use_graph:
-            if self.cur not in self._graphs:
+            key = (self.cur, self.ehi)
+            if key not in self._graphs:
+                if len(self._graphs) >= 8:
+                    g0, _ = self._graphs.pop(next(iter(self._graphs)))
+                    self.lib.graph_destroy(g0)
                 self.lib.graph_begin(self.stream)
                 try:
                     nxt = self._step_kernels(self.stream)
                 finally:
                     ge = ct.c_void_p()
                     self.lib.graph_end(self.stream, ct.byref(ge))
-                self._graphs[self.cur] = (ge.value, nxt)
-            g, nxt = self._graphs[self.cur]
+                self._graphs[key] = (ge.value, nxt)
+                self.ncaptures += 1
+            g, nxt = self._graphs[key]
             self.lib.graph_launch(g, self.stream)
             self.cur = nxt
         else:
             self.cur = self._step_kernels(self.stream)
         for v in self.solves.values():
             v[3] += 1
+        self.nsteps_run += 1
+        self.host_s += time.perf_counter() - t0
 
     def sync(self):
         self.lib.stream_sync(self.stream)
@@ -498,7 +611,7 @@ class ServingEngine:
     """Admits, batches and steps solves; see the module docstring."""
 
     def __init__(self, backend, budget_bytes=48*2**30, max_neles=24**3,
-                 classes=None, slots_per_class=6):
+                 classes=None, slots_per_class=6, fused=True):
         self.be = backend
         self.budget = budget_bytes
         self.pools = {}
@@ -513,7 +626,7 @@ class ServingEngine:
         for c in classes:
             ecap = int(max_neles*slots_per_class*scale)
             ecap = max(ecap - ecap % SOA, max_neles - max_neles % -SOA)
-            self.pools[c] = Pool(backend, c[0], c[1], ecap)
+            self.pools[c] = Pool(backend, c[0], c[1], ecap, fused)
         self.hbm_bytes = sum(p.nbytes for p in self.pools.values())
 
     def box(self, N, order):
@@ -610,3 +723,29 @@ def summarize(done):
             'mean_s': float(lat.mean()),
             'makespan_s': float(t_end - t_start),
             'gdof_rhs_per_s': work/(t_end - t_start)/1e9}
+
+
+def sequential_bound(done):
+    """From a one-solve-at-a-time run: each solve's alone time (admission
+    to result) and rate; per (system, order) class the best alone rate (its
+    largest solves, near the class's saturated rate); the work share of the
+    solves that are launch-bound alone (rate below half their class's best);
+    and the ratio batched / one-at-a-time that running every solve at its
+    class's best rate would give (the bound batching can reach on this
+    stream, short of overlapping classes)."""
+    cls = {}
+    rows = []
+    for r in done:
+        t = r.finish_time - r.admit_time
+        w = r.ndofs*4*r.nsteps
+        rows.append(((r.system, r.order), w, t))
+        k = (r.system, r.order)
+        cls[k] = max(cls.get(k, 0.0), w/t)
+    t_seq = sum(t for _, _, t in rows)
+    t_ideal = sum(w/cls[k] for k, w, _ in rows)
+    work = sum(w for _, w, _ in rows)
+    lb = sum(w for k, w, t in rows if w/t < 0.5*cls[k])
+    return {'ratio_max': t_seq/t_ideal,
+            'launch_bound_work_share': lb/work,
+            'class_best_gdof_rhs_per_s': {f'{k[0]}-p{k[1]}': v/1e9
+                                          for k, v in sorted(cls.items())}}

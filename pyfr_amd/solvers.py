@@ -1391,26 +1391,23 @@ class BaseSystem:
     def rhs_rk4_stage(self, t, stage, uin, nxt, r1, r0, dt, mixed=False):
         """RHS of RK4 stage `stage` at t with the stage's register updates
         in the divergence kernel's epilogue (``steppers.py:67-108``; mixed:
-        the precision form's FP64 updates).  dt is a kernel argument of the
-        captured graph, so a few graphs are kept per stage, one per dt (the
-        last steps before an output time re-split dt), each captured
-        once."""
+        the precision form's FP64 updates).  The epilogues read dt from
+        device memory (bind writes it on the stream), so each stage's graph
+        is captured once, whatever the steps before an output time."""
         key = ('rk4', stage, uin, nxt, r1, r0, mixed)
-        byd = self._kcache.setdefault(key, {})
-        if dt not in byd:
+        if key not in self._kcache:
             ks = self._fused_kernels(uin, nxt, (r1, r0),
                                      f'rk4{"m" if mixed else ""}-{stage}')
             g = self.backend.graph()
             g.add_all(ks)
             g.commit()
-            for k in ks:
-                if 'dtb' in k.rtargs:
-                    k.bind(dta=dt)
-            if len(byd) >= 4:
-                del byd[next(iter(byd))]
-            byd[dt] = g
+            g.stage_kernels = [k for k in ks if 'dta' in k.devargs]
+            self._kcache[key] = g
+        g = self._kcache[key]
+        for k in g.stage_kernels:
+            k.bind(dta=dt)
         self._set_time(t, uin)
-        self.backend.run_graph(byd[dt])
+        self.backend.run_graph(g)
 
     def ele_scal_upts(self, idx):
         """The bank's arrays as stored (u - u0 for a solution bank of the

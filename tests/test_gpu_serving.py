@@ -11,11 +11,11 @@ from tests.fr_cases import rel_l2
 pytestmark = pytest.mark.gpu
 
 
-def _engine(backend_factory):
+def _engine(backend_factory, fused=True):
     from pyfr_amd.serving import ServingEngine
     be = backend_factory()
     return ServingEngine(be, budget_bytes=2*2**30, max_neles=6**3,
-                         slots_per_class=4)
+                         slots_per_class=4, fused=fused)
 
 
 def _reqs(nsteps=8):
@@ -92,3 +92,19 @@ def test_served_solve_matches_oracle(backend_factory, rid):
     err = rel_l2(got, U)
     print(f'\n[serving] solve {rid} vs oracle rel L2 {err:.2e}', flush=True)
     assert err < 1e-12
+
+
+def test_fused_matches_unfused(backend_factory):
+    """The fused step (RK4 updates in the divergence epilogue, per-element
+    dt) and the reference's unfused sequence give the same solves to
+    rounding (relative L2 <= 1e-13)."""
+    res = []
+    for fused in (True, False):
+        eng = _engine(backend_factory, fused)
+        res.append({r.rid: r.result for r in eng.run(_reqs(),
+                                                     mode='batched')})
+    for rid in res[0]:
+        err = rel_l2(res[0][rid], res[1][rid])
+        print(f'\n[serving] fused vs unfused solve {rid}: {err:.2e}',
+              flush=True)
+        assert err < 1e-13

@@ -156,6 +156,28 @@ def test_aosoa_layout_roundtrip():
     assert np.array_equal(m4._unpack(m4._pack(a4).ravel()), a4)
 
 
+def test_aosoa_row_padding():
+    """[backend-hip] row-pad: rows of AoSoA arrays padded by a multiple of
+    64 elements; the layout formula holds with the padded leading
+    dimension and the pad never reaches the unpacked array."""
+    from pyfr_amd.backend import Matrix
+    be = _FakeBackend()
+    be.row_pad = 128
+    a = np.random.default_rng(2).standard_normal((7, 5, 130))
+    m = Matrix(be, np.float64, a.shape, None, None, {'align'})
+    assert m.ncol == 5*192 and m.leaddim == 5*192 + 128
+    assert m.nbytes == 7*m.leaddim*8
+    buf = m._pack(a)
+    assert buf.shape == (7, m.leaddim) and not buf[:, m.ncol:].any()
+    flat = buf.ravel()
+    for r, v, e in [(0, 0, 0), (3, 2, 65), (6, 4, 129), (2, 1, 63)]:
+        assert flat[r*m.leaddim + ((e//64)*5 + v)*64 + e % 64] == a[r, v, e]
+    assert np.array_equal(m._unpack(flat), a)
+    a4 = np.random.default_rng(3).standard_normal((3, 4, 5, 70))
+    m4 = Matrix(be, np.float64, a4.shape, None, None, {'align'})
+    assert np.array_equal(m4._unpack(m4._pack(a4).ravel()), a4)
+
+
 def test_view_mapping_formula():
     from pyfr_amd.backend import Matrix, View
     be = _FakeBackend()
