@@ -170,7 +170,9 @@ int frhip_mul2_plan_set_variant(void *plan, int variant);
  * and without the LDS-staged operator (variant codes as above, without the
  * scalar kernel), variant 32: one block of five wavefronts per
  * 64-element block, the flux evaluated once per point into LDS, and (FP32)
- * variant 64: the 4x4x1 MFMA form.  c may be u itself (an RHS in place):
+ * variant 64: the 4x4x1 MFMA form, 65: the same in blocks of the five
+ * wavefronts of an element block (one workgroup, so one XCD's L2 serves
+ * every variable's reads of u).  c may be u itself (an RHS in place):
  * such a call runs variant 32, whose block barriers order every read of u
  * before any write of c, whatever the plan's variant.
  * ep 10..13: RK4 stages 1..4 on four registers, with the update axnpby of

@@ -60,6 +60,19 @@ __device__ __forceinline__ V4<double>::type mfma4(double a, double b,
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+// The RK4 step of epilogues 10..13: a device double (xa's third pointer),
+// so that captured graphs survive a change of step; read once per kernel,
+// as stores through the other pointers could alias it.
+template <int EP>
+__device__ __forceinline__ double rk4_dt(void *xa, double dta)
+{
+    if constexpr (EP >= 10 && EP <= 17)
+        return *static_cast<const double *>(
+            static_cast<void *const *>(xa)[2]);
+    else
+        return dta;
+}
+
 // One output value at linear offset off = row*ldc + col; ridx indexes rcp.
 template <typename T, int EP>
 __device__ __forceinline__ void epi1(T acc, T *__restrict__ c, long off,
@@ -91,14 +104,11 @@ __device__ __forceinline__ void epi1(T acc, T *__restrict__ c, long off,
         }
         else if constexpr (EP >= 10) {
             // RK4 stage EP - 9 with four registers: xa points to the device
-            // pointers {r1, r0, dt} (dt a device double, read here so that
-            // captured graphs survive a change of step); c receives the
-            // next stage's input.  Each update is the axnpby of
-            // pyfr/integrators/explicit/steppers.py:67-108 it replaces,
-            // operands in the same order.
+            // pointers {r1, r0, dt}, dta the step read from dt once per
+            // kernel (rk4_dt); c receives the next stage's input.  Each
+            // update is the axnpby of pyfr/integrators/explicit/
+            // steppers.py:67-108 it replaces, operands in the same order.
             T *const *pp = static_cast<T *const *>(xa);
-            dta = *static_cast<const double *>(
-                static_cast<void *const *>(xa)[2]);
             T *r1 = pp[0] + off;
             const T r0 = pp[1][off];
             if constexpr (EP == 10) {           // r1 = k; r2 = r0 + dt/2 k
@@ -145,6 +155,7 @@ __device__ __forceinline__ void dense_tiles(
     long ldr, int nvar, void *__restrict__ xa, double dta, double dtb,
     T alpha, T beta)
 {
+    dta = rk4_dt<EP>(xa, dta);
     using V = typename V4<T>::type;
     const int lane = threadIdx.x & 63;
     const int kr = lane >> 4, cq = (lane & 15)*4;
@@ -279,6 +290,7 @@ __device__ __forceinline__ void flux_tiles(
     long ldc, const T *__restrict__ rcp, long ldr, void *__restrict__ xa,
     double dta, double dtb, T gm1)
 {
+    dta = rk4_dt<EP>(xa, dta);
     using V = typename V4<T>::type;
     const int lane = threadIdx.x & 63;
     const int kr = lane >> 4, cq = (lane & 15)*4;
@@ -458,6 +470,7 @@ __global__ __launch_bounds__(320) void k_tdivf_flux_blk(
     long ldc, const T *__restrict__ rcp, long ldr, void *__restrict__ xa,
     double dta, double dtb, T gm1)
 {
+    dta = rk4_dt<EP>(xa, dta);
     using V = typename V4<T>::type;
     extern __shared__ __attribute__((aligned(16))) char fr_smem[];
     T *fbuf = reinterpret_cast<T *>(fr_smem);       // [2][4][3][5][64]
@@ -589,6 +602,7 @@ __global__ __launch_bounds__(256) void k_dense_fma(
     T *__restrict__ c, long ldc, const T *__restrict__ rcp, long ldr,
     int nvar, void *__restrict__ xa, double dta, double dtb, T alpha, T beta)
 {
+    dta = rk4_dt<EP>(xa, dta);
     const long j = (long)blockIdx.x*blockDim.x + threadIdx.x;
     if (j >= ncol) return;
     const long e = (j/(64L*nvar))*64 + (j & 63);
@@ -639,17 +653,19 @@ __device__ __forceinline__ void mfma4x4_chunk(
     (mfma4x4_k<G, KK>(acc, a, b[KK]), ...);
 }
 
-// One wavefront per (64-element block, variable); the wavefronts of an
-// element block may sit in different blocks, so c must not alias u (an RHS
-// in place runs the block-cooperative form instead).
+// One wavefront per (64-element block, variable), in blocks of 4 or 5
+// wavefronts (FR_V4 / FR_V4B); no barrier orders the variables' reads of u
+// before another's writes, so c must not alias u (an RHS in place runs the
+// block-cooperative form instead).
 template <int G, int EP>
-__global__ __launch_bounds__(256) void k_tdivf_flux4(
+__global__ __launch_bounds__(320) void k_tdivf_flux4(
     int m, int nupts, int nc0, int nc1, int k1, const float *__restrict__ af,
     long ncol, const float *__restrict__ u, long ldu,
     const float *__restrict__ sm, long lds, const float *__restrict__ b1,
     long ld1, float *__restrict__ c, long ldc, const float *__restrict__ rcp,
     long ldr, void *__restrict__ xa, double dta, double dtb, float gm1)
 {
+    dta = rk4_dt<EP>(xa, dta);
     const int lane = threadIdx.x & 63;
     const long w = (long)blockIdx.x*(blockDim.x >> 6) + (threadIdx.x >> 6);
     if (w*64 >= ncol) return;
@@ -775,6 +791,10 @@ constexpr int FR_LDSV = 16;
 constexpr int FR_BLKV = 32;
 // flux-fused kernels only, FP32: the 4x4x1 MFMA form (k_tdivf_flux4)
 constexpr int FR_V4 = 64;
+// ... with one block of five wavefronts per 64-element block (all its
+// variables in one workgroup, so on one XCD and its L2: u is fetched from
+// HBM once, not once per XCD the variables land on)
+constexpr int FR_V4B = 65;
 constexpr size_t FR_LDS_MAX = 65536;
 
 static bool variant_valid(int v, size_t ldsbytes, bool scalar_ok)
@@ -1042,6 +1062,7 @@ __global__ __launch_bounds__(256) void k_dense_bf16(
     const float *__restrict__ rcp, long ldr, int nvar,
     void *__restrict__ xa, double dta, double dtb, float alpha, float beta)
 {
+    dta = rk4_dt<EP>(xa, dta);
     using V = V4<float>::type;
     const int lane = threadIdx.x & 63;
     const long c0 = ((long)blockIdx.x*(blockDim.x >> 6) +
@@ -1218,8 +1239,9 @@ static void launch_flux_ep(const FluxPlan *p, int wpb, long ncol,
                            double dtb, double gamma, hipStream_t s)
 {
     if constexpr (sizeof(T) == 4) {
-        if (wpb == FR_V4) {
-            const dim3 grid(cdiv(ncol/64, 4)), block(256);
+        if (wpb == FR_V4 || wpb == FR_V4B) {
+            const int wv = wpb == FR_V4B ? 5 : 4;
+            const dim3 grid(cdiv(ncol/64, wv)), block(64*wv);
 #define FR_G4(GV)                                                           \
             case GV:                                                        \
                 hipLaunchKernelGGL((k_tdivf_flux4<GV, EP>), grid, block, 0, \
@@ -1335,9 +1357,9 @@ static int flux_launch(const FluxPlan *p, int ep, long ncol, const void *u,
     FR_CHECK(ep < 2 || xa, "RK epilogue needs the r1 / accumulator array");
     const size_t esz = p->d.dtype == FRHIP_F64 ? 8 : 4;
     int wpb = p->d.variant > 0 ? p->d.variant : 4;
-    if (wpb == FR_V4 && !p->af4)
+    if ((wpb == FR_V4 || wpb == FR_V4B) && !p->af4)
         wpb = 4;
-    if (wpb != FR_BLKV && wpb != FR_V4 &&
+    if (wpb != FR_BLKV && wpb != FR_V4 && wpb != FR_V4B &&
         !variant_valid(wpb, (size_t)p->d.mt*(p->d.ks0 + p->d.ks1)*64*esz,
                        false))
         wpb &= ~FR_LDSV;
@@ -1496,9 +1518,10 @@ int frhip_tdivf_plan_tune(void *plan, int ncol, const void *u, long ldu,
     int bestv = 4, rc = 0;
     const size_t ldsb = (size_t)p->d.mt*(p->d.ks0 + p->d.ks1)*64*esz;
     for (int v : {1, 2, 4, FR_LDSV + 1, FR_LDSV + 2, FR_LDSV + 4,
-                  FR_BLKV, FR_V4}) {
-        if (v == FR_V4 ? !p->af4
-                       : v != FR_BLKV && !variant_valid(v, ldsb, false))
+                  FR_BLKV, FR_V4, FR_V4B}) {
+        if (v == FR_V4 || v == FR_V4B
+                ? !p->af4
+                : v != FR_BLKV && !variant_valid(v, ldsb, false))
             continue;
         FluxPlan q = *p;
         q.d.variant = v;
@@ -1533,7 +1556,8 @@ int frhip_tdivf_plan_set_variant(void *plan, int variant)
     FluxPlan *p = static_cast<FluxPlan *>(plan);
     FR_CHECK(p, "null plan");
     const size_t esz = p->d.dtype == FRHIP_F64 ? 8 : 4;
-    FR_CHECK(variant == FR_BLKV || (variant == FR_V4 && p->af4) ||
+    FR_CHECK(variant == FR_BLKV ||
+             ((variant == FR_V4 || variant == FR_V4B) && p->af4) ||
              variant_valid(variant,
                            (size_t)p->d.mt*(p->d.ks0 + p->d.ks1)*64*esz,
                            false), "invalid variant");

@@ -203,12 +203,13 @@ def test_simplex_rhs_parity(backend_factory, n, nt, npr, order, fused):
 
 
 @pytest.mark.parametrize('precision', ['double', 'single'])
-@pytest.mark.parametrize('variant', [1, 2, 4, 17, 18, 20, 32, 64])
+@pytest.mark.parametrize('variant', [1, 2, 4, 17, 18, 20, 32, 64, 65])
 def test_flux_variants_parity(backend_factory, precision, variant):
     """Every variant of the flux-fused divergence kernel the tuner may pick
     (1/2/4 waves per block, +16 LDS-staged operator, 32 block-cooperative
-    flux through LDS, 64 the FP32 4x4x1 MFMA form) against the oracle, at
-    p = 3 (tet 20 / prism 40 solution points) and p = 4 (35 / 75)."""
+    flux through LDS, 64 the FP32 4x4x1 MFMA form, 65 the same with an
+    element block's five wavefronts in one workgroup) against the oracle,
+    at p = 3 (tet 20 / prism 40 solution points) and p = 4 (35 / 75)."""
     from pyfr_amd._lib import FrHipError
     from pyfr_amd.backend import MulProvider
     # no graphs: a kernel reads its plan's variant at each launch
@@ -233,7 +234,7 @@ def test_flux_variants_parity(backend_factory, precision, variant):
                 # LDS-staged operators larger than the LDS budget are
                 # refused (the FP64 ones at p = 4), and the 4x4x1 form
                 # exists in FP32 only; the rest must run
-                assert variant & 16 or (variant == 64 and
+                assert variant & 16 or (variant in (64, 65) and
                                         precision == 'double'), variant
         if not taken:
             continue
@@ -613,7 +614,7 @@ def test_flux_fused_in_place(backend_factory, precision):
         lib.memcpy_dtoh(out.ctypes.data, dst, sz, None)
         return out
 
-    variants = (1, 2, 4, 32) + ((64,) if precision == 'single' else ())
+    variants = (1, 2, 4, 32) + ((64, 65) if precision == 'single' else ())
     try:
         lib.tdivf_plan_set_variant(plan, 32)
         want = run(du, dc)
