@@ -514,6 +514,11 @@ def counter_probe(c, precision):
     integ.run_steps(1, c['dt'])
     be.wait()
     kerns = rhs_kernel_list(be, integ, c, precision)
+    tuned = [t for p in be._providers if hasattr(p, 'tuning_report')
+             for t in p.tuning_report()]
+    print('# tuned variants: ' + ', '.join(
+        f'{t["kernel"]} {t["operator"]} -> {t["variant"]}' for t in tuned),
+        file=sys.stderr, flush=True)
     for k in kerns:
         print(f'# dispatch {k.name}', file=sys.stderr, flush=True)
         k.run(be.stream)
@@ -639,6 +644,9 @@ def measure(c, precision, steps, warmup, kernel_reps):
              for t in p.tuning_report()]
     if tuned:
         out['tuned_variants'] = tuned
+        print('# tuned variants: ' + ', '.join(
+            f'{t["kernel"]} {t["operator"]} -> {t["variant"]}'
+            for t in tuned), file=sys.stderr)
     del integ, sysm, kerns
     be.wait()
     return out

@@ -73,6 +73,41 @@ __device__ __forceinline__ double rk4_dt(void *xa, double dta)
         return dta;
 }
 
+// RK4 stage EP - 9 (EP 10..13) on one value: t the stage's RHS, r0 the
+// step's initial state, x1 the register r1 holds (unused in stage 1); n1
+// the new r1 and nc the next stage's input (unused in stage 4).  Each
+// update is the axnpby of pyfr/integrators/explicit/steppers.py:67-108 it
+// replaces, operands in the same order.
+template <typename T, int EP>
+__device__ __forceinline__ void rk4_vals(T t, T r0, T x1, double dta, T &n1,
+                                         T &nc)
+{
+    if constexpr (EP == 10) {           // r1 = k; r2 = r0 + dt/2 k
+        n1 = t;
+        T a = T(1)*r0;
+        a = a + T(dta/2.0)*t;
+        nc = a;
+    }
+    else if constexpr (EP == 11) {      // r1 = dt/6 r1 + r0 + dt/3 k
+        T a = T(dta/6.0)*x1;            // r3 = dt/2 k + r0
+        a = a + T(1)*r0;
+        a = a + T(dta/3.0)*t;
+        n1 = a;
+        T b = T(dta/2.0)*t;
+        b = b + T(1)*r0;
+        nc = b;
+    }
+    else if constexpr (EP == 12) {      // r1 += dt/3 k; r2 = dt k + r0
+        n1 = x1 + T(dta/3.0)*t;
+        T b = T(dta)*t;
+        b = b + T(1)*r0;
+        nc = b;
+    }
+    else {                              // r1 += dt/6 k
+        n1 = x1 + T(dta/6.0)*t;
+    }
+}
+
 // One output value at linear offset off = row*ldc + col; ridx indexes rcp.
 template <typename T, int EP>
 __device__ __forceinline__ void epi1(T acc, T *__restrict__ c, long off,
@@ -105,36 +140,15 @@ __device__ __forceinline__ void epi1(T acc, T *__restrict__ c, long off,
         else if constexpr (EP >= 10) {
             // RK4 stage EP - 9 with four registers: xa points to the device
             // pointers {r1, r0, dt}, dta the step read from dt once per
-            // kernel (rk4_dt); c receives the next stage's input.  Each
-            // update is the axnpby of pyfr/integrators/explicit/
-            // steppers.py:67-108 it replaces, operands in the same order.
+            // kernel (rk4_dt); c receives the next stage's input.
             T *const *pp = static_cast<T *const *>(xa);
             T *r1 = pp[0] + off;
             const T r0 = pp[1][off];
-            if constexpr (EP == 10) {           // r1 = k; r2 = r0 + dt/2 k
-                *r1 = t;
-                T a = T(1)*r0;
-                a = a + T(dta/2.0)*t;
-                c[off] = a;
-            }
-            else if constexpr (EP == 11) {      // r1 = dt/6 r1 + r0 + dt/3 k
-                T a = T(dta/6.0)*(*r1);         // r3 = dt/2 k + r0
-                a = a + T(1)*r0;
-                a = a + T(dta/3.0)*t;
-                *r1 = a;
-                T b = T(dta/2.0)*t;
-                b = b + T(1)*r0;
-                c[off] = b;
-            }
-            else if constexpr (EP == 12) {      // r1 += dt/3 k; r2 = dt k + r0
-                *r1 += T(dta/3.0)*t;
-                T b = T(dta)*t;
-                b = b + T(1)*r0;
-                c[off] = b;
-            }
-            else {                              // r1 += dt/6 k
-                *r1 += T(dta/6.0)*t;
-            }
+            T n1, nc;
+            rk4_vals<T, EP>(t, r0, EP == 10 ? T(0) : *r1, dta, n1, nc);
+            *r1 = n1;
+            if constexpr (EP != 13)
+                c[off] = nc;
         }
         else {
             double *r1 = static_cast<double *>(xa) + off;
@@ -656,9 +670,14 @@ __device__ __forceinline__ void mfma4x4_chunk(
 // One wavefront per (64-element block, variable), in blocks of 4 or 5
 // wavefronts (FR_V4 / FR_V4B); no barrier orders the variables' reads of u
 // before another's writes, so c must not alias u (an RHS in place runs the
-// block-cooperative form instead).
+// block-cooperative form instead).  Up to 10 row groups (tets p <= 4) the
+// kernel is held to 128 VGPRs, four wavefronts per SIMD: uncapped, the
+// epilogues with 1/|J| take 132-133 and run three (a few spilled registers
+// cost less than the lost wavefront: p = 4 tets, 0.682 -> 0.648 ms per RHS
+// and 6.66 -> 6.09 ms per RK4 step on the 48^3 x 6 box).
 template <int G, int EP>
-__global__ __launch_bounds__(320) void k_tdivf_flux4(
+__global__ __launch_bounds__(320)
+__attribute__((amdgpu_waves_per_eu(G <= 10 ? 4 : 2))) void k_tdivf_flux4(
     int m, int nupts, int nc0, int nc1, int k1, const float *__restrict__ af,
     long ncol, const float *__restrict__ u, long ldu,
     const float *__restrict__ sm, long lds, const float *__restrict__ b1,
@@ -775,8 +794,8 @@ __global__ __launch_bounds__(320) void k_tdivf_flux4(
             else if constexpr (EP == 1)
                 c[off] = -rcp[row*ldr + ecol]*acc[g][i];
             else
-                epi1<float, EP>(acc[g][i], c, off, rcp, row*ldr + ecol, xa,
-                                dta, dtb, 1.0f, 0.0f);
+                epi1<float, EP>(acc[g][i], c, off, rcp, row*ldr + ecol,
+                                xa, dta, dtb, 1.0f, 0.0f);
         }
 }
 

@@ -284,17 +284,32 @@ def _rk4_dict(rhs, u, dt, step=None):
     return unflat(step(lambda v: flat(rhs(unflat(v))), flat(u), dt))
 
 
-@pytest.mark.parametrize('precision', ['double', 'single'])
-def test_simplex_rk4_fused_stages(backend_factory, precision):
+@pytest.mark.parametrize('precision,variant,order', [
+    ('double', None, 3), ('single', None, 3), ('single', 64, 3),
+    ('single', 65, 3), ('single', 64, 4), ('single', 65, 4)])
+def test_simplex_rk4_fused_stages(backend_factory, precision, variant,
+                                  order):
     """RK4 with each stage's register updates in the flux-fused divergence
     epilogue against the reference kernel sequence (separate axnpby),
-    10 steps on a mixed mesh."""
-    be = backend_factory(precision=precision)
-    integ, _, _ = _mixed(be, 3, 2, 1, 3, dt=2e-3)
+    10 steps on a mixed mesh; the tuner's kernel variants, or the 4x4x1
+    MFMA form (64 / 65, its RK4 epilogue batching the rows' loads ahead
+    of their stores) forced on every flux-fused plan."""
+    from pyfr_amd.backend import MulProvider
+    # forced variants: no graphs, a kernel reads its plan's variant at
+    # each launch
+    be = backend_factory(precision=precision, graphs=variant is None)
+    integ, _, _ = _mixed(be, 3, 2, 1, order, dt=2e-3)
     assert integ.system.fused_rk4
+    if variant is not None:
+        integ.system.rhs(0.0, 0, 1)       # builds (and tunes) the plans
+        mp = next(p for p in be._providers if isinstance(p, MulProvider))
+        plans = [p for p, _, _ in mp._plans3.values()]
+        assert plans
+        for plan in plans:
+            be.lib.tdivf_plan_set_variant(plan, variant)
     integ.run_steps(10, 2e-3)
     ref_be = backend_factory(precision=precision, fused=False)
-    ref, _, _ = _mixed(ref_be, 3, 2, 1, 3, dt=2e-3)
+    ref, _, _ = _mixed(ref_be, 3, 2, 1, order, dt=2e-3)
     assert not ref.system.fused_rk4
     ref.run_steps(10, 2e-3)
     names = integ.system.ele_types
@@ -679,3 +694,37 @@ def test_simplex_curved_free_stream(backend_factory):
     integ.system.rhs(0.0, 0, 1)
     r = integ.system.ele_scal_upts(1)
     assert max(np.abs(a).max() for a in r) < 1e-10
+
+
+@pytest.mark.parametrize('sorder', [2, 3])
+def test_simplex_curved_gmsh_import(backend_factory, sorder):
+    """Curved tets and prisms read from a high-order .msh file (Gmsh types
+    11 / 13, 29 / 90) run the RHS of the generated mesh they were written
+    from (which the oracle pins above): same shape points, same pairs."""
+    import io
+    from pyfr_amd.gmsh import GmshMesh
+    from pyfr_amd.mesh import SimplexBoxMesh
+    from pyfr_amd.solvers import get_integrator
+    from tests.msh_files import mesh_to_msh
+    from tests.test_gmsh import _sim_sides
+    n, nt, npr, order = 2, 1, 1, 2
+    L = 2*np.pi
+    flat = SimplexBoxMesh.stacked(n, nt, npr, sorder=sorder)
+    sides = _sim_sides(flat, (L, L, (nt + npr)*L/n))
+    gen = SimplexBoxMesh.stacked(n, nt, npr, sorder=sorder, warp=_warp)
+    mesh = GmshMesh(io.StringIO(mesh_to_msh(gen, sides, '2.2')))
+    print(f'\n[simplex gmsh] k={sorder}', flush=True)
+    out = []
+    for m in (gen, mesh):
+        be = backend_factory()
+        cfg = make_simplex_cfg(order, be.precision, ics=mode_ics(seed=3))
+        integ = get_integrator(be, m, cfg)
+        for e in integ.system.ele_map.values():
+            assert e.curved_simplex
+        integ.system.rhs(0.0, 0, 1)
+        out.append(_cat(dict(zip(integ.system.ele_types,
+                                 integ.system.ele_scal_upts(1))),
+                        integ.system.ele_types))
+    err = rel_l2(out[1], out[0])
+    print(f'[simplex gmsh] rel L2 {err:.3e}', flush=True)
+    assert err < 1e-13
