@@ -15,12 +15,24 @@ reference's reader and nodal assembler (``pyfr/readers/gmsh.py:28-489``,
   index, element index) key, where the codec lists element types in Gmsh
   type-number order (tet 4, hex 5, pri 6) and their faces in order;
 * elements of a type keep their file order, grouped by (Gmsh type,
-  physical entities) sorted, as the reference's volume merge stacks them.
+  physical entities) sorted, as the reference's volume merge stacks them;
+* high-order elements (shape orders 2-4: Gmsh types 11/29/30 tets,
+  12/92/93 hexes, 13/90/91 prisms; faces 9/21/23 and 10/36/37) take their
+  nodes to PyFR's shape-point order through node maps derived from Gmsh's
+  ordering rules (``gmsh_nodemap``; the reference's tables are
+  pyfr/readers/gmsh.py:58-150); connectivity uses their corner nodes
+  (pyfr/readers/base.py:148-163);
+* elements within ``lintol`` of the first-order map of their corners are
+  snapped to it and count as linear, the others as curved, which come
+  first within their type (pyfr/readers/base.py:22-45, 445-482).  Hexes
+  carry the flags in ``spts_curved``; a tet or prism type with any curved
+  element keeps every element's shape points (the solver then takes the
+  per-point metric of all of them), one without keeps the corners only.
 
-Scope: first-order elements (Gmsh types 4, 5, 6; surfaces 2 and 3).
-Higher-order node orderings (curved elements come from the generated
-meshes), pyramids and 2-D meshes are declined.  The ``.pyfrm`` (HDF5) writer is not built: h5py is not
-available here, and the solver takes the mesh object directly.
+Scope: 3-D meshes of tets, hexes and prisms, one shape order per element
+type.  Pyramids and 2-D meshes are declined.  The ``.pyfrm`` (HDF5) writer
+is not built: h5py is not available here, and the solver takes the mesh
+object directly.
 """
 from collections import defaultdict
 import re
@@ -29,15 +41,168 @@ import numpy as np
 
 from pyfr_amd.mesh import FACE_VERTS, fuzzysort_batched
 
-# Gmsh element type -> (PyFR type, node count); first order only
-GMSH_TYPES = {2: ('tri', 3), 3: ('quad', 4), 4: ('tet', 4), 5: ('hex', 8),
-              6: ('pri', 6)}
-# Gmsh corner-node order -> PyFR vertex order (pyfr/readers/gmsh.py:46-150)
+# Gmsh element type -> (PyFR type, node count): the types the reference
+# reads (pyfr/readers/gmsh.py:34-42) for the shapes this solver has
+GMSH_TYPES = {2: ('tri', 3), 9: ('tri', 6), 21: ('tri', 10), 23: ('tri', 15),
+              3: ('quad', 4), 10: ('quad', 9), 36: ('quad', 16),
+              37: ('quad', 25),
+              4: ('tet', 4), 11: ('tet', 10), 29: ('tet', 20),
+              30: ('tet', 35),
+              5: ('hex', 8), 12: ('hex', 27), 92: ('hex', 64),
+              93: ('hex', 125),
+              6: ('pri', 6), 13: ('pri', 18), 90: ('pri', 40),
+              91: ('pri', 75)}
+# pyramids (any order): read by the reference, no pyramid elements here
+_DECLINED = {7, 14, 118, 119}
+NVERTS = {'tri': 3, 'quad': 4, 'tet': 4, 'hex': 8, 'pri': 6}
+
+# Gmsh's reference-element conventions, in its corner numbering: the edges
+# (directed: edge nodes run from the first corner to the second) and the
+# faces (their interior nodes follow the face's corner order)
+_G_EDGES = {
+    'tri': [(0, 1), (1, 2), (2, 0)],
+    'quad': [(0, 1), (1, 2), (2, 3), (3, 0)],
+    'tet': [(0, 1), (1, 2), (2, 0), (3, 0), (3, 2), (3, 1)],
+    'hex': [(0, 1), (0, 3), (0, 4), (1, 2), (1, 5), (2, 3), (2, 6), (3, 7),
+            (4, 5), (4, 7), (5, 6), (6, 7)],
+    'pri': [(0, 1), (0, 2), (0, 3), (1, 2), (1, 4), (2, 5), (3, 4), (3, 5),
+            (4, 5)]}
+_G_FACES = {
+    'tri': [], 'quad': [],
+    'tet': [(0, 2, 1), (0, 1, 3), (0, 3, 2), (3, 1, 2)],
+    'hex': [(0, 3, 2, 1), (0, 1, 5, 4), (0, 4, 7, 3), (1, 2, 6, 5),
+            (2, 3, 7, 6), (4, 5, 6, 7)],
+    'pri': [(0, 2, 1), (3, 4, 5), (0, 1, 4, 3), (0, 3, 5, 2), (1, 2, 5, 4)]}
+# Gmsh corner j -> PyFR vertex: NODEMAP[kind][i] is the Gmsh corner at
+# PyFR vertex i (std_ele(1) order)
 NODEMAP = {'tet': [0, 1, 2, 3], 'hex': [0, 1, 3, 2, 4, 5, 7, 6],
            'pri': [0, 1, 2, 3, 4, 5], 'tri': [0, 1, 2], 'quad': [0, 1, 3, 2]}
-# Higher-order / other types the reference reads but this build declines
-_DECLINED = {8, 26, 27, 9, 21, 23, 10, 36, 37, 11, 29, 30, 12, 92, 93,
-             13, 90, 91, 7, 14, 118, 119}
+
+
+def _inset(kind, V, k):
+    """Corners of the sub-element holding an order-k element's interior
+    nodes: each corner moved 1/k of the way along each of its edges."""
+    out = []
+    for i, v in enumerate(V):
+        nb = [b if a == i else a for a, b in _G_EDGES[kind] if i in (a, b)]
+        out.append(v + sum(V[j] - v for j in nb)/k)
+    return out
+
+
+def _gmsh_points(kind, k, V):
+    """Reference coordinates of an order-k Gmsh element's nodes in Gmsh
+    order, given its corners V in Gmsh corner order.  Gmsh orders nodes
+    recursively: corners; the k - 1 nodes of each edge; each face's
+    interior as an element of that face's shape and order k - 3 (triangles)
+    or k - 2 (quadrilaterals) on the inset face; the volume's interior as an
+    order k - 2 hex or order k - 4 tet on the inset corners, or, for prisms,
+    the interior of the bottom triangle (outer) times the interior of a
+    vertical edge (inner, as an order k - 2 line: both ends, then the
+    rest).  Order 0 is the single centre point."""
+    if k == 0:
+        return [sum(V)/len(V)]
+    pts = list(V)
+    for a, b in _G_EDGES[kind]:
+        pts += [V[a] + (V[b] - V[a])*i/k for i in range(1, k)]
+    if kind in ('tri', 'quad'):
+        inner = k - 3 if kind == 'tri' else k - 2
+        if inner >= 0:
+            pts += _gmsh_points(kind, inner, _inset(kind, V, k))
+        return pts
+    for fv in _G_FACES[kind]:
+        fk = 'tri' if len(fv) == 3 else 'quad'
+        inner = k - 3 if fk == 'tri' else k - 2
+        if inner >= 0:
+            pts += _gmsh_points(fk, inner, _inset(fk, [V[i] for i in fv], k))
+    if kind == 'hex' and k >= 2:
+        pts += _gmsh_points('hex', k - 2, _inset('hex', V, k))
+    elif kind == 'tet' and k >= 4:
+        pts += _gmsh_points('tet', k - 4, _inset('tet', V, k))
+    elif kind == 'pri' and k >= 3:
+        tri = _gmsh_points('tri', k - 3, _inset('tri', V[:3], k))
+        zs = [i for i in range(1, k)]
+        zs = [zs[0], zs[-1], *zs[1:-1]] if len(zs) > 1 else zs
+        z0, z1 = V[0][2], V[3][2]
+        pts += [np.array([t[0], t[1], z0 + (z1 - z0)*i/k])
+                for t in tri for i in zs]
+    return pts
+
+
+def std_ele(kind, k):
+    """PyFR's equispaced order-k shape points of a kind
+    (pyfr/shapes.py: TensorProdShape.std_ele, TriShape / TetShape /
+    PriShape.std_ele): tensor shapes x fastest; simplices x fastest within
+    rows of falling length."""
+    t = np.linspace(-1, 1, k + 1)
+    if kind == 'quad':
+        return np.array([(x, y) for y in t for x in t])
+    if kind == 'hex':
+        return np.array([(x, y, z) for z in t for y in t for x in t])
+    if kind == 'tri':
+        return np.array([(x, y) for i, y in enumerate(t)
+                         for x in t[:k + 1 - i]])
+    if kind == 'tet':
+        return np.array([(x, y, z) for i, z in enumerate(t)
+                         for j, y in enumerate(t[:k + 1 - i])
+                         for x in t[:k + 1 - i - j]])
+    if kind == 'pri':
+        return np.array([(x, y, z) for z in t for i, y in enumerate(t)
+                         for x in t[:k + 1 - i]])
+    raise ValueError(f'Unknown element kind {kind}')
+
+
+def npts_order(kind, n):
+    """Shape order k of a kind with n equispaced points."""
+    for k in range(1, 32):
+        m = len(std_ele(kind, k))
+        if m == n:
+            return k
+        if m > n:
+            break
+    raise ValueError(f'No {kind} of {n} nodes')
+
+
+def gmsh_nodemap(kind, k):
+    """Node map of an order-k element: entry i is the index, in a Gmsh
+    element's node list, of the node at PyFR shape point i
+    (``std_ele(k)``), the form of the reference's tables
+    (pyfr/readers/gmsh.py:58-150), derived here from Gmsh's ordering
+    rules (``_gmsh_points``) by matching reference coordinates."""
+    ref = std_ele(kind, 1)
+    V = [None]*len(ref)
+    for i, j in enumerate(NODEMAP[kind]):
+        V[j] = np.asarray(ref[i], dtype=float)
+    g = np.array(_gmsh_points(kind, k, V))
+    p = std_ele(kind, k)
+    if len(g) != len(p):
+        raise AssertionError(f'{kind} order {k}: {len(g)} Gmsh nodes')
+    d = np.abs(p[:, None, :] - g[None, :, :]).max(axis=2)
+    m = d.argmin(axis=1)
+    if not (np.all(d[np.arange(len(p)), m] < 1e-9) and
+            len(set(m.tolist())) == len(p)):
+        raise AssertionError(f'{kind} order {k}: nodes do not match')
+    return [int(x) for x in m]
+
+
+def _vertex_index(kind, k):
+    """Positions of the PyFR vertices (std_ele(1) order) in std_ele(k)."""
+    p, v = std_ele(kind, k), std_ele(kind, 1)
+    return [int(np.abs(p - x).max(axis=1).argmin()) for x in v]
+
+
+def _linear_weights(kind, k):
+    """(nspts, nverts) values of the first-order nodal basis of a kind
+    (trilinear hex, affine tet, prism affine in x, y times linear in z) at
+    its order-k shape points."""
+    def mono(x):
+        x, y, z = x[:, 0], x[:, 1], x[:, 2]
+        one = np.ones_like(x)
+        if kind == 'hex':
+            return np.stack([one, x, y, z, x*y, x*z, y*z, x*y*z], 1)
+        if kind == 'tet':
+            return np.stack([one, x, y, z], 1)
+        return np.stack([one, x, y, z, x*z, y*z], 1)
+    return mono(std_ele(kind, k)) @ np.linalg.inv(mono(std_ele(kind, 1)))
 
 
 FACES = FACE_VERTS
@@ -69,7 +234,8 @@ class _Lines:
 class GmshMesh:
     ndims = 3
 
-    def __init__(self, msh):
+    def __init__(self, msh, lintol=1e-5):
+        self.lintol = lintol
         if isinstance(msh, str):
             with open(msh) as f:
                 self._read(_Lines(f))
@@ -163,12 +329,11 @@ class GmshMesh:
         it.expect('$EndNodes')
 
     def _add_ele(self, gtype, pents, nodes):
-        if gtype in (1, 15):
-            return              # first-order lines and points: not used
+        if gtype in (1, 8, 26, 27, 15):
+            return              # lines and points: not used in 3-D
         if gtype in _DECLINED:
             raise NotImplementedError(
-                f'Gmsh element type {gtype} is not supported (first-order '
-                'tets, hexes and prisms only)')
+                f'Gmsh element type {gtype} (pyramid) is not supported')
         if gtype not in GMSH_TYPES:
             raise ValueError(f'Unsupported element type {gtype}')
         if len(nodes) != GMSH_TYPES[gtype][1]:
@@ -219,30 +384,39 @@ class GmshMesh:
         if any(set(p) != {'l', 'r'} for p in per.values()):
             raise ValueError('Unpaired periodic boundary in mesh')
 
-        # volume elements per PyFR type, (gmsh type, pents) groups sorted
-        spts, order = {}, []
-        vnodes = defaultdict(list)
+        # volume elements per PyFR type, (gmsh type, pents) groups sorted;
+        # the types in the order of their Gmsh type numbers (the codec's)
+        vnodes, hnodes, sorders = defaultdict(list), defaultdict(list), {}
         surf = defaultdict(list)
+        etypes = []
         for (gtype, pents), rows in sorted(self._elenodes.items()):
-            petype = GMSH_TYPES[gtype][0]
+            petype, nn = GMSH_TYPES[gtype]
             arr = np.array(rows, dtype=np.int64) - self._nodeoff
-            arr = arr[:, NODEMAP[petype]]
+            # first-order representation: the corners, PyFR vertex order
+            corners = arr[:, :NVERTS[petype]][:, NODEMAP[petype]]
             if any(p in vol for p in pents):
-                vnodes[petype].append(arr)
+                if petype not in ETYPE_ORDER:
+                    raise ValueError(f'{petype} elements in a 3-D volume')
+                k = npts_order(petype, nn)
+                if sorders.setdefault(petype, k) != k:
+                    raise NotImplementedError(
+                        f'{petype} elements of more than one shape order')
+                if petype not in etypes:
+                    etypes.append(petype)
+                vnodes[petype].append(corners)
+                hnodes[petype].append(arr[:, gmsh_nodemap(petype, k)])
             else:
                 for p in pents:
-                    surf[p].append(arr)
-        self.etypes = [t for t in ETYPE_ORDER if t in vnodes]
-        if set(vnodes) - set(self.etypes):
-            raise NotImplementedError('Unsupported volume element type')
-        self._vnodes = {t: np.vstack(vnodes[t]) for t in self.etypes}
-        for t, nd in self._vnodes.items():
-            pts = self._nodes[nd]                         # (neles, nv, 3)
-            if np.isnan(pts).any():
+                    surf[p].append(corners)
+        self.etypes = etypes
+        self._vnodes = {t: np.vstack(vnodes[t]) for t in etypes}
+        hn = {t: np.vstack(hnodes[t]) for t in etypes}
+        for nd in hn.values():
+            if np.isnan(self._nodes[nd]).any():
                 raise ValueError('Element references an undefined node')
-            spts[t] = pts.swapaxes(0, 1).copy()
-        self.spts = spts
         self.neles = sum(v.shape[0] for v in self._vnodes.values())
+        self.sorders = sorders
+        nodes, curved = self._linearise(hn)
 
         # every face of every volume element keyed by its sorted nodes
         owner = {}
@@ -309,4 +483,47 @@ class GmshMesh:
             lhs.append(a)
             rhs.append(b)
         srt = sorted(range(len(lhs)), key=lambda i: key(lhs[i]))
-        self.con = ([lhs[i] for i in srt], [rhs[i] for i in srt])
+        lhs, rhs = [lhs[i] for i in srt], [rhs[i] for i in srt]
+
+        # curved elements first within each type (stable), the element
+        # numbers of the interfaces and boundaries renumbered to match
+        self.spts, newidx = {}, {}
+        for t in etypes:
+            perm = np.argsort(~curved[t], kind='stable')
+            newidx[t] = np.argsort(perm)
+            sp = nodes[hn[t][perm]].swapaxes(0, 1)       # (nspts, neles, 3)
+            if not curved[t].any():
+                sp = sp[_vertex_index(t, sorders[t])]
+            elif t == 'hex':
+                self.spts_curved = {'hex': curved[t][perm]}
+            self.spts[t] = np.ascontiguousarray(sp)
+
+        def renum(q):
+            t, e, f = q
+            return (t, int(newidx[t][e]), f)
+        self.con = ([renum(q) for q in lhs], [renum(q) for q in rhs])
+        self.bcon = {n: [renum(q) for q in fs] for n, fs in self.bcon.items()}
+
+    def _linearise(self, hn):
+        """Snap the elements within ``lintol`` of the first-order map of
+        their corners to it (pyfr/readers/base.py:445-482): per element and
+        axis, the largest deviation of a node from that map over the
+        element's extent must be below lintol.  Returns the snapped node
+        array and the curved flags per type."""
+        nodes = self._nodes.copy()
+        curved = {}
+        for t in self.etypes:
+            idx, k = hn[t], self.sorders[t]
+            if k == 1:
+                curved[t] = np.zeros(len(idx), dtype=bool)
+                continue
+            e = nodes[idx]                               # (neles, nsp, 3)
+            W = _linear_weights(t, k)                    # (nsp, nverts)
+            le = np.einsum('pv,evd->epd', W, e[:, _vertex_index(t, k)])
+            num = np.abs(e - le).max(axis=1)
+            den = np.ptp(e, axis=1)
+            with np.errstate(divide='ignore', invalid='ignore'):
+                lin = np.all(num/den < self.lintol, axis=1)
+            nodes[idx[lin]] = le[lin]
+            curved[t] = ~lin
+        return nodes, curved
