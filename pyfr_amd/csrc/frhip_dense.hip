@@ -60,6 +60,17 @@ __device__ __forceinline__ V4<double>::type mfma4(double a, double b,
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+// Workgroup index with each XCD given a contiguous range: workgroups are
+// dispatched to the 8 XCDs round-robin (block b to XCD b % 8), so
+// consecutive indices, whose wavefronts read the same element block's data,
+// would otherwise sit in different L2s and fetch it from HBM once each.
+__device__ __forceinline__ unsigned xcd_block()
+{
+    const unsigned b = blockIdx.x, nb = gridDim.x;
+    const unsigned per = nb/8, rem = nb % 8, x = b % 8;
+    return x*per + min(x, rem) + b/8;
+}
+
 // The RK4 step of epilogues 10..13: a device double (xa's third pointer),
 // so that captured graphs survive a change of step; read once per kernel,
 // as stores through the other pointers could alias it.
@@ -686,7 +697,7 @@ __attribute__((amdgpu_waves_per_eu(G <= 10 ? 4 : 2))) void k_tdivf_flux4(
 {
     dta = rk4_dt<EP>(xa, dta);
     const int lane = threadIdx.x & 63;
-    const long w = (long)blockIdx.x*(blockDim.x >> 6) + (threadIdx.x >> 6);
+    const long w = (long)xcd_block()*(blockDim.x >> 6) + (threadIdx.x >> 6);
     if (w*64 >= ncol) return;
     const long c0 = w*64;
     const int v = (int)((c0 >> 6) % 5);
