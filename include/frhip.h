@@ -69,6 +69,12 @@ int frhip_device_sync(void);
 int frhip_event_create(void **event);
 int frhip_event_destroy(void *event);
 int frhip_event_record(void *event, void *stream);
+/* Work later enqueued on stream waits for event (hipStreamWaitEvent).  On a
+ * capturing stream's event this joins the capture: the fork and join edges
+ * of a graph's independent kernel chains, in place of the explicit node
+ * dependencies of the reference's graph builder
+ * (pyfr/backends/hip/driver.py:303-399). */
+int frhip_stream_wait_event(void *stream, void *event);
 int frhip_event_sync(void *event);
 /* *done = 1 if all work before the event has completed, else 0. */
 int frhip_event_query(void *event, int *done);

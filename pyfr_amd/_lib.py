@@ -39,6 +39,7 @@ _FUNCS = {
     'frhip_event_create': [ct.POINTER(_P)],
     'frhip_event_destroy': [_P],
     'frhip_event_record': [_P, _P],
+    'frhip_stream_wait_event': [_P, _P],
     'frhip_event_sync': [_P],
     'frhip_event_query': [_P, ct.POINTER(_I)],
     'frhip_event_elapsed': [ct.POINTER(ct.c_float), _P, _P],

@@ -415,6 +415,28 @@ class MetaKernel(Kernel):
         pass
 
 
+class Concurrent:
+    """Kernel chains with no dependencies between them: each chain runs in
+    order, all of them after what precedes the group in a graph and before
+    what follows (per element type: the disu of tets and of prisms, their
+    divergence kernels).  The reference states such independence as node
+    dependencies (``pyfr/backends/base/types.py:343-533``)."""
+
+    def __init__(self, chains):
+        self.chains = [list(c) for c in chains if c]
+
+    def kernels(self):
+        return [k for c in self.chains for k in c]
+
+
+def flat_kernels(items):
+    """The kernels of a list of kernels and Concurrent groups, in order."""
+    out = []
+    for k in items:
+        out += k.kernels() if isinstance(k, Concurrent) else [k]
+    return out
+
+
 class Graph:
     """Kernel DAG replayed as a hipGraph (``types.py:343-533``,
     ``hip/types.py:79-116``).
@@ -422,7 +444,10 @@ class Graph:
     Kernels are recorded in insertion order, which respects the declared
     dependencies on a single stream.  On first run (and whenever a member
     kernel's bound arguments change) the sequence is captured into a HIP
-    graph; later runs launch that graph."""
+    graph; later runs launch that graph.  The chains of a Concurrent group
+    are captured on side streams forked from and joined back to the
+    capturing stream, so the graph holds them as parallel branches; run
+    eagerly (no graphs), everything runs in order on one stream."""
 
     def __init__(self, backend):
         self.backend = backend
@@ -431,10 +456,14 @@ class Graph:
         self.committed = False
         self._exec = None
         self._versions = None
+        self._groups = {}           # klist index -> chain lengths
+        self._events = []
 
     def add(self, kern, deps=[], pdeps=[]):
         if self.committed:
             raise RuntimeError('Can not add nodes to a committed graph')
+        if isinstance(kern, Concurrent):
+            return self.add_concurrent(kern.chains)
         for d in deps:
             if d not in self.knodes:
                 raise ValueError('Dependency not in graph')
@@ -445,6 +474,14 @@ class Graph:
         for k in kerns:
             self.add(k, deps)
 
+    def add_concurrent(self, chains):
+        chains = [list(c) for c in chains if c]
+        if len(chains) > 1:
+            self._groups[len(self.klist)] = [len(c) for c in chains]
+        for c in chains:
+            for k in c:
+                self.add(k)
+
     def add_mpi_req(self, req, deps=[]):
         raise NotImplementedError('MPI is not supported on this backend')
 
@@ -454,6 +491,43 @@ class Graph:
 
     def commit(self):
         self.committed = True
+
+    def _event(self, i):
+        while len(self._events) <= i:
+            e = ct.c_void_p()
+            self.backend.lib.event_create(ct.byref(e))
+            self._events.append(e.value)
+        return self._events[i]
+
+    def _capture(self, stream):
+        """Launch the kernels on a capturing stream; the chains of each
+        concurrent group beyond the first on side streams that wait for an
+        event recorded before the group, and that the stream waits for
+        after it."""
+        be, lib = self.backend, self.backend.lib
+        i, nev, n = 0, 0, len(self.klist)
+        while i < n:
+            sizes = self._groups.get(i)
+            if not sizes:
+                self.klist[i].launch(stream)
+                i += 1
+                continue
+            side = be.side_streams(len(sizes) - 1)
+            fork = self._event(nev)
+            nev += 1
+            lib.event_record(fork, stream)
+            for s in side:
+                lib.stream_wait_event(s, fork)
+            for j, sz in enumerate(sizes):
+                st = stream if j == 0 else side[j - 1]
+                for k in self.klist[i:i + sz]:
+                    k.launch(st)
+                i += sz
+            for s in side:
+                join = self._event(nev)
+                nev += 1
+                lib.event_record(join, s)
+                lib.stream_wait_event(stream, join)
 
     def _cur_versions(self):
         return tuple(k.version for k in self.klist)
@@ -474,8 +548,7 @@ class Graph:
             be.ncaptures += 1
             be.lib.graph_begin(stream)
             try:
-                for k in self.klist:
-                    k.launch(stream)
+                self._capture(stream)
             finally:
                 ge = ct.c_void_p()
                 be.lib.graph_end(stream, ct.byref(ge))
@@ -486,6 +559,8 @@ class Graph:
         try:
             if self._exec is not None:
                 self.backend.lib.graph_destroy(self._exec)
+            for e in self._events:
+                self.backend.lib.event_destroy(e)
         except Exception:
             pass
 
@@ -1251,6 +1326,9 @@ class AMDBackend:
         s = ct.c_void_p()
         self.lib.stream_create(ct.byref(s))
         self.stream = s.value
+        # side streams for the concurrent branches of captured graphs
+        # (default priority, like the main stream)
+        self._side = []
 
         self.mats = WeakValueDictionary()
         self._mat_counter = it.count()
@@ -1357,6 +1435,14 @@ class AMDBackend:
     def graph(self):
         return Graph(self)
 
+    def side_streams(self, n):
+        """n streams besides the main one, created on first use."""
+        while len(self._side) < n:
+            st = ct.c_void_p()
+            self.lib.stream_create(ct.byref(st))
+            self._side.append(st.value)
+        return self._side[:n]
+
     def run_kernels(self, kernels, wait=False):
         for k in kernels:
             k.run(self.stream)
@@ -1384,6 +1470,8 @@ class AMDBackend:
                 self.lib.host_free(p)
             for p in self._programs:
                 self.lib.expr_program_destroy(p)
+            for st in self._side:
+                self.lib.stream_destroy(st)
             self.lib.stream_destroy(self.stream)
         except Exception:
             pass

@@ -218,6 +218,15 @@ int frhip_event_destroy(void *event)
     FR_API_END
 }
 
+int frhip_stream_wait_event(void *stream, void *event)
+{
+    FR_API_BEGIN
+    FR_HIP(hipStreamWaitEvent(as_stream(stream),
+                              static_cast<hipEvent_t>(event), 0));
+    return 0;
+    FR_API_END
+}
+
 int frhip_event_record(void *event, void *stream)
 {
     FR_API_BEGIN

@@ -1319,7 +1319,7 @@ class BaseSystem:
         return self._kcache[key]
 
     def _try_fused(self, uin, fout, xa=None, stage=None):
-        from pyfr_amd.backend import NotSuitableError
+        from pyfr_amd.backend import NotSuitableError, flat_kernels
         try:
             ks = self._fused_kernels(uin, fout, xa, stage)
         except NotSuitableError:
@@ -1328,7 +1328,7 @@ class BaseSystem:
         g.add_all(ks)
         g.commit()
         # every element type's stage kernel takes the stage arguments
-        g.stage_kernels = [k for k in ks if 'dtb' in k.rtargs]
+        g.stage_kernels = [k for k in flat_kernels(ks) if 'dtb' in k.rtargs]
         return [g]
 
     @property
@@ -1394,6 +1394,7 @@ class BaseSystem:
         the precision form's FP64 updates).  The epilogues read dt from
         device memory (bind writes it on the stream), so each stage's graph
         is captured once, whatever the steps before an output time."""
+        from pyfr_amd.backend import flat_kernels
         key = ('rk4', stage, uin, nxt, r1, r0, mixed)
         if key not in self._kcache:
             ks = self._fused_kernels(uin, nxt, (r1, r0),
@@ -1401,7 +1402,8 @@ class BaseSystem:
             g = self.backend.graph()
             g.add_all(ks)
             g.commit()
-            g.stage_kernels = [k for k in ks if 'dta' in k.devargs]
+            g.stage_kernels = [k for k in flat_kernels(ks)
+                               if 'dta' in k.devargs]
             self._kcache[key] = g
         g = self._kcache[key]
         for k in g.stage_kernels:
@@ -1496,32 +1498,41 @@ class EulerSystem(BaseSystem):
     bcinterscls = EulerBCInters
 
     def _fused_kernels(self, uin, fout, xa, stage):
-        from pyfr_amd.backend import NotSuitableError
+        """The fused RHS as kernels and Concurrent groups: the element
+        types' kernels of one phase touch only their own arrays, so a
+        captured graph runs them side by side."""
+        from pyfr_amd.backend import Concurrent, NotSuitableError
         eles = list(self.ele_map.values())
-        ks = [e.kernels['disu'](uin) for e in eles]
+        ks = [Concurrent([[e.kernels['disu'](uin)] for e in eles])]
         ks += [i.kernels['comm_flux']() for i in self._inters]
+        chains = []
         for e in eles:
             if 'tdivf_flux' in e.kernels:
                 try:
-                    ks.append(e.kernels['tdivf_flux'](uin, fout, xa, stage))
+                    chains.append([e.kernels['tdivf_flux'](uin, fout, xa,
+                                                           stage)])
                     continue
                 except NotSuitableError:
                     pass
-            ks += [e.kernels['tdisf'](uin),
-                   e.kernels['tdivf'](fout, xa, stage)]
+            chains.append([e.kernels['tdisf'](uin),
+                           e.kernels['tdivf'](fout, xa, stage)])
+        ks.append(Concurrent(chains))
         return ks
 
     def _build_graphs(self, uin, fout):
         """One graph holding ``g_intf`` then ``g_flux_div`` of
         ``pyfr/solvers/baseadvec/system.py:57-136`` (no MPI to overlap)."""
+        from pyfr_amd.backend import Concurrent
         be = self.backend
         g = be.graph()
         eles = list(self.ele_map.values())
-        ks = [e.kernels['disu'](uin) for e in eles]
+        ks = [Concurrent([[e.kernels['disu'](uin)] for e in eles])]
         ks += [i.kernels['comm_flux']() for i in self._inters]
-        for e in eles:
-            ks += [e.kernels['tdisf'](uin), e.kernels['tdivtpcorf'](fout),
-                   e.kernels['tdivtconf'](fout), e.kernels['negdivconf'](fout)]
+        ks.append(Concurrent([[e.kernels['tdisf'](uin),
+                               e.kernels['tdivtpcorf'](fout),
+                               e.kernels['tdivtconf'](fout),
+                               e.kernels['negdivconf'](fout)]
+                              for e in eles]))
         g.add_all(ks)
         g.commit()
         return [g]
@@ -1539,36 +1550,38 @@ class NavierStokesSystem(BaseSystem):
                 if 'copy_fpts' in e.kernels]
 
     def _fused_kernels(self, uin, fout, xa, stage):
+        from pyfr_amd.backend import Concurrent
         eles = list(self.ele_map.values())
-        ks = [e.kernels['disu'](uin) for e in eles]
-        ks += self._copy_fpts(eles)
+        ks = [Concurrent([[e.kernels['disu'](uin)] + self._copy_fpts([e])
+                          for e in eles])]
         ks += [i.kernels['con_u']() for i in self._inters]
-        for e in eles:
-            ks += [e.kernels['tgradf'](uin), e.kernels['tdisf_fused'](uin),
-                   e.kernels['gradcoru_fpts']()]
+        ks.append(Concurrent([[e.kernels['tgradf'](uin),
+                               e.kernels['tdisf_fused'](uin),
+                               e.kernels['gradcoru_fpts']()] for e in eles]))
         ks += [i.kernels['comm_flux']() for i in self._inters]
-        for e in eles:
-            ks += [e.kernels['tdivf'](fout, xa, stage)]
+        ks.append(Concurrent([[e.kernels['tdivf'](fout, xa, stage)]
+                              for e in eles]))
         return ks
 
     def _build_graphs(self, uin, fout):
         """``pyfr/solvers/baseadvecdiff/system.py:53-231`` with gradient
         fusion, as one graph."""
+        from pyfr_amd.backend import Concurrent
         be = self.backend
         g = be.graph()
         eles = list(self.ele_map.values())
-        ks = [e.kernels['disu'](uin) for e in eles]
-        ks += self._copy_fpts(eles)
+        ks = [Concurrent([[e.kernels['disu'](uin)] + self._copy_fpts([e])
+                          for e in eles])]
         ks += [i.kernels['con_u']() for i in self._inters]
-        for e in eles:
-            ks += [e.kernels['tgradpcoru_upts'](uin),
-                   e.kernels['tgradcoru_upts'](),
-                   e.kernels['tdisf_fused'](uin),
-                   e.kernels['gradcoru_fpts']()]
+        ks.append(Concurrent([[e.kernels['tgradpcoru_upts'](uin),
+                               e.kernels['tgradcoru_upts'](),
+                               e.kernels['tdisf_fused'](uin),
+                               e.kernels['gradcoru_fpts']()] for e in eles]))
         ks += [i.kernels['comm_flux']() for i in self._inters]
-        for e in eles:
-            ks += [e.kernels['tdivtpcorf'](fout), e.kernels['tdivtconf'](fout),
-                   e.kernels['negdivconf'](fout)]
+        ks.append(Concurrent([[e.kernels['tdivtpcorf'](fout),
+                               e.kernels['tdivtconf'](fout),
+                               e.kernels['negdivconf'](fout)]
+                              for e in eles]))
         g.add_all(ks)
         g.commit()
         return [g]

@@ -728,3 +728,21 @@ def test_simplex_curved_gmsh_import(backend_factory, sorder):
     err = rel_l2(out[1], out[0])
     print(f'[simplex gmsh] rel L2 {err:.3e}', flush=True)
     assert err < 1e-13
+
+
+def test_concurrent_graph_branches_tet_prism(backend_factory):
+    """Tets and prisms (the dense path, RK4 stages fused): the graphs with
+    per-type branches against the eager one-stream run, bit for bit."""
+    print('\n[concurrent graph] tet/prism rk4', flush=True)
+    runs = []
+    for graphs in (True, False):
+        be = backend_factory(graphs=graphs)
+        integ, _, _ = _mixed(be, 3, 2, 1, 3, dt=2e-3)
+        assert integ.system.fused_rk4
+        out = []
+        for _ in range(2):
+            integ.run_steps(5, 2e-3)
+            out.append(np.concatenate([a.ravel() for a in integ.soln]))
+        runs.append(out)
+    for a, b in zip(*runs):
+        assert np.array_equal(a, b)
