@@ -439,11 +439,65 @@ def rk45_step(rhs, u, dt, t=None):
     return r2
 
 
+RK34_A = [11847461282814 / 36547543011857, 3943225443063 / 7078155732230,
+          -346793006927 / 4029903576067]
+RK34_B = [1017324711453 / 9774461848756, 8237718856693 / 13685301971492,
+          57731312506979 / 19404895981398,
+          -101169746363290 / 37734290219643]
+
+
+def rkvdh2_step(rhs, u, dt, a, b, t=None):
+    """One step of a 2-register low-storage scheme (``steppers.py:111-190``,
+    ``kernels/rkvdh2.mako:4-31``) with coefficients a, b."""
+    n = len(b)
+    c = [0.0] + [sum(b[:i]) + ai for i, ai in enumerate(a)]
+    r1, r2 = u.copy(), None
+    for i in range(n):
+        v = r1 if i == 0 else r2
+        r2 = rhs(v) if t is None else rhs(v, t + c[i]*dt)
+        if i < n - 1:
+            n1 = r1 + (dt*a[i])*r2
+            n2 = r1 + (dt*b[i])*r2
+            r1, r2 = n1, n2
+        else:
+            r1 = r1 + (dt*b[i])*r2
+        r1, r2 = r2, r1
+    return r2
+
+
+def rk34_step(rhs, u, dt, t=None):
+    """Low-storage RK34 (``steppers.py:193-215``)."""
+    return rkvdh2_step(rhs, u, dt, RK34_A, RK34_B, t)
+
+
+def euler_step(rhs, u, dt, t=None):
+    """Forward Euler (``steppers.py:11-23``): u + dt f(u)."""
+    f = rhs(u) if t is None else rhs(u, t)
+    return 1.0*u + dt*f
+
+
+def tvdrk3_step(rhs, u, dt, t=None):
+    """TVD-RK3 (``steppers.py:26-57``) with the reference's axnpby operand
+    order; stage times t, t + dt, t + dt/2."""
+    g = (lambda v, c: rhs(v)) if t is None else (lambda v, c: rhs(v, t + c*dt))
+    r0 = u
+    r2 = g(r0, 0.0)
+    r1 = 1.0*r0 + dt*r2
+    r2 = g(r1, 1.0)
+    r1 = 0.25*r1 + 0.75*r0 + (0.25*dt)*r2
+    r2 = g(r1, 0.5)
+    r1 = (2.0/3.0)*r1 + (1.0/3.0)*r0 + (2.0/3.0*dt)*r2
+    return r1
+
+
+STEPS = {'euler': euler_step, 'tvd-rk3': tvdrk3_step, 'rk34': rk34_step}
+
+
 def advance(rhs, u, dt, tend, scheme='rk4', tstart=0.0, timed=False):
     """Fixed-dt ``none`` controller with Kahan-summed time
     (``explicit/controllers.py:48-60``, ``integrators/base.py:282-296``).
     timed: call rhs(u, stage time).  Returns (u, nsteps)."""
-    step = rk4_step if scheme == 'rk4' else rk45_step
+    step = {'rk4': rk4_step, 'rk45': rk45_step, **STEPS}[scheme]
     tcurr, comp, nsteps = tstart, 0.0, 0
     while tcurr < tend:
         h = clamp_dt(dt, tend, tcurr)

@@ -1393,10 +1393,11 @@ static int flux_launch(const FluxPlan *p, int ep, long ncol, const void *u,
         !variant_valid(wpb, (size_t)p->d.mt*(p->d.ks0 + p->d.ks1)*64*esz,
                        false))
         wpb &= ~FR_LDSV;
-    // an RHS in place (c == u): only the block-cooperative form keeps an
-    // element block's five wavefronts in one block, whose barriers order
-    // all reads of u before any write of c
-    if (c == u && ep != 13)
+    // an RHS in place (c == u), or an RK45 stage whose accumulator is the
+    // input (xa == u: the first stage, rkvdh2 with r1 the state): only the
+    // block-cooperative form keeps an element block's five wavefronts in
+    // one block, whose barriers order all reads of u before any write
+    if ((c == u && ep != 13) || (xa == u && ep >= 2 && ep <= 5))
         wpb = FR_BLKV;
     const int rc = p->d.dtype == FRHIP_F64
         ? launch_flux_t<double>(p, wpb, ep, ncol, u, ldu, sm, lds, fn, ldf, c,

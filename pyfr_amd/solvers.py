@@ -1862,8 +1862,67 @@ class MixedRK45Integrator(RK45Integrator):
         return 0
 
 
-INTEGRATORS = {'rk4': RK4Integrator, 'rk45': RK45Integrator}
-MIXED_INTEGRATORS = {'rk4': MixedRK4Integrator, 'rk45': MixedRK45Integrator}
+class RK34Integrator(RK45Integrator):
+    """Low-storage RK34 (``steppers.py:193-215``): the rkvdh2 scheme with
+    four stages, fused stage updates where RK45's are."""
+    nrhs_per_step = 4
+    a = [11847461282814 / 36547543011857, 3943225443063 / 7078155732230,
+         -346793006927 / 4029903576067]
+    b = [1017324711453 / 9774461848756, 8237718856693 / 13685301971492,
+         57731312506979 / 19404895981398, -101169746363290 / 37734290219643]
+
+
+class MixedRK34Integrator(MixedRK45Integrator):
+    """RK34 for the precision form (FP64 accumulated solution)."""
+    nrhs_per_step = 4
+    a, b = RK34Integrator.a, RK34Integrator.b
+
+
+class EulerIntegrator(BaseIntegrator):
+    """Forward Euler (``steppers.py:11-23``): the state stays in one
+    register, the RHS goes to the other, ``add(1, u, dt, f)``.  Where the
+    RK45 stages are fused, the update is that epilogue's last-stage form
+    (u += dt f in the divergence kernel)."""
+    nregs = 2
+    nrhs_per_step = 1
+
+    def step(self, t, dt):
+        ut = self.idxcurr
+        f = 1 - ut
+        if self.system.fused_stages:
+            self.system.rhs_stage(t, ut, f, ut, 0, 0.0, dt, True)
+            self.nrhsevals += 1
+            return ut
+        self._rhs(t, ut, f)
+        self._add(1.0, ut, dt, f)
+        return ut
+
+
+class TVDRK3Integrator(BaseIntegrator):
+    """TVD-RK3 (``steppers.py:26-57``), three registers, the reference's
+    combinations and operand order."""
+    nregs = 3
+    nrhs_per_step = 3
+
+    def step(self, t, dt):
+        add, rhs = self._add, self._rhs
+        r0, r1, r2 = 0, 1, 2
+        if r0 != self.idxcurr:
+            r0, r1 = r1, r0
+        rhs(t, r0, r2)
+        add(0.0, r1, 1.0, r0, dt, r2)
+        rhs(t + dt, r1, r2)
+        add(0.25, r1, 0.75, r0, 0.25*dt, r2)
+        rhs(t + 0.5*dt, r1, r2)
+        add(2.0/3.0, r1, 1.0/3.0, r0, 2.0/3.0*dt, r2)
+        return r1
+
+
+INTEGRATORS = {'euler': EulerIntegrator, 'tvd-rk3': TVDRK3Integrator,
+               'rk4': RK4Integrator, 'rk34': RK34Integrator,
+               'rk45': RK45Integrator}
+MIXED_INTEGRATORS = {'rk4': MixedRK4Integrator, 'rk34': MixedRK34Integrator,
+                     'rk45': MixedRK45Integrator}
 
 
 def get_integrator(backend, mesh, cfg):
@@ -1873,6 +1932,6 @@ def get_integrator(backend, mesh, cfg):
     if getattr(backend, 'precision', 'double') == 'mixed':
         if scheme not in MIXED_INTEGRATORS:
             raise NotImplementedError('The mixed precision form is '
-                                      'implemented for rk4 and rk45')
+                                      'implemented for rk4, rk34 and rk45')
         return MIXED_INTEGRATORS[scheme](backend, mesh, cfg)
     return INTEGRATORS[scheme](backend, mesh, cfg)

@@ -167,6 +167,32 @@ def test_ns_rk45_steps(backend_factory, fused):
     assert err < 1e-12
 
 
+@pytest.mark.parametrize('fused', [True, False])
+@pytest.mark.parametrize('scheme,system', [('euler', 'euler'),
+                                           ('tvd-rk3', 'euler'),
+                                           ('rk34', 'euler'),
+                                           ('rk34', 'navier-stokes'),
+                                           ('euler', 'navier-stokes')])
+def test_other_steppers(backend_factory, scheme, system, fused):
+    """Forward Euler, TVD-RK3 and RK34 (``steppers.py:11-57``, ``:193-215``)
+    against the oracle's steppers over 20 steps; fused, Euler's and RK34's
+    updates ride in the divergence epilogue as RK45's do."""
+    be = backend_factory(fused=fused)
+    integ, orc = _systems(be, system, 4, 3, scheme=scheme)
+    if scheme != 'tvd-rk3':
+        assert integ.system.fused_stages == fused
+    dt = 2e-4 if system == 'navier-stokes' else 1e-3
+    integ.dt = dt
+    integ.advance_to(20*dt)
+    u_ref, nsteps = advance(orc.rhs, orc.ics(), dt, 20*dt, scheme)
+    assert integ.nsteps == nsteps
+    assert integ.nrhsevals == nsteps*integ.nrhs_per_step
+    err = rel_l2(integ.soln[0], u_ref)
+    print(f'\n[{scheme}] {system} {nsteps} steps rel L2 = {err:.3e}',
+          flush=True)
+    assert err < 1e-12
+
+
 def test_graph_replay_matches_eager(backend_factory):
     """The captured-graph path equals the eager launch sequence on every
     replay, with eager work and synchronisation between replays."""

@@ -82,13 +82,13 @@ def test_base_state_refusals(backend_factory):
     assert get_integrator(be, BoxMesh(4), cfg).system.base is None
 
 
-@pytest.mark.parametrize('scheme,nsteps', [('rk45', 20), ('rk4', 20)])
+@pytest.mark.parametrize('scheme,nsteps', [('rk45', 20), ('rk4', 20),
+                                           ('rk34', 20)])
 def test_precision_form_per_field(backend_factory, scheme, nsteps):
     """NS hex p=3, 4^3, dt = 2e-4: the precision form (fused and unfused)
     against FP64, per field; the full-state form (base-state off) for
     comparison; FP64 conservation sums of the FP64 state."""
-    from pyfr_amd.solvers import MixedRK4Integrator, MixedRK45Integrator
-    mixed_cls = {'rk4': MixedRK4Integrator, 'rk45': MixedRK45Integrator}
+    from pyfr_amd.solvers import MIXED_INTEGRATORS as mixed_cls
     sols = {}
     for prec, fused, base in (('double', True, 'auto'),
                               ('mixed', True, 'auto'),

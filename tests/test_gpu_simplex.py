@@ -573,7 +573,8 @@ def test_flux_fused_in_place(backend_factory, precision):
     RK steppers call it): the flux-fused kernel reads u and writes its
     output in one launch, so an in-place call runs the block-cooperative
     form, whose barriers order all reads of u before any write, whatever
-    the plan's variant.  Checked on 8,192 element blocks (more blocks than
+    the plan's variant; so does an RK45 stage whose accumulator is the
+    input (xa == u).  Checked on 8,192 element blocks (more blocks than
     the device holds at once) for every variant: the in-place result
     equals, bit for bit, that form's out-of-place result."""
     import ctypes as ct
@@ -640,6 +641,39 @@ def test_flux_fused_in_place(backend_factory, precision):
             lib.memcpy_dtod(dw, du, sz, None)
             got = run(dw, dw)
             assert np.array_equal(got, want), v
+
+        # RK45 stage epilogues with the accumulator the input (xa == u,
+        # the first stage): against the same form with xa a copy of u
+        dx = ct.c_void_p()
+        lib.malloc(ct.byref(dx), sz)
+        bufs['x'] = dx
+
+        def run_rk(src, xa, dst, last):
+            lib.tdivf_flux_euler(plan, ncol, src, ncol, dsm, neles, dfn,
+                                 ncol, dst, ncol, 2, drcp, 0, xa, 0.3, -0.7,
+                                 last, 1.4, None)
+            lib.stream_sync(None)
+            outs = []
+            for q in (dst, xa):
+                o = np.empty_like(u)
+                lib.memcpy_dtoh(o.ctypes.data, q, sz, None)
+                outs.append(o)
+            return outs
+
+        for last in (0, 1):
+            lib.tdivf_plan_set_variant(plan, 32)
+            lib.memcpy_dtod(dx, du, sz, None)
+            want_c, want_x = run_rk(du, dx, dc, last)
+            for v in variants:
+                print(f'[xa == u] {precision} variant {v} last={last}',
+                      flush=True)
+                lib.tdivf_plan_set_variant(plan, v)
+                lib.memcpy_dtod(dw, du, sz, None)
+                lib.memcpy_dtod(dx, du, sz, None)
+                got_c, got_x = run_rk(dw, dw, dx, last)
+                assert np.array_equal(got_x, want_x), v
+                if not last:
+                    assert np.array_equal(got_c, want_c), v
     finally:
         for p in bufs.values():
             lib.free(p)
@@ -746,3 +780,33 @@ def test_concurrent_graph_branches_tet_prism(backend_factory):
         runs.append(out)
     for a, b in zip(*runs):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize('scheme', ['euler', 'tvd-rk3', 'rk34'])
+@pytest.mark.parametrize('precision', ['double', 'single'])
+def test_simplex_other_steppers(backend_factory, scheme, precision):
+    """Forward Euler, TVD-RK3 and RK34 on tets and prisms (flux-fused
+    kernels; Euler and RK34 with the stage update in the epilogue, whose
+    accumulator is the RHS input), 10 steps against the oracle (FP64) and
+    the unfused kernel sequence (both precisions)."""
+    from oracle.rhs import STEPS
+    be = backend_factory(precision=precision)
+    integ, orc, u0 = _mixed(be, 3, 2, 1, 3, scheme=scheme, dt=2e-3)
+    names = integ.system.ele_types
+    print(f'\n[simplex {scheme}] {precision}', flush=True)
+    integ.run_steps(10, 2e-3)
+    got = _cat(dict(zip(names, integ.soln)), names)
+    ref_be = backend_factory(precision=precision, fused=False)
+    ref, _, _ = _mixed(ref_be, 3, 2, 1, 3, scheme=scheme, dt=2e-3)
+    ref.run_steps(10, 2e-3)
+    err = rel_l2(got, _cat(dict(zip(names, ref.soln)), names))
+    print(f'[simplex {scheme}] fused vs unfused rel L2 {err:.3e}',
+          flush=True)
+    assert err < (1e-14 if precision == 'double' else 1e-5)
+    if precision == 'double':
+        u = u0
+        for _ in range(10):
+            u = _rk4_dict(orc.rhs, u, 2e-3, STEPS[scheme])
+        err = rel_l2(got, _cat(u, names))
+        print(f'[simplex {scheme}] vs oracle rel L2 {err:.3e}', flush=True)
+        assert err < 1e-12

@@ -27,8 +27,9 @@ from oracle.basis import (HexShapeOracle, clean, gauss_legendre_hex,
 from oracle.geom import trilinear_ploc
 from oracle.mesh import box_hex_verts, pair_faces_bruteforce
 from oracle.physics import hllc, inviscid_flux, rusanov, viscous_flux_add
-from oracle.rhs import (OracleHexSystem, advance, clamp_dt, rk4_step,
-                        rk45_step)
+from oracle.rhs import (RK45_A, RK45_B, OracleHexSystem, advance, clamp_dt,
+                        euler_step, rk4_step, rk34_step, rk45_step,
+                        rkvdh2_step, tvdrk3_step)
 
 GOLD = os.path.join(os.path.dirname(__file__), 'golden')
 
@@ -202,7 +203,8 @@ def test_rk_orders():
 
     def f(y):
         return lam*y
-    for step, order in ((rk4_step, 4), (rk45_step, 4)):
+    for step, order in ((rk4_step, 4), (rk45_step, 4), (rk34_step, 3),
+                        (tvdrk3_step, 3), (euler_step, 1)):
         errs = []
         for dt in (0.1, 0.05):
             y = np.array([1.0 + 0j])
@@ -210,6 +212,34 @@ def test_rk_orders():
                 y = step(f, y, dt)
             errs.append(abs(y[0] - np.exp(lam)))
         assert np.log2(errs[0]/errs[1]) > order - 0.3
+
+
+def test_rk_one_step_closed_forms():
+    """On y' = lam y one step multiplies y by the scheme's stability
+    polynomial: 1 + z (Euler); the cubic Taylor polynomial (TVD-RK3, a
+    3-stage third-order scheme); for the low-storage schemes the Taylor
+    terms up to their order, z = lam dt.  A dropped or misplaced stage
+    term changes a coefficient."""
+    from math import factorial
+    lam = -0.7 + 0.4j
+    for step, order, exact in ((euler_step, 1, True), (tvdrk3_step, 3, True),
+                               (rk34_step, 3, False), (rk45_step, 4, False),
+                               (rk4_step, 4, True)):
+        # fit the polynomial from several z (degree <= stages)
+        zs = np.array([0.05, 0.1, 0.15, 0.2, 0.25, 0.3])*lam
+        vals = np.array([step(lambda y, l=z: l*y, np.array([1.0 + 0j]),
+                              1.0)[0] for z in zs])
+        V = np.vander(zs, 6, increasing=True)
+        coef = np.linalg.solve(V, vals)
+        for k in range(order + 1):
+            assert abs(coef[k] - 1/factorial(k)) < 1e-9, (step, k, coef)
+        if exact:
+            assert np.allclose(coef[order + 1:], 0, atol=1e-9)
+    # the shared low-storage step with RK45's table is rk45_step
+    y = np.array([1.0, -2.0])
+    f = (lambda v: np.array([-v[1], v[0]]))
+    assert np.array_equal(rkvdh2_step(f, y, 0.1, RK45_A, RK45_B),
+                          rk45_step(f, y, 0.1))
 
 
 def test_controller_cfg0_steps():
