@@ -40,6 +40,7 @@ import re
 import numpy as np
 
 from pyfr_amd.mesh import FACE_VERTS, fuzzysort_batched
+from pyfr_amd.native import RawMesh, element_colouring, ele_dtype
 
 # Gmsh element type -> (PyFR type, node count): the types the reference
 # reads (pyfr/readers/gmsh.py:34-42) for the shapes this solver has
@@ -363,38 +364,47 @@ class GmshMesh:
 
     # -- assembly -------------------------------------------------------------
     def _assemble(self):
+        """The raw mesh of the reference's importer (codec, element records
+        with their per-face links, periodic pairs; pyfr/readers/gmsh.py:
+        436-487, pyfr/readers/base.py:107-482), then the load path of its
+        native reader (``native.RawMesh.load``)."""
         if not self._pnames:
             raise ValueError('Mesh has no physical names')
         voldim = max(d for d, _, _ in self._pnames)
         if voldim != 3:
             raise NotImplementedError('Only 3-D meshes are supported')
-        vol, bnd, per = set(), {}, defaultdict(dict)
+        # volume groups by name; boundaries and periodic pairs in the
+        # order of their physical names (the first-named side of a pair is
+        # its left one, as the reference takes them)
+        vol, bnd, per = {}, {}, defaultdict(list)
         for dim, pent, name in self._pnames:
             if name.startswith('periodic'):
                 m = re.match(r'periodic[ _-]([a-z0-9]+)[ _-](l|r)$', name)
                 if not m:
                     raise ValueError('Invalid periodic boundary condition')
-                per[m[1]][m[2]] = pent
+                per[m[1]].append(pent)
             elif dim == voldim:
-                vol.add(pent)
+                vol[name] = pent
             else:
                 bnd[pent] = name
         if not vol:
             raise ValueError('No volume elements in mesh')
-        if any(set(p) != {'l', 'r'} for p in per.values()):
+        if any(len(p) != 2 for p in per.values()):
             raise ValueError('Unpaired periodic boundary in mesh')
+        pbits = {p: 1 << i for i, (_, p) in enumerate(sorted(vol.items()))}
 
         # volume elements per PyFR type, (gmsh type, pents) groups sorted;
         # the types in the order of their Gmsh type numbers (the codec's)
-        vnodes, hnodes, sorders = defaultdict(list), defaultdict(list), {}
-        surf = defaultdict(list)
-        etypes = []
+        vnodes, hnodes, tags = defaultdict(list), defaultdict(list), \
+            defaultdict(list)
+        sorders, surf, etypes = {}, defaultdict(list), []
         for (gtype, pents), rows in sorted(self._elenodes.items()):
             petype, nn = GMSH_TYPES[gtype]
             arr = np.array(rows, dtype=np.int64) - self._nodeoff
             # first-order representation: the corners, PyFR vertex order
             corners = arr[:, :NVERTS[petype]][:, NODEMAP[petype]]
-            if any(p in vol for p in pents):
+            mask = sum(pbits[p] for p in pents if p in pbits)
+            if mask:
                 if petype not in ETYPE_ORDER:
                     raise ValueError(f'{petype} elements in a 3-D volume')
                 k = npts_order(petype, nn)
@@ -405,37 +415,46 @@ class GmshMesh:
                     etypes.append(petype)
                 vnodes[petype].append(corners)
                 hnodes[petype].append(arr[:, gmsh_nodemap(petype, k)])
+                tags[petype].append(np.full(len(arr), mask, dtype=np.uint64))
             else:
                 for p in pents:
                     surf[p].append(corners)
         self.etypes = etypes
-        self._vnodes = {t: np.vstack(vnodes[t]) for t in etypes}
+        vn = {t: np.vstack(vnodes[t]) for t in etypes}
         hn = {t: np.vstack(hnodes[t]) for t in etypes}
         for nd in hn.values():
             if np.isnan(self._nodes[nd]).any():
                 raise ValueError('Element references an undefined node')
-        self.neles = sum(v.shape[0] for v in self._vnodes.values())
         self.sorders = sorders
         nodes, curved = self._linearise(hn)
 
+        # codec (pyfr/readers/base.py:363-398, gmsh.py:481-483)
+        codec = []
+        for t in etypes:
+            codec.append(f'eles/{t}')
+            codec += [f'eles/{t}/face/{i}' for i in range(len(FACES[t]))]
+        codec += [f'bc/{n}' for n in bnd.values()]
+        codec += [f'tag/{n}' for n in sorted(vol)]
+        cix = {c: i for i, c in enumerate(codec)}
+        links = {t: np.full((len(vn[t]), len(FACES[t]), 2), -2,
+                            dtype=np.int64) for t in etypes}
+
+        def link(a, b):
+            (ta, ea, fa), (tb, eb, fb) = a, b
+            links[ta][ea, fa] = cix[f'eles/{tb}/face/{fb}'], eb
+            links[tb][eb, fb] = cix[f'eles/{ta}/face/{fa}'], ea
+
         # every face of every volume element keyed by its sorted nodes
         owner = {}
-        pairs = []
-        cbase = {}
-        c = 0
-        for t in self.etypes:
-            cbase[t] = c
-            c += len(FACES[t])
-        for t in self.etypes:
-            nd = self._vnodes[t]
+        for t in etypes:
             for f, fv in enumerate(FACES[t]):
-                keys = np.sort(nd[:, list(fv)], axis=1)
+                keys = np.sort(vn[t][:, list(fv)], axis=1)
                 for e, k in enumerate(map(tuple, keys)):
                     other = owner.pop(k, None)
                     if other is None:
                         owner[k] = (t, e, f)
                     else:
-                        pairs.append((other, (t, e, f)))
+                        link(other, (t, e, f))
 
         # periodic faces: fuzzy-sorted centroids of each side
         # (pyfr/readers/base.py:263-292)
@@ -445,10 +464,13 @@ class GmshMesh:
                 out[r.shape[1]].append(r)
             return {w: np.vstack(v) for w, v in out.items()}
 
-        for k, lr in sorted(per.items()):
-            lft, rgt = by_ftype(surf.pop(lr['l'])), by_ftype(surf.pop(lr['r']))
+        pdt = [('cidx', np.int16), ('off', np.int64)]
+        pmap = {}
+        for k, (lp, rp) in per.items():
+            lft, rgt = by_ftype(surf.pop(lp, [])), by_ftype(surf.pop(rp, []))
             if set(lft) != set(rgt):
                 raise ValueError(f'Periodic boundary {k} is unbalanced')
+            plist = []
             for w in sorted(lft):
                 sides = []
                 for fn in (lft[w], rgt[w]):
@@ -458,51 +480,41 @@ class GmshMesh:
                                   for i in idx])
                 if len(sides[0]) != len(sides[1]):
                     raise ValueError(f'Periodic boundary {k} is unbalanced')
-                pairs += list(zip(*sides))
+                for a, b in zip(*sides):
+                    link(a, b)
+                    plist.append([(cix[f'eles/{a[0]}/face/{a[2]}'], a[1]),
+                                  (cix[f'eles/{b[0]}/face/{b[2]}'], b[1])])
+            pmap[k] = np.array(plist, dtype=pdt).reshape(-1, 2)
 
         # named boundaries
-        self.bcon = {}
-        for pent, rows in sorted(surf.items()):
-            if pent not in bnd:
-                continue
-            faces = []
-            for fns in rows:
+        for pent, name in bnd.items():
+            for fns in surf.get(pent, []):
                 for fn in fns:
-                    faces.append(owner.pop(tuple(sorted(fn))))
-            self.bcon[bnd[pent]] = faces
+                    t, e, f = owner.pop(tuple(sorted(fn)))
+                    links[t][e, f] = cix[f'bc/{name}'], -1
         if owner:
             raise ValueError('Unpaired faces in mesh')
 
-        def key(q):
-            t, e, f = q
-            return (cbase[t] + f, e)
-        lhs, rhs = [], []
-        for a, b in pairs:
-            if key(b) < key(a):
-                a, b = b, a
-            lhs.append(a)
-            rhs.append(b)
-        srt = sorted(range(len(lhs)), key=lambda i: key(lhs[i]))
-        lhs, rhs = [lhs[i] for i in srt], [rhs[i] for i in srt]
-
-        # curved elements first within each type (stable), the element
-        # numbers of the interfaces and boundaries renumbered to match
-        self.spts, newidx = {}, {}
+        # element records (pyfr/readers/base.py:384-396) and colours
+        eles = {}
         for t in etypes:
-            perm = np.argsort(~curved[t], kind='stable')
-            newidx[t] = np.argsort(perm)
-            sp = nodes[hn[t][perm]].swapaxes(0, 1)       # (nspts, neles, 3)
-            if not curved[t].any():
-                sp = sp[_vertex_index(t, sorders[t])]
-            elif t == 'hex':
-                self.spts_curved = {'hex': curved[t][perm]}
-            self.spts[t] = np.ascontiguousarray(sp)
+            rec = np.zeros(len(hn[t]), dtype=ele_dtype(hn[t].shape[1],
+                                                       len(FACES[t])))
+            rec['nodes'] = hn[t]
+            rec['curved'] = curved[t]
+            rec['faces']['cidx'] = links[t][..., 0]
+            rec['faces']['off'] = links[t][..., 1]
+            rec['tags'] = np.concatenate(tags[t])
+            eles[t] = rec
+        element_colouring(eles, codec)
+        self.raw = RawMesh(nodes, eles, codec, pmap)
+        self.raw.load(self)
+        self.codec = codec
 
-        def renum(q):
-            t, e, f = q
-            return (t, int(newidx[t][e]), f)
-        self.con = ([renum(q) for q in lhs], [renum(q) for q in rhs])
-        self.bcon = {n: [renum(q) for q in fs] for n, fs in self.bcon.items()}
+    def write_pyfrm(self, fname):
+        """Write the mesh as a .pyfrm file (format version 2), as
+        ``pyfr import`` would (pyfr/readers/base.py:67-105)."""
+        self.raw.write(fname)
 
     def _linearise(self, hn):
         """Snap the elements within ``lintol`` of the first-order map of

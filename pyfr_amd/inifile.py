@@ -1,7 +1,8 @@
 """Minimal configuration object with the reading interface of the reference's
 ``Inifile`` (``pyfr/inifile.py:31-161``): ``get``, ``getint``, ``getfloat``,
-``getbool``, ``hasopt``, ``set``, ``items`` and ``items_as``."""
+``getbool``, ``hasopt``, ``set``, ``items``, ``items_as`` and ``tostr``."""
 import configparser
+import io
 import re
 
 
@@ -27,6 +28,12 @@ class Inifile:
 
     def hasopt(self, sect, opt):
         return self._cp.has_option(sect, opt)
+
+    def tostr(self):
+        """The configuration as ini text (what solution files record)."""
+        buf = io.StringIO()
+        self._cp.write(buf)
+        return buf.getvalue()
 
     _NONE = object()
 

@@ -72,7 +72,7 @@ def test_tet_prism_round_trip(version):
     L = 2*np.pi
     sides = _simplex_sides(ref, (L, L, (nt + npr)*L/n))
     m = GmshMesh(io.StringIO(mesh_to_msh(ref, sides, version)))
-    assert m.etypes == ['tet', 'pri']
+    assert m.etypes == ['pri', 'tet']          # name order, as loaded
     for t in m.etypes:
         assert np.allclose(m.spts[t], ref.spts[t], rtol=0, atol=1e-12)
     assert _pairs(m) == _pairs(ref)
@@ -258,7 +258,7 @@ def test_high_order_simplex_round_trip(version, k):
     ref = SimplexBoxMesh.stacked(n, nt, npr, sorder=k, warp=_swarp)
     m = GmshMesh(io.StringIO(mesh_to_msh(ref, sides, version,
                                          hofaces=k == 3)))
-    assert m.etypes == ['tet', 'pri'] and m.sorders == {'tet': k, 'pri': k}
+    assert m.etypes == ['pri', 'tet'] and m.sorders == {'tet': k, 'pri': k}
     for t in m.etypes:
         assert np.allclose(m.spts[t], ref.spts[t], rtol=0, atol=1e-12)
     assert _pairs(m) == _pairs(ref)
