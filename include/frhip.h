@@ -271,10 +271,14 @@ int frhip_negdivconf(int dtype, int npts, int neles, void *t, long ldt,
  * mapping) with variable stride 64; vector views add a row stride.
  * Euler intcflux: pyfr/solvers/euler/kernels/intcflux.mako:6-22 with
  * rsolvers/rusanov.mako:4-25 or rsolvers/hllc.mako:4-77. nl: (3, n), ld ldn. */
+/* pmin (here and in frhip_intcflux_ns, frhip_bccflux_*): the exact solver's
+ * floor for negative Newton iterates of the star pressure, [solver-interfaces]
+ * p-min (pyfr/solvers/euler/inters.py:12-26); negative for the default,
+ * 5 x machine epsilon of the working precision.  Other solvers ignore it. */
 int frhip_intcflux_euler(int dtype, int rsolver, int n, void *base,
                          const int32_t *lmap, const int32_t *rmap,
                          const void *nl, long ldn, double gamma,
-                         const double *bstate, void *stream);
+                         double pmin, const double *bstate, void *stream);
 
 /* NS intconu: pyfr/solvers/navstokes/kernels/intconu.mako:4-19. */
 int frhip_intconu(int dtype, int n, const void *ubase, const int32_t *ulin,
@@ -296,7 +300,8 @@ int frhip_intcflux_ns(int dtype, int rsolver, int n, void *ubase,
                       const int32_t *grmap, long gstride,
                       const int32_t *glstr, const int32_t *grstr,
                       const void *nl, long ldn, const double *consts,
-                      const void *mu_p, const double *bstate, void *stream);
+                      double pmin, const void *mu_p, const double *bstate,
+                      void *stream);
 
 /* ---- boundary interfaces, 1-D over n boundary flux points ----
  * Types: the `type` keys of [soln-bcs-*] (pyfr/solvers/euler/inters.py:57-102,
@@ -322,7 +327,7 @@ enum { FRHIP_BC_SUP_IN_FA = 0, FRHIP_BC_SUP_OUT_FN = 1,
 int frhip_bccflux_euler(int dtype, int bctype, int rsolver, int n,
                         void *base, const int32_t *lmap, const void *nl,
                         long ldn, const void *bcv, long ldb,
-                        const double *consts, void *stream);
+                        const double *consts, double pmin, void *stream);
 
 /* NS bcconu: pyfr/solvers/navstokes/kernels/bcconu.mako:6-15 (the type's
  * bc_ldg_state, navstokes/kernels/bcs/<type>.mako), ulout = LDG state. */
@@ -341,7 +346,7 @@ int frhip_bccflux_ns(int dtype, int bctype, int rsolver, int n, void *ubase,
                      const int32_t *glmap, long gstride,
                      const int32_t *glstr, const void *nl, long ldn,
                      const void *bcv, long ldb, const double *consts,
-                     void *stream);
+                     double pmin, void *stream);
 
 /* ---- boundary values that depend on time ----
  * The reference splices each [soln-bcs-*] value expression into the kernel

@@ -75,17 +75,17 @@ _FUNCS = {
     'frhip_tflux_ns_grad': [_I, _I, _I, _I, _P, _L, _P, _L, _P, _L, _P, _P,
                             _P, _P],
     'frhip_negdivconf': [_I, _I, _I, _P, _L, _P, _L, _P],
-    'frhip_intcflux_euler': [_I, _I, _I, _P, _PI32, _PI32, _P, _L, _D, _P,
-                             _P],
+    'frhip_intcflux_euler': [_I, _I, _I, _P, _PI32, _PI32, _P, _L, _D, _D,
+                             _P, _P],
     'frhip_intconu': [_I, _I, _P, _PI32, _PI32, _P, _PI32, _PI32, _D, _P],
     'frhip_intcflux_ns': [_I, _I, _I, _P, _PI32, _PI32, _P, _PI32, _PI32,
-                          _L, _PI32, _PI32, _P, _L, _P, _P, _P, _P],
+                          _L, _PI32, _PI32, _P, _L, _P, _D, _P, _P, _P],
     'frhip_bccflux_euler': [_I, _I, _I, _I, _P, _PI32, _P, _L, _P, _L, _P,
-                            _P],
+                            _D, _P],
     'frhip_bcconu': [_I, _I, _I, _P, _PI32, _P, _PI32, _P, _L, _P, _L, _P,
                      _P],
     'frhip_bccflux_ns': [_I, _I, _I, _I, _P, _PI32, _P, _PI32, _L, _PI32,
-                         _P, _L, _P, _L, _P, _P],
+                         _P, _L, _P, _L, _P, _D, _P],
     'frhip_bc_program_create': [ct.POINTER(_P), _I, _P, _P, _P, _I],
     'frhip_bc_program_destroy': [_P],
     'frhip_bc_program_set_time': [_P, _D, _P],

@@ -1120,13 +1120,14 @@ class PointwiseProvider:
                'exact': 5}
         if rs not in ids:
             raise NotSuitableError(f'Riemann solver {rs} not available')
-        # the exact solver's pressure floor is compiled in as the
-        # reference's default, 5 x machine epsilon
-        pm = tplargs.get('p_min')
-        if rs == 'exact' and pm is not None and \
-                pm != 5*float(np.finfo(dtype).eps):
-            raise NotSuitableError('only the default p-min is supported')
         return ids[rs]
+
+    @staticmethod
+    def _pmin(tplargs):
+        """The exact solver's pressure floor (``p_min`` template argument,
+        pyfr/solvers/euler/inters.py:12-26); -1 for the default."""
+        pm = tplargs.get('p_min')
+        return -1.0 if pm is None else float(pm)
 
     def intcflux(self, tplargs, dims, ul, ur, nl, gradul=None, gradur=None,
                  artvisc=None, **kw):
@@ -1143,7 +1144,8 @@ class PointwiseProvider:
                                    'solver')
         if gradul is None:
             args = (dt, rs, n, ul.basedata, ul.mapping.data, ur.mapping.data,
-                    nl.data, nl.leaddim, c['gamma'], base)
+                    nl.data, nl.leaddim, c['gamma'], self._pmin(tplargs),
+                    base)
             return Kernel(lambda stream: lib.intcflux_euler(*args, stream),
                           views=(ul, ur), mats=(nl,), name='intcflux')
         if artvisc is not None:
@@ -1155,7 +1157,8 @@ class PointwiseProvider:
             c.get('cpTref', 1.0), c.get('cpTs', 0.0))
         args = (dt, rs, n, ul.basedata, ul.mapping.data, ur.mapping.data,
                 gradul.basedata, gradul.mapping.data, gradur.mapping.data,
-                *gs, nl.data, nl.leaddim, consts, None, base)
+                *gs, nl.data, nl.leaddim, consts, self._pmin(tplargs), None,
+                base)
         return Kernel(lambda stream: lib.intcflux_ns(*args, stream),
                       views=(ul, ur, gradul, gradur), mats=(nl,),
                       name='intcflux_ns')
@@ -1205,7 +1208,7 @@ class PointwiseProvider:
         ldb = 0 if bcvals is None else bcvals.leaddim
         if euler:
             args = (dt, bct, rs, n, ul.basedata, ul.mapping.data, nl.data,
-                    nl.leaddim, bv, ldb, consts)
+                    nl.leaddim, bv, ldb, consts, self._pmin(tplargs))
             return Kernel(lambda stream: lib.bccflux_euler(*args, stream),
                           views=(ul,), mats=(nl, bcvals), name='bccflux')
         if artvisc is not None:
@@ -1214,7 +1217,7 @@ class PointwiseProvider:
             (gradul.basedata, gradul.mapping.data,
              *View.stride_args(gradul))
         args = (dt, bct, rs, n, ul.basedata, ul.mapping.data, *g, nl.data,
-                nl.leaddim, bv, ldb, consts)
+                nl.leaddim, bv, ldb, consts, self._pmin(tplargs))
         return Kernel(lambda stream: lib.bccflux_ns(*args, stream),
                       views=(ul, gradul), mats=(nl, bcvals),
                       name='bccflux_ns')

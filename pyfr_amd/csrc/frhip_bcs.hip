@@ -32,10 +32,10 @@ struct BCConsts {
 };
 
 template <typename T>
-static BCConsts<T> bc_consts(const double *c)
+static BCConsts<T> bc_consts(const double *c, double pmin)
 {
     BCConsts<T> k;
-    k.ns = ns_consts<T>(c);
+    k.ns = ns_consts<T>(c, pmin);
     const double g = c[0], gmo = g - 1.0;
     const double cpTw = c[8], cpTt = c[9], pt = c[10];
     const double Rdcp = gmo/g;
@@ -220,7 +220,8 @@ __global__ __launch_bounds__(256) void k_bccflux_euler(
     const T mag = unit_normal(nl, ldn, i, nn);
     load_bv<T, BC>(bcv, ldb, i, bv);
     bc_state<T, BC, false>(ul, nn, bv, k, ur);
-    rsolve<T, RS>(ul, ur, nn, fn, k.ns.gm1, k.ns.gamma, k.ns.qgamma);
+    rsolve<T, RS>(ul, ur, nn, fn, k.ns.gm1, k.ns.gamma, k.ns.qgamma,
+                  k.ns.pmin);
     for (int v = 0; v < 5; ++v) base[li + v*SOA] = mag*fn[v];
 }
 
@@ -264,7 +265,8 @@ __global__ __launch_bounds__(256) void k_bccflux_ns(
 
     if constexpr (BC == FRHIP_BC_SLP_ADIA_WALL) {
         bc_state<T, BC, false>(ul, nn, bv, k, ur);
-        rsolve<T, RS>(ul, ur, nn, fi, k.ns.gm1, k.ns.gamma, k.ns.qgamma);
+        rsolve<T, RS>(ul, ur, nn, fi, k.ns.gm1, k.ns.gamma, k.ns.qgamma,
+                  k.ns.pmin);
         for (int v = 0; v < 5; ++v) base[li + v*SOA] = mag*fi[v];
     }
     else {
@@ -279,7 +281,8 @@ __global__ __launch_bounds__(256) void k_bccflux_ns(
         bc_grad_state<T, BC>(imperm ? url : ul, nn, gl, gr);
         viscous_flux_add(url, gr, fvr, k.ns);
         bc_state<T, BC, false>(ul, nn, bv, k, ur);
-        rsolve<T, RS>(ul, ur, nn, fi, k.ns.gm1, k.ns.gamma, k.ns.qgamma);
+        rsolve<T, RS>(ul, ur, nn, fi, k.ns.gm1, k.ns.gamma, k.ns.qgamma,
+                  k.ns.pmin);
         for (int v = 0; v < 5; ++v) {
             T fvcomm = nn[0]*fvr[0][v] + nn[1]*fvr[1][v] + nn[2]*fvr[2][v];
             if (!imperm && k.ns.tau != T(0))
@@ -343,7 +346,7 @@ extern "C" {
 int frhip_bccflux_euler(int dtype, int bctype, int rsolver, int n,
                         void *base, const int32_t *lmap, const void *nl,
                         long ldn, const void *bcv, long ldb,
-                        const double *consts, void *stream)
+                        const double *consts, double pmin, void *stream)
 {
     FR_API_BEGIN
     FR_DTYPE_OK(dtype);
@@ -356,7 +359,7 @@ int frhip_bccflux_euler(int dtype, int bctype, int rsolver, int n,
     const dim3 grid(cdiv(n, 256)), block(256);
     auto go = [&](auto tag) {
         using T = decltype(tag);
-        const BCConsts<T> k = bc_consts<T>(consts);
+        const BCConsts<T> k = bc_consts<T>(consts, pmin);
         dispatch<0, FRHIP_RS_EXACT + 1>(rsolver, [&](auto rs) {
             dispatch<0, FRHIP_BC_SLP_ADIA_WALL + 1>(bctype, [&](auto bc) {
                 hipLaunchKernelGGL(
@@ -389,7 +392,7 @@ int frhip_bcconu(int dtype, int bctype, int n, const void *ub,
     const dim3 grid(cdiv(n, 256)), block(256);
     auto go = [&](auto tag) {
         using T = decltype(tag);
-        const BCConsts<T> k = bc_consts<T>(consts);
+        const BCConsts<T> k = bc_consts<T>(consts, -1.0);
         dispatch<0, FRHIP_BC_SUB_OUT_FP + 1>(bctype, [&](auto bc) {
             hipLaunchKernelGGL((k_bcconu<T, decltype(bc)::value>), grid,
                                block, 0, as_stream(stream), n, (const T *)ub,
@@ -409,7 +412,7 @@ int frhip_bccflux_ns(int dtype, int bctype, int rsolver, int n, void *base,
                      const int32_t *glmap, long gstride,
                      const int32_t *glstr, const void *nl, long ldn,
                      const void *bcv, long ldb, const double *consts,
-                     void *stream)
+                     double pmin, void *stream)
 {
     FR_API_BEGIN
     FR_DTYPE_OK(dtype);
@@ -424,7 +427,7 @@ int frhip_bccflux_ns(int dtype, int bctype, int rsolver, int n, void *base,
     const dim3 grid(cdiv(n, 256)), block(256);
     auto go = [&](auto tag) {
         using T = decltype(tag);
-        const BCConsts<T> k = bc_consts<T>(consts);
+        const BCConsts<T> k = bc_consts<T>(consts, pmin);
         dispatch<0, FRHIP_RS_EXACT + 1>(rsolver, [&](auto rs) {
             dispatch<0, FRHIP_BC_SUB_OUT_FP + 1>(bctype, [&](auto bc) {
                 hipLaunchKernelGGL(

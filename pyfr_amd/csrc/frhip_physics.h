@@ -387,9 +387,10 @@ __device__ __forceinline__ void rsolve_roem(const T ul[5], const T ur[5],
 
 // Exact Riemann solver (Toro 2009 ch. 4, zero covolume) in the face frame;
 // rsolvers/exact.mako:14-206 through rsolve1d.mako:1-21.  The gamma ratios
-// are formed in double, as the reference forms its template literals; p_min
-// is the reference's default 5 x machine epsilon of T
-// (pyfr/solvers/euler/inters.py:12-26).
+// are formed in double, as the reference forms its template literals; pfloor
+// replaces negative Newton iterates of the star pressure: [solver-interfaces]
+// p-min, by default 5 x machine epsilon of T (pyfr/solvers/euler/
+// inters.py:12-26).
 template <typename T>
 __device__ __forceinline__ void exact_star_flux(T p, T ps, T rs, T cs, T &f,
                                                 T &fd, T gprtg, T trgm,
@@ -412,14 +413,14 @@ __device__ __forceinline__ void exact_star_flux(T p, T ps, T rs, T cs, T &f,
 template <typename T>
 __device__ __forceinline__ void rsolve_exact(const T ul[5], const T ur[5],
                                              const T n[3], T nf[5], T gm1,
-                                             T gamma)
+                                             T gamma, T pfloor)
 {
     const double g = (double)gamma, gm = g - 1.0;
     const T hgm = T(0.5*gm), grgm = T(g/gm), trgm = T(2/gm);
     const T trgp = T(2/(g + 1)), gmrtg = T(gm/(2*g));
     const T gprtg = T((g + 1)/(2*g)), tgrgm = T((2*g)/gm);
     const T gmrgp = T(gm/(g + 1)), rgamma = T(1/g);
-    const T p_min = T(5)*std::numeric_limits<T>::epsilon();
+    const T p_min = pfloor;
 
     T tl[5], tr[5];
     tl[0] = ul[0]; tr[0] = ur[0]; tl[4] = ul[4]; tr[4] = ur[4];
@@ -531,14 +532,14 @@ __device__ __forceinline__ void rsolve_exact(const T ul[5], const T ur[5],
 template <typename T, int RS>
 __device__ __forceinline__ void rsolve(const T ul[5], const T ur[5],
                                        const T n[3], T nf[5], T gm1, T gamma,
-                                       T qgamma)
+                                       T qgamma, T pmin)
 {
     if constexpr (RS == FRHIP_RS_HLLC) rsolve_hllc(ul, ur, n, nf, gm1, gamma);
     else if constexpr (RS == FRHIP_RS_HLL) rsolve_hll(ul, ur, n, nf, gm1, gamma);
     else if constexpr (RS == FRHIP_RS_ROE) rsolve_roe(ul, ur, n, nf, gm1);
     else if constexpr (RS == FRHIP_RS_ROEM) rsolve_roem(ul, ur, n, nf, gm1);
     else if constexpr (RS == FRHIP_RS_EXACT)
-        rsolve_exact(ul, ur, n, nf, gm1, gamma);
+        rsolve_exact(ul, ur, n, nf, gm1, gamma, pmin);
     else rsolve_rusanov(ul, ur, n, nf, gm1, qgamma);
 }
 
@@ -546,6 +547,7 @@ __device__ __forceinline__ void rsolve(const T ul[5], const T ur[5],
 template <typename T>
 struct NSConsts {
     T gamma, gm1, qgamma, mu, gpr, beta, tau, rcpTref, musuth, cpTs, suthfac;
+    T pmin;     // the exact solver's pressure floor
     int suth;
 
     // Copy with the viscosity replaced (per-element / per-pair viscosity)
@@ -605,10 +607,19 @@ __device__ __forceinline__ void viscous_flux_add(const T u[5], const T g[3][5],
     f[2][4] += uu*t_xz + vv*t_yz + ww*t_zz + -mu_c*c.gpr*T_z;
 }
 
+// The exact solver's pressure floor: pmin, or where negative the
+// default 5 x machine epsilon of T.
 template <typename T>
-inline NSConsts<T> ns_consts(const double *c)
+inline T exact_pmin(double pmin)
+{
+    return pmin >= 0 ? (T)pmin : T(5)*std::numeric_limits<T>::epsilon();
+}
+
+template <typename T>
+inline NSConsts<T> ns_consts(const double *c, double pmin)
 {
     NSConsts<T> k;
+    k.pmin = exact_pmin<T>(pmin);
     const double gamma = c[0], mu = c[1], Pr = c[2];
     k.gamma = (T)gamma;
     k.gm1 = (T)(gamma - 1);

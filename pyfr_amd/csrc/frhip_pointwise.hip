@@ -271,7 +271,7 @@ template <typename T, int RS, bool BASE>
 __global__ __launch_bounds__(256) void k_intcflux_euler(
     int n, T *__restrict__ base, const int32_t *__restrict__ lmap,
     const int32_t *__restrict__ rmap, const T *__restrict__ nl, long ldn,
-    T gamma, T gm1, T qgamma, BaseState<T> bs)
+    T gamma, T gm1, T qgamma, T pmin, BaseState<T> bs)
 {
     const int i = blockIdx.x*blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -290,7 +290,7 @@ __global__ __launch_bounds__(256) void k_intcflux_euler(
         rsolve_rusanov_base(ul, ur, nn, fn, gm1, qgamma, bs);
     }
     else
-        rsolve<T, RS>(ul, ur, nn, fn, gm1, gamma, qgamma);
+        rsolve<T, RS>(ul, ur, nn, fn, gm1, gamma, qgamma, pmin);
     for (int v = 0; v < 5; ++v) {
         base[li + v*SOA] = mag*fn[v];
         base[ri + v*SOA] = -mag*fn[v];
@@ -357,7 +357,7 @@ __global__ __launch_bounds__(256) void k_intcflux_ns(
         base_full(ur, urf, bs);
     }
     else {
-        rsolve<T, RS>(ul, ur, nn, fi, c.gm1, c.gamma, c.qgamma);
+        rsolve<T, RS>(ul, ur, nn, fi, c.gm1, c.gamma, c.qgamma, c.pmin);
         for (int v = 0; v < 5; ++v) {
             ulf[v] = ul[v];
             urf[v] = ur[v];
@@ -648,13 +648,13 @@ int frhip_tflux_ns_grad(int dtype, int linear, int npts, int neles,
                        (const T *)geo, ldgeo, (const T *)upts, c,           \
                        (const T *)mu_e)
     if (dtype == FRHIP_F64) {
-        auto c = ns_consts<double>(consts);
+        auto c = ns_consts<double>(consts, -1.0);
         if (linear == 0) FR_LAUNCH_NG(double, 0);
         else if (linear == 1) FR_LAUNCH_NG(double, 1);
         else FR_LAUNCH_NG(double, 2);
     }
     else {
-        auto c = ns_consts<float>(consts);
+        auto c = ns_consts<float>(consts, -1.0);
         if (linear == 0) FR_LAUNCH_NG(float, 0);
         else if (linear == 1) FR_LAUNCH_NG(float, 1);
         else FR_LAUNCH_NG(float, 2);
@@ -695,13 +695,13 @@ int frhip_tflux_ns(int dtype, int linear, int npts, int neles,
                            (const T *)upts, (const T *)rcpdjac, ldr, c,     \
                            (const T *)mu_e, BaseState<T>{})
     if (dtype == FRHIP_F64) {
-        auto c = ns_consts<double>(consts);
+        auto c = ns_consts<double>(consts, -1.0);
         if (linear == 0) FR_LAUNCH_NS(double, 0);
         else if (linear == 1) FR_LAUNCH_NS(double, 1);
         else FR_LAUNCH_NS(double, 2);
     }
     else {
-        auto c = ns_consts<float>(consts);
+        auto c = ns_consts<float>(consts, -1.0);
         if (linear == 0) FR_LAUNCH_NS(float, 0);
         else if (linear == 1) FR_LAUNCH_NS(float, 1);
         else FR_LAUNCH_NS(float, 2);
@@ -735,7 +735,7 @@ int frhip_negdivconf(int dtype, int npts, int neles, void *t, long ldt,
 int frhip_intcflux_euler(int dtype, int rsolver, int n, void *base,
                          const int32_t *lmap, const int32_t *rmap,
                          const void *nl, long ldn, double gamma,
-                         const double *bstate, void *stream)
+                         double pmin, const double *bstate, void *stream)
 {
     FR_API_BEGIN
     FR_DTYPE_OK(dtype);
@@ -744,29 +744,31 @@ int frhip_intcflux_euler(int dtype, int rsolver, int n, void *base,
              "the base-state form takes the Rusanov solver");
     if (n == 0) return 0;
     dim3 grid(cdiv(n, 256)), block(256);
+    const double pm64 = exact_pmin<double>(pmin);
+    const float pm32 = exact_pmin<float>(pmin);
 #define FR_L64(RS)                                                          \
     hipLaunchKernelGGL((k_intcflux_euler<double, RS, false>), grid, block,  \
                        0, as_stream(stream), n, (double *)base, lmap, rmap, \
                        (const double *)nl, ldn, gamma, gamma - 1,           \
-                       0.25*gamma, BaseState<double>{})
+                       0.25*gamma, pm64, BaseState<double>{})
 #define FR_L32(RS)                                                          \
     hipLaunchKernelGGL((k_intcflux_euler<float, RS, false>), grid, block,   \
                        0, as_stream(stream), n, (float *)base, lmap, rmap,  \
                        (const float *)nl, ldn, (float)gamma,                \
-                       (float)(gamma - 1), (float)(0.25*gamma),             \
+                       (float)(gamma - 1), (float)(0.25*gamma), pm32,       \
                        BaseState<float>{})
     if (bstate && dtype == FRHIP_F64)
         hipLaunchKernelGGL(
             (k_intcflux_euler<double, FRHIP_RS_RUSANOV, true>), grid, block,
             0, as_stream(stream), n, (double *)base, lmap, rmap,
-            (const double *)nl, ldn, gamma, gamma - 1, 0.25*gamma,
+            (const double *)nl, ldn, gamma, gamma - 1, 0.25*gamma, pm64,
             base_state<double>(bstate, gamma));
     else if (bstate)
         hipLaunchKernelGGL(
             (k_intcflux_euler<float, FRHIP_RS_RUSANOV, true>), grid, block,
             0, as_stream(stream), n, (float *)base, lmap, rmap,
             (const float *)nl, ldn, (float)gamma, (float)(gamma - 1),
-            (float)(0.25*gamma), base_state<float>(bstate, gamma));
+            (float)(0.25*gamma), pm32, base_state<float>(bstate, gamma));
     else if (dtype == FRHIP_F64) {
         FR_RS_DISPATCH(rsolver, FR_L64)
     }
@@ -809,7 +811,8 @@ int frhip_intcflux_ns(int dtype, int rsolver, int n, void *ubase,
                       const int32_t *grmap, long gstride,
                       const int32_t *glstr, const int32_t *grstr,
                       const void *nl, long ldn, const double *consts,
-                      const void *mu_p, const double *bstate, void *stream)
+                      double pmin, const void *mu_p, const double *bstate,
+                      void *stream)
 {
     FR_API_BEGIN
     FR_DTYPE_OK(dtype);
@@ -823,7 +826,7 @@ int frhip_intcflux_ns(int dtype, int rsolver, int n, void *ubase,
     hipLaunchKernelGGL((k_intcflux_ns<T, RS, B>), grid, block, 0,           \
                        as_stream(stream), n, (T *)ubase, lmap, rmap,        \
                        (const T *)gbase, glmap, grmap, gstride, glstr,      \
-                       grstr, (const T *)nl, ldn, ns_consts<T>(consts),     \
+                       grstr, (const T *)nl, ldn, ns_consts<T>(consts, pmin),     \
                        (const T *)mu_p, BS)
 #define FR_L64(RS) FR_LNS(double, RS, false, BaseState<double>{})
 #define FR_L32(RS) FR_LNS(float, RS, false, BaseState<float>{})

@@ -410,7 +410,7 @@ class Pool:
         if not self.ns:
             lib.intcflux_euler(d, 0, mp['n'], self.scal_fpts.data,
                                mp['l'], mp['r'], mp['nl'],
-                               mp['ldn'], GAMMA, None, s)
+                               mp['ldn'], GAMMA, -1.0, None, s)
             lib.tflux_euler(d, int(self.linear), Nu, E, u.data, u.leaddim,
                             self.vect_upts.data, self.vect_upts.leaddim,
                             geo.data, geo.leaddim, self.upts.data, GAMMA,
@@ -437,7 +437,7 @@ class Pool:
             lib.intcflux_ns(d, 0, mp['n'], self.scal_fpts.data,
                             mp['l'], mp['r'], self.vect_fpts.data,
                             mp['l'], mp['r'], Nf*u.leaddim, None, None,
-                            mp['nl'], mp['ldn'], consts,
+                            mp['nl'], mp['ldn'], consts, -1.0,
                             mp['mu'], None, s)
         mm('M1 - M3*M2', self.vect_upts, f)
         mm('M3', self.scal_fpts, f, 1.0)
@@ -463,7 +463,7 @@ class Pool:
                 sf.leaddim, 1.0, 0.0, s)
         if not self.ns:
             lib.intcflux_euler(d, 0, mp['n'], sf.data, mp['l'], mp['r'],
-                               mp['nl'], mp['ldn'], GAMMA, None, s)
+                               mp['nl'], mp['ldn'], GAMMA, -1.0, None, s)
             lib.tflux_euler(d, int(self.linear), Nu, E, u.data, u.leaddim,
                             vu.data, vu.leaddim, geo.data, geo.leaddim,
                             self.upts.data, GAMMA, None, s)
@@ -487,8 +487,8 @@ class Pool:
                         1.0, 0.0, s)
             lib.intcflux_ns(d, 0, mp['n'], sf.data, mp['l'], mp['r'],
                             vf.data, mp['l'], mp['r'], Nf*vf.leaddim, None,
-                            None, mp['nl'], mp['ldn'], consts, mp['mu'],
-                            None, s)
+                            None, mp['nl'], mp['ldn'], consts, -1.0,
+                            mp['mu'], None, s)
         lib.mul2(self._plans2['div'], ncol, vu.data, vu.leaddim, sf.data,
                  sf.leaddim, nxt.data, nxt.leaddim, 19 + stage,
                  self.rcpdjac.data, self.rcpdjac.leaddim,

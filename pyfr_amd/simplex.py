@@ -174,6 +174,29 @@ SF_TRI_RULES = {1: ('williams-shunn', 6), 2: ('witherden-vincent', 12),
                 3: ('witherden-vincent', 16), 4: ('witherden-vincent', 25)}
 
 
+def qdeg_rule(shape, qdeg=None, name=None, npts=None):
+    """(rule, npts) of a tabulated quadrature rule chosen as the
+    reference chooses (``pyfr/quadrules/__init__.py:52-79``): among the
+    rules of the shape of degree >= qdeg (and of the named family, and of
+    npts points, where given), the fewest points when qdeg is given, the
+    highest degree when only npts is; ties go to the first in name order.
+    Tets, prisms and triangles need the chosen rule's points tabulated."""
+    from pyfr_amd.quadtables import CATALOG, TABLES
+    best = None
+    for r, n, d, _ in CATALOG[shape]:
+        if ((not name or name == r) and (not npts or npts == n) and
+                (not qdeg or qdeg <= d)):
+            if (best is None or (npts and d > best[2]) or
+                    (qdeg and n < best[1])):
+                best = (r, n, d)
+    if best is None:
+        raise ValueError(f'No suitable {shape} quadrature rule found')
+    if shape in ('tet', 'pri', 'tri') and (shape, *best[:2]) not in TABLES:
+        raise NotImplementedError(f'{shape} quadrature rule {best[:2]} is '
+                                  'not tabulated')
+    return best[:2]
+
+
 class _SimplexShape:
     ndims = 3
     name = None

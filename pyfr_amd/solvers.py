@@ -147,12 +147,12 @@ class FluidElements:
         [solver-elements-<name>] quad-pts / quad-npts naming one of the
         tabulated rules."""
         from pyfr_amd.quadtables import TABLES
-        from pyfr_amd.simplex import AA_RULES
+        from pyfr_amd.simplex import AA_RULES, qdeg_rule
         sect = f'solver-elements-{name}'
         if cfg.hasopt(sect, 'quad-deg'):
-            raise NotImplementedError('quad-deg rule selection is not '
-                                      'supported; give quad-pts and '
-                                      'quad-npts')
+            return qdeg_rule(name, cfg.getint(sect, 'quad-deg'),
+                             cfg.get(sect, 'quad-pts', None),
+                             cfg.getint(sect, 'quad-npts', None))
         if cfg.hasopt(sect, 'quad-pts') or cfg.hasopt(sect, 'quad-npts'):
             key = (name, cfg.get(sect, 'quad-pts'),
                    cfg.getint(sect, 'quad-npts'))
@@ -173,13 +173,13 @@ class FluidElements:
         [solver-interfaces-tri] quad-pts / quad-npts naming a tabulated
         rule; quads Gauss-Legendre as for hexes."""
         from pyfr_amd.quadtables import TABLES
-        from pyfr_amd.simplex import SF_TRI_RULES
+        from pyfr_amd.simplex import SF_TRI_RULES, qdeg_rule
         sect = 'solver-interfaces-tri'
         if cfg.hasopt(sect, 'quad-deg'):
-            raise NotImplementedError('quad-deg rule selection is not '
-                                      'supported; give quad-pts and '
-                                      'quad-npts')
-        if cfg.hasopt(sect, 'quad-pts') or cfg.hasopt(sect, 'quad-npts'):
+            tri = qdeg_rule('tri', cfg.getint(sect, 'quad-deg'),
+                            cfg.get(sect, 'quad-pts', None),
+                            cfg.getint(sect, 'quad-npts', None))
+        elif cfg.hasopt(sect, 'quad-pts') or cfg.hasopt(sect, 'quad-npts'):
             key = ('tri', cfg.get(sect, 'quad-pts'),
                    cfg.getint(sect, 'quad-npts'))
             if key not in TABLES or TABLES[key][1][0] is None:
@@ -200,10 +200,23 @@ class FluidElements:
         (``pyfr/shapes.py:147-185``): the element rule of flux
         anti-aliasing ([solver-elements-hex], ndims 3) or the face rule of
         surface-flux anti-aliasing ([solver-interfaces-quad], ndims 2); by
-        default (order + 2)^ndims points, or quad-npts."""
+        default (order + 2)^ndims points, or quad-npts, or the rule
+        quad-deg selects (which must be a Gauss-Legendre one)."""
         if cfg.hasopt(sect, 'quad-deg'):
-            raise NotImplementedError('quad-deg rule selection is not '
-                                      'supported; give quad-npts')
+            from pyfr_amd.simplex import qdeg_rule
+            r, npts = qdeg_rule('hex' if ndims == 3 else 'quad',
+                                cfg.getint(sect, 'quad-deg'),
+                                cfg.get(sect, 'quad-pts', None),
+                                cfg.getint(sect, 'quad-npts', None))
+            if r != 'gauss-legendre':
+                raise NotImplementedError(
+                    f'quad-deg selects the {r} rule of {npts} points; '
+                    'only Gauss-Legendre rules are supported here (give '
+                    'quad-pts = gauss-legendre)')
+            n1 = round(npts ** (1/ndims))
+            if n1 < order + 1:
+                raise ValueError(f'Invalid quad-deg in [{sect}]')
+            return n1
         if cfg.get(sect, 'quad-pts', 'gauss-legendre') != 'gauss-legendre':
             raise NotImplementedError('only Gauss-Legendre quadrature '
                                       'points are supported')

@@ -58,7 +58,7 @@ def _dev(lib, arr):
     return p
 
 
-def _device_flux(be, name, ul, ur, nl):
+def _device_flux(be, name, ul, ur, nl, pmin=-1.0):
     """The intcflux kernel (FP64) on N state pairs (N a multiple of 64):
     the flux written to the left side, and that of the right side."""
     N = ul.shape[1]
@@ -79,7 +79,7 @@ def _device_flux(be, name, ul, ur, nl):
     base, lm, rm, nt = bufs
     try:
         lib.intcflux_euler(0, SOLVERS[name], N, base, lm, rm, nt, N, 1.4,
-                           None, None)
+                           pmin, None, None)
         lib.device_sync()
         out = np.empty_like(host)
         lib.memcpy_dtoh(out.ctypes.data, base, out.nbytes, None)
@@ -146,3 +146,48 @@ def test_intcflux_contacts(backend_factory, name, U):
         f, _, _ = inviscid_flux(ul if U > 0 else ur, 1.4)
         ref = (n[:, None, :]*f).sum(0)
     assert np.abs(got - scale*ref).max() < 1e-12*np.abs(scale*ref).max()
+
+
+def test_exact_pmin(backend_factory):
+    """The exact solver's p-min ([solver-interfaces] p-min): on state pairs
+    where a Newton iterate of the star pressure goes negative (a dense
+    state driving into a light, low-pressure one; no vacuum), the floor
+    decides the flux.  The kernel with p-min = 1e-3 and with the default
+    (5 x machine epsilon) each match the oracle with that floor, and the
+    two differ."""
+    from oracle.physics import exact
+    be = backend_factory()
+    rng = np.random.default_rng(2)
+    N = 200000
+
+    def st():
+        rho = np.exp(rng.uniform(np.log(0.01), np.log(10), N))
+        p = np.exp(rng.uniform(np.log(1e-4), np.log(100), N))
+        u = rng.normal(0, 3, N)
+        return np.stack([rho, rho*u, 0*rho, 0*rho, p/0.4 + 0.5*rho*u*u])
+    ul, ur = st(), st()
+    n = np.zeros((3, N))
+    n[0] = 1.0
+    with np.errstate(all='ignore'):
+        f0 = exact(ul, ur, n, 1.4)
+        f1 = exact(ul, ur, n, 1.4, p_min=1e-3)
+    u_l, c_l = _prim(ul, n)
+    u_r, c_r = _prim(ur, n)
+    ok = (np.isfinite(f0).all(0) & np.isfinite(f1).all(0) &
+          (5.0*(c_l + c_r) > 1.2*(u_r - u_l)))
+    rel = np.abs(f1 - f0).max(0)/np.abs(f0).max(0)
+    sel = np.flatnonzero(ok & (rel > 1e-6))
+    assert len(sel) >= 64
+    sel = sel[:64*(len(sel)//64)]
+    ul, ur, n = ul[:, sel], ur[:, sel], n[:, sel]
+    print(f'\n[exact p-min] {len(sel)} pairs', flush=True)
+    got = {}
+    for pm in (1e-3, -1.0):
+        got[pm], _ = _device_flux(be, 'exact', ul, ur, n, pmin=pm)
+        with np.errstate(all='ignore'):
+            ref = exact(ul, ur, n, 1.4, **({'p_min': pm} if pm > 0 else {}))
+        err = np.abs(got[pm] - ref).max()/np.abs(ref).max()
+        print(f'[exact p-min] p-min {pm}: max err / max flux {err:.2e}',
+              flush=True)
+        assert err < 1e-11
+    assert np.abs(got[1e-3] - got[-1.0]).max() > 1e-3*np.abs(got[-1.0]).max()

@@ -727,3 +727,102 @@ def test_curved_simplex_free_stream_and_conservation(layout):
                 oe.pnorm = np.einsum('pi,pike->pek', sh.norm_fpts, s)
     r = o.rhs(o.ics(fns))
     assert max(np.abs(a).max() for a in r.values()) > 1e-6
+
+
+def _mono_exact(shape, a, b, c=0):
+    """Exact integral of x^a y^b (z^c) over the reference element: the
+    triangle / tet with corners at -1 and the prism (triangle x [-1, 1]),
+    from the unit-simplex moments i! j! (k!) / (i + j (+ k) + 2 (3))!."""
+    from math import comb, factorial
+
+    def simplex(powers):
+        # integral of prod (2 u_i - 1)^p_i over the unit simplex, scaled
+        # to the reference element (Jacobian 2^dim)
+        d = len(powers)
+        tot = 0.0
+        terms = [[(comb(p, k), k, p - k) for k in range(p + 1)]
+                 for p in powers]
+        import itertools as it
+        for choice in it.product(*terms):
+            coef, sign = 1.0, 1
+            ks = []
+            for cb, k, rest in choice:
+                coef *= cb*2.0**k
+                sign *= (-1)**rest
+                ks.append(k)
+            mom = np.prod([factorial(k) for k in ks])/factorial(sum(ks) + d)
+            tot += sign*coef*mom
+        return 2.0**d*tot
+    if shape == 'tri':
+        return simplex([a, b])
+    if shape == 'tet':
+        return simplex([a, b, c])
+    line = (1 - (-1)**(c + 1))/(c + 1)
+    return simplex([a, b])*line
+
+
+@pytest.mark.parametrize('shape', ['tri', 'tet', 'pri'])
+def test_quadrature_catalog_degrees(shape):
+    """Every tabulated rule with weights integrates every monomial up to the
+    degree its catalog entry states exactly (1e-12), and misses some
+    monomial of the next degree (the stated degree is the rule's own);
+    this pins the point / weight tables and the catalog together."""
+    from pyfr_amd.quadtables import CATALOG, TABLES
+    deg = {(r, n): d for r, n, d, _ in CATALOG[shape]}
+    nd = 2 if shape == 'tri' else 3
+    checked = 0
+    for (s, r, n), (pts, wts) in TABLES.items():
+        if s != shape or wts[0] is None:
+            continue
+        d = deg[r, n]
+        x = np.array(pts)
+        w = np.array(wts)
+        assert len(w) == n
+        worst, nxt = 0.0, 0.0
+        for tot in range(d + 2):
+            for a in range(tot + 1):
+                for b in range(tot + 1 - a):
+                    c = tot - a - b
+                    if nd == 2 and c:
+                        continue
+                    f = x[:, 0]**a*x[:, 1]**b*(x[:, 2]**c if nd == 3 else 1)
+                    e = abs(w @ f - _mono_exact(shape, a, b, c))
+                    if tot <= d:
+                        worst = max(worst, e)
+                    else:
+                        nxt = max(nxt, e)
+        assert worst < 1e-12, (r, n, worst)
+        # prism products of a triangle and a Gauss line rule can exceed
+        # their stated degree in some monomials but not in all
+        assert nxt > 1e-12, (r, n)
+        checked += 1
+    assert checked >= 5
+
+
+def test_quad_deg_selection():
+    """quad-deg picks the rule the reference's search picks: the defaults
+    of flux and surface-flux anti-aliasing come back from their degrees
+    (2p + 1 tets, 2(p + 2) prisms, 2(p + 1) triangles); a named family,
+    a fixed point count, and an untabulated choice."""
+    from pyfr_amd.cases import make_simplex_cfg
+    from pyfr_amd.simplex import AA_RULES, SF_TRI_RULES, qdeg_rule
+    from pyfr_amd.solvers import FluidElements
+    for p in range(1, 5):
+        assert qdeg_rule('tet', 2*p + 1) == AA_RULES['tet'][p]
+        assert qdeg_rule('pri', 2*(p + 2)) == AA_RULES['pri'][p]
+        assert qdeg_rule('tri', 2*(p + 1)) == SF_TRI_RULES[p]
+    assert qdeg_rule('hex', 5, 'gauss-legendre') == ('gauss-legendre', 27)
+    assert qdeg_rule('hex', 5) == ('witherden-vincent', 14)
+    assert qdeg_rule('tet', npts=35) == ('witherden-vincent', 35)
+    assert qdeg_rule('tet', 8) == ('witherden-vincent', 46)
+    with pytest.raises(NotImplementedError):
+        qdeg_rule('tet', 15)
+    cfg = make_simplex_cfg(3)
+    cfg.set('solver-elements-tet', 'quad-deg', '8')
+    assert FluidElements._simplex_qrule(cfg, 'tet', 3) == \
+        ('witherden-vincent', 46)
+    cfg.set('solver-elements-hex', 'quad-deg', '7')
+    with pytest.raises(NotImplementedError):
+        FluidElements._gl_n1(cfg, 'solver-elements-hex', 3, 3)
+    cfg.set('solver-elements-hex', 'quad-pts', 'gauss-legendre')
+    assert FluidElements._gl_n1(cfg, 'solver-elements-hex', 3, 3) == 4
