@@ -24,7 +24,9 @@ text tables by the reference (``pyfr/quadrules/{tet,tri,pri}/*.txt``):
   (from the file names, ``<rule>-n<npts>-d<degree>-<flags>.txt``), which
   ``quad-deg`` selection searches (``pyfr/quadrules/__init__.py:52-79``);
   the points of the rules that selection picks for degrees up to 10 (tets,
-  prisms) and 12 (triangles) are tabulated too.
+  prisms) and 12 (triangles), over all families and within the solution
+  points' families (the default of the initial conditions' projection),
+  are tabulated too.
 
 These are data (point coordinates and weights), needed bit for bit because
 the FR operators are defined by them.  Run in the development container only:
@@ -66,6 +68,10 @@ RULES = {
 }
 NDIM = {'tet': 3, 'tri': 2, 'pri': 3}
 QDEG_MAX = {'tet': 10, 'pri': 10, 'tri': 12}
+# the solution points' families, which the projection of initial
+# conditions searches by default (pyfr/solvers/base/elements.py:94-108)
+IC_FAMILIES = {'tet': ('shunn-ham',),
+               'pri': ('williams-shunn~gauss-legendre',)}
 
 
 def catalog():
@@ -97,8 +103,10 @@ def main():
     rules = dict(RULES)
     for shape, qmax in QDEG_MAX.items():
         for q in range(1, qmax + 1):
-            r = select(cat[shape], q)
-            rules.setdefault((shape, r[0], r[1]), f'{shape}/{r[4]}')
+            for fam in (None, *IC_FAMILIES.get(shape, ())):
+                r = select(cat[shape], q, fam)
+                if r:
+                    rules.setdefault((shape, r[0], r[1]), f'{shape}/{r[4]}')
     out = ['"""Tabulated point sets (generated by make_quadtables.py; see its',
            'docstring for provenance).  TABLES[(shape, rule, npts)] =',
            '(points, weights) with points as tuples of floats; CATALOG[shape]',

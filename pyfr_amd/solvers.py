@@ -235,17 +235,58 @@ class FluidElements:
         return [rho, *rhovs, E]
 
     def set_ics_from_cfg(self):
+        """``pyfr/solvers/base/elements.py:86-129``: the [soln-ics]
+        expressions at the solution points, or, with [soln-ics]
+        quad-deg(-<etype>), at the points of the rule of that degree from
+        the family quad-pts(-<etype>) (default: the solution points' family)
+        and L2-projected onto the element's space, variable by variable in
+        conservative form."""
         env = self.cfg.items_as('constants', float)
-        coords = self.geom.ploc_at(self.basis.upts)
+        qdeg = (self.cfg.getint('soln-ics', f'quad-deg-{self.name}', 0) or
+                self.cfg.getint('soln-ics', 'quad-deg', 0))
+        if qdeg:
+            qpts, m8 = self._ics_projection(qdeg)
+        else:
+            qpts, m8 = self.basis.upts, None
+        coords = self.geom.ploc_at(qpts)
         env |= dict(zip('xyz', coords.swapaxes(0, 1)))
         ics = [_npeval(self.cfg.getexpr('soln-ics', v), env)
                for v in self.privars]
-        ics = [np.broadcast_to(v, (self.nupts, self.neles)) for v in ics]
+        ics = [np.broadcast_to(v, (len(qpts), self.neles)) for v in ics]
         con = self.pri_to_con(ics, self.cfg.getfloat('constants', 'gamma'))
         out = np.empty((self.nupts, self.nvars, self.neles))
         for i, v in enumerate(con):
-            out[:, i] = v
+            out[:, i] = v if m8 is None else m8 @ v
         return out
+
+    def _ics_projection(self, qdeg):
+        """(points, L2 projection onto the solution points) of the
+        quadrature rule of degree >= qdeg (``pyfr/shapes.py:18-19``)."""
+        from pyfr_amd.shapes import gauss_legendre, l2_projection_1d
+        from pyfr_amd.simplex import qdeg_rule
+        if self.name == 'hex':
+            name = self.cfg.get('soln-ics', 'quad-pts-hex',
+                                self.cfg.get('soln-ics', 'quad-pts',
+                                             'gauss-legendre'))
+            r, npts = qdeg_rule('hex', qdeg, name)
+            if r != 'gauss-legendre':
+                raise NotImplementedError('IC projection on hexes takes '
+                                          'Gauss-Legendre rules')
+            n = round(npts ** (1/3))
+            x = gauss_legendre(n)[0]
+            P = l2_projection_1d(self.basis.x1d, n)
+            pts = np.array([(x[i], x[j], x[k]) for k in range(n)
+                            for j in range(n) for i in range(n)])
+            return pts, np.kron(P, np.kron(P, P))
+        from pyfr_amd.quadtables import TABLES
+        upts_rule = {'tet': 'shunn-ham',
+                     'pri': 'williams-shunn~gauss-legendre'}[self.name]
+        name = self.cfg.get('soln-ics', f'quad-pts-{self.name}',
+                            upts_rule)
+        key = (self.name, *qdeg_rule(self.name, qdeg, name))
+        qp, qw = (np.array(a, dtype=float) for a in TABLES[key])
+        ob = self.basis.ubasis.ortho
+        return qp, ob.at(self.basis.upts).T @ (qw*ob.at(qp))
 
     # -- geometry
     @cached_property

@@ -1,0 +1,112 @@
+"""Initial conditions by L2 projection ([soln-ics] quad-deg,
+``pyfr/solvers/base/elements.py:86-129``), CPU: fields in the element's
+space come back as they are evaluated at the solution points; for others
+the projection is the one an independent basis gives (the oracle's
+orthonormal simplex bases, NumPy's Legendre polynomials for hexes) with the
+rule of that degree, and it converges to point evaluation with the
+resolution."""
+import numpy as np
+import pytest
+
+import oracle.basis as ob
+from pyfr_amd.cases import make_cfg, make_simplex_cfg
+from pyfr_amd.mesh import BoxMesh, SimplexBoxMesh
+from pyfr_amd.solvers import SHAPES, SYSTEMS
+
+POLY = {'rho': '1 + 0.1*x - 0.05*y', 'u': '0.2', 'v': '-0.1', 'w': '0.05',
+        'p': '1 + 0.05*z'}
+WAVE = {'rho': '1 + 0.2*sin(x)*cos(y)', 'u': '0.3*sin(z)', 'v': '0',
+        'w': '0.1', 'p': '1'}
+
+
+def _eles(etype, order, ics, qdeg=None, n=2):
+    if etype == 'hex':
+        mesh = BoxMesh(n)
+        cfg = make_cfg('euler', order)
+    else:
+        mesh = SimplexBoxMesh.stacked(n, 1, 1)
+        cfg = make_simplex_cfg(order)
+    for k, v in ics.items():
+        cfg.set('soln-ics', k, v)
+    if qdeg:
+        cfg.set('soln-ics', 'quad-deg', str(qdeg))
+    cls = SYSTEMS['euler'].elementscls
+    return cls(SHAPES[etype], mesh.spts[etype], cfg)
+
+
+@pytest.mark.parametrize('etype', ['hex', 'tet', 'pri'])
+@pytest.mark.parametrize('order', [1, 2, 3])
+def test_projection_keeps_fields_of_the_space(etype, order):
+    """Conservative fields linear in x, y, z (affine elements) lie in the
+    space: projecting them changes nothing."""
+    e0 = _eles(etype, order, POLY)
+    e1 = _eles(etype, order, POLY, qdeg=2*order + 1)
+    a, b = e0.set_ics_from_cfg(), e1.set_ics_from_cfg()
+    assert np.abs(a - b).max() < 1e-12
+
+
+@pytest.mark.parametrize('etype', ['tet', 'pri'])
+def test_simplex_projection_matches_oracle_basis(etype):
+    """The projection of a non-polynomial field equals the one built from
+    the oracle's orthonormal basis and the same quadrature rule, element by
+    element."""
+    from pyfr_amd.quadtables import TABLES
+    from pyfr_amd.simplex import qdeg_rule
+    order, qdeg = 2, 6
+    e = _eles(etype, order, WAVE, qdeg=qdeg)
+    got = e.set_ics_from_cfg()
+    name = {'tet': 'shunn-ham', 'pri': 'williams-shunn~gauss-legendre'}
+    qp, qw = (np.array(a) for a in TABLES[(etype,
+                                           *qdeg_rule(etype, qdeg,
+                                                      name[etype]))])
+    basis = {'tet': ob.TetBasis, 'pri': ob.PriBasis}[etype](order,
+                                                            e.basis.upts)
+    phi_u = basis.ortho_basis_at(e.basis.upts, do_clean=False)
+    phi_q = basis.ortho_basis_at(qp, do_clean=False)
+    P = phi_u.T @ (qw*phi_q)
+    x = e.geom.ploc_at(qp)                              # (nq, 3, E)
+    rho = 1 + 0.2*np.sin(x[:, 0])*np.cos(x[:, 1])
+    assert np.abs(got[:, 0] - P @ rho).max() < 1e-12
+
+
+def test_hex_projection_matches_legendre():
+    """Hexes: the tensor projection with the Gauss-Legendre rule of degree
+    >= quad-deg, against NumPy's Legendre polynomials."""
+    from numpy.polynomial import legendre as L
+    order, qdeg = 2, 7                      # 4 points per direction
+    e = _eles('hex', order, WAVE, qdeg=qdeg)
+    got = e.set_ics_from_cfg()
+    xq, wq = L.leggauss(4)
+    xs = e.basis.x1d
+    V = np.array([[L.legval(t, np.eye(order + 1)[k])*np.sqrt(k + 0.5)
+                   for k in range(order + 1)] for t in xs])
+    Vq = np.array([[L.legval(t, np.eye(order + 1)[k])*np.sqrt(k + 0.5)
+                    for k in range(order + 1)] for t in xq])
+    P1 = V @ (Vq.T*wq)
+    P = np.kron(P1, np.kron(P1, P1))
+    qp = np.array([(xq[i], xq[j], xq[k]) for k in range(4)
+                   for j in range(4) for i in range(4)])
+    x = e.geom.ploc_at(qp)
+    rho = 1 + 0.2*np.sin(x[:, 0])*np.cos(x[:, 1])
+    assert np.abs(got[:, 0] - P @ rho).max() < 1e-12
+
+
+@pytest.mark.parametrize('etype', ['hex', 'tet', 'pri'])
+def test_projection_converges_to_point_values(etype):
+    """Projection and point evaluation differ by O(h^(p+1)): halving h cuts
+    the difference by about 2^(p+1) (p = 2)."""
+    d = []
+    for n in (2, 4):
+        a = _eles(etype, 2, WAVE, n=n).set_ics_from_cfg()
+        b = _eles(etype, 2, WAVE, qdeg=6, n=n).set_ics_from_cfg()
+        d.append(np.abs(a - b).max())
+    assert d[1] < d[0]/4, d
+
+
+def test_quad_deg_per_type():
+    """quad-deg-<etype> takes precedence over quad-deg."""
+    e = _eles('tet', 2, WAVE)
+    e.cfg.set('soln-ics', 'quad-deg', '1')
+    e.cfg.set('soln-ics', 'quad-deg-tet', '6')
+    ref = _eles('tet', 2, WAVE, qdeg=6).set_ics_from_cfg()
+    assert np.array_equal(e.set_ics_from_cfg(), ref)
