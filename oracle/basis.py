@@ -329,8 +329,9 @@ class HexShapeOracle:
 
 
 # ---------------------------------------------------------------------------
-# Simplex and prism bases (pyfr/polys.py:117-167, 198-276, 279-338) and the
-# TetShape / PriShape operators (pyfr/shapes.py:476-570).  Point tables are
+# Simplex, prism and pyramid bases (pyfr/polys.py:117-167, 198-276,
+# 279-392) and the TetShape / PriShape / PyrShape operators
+# (pyfr/shapes.py:476-640).  Point tables are
 # passed in by the caller (the tests take them from the product's embedded
 # tables, which are themselves checked against the reference's rule files).
 # ---------------------------------------------------------------------------
@@ -446,6 +447,56 @@ class PriBasis(_Basis):
                 for pij, qij, rij in pab for hk, dhk in zip(hc, dhc)]
 
 
+class PyrBasis(_Basis):
+    """``pyfr/polys.py:332-392``: (1 - c)^(i + j) P_i(a) P_j(b)
+    P_k^(2(i+j+1), 0)(c), a = 2p/(1 - r), b = 2q/(1 - r), c = r, scaled to
+    unit norm; i, j <= order, k <= order - max(i, j)."""
+
+    def _abc(self, p, q, r):
+        with np.errstate(divide='ignore', invalid='ignore'):
+            a = np.where(r != 1, 2*p/(1 - r), 0)
+            b = np.where(r != 1, 2*q/(1 - r), 0)
+        return a, b, r
+
+    def _scaled_leg(self, z, diff=False):
+        f = jacobi_diff if diff else jacobi
+        return [2**(-k - 0.25)*(k + 0.5)**0.5*jp
+                for k, jp in enumerate(f(self.order, 0, 0, z))]
+
+    def _ortho(self, p, q, r):
+        a, b, c = self._abc(p, q, r)
+        pa, pb = self._scaled_leg(a), self._scaled_leg(b)
+        ob = []
+        for i, pi in enumerate(pa):
+            for j, pj in enumerate(pb):
+                jc = jacobi(self.order - max(i, j), 2*(i + j + 1), 0, c)
+                for k, pk in enumerate(jc):
+                    ob.append((2*(i + j + k) + 3)**0.5*(1 - c)**(i + j)
+                              * pi*pj*pk)
+        return ob
+
+    def _jac_ortho(self, p, q, r):
+        a, b, c = self._abc(p, q, r)
+        f, df = self._scaled_leg(a), self._scaled_leg(a, True)
+        g, dg = self._scaled_leg(b), self._scaled_leg(b, True)
+        ob = []
+        for i, (fi, dfi) in enumerate(zip(f, df)):
+            for j, (gj, dgj) in enumerate(zip(g, dg)):
+                n = self.order - max(i, j)
+                h = jacobi(n, 2*(i + j + 1), 0, c)
+                dh = jacobi_diff(n, 2*(i + j + 1), 0, c)
+                # d/dp and d/dq through a, b; d/dr through a, b and c
+                w = (1 - c)**(i + j - 1) if i + j > 0 else 1
+                for k, (hk, dhk) in enumerate(zip(h, dh)):
+                    ck = (2*(i + j + k) + 3)**0.5
+                    dp = 2*w*dfi*gj*hk
+                    dq = 2*w*fi*dgj*hk
+                    dr = (w*(a*dfi*gj + b*fi*dgj - (i + j)*fi*gj)*hk
+                          + (1 - c)**(i + j)*fi*gj*dhk)
+                    ob.append([ck*dp, ck*dq, ck*dr])
+        return ob
+
+
 TET_FACES = [
     ('tri', lambda s, t: (s, t, -1*np.ones_like(s)), (0, 0, -1)),
     ('tri', lambda s, t: (s, -1*np.ones_like(s), t), (0, -1, 0)),
@@ -459,17 +510,27 @@ PRI_FACES = [
     ('quad', lambda s, t: (-s, s, t), (1, 1, 0)),
     ('quad', lambda s, t: (-1*np.ones_like(s), s, t), (-1, 0, 0)),
 ]
+# pyfr/shapes.py:573-585
+PYR_FACES = [
+    ('quad', lambda s, t: (s, t, -1*np.ones_like(s)), (0, 0, -1)),
+    ('tri', lambda s, t: (s + (t + 1)/2, (t - 1)/2, t), (0, -1, 0.5)),
+    ('tri', lambda s, t: ((1 - t)/2, -s - (t + 1)/2, t), (1, 0, 0.5)),
+    ('tri', lambda s, t: (-s - (t + 1)/2, (1 - t)/2, t), (0, 1, 0.5)),
+    ('tri', lambda s, t: ((t - 1)/2, s + (t + 1)/2, t), (-1, 0, 0.5)),
+]
 
 
 class SimplexShapeOracle:
-    """TetShape / PriShape operators by the reference's algorithm.
+    """TetShape / PriShape / PyrShape operators by the reference's
+    algorithm.
 
     tables: {'upts': (pts, wts), 'tri_fpts': pts, 'tri_q36': (pts, wts)}."""
 
     def __init__(self, name, order, tables):
         self.name, self.order = name, order
-        self.faces = TET_FACES if name == 'tet' else PRI_FACES
-        basis = TetBasis if name == 'tet' else PriBasis
+        self.faces = {'tet': TET_FACES, 'pri': PRI_FACES,
+                      'pyr': PYR_FACES}[name]
+        basis = {'tet': TetBasis, 'pri': PriBasis, 'pyr': PyrBasis}[name]
         self.upts = np.asarray(tables['upts'][0], dtype=float)
         self.nupts = len(self.upts)
         self.ubasis = basis(order, self.upts)

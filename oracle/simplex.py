@@ -1,5 +1,5 @@
 """Oracle: Euler and Navier-Stokes FR right-hand sides on meshes of
-straight-sided tetrahedra and prisms (test infrastructure).
+straight-sided tetrahedra, prisms and pyramids (test infrastructure).
 
 Same kernel decompositions as the hex oracle: Euler ``pyfr/solvers/
 baseadvec/system.py:57-136`` (disu -> intcflux -> tdisf -> tdivtpcorf ->
@@ -18,7 +18,7 @@ periods), independent of the product's node-based connectivity.
 """
 import numpy as np
 
-from oracle.basis import HexBasis, HexShapeOracle, PriBasis, \
+from oracle.basis import HexBasis, HexShapeOracle, PriBasis, PyrBasis, \
     SimplexShapeOracle, TetBasis
 from oracle.bcs import bccflux_euler, bccflux_ns, bcconu
 from oracle.geom import kopriva_metric
@@ -30,13 +30,23 @@ STD_VERTS = {
                     dtype=float),
     'pri': np.array([(-1, -1, -1), (1, -1, -1), (-1, 1, -1), (-1, -1, 1),
                      (1, -1, 1), (-1, 1, 1)], dtype=float),
+    'pyr': np.array([(-1, -1, -1), (1, -1, -1), (-1, 1, -1), (1, 1, -1),
+                     (0, 0, 1)], dtype=float),
 }
-LINBASIS = {'tet': TetBasis, 'pri': PriBasis, 'hex': HexBasis}
+LINBASIS = {'tet': TetBasis, 'pri': PriBasis, 'pyr': PyrBasis,
+            'hex': HexBasis}
 
 
 def std_ele(name, k):
     """Equispaced order-k shape points: ``TetShape.std_ele``
-    (``pyfr/shapes.py:502-510``) and ``PriShape.std_ele`` (``:555-562``)."""
+    (``pyfr/shapes.py:502-510``), ``PriShape.std_ele`` (``:555-562``) and
+    ``PyrShape.std_ele`` (``:619-627``: the 2k + 1 points of [-1, 1],
+    every other one per layer)."""
+    if name == 'pyr':
+        x = np.linspace(-1, 1, 2*k + 1)
+        return np.array([(p, q, r) for i, r in enumerate(x[::2])
+                         for q in x[i:2*k + 1 - i:2]
+                         for p in x[i:2*k + 1 - i:2]])
     x = np.linspace(-1, 1, k + 1)
     if name == 'hex':
         return np.array([(p, q, r) for r in x for q in x for p in x])
@@ -51,7 +61,8 @@ def std_ele(name, k):
 def shape_order(name, nspts):
     for k in range(1, 16):
         n = {'tet': (k + 1)*(k + 2)*(k + 3)//6, 'hex': (k + 1)**3,
-             'pri': (k + 1)**2*(k + 2)//2}[name]
+             'pri': (k + 1)**2*(k + 2)//2,
+             'pyr': (k + 1)*(k + 2)*(2*k + 3)//6}[name]
         if n == nspts:
             return k
     raise ValueError('Invalid shape point count')

@@ -1254,11 +1254,12 @@ struct FluxPlan {
 };
 
 // Row-group counts with a k_tdivf_flux4 instantiation: tets p = 1..4
-// (4, 10, 20, 35 rows) and prisms p = 1..4 (6, 18, 40, 75 rows)
+// (4, 10, 20, 35 rows), prisms p = 1..4 (6, 18, 40, 75 rows) and pyramids
+// p = 1..4 (5, 14, 30, 55 rows)
 static bool flux4_ok(int g)
 {
-    return g == 1 || g == 2 || g == 3 || g == 5 || g == 9 || g == 10 ||
-           g == 19;
+    return g == 1 || g == 2 || g == 3 || g == 4 || g == 5 || g == 8 ||
+           g == 9 || g == 10 || g == 14 || g == 19;
 }
 
 template <typename T, int EP>
@@ -1283,8 +1284,8 @@ static void launch_flux_ep(const FluxPlan *p, int wpb, long ncol,
                                    dtb, (float)(gamma - 1));                \
                 break;
             switch (p->g4) {
-                FR_G4(1) FR_G4(2) FR_G4(3) FR_G4(5) FR_G4(9) FR_G4(10)
-                FR_G4(19)
+                FR_G4(1) FR_G4(2) FR_G4(3) FR_G4(4) FR_G4(5) FR_G4(8)
+                FR_G4(9) FR_G4(10) FR_G4(14) FR_G4(19)
             }
 #undef FR_G4
             return;

@@ -168,27 +168,38 @@ class HexGeometry:
 
 
 class AffineGeometry:
-    """Metric terms of straight-sided tetrahedra and right prisms, whose
-    reference-to-physical maps are affine: the reference's metric pipeline
+    """Metric terms of straight-sided tetrahedra, right prisms and
+    pyramids with parallelogram bases, whose reference-to-physical maps are
+    affine: the reference's metric pipeline
     (``elements.py:331-440``) then yields one S = |J| J^-1 and |J| per
     element at every point.
 
     eles: (nverts, neles, 3) in ``std_ele(1)`` vertex order; vertices 1, 2,
-    3 are the neighbours of vertex 0 along the three reference axes for both
-    shapes, so dX/dxi_a = (v_{a+1} - v_0)/2.
+    3 are the neighbours of vertex 0 along the three reference axes for
+    tets and prisms, so dX/dxi_a = (v_{a+1} - v_0)/2.
     """
 
     def __init__(self, eles, name):
         v = np.asarray(eles, dtype=float)
         self.name = name
         self.v0 = v[0]                                   # (E, 3)
-        j = np.stack([(v[a + 1] - v[0])/2 for a in range(3)], axis=1)
+        if name == 'pyr':
+            # vertices 1, 2 neighbour vertex 0 on the base, 3 is opposite
+            # it and 4 is the apex: dX/dxi = (v1 - v0)/2, (v2 - v0)/2 and
+            # (v4 - base centre)/2 when the base is a parallelogram
+            # (PyrShape.jac_exprs, pyfr/shapes.py:587-604)
+            if np.abs(v[3] - v[1] - v[2] + v[0]).max() > 1e-10:
+                raise ValueError('pyramids must have parallelogram bases')
+            j = np.stack([(v[1] - v[0])/2, (v[2] - v[0])/2,
+                          (v[4] - (v[1] + v[2])/2)/2], axis=1)
+        else:
+            j = np.stack([(v[a + 1] - v[0])/2 for a in range(3)], axis=1)
         if name == 'pri':
             # a right prism: the top triangle is the bottom one translated
             if (np.abs(v[4] - v[3] - (v[1] - v[0])).max() > 1e-12 or
                     np.abs(v[5] - v[3] - (v[2] - v[0])).max() > 1e-12):
                 raise ValueError('prisms must be affine (right prisms)')
-        elif name != 'tet':
+        elif name not in ('tet', 'pyr'):
             raise ValueError(f'no affine map for {name}')
         self.jac = j                                     # (E, a, b)
         s = np.empty_like(j)
@@ -241,11 +252,12 @@ class SimplexLagrange:
 
 
 def simplex_shape_order(name, nspts):
-    """Shape order k of a tet ((k+1)(k+2)(k+3)/6 points) or prism
-    ((k+1)^2(k+2)/2 points)."""
+    """Shape order k of a tet ((k+1)(k+2)(k+3)/6 points), prism
+    ((k+1)^2(k+2)/2 points) or pyramid ((k+1)(k+2)(2k+3)/6 points)."""
     for k in range(1, 16):
-        n = ((k + 1)*(k + 2)*(k + 3)//6 if name == 'tet'
-             else (k + 1)**2*(k + 2)//2)
+        n = {'tet': (k + 1)*(k + 2)*(k + 3)//6,
+             'pri': (k + 1)**2*(k + 2)//2,
+             'pyr': (k + 1)*(k + 2)*(2*k + 3)//6}[name]
         if n == nspts:
             return k
     raise ValueError(f'Invalid {name} shape point count {nspts}')

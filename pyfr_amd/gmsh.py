@@ -52,10 +52,10 @@ GMSH_TYPES = {2: ('tri', 3), 9: ('tri', 6), 21: ('tri', 10), 23: ('tri', 15),
               5: ('hex', 8), 12: ('hex', 27), 92: ('hex', 64),
               93: ('hex', 125),
               6: ('pri', 6), 13: ('pri', 18), 90: ('pri', 40),
-              91: ('pri', 75)}
-# pyramids (any order): read by the reference, no pyramid elements here
-_DECLINED = {7, 14, 118, 119}
-NVERTS = {'tri': 3, 'quad': 4, 'tet': 4, 'hex': 8, 'pri': 6}
+              91: ('pri', 75),
+              7: ('pyr', 5), 14: ('pyr', 14), 118: ('pyr', 30),
+              119: ('pyr', 55)}
+NVERTS = {'tri': 3, 'quad': 4, 'tet': 4, 'hex': 8, 'pri': 6, 'pyr': 5}
 
 # Gmsh's reference-element conventions, in its corner numbering: the edges
 # (directed: edge nodes run from the first corner to the second) and the
@@ -67,17 +67,20 @@ _G_EDGES = {
     'hex': [(0, 1), (0, 3), (0, 4), (1, 2), (1, 5), (2, 3), (2, 6), (3, 7),
             (4, 5), (4, 7), (5, 6), (6, 7)],
     'pri': [(0, 1), (0, 2), (0, 3), (1, 2), (1, 4), (2, 5), (3, 4), (3, 5),
-            (4, 5)]}
+            (4, 5)],
+    'pyr': [(0, 1), (0, 3), (0, 4), (1, 2), (1, 4), (2, 3), (2, 4), (3, 4)]}
 _G_FACES = {
     'tri': [], 'quad': [],
     'tet': [(0, 2, 1), (0, 1, 3), (0, 3, 2), (3, 1, 2)],
     'hex': [(0, 3, 2, 1), (0, 1, 5, 4), (0, 4, 7, 3), (1, 2, 6, 5),
             (2, 3, 7, 6), (4, 5, 6, 7)],
-    'pri': [(0, 2, 1), (3, 4, 5), (0, 1, 4, 3), (0, 3, 5, 2), (1, 2, 5, 4)]}
+    'pri': [(0, 2, 1), (3, 4, 5), (0, 1, 4, 3), (0, 3, 5, 2), (1, 2, 5, 4)],
+    'pyr': [(0, 1, 4), (3, 0, 4), (1, 2, 4), (2, 3, 4), (0, 3, 2, 1)]}
 # Gmsh corner j -> PyFR vertex: NODEMAP[kind][i] is the Gmsh corner at
 # PyFR vertex i (std_ele(1) order)
 NODEMAP = {'tet': [0, 1, 2, 3], 'hex': [0, 1, 3, 2, 4, 5, 7, 6],
-           'pri': [0, 1, 2, 3, 4, 5], 'tri': [0, 1, 2], 'quad': [0, 1, 3, 2]}
+           'pri': [0, 1, 2, 3, 4, 5], 'pyr': [0, 1, 3, 2, 4],
+           'tri': [0, 1, 2], 'quad': [0, 1, 3, 2]}
 
 
 def _inset(kind, V, k):
@@ -119,6 +122,12 @@ def _gmsh_points(kind, k, V):
         pts += _gmsh_points('hex', k - 2, _inset('hex', V, k))
     elif kind == 'tet' and k >= 4:
         pts += _gmsh_points('tet', k - 4, _inset('tet', V, k))
+    elif kind == 'pyr' and k >= 3:
+        # an order k - 3 pyramid: base corners inset as for the other
+        # shapes, the apex moved 2/k of the way towards the base centre
+        inner = _inset('pyr', V, k)
+        inner[4] = V[4] + (sum(V[:4])/4 - V[4])*2/k
+        pts += _gmsh_points('pyr', k - 3, inner)
     elif kind == 'pri' and k >= 3:
         tri = _gmsh_points('tri', k - 3, _inset('tri', V[:3], k))
         zs = [i for i in range(1, k)]
@@ -149,6 +158,9 @@ def std_ele(kind, k):
     if kind == 'pri':
         return np.array([(x, y, z) for z in t for i, y in enumerate(t)
                          for x in t[:k + 1 - i]])
+    if kind == 'pyr':
+        from pyfr_amd.simplex import PyrShape
+        return PyrShape.std_ele(k)
     raise ValueError(f'Unknown element kind {kind}')
 
 
@@ -202,13 +214,15 @@ def _linear_weights(kind, k):
             return np.stack([one, x, y, z, x*y, x*z, y*z, x*y*z], 1)
         if kind == 'tet':
             return np.stack([one, x, y, z], 1)
+        if kind == 'pyr':
+            return np.stack([one, x, y, z, x*y], 1)
         return np.stack([one, x, y, z, x*z, y*z], 1)
     return mono(std_ele(kind, k)) @ np.linalg.inv(mono(std_ele(kind, 1)))
 
 
 FACES = FACE_VERTS
 # codec order of the volume element types (Gmsh type number)
-ETYPE_ORDER = ['tet', 'hex', 'pri']
+ETYPE_ORDER = ['tet', 'hex', 'pri', 'pyr']
 
 
 class _Lines:
@@ -332,9 +346,6 @@ class GmshMesh:
     def _add_ele(self, gtype, pents, nodes):
         if gtype in (1, 8, 26, 27, 15):
             return              # lines and points: not used in 3-D
-        if gtype in _DECLINED:
-            raise NotImplementedError(
-                f'Gmsh element type {gtype} (pyramid) is not supported')
         if gtype not in GMSH_TYPES:
             raise ValueError(f'Unsupported element type {gtype}')
         if len(nodes) != GMSH_TYPES[gtype][1]:
@@ -403,6 +414,14 @@ class GmshMesh:
             arr = np.array(rows, dtype=np.int64) - self._nodeoff
             # first-order representation: the corners, PyFR vertex order
             corners = arr[:, :NVERTS[petype]][:, NODEMAP[petype]]
+            if petype == 'pyr':
+                # the affine pyramid map needs a parallelogram base
+                # (pyfr/readers/base.py:135-146)
+                v = self._nodes[corners[:, :4]]
+                if np.any(np.abs(v[:, 0] - v[:, 1] - v[:, 2] + v[:, 3])
+                          > 1e-10):
+                    raise ValueError('Pyramids with non-parallelogram bases '
+                                     'are currently unsupported')
             mask = sum(pbits[p] for p in pents if p in pbits)
             if mask:
                 if petype not in ETYPE_ORDER:

@@ -1,11 +1,13 @@
 """Regenerate pyfr_amd/quadtables.py from the published point sets held as
-text tables by the reference (``pyfr/quadrules/{tet,tri,pri}/*.txt``):
+text tables by the reference (``pyfr/quadrules/{tet,tri,pri,pyr}/*.txt``):
 
 * tetrahedron solution points: Shunn & Ham (2012), n = 4, 10, 20, 35;
 * triangle flux points: Williams & Shunn (2014), n = 3, 6, 10, 15, and the
   36-point rule the reference integrates triangular faces with;
 * prism solution points: the Williams-Shunn triangle x Gauss-Legendre line
   product, n = 6, 18, 40, 75;
+* pyramid solution points: the reference's Gauss-Legendre pyramid sets,
+  n = 5, 14, 30, 55 (points only; no weights);
 * element quadrature rules of flux anti-aliasing at p = 1..4: the rule the
   reference's default selection picks (``pyfr/shapes.py:146-176``, with
   ``pyfr/quadrules/__init__.py:52-79``: the fewest points among the rules of
@@ -20,11 +22,12 @@ text tables by the reference (``pyfr/quadrules/{tet,tri,pri}/*.txt``):
   Vincent n = 12, 16, 25.
 
 * CATALOG[shape]: (rule, npts, degree, flags) of every rule file of stated
-  degree the reference holds for tets, prisms, triangles, hexes and quads
+  degree the reference holds for tets, prisms, pyramids, triangles, hexes
+  and quads
   (from the file names, ``<rule>-n<npts>-d<degree>-<flags>.txt``), which
   ``quad-deg`` selection searches (``pyfr/quadrules/__init__.py:52-79``);
   the points of the rules that selection picks for degrees up to 10 (tets,
-  prisms) and 12 (triangles), over all families and within the solution
+  prisms, pyramids) and 12 (triangles), over all families and within the solution
   points' families (the default of the initial conditions' projection),
   are tabulated too.
 
@@ -65,9 +68,13 @@ RULES = {
     ('tri', 'witherden-vincent', 12): 'tri/witherden-vincent-n12-d6-sp.txt',
     ('tri', 'witherden-vincent', 16): 'tri/witherden-vincent-n16-d8-sp.txt',
     ('tri', 'witherden-vincent', 25): 'tri/witherden-vincent-n25-d10-sp.txt',
+    ('pyr', 'gauss-legendre', 5): 'pyr/gauss-legendre-n5-su.txt',
+    ('pyr', 'gauss-legendre', 14): 'pyr/gauss-legendre-n14-su.txt',
+    ('pyr', 'gauss-legendre', 30): 'pyr/gauss-legendre-n30-su.txt',
+    ('pyr', 'gauss-legendre', 55): 'pyr/gauss-legendre-n55-su.txt',
 }
-NDIM = {'tet': 3, 'tri': 2, 'pri': 3}
-QDEG_MAX = {'tet': 10, 'pri': 10, 'tri': 12}
+NDIM = {'tet': 3, 'tri': 2, 'pri': 3, 'pyr': 3}
+QDEG_MAX = {'tet': 10, 'pri': 10, 'tri': 12, 'pyr': 10}
 # the solution points' families, which the projection of initial
 # conditions searches by default (pyfr/solvers/base/elements.py:94-108)
 IC_FAMILIES = {'tet': ('shunn-ham',),
@@ -76,7 +83,7 @@ IC_FAMILIES = {'tet': ('shunn-ham',),
 
 def catalog():
     out = {}
-    for shape in ('tet', 'pri', 'tri', 'hex', 'quad'):
+    for shape in ('tet', 'pri', 'pyr', 'tri', 'hex', 'quad'):
         rows = []
         for fn in sorted(os.listdir(os.path.join(SRC, shape))):
             m = re.match(r'([a-zA-Z0-9\-~+]+)-n(\d+)(?:-d(\d+))?'

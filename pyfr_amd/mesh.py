@@ -182,8 +182,11 @@ PRI_FACE_VERTS = [(0, 1, 2), (3, 4, 5), (0, 1, 4, 3), (1, 2, 5, 4),
 # in HexShape (x fastest) order
 HEX_FACE_VERTS = [tuple(v for v in range(8) if ((v >> ax) & 1) == (side > 0))
                   for ax, side in HEX_FACE_AXES]
+# pyramid: the base (vertices 0, 1, 3, 2 around it), then the triangles
+# towards -y, +x, +y and -x (PyrShape, pyfr/shapes.py:573-585)
+PYR_FACE_VERTS = [(0, 1, 3, 2), (0, 1, 4), (1, 3, 4), (2, 3, 4), (0, 2, 4)]
 FACE_VERTS = {'tet': TET_FACE_VERTS, 'pri': PRI_FACE_VERTS,
-              'hex': HEX_FACE_VERTS}
+              'hex': HEX_FACE_VERTS, 'pyr': PYR_FACE_VERTS}
 
 
 def _kuhn_offsets():
@@ -228,10 +231,57 @@ def _hex_offsets():
                       for x in (0, 1)]])                # (1, 8, 3)
 
 
+def _oriented(verts, name):
+    """Vertices in std_ele(1) order with a positive Jacobian: swap the
+    two neighbours of vertex 0 that span the reference x and y axes
+    (tets: 1, 2; pyramids: 1, 2 with 3 opposite 0 on the base)."""
+    v = np.array(verts, dtype=float)
+    if name == 'tet':
+        a, b, c = v[1] - v[0], v[2] - v[0], v[3] - v[0]
+    else:
+        a, b, c = v[1] - v[0], v[2] - v[0], v[4] - (v[1] + v[2])/2
+    if np.linalg.det(np.array([a, b, c])) < 0:
+        v[[1, 2]] = v[[2, 1]]
+    return v
+
+
+def _pyr_offsets(tets=False):
+    """The 6 pyramids of a unit cube with their apex at its centre, one on
+    each face (bases on the cube faces, so neighbouring cells and hex or
+    'prx' layers meet them on whole quad faces).  tets: the z-face
+    pyramids split into two tets along the (0,0)-(1,1) face diagonal, so
+    the cube's z faces conform with tet and 'pri' layers; returns
+    (pyramids (4, 5, 3), tets (4, 4, 3))."""
+    c = (0.5, 0.5, 0.5)
+    pyrs, tts = [], []
+    for ax in (2, 0, 1):
+        for side in (0, 1):
+            u, w = [a for a in range(3) if a != ax]
+            quad = []
+            for j in (0, 1):
+                for i in (0, 1):
+                    p = [0, 0, 0]
+                    p[ax], p[u], p[w] = side, i, j
+                    quad.append(p)
+            if tets and ax == 2:
+                # (0,0), (1,0), (1,1) and (0,0), (1,1), (0,1)
+                for tri in ((0, 1, 3), (0, 3, 2)):
+                    tts.append(_oriented([quad[t] for t in tri] + [c],
+                                         'tet'))
+            else:
+                pyrs.append(_oriented(quad + [c], 'pyr'))
+    return np.array(pyrs), np.array(tts)
+
+
 CELL_SPLITS = {'tet': _kuhn_offsets(), 'pri': _prism_offsets(),
-               'prx': _prism_x_offsets(), 'hex': _hex_offsets()}
+               'prx': _prism_x_offsets(), 'hex': _hex_offsets(),
+               'pyr': _pyr_offsets()[0]}
+# mixed splits: a cell split into elements of several types
+MIXED_SPLITS = {'pyt': {'pyr': _pyr_offsets(True)[0],
+                        'tet': _pyr_offsets(True)[1]}}
 # element type of each cell split
-SPLIT_ETYPE = {'tet': 'tet', 'pri': 'pri', 'prx': 'pri', 'hex': 'hex'}
+SPLIT_ETYPE = {'tet': 'tet', 'pri': 'pri', 'prx': 'pri', 'hex': 'hex',
+               'pyr': 'pyr'}
 
 
 class SimplexBoxMesh:
@@ -242,8 +292,12 @@ class SimplexBoxMesh:
     each block is periodic on its own along the axes where ``periodic`` is
     true, and the faces on its other sides are boundaries named
     'x0'/'x1', 'y0'/'y1', 'z0'/'z1' (low/high side).  Cells split into 6
-    tets ('tet'), 2 prisms extruded along z ('pri') or along x ('prx'), or
-    stay one hex ('hex') (``CELL_SPLITS``).  Tet and 'pri' layers share the
+    tets ('tet'), 2 prisms extruded along z ('pri') or along x ('prx'), 6
+    pyramids with their apex at the cell centre ('pyr'), 4 such pyramids on
+    the x and y faces and 4 tets on the z faces ('pyt'), or stay one hex
+    ('hex') (``CELL_SPLITS``, ``MIXED_SPLITS``).  'pyr' layers meet hex
+    and 'prx' layers on quad faces; 'pyt' layers meet tet and 'pri' layers
+    on the tets' diagonal.  Tet and 'pri' layers share the
     cube diagonal, so their faces conform; 'prx' and hex layers meet on
     whole quad faces.  A tet or 'pri' layer next to a hex or 'prx' layer
     leaves faces unpaired (an error).
@@ -260,7 +314,7 @@ class SimplexBoxMesh:
     of order k > 1 are all flagged curved in ``spts_curved``.
     """
     ndims = 3
-    etype_order = ('hex', 'tet', 'pri')
+    etype_order = ('hex', 'tet', 'pri', 'pyr')
 
     def __init__(self, blocks, periodic=(True, True, True), sorder=1,
                  warp=None):
@@ -273,20 +327,23 @@ class SimplexBoxMesh:
                 raise ValueError('layers must fill the block')
             k0 = 0
             for split, nl in layers:
-                et = SPLIT_ETYPE[split]
                 kk, jj, ii = np.meshgrid(np.arange(k0, k0 + nl),
                                          np.arange(ny), np.arange(nx),
                                          indexing='ij')
                 cell = np.stack([ii.ravel(), jj.ravel(), kk.ravel()], 1)
-                off = CELL_SPLITS[split]                # (ns, nv, 3)
-                # split-major order: consecutive elements have the same
-                # orientation, so their neighbours' flux points sit in the
-                # same rows and interface gathers stay coalesced
-                lat = off[:, None] + cell[None, :, None, :]  # (ns, C, nv, 3)
-                lat = lat.reshape(-1, off.shape[1], 3)       # (E, nv, 3)
-                verts[et].append(lat*h)
-                nodes[et].append((lat, np.full(len(lat), noff),
-                                  np.array([nx, ny, nz])))
+                parts = MIXED_SPLITS.get(split,
+                                         {SPLIT_ETYPE.get(split): None})
+                for et, off in parts.items():
+                    if off is None:
+                        off = CELL_SPLITS[split]        # (ns, nv, 3)
+                    # split-major order: consecutive elements have the same
+                    # orientation, so their neighbours' flux points sit in
+                    # the same rows and interface gathers stay coalesced
+                    lat = off[:, None] + cell[None, :, None, :]
+                    lat = lat.reshape(-1, off.shape[1], 3)   # (E, nv, 3)
+                    verts[et].append(lat*h)
+                    nodes[et].append((lat, np.full(len(lat), noff),
+                                      np.array([nx, ny, nz])))
                 k0 += nl
             noff += 1
         self.spts, self._nodes = {}, {}

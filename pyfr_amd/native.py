@@ -42,7 +42,7 @@ import numpy as np
 from pyfr_amd.h5 import File
 
 CREATOR = 'pyfr-amd'
-NVERTS = {'tet': 4, 'hex': 8, 'pri': 6}
+NVERTS = {'tet': 4, 'hex': 8, 'pri': 6, 'pyr': 5}
 
 
 def ele_dtype(nnodes, nfaces):

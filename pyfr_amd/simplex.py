@@ -1,18 +1,19 @@
-"""FR operators for tetrahedra and prisms (dense operators).
+"""FR operators for tetrahedra, prisms and pyramids (dense operators).
 
-Shapes follow the reference's ``TetShape`` and ``PriShape``
-(``pyfr/shapes.py:476-570``): reference elements, face tables (projections and
-unnormalised reference normals), solution points (Shunn-Ham tets,
-Williams-Shunn x Gauss-Legendre prisms), flux points (Williams-Shunn
-triangles, Gauss-Legendre quads) and the operator layouts m0..m6
-(``pyfr/shapes.py:89-135``, ``:231-259``).
+Shapes follow the reference's ``TetShape``, ``PriShape`` and ``PyrShape``
+(``pyfr/shapes.py:476-640``): reference elements, face tables (projections
+and unnormalised reference normals), solution points (Shunn-Ham tets,
+Williams-Shunn x Gauss-Legendre prisms, Gauss-Legendre pyramids), flux
+points (Williams-Shunn triangles, Gauss-Legendre quads) and the operator
+layouts m0..m6 (``pyfr/shapes.py:89-135``, ``:231-259``).
 
 The orthonormal bases are built here independently of the reference's Jacobi
 formulas: products of Legendre polynomials span the polynomial space
 (P_p for tets and triangles, P_p(tri) x P_p(line) for prisms) and are
 orthonormalised by Cholesky against an exact collapsed (Duffy) Gauss-Legendre
-quadrature.  The nodal operators and the correction functions (which use the
-space's reproducing kernel) do not depend on which orthonormal basis spans
+quadrature (for pyramids, homogenised Legendre products, ``_span_pyr``).  The
+nodal operators and the correction functions (which use the space's
+reproducing kernel) do not depend on which orthonormal basis spans
 the space, so the result matches the reference's operators to rounding.
 """
 from functools import cached_property
@@ -48,6 +49,9 @@ def _degrees(kind, p):
     if kind == 'pri':
         return [(i, j, k) for i in range(p + 1) for j in range(p + 1 - i)
                 for k in range(p + 1)]
+    if kind == 'pyr':
+        return [(i, j, k) for i in range(p + 1) for j in range(p + 1)
+                for k in range(p + 1 - max(i, j))]
     if kind == 'quad':
         return [(i, j) for i in range(p + 1) for j in range(p + 1)]
     if kind == 'line':
@@ -84,7 +88,32 @@ def _collapsed_rule(kind, n):
         pts = np.array([(*tp, z) for z in g for tp in t])
         wts = np.array([tw_*wz for wz in w for tw_ in tw])
         return pts, wts
+    if kind == 'pyr':
+        # (x, y, z) = (a s, b s, c) with s = (1 - c)/2 over the cube
+        a, b, c = np.meshgrid(g, g, g, indexing='ij')
+        wa, wb, wc = np.meshgrid(w, w, w, indexing='ij')
+        a, b, c = a.ravel(), b.ravel(), c.ravel()
+        s = (1 - c)/2
+        return (np.stack([a*s, b*s, c], 1),
+                (wa*wb*wc).ravel()*s**2)
     raise ValueError(kind)
+
+
+def _homleg(n, x, s):
+    """Q_k(x, s) = s^k P_k(x/s), k = 0..n, and its partial derivatives in
+    x and s, by the Legendre recurrence multiplied through by s^k (a
+    polynomial in x and s, regular where s = 0)."""
+    Q = np.zeros((n + 1,) + x.shape)
+    Qx, Qs = np.zeros_like(Q), np.zeros_like(Q)
+    Q[0] = 1.0
+    if n >= 1:
+        Q[1], Qx[1] = x, 1.0
+    for k in range(2, n + 1):
+        a, b = (2*k - 1)/k, (k - 1)/k
+        Q[k] = a*x*Q[k - 1] - b*s*s*Q[k - 2]
+        Qx[k] = a*(Q[k - 1] + x*Qx[k - 1]) - b*s*s*Qx[k - 2]
+        Qs[k] = a*x*Qs[k - 1] - b*(2*s*Q[k - 2] + s*s*Qs[k - 2])
+    return Q, Qx, Qs
 
 
 class OrthoBasis:
@@ -93,7 +122,11 @@ class OrthoBasis:
     def __init__(self, kind, order):
         self.kind, self.order = kind, order
         self.degs = _degrees(kind, order)
-        qp, qw = _collapsed_rule(kind, order + 3)
+        # the Gram matrix's integrands have degree <= 2p in each collapsed
+        # coordinate, plus 2(p + 1) in c for pyramids (s^(i + j) factors and
+        # the s^2 Jacobian)
+        qp, qw = _collapsed_rule(kind, 2*order + 3 if kind == 'pyr'
+                                 else order + 3)
         V = self._span(qp)                               # (nb, nq)
 
         # phi = C @ span; the Legendre-product Gram matrix is ill
@@ -109,6 +142,8 @@ class OrthoBasis:
     def _span(self, pts, deriv=None):
         pts = np.atleast_2d(pts)
         p = self.order
+        if self.kind == 'pyr':
+            return self._span_pyr(pts, deriv)
         nd = pts.shape[1]
         PD = [_leg(p, pts[:, d]) for d in range(nd)]
         out = []
@@ -117,6 +152,32 @@ class OrthoBasis:
             for d in range(nd):
                 P, D = PD[d]
                 v = v*(D[dg[d]] if d == deriv else P[dg[d]])
+            out.append(v)
+        return np.array(out)
+
+    def _span_pyr(self, pts, deriv):
+        """The pyramid space: Q_i(x, s) Q_j(y, s) P_k(z), s = (1 - z)/2,
+        i, j <= p, k <= p - max(i, j).  For fixed (i, j) the reference's
+        (1 - z)^(i + j) P_i(2x/(1 - z)) P_j(2y/(1 - z)) P_k^(2(i+j+1),0)(z)
+        (pyfr/polys.py:332-392) is 2^(i + j) Q_i Q_j times a degree-k
+        polynomial in z, so both span the same space."""
+        x, y, z = pts.T
+        s = (1 - z)/2
+        p = self.order
+        qx, qxx, qxs = _homleg(p, x, s)
+        qy, qyy, qys = _homleg(p, y, s)
+        L, D = _leg(p, z)
+        out = []
+        for i, j, k in self.degs:
+            if deriv is None:
+                v = qx[i]*qy[j]*L[k]
+            elif deriv == 0:
+                v = qxx[i]*qy[j]*L[k]
+            elif deriv == 1:
+                v = qx[i]*qyy[j]*L[k]
+            else:
+                v = (qx[i]*qy[j]*D[k] -
+                     (qxs[i]*qy[j] + qx[i]*qys[j])*L[k]/2)
             out.append(v)
         return np.array(out)
 
@@ -166,6 +227,9 @@ AA_RULES = {
             3: ('witherden-vincent', 35), 4: ('witherden-vincent', 59)},
     'pri': {1: ('witherden-vincent', 28), 2: ('witherden', 46),
             3: ('witherden', 82), 4: ('witherden', 128)},
+    # pyramids: degree >= 2(p + 1)
+    'pyr': {1: ('witherden-vincent', 10), 2: ('witherden-vincent', 24),
+            3: ('witherden', 44), 4: ('witherden', 76)},
 }
 # Triangle-face rule of surface-flux anti-aliasing per order (the fewest
 # points among the symmetric rules of degree >= 2(p + 1),
@@ -375,7 +439,13 @@ class _SimplexShape:
 
     @cached_property
     def upts_wts(self):
-        return np.array(self._upts_rule[1])
+        w = self._upts_rule[1]
+        if w[0] is not None:
+            return np.array(w)
+        # point sets without weights: the interpolatory weights, the
+        # integrals of the nodal basis functions over the element
+        qp, qw = _collapsed_rule(self.name, self.order + 3)
+        return self.ubasis.nodal_basis_at(qp).T @ qw
 
 
 class TetShape(_SimplexShape):
@@ -439,4 +509,39 @@ class PriShape(_SimplexShape):
                          for p in pts1d[:(sptord + 1 - i)]])
 
 
-SIMPLEX_SHAPES = {'tet': TetShape, 'pri': PriShape}
+class PyrShape(_SimplexShape):
+    """``pyfr/shapes.py:573-640``: a square base on z = -1 and the apex at
+    (0, 0, 1); four triangular faces whose unnormalised reference normals
+    have z component 1/2."""
+    name = 'pyr'
+    nverts = 5
+    faces = [
+        ('quad', lambda s, t: (s, t, -1.0), (0, 0, -1)),
+        ('tri', lambda s, t: (s + (t + 1)/2, (t - 1)/2, t), (0, -1, 0.5)),
+        ('tri', lambda s, t: ((1 - t)/2, -s - (t + 1)/2, t), (1, 0, 0.5)),
+        ('tri', lambda s, t: (-s - (t + 1)/2, (1 - t)/2, t), (0, 1, 0.5)),
+        ('tri', lambda s, t: ((t - 1)/2, s + (t + 1)/2, t), (-1, 0, 0.5)),
+    ]
+
+    @cached_property
+    def _upts_rule(self):
+        n = self.order + 1
+        return TABLES[('pyr', 'gauss-legendre', n*(n + 1)*(2*n + 1)//6)]
+
+    @cached_property
+    def upts(self):
+        return np.array(self._upts_rule[0])
+
+    @staticmethod
+    def std_ele(sptord):
+        """``PyrShape.std_ele`` (``shapes.py:619-627``): layer i at
+        z = -1 + 2i/k holds (k + 1 - i)^2 points, x fastest, at every other
+        one of the 2k + 1 equispaced points of [-1, 1] from the i-th."""
+        k = sptord
+        t = np.linspace(-1, 1, 2*k + 1)
+        return np.array([(x, y, z) for i, z in enumerate(t[::2])
+                         for y in t[i:2*k + 1 - i:2]
+                         for x in t[i:2*k + 1 - i:2]])
+
+
+SIMPLEX_SHAPES = {'tet': TetShape, 'pri': PriShape, 'pyr': PyrShape}

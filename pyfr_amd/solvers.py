@@ -280,7 +280,8 @@ class FluidElements:
             return pts, np.kron(P, np.kron(P, P))
         from pyfr_amd.quadtables import TABLES
         upts_rule = {'tet': 'shunn-ham',
-                     'pri': 'williams-shunn~gauss-legendre'}[self.name]
+                     'pri': 'williams-shunn~gauss-legendre',
+                     'pyr': 'gauss-legendre'}[self.name]
         name = self.cfg.get('soln-ics', f'quad-pts-{self.name}',
                             upts_rule)
         key = (self.name, *qdeg_rule(self.name, qdeg, name))

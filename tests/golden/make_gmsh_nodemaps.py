@@ -1,7 +1,7 @@
 """Write tests/golden/gmsh_nodemaps.json: a SHA-256 digest of each Gmsh ->
 PyFR node map the reference tabulates (``_nodemaps`` in
 ``pyfr/readers/gmsh.py:58-150``) for the shapes this build reads (tets,
-hexes, prisms and their tri / quad faces; pyramids and lines are left out).
+hexes, prisms, pyramids and their tri / quad faces; lines are left out).
 The reference's file is parsed as text (``ast.literal_eval`` of the one
 dict literal; nothing imported or executed), and only the digests are kept:
 they pin the maps the reader derives from Gmsh's ordering rules without
@@ -24,7 +24,7 @@ import numpy as np
 SRC = '/root/reference/pyfr/readers/gmsh.py'
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)),
                    'gmsh_nodemaps.json')
-KINDS = {'tet', 'hex', 'pri', 'tri', 'quad'}
+KINDS = {'tet', 'hex', 'pri', 'pyr', 'tri', 'quad'}
 SKIP = {('tri', 21)}
 
 

@@ -79,6 +79,28 @@ def test_tet_prism_round_trip(version):
     assert m.bcon == {}
 
 
+@pytest.mark.parametrize('version', ['2.2', '4.1'])
+@pytest.mark.parametrize('layers,k', [([('pyr', 2)], 1),
+                                      ([('hex', 1), ('pyr', 1)], 2),
+                                      ([('tet', 1), ('pyt', 1)], 3),
+                                      ([('pyt', 1), ('pri', 1)], 4)])
+def test_pyramid_round_trip(version, layers, k):
+    """Pyramids alone and with hexes, tets and prisms, first order and
+    warped of shape order k (quad faces with hofaces at k = 3): shape
+    points, element types and pairs as generated."""
+    L = 2*np.pi
+    blk = [((2, 2, 2), L/2, layers)]
+    flat = SimplexBoxMesh(blk, sorder=k)
+    sides = _sim_sides(flat, (L, L, L))
+    ref = SimplexBoxMesh(blk, sorder=k, warp=_swarp if k > 1 else None)
+    m = GmshMesh(io.StringIO(mesh_to_msh(ref, sides, version,
+                                         hofaces=k == 3)))
+    assert m.etypes == sorted(ref.spts)
+    for t in m.etypes:
+        assert np.allclose(m.spts[t], ref.spts[t], rtol=0, atol=1e-12)
+    assert _pairs(m) == _pairs(ref)
+
+
 def _tiny(version='4.1'):
     raw = BoxMesh(1, periodic=(False, False, False))
     spec = {s: 'wall' for s in ('x0', 'x1', 'y0', 'y1', 'z0', 'z1')}
@@ -89,11 +111,14 @@ def test_declines_and_errors():
     txt = _tiny('2.2')
     m = GmshMesh(io.StringIO(txt))
     assert len(m.bcon['wall']) == 6 and m.con == ([], [])
-    # a pyramid (Gmsh type 7, 5 nodes) is declined; a second-order hex
-    # (type 12) with 8 nodes is malformed
-    with pytest.raises(NotImplementedError, match='pyramid'):
-        GmshMesh(io.StringIO(re.sub(r' 5 2 1 1 ((\d+ ){4}\d+).*',
-                                    r' 7 2 1 1 \1', txt)))
+    # a pyramid (Gmsh type 7) whose base is not a parallelogram (the
+    # hex's Gmsh corners 0, 1, 2, 4) is declined, as by the reference; a
+    # second-order hex (type 12) with 8 nodes is malformed
+    hx = re.search(r' 5 2 1 1 ((\d+ ){7}\d+)', txt)[1].split()
+    bad = ' '.join(hx[i] for i in (0, 1, 2, 4, 6))
+    with pytest.raises(ValueError, match='parallelogram'):
+        GmshMesh(io.StringIO(txt.replace(' 5 2 1 1 ' + ' '.join(hx),
+                                         ' 7 2 1 1 ' + bad, 1)))
     with pytest.raises(ValueError, match='node count'):
         GmshMesh(io.StringIO(txt.replace(' 5 2 1 1 ', ' 12 2 1 1 ')))
     # a boundary face without a physical name leaves a face unpaired
@@ -119,12 +144,12 @@ GOLDEN = os.path.join(os.path.dirname(__file__), 'golden',
 
 def test_nodemaps_pinned():
     """Every node map the reference tabulates for these shapes (orders 1-5
-    hexes and quads, 1-6 tets and prisms, 1-4 triangles) equals the one
-    derived from Gmsh's ordering rules."""
+    hexes and quads, 1-6 tets, prisms and pyramids, 1-4 triangles) equals
+    the one derived from Gmsh's ordering rules."""
     from pyfr_amd.gmsh import gmsh_nodemap, npts_order
     with open(GOLDEN) as f:
         want = json.load(f)
-    assert len(want) == 26
+    assert len(want) == 32
     for key, dig in want.items():
         kind, n = key.split('-')
         m = gmsh_nodemap(kind, npts_order(kind, int(n)))

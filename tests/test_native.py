@@ -56,6 +56,20 @@ def test_simplex_round_trip(tmp_path):
     _same(g, NativeMesh(fn))
 
 
+@pytest.mark.parametrize('layers,k', [([('hex', 1), ('pyr', 1)], 1),
+                                      ([('tet', 1), ('pyt', 1)], 2)])
+def test_pyramid_round_trip(tmp_path, layers, k):
+    L = 2*np.pi
+    blk = [((2, 2, 2), L/2, layers)]
+    sides = _sim_sides(SimplexBoxMesh(blk, sorder=k), (L, L, L))
+    gen = SimplexBoxMesh(blk, sorder=k, warp=_swarp if k > 1 else None)
+    g = GmshMesh(io.StringIO(mesh_to_msh(gen, sides)))
+    assert 'pyr' in g.etypes
+    fn = str(tmp_path/'pyr.pyfrm')
+    g.write_pyfrm(fn)
+    _same(g, NativeMesh(fn))
+
+
 def test_file_contents(tmp_path):
     """Two hexes stacked in z, walls on every side: the datasets and the
     face links by hand."""

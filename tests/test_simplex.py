@@ -732,7 +732,8 @@ def test_curved_simplex_free_stream_and_conservation(layout):
 def _mono_exact(shape, a, b, c=0):
     """Exact integral of x^a y^b (z^c) over the reference element: the
     triangle / tet with corners at -1 and the prism (triangle x [-1, 1]),
-    from the unit-simplex moments i! j! (k!) / (i + j (+ k) + 2 (3))!."""
+    from the unit-simplex moments i! j! (k!) / (i + j (+ k) + 2 (3))!; the
+    pyramid layer by layer."""
     from math import comb, factorial
 
     def simplex(powers):
@@ -757,11 +758,19 @@ def _mono_exact(shape, a, b, c=0):
         return simplex([a, b])
     if shape == 'tet':
         return simplex([a, b, c])
+    if shape == 'pyr':
+        # the layer at height z is the square |x|, |y| <= s = (1 - z)/2:
+        # int x^a y^b = 4 s^(a+b+2)/((a+1)(b+1)) (a, b even), then z = 1 - 2s
+        if a % 2 or b % 2:
+            return 0.0
+        m = a + b + 2
+        zint = 2*sum(comb(c, k)*(-2.0)**k/(m + k + 1) for k in range(c + 1))
+        return 4*zint/((a + 1)*(b + 1))
     line = (1 - (-1)**(c + 1))/(c + 1)
     return simplex([a, b])*line
 
 
-@pytest.mark.parametrize('shape', ['tri', 'tet', 'pri'])
+@pytest.mark.parametrize('shape', ['tri', 'tet', 'pri', 'pyr'])
 def test_quadrature_catalog_degrees(shape):
     """Every tabulated rule with weights integrates every monomial up to the
     degree its catalog entry states exactly (1e-12), and misses some
