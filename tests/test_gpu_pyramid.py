@@ -49,7 +49,10 @@ def _tables(et, p, aa=()):
 
 
 def _case(be, layers, order, system='euler', bcs=False, rsolver='rusanov',
-          scheme='rk4', dt=1e-3, aa=(), n=2, beta=0.0, tau=0.0):
+          scheme='rk4', dt=1e-3, aa=(), n=2, beta=0.0, tau=0.0, sorder=1,
+          warp=None, mesh=None):
+    """The integrator and the oracle on a box of the layers (or on mesh,
+    whose elements the oracle takes as they are)."""
     from oracle.simplex import OracleMixedSystem
     from pyfr_amd.mesh import SimplexBoxMesh
     from pyfr_amd.solvers import get_integrator
@@ -79,7 +82,9 @@ def _case(be, layers, order, system='euler', bcs=False, rsolver='rusanov',
                     vals[k] = eval_ics({k: v})[k]
             obcs[side] = (bctype, vals)
     nz = sum(k for _, k in layers)
-    mesh = SimplexBoxMesh([((n, n, nz), L/n, layers)], periodic)
+    if mesh is None:
+        mesh = SimplexBoxMesh([((n, n, nz), L/n, layers)], periodic,
+                              sorder=sorder, warp=warp)
     integ = get_integrator(be, mesh, cfg)
     verts = {et: v.transpose(0, 2, 1) for et, v in mesh.spts.items()}
     idx = {et: np.arange(v.shape[1]) for et, v in mesh.spts.items()}
@@ -207,3 +212,46 @@ def test_pyramid_rk4_trajectory(backend_factory, layout, precision):
     err = rel_l2(_cat(dict(zip(names, integ.soln)), names), _cat(u, names))
     print(f'[pyr rk4] 10 steps rel L2 {err:.3e}', flush=True)
     assert err < (1e-12 if precision == 'double' else 2e-5)
+
+
+@pytest.mark.parametrize('system,layout,order,sorder', [
+    ('euler', 'pyr', 2, 2), ('euler', 'tet-pyt', 3, 3),
+    ('navier-stokes', 'pyr', 3, 2), ('navier-stokes', 'hex-pyr', 2, 2)])
+def test_pyramid_curved_rhs_parity(backend_factory, system, layout, order,
+                                   sorder):
+    """Pyramids given by order-k shape points take the curved path (the
+    stored invariant-form metric at every point).  The box is not warped:
+    warped pyramids do not meet their neighbours exactly (their shape on a
+    triangular face depends on shape points off it), so the oracle could
+    not pair their faces by coordinates; tests/test_pyramid.py checks the
+    warped metric itself against the oracle."""
+    be = backend_factory()
+    tag = f'pyr curved {system} {layout} p={order} k={sorder}'
+    print(f'\n[{tag}]', flush=True)
+    integ, orc, u0 = _case(be, LAYOUTS[layout], order, system,
+                           sorder=sorder)
+    e = integ.system.ele_map['pyr']
+    assert e.curved_simplex and e.mesh_regions == {'curved': e.neles}
+    _check(integ, orc, u0, order, tag)
+
+
+def test_pyramid_gmsh_pyfrm_rhs_parity(backend_factory, tmp_path):
+    """A pyramid-tet mesh written as Gmsh, imported, written as .pyfrm and
+    read back runs the RHS of the oracle on the file's elements."""
+    import io
+    from pyfr_amd.gmsh import GmshMesh
+    from pyfr_amd.mesh import SimplexBoxMesh
+    from pyfr_amd.native import NativeMesh
+    from tests.msh_files import mesh_to_msh
+    from tests.test_gmsh import _sim_sides
+    blk = [((2, 2, 2), L/2, LAYOUTS['tet-pyt'])]
+    gen = SimplexBoxMesh(blk)
+    fn = str(tmp_path/'pyt.pyfrm')
+    GmshMesh(io.StringIO(mesh_to_msh(gen, _sim_sides(gen, (L, L, L))))) \
+        .write_pyfrm(fn)
+    mesh = NativeMesh(fn)
+    assert sorted(mesh.etypes) == ['pyr', 'tet']
+    be = backend_factory()
+    print('\n[pyr pyfrm]', flush=True)
+    integ, orc, u0 = _case(be, LAYOUTS['tet-pyt'], 3, mesh=mesh)
+    _check(integ, orc, u0, 3, 'pyr pyfrm')

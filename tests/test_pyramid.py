@@ -345,3 +345,30 @@ def test_oracle_pyramid_rhs_converges():
     assert np.log2(e2/e4) > 2 - 0.7, (e2, e4)
     p2, p4 = _density_wave_error(2, 2, True), _density_wave_error(4, 2, True)
     assert np.log2(p2/p4) < 2 - 0.7
+
+
+@pytest.mark.parametrize('p', [1, 2, 3])
+def test_curved_pyramid_geometry_matches_oracle(p):
+    """Warped order-2 pyramids: the product's stored metric (its pyramid
+    nodal bases) against the oracle's (the reference's Jacobi bases).
+    Warped pyramids do not in general meet their neighbours exactly: the
+    pyramid space restricted to a triangular face reaches degree 2k, so a
+    face's shape is not fixed by the shape points on it alone."""
+    from oracle.simplex import _Etype
+    from pyfr_amd.geometry import CurvedSimplexGeometry
+    from pyfr_amd.mesh import SimplexBoxMesh
+    from tests.test_simplex import _warp
+    L = 2*np.pi
+    m = SimplexBoxMesh([((2, 2, 2), L/2, [('pyr', 2)])], sorder=2,
+                       warp=_warp)
+    v = m.spts['pyr']
+    oe = _Etype('pyr', p, _tables(p), v.transpose(0, 2, 1))
+    g = CurvedSimplexGeometry(v, p, 'pyr')
+    assert oe.curved
+    sh = oe.sh
+    assert np.abs(g.ploc_at(sh.upts) - oe.ploc(sh.upts)).max() < 1e-13
+    assert np.abs(g.smat_at(sh.upts).transpose(1, 0, 2, 3) -
+                  oe.smats).max() < 1e-12
+    assert np.abs(g.rcpdjac_at(sh.upts) - oe.rcpdjac).max() < 1e-12
+    assert np.abs(g.pnorm_at(sh.fpts, sh.norm_fpts) - oe.pnorm).max() \
+        < 1e-12
