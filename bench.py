@@ -8,7 +8,7 @@ Taylor-Green initial field at Mach 0.1, Re 1600.  A step is one RK45 step
 (5 RHS evaluations + 5 stage updates) with all data resident in HBM.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W]
-                  [--config cfg1|cfg0|cfg3|hex64rk4]
+                  [--config cfg1|cfg0|cfg3|hex64rk4|pyr]
 
 Prints one JSON line: `value` is the FP64 plain GDOF*RHS/s; a roofline
 object for the dominant kernel (algorithmic bytes per launch / mean launch
@@ -45,6 +45,10 @@ CONFIGS = {
     'cfg3': dict(system='euler', ntet=48, npri=16, order=4, scheme='rk4',
                  dt=1e-3, precision='single',
                  workload='euler-tet48^3x6+pri16^3x2-p4-rk4-fp32'),
+    # pyramids on the same dense path: a periodic box of 32^3 cells x 6
+    # pyramids (apex at the cell centre), p=4 (55 solution points), FP32
+    'pyr': dict(system='euler', npyr=32, order=4, scheme='rk4', dt=1e-3,
+                precision='single', workload='euler-pyr32^3x6-p4-rk4-fp32'),
 }
 
 MFMA_PEAK_TFS = {'f32': 157.3, 'f64': 78.6}
@@ -58,7 +62,7 @@ ROW_PAD = None
 
 def make_cfg(c, precision='double'):
     from pyfr_amd.cases import make_cfg as mk, make_simplex_cfg, mode_ics
-    if 'ntet' in c:
+    if 'ntet' in c or 'npyr' in c:
         cfg = make_simplex_cfg(c['order'], precision, c['scheme'], c['dt'],
                                mode_ics(seed=0, nmodes=8, amp=0.05))
     else:
@@ -74,6 +78,9 @@ def make_mesh(c):
     from pyfr_amd.mesh import BoxMesh, SimplexBoxMesh
     if 'ntet' in c:
         return SimplexBoxMesh.disjoint(c['ntet'], c['npri'])
+    if 'npyr' in c:
+        n = c['npyr']
+        return SimplexBoxMesh([((n, n, n), 2*np.pi/n, [('pyr', n)])])
     return BoxMesh(c['n'])
 
 
@@ -202,6 +209,8 @@ def kernel_flops(k):
 def mesh_label(c):
     if 'ntet' in c:
         return f'{c["ntet"]}^3x6 tet + {c["npri"]}^3x2 pri (periodic boxes)'
+    if 'npyr' in c:
+        return f'{c["npyr"]}^3x6 pyr (periodic box)'
     return f'{c["n"]}^3 hex'
 
 
@@ -458,7 +467,8 @@ def main():
         'warmup': args.warmup, 'ms_per_step': round(res['ms_per_step'], 3),
         'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
         'dtype': res['dtype'],
-        'data': ('synthetic (8 seeded random-phase modes)' if 'ntet' in c
+        'data': ('synthetic (8 seeded random-phase modes)'
+                 if 'ntet' in c or 'npyr' in c
                  else 'synthetic (analytic Taylor-Green field)'),
         'config': res['config'], 'roofline': res['roofline'],
         'rhs_kernels': res['rhs_kernels'],
@@ -498,6 +508,7 @@ def main():
     if not args.no_dense and args.config == 'cfg1':
         line['dense_mixed_mesh'] = dense_report(args)
     line['cpu_baseline'] = (None if args.no_cpu_baseline or 'ntet' in c
+                            or 'npyr' in c
                             else cpu_baseline(c))
     print(json.dumps(line))
 
