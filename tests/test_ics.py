@@ -23,6 +23,10 @@ def _eles(etype, order, ics, qdeg=None, n=2):
     if etype == 'hex':
         mesh = BoxMesh(n)
         cfg = make_cfg('euler', order)
+    elif etype == 'pyr':
+        mesh = SimplexBoxMesh([((n, n, n), 2*np.pi/n, [('pyr', n)])])
+        cfg = make_simplex_cfg(order)
+        cfg.set('soln-ics', 'quad-pts-pyr', 'witherden-vincent')
     else:
         mesh = SimplexBoxMesh.stacked(n, 1, 1)
         cfg = make_simplex_cfg(order)
@@ -110,3 +114,24 @@ def test_quad_deg_per_type():
     e.cfg.set('soln-ics', 'quad-deg-tet', '6')
     ref = _eles('tet', 2, WAVE, qdeg=6).set_ics_from_cfg()
     assert np.array_equal(e.set_ics_from_cfg(), ref)
+
+
+@pytest.mark.parametrize('order', [1, 2, 3])
+def test_pyramid_projection(order):
+    """Pyramids: the Gauss-Legendre solution points carry no catalogued
+    degree, so the default family has no rule of any degree (an error, as
+    in the reference's search); with quad-pts-pyr = witherden-vincent the
+    projection keeps linear fields and converges to point evaluation."""
+    e0 = _eles('pyr', order, POLY)
+    e1 = _eles('pyr', order, POLY, qdeg=2*order + 1)
+    assert np.abs(e0.set_ics_from_cfg() - e1.set_ics_from_cfg()).max() \
+        < 1e-12
+    e1.cfg.set('soln-ics', 'quad-pts-pyr', 'gauss-legendre')
+    with pytest.raises(ValueError):
+        e1.set_ics_from_cfg()
+    d = []
+    for n in (2, 4):
+        a = _eles('pyr', 2, WAVE, n=n).set_ics_from_cfg()
+        b = _eles('pyr', 2, WAVE, qdeg=6, n=n).set_ics_from_cfg()
+        d.append(np.abs(a - b).max())
+    assert d[1] < d[0]/4, d
