@@ -27,9 +27,9 @@ text tables by the reference (``pyfr/quadrules/{tet,tri,pri,pyr}/*.txt``):
   (from the file names, ``<rule>-n<npts>-d<degree>-<flags>.txt``), which
   ``quad-deg`` selection searches (``pyfr/quadrules/__init__.py:52-79``);
   the points of the rules that selection picks for degrees up to 10 (tets,
-  prisms, pyramids) and 12 (triangles), over all families and within the solution
-  points' families (the default of the initial conditions' projection),
-  are tabulated too.
+  prisms, pyramids) and 12 (triangles), over all families and within the
+  solution points' families (the default of the initial conditions'
+  projection), are tabulated too.
 
 These are data (point coordinates and weights), needed bit for bit because
 the FR operators are defined by them.  Run in the development container only:
