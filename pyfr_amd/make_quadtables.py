@@ -8,6 +8,10 @@ text tables by the reference (``pyfr/quadrules/{tet,tri,pri,pyr}/*.txt``):
   product, n = 6, 18, 40, 75;
 * pyramid solution points: the reference's Gauss-Legendre pyramid sets,
   n = 5, 14, 30, 55 (points only; no weights);
+* the other solution / flux point families the reference offers at
+  p = 1..4: alpha-optimised tets and triangles, alpha-optimised and
+  Williams-Shunn triangles x Gauss-Legendre-Lobatto lines for prisms, and
+  Gauss-Legendre-Lobatto pyramids;
 * element quadrature rules of flux anti-aliasing at p = 1..4: the rule the
   reference's default selection picks (``pyfr/shapes.py:146-176``, with
   ``pyfr/quadrules/__init__.py:52-79``: the fewest points among the rules of
@@ -73,6 +77,20 @@ RULES = {
     ('pyr', 'gauss-legendre', 30): 'pyr/gauss-legendre-n30-su.txt',
     ('pyr', 'gauss-legendre', 55): 'pyr/gauss-legendre-n55-su.txt',
 }
+# the other solution point families of [solver-elements-<type>] soln-pts
+# and [solver-interfaces-tri] flux-pts at p = 1..4
+for _n in (4, 10, 20, 35):
+    RULES[('tet', 'alpha-opt', _n)] = f'tet/alpha-opt-n{_n}-su.txt'
+for _n, _d in ((6, 1), (18, 3), (40, 5), (75, 7)):
+    RULES[('pri', 'alpha-opt~gauss-legendre-lobatto', _n)] = \
+        f'pri/alpha-opt~gauss-legendre-lobatto-n{_n}-stu.txt'
+    RULES[('pri', 'williams-shunn~gauss-legendre-lobatto', _n)] = \
+        f'pri/williams-shunn~gauss-legendre-lobatto-n{_n}-d{_d}-pstu.txt'
+for _n in (5, 14, 30, 55):
+    RULES[('pyr', 'gauss-legendre-lobatto', _n)] = \
+        f'pyr/gauss-legendre-lobatto-n{_n}-su.txt'
+for _n in (3, 6, 10, 15):
+    RULES[('tri', 'alpha-opt', _n)] = f'tri/alpha-opt-n{_n}-su.txt'
 NDIM = {'tet': 3, 'tri': 2, 'pri': 3, 'pyr': 3}
 QDEG_MAX = {'tet': 10, 'pri': 10, 'tri': 12, 'pyr': 10}
 # the solution points' families, which the projection of initial

@@ -266,12 +266,27 @@ class _SimplexShape:
     name = None
     faces = None
 
-    def __init__(self, order, qrule=None, iqrules=None):
+    # the default solution point family ([solver-elements-<name>]
+    # soln-pts)
+    upts_default = None
+
+    def __init__(self, order, qrule=None, iqrules=None, upts_name=None,
+                 tri_fpts=None):
         """qrule: (rule name, npts) of the element quadrature rule of flux
         anti-aliasing, or None.  iqrules: surface-flux anti-aliasing, the
         face rules {'tri': (rule name, npts), 'quad': Gauss-Legendre points
-        per direction}; the flux points of each face are that rule's."""
+        per direction}; the flux points of each face are that rule's.
+        upts_name: the solution point family (soln-pts), tri_fpts: the
+        triangle-face flux point family (flux-pts); None for the defaults."""
         self.order = order
+        self.upts_name = upts_name or self.upts_default
+        self.tri_fpts = tri_fpts or 'williams-shunn'
+        for key in ((self.name, self.upts_name, self._nupts_of(order)),
+                    ('tri', self.tri_fpts, (order + 1)*(order + 2)//2)):
+            if key not in TABLES:
+                raise NotImplementedError(f'{key[0]} point set {key[1]} '
+                                          f'with {key[2]} points is not '
+                                          'tabulated')
         self.qrule = qrule
         self.iqrules = iqrules
         p = order
@@ -288,7 +303,7 @@ class _SimplexShape:
     def _std_face_pts(self, kind):
         p = self.order
         if kind == 'tri':
-            pts, _ = TABLES[('tri', 'williams-shunn', (p + 1)*(p + 2)//2)]
+            pts, _ = TABLES[('tri', self.tri_fpts, (p + 1)*(p + 2)//2)]
             return np.array(pts)
         g, _ = gauss_legendre(p + 1)
         return np.array([(g[i], g[j]) for j in range(p + 1)
@@ -452,6 +467,7 @@ class TetShape(_SimplexShape):
     """``pyfr/shapes.py:476-518``."""
     name = 'tet'
     nverts = 4
+    upts_default = 'shunn-ham'
     faces = [
         ('tri', lambda s, t: (s, t, -1.0), (0, 0, -1)),
         ('tri', lambda s, t: (s, -1.0, t), (0, -1, 0)),
@@ -459,10 +475,13 @@ class TetShape(_SimplexShape):
         ('tri', lambda s, t: (s, t, -s - t - 1), (1, 1, 1)),
     ]
 
+    @staticmethod
+    def _nupts_of(p):
+        return (p + 1)*(p + 2)*(p + 3)//6
+
     @cached_property
     def _upts_rule(self):
-        n = (self.order + 1)*(self.order + 2)*(self.order + 3)//6
-        return TABLES[('tet', 'shunn-ham', n)]
+        return TABLES[('tet', self.upts_name, self._nupts_of(self.order))]
 
     @cached_property
     def upts(self):
@@ -482,6 +501,7 @@ class PriShape(_SimplexShape):
     """``pyfr/shapes.py:521-570``."""
     name = 'pri'
     nverts = 6
+    upts_default = 'williams-shunn~gauss-legendre'
     faces = [
         ('tri', lambda s, t: (s, t, -1.0), (0, 0, -1)),
         ('tri', lambda s, t: (s, t, 1.0), (0, 0, 1)),
@@ -490,10 +510,13 @@ class PriShape(_SimplexShape):
         ('quad', lambda s, t: (-1.0, s, t), (-1, 0, 0)),
     ]
 
+    @staticmethod
+    def _nupts_of(p):
+        return (p + 1)**2*(p + 2)//2
+
     @cached_property
     def _upts_rule(self):
-        n = (self.order + 1)**2*(self.order + 2)//2
-        return TABLES[('pri', 'williams-shunn~gauss-legendre', n)]
+        return TABLES[('pri', self.upts_name, self._nupts_of(self.order))]
 
     @cached_property
     def upts(self):
@@ -515,6 +538,7 @@ class PyrShape(_SimplexShape):
     have z component 1/2."""
     name = 'pyr'
     nverts = 5
+    upts_default = 'gauss-legendre'
     faces = [
         ('quad', lambda s, t: (s, t, -1.0), (0, 0, -1)),
         ('tri', lambda s, t: (s + (t + 1)/2, (t - 1)/2, t), (0, -1, 0.5)),
@@ -523,10 +547,13 @@ class PyrShape(_SimplexShape):
         ('tri', lambda s, t: ((t - 1)/2, s + (t + 1)/2, t), (-1, 0, 0.5)),
     ]
 
+    @staticmethod
+    def _nupts_of(p):
+        return (p + 1)*(p + 2)*(2*p + 3)//6
+
     @cached_property
     def _upts_rule(self):
-        n = self.order + 1
-        return TABLES[('pyr', 'gauss-legendre', n*(n + 1)*(2*n + 1)//6)]
+        return TABLES[('pyr', self.upts_name, self._nupts_of(self.order))]
 
     @cached_property
     def upts(self):

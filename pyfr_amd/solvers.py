@@ -104,12 +104,27 @@ class FluidElements:
         if self.antialias - {'flux', 'surf-flux'}:
             raise ValueError('Invalid anti-alias options')
         self.fluxaa = 'flux' in self.antialias
-        if self.antialias and basiscls is not SHAPES['hex']:
+        # solution and face point families (pyfr/shapes.py:188-190,
+        # :205-218): any tabulated set for tets, prisms and pyramids and
+        # their triangular faces; Gauss-Legendre for hexes and quad faces
+        # (the generated hex kernels carry those operators)
+        qfp = cfg.get('solver-interfaces-quad', 'flux-pts', 'gauss-legendre')
+        if qfp != 'gauss-legendre':
+            raise NotImplementedError(f'quad flux-pts {qfp} is not built')
+        if basiscls is SHAPES['hex']:
+            sp = cfg.get('solver-elements-hex', 'soln-pts', 'gauss-legendre')
+            if sp != 'gauss-legendre':
+                raise NotImplementedError(f'hex soln-pts {sp} is not built')
+        if basiscls is not SHAPES['hex']:
             qrule = (self._simplex_qrule(cfg, basiscls.name, order)
                      if self.fluxaa else None)
             iq = (self._simplex_iqrules(cfg, order)
                   if 'surf-flux' in self.antialias else None)
-            self.basis = basis = basiscls(order, qrule, iq)
+            self.basis = basis = basiscls(
+                order, qrule, iq,
+                cfg.get(f'solver-elements-{basiscls.name}', 'soln-pts',
+                        None),
+                cfg.get('solver-interfaces-tri', 'flux-pts', None))
         elif self.antialias:
             nq1 = (self._gl_n1(cfg, 'solver-elements-hex', 3, order)
                    if self.fluxaa else None)
@@ -279,11 +294,8 @@ class FluidElements:
                             for j in range(n) for i in range(n)])
             return pts, np.kron(P, np.kron(P, P))
         from pyfr_amd.quadtables import TABLES
-        upts_rule = {'tet': 'shunn-ham',
-                     'pri': 'williams-shunn~gauss-legendre',
-                     'pyr': 'gauss-legendre'}[self.name]
         name = self.cfg.get('soln-ics', f'quad-pts-{self.name}',
-                            upts_rule)
+                            self.basis.upts_name)
         key = (self.name, *qdeg_rule(self.name, qdeg, name))
         qp, qw = (np.array(a, dtype=float) for a in TABLES[key])
         ob = self.basis.ubasis.ortho

@@ -21,19 +21,18 @@ pytestmark = pytest.mark.gpu
 L = 2*np.pi
 
 
-def _tables(et, p, aa=()):
+def _tables(et, p, aa=(), fams=None):
+    """Oracle tables; fams: {etype: soln-pts, 'tri': flux-pts}."""
     from pyfr_amd.quadtables import TABLES
     from pyfr_amd.shapes import gauss_legendre
     from pyfr_amd.simplex import AA_RULES, SF_TRI_RULES, SIMPLEX_SHAPES
     if et == 'hex':
         return ({'nq1': p + 2} if 'flux' in aa else {}) | (
             {'nf1': p + 2} if 'surf-flux' in aa else {})
-    sh = SIMPLEX_SHAPES[et](p)
-    t = {'upts': TABLES[(et, *{'tet': ('shunn-ham',),
-                               'pri': ('williams-shunn~gauss-legendre',),
-                               'pyr': ('gauss-legendre',)}[et],
-                         sh.nupts)],
-         'tri_fpts': TABLES[('tri', 'williams-shunn',
+    fams = fams or {}
+    sh = SIMPLEX_SHAPES[et](p, upts_name=fams.get(et))
+    t = {'upts': TABLES[(et, sh.upts_name, sh.nupts)],
+         'tri_fpts': TABLES[('tri', fams.get('tri', 'williams-shunn'),
                              (p + 1)*(p + 2)//2)][0],
          'tri_q36': TABLES[('tri', 'williams-shunn', 36)]}
     if 'flux' in aa:
@@ -50,7 +49,7 @@ def _tables(et, p, aa=()):
 
 def _case(be, layers, order, system='euler', bcs=False, rsolver='rusanov',
           scheme='rk4', dt=1e-3, aa=(), n=2, beta=0.0, tau=0.0, sorder=1,
-          warp=None, mesh=None):
+          warp=None, mesh=None, fams=None):
     """The integrator and the oracle on a box of the layers (or on mesh,
     whose elements the oracle takes as they are)."""
     from oracle.simplex import OracleMixedSystem
@@ -63,6 +62,11 @@ def _case(be, layers, order, system='euler', bcs=False, rsolver='rusanov',
     cfg.set('backend-hip', 'fused', str(be.fused))
     if aa:
         cfg.set('solver', 'anti-alias', ', '.join(aa))
+    for et, fam in (fams or {}).items():
+        if et == 'tri':
+            cfg.set('solver-interfaces-tri', 'flux-pts', fam)
+        else:
+            cfg.set(f'solver-elements-{et}', 'soln-pts', fam)
     c = {'gamma': 1.4}
     if system == 'navier-stokes':
         cfg.set('constants', 'mu', repr(MU))
@@ -88,7 +92,7 @@ def _case(be, layers, order, system='euler', bcs=False, rsolver='rusanov',
     integ = get_integrator(be, mesh, cfg)
     verts = {et: v.transpose(0, 2, 1) for et, v in mesh.spts.items()}
     idx = {et: np.arange(v.shape[1]) for et, v in mesh.spts.items()}
-    orc = OracleMixedSystem(verts, order, {et: _tables(et, order, aa)
+    orc = OracleMixedSystem(verts, order, {et: _tables(et, order, aa, fams)
                                            for et in verts},
                             [((L, L, nz*L/n), idx)], rsolver=rsolver,
                             system=system, constants=c, periodic=periodic,
@@ -255,3 +259,26 @@ def test_pyramid_gmsh_pyfrm_rhs_parity(backend_factory, tmp_path):
     print('\n[pyr pyfrm]', flush=True)
     integ, orc, u0 = _case(be, LAYOUTS['tet-pyt'], 3, mesh=mesh)
     _check(integ, orc, u0, 3, 'pyr pyfrm')
+
+
+@pytest.mark.parametrize('system,layout,order,fams', [
+    ('euler', 'tet-pyt', 3, {'tet': 'alpha-opt',
+                             'pyr': 'gauss-legendre-lobatto',
+                             'tri': 'alpha-opt'}),
+    ('navier-stokes', 'pyt-pri', 2,
+     {'pri': 'williams-shunn~gauss-legendre-lobatto',
+      'pyr': 'gauss-legendre-lobatto'}),
+    ('euler', 'pyt-pri', 4, {'pri': 'alpha-opt~gauss-legendre-lobatto',
+                             'tet': 'alpha-opt', 'tri': 'alpha-opt'})])
+def test_point_families_rhs_parity(backend_factory, system, layout, order,
+                                   fams):
+    """[solver-elements-<type>] soln-pts and [solver-interfaces-tri]
+    flux-pts other than the defaults, on tet/pyramid/prism meshes."""
+    be = backend_factory()
+    tag = f'families {system} {layout} p={order} {fams}'
+    print(f'\n[{tag}]', flush=True)
+    integ, orc, u0 = _case(be, LAYOUTS[layout], order, system, fams=fams)
+    for et, fam in fams.items():
+        if et in integ.system.ele_map:
+            assert integ.system.ele_map[et].basis.upts_name == fam
+    _check(integ, orc, u0, order, tag)
