@@ -22,8 +22,8 @@ written against ``pyfr/backends/base``:
   ``pyfr/integrators/base.py:282-296``, ``:401-426``,
   ``pyfr/integrators/explicit/controllers.py:42-60``.
 
-Scope: meshes of hexahedra, straight-sided tetrahedra and right prisms on
-one GPU, periodic or with the reference's boundary conditions (values may
+Scope: meshes of hexahedra, tetrahedra, prisms and pyramids (straight or
+curved) on one GPU, periodic or with the reference's boundary conditions (values may
 depend on x, y, z and t; the controlled char-riem-inv types included);
 anti-aliasing (flux, surf-flux) on every element type; no MPI interfaces,
 no source terms.
@@ -136,10 +136,11 @@ class FluidElements:
         self.name = basis.name
         self.nupts, self.nfpts = basis.nupts, basis.nfpts
         self.nfacefpts = basis.nfacefpts
-        # straight-sided tets and right prisms (vertices only) have one S
-        # and |J| per element; other tets and prisms (order-k shape points,
-        # or prisms whose map is not affine) take the stored metric at every
-        # point, all elements of the type in the curved region
+        # straight-sided tets, right prisms and pyramids with parallelogram
+        # bases (vertices only) have one S and |J| per element; others
+        # (order-k shape points, or prisms whose map is not affine) take the
+        # stored metric at every point, all elements of the type in the
+        # curved region
         self.affine = False
         self.curved_simplex = False
         if self.name in SIMPLEX_SHAPES:
