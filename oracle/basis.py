@@ -200,10 +200,26 @@ class HexShapeOracle:
     ndims = 3
     nfaces = 6
 
-    def __init__(self, order):
+    def __init__(self, order, upts='gauss-legendre'):
+        """upts: 'gauss-legendre' or 'gauss-legendre-lobatto' solution
+        points (``[solver-elements-hex] soln-pts``); the flux points are
+        Gauss-Legendre either way."""
         self.order = order
         n = order + 1
-        self.upts, self.upts_wts = gauss_legendre_hex(n)
+        if upts == 'gauss-legendre':
+            self.upts, self.upts_wts = gauss_legendre_hex(n)
+        else:
+            # Lobatto: the ends and the roots of P'_(n-1); weights
+            # 2/(n(n - 1) P_(n-1)^2)
+            c = np.zeros(n)
+            c[-1] = 1
+            x = np.concatenate([[-1.0], np.sort(np.polynomial.legendre.legroots(
+                np.polynomial.legendre.legder(c))), [1.0]])
+            w = 2/(n*(n - 1)*np.polynomial.legendre.legval(x, c)**2)
+            self.upts = np.array([(x[i], x[j], x[k]) for k in range(n)
+                                  for j in range(n) for i in range(n)])
+            self.upts_wts = np.array([w[i]*w[j]*w[k] for k in range(n)
+                                      for j in range(n) for i in range(n)])
         self.nupts = n**3
         self.ubasis = HexBasis(order, self.upts)
         fq, self.fq_wts = gauss_legendre_quad(n)

@@ -78,9 +78,9 @@ class _Etype:
         if name == 'hex':
             # GL solution and flux points (HexShapeOracle); its face points
             # are the tensor GL points the prisms' quad faces also carry
-            if set(tables) - {'nq1', 'nf1'}:
-                raise ValueError('hex tables take only nq1 / nf1')
-            sh = HexShapeOracle(order)
+            if set(tables) - {'nq1', 'nf1', 'upts'}:
+                raise ValueError('hex tables take only nq1 / nf1 / upts')
+            sh = HexShapeOracle(order, tables.get('upts', 'gauss-legendre'))
             if 'nf1' in tables:
                 sh.set_surface_quadrature(tables['nf1'])
         else:

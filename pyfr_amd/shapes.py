@@ -77,6 +77,38 @@ def gauss_legendre(n):
     return np.array(x), np.array(w)
 
 
+def gauss_legendre_lobatto(n):
+    """n-point Gauss-Legendre-Lobatto nodes (ascending: -1, the roots of
+    P'_(n-1), 1) and weights 2/(n(n - 1) P_(n-1)(x)^2), by the Newton
+    iteration x <- x - (x P_N - P_(N-1))/((N + 1) P_N), N = n - 1, from the
+    Chebyshev-Gauss-Lobatto points (Trefethen's formulation), in plain
+    Python floats so every host gets the same bits."""
+    if n < 2:
+        raise ValueError('Gauss-Legendre-Lobatto rules need n >= 2')
+    N = n - 1
+    xs, ws = [], []
+    for i in range(n):
+        x = -math.cos(math.pi*i/N)
+        for _ in range(100):
+            p0, p1 = 1.0, x
+            for k in range(2, N + 1):
+                p0, p1 = p1, ((2*k - 1)*x*p1 - (k - 1)*p0)/k
+            dx = (x*p1 - p0)/((N + 1)*p1)
+            x -= dx
+            if abs(dx) < 1e-17:
+                break
+        if i in (0, N):
+            x = float(2*(i == N) - 1)
+        p0, p1 = 1.0, x
+        for k in range(2, N + 1):
+            p0, p1 = p1, ((2*k - 1)*x*p1 - (k - 1)*p0)/k
+        xs.append(x)
+        ws.append(2/(N*(N + 1)*p1*p1))
+    if n % 2:
+        xs[N//2] = 0.0
+    return np.array(xs), np.array(ws)
+
+
 def lagrange_at(x, pts):
     """Values l_j(pts[q]) of the Lagrange basis through x: (len(pts), n)."""
     n = len(x)
@@ -166,12 +198,14 @@ def l2_projection_1d(x, nq):
 
 
 class HexShape:
-    """Hex FR element of order p with GL solution and GL flux points."""
+    """Hex FR element of order p with Gauss-Legendre (or
+    Gauss-Legendre-Lobatto) solution points and Gauss-Legendre flux
+    points."""
     name = 'hex'
     ndims = 3
     nfaces = 6
 
-    def __init__(self, order, nq1=None, nf1=None):
+    def __init__(self, order, nq1=None, nf1=None, upts_name=None):
         """nq1: points per direction of the Gauss-Legendre quadrature rule
         of flux anti-aliasing (default order + 2, the reference's
         ``(order + 2)**3`` hex rule, ``pyfr/shapes.py:147-176``).  nf1:
@@ -182,7 +216,20 @@ class HexShape:
             raise ValueError('Order must be >= 1')
         self.order = order
         self.n = n = order + 1
-        self.x1d, self.w1d = gauss_legendre(n)
+        # solution points ([solver-elements-hex] soln-pts): Gauss-Legendre,
+        # whose lines through the solution points end at the standard face
+        # points, or Gauss-Legendre-Lobatto, whose do not
+        self.upts_name = upts_name or 'gauss-legendre'
+        if self.upts_name == 'gauss-legendre':
+            self.x1d, self.w1d = gauss_legendre(n)
+        elif self.upts_name == 'gauss-legendre-lobatto':
+            self.x1d, self.w1d = gauss_legendre_lobatto(n)
+        else:
+            raise NotImplementedError(f'hex soln-pts {self.upts_name} is '
+                                      'not built')
+        self.on_lines = self.upts_name == 'gauss-legendre'
+        # the standard face points, which carry the face bases
+        self.xs1d = gauss_legendre(n)[0]
         self.nupts = n**3
         self.nf1 = nf = nf1 or n
         self.surf_flux = nf1 is not None
@@ -270,7 +317,7 @@ class HexShape:
     @cached_property
     def m0(self):
         n = self.n
-        if self.surf_flux:
+        if self.surf_flux or not self.on_lines:
             return self._m0_surf
         m = np.zeros((self.nfpts, self.nupts))
         for f in range(6):
@@ -323,17 +370,31 @@ class HexShape:
     def m3(self):
         n = self.n
         m = np.zeros((self.nupts, 6*n*n))
-        for f in range(6):
-            for b in range(n):
-                for a in range(n):
-                    line, _, side = self.fpt_line(f, a, b)
-                    m[line, f*n*n + a + n*b] = self.hend[side]
+        if self.on_lines:
+            for f in range(6):
+                for b in range(n):
+                    for a in range(n):
+                        line, _, side = self.fpt_line(f, a, b)
+                        m[line, f*n*n + a + n*b] = self.hend[side]
+        else:
+            # the lifting of face point (a, b): the 1-D kernel end function
+            # along the normal times the face's Lagrange basis on the
+            # standard face points, at the solution points' coordinates
+            Ls = lagrange_at(self.xs1d, self.x1d)             # (n, n)
+            for f, (nax, side, sax, tax) in enumerate(HEX_FACES):
+                for b, a in it.product(range(n), repeat=2):
+                    col = f*n*n + a + n*b
+                    for l, s_, t in it.product(range(n), repeat=3):
+                        c = [0, 0, 0]
+                        c[nax], c[sax], c[tax] = l, s_, t
+                        m[self.uidx(*c), col] = (self.hend[side][l] *
+                                                 Ls[s_, a]*Ls[t, b])
         if self.surf_flux:
             # correction from the standard face points, fed by the L2
             # projection of each face's quadrature-point values onto the
             # order-p face basis: m3 @ block_diag(proj_l2(iqrule, facebasis))
             # (pyfr/shapes.py:103-113, :18-19)
-            P = l2_projection_1d(self.x1d, self.nf1)
+            P = l2_projection_1d(self.xs1d, self.nf1)
             fp = np.kron(P, P)
             bd = np.zeros((6*n*n, self.nfpts))
             for f in range(6):

@@ -106,15 +106,12 @@ class FluidElements:
         self.fluxaa = 'flux' in self.antialias
         # solution and face point families (pyfr/shapes.py:188-190,
         # :205-218): any tabulated set for tets, prisms and pyramids and
-        # their triangular faces; Gauss-Legendre for hexes and quad faces
-        # (the generated hex kernels carry those operators)
+        # their triangular faces; Gauss-Legendre or -Lobatto hex solution
+        # points; Gauss-Legendre quad faces
         qfp = cfg.get('solver-interfaces-quad', 'flux-pts', 'gauss-legendre')
         if qfp != 'gauss-legendre':
             raise NotImplementedError(f'quad flux-pts {qfp} is not built')
-        if basiscls is SHAPES['hex']:
-            sp = cfg.get('solver-elements-hex', 'soln-pts', 'gauss-legendre')
-            if sp != 'gauss-legendre':
-                raise NotImplementedError(f'hex soln-pts {sp} is not built')
+        hsp = cfg.get('solver-elements-hex', 'soln-pts', None)
         if basiscls is not SHAPES['hex']:
             qrule = (self._simplex_qrule(cfg, basiscls.name, order)
                      if self.fluxaa else None)
@@ -130,9 +127,9 @@ class FluidElements:
                    if self.fluxaa else None)
             nf1 = (self._gl_n1(cfg, 'solver-interfaces-quad', 2, order)
                    if 'surf-flux' in self.antialias else None)
-            self.basis = basis = basiscls(order, nq1, nf1)
+            self.basis = basis = basiscls(order, nq1, nf1, hsp)
         else:
-            self.basis = basis = basiscls(order)
+            self.basis = basis = basiscls(order, upts_name=hsp)
         self.name = basis.name
         self.nupts, self.nfpts = basis.nupts, basis.nfpts
         self.nfacefpts = basis.nfacefpts

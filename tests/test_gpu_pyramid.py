@@ -26,10 +26,11 @@ def _tables(et, p, aa=(), fams=None):
     from pyfr_amd.quadtables import TABLES
     from pyfr_amd.shapes import gauss_legendre
     from pyfr_amd.simplex import AA_RULES, SF_TRI_RULES, SIMPLEX_SHAPES
+    fams = fams or {}
     if et == 'hex':
         return ({'nq1': p + 2} if 'flux' in aa else {}) | (
-            {'nf1': p + 2} if 'surf-flux' in aa else {})
-    fams = fams or {}
+            {'nf1': p + 2} if 'surf-flux' in aa else {}) | (
+            {'upts': fams['hex']} if 'hex' in fams else {})
     sh = SIMPLEX_SHAPES[et](p, upts_name=fams.get(et))
     t = {'upts': TABLES[(et, sh.upts_name, sh.nupts)],
          'tri_fpts': TABLES[('tri', fams.get('tri', 'williams-shunn'),
@@ -269,11 +270,16 @@ def test_pyramid_gmsh_pyfrm_rhs_parity(backend_factory, tmp_path):
      {'pri': 'williams-shunn~gauss-legendre-lobatto',
       'pyr': 'gauss-legendre-lobatto'}),
     ('euler', 'pyt-pri', 4, {'pri': 'alpha-opt~gauss-legendre-lobatto',
-                             'tet': 'alpha-opt', 'tri': 'alpha-opt'})])
+                             'tet': 'alpha-opt', 'tri': 'alpha-opt'}),
+    ('euler', 'hex-pyr', 3, {'hex': 'gauss-legendre-lobatto'}),
+    ('navier-stokes', 'hex-pyr', 2, {'hex': 'gauss-legendre-lobatto',
+                                     'pyr': 'gauss-legendre-lobatto'})])
 def test_point_families_rhs_parity(backend_factory, system, layout, order,
                                    fams):
     """[solver-elements-<type>] soln-pts and [solver-interfaces-tri]
-    flux-pts other than the defaults, on tet/pyramid/prism meshes."""
+    flux-pts other than the defaults, on tet/pyramid/prism meshes, and
+    Gauss-Legendre-Lobatto hexes (dense operator plans: the generated hex
+    kernels are for Gauss-Legendre points)."""
     be = backend_factory()
     tag = f'families {system} {layout} p={order} {fams}'
     print(f'\n[{tag}]', flush=True)

@@ -10,7 +10,11 @@ Gauss-Legendre-Lobatto line, the pyramid set is layered on the
 Gauss-Lobatto nodes, every set lies in its element and is symmetric), and
 the operators they give are checked against the oracle and for exactness on
 degree-p polynomials.  Hex solution points and quad flux points other than
-Gauss-Legendre are declined loudly."""
+Gauss-Legendre are declined loudly.  Hex solution points may be
+Gauss-Legendre-Lobatto: the operators (the tensor forms, with the face
+basis on the Gauss-Legendre face points off the solution-point lines) match
+the oracle's general construction (``pyfr/shapes.py:89-135``, ``:231-259``).
+"""
 import numpy as np
 import pytest
 
@@ -122,7 +126,7 @@ def test_declined_families():
     from pyfr_amd.solvers import SHAPES, SYSTEMS
     cls = SYSTEMS['euler'].elementscls
     cfg = make_cfg('euler', 2)
-    cfg.set('solver-elements-hex', 'soln-pts', 'gauss-legendre-lobatto')
+    cfg.set('solver-elements-hex', 'soln-pts', 'alpha-opt')
     with pytest.raises(NotImplementedError, match='hex soln-pts'):
         cls(SHAPES['hex'], BoxMesh(2).spts['hex'], cfg)
     cfg = make_simplex_cfg(2)
@@ -134,3 +138,28 @@ def test_declined_families():
     cfg.set('solver-elements-tet', 'soln-pts', 'no-such-set')
     with pytest.raises(NotImplementedError, match='not tabulated'):
         cls(SHAPES['tet'], m.spts['tet'], cfg)
+
+
+@pytest.mark.parametrize('p', [1, 2, 3, 4])
+@pytest.mark.parametrize('nf1', [None, 'p+2'])
+def test_hex_lobatto_operators(p, nf1):
+    """Gauss-Legendre-Lobatto hexes, plain and with surface-flux
+    anti-aliasing: the product's tensor operators against the oracle's
+    general construction; the Lobatto rule is the ends and the roots of
+    P'_p with weights exact to degree 2p - 1."""
+    from oracle.basis import HexShapeOracle
+    from pyfr_amd.shapes import HexShape, gauss_legendre_lobatto
+    x, w = gauss_legendre_lobatto(p + 1)
+    g = _gll(p + 1)
+    assert np.abs(x - g).max() < 1e-15
+    for k in range(2*p):
+        assert abs(w @ x**k - (1 - (-1)**(k + 1))/(k + 1)) < 1e-14
+    nf = p + 2 if nf1 else None
+    s = HexShape(p, None, nf, 'gauss-legendre-lobatto')
+    o = HexShapeOracle(p, 'gauss-legendre-lobatto')
+    if nf:
+        o.set_surface_quadrature(nf)
+    assert np.abs(s.upts - o.upts).max() < 1e-15
+    for op in OPS:
+        a, b = s.opmat(op), o.opmat(op)
+        assert np.abs(a - b).max() < 1e-13*max(1, np.abs(b).max()), op
