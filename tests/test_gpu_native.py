@@ -1,7 +1,8 @@
 """GPU runs on native files: a mesh imported from Gmsh, written as .pyfrm
 and read back runs the boundary-condition RHS against the oracle (as the
 Gmsh-read mesh does, tests/test_gpu_bcs.py); a solution written as .pyfrs
-and read back restarts a run bit for bit (tet/prism and hex meshes).
+and read back restarts a run bit for bit (tet/prism, pyramid/tet and hex
+meshes).
 Tolerances: RHS max(1e-12, 10 x the oracle's input-noise floor); restart
 exact on hexes, 1e-13 on tets and prisms (a second backend's tuner may pick
 another dense-kernel variant)."""
@@ -106,3 +107,26 @@ def test_pyfrs_restart_tet_prism(backend_factory, tmp_path):
     _restart(backend_factory, tmp_path, mesh,
              lambda be: make_simplex_cfg(3, be.precision, 'rk4', 2e-3,
                                          mode_ics(seed=3)), 2e-3, False)
+
+
+def test_pyfrs_restart_pyramid_tet(backend_factory, tmp_path):
+    """Pyramids with tets (Gauss-Legendre-Lobatto pyramid points) through
+    Gmsh, .pyfrm and a .pyfrs restart."""
+    from pyfr_amd.gmsh import GmshMesh
+    from pyfr_amd.mesh import SimplexBoxMesh
+    from pyfr_amd.native import NativeMesh
+    from tests.msh_files import mesh_to_msh
+    from tests.test_gmsh import _sim_sides
+    L = 2*np.pi
+    gen = SimplexBoxMesh([((2, 2, 2), L/2, [('tet', 1), ('pyt', 1)])])
+    fn = str(tmp_path/'pyt.pyfrm')
+    GmshMesh(io.StringIO(mesh_to_msh(gen, _sim_sides(gen, (L, L, L))))) \
+        .write_pyfrm(fn)
+    mesh = NativeMesh(fn)
+
+    def cfg(be):
+        c = make_simplex_cfg(3, be.precision, 'rk4', 2e-3, mode_ics(seed=3))
+        c.set('solver-elements-pyr', 'soln-pts', 'gauss-legendre-lobatto')
+        return c
+    print('\n[pyfrs restart] pyramid/tet', flush=True)
+    _restart(backend_factory, tmp_path, mesh, cfg, 2e-3, False)
